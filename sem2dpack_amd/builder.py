@@ -1,0 +1,1159 @@
+"""Host-side problem set-up for structured (``MESH_CART``) SEM2DPACK inputs.
+
+This module restates, in numpy plus one native routine (RCM), the part of the
+reference initialisation ``init_main`` (``SRC/init.f90:16-131``) that produces
+the arrays consumed by the explicit time step.  Every array keeps the
+reference's meaning, units and (1-based) node numbering so that outputs can be
+compared node by node with the reference:
+
+=====================  =====================================================
+reference routine      here
+=====================  =====================================================
+``CART_build``         :func:`_cart_mesh`  (``SRC/mesh_cartesian.f90:219-314``)
+``MESH_STRUCTURED_*``  :func:`_cart_mesh`, :func:`_rcm_perm`
+                       (``SRC/mesh_structured.f90:12-269``, ``SRC/rcm.f90``)
+``SE_init_numbering``  :func:`_numbering` (``SRC/spec_grid.f90:198-314``)
+``SE_init_coord``      :func:`_numbering` (``SRC/spec_grid.f90:321-374``)
+``get_GLL_info``       :func:`gll_info` (``SRC/gll.f90:19-686``)
+``SE_Jacobian`` etc.   :func:`_jacobians` (``SRC/spec_grid.f90:560-660``)
+``CHECK_grid``         :func:`_grid_cfl` (``SRC/init.f90:145-289``)
+``TIME_init``          :func:`build` (``SRC/time.f90:323-395``)
+``MAT_ELAST_init_a``   :func:`_elastic_coefs` (``SRC/mat_elastic.f90:290-360``)
+``MAT_MASS_init``      :func:`build` (``SRC/mat_mass.f90:29-61``)
+``BC_ABSO_init``       :func:`_abso_init` (``SRC/bc_abso.f90:115-266``)
+``BC_DYNFLT_init``     :func:`_dynflt_init` (``SRC/bc_dynflt.f90:231-520``)
+``SO_init``            :func:`_sources` (``SRC/src_gen.f90:216-262``)
+``REC_init``           :func:`_receivers` (``SRC/receivers.f90:168-303``)
+=====================  =====================================================
+
+Floating-point expressions follow the reference's operation order where the
+order is visible in the source (shape functions, Jacobians, coefficient
+products, mass assembly order, GLL iterations), so that numbering, coordinates,
+time step, mass and coefficients agree bit for bit with the reference.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+from . import parfile as _pf
+from ._lib import load_host_lib
+
+PI = 3.141592653589793  # SRC/constants.f90:44
+TINY_XABS = 1e-3        # SRC/constants.f90:36
+
+# side and edge identifiers of SRC/fem_grid.f90:74-87
+EDGE_D, EDGE_R, EDGE_U, EDGE_L = 1, 2, 3, 4
+SIDE_D, SIDE_R, SIDE_U, SIDE_L = 1, 2, 3, 4
+FAULT_D, FAULT_U = 5, 6
+
+
+class InputError(ValueError):
+    """Raised where the reference would stop with IO_abort (SRC/stdio.f90:205-214)."""
+
+
+# ---------------------------------------------------------------------------
+# GLL quadrature (Gauss-Lobatto-Jacobi points of Fischer & Patera's spectral
+# library, as used by SRC/gll.f90), restated so that the nodes, weights and
+# derivative matrix agree bit for bit with the reference.
+# ---------------------------------------------------------------------------
+def _gammaf(x):
+    """gammaf (SRC/gll.f90:205-226): table of the values the quadrature needs."""
+    sp = math.sqrt(PI)
+    table = {-0.5: -2.0 * sp, 0.5: sp, 1.0: 1.0, 2.0: 1.0, 1.5: sp / 2.0, 2.5: 1.5 * sp / 2.0,
+             3.5: 2.5 * 1.5 * sp / 2.0, 3.0: 2.0, 4.0: 6.0, 5.0: 24.0, 6.0: 120.0}
+    return table.get(x, 1.0)
+
+
+def _jacobf(n, alp, bet, x):
+    """Jacobi polynomial P_n and derivative with the two previous orders (SRC/gll.f90:377-418)."""
+    apb = alp + bet
+    poly, pder = 1.0, 0.0
+    psave = pdsave = 0.0
+    if n == 0:
+        return poly, pder, 0.0, 0.0, 0.0, 0.0
+    polyl, pderl = poly, pder
+    poly = (alp - bet + (apb + 2.0) * x) / 2.0
+    pder = (apb + 2.0) / 2.0
+    if n == 1:
+        return poly, pder, 0.0, 0.0, 0.0, 0.0
+    for k in range(2, n + 1):
+        dk = float(k)
+        a1 = 2.0 * dk * (dk + apb) * (2.0 * dk + apb - 2.0)
+        a2 = (2.0 * dk + apb - 1.0) * (alp * alp - bet * bet)
+        b3 = (2.0 * dk + apb - 2.0)
+        a3 = b3 * (b3 + 1.0) * (b3 + 2.0)
+        a4 = 2.0 * (dk + alp - 1.0) * (dk + bet - 1.0) * (2.0 * dk + apb)
+        polyn = ((a2 + a3 * x) * poly - a4 * polyl) / a1
+        pdern = ((a2 + a3 * x) * pder - a4 * pderl + a3 * poly) / a1
+        psave, pdsave = polyl, pderl
+        polyl, poly = poly, polyn
+        pderl, pder = pder, pdern
+    return poly, pder, polyl, pderl, psave, pdsave
+
+
+def _jacg(np_, alpha, beta):
+    """Gauss-Jacobi points by Newton iteration with deflation (SRC/gll.f90:300-375)."""
+    xjac = [0.0] * np_
+    n = np_ - 1
+    dth = 4.0 * math.atan(1.0) / (2.0 * float(n) + 2.0)
+    xlast = 0.0
+    for j in range(1, np_ + 1):
+        if j == 1:
+            x = math.cos((2.0 * (float(j) - 1.0) + 1.0) * dth)
+        else:
+            x1 = math.cos((2.0 * (float(j) - 1.0) + 1.0) * dth)
+            x = (x1 + xlast) / 2.0
+        for _ in range(10):
+            p, pd = _jacobf(np_, alpha, beta, x)[:2]
+            recsum = 0.0
+            for i in range(1, j):
+                recsum = recsum + 1.0 / (x - xjac[np_ - i])
+            delx = -p / (pd - recsum * p)
+            x = x + delx
+            if abs(delx) < 1.0e-12:
+                break
+        xjac[np_ - j] = x
+        xlast = x
+    for i in range(np_):  # selection sort, ascending
+        xmin, jmin = 2.0, i
+        for j in range(i, np_):
+            if xjac[j] < xmin:
+                xmin, jmin = xjac[j], j
+        if jmin != i:
+            xjac[i], xjac[jmin] = xjac[jmin], xjac[i]
+    return xjac
+
+
+def _pnormj(n, alpha, beta):
+    """SRC/gll.f90:476-507."""
+    dn = float(n)
+    const = alpha + beta + 1.0
+    if n <= 1:
+        prod = _gammaf(dn + alpha) * _gammaf(dn + beta)
+        prod = prod / (_gammaf(dn) * _gammaf(dn + alpha + beta))
+        return prod * 2.0 ** const / (2.0 * dn + const)
+    prod = _gammaf(alpha + 1.0) * _gammaf(beta + 1.0)
+    prod = prod / (2.0 * (1.0 + const) * _gammaf(const + 1.0))
+    prod = prod * (1.0 + alpha) * (2.0 + alpha)
+    prod = prod * (1.0 + beta) * (2.0 + beta)
+    for i in range(3, n + 1):
+        di = float(i)
+        frac = (di + alpha) * (di + beta) / (di * (di + alpha + beta))
+        prod = prod * frac
+    return prod * 2.0 ** const / (2.0 * dn + const)
+
+
+def _endw(n, alpha, beta, which):
+    """endw1 / endw2 (SRC/gll.f90:79-203)."""
+    apb = alpha + beta
+    if n == 0:
+        return 0.0
+    if which == 1:
+        f1 = _gammaf(alpha + 2.0) * _gammaf(beta + 1.0) / _gammaf(apb + 3.0)
+    else:
+        f1 = _gammaf(alpha + 1.0) * _gammaf(beta + 2.0) / _gammaf(apb + 3.0)
+    f1 = f1 * (apb + 2.0) * 2.0 ** (apb + 2.0) / 2.0
+    if n == 1:
+        return f1
+    if which == 1:
+        fint1 = _gammaf(alpha + 2.0) * _gammaf(beta + 1.0) / _gammaf(apb + 3.0)
+    else:
+        fint1 = _gammaf(alpha + 1.0) * _gammaf(beta + 2.0) / _gammaf(apb + 3.0)
+    fint1 = fint1 * 2.0 ** (apb + 2.0)
+    fint2 = _gammaf(alpha + 2.0) * _gammaf(beta + 2.0) / _gammaf(apb + 4.0)
+    fint2 = fint2 * 2.0 ** (apb + 3.0)
+    if which == 1:
+        f2 = (-2.0 * (beta + 2.0) * fint1 + (apb + 4.0) * fint2) * (apb + 3.0) / 4.0
+    else:
+        f2 = (2.0 * (alpha + 2.0) * fint1 - (apb + 4.0) * fint2) * (apb + 3.0) / 4.0
+    if n == 2:
+        return f2
+    f3 = 0.0
+    for i in range(3, n + 1):
+        di = float(i - 1)
+        abn = alpha + beta + di
+        abnn = abn + di
+        a1 = -(2.0 * (di + alpha) * (di + beta)) / (abn * abnn * (abnn + 1.0))
+        a2 = (2.0 * (alpha - beta)) / (abnn * (abnn + 2.0))
+        a3 = (2.0 * (abn + 1.0)) / ((abnn + 2.0) * (abnn + 1.0))
+        f3 = -(a2 * f2 + a1 * f1) / a3
+        f1, f2 = f2, f3
+    return f3
+
+
+def _pnleg(z, n):
+    p1, p2, p3 = 1.0, z, z
+    for k in range(1, n):
+        fk = float(k)
+        p3 = ((2.0 * fk + 1.0) * z * p2 - fk * p1) / (fk + 1.0)
+        p1, p2 = p2, p3
+    return p3
+
+
+def _pndleg(z, n):
+    p1, p2, p1d, p2d, p3d = 1.0, z, 0.0, 1.0, 1.0
+    for k in range(1, n):
+        fk = float(k)
+        p3 = ((2.0 * fk + 1.0) * z * p2 - fk * p1) / (fk + 1.0)
+        p3d = ((2.0 * fk + 1.0) * p2 + (2.0 * fk + 1.0) * z * p2d - fk * p1d) / (fk + 1.0)
+        p1, p2, p1d, p2d = p2, p3, p2d, p3d
+    return p3d
+
+
+def gll_info(n=5):
+    """get_GLL_info (SRC/gll.f90:19-36): nodes x, weights w, H[i, j] = h'_i(x_j)."""
+    z = [0.0] * n
+    w = [0.0] * n
+    nm1 = n - 2
+    if nm1 > 0:  # zwgjd on the interior with alpha = beta = 1
+        zi = _jacg(nm1, 1.0, 1.0)
+        np1, np2 = nm1, nm1 + 1
+        dnp1, dnp2 = float(np1), float(np2)
+        fac1 = dnp1 + 1.0 + 1.0 + 1.0
+        fac2 = fac1 + dnp1
+        fac3 = fac2 + 1.0
+        fnorm = _pnormj(np1, 1.0, 1.0)
+        rcoef = (fnorm * fac2 * fac3) / (2.0 * fac1 * dnp2)
+        for i in range(nm1):
+            p, pd, pm1, pdm1, _, _ = _jacobf(np2, 1.0, 1.0, zi[i])
+            z[i + 1] = zi[i]
+            w[i + 1] = -rcoef / (p * pdm1)
+    z[0], z[n - 1] = -1.0, 1.0
+    for i in range(1, n - 1):
+        w[i] = w[i] / (1.0 - z[i] * z[i])
+    N = n - 1
+    pd = _jacobf(N, 0.0, 0.0, z[0])[1]
+    w[0] = _endw(N, 0.0, 0.0, 1) / (2.0 * pd)
+    pd = _jacobf(N, 0.0, 0.0, z[n - 1])[1]
+    w[n - 1] = _endw(N, 0.0, 0.0, 2) / (2.0 * pd)
+    if n % 2 != 0:
+        z[(n - 1) // 2] = 0.0
+    H = np.zeros((n, n))
+    for ix in range(n):
+        for ip in range(n):
+            i, j = ip, ix
+            if i == 0 and j == 0:
+                v = -float(N) * (float(N) + 1.0) / 4.0
+            elif i == N and j == N:
+                v = float(N) * (float(N) + 1.0) / 4.0
+            elif i == j:
+                v = 0.0
+            else:
+                r1, r2, r3 = _pnleg(z[j], N), _pndleg(z[j], N), _pnleg(z[i], N)
+                v = r1 / (r3 * (z[j] - z[i])) + (1.0 - z[j] * z[j]) * r2 / (
+                    float(N) * (float(N) + 1.0) * r3 * (z[j] - z[i]) * (z[j] - z[i]))
+            H[ip, ix] = v
+    return np.array(z), np.array(w), H
+
+
+def gll5():
+    return gll_info(5)
+
+
+# ---------------------------------------------------------------------------
+# Problem container
+# ---------------------------------------------------------------------------
+class Problem:
+    """Arrays of one SEM problem, reference numbering and meaning.
+
+    Layouts (C order, equal to the reference's Fortran column-major memory):
+      ibool[nelem, ngll, ngll]   ibool[e, j, i] = reference ibool(i, j, e), 1-based
+      coord[npoin, 2]            reference coord(:, k)
+      rmass[ndof, npoin]         reference rmass(k, c) (inverse of the assembled mass)
+      coefs                      shared:      [nelast, ngll, ngll]      (a(i,j,k) -> [k, j, i])
+                                 per element: [nelem, nelast, ngll, ngll]
+      hprime[ngll, ngll]         hprime[i, j] = h'_i(x_j)
+    """
+
+    def __init__(self):
+        self.sources = []
+        self.abso = []
+        self.fault = None
+        self.receivers = None
+
+
+# ---------------------------------------------------------------------------
+# mesh + numbering
+# ---------------------------------------------------------------------------
+def _cart_mesh(g):
+    xlim = g.get("xlim")
+    zlim = g.get("zlim")
+    nel = g.get("nelem")
+    if xlim is None or zlim is None:
+        raise InputError("CART_read: you must set xlim and zlim")
+    nx, nz = int(nel[0]), int(nel[1])
+    if nx <= 0 or nz <= 0:
+        raise InputError("CART_read: nelem must be positive")
+    ezflt = int(g.get("ezflt", 0))
+    if g.get("faultx", False):
+        ezflt = -1
+    if ezflt >= nz:
+        raise InputError("CART_read: ezflt must be < nelem(2)")
+    if ezflt == -1:
+        ezflt = nz // 2
+    if ezflt < -1:
+        raise InputError("CART_read: ezflt must be >= -1")
+    if g.get("split", False):
+        raise InputError("MESH_CART split is not supported by this build")
+    return dict(xmin=float(xlim[0]), xmax=float(xlim[1]), zmin=float(zlim[0]), zmax=float(zlim[1]),
+                nx=nx, nz=nz, ezflt=ezflt, fztag=int(g.get("fztag", 0)), fznz=int(g.get("fznz", 1)))
+
+
+def _rcm_perm(nx, nz):
+    """Element permutation of MESH_STRUCTURED_renumber (SRC/mesh_structured.f90:204-269)."""
+    jj, ii = np.meshgrid(np.arange(1, nz + 1), np.arange(1, nx + 1), indexing="ij")
+    ii = ii.ravel()
+    jj = jj.ravel()
+    offs = [(-1, -1), (0, -1), (1, -1), (-1, 0), (1, 0), (-1, 1), (0, 1), (1, 1)]
+    cols = []
+    valid = []
+    for di, dj in offs:
+        i2, j2 = ii + di, jj + dj
+        ok = (i2 >= 1) & (i2 <= nx) & (j2 >= 1) & (j2 <= nz)
+        cols.append((j2 - 1) * nx + i2)
+        valid.append(ok)
+    cols = np.stack(cols, axis=1)
+    valid = np.stack(valid, axis=1)
+    counts = valid.sum(axis=1)
+    adj = cols[valid].astype(np.int32)  # row-major -> per element in offs order
+    adj_row = np.empty(nx * nz + 1, dtype=np.int32)
+    adj_row[0] = 1
+    np.cumsum(counts, out=adj_row[1:])
+    adj_row[1:] += 1
+    perm = np.zeros(nx * nz, dtype=np.int32)
+    lib = load_host_lib()
+    rc = lib.sem2d_genrcm(int(nx * nz), int(adj.size), adj_row.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                          adj.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                          perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    if rc != 0:
+        raise RuntimeError(f"sem2d_genrcm failed ({rc})")
+    return perm  # perm[k] (1-based values): old element index of new element k+1
+
+
+# local numbering order inside the first element that touches a node
+# (SRC/spec_grid.f90:252-285): interior, edges D,R,U,L counterclockwise, vertices
+def _local_code(ngll=5):
+    C = np.full((ngll, ngll), -1, dtype=np.int64)  # C[j, i]
+    c = 0
+    for j in range(1, ngll - 1):
+        for i in range(1, ngll - 1):
+            C[j, i] = c
+            c += 1
+    for i in range(1, ngll - 1):          # edge D, i increasing
+        C[0, i] = c; c += 1
+    for j in range(1, ngll - 1):          # edge R, j increasing
+        C[j, ngll - 1] = c; c += 1
+    for i in range(ngll - 2, 0, -1):      # edge U, i decreasing
+        C[ngll - 1, i] = c; c += 1
+    for j in range(ngll - 2, 0, -1):      # edge L, j decreasing
+        C[j, 0] = c; c += 1
+    for (i, j) in ((0, 0), (ngll - 1, 0), (ngll - 1, ngll - 1), (0, ngll - 1)):
+        C[j, i] = c; c += 1
+    return C
+
+
+def _numbering(mesh, perm, xgll):
+    ngll = 5
+    nx, nz, ezflt = mesh["nx"], mesh["nz"], mesh["ezflt"]
+    nelem = nx * nz
+    NX1 = 4 * nx + 1
+    NZ1 = 4 * nz + 1 + (1 if ezflt > 0 else 0)
+    old = perm.astype(np.int64) - 1
+    ix = old % nx
+    iz = old // nx
+    shift = (iz >= ezflt).astype(np.int64) if ezflt > 0 else np.zeros_like(iz)
+    loc = np.arange(ngll)
+    # node-grid index of each (e, j, i)
+    X = 4 * ix[:, None, None] + loc[None, None, :]
+    Z = 4 * iz[:, None, None] + loc[None, :, None] + shift[:, None, None]
+    G = Z * NX1 + X  # (nelem, 5, 5)
+    npoin = NX1 * NZ1
+    k = np.broadcast_to(np.arange(nelem)[:, None, None], G.shape)
+    first = np.full(npoin, nelem, dtype=np.int64)
+    np.minimum.at(first, G.ravel(), k.ravel())
+    last = np.full(npoin, -1, dtype=np.int64)
+    np.maximum.at(last, G.ravel(), k.ravel())
+    if (first == nelem).any():
+        raise RuntimeError("numbering: node not covered by any element")
+    C = _local_code(ngll)
+    code = np.empty(npoin, dtype=np.int64)
+    sel = first[G] == k
+    code[G[sel]] = np.broadcast_to(C[None], G.shape)[sel]
+    order = np.lexsort((code, first))
+    iglob = np.empty(npoin, dtype=np.int64)
+    iglob[order] = np.arange(1, npoin + 1)
+    ibool = iglob[G].astype(np.int32)  # [e, j, i], 1-based
+
+    # control nodes (SRC/mesh_cartesian.f90:253-267)
+    xc = mesh["xmin"] + (mesh["xmax"] - mesh["xmin"]) / float(nx) * np.arange(nx + 1, dtype=np.float64)
+    if ezflt > 0:
+        zi = np.concatenate([np.arange(0, ezflt + 1), np.arange(ezflt, nz + 1)]).astype(np.float64)
+    else:
+        zi = np.arange(0, nz + 1, dtype=np.float64)
+    zc = mesh["zmin"] + (mesh["zmax"] - mesh["zmin"]) / float(nz) * zi
+    # element corners (knods order 1..4 = (i,j),(i+1,j),(i+1,j+1),(i,j+1))
+    czr = iz + shift  # control row of the element's lower edge
+    cx = np.stack([xc[ix], xc[ix + 1], xc[ix + 1], xc[ix]], axis=1)
+    cz = np.stack([zc[czr], zc[czr], zc[czr + 1], zc[czr + 1]], axis=1)
+
+    # coordinates: written by the last element touching the node (SRC/spec_grid.f90:342-349)
+    shp = _shape_q4(xgll)  # (5(j), 5(i), 4)
+    coord = np.empty((npoin, 2))
+    selL = last[G] == k
+    e_of = np.broadcast_to(np.arange(nelem)[:, None, None], G.shape)[selL]
+    j_of = np.broadcast_to(loc[None, :, None], G.shape)[selL]
+    i_of = np.broadcast_to(loc[None, None, :], G.shape)[selL]
+    gidx = iglob[G[selL]] - 1
+    s = shp[j_of, i_of]  # (m, 4)
+    cxe = cx[e_of]
+    cze = cz[e_of]
+    coord[gidx, 0] = ((cxe[:, 0] * s[:, 0] + cxe[:, 1] * s[:, 1]) + cxe[:, 2] * s[:, 2]) + cxe[:, 3] * s[:, 3]
+    coord[gidx, 1] = ((cze[:, 0] * s[:, 0] + cze[:, 1] * s[:, 1]) + cze[:, 2] * s[:, 2]) + cze[:, 3] * s[:, 3]
+    return dict(ibool=ibool, coord=coord, npoin=npoin, cx=cx, cz=cz, ix=ix, iz=iz, shift=shift,
+                G=G, iglob_of_grid=iglob, NX1=NX1, NZ1=NZ1)
+
+
+def _shape_q4(xgll):
+    """Q4 shape functions at the GLL points (SRC/elem_q4.f90:37-53), [j, i, k]."""
+    q = 0.25
+    out = np.empty((5, 5, 4))
+    for j in range(5):
+        t = xgll[j]
+        for i in range(5):
+            s = xgll[i]
+            sp, sm, tp, tm = s + 1.0, s - 1.0, t + 1.0, t - 1.0
+            out[j, i] = [q * sm * tm, -(q * sp * tm), q * sp * tp, -(q * sm * tp)]
+    return out
+
+
+def _dshape_q4(xgll):
+    """Q4 shape derivatives (SRC/elem_q4.f90:57-78), [j, i, k, (dxi, deta)]."""
+    q = 0.25
+    out = np.empty((5, 5, 4, 2))
+    for j in range(5):
+        t = xgll[j]
+        for i in range(5):
+            s = xgll[i]
+            sp, sm, tp, tm = s + 1.0, s - 1.0, t + 1.0, t - 1.0
+            out[j, i, :, 0] = [q * tm, -(q * tm), q * tp, -(q * tp)]
+            out[j, i, :, 1] = [q * sm, -(q * sp), q * sp, -(q * sm)]
+    return out
+
+
+def _jacobians(cx, cz, xgll):
+    """Jacobian entries at every (e, j, i): J[a][b] = sum_k coorg(a,k) dshape(k,b)."""
+    ds = _dshape_q4(xgll)  # j,i,k,b
+
+    def mm(c, b):
+        d = ds[None, :, :, :, b]
+        cc = c[:, None, None, :]
+        return ((cc[..., 0] * d[..., 0] + cc[..., 1] * d[..., 1]) + cc[..., 2] * d[..., 2]) + cc[..., 3] * d[..., 3]
+
+    J11 = mm(cx, 0)  # dx/dxi
+    J12 = mm(cx, 1)  # dx/deta
+    J21 = mm(cz, 0)  # dz/dxi
+    J22 = mm(cz, 1)  # dz/deta
+    return J11, J12, J21, J22
+
+
+def _inv_jac(J11, J12, J21, J22):
+    det = J11 * J22 - J12 * J21
+    if (det <= 0).any():
+        raise InputError("SE_InverseJacobian: undefined Jacobian")
+    # SRC/utils.f90:92-111: B = [[J22, -J12], [-J21, J11]] / det
+    return J22 / det, (-J12) / det, (-J21) / det, J11 / det  # DxiDx, DxiDz, DetaDx, DetaDz
+
+
+# ---------------------------------------------------------------------------
+# materials
+# ---------------------------------------------------------------------------
+def _materials(par):
+    mats = {}
+    for g in par.all("MATERIAL"):
+        tag = int(g.get("tag", 0))
+        kind = g.get("kind", "ELAST")
+        kinds = kind if isinstance(kind, list) else [kind]
+        if tag <= 0:
+            raise InputError("MAT_read: tag must be positive")
+        for k in kinds:
+            if k not in ("ELAST",):
+                raise InputError(f"material kind {k!r} is not supported by this build")
+        me = par.after(g, "MAT_ELASTIC")
+        if me is None:
+            raise InputError("MAT_ELAST_read: MAT_ELASTIC block not found")
+        for key in ("rhoh", "cph", "csh", "c11", "c11h"):
+            if me.get(key):
+                raise InputError(f"MAT_ELASTIC {key} is not supported by this build")
+        rho, cp, cs = float(me.get("rho", 0)), float(me.get("cp", 0)), float(me.get("cs", 0))
+        if rho <= 0:
+            raise InputError("MAT_ELAST_read: undefined density (rho)")
+        mu = rho * cs * cs
+        kmod = rho * cp * cp
+        lam = rho * (cp * cp - 2.0 * cs * cs)
+        poiss = 0.5 * lam / (kmod - mu)
+        if poiss < 0.0 or poiss > 0.5:
+            raise InputError("Poisson's ratio out of range !")
+        mats[tag] = dict(rho=rho, cp=cp, cs=cs, lam=lam, mu=mu)
+    if not mats:
+        raise InputError("MAT_read: MATERIAL block not found")
+    return mats
+
+
+def _elastic_coefs(DxiDx, DetaDz, w2J, lam, mu, ndof):
+    """Flat-grid coefficients a(:,:,k) of SRC/mat_elastic.f90:327-358, one element."""
+    if ndof == 1:
+        a = np.stack([mu * DxiDx * DxiDx, mu * DetaDz * DetaDz])
+    else:
+        Kx = lam + 2.0 * mu
+        a = np.stack([Kx * DxiDx * DxiDx, lam * DxiDx * DetaDz, Kx * DetaDz * DetaDz,
+                      mu * DetaDz * DetaDz, mu * DxiDx * DetaDz, mu * DxiDx * DxiDx])
+    return (-w2J)[None] * a
+
+
+# ---------------------------------------------------------------------------
+# boundaries
+# ---------------------------------------------------------------------------
+def _bound_elements(mesh):
+    """Boundary element lists (old element indices, 1-based) of SRC/mesh_structured.f90:101-196."""
+    nx, nz, ezflt = mesh["nx"], mesh["nz"], mesh["ezflt"]
+    b = {}
+    b[SIDE_D] = ([i + 1 for i in range(nx)], EDGE_D)
+    b[SIDE_R] = ([j * nx + nx for j in range(nz)], EDGE_R)
+    b[SIDE_U] = ([(nz - 1) * nx + i + 1 for i in range(nx)], EDGE_U)
+    b[SIDE_L] = ([j * nx + 1 for j in range(nz)], EDGE_L)
+    if ezflt > 0:
+        b[FAULT_U] = ([ezflt * nx + i + 1 for i in range(nx)], EDGE_D)
+        b[FAULT_D] = ([(ezflt - 1) * nx + i + 1 for i in range(nx)], EDGE_U)
+    return b
+
+
+def _edge_ij(edge, ngll=5):
+    k = np.arange(ngll)
+    if edge == EDGE_D:
+        return k, np.zeros(ngll, int)
+    if edge == EDGE_R:
+        return np.full(ngll, ngll - 1), k
+    if edge == EDGE_U:
+        return k[::-1], np.full(ngll, ngll - 1)
+    return np.zeros(ngll, int), k[::-1]
+
+
+class _Bound:
+    pass
+
+
+def _bound_topo(tag, mesh, num, perm_inv):
+    """BC_set_bulk_node (SRC/spec_grid.f90:676-822): node list sorted by coordinate."""
+    elems_old, edge = _bound_elements(mesh)[tag]
+    elem = perm_inv[np.array(elems_old) - 1]  # 0-based new element index
+    ibool = num["ibool"]
+    coord = num["coord"]
+    ii, jj = _edge_ij(edge)
+    nodes = []
+    bibool = np.empty((len(elem), 5), dtype=np.int64)
+    index = {}
+    for n, e in enumerate(elem):
+        for kloc in range(5):
+            g = int(ibool[e, jj[kloc], ii[kloc]])
+            if g not in index:
+                index[g] = len(nodes)
+                nodes.append(g)
+            bibool[n, kloc] = index[g]
+    nodes = np.array(nodes, dtype=np.int64)
+    bc = coord[nodes - 1]
+    Lx = bc[:, 0].max() - bc[:, 0].min()
+    Lz = bc[:, 1].max() - bc[:, 1].min()
+    key = bc[:, 0] if Lx > Lz else bc[:, 1]
+    isort = np.argsort(key, kind="stable")
+    back = np.empty_like(isort)
+    back[isort] = np.arange(len(isort))
+    t = _Bound()
+    t.tag = tag
+    t.elem = elem
+    t.edge = edge
+    t.node = nodes[isort]              # 1-based global nodes
+    t.ibool = back[bibool]             # (nelem_b, 5) -> index into t.node
+    return t
+
+
+def _normal_and_weights(t, jac, wgll):
+    """BC_get_normal_and_weights (SRC/spec_grid.f90:961-1011)."""
+    J11, J12, J21, J22 = jac
+    npb = len(t.node)
+    NORM = np.zeros((npb, 2))
+    W = np.zeros(npb)
+    ii, jj = _edge_ij(t.edge)
+    dim_t = 0 if t.edge in (EDGE_D, EDGE_U) else 1
+    sign = -1.0 if t.edge in (EDGE_U, EDGE_L) else 1.0
+    for n, e in enumerate(t.elem):
+        for k in range(5):
+            i, j = ii[k], jj[k]
+            if dim_t == 0:
+                dx, dz = J11[e, j, i], J21[e, j, i]
+            else:
+                dx, dz = J12[e, j, i], J22[e, j, i]
+            jac1d = math.sqrt(dx * dx + dz * dz)
+            tx, tz = sign * dx / jac1d, sign * dz / jac1d
+            b = t.ibool[n, k]
+            NORM[b, 0] += tz
+            NORM[b, 1] += -tx
+            W[b] += wgll[k] * jac1d
+    for n in range(len(t.elem)):
+        for b in (t.ibool[n, 0], t.ibool[n, 4]):
+            nrm = math.sqrt(NORM[b, 0] ** 2 + NORM[b, 1] ** 2)
+            NORM[b] = NORM[b] / nrm
+    return NORM, W
+
+
+def _abso_init(t, jac, wgll, mat_of_elem, M, coefA2Vrhs, ndof, stacey):
+    """BC_ABSO_init (SRC/bc_abso.f90:115-266), Clayton-Engquist on flat boundaries."""
+    if stacey:
+        raise InputError("BC_ABSORB stacey=T is not supported by this build")
+    J11, J12, J21, J22 = jac
+    NORM, _ = _normal_and_weights(t, jac, wgll)
+    if np.all(np.abs(NORM[:, 0]) < TINY_XABS):
+        geo_tan, geo_nor = 0, 1
+    elif np.all(np.abs(NORM[:, 1]) < TINY_XABS):
+        geo_tan, geo_nor = 1, 0
+    else:
+        raise InputError("BC_ABSO_init: only flat boundaries are supported by this build")
+    dim_t = 0 if t.edge in (EDGE_D, EDGE_U) else 1
+    ii, jj = _edge_ij(t.edge)
+    C = np.zeros((len(t.node), ndof))
+    for n, e in enumerate(t.elem):
+        m = mat_of_elem[e]
+        rho = m["rho"]
+        c = [0.0, 0.0]
+        c[geo_nor] = m["cp"]
+        c[geo_tan] = m["cs"]
+        for k in range(5):
+            i, j = ii[k], jj[k]
+            if dim_t == 0:
+                a, b = J11[e, j, i], J21[e, j, i]
+            else:
+                a, b = J12[e, j, i], J22[e, j, i]
+            coef = wgll[k] * math.sqrt(a * a + b * b)
+            bck = t.ibool[n, k]
+            if ndof == 1:
+                C[bck, 0] = C[bck, 0] + rho * c[geo_tan] * coef
+            else:
+                C[bck, 0] = C[bck, 0] + rho * c[0] * coef
+                C[bck, 1] = C[bck, 1] + rho * c[1] * coef
+    nodes0 = t.node - 1
+    M[:, nodes0] = M[:, nodes0] + coefA2Vrhs * C.T
+    return dict(tag=t.tag, node=t.node.astype(np.int32), C=C)
+
+
+# ---------------------------------------------------------------------------
+# distributions used by the fault inputs (constant, PWCONR, LINEAR)
+# ---------------------------------------------------------------------------
+def _read_cd(par, group_iter, value, name_h):
+    """DIST_CD_Read (SRC/distribution_cd.f90:26-63): a constant or the next &DIST_<name> block."""
+    if not name_h:
+        return ("const", float(value))
+    want = "DIST_" + name_h.upper()
+    g = next(group_iter, None)
+    while g is not None and g.name != want:
+        g = next(group_iter, None)
+    if g is None:
+        raise InputError(f"{want} block not found")
+    if want == "DIST_PWCONR":
+        num = int(g.get("num"))
+        ref = g.get("ref", [0.0, 0.0])
+        vals = " ".join(g.data_lines).replace("d", "e").replace("D", "e").split()
+        vals = [float(v) for v in vals]
+        if num < 2:
+            raise InputError("read_pwconr_dist: needs more than 2 zones (num)")
+        return ("pwconr", dict(num=num, ref=[float(ref[0]), float(ref[1])], rad=vals[:num - 1],
+                               val=vals[num - 1:2 * num - 1]))
+    raise InputError(f"distribution {want} is not supported by this build")
+
+
+def _gen_cd(cd, coord):
+    kind, p = cd
+    if kind == "const":
+        return np.full(coord.shape[0], p)
+    if kind == "pwconr":  # SRC/distribution_pwconr.f90 generate_pwconr_dist
+        out = np.empty(coord.shape[0])
+        for n in range(coord.shape[0]):
+            rad = math.sqrt((coord[n, 0] - p["ref"][0]) ** 2 + (coord[n, 1] - p["ref"][1]) ** 2)
+            z = p["num"] - 1
+            for iz in range(p["num"] - 1):
+                if rad <= p["rad"][iz]:
+                    z = iz
+                    break
+            out[n] = p["val"][z]
+        return out
+    raise ValueError(kind)
+
+
+def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma, nt, ndof):
+    """BC_DYNFLT_read + BC_DYNFLT_init (SRC/bc_dynflt.f90:102-520)."""
+    g = par.after(bcdef, "BC_DYNFLT")
+    if g is None:
+        raise InputError("BC_DYNFLT_read: BC_DYNFLT input block not found")
+    tags = bcdef.get("tags")
+    if not isinstance(tags, list) or len(tags) < 2 or int(tags[1]) <= 0:
+        raise InputError("DYNFLT on a single side (symmetry) is not supported by this build")
+    tags = [int(tags[0]), int(tags[1])]
+    it = iter(par.next_groups(g))
+    cd = {}
+    # order of DIST_CD_Read calls: SRC/bc_dynflt.f90:166-174
+    for key, dflt in (("cohesion", 0.0), ("tn", 0.0), ("tt", 0.0), ("sxx", 0.0), ("sxy", 0.0),
+                      ("sxz", 0.0), ("syz", 0.0), ("szz", 0.0), ("v", 1e-12)):
+        cd[key] = _read_cd(par, it, g.get(key, dflt), g.get(key + "h", ""))
+    friction = g.get("friction", "SWF")
+    fr = friction if isinstance(friction, list) else [friction]
+    fr = [f for f in fr if f]
+    if len(fr) != 1 or fr[0] not in ("SWF", "RSF"):
+        raise InputError("BC_DYNFLT friction must be 'SWF' or 'RSF' in this build")
+    law = fr[0]
+    # friction block: the next &BC_DYNFLT_SWF / _RSF after the distributions
+    rest = list(it)
+    fb = next((x for x in rest if x.name == "BC_DYNFLT_" + law), None)
+    fr_it = iter(rest[rest.index(fb) + 1:]) if fb is not None else iter([])
+    fp = {}
+    if law == "SWF":
+        vals = {"kind": 1, "dc": 0.5, "mus": 0.6, "mud": 0.5, "alpha": 0.0, "p": 3.0, "healing": False}
+        if fb is not None:
+            for k in vals:
+                if k in fb:
+                    vals[k] = fb.get(k)
+        if int(vals["kind"]) not in (1, 2, 3):
+            raise InputError("BC_DYNFLT_SWF: invalid kind")
+        fp["kind"] = int(vals["kind"])
+        fp["healing"] = bool(vals["healing"])
+        for k in ("dc", "mus", "mud", "alpha", "p"):
+            fp[k + "_cd"] = _read_cd(par, fr_it, vals[k], fb.get(k + "h", "") if fb else "")
+    else:
+        vals = {"kind": 1, "dc": 0.5, "mus": 0.6, "a": 0.01, "b": 0.02, "vstar": 1.0, "theta": 0.0,
+                "vc": 1e-6}
+        if fb is not None:
+            for k in vals:
+                if k in fb:
+                    vals[k] = fb.get(k)
+        if int(vals["kind"]) not in (1, 2, 3, 4):
+            raise InputError("BC_DYNFLT_RSF: invalid kind")
+        fp["kind"] = int(vals["kind"])
+        for k in ("dc", "mus", "a", "b", "vstar", "theta", "vc"):
+            fp[k + "_cd"] = _read_cd(par, fr_it, vals[k], fb.get(k + "h", "") if fb else "")
+    nor = par.after(g, "BC_DYNFLT_NOR")
+    nkind = int(nor.get("kind", 1)) if nor is not None else 1
+    if nkind != 1:
+        raise InputError("BC_DYNFLT_NOR kind other than 1 (Coulomb) is not supported by this build")
+
+    t1 = _bound_topo(tags[0], mesh, num, perm_inv)
+    t2 = _bound_topo(tags[1], mesh, num, perm_inv)
+    if len(t1.elem) != len(t2.elem) or len(t1.node) != len(t2.node):
+        raise InputError("bc_dynflt_init: boundaries do not match")
+    keep = t1.node != t2.node
+    node1 = t1.node[keep]
+    node2 = t2.node[keep]
+    coord = num["coord"]
+    c1 = coord[node1 - 1]
+    c2 = coord[node2 - 1]
+    if (np.abs(c1 - c2) > TINY_XABS).any():
+        raise InputError("bc_dynflt_init: coordinates on boundaries do not match properly")
+    NORM, W = _normal_and_weights(t1, jac, wgll)
+    n1 = NORM[keep]
+    B = np.repeat(W[keep][:, None], ndof, axis=1)
+    coefA2V = gamma * dt
+    coefA2D = beta * dt * dt
+    invM1 = 1.0 / M[:, node1 - 1].T
+    invM2 = 1.0 / M[:, node2 - 1].T
+    Z = 1.0 / (coefA2V * B * (invM1 + invM2))
+    npf = len(node1)
+    f = dict(tags=tags, node1=node1.astype(np.int32), node2=node2.astype(np.int32), coord=c1, n1=n1, B=B,
+             invM1=invM1, invM2=invM2, Z=Z, coefA2V=coefA2V, coefA2D=coefA2D, law=law,
+             opening=bool(g.get("opening", True)))
+    if law == "SWF":
+        f["swf_kind"] = fp["kind"]
+        f["healing"] = fp["healing"]
+        for k in ("dc", "mus", "mud", "alpha", "p"):
+            f[k] = _gen_cd(fp[k + "_cd"], c1)
+        f["theta"] = np.zeros(npf)
+    else:
+        f["rsf_kind"] = fp["kind"]
+        for k in ("dc", "mus", "a", "b", "vstar", "theta", "vc"):
+            f[k] = _gen_cd(fp[k + "_cd"], c1)
+        f["Tc"] = f["dc"] / f["vstar"]
+        f["coeft"] = np.exp(-dt / f["Tc"])
+    V0 = np.zeros((npf, ndof))
+    if law == "RSF":
+        V0[:, 0] = _gen_cd(cd["v"], c1)
+    f["V0"] = V0
+    Tt0 = _gen_cd(cd["tt"], c1)
+    Tn0 = _gen_cd(cd["tn"], c1)
+    Sxx, Sxy, Sxz = _gen_cd(cd["sxx"], c1), _gen_cd(cd["sxy"], c1), _gen_cd(cd["sxz"], c1)
+    Syz, Szz = _gen_cd(cd["syz"], c1), _gen_cd(cd["szz"], c1)
+    nxv, nzv = n1[:, 0], n1[:, 1]
+    Tx = Sxx * nxv + Sxz * nzv
+    Ty = Sxy * nxv + Syz * nzv
+    Tz = Sxz * nxv + Szz * nzv
+    T0 = np.empty((npf, 2))
+    T0[:, 0] = Tt0 + Ty if ndof == 1 else Tt0 + Tx * nzv - Tz * nxv
+    T0[:, 1] = Tn0 + Tx * nxv + Tz * nzv
+    f["T0"] = T0
+    f["cohesion"] = _gen_cd(cd["cohesion"], c1)
+    if (f["cohesion"] < 0).any():
+        raise InputError("bc_dynflt_init: cohesion must be positive")
+    # initial friction coefficient (SRC/bc_dynflt.f90:404-413)
+    if law == "SWF":
+        f["MU0"] = swf_mu(f["swf_kind"], f["theta"], f["dc"], f["mus"], f["mud"], f["p"], f["alpha"])
+    else:
+        f["MU0"] = rsf_mu(f["rsf_kind"], V0[:, 0], f)
+    # outputs (SRC/bc_dynflt.f90:419-520)
+    oxi = g.get("oxi", [1, 2 ** 31 - 1, 1])
+    oix1 = max(int(oxi[0]), 1)
+    oixn = min(int(oxi[1]), npf)
+    oixd = int(oxi[2])
+    oitd = max(1, int(round(float(g.get("otd", 0.0)) / dt)))
+    f.update(oix1=oix1, oixn=oixn, oixd=oixd, oitd=oitd, odt=dt * oitd,
+             oit=int(round(float(g.get("ot1", 0.0)) / dt)), osides=bool(g.get("osides", False)))
+    return f
+
+
+def swf_mu(kind, theta, dc, mus, mud, p, alpha):
+    """swf_mu (SRC/bc_dynflt_swf.f90:142-162)."""
+    if kind == 1:
+        mu = mus - (mus - mud) * np.minimum(theta / dc, 1.0)
+    elif kind == 2:
+        mu = mud - (mud - mus) * np.exp(-theta / dc)
+    else:
+        mu = mud + (mus - mud) / (1.0 + theta / dc) ** p
+    return mu + alpha * theta
+
+
+def rsf_mu(kind, v, f):
+    """rsf_mu (SRC/bc_dynflt_rsf.f90:165-184)."""
+    a, b, mus, vstar, dc, th = f["a"], f["b"], f["mus"], f["vstar"], f["dc"], f["theta"]
+    if kind == 1:
+        return mus + a * np.abs(v) / (np.abs(v) + vstar) - b * th / (th + dc)
+    if kind in (2, 3):
+        return a * np.arcsinh(np.abs(v) / (2.0 * vstar) * np.exp((mus + b * np.log(vstar * th / dc)) / a))
+    return a * np.arcsinh(np.abs(v) / (2.0 * vstar) * np.exp((mus + b * np.log(f["vc"] * th / dc + 1)) / a))
+
+
+# ---------------------------------------------------------------------------
+# sources and receivers
+# ---------------------------------------------------------------------------
+def _nearest_node(coord, p):
+    """SE_find_nearest_node (SRC/spec_grid.f90:411-434): the last node at minimum distance."""
+    d2 = (p[0] - coord[:, 0]) ** 2 + (p[1] - coord[:, 1]) ** 2
+    m = d2.min()
+    idx = np.nonzero(d2 <= m)[0][-1]
+    return int(idx) + 1, math.sqrt(m)
+
+
+def ricker(t, f0, t0, ampli):
+    """ricker (SRC/stf_ricker.f90:89-101)."""
+    arg = PI * f0 * (t - t0)
+    arg = arg * arg
+    return -ampli * (1.0 - 2.0 * arg) * math.exp(-arg)
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def _sources(par, num, jac, hprime, ndof, dt, nt):
+    g = par.first("SRC_DEF")
+    if g is None:
+        return []
+    stf = g.get("stf", " ")
+    mech = g.get("mechanism", " ")
+    if str(stf).strip() == "" or str(mech).strip() == "":
+        raise InputError("SO_read: stf/mechanism not set")
+    if g.get("file", "none") != "none":
+        raise InputError("SRC_DEF file= is not supported by this build")
+    coord_in = g.get("coord")
+    if stf != "RICKER":
+        raise InputError(f"stf {stf!r} is not supported by this build")
+    # STF_RICKER reals are single precision (SRC/stf_ricker.f90:53-73)
+    sr = par.after(g, "STF_RICKER")
+    if sr is None:
+        raise InputError("RICKER_read: STF_RICKER input block not found")
+    f0 = _f32(sr.get("f0", 0.0))
+    onset = _f32(sr.get("onset", 0.0))
+    amp = _f32(sr.get("ampli", 1.0))
+    if f0 <= 0:
+        raise InputError("RICKER_read: f0 must be positive")
+    if onset < 1.0 / f0:
+        raise InputError("RICKER_read: Onset time too small (IO_warning aborts)")
+    coord = num["coord"]
+    ibool = num["ibool"]
+    iglob, _ = _nearest_node(coord, coord_in)
+    src = dict(iglob=iglob, f0=f0, t0=onset, stf_ampli=amp, tdelay=0.0, ampli=1.0, mechanism=mech)
+    if mech == "FORCE":
+        fg = par.after(g, "SRC_FORCE")
+        angle = float(fg.get("angle", 0.0)) if fg is not None else 0.0
+        angle = angle * PI / 180.0
+        src["kind"] = "force"
+        src["dir"] = np.array([-math.sin(angle), math.cos(angle)])
+    elif mech in ("EXPLOSION", "MOMENT", "DOUBLE_COUPLE"):
+        Mt = np.zeros((2, ndof))
+        if mech == "EXPLOSION":
+            if ndof != 2:
+                raise InputError("SRC_MOMENT_read: explosion only allowed in PSV (ndof=2)")
+            Mt[:] = [[1.0, 0.0], [0.0, 1.0]]
+        elif mech == "DOUBLE_COUPLE":
+            dg = par.after(g, "SRC_DOUBLE_COUPLE")
+            if dg is None:
+                raise InputError("SRC_MOMENT_read: SRC_DOUBLE_COUPLE input block not found")
+            dip = float(dg.get("dip", 90.0)) * PI / 180.0
+            n = [math.sin(dip), math.cos(dip)]
+            if ndof == 2:
+                r = [-math.cos(dip), math.sin(dip)]
+                Mt[0, 0] = 2.0 * r[0] * n[0]
+                Mt[0, 1] = r[0] * n[1] + r[1] * n[0]
+                Mt[1, 0] = Mt[0, 1]
+                Mt[1, 1] = 2.0 * r[1] * n[1]
+            else:
+                Mt[0, 0] = n[0]
+                Mt[1, 0] = n[1]
+        else:
+            mg = par.after(g, "SRC_MOMENT")
+            if mg is None:
+                raise InputError("SRC_MOMENT_read: SRC_MOMENT input block not found")
+            if ndof == 2:
+                Mt[:] = [[float(mg.get("mxx", 0)), float(mg.get("mxz", 0))],
+                         [float(mg.get("mzx", 0)), float(mg.get("mzz", 0))]]
+            else:
+                Mt[:, 0] = [float(mg.get("myx", 0)), float(mg.get("myz", 0))]
+        src.update(_moment_stencil(iglob, Mt, ibool, jac, hprime, ndof))
+        src["kind"] = "moment"
+    else:
+        raise InputError(f"mechanism {mech!r} is not supported by this build")
+    # amplitude table at t = it*dt, it = 1..nt (SRC/src_gen.f90:290-317)
+    src["stf_table"] = np.array([ricker(it * dt - src["tdelay"], f0, onset, amp) * src["ampli"]
+                                 for it in range(1, nt + 1)])
+    return [src]
+
+
+def _moment_stencil(iglob, Mt, ibool, jac, hprime, ndof):
+    """SRC_MOMENT_init (SRC/src_moment.f90:170-213)."""
+    J11, J12, J21, J22 = jac
+    hits = np.argwhere(ibool == iglob)  # (e, j, i) sorted by e, then j, then i
+    iglob_xi, iglob_eta, coef_xi, coef_eta = [], [], [], []
+    for e, j, i in hits:
+        iglob_xi.append(ibool[e, j, :].copy())
+        iglob_eta.append(ibool[e, :, i].copy())
+        a11, a12, a21, a22 = J11[e, j, i], J12[e, j, i], J21[e, j, i], J22[e, j, i]
+        det = a11 * a22 - a12 * a21
+        ji = np.array([[a22 / det, (-a12) / det], [(-a21) / det, a11 / det]])
+        cx = np.zeros((ndof, 5))
+        ce = np.zeros((ndof, 5))
+        if ndof == 2:
+            G = np.zeros((2, 2))
+            jt = ji.T
+            for r in range(2):
+                for c in range(2):
+                    G[r, c] = Mt[r, 0] * jt[0, c] + Mt[r, 1] * jt[1, c]
+            cx[0] = G[0, 0] * hprime[:, i]
+            ce[0] = G[0, 1] * hprime[:, j]
+            cx[1] = G[1, 0] * hprime[:, i]
+            ce[1] = G[1, 1] * hprime[:, j]
+        else:
+            G0 = ji[0, 0] * Mt[0, 0] + ji[0, 1] * Mt[1, 0]
+            G1 = ji[1, 0] * Mt[0, 0] + ji[1, 1] * Mt[1, 0]
+            cx[0] = G0 * hprime[:, i]
+            ce[0] = G1 * hprime[:, j]
+        coef_xi.append(cx)
+        coef_eta.append(ce)
+    return dict(iglob_xi=np.array(iglob_xi, dtype=np.int32), iglob_eta=np.array(iglob_eta, dtype=np.int32),
+                coef_xi=np.array(coef_xi), coef_eta=np.array(coef_eta))
+
+
+def _receivers(par, coord, nt):
+    g = par.first("REC_LINE")
+    if g is None:
+        return None
+    number = int(g.get("number", 0))
+    isamp = int(g.get("isamp", 1))
+    field = str(g.get("field", "V")).strip()
+    if field not in ("D", "V", "A"):
+        raise InputError('REC_read: parameter field has wrong value [D,V,A]')
+    if not g.get("atnode", True):
+        raise InputError("REC_LINE AtNode=F is not supported by this build")
+    if g.get("file", "none") != "none":
+        raise InputError("REC_LINE file= is not supported by this build")
+    first = [float(v) for v in g.get("first")]
+    last = [float(v) for v in g.get("last")]
+    pts = []
+    for i in range(1, number + 1):
+        if number > 1:
+            f = (i - 1) / float(number - 1)
+            pts.append((first[0] + f * (last[0] - first[0]), first[1] + f * (last[1] - first[1])))
+        else:
+            pts.append((first[0], first[1]))
+    nodes = []
+    for p in pts:  # REC_posit dedup (SRC/receivers.f90:247-272), incl. its "irec > 1" test
+        ip, _ = _nearest_node(coord, p)
+        if len(nodes) > 1 and ip in nodes:
+            continue
+        nodes.append(ip)
+    return dict(iglob=np.array(nodes, dtype=np.int32), isamp=isamp, field=field,
+                nsamp=nt // isamp + 1, coord=coord[np.array(nodes) - 1])
+
+
+# ---------------------------------------------------------------------------
+def build(par, nsteps=None):
+    """Build a :class:`Problem` from a parsed Par.inp (:func:`parfile.read_par`)."""
+    if isinstance(par, (str, os.PathLike)):
+        par = _pf.read_par(par)
+    gen = par.first("GENERAL")
+    if gen is None:
+        raise InputError("GENERAL input block not found")
+    ngll = int(gen.get("ngll", 9))
+    ndof = int(gen.get("ndof", 2))
+    fmax = float(gen.get("fmax", 0.0))
+    if ngll != 5:
+        raise InputError("this build supports ngll=5 only")
+    if ndof not in (1, 2):
+        raise InputError("ndof must be 1 or 2")
+    if "w" in gen.values:
+        raise InputError("2.5D (W) is not supported by this build")
+    md = par.first("MESH_DEF")
+    if md is None or str(md.get("method", "")).upper() != "CARTESIAN":
+        raise InputError("this build supports MESH_DEF method='CARTESIAN' only")
+    mesh = _cart_mesh(par.first("MESH_CART"))
+    for d in par.all("MESH_CART_DOMAIN"):
+        raise InputError("MESH_CART_DOMAIN is not supported by this build")
+
+    pb = Problem()
+    pb.ngll, pb.ndof = ngll, ndof
+    pb.mesh = mesh
+    xgll, wgll, H = gll5()
+    pb.xgll, pb.wgll, pb.hprime = xgll, wgll, H
+    nx, nz = mesh["nx"], mesh["nz"]
+    pb.nelem = nelem = nx * nz
+    perm = _rcm_perm(nx, nz)
+    perm_inv = np.empty(nelem, dtype=np.int64)
+    perm_inv[perm - 1] = np.arange(nelem)
+    pb.perm = perm
+    num = _numbering(mesh, perm, xgll)
+    pb.ibool = num["ibool"]
+    pb.coord = num["coord"]
+    pb.npoin = npoin = num["npoin"]
+    pb.elem_ix, pb.elem_iz = num["ix"], num["iz"]
+
+    # element tags (single domain tag 1, optional fault zone tag)
+    tags = np.ones(nelem, dtype=np.int64)
+    if mesh["fztag"] > 0:
+        j1 = max(mesh["ezflt"] + 1 - mesh["fznz"], 1)
+        j2 = min(mesh["ezflt"] + mesh["fznz"], nz)
+        rows = num["iz"] + 1
+        tags[(rows >= j1) & (rows <= j2)] = mesh["fztag"]
+    mats = _materials(par)
+    for t in np.unique(tags):
+        if int(t) not in mats:
+            raise InputError("MAT_init_prop: material tag missing")
+    mat_of_elem = [mats[int(t)] for t in tags]
+    pb.tags = tags
+
+    jac = _jacobians(num["cx"], num["cz"], xgll)
+    J11, J12, J21, J22 = jac
+    DxiDx, DxiDz, DetaDx, DetaDz = _inv_jac(*jac)
+    wgll2 = wgll[None, :] * wgll[:, None]  # [j, i] = wgll(i)*wgll(j)
+    dvol = (J11 * J22 - J12 * J21) * wgll2[None]
+
+    # CHECK_grid + TIME_init
+    ibool0 = pb.ibool - 1
+    ex = pb.coord[ibool0, 0]
+    ez = pb.coord[ibool0, 1]
+    d1 = np.sqrt((ex[:, :-1, 1:] - ex[:, :-1, :-1]) ** 2 + (ez[:, :-1, 1:] - ez[:, :-1, :-1]) ** 2)
+    d2 = np.sqrt((ex[:, 1:, :-1] - ex[:, :-1, :-1]) ** 2 + (ez[:, 1:, :-1] - ez[:, :-1, :-1]) ** 2)
+    cvel = np.array([m["cp"] if ndof == 2 else m["cs"] for m in mat_of_elem])
+    grid_cfl = float((cvel[:, None, None] / np.minimum(d1, d2)).max())
+    tg = par.first("TIME")
+    kind = tg.get("kind", "leapfrog") if tg is not None else "leapfrog"
+    if kind != "newmark":
+        raise InputError("this build supports TIME kind='newmark' only")
+    nbsteps = int(tg.get("nbsteps", 0))
+    dt_in = float(tg.get("dt", 0.0))
+    courant = float(tg.get("courant", 0.5))
+    total = float(tg.get("totaltime", 0.0))
+    if courant < 0 or courant > 0.6:
+        raise InputError("TIME: Courant out of range [0,0.6]")
+    if nbsteps * total != 0:
+        raise InputError("TIME: bad combination of settings, NbSteps or TotalTime")
+    tn = par.after(tg, "TIME_NEWMARK")
+    beta = float(tn.get("beta", 0.0)) if tn is not None else 0.0
+    gamma = float(tn.get("gamma", 0.5)) if tn is not None else 0.5
+    if dt_in > 0:
+        if total > 0:
+            nbsteps = math.ceil(total / dt_in)
+        dt = dt_in
+        courant = grid_cfl * dt
+    else:
+        dt = courant / grid_cfl
+        if total > 0:
+            nbsteps = math.ceil(total / dt)
+    omega_max = math.sqrt(2.0 / gamma)
+    critical = omega_max * 3.0 / 7.0 / math.sqrt(2.0)
+    if courant > critical:
+        raise InputError("TIME_init: Courant number too high (IO_warning aborts)")
+    pb.dt, pb.beta, pb.gamma, pb.alpha = dt, beta, gamma, 1.0
+    pb.nt = int(nbsteps if nsteps is None else nsteps)
+    pb.grid_cfl = grid_cfl
+
+    # MAT_init_work: flat grid, elastic.  Homogeneous materials share the
+    # array of the first element of their tag (SRC/mat_gen.f90:357-365).
+    pb.nelast = 2 if ndof == 1 else 6
+    utags = np.unique(tags)
+    if len(utags) == 1:
+        e1 = 0
+        m = mat_of_elem[e1]
+        pb.coefs = _elastic_coefs(DxiDx[e1], DetaDz[e1], dvol[e1], m["lam"], m["mu"], ndof)
+        pb.coef_mode = "shared"
+    else:
+        coefs = np.empty((nelem, pb.nelast, 5, 5))
+        first_of = {int(t): int(np.nonzero(tags == t)[0][0]) for t in utags}
+        for t in utags:
+            e1 = first_of[int(t)]
+            m = mats[int(t)]
+            coefs[tags == t] = _elastic_coefs(DxiDx[e1], DetaDz[e1], dvol[e1], m["lam"], m["mu"], ndof)
+        pb.coefs = coefs
+        pb.coef_mode = "per_element"
+
+    # MAT_MASS_init: sequential element-order assembly (np.add.at is sequential)
+    rho = np.array([m["rho"] for m in mat_of_elem])
+    massloc = rho[:, None, None] * dvol
+    M = np.zeros((ndof, npoin))
+    np.add.at(M[0], ibool0.ravel(), massloc.ravel())
+    if ndof == 2:
+        M[1] = M[0]
+    pb.mass_raw = M.copy()
+
+    # BC_init in list order (periodic unsupported)
+    coefA2Vrhs = 1.0 * gamma * dt
+    bcdefs = par.all("BC_DEF")
+    present = set(_bound_elements(mesh).keys())
+    for bd in bcdefs:
+        kind = bd.get("kind", " ")
+        tag = bd.get("tag", 0)
+        tagsv = bd.get("tags", None)
+        t1 = int(tag) if int(tag) > 0 else (int(tagsv[0]) if isinstance(tagsv, list) else 0)
+        if t1 <= 0:
+            raise InputError("bc_read: tag(s) are null or not set")
+        if t1 not in present:
+            continue  # IS_EMPTY
+        if kind == "ABSORB":
+            ab = par.after(bd, "BC_ABSORB")
+            stacey = bool(ab.get("stacey", False)) if ab is not None else False
+            t = _bound_topo(t1, mesh, num, perm_inv)
+            pb.abso.append(_abso_init(t, jac, wgll, mat_of_elem, M, coefA2Vrhs, ndof, stacey))
+        elif kind == "DYNFLT":
+            if pb.fault is not None:
+                raise InputError("more than one DYNFLT boundary is not supported by this build")
+            pb.fault = _dynflt_init(par, bd, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma, pb.nt, ndof)
+        else:
+            raise InputError(f"BC kind {kind!r} is not supported by this build")
+
+    pb.receivers = _receivers(par, pb.coord, pb.nt)
+    pb.sources = _sources(par, num, jac, H, ndof, dt, pb.nt)
+    pb.rmass = 1.0 / M
+    pb.mass = M
+    return pb
