@@ -1,0 +1,152 @@
+/*
+ * sem2d.h -- C ABI of the MI355X explicit spectral-element time step.
+ *
+ * This is the drop-in boundary for the time loop of SEM2DPACK (SRC/main.f90:51-99):
+ * the host (the reference's own Fortran initialisation, through ISO_C_BINDING, or
+ * any other host) builds the problem once, hands its arrays to sem2d_create and
+ * the sem2d_set_* calls, then advances the solution with sem2d_step and reads
+ * results back with the sem2d_fetch_* calls.
+ *
+ * Conventions (those of the reference's Fortran arrays, so Fortran can pass its
+ * arrays unchanged):
+ *   - node indices are 1-based global GLL node numbers (SRC/spec_grid.f90:198-314);
+ *   - nodal fields are [ndof][npoin] (Fortran field(npoin,ndof), column-major),
+ *     so each component is contiguous;
+ *   - ibool is [nelem][ngll][ngll] (Fortran ibool(i,j,e), i fastest);
+ *   - element coefficients are [nelem][nelast][ngll][ngll] (Fortran a(i,j,k) of
+ *     each element, SRC/mat_elastic.f90:11-14), or a single [nelast][ngll][ngll]
+ *     set shared by all elements (SRC/mat_gen.f90:357-365);
+ *   - hprime is [ngll][ngll] column-major: hprime[j*ngll+i] = h'_i(x_j).
+ * Every call returns 0 on success and a non-zero code otherwise; the message is
+ * available from sem2d_last_error (the reference prints and stops instead,
+ * SRC/stdio.f90:205-214).  Host arrays are only read during the call.  One
+ * handle owns one HIP stream and its device memory; a handle is not re-entrant.
+ * ngll must be 5 (the reference's optimised OPT_NGLL, SRC/constants.f90:6).
+ */
+#ifndef SEM2D_H
+#define SEM2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sem2d_solver sem2d_solver;
+
+/* Problem description for sem2d_create. */
+typedef struct sem2d_desc {
+  int ndof;              /* 1 = SH (antiplane), 2 = P-SV (in-plane) */
+  int ngll;              /* must be 5 */
+  int nelem, npoin;
+  int nelast;            /* 2, 3 (SH flat/general), 6, 10 (P-SV flat/general) */
+  int coef_shared;       /* 1: coefs holds one set for all elements */
+  const int *ibool;      /* [nelem][ngll][ngll], 1-based */
+  const double *coefs;   /* [nelem or 1][nelast][ngll][ngll] */
+  const double *hprime;  /* [ngll][ngll], hprime[j*ngll+i] = h'_i(x_j) */
+  const double *rmass;   /* [ndof][npoin], inverse of the assembled diagonal mass */
+  double dt, beta, gamma;/* Newmark time step and coefficients (SRC/time.f90:5-11) */
+  int bulk_fp32;         /* 0: FP64 fields and element update; 1: FP32 bulk update */
+  /* Optional structured-grid description (a MESH_CART box); nx = 0 when absent.
+     elem_of_cell[iz*nx+ix] is the 1-based element index of cell (ix, iz); ezflt is
+     the element row under a split fault (0 for none), as in SRC/mesh_cartesian.f90. */
+  int nx, nz, ezflt;
+  const int *elem_of_cell;
+} sem2d_desc;
+
+/* Dynamic fault description (replaces the state of bc_dynflt_type, SRC/bc_dynflt.f90:17-38,
+   as built by BC_DYNFLT_init, SRC/bc_dynflt.f90:231-520).  Arrays are [ncomp][npoin]. */
+typedef struct sem2d_fault_desc {
+  int npoin;                          /* fault node pairs */
+  const int *node1, *node2;           /* 1-based nodes of side 1 and side 2 */
+  const double *n1;                   /* [2][npoin] normal of side 1 */
+  const double *B, *invM1, *invM2, *Z;/* [ndof][npoin] */
+  const double *T0;                   /* [2][npoin] initial traction (tangent, normal) */
+  const double *cohesion;             /* [npoin] */
+  double coefA2V, coefA2D;
+  int allow_opening;
+  int law;                            /* 1 = slip weakening, 2 = rate and state */
+  int kind;                           /* SWF kind 1..3 (SRC/bc_dynflt_swf.f90:148-158),
+                                         RSF kind 1..4 (SRC/bc_dynflt_rsf.f90:165-184) */
+  int healing;                        /* SWF instantaneous healing */
+  /* friction parameters, [npoin] each:
+       SWF: p[0]=Dc p[1]=MuS p[2]=MuD p[3]=alpha p[4]=p
+       RSF: p[0]=Dc p[1]=MuS p[2]=a   p[3]=b     p[4]=Vstar p[5]=Vc */
+  const double *p[6];
+  const double *theta0;               /* [npoin] initial state variable */
+  const double *V0;                   /* [npoin] initial slip rate (RSF) or NULL */
+  /* output decimation as in SRC/bc_dynflt.f90:419-433: records at steps
+     it = oit, oit+oitd, ... over nodes oix1:oixn:oixd (1-based, inclusive) */
+  int oit, oitd, oix1, oixn, oixd;
+} sem2d_fault_desc;
+
+/* Create a solver from the problem arrays: replaces pb%fields / compute_Fint set-up
+   (SRC/fields.f90:52-90, SRC/mat_gen.f90:323-410).  Fields start at zero. */
+int sem2d_create(const sem2d_desc *desc, sem2d_solver **out);
+
+/* One absorbing boundary (Clayton-Engquist, flat), call order = BC application order:
+   replaces BC_ABSO_apply (SRC/bc_abso.f90:286-336).  C is [ndof][n]. */
+int sem2d_set_abso(sem2d_solver *h, int n, const int *node, const double *C);
+
+/* Point force at a node: replaces FORCE_add (SRC/src_force.f90:77-90).  dir[2] is used
+   when ndof = 2.  Returns the source index through *isrc. */
+int sem2d_set_force(sem2d_solver *h, int iglob, const double *dir, int *isrc);
+
+/* Moment tensor stencil: replaces SRC_MOMENT_add (SRC/src_moment.f90:217-235).
+   iglob_xi/eta are [nel][ngll], coef_xi/eta are [nel][ndof][ngll]. */
+int sem2d_set_moment(sem2d_solver *h, int nel, const int *iglob_xi, const int *iglob_eta,
+                     const double *coef_xi, const double *coef_eta, int *isrc);
+
+/* Source amplitude table: ampl[it-1] = STF_get(it*dt - tdelay)*ampli for it = 1..nt
+   (SRC/src_gen.f90:290-317, SRC/stf_gen.f90:106-127); steps beyond nt use 0. */
+int sem2d_set_stf(sem2d_solver *h, int isrc, int nt, const double *ampl);
+
+/* Receivers at GLL nodes: replaces REC_store with AtNode=T (SRC/receivers.f90:309-344).
+   field: 0 = displacement, 1 = velocity, 2 = acceleration.  A sample is taken at the
+   current step when registered (step 0) and after every step it with it % isamp == 0;
+   nsamp samples are kept. */
+int sem2d_set_receivers(sem2d_solver *h, int nrec, const int *iglob, int isamp, int field,
+                        int nsamp);
+
+/* Dynamic fault: replaces BC_DYNFLT_apply (SRC/bc_dynflt.f90:569-689) and the
+   record part of BC_DYNFLT_write (SRC/bc_dynflt.f90:751-778).  nsamp records are
+   kept; a record is taken at registration when the current step >= oit. */
+int sem2d_set_fault(sem2d_solver *h, const sem2d_fault_desc *f, int nsamp);
+
+/* Advance nsteps Newmark steps: replaces the loop body of SRC/main.f90:51-99
+   (solve -> solve_Newmark, SRC/solver.f90:42-84, plus REC_store and BC_write records).
+   May return before the device finishes; fetch calls synchronise. */
+int sem2d_step(sem2d_solver *h, int nsteps);
+
+/* Fields [ndof][npoin] (any pointer may be NULL). */
+int sem2d_fetch_fields(sem2d_solver *h, double *d, double *v, double *a);
+int sem2d_set_fields(sem2d_solver *h, const double *d, const double *v, const double *a);
+
+/* Seismograms [nsamp][ndof][nrec] in FP64 (the reference stores float32). */
+int sem2d_fetch_seismo(sem2d_solver *h, double *out, int *nsamp_taken);
+
+/* Fault records [nsamp][6][nout]: slip, slip rate, shear traction, normal traction,
+   friction coefficient, stick traction (the six lines of SRC/bc_dynflt.f90:762-767)
+   with nout = (oixn-oix1)/oixd + 1. */
+int sem2d_fetch_fault(sem2d_solver *h, double *out, int *nsamp_taken);
+
+/* Current fault state of all pairs: D, V [ndof][npoin], T, Tstick [2][npoin], MU [npoin]. */
+int sem2d_fetch_fault_state(sem2d_solver *h, double *D, double *V, double *T, double *MU,
+                            double *Tstick);
+
+/* f = -K d assembled over all elements: compute_Fint (SRC/solver.f90:273-320) with
+   MAT_ELAST_f (SRC/mat_elastic.f90:396-426).  d and f are [ndof][npoin] host arrays. */
+int sem2d_compute_fint(sem2d_solver *h, const double *d, double *f);
+
+/* Current step index (number of steps taken). */
+int sem2d_get_step(sem2d_solver *h, int *it);
+
+/* Average duration in ms of the kernels launched by the last sem2d_step call, measured
+   with HIP events on the handle's stream (force+step kernels only). */
+int sem2d_last_step_ms(sem2d_solver *h, double *ms_per_step);
+
+const char *sem2d_last_error(sem2d_solver *h);
+void sem2d_destroy(sem2d_solver *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEM2D_H */

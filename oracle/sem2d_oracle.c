@@ -1,0 +1,682 @@
+/*
+ * sem2d_oracle.c -- CPU restatement of the SEM2DPACK explicit time step.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the oracle against which the HIP library is
+ * checked.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py load it; the product never does.
+ *
+ * It implements the C ABI of include/sem2d.h under the prefix oracle_, in plain
+ * sequential C that follows the reference's Fortran statement by statement and in
+ * the same floating-point operation order (compile with -ffp-contract=off):
+ *
+ *   oracle_step          solve_Newmark        SRC/solver.f90:42-84
+ *   fint()               compute_Fint         SRC/solver.f90:273-320 (element order)
+ *     gather/scatter     FIELD_get_elem_sub / FIELD_add_elem_2  SRC/fields.f90:113-189
+ *     kd2_psv, kd2_sh    ELAST_KD2_PSV/SH, My_MATMUL  SRC/mat_elastic.f90:589-641,739-799
+ *   sources              SO_add, FORCE_add, SRC_MOMENT_add  SRC/src_gen.f90:290-317,
+ *                        SRC/src_force.f90:77-90, SRC/src_moment.f90:217-235
+ *   abso_apply           BC_ABSO_apply (flat, no Stacey)  SRC/bc_abso.f90:286-336
+ *   fault_apply          BC_DYNFLT_apply      SRC/bc_dynflt.f90:569-739
+ *     swf                swf_mu, swf_update_state  SRC/bc_dynflt_swf.f90:142-194
+ *     rsf                rsf_solver, rsf_update_theta, rsf_update_V, nr_solver,
+ *                        nr_fric_func_tau, rsf_mu  SRC/bc_dynflt_rsf.f90:165-568
+ *     normal             normal_update kind 1  SRC/bc_dynflt_normal.f90:118-136
+ *   receivers            REC_store (AtNode)   SRC/receivers.f90:309-344
+ *   fault records        BC_DYNFLT_write      SRC/bc_dynflt.f90:751-778
+ *
+ * Known deviation: in the rate-and-state branch the reference leaves the local
+ * Tstick array unset (SRC/bc_dynflt.f90:620-636 never assigns it); here Tstick is
+ * the stick traction minus T0 for both friction laws.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/sem2d.h"
+
+#define NG 5
+
+typedef struct {
+  int n;
+  int *node;    /* 0-based */
+  double *C;    /* [ndof][n] */
+} abso_t;
+
+typedef struct {
+  int kind; /* 1 force, 2 moment */
+  int iglob;
+  double dir[2];
+  int nel;
+  int *ixi, *ieta;       /* [nel][NG] 0-based */
+  double *cxi, *ceta;    /* [nel][ndof][NG] */
+  int nt;
+  double *ampl;
+} source_t;
+
+typedef struct {
+  int np, ndof;
+  int *n1, *n2; /* 0-based */
+  double *nrm;  /* [2][np] */
+  double *B, *invM1, *invM2, *Z; /* [ndof][np] */
+  double *T0, *coh;
+  double a2v, a2d;
+  int opening, law, kind, healing;
+  double *p[6];
+  double *theta, *Tc, *coeft;
+  double *D, *V, *T, *Tstick, *MU, *sigma;
+  int oit, oitd, oix1, oixn, oixd, nout;
+  int nsamp, taken;
+  double *rec; /* [nsamp][6][nout] */
+} fault_t;
+
+struct oracle_solver {
+  int ndof, nelem, npoin, nelast, shared;
+  int *ibool; /* 0-based [nelem][NG][NG] */
+  double *coefs;
+  double H[NG][NG]; /* H[i][j] = h'_i(x_j) */
+  double *rmass;
+  double dt, beta, gamma;
+  double *d, *v, *a;
+  int nabso;
+  abso_t abso[16];
+  int nsrc;
+  source_t src[64];
+  int nrec, isamp, field, nsamp, taken;
+  int *rnode;
+  double *seis;
+  fault_t *flt;
+  int it;
+  char err[256];
+};
+typedef struct oracle_solver oracle_solver;
+
+static int fail(oracle_solver *h, int code, const char *msg) {
+  if (h) snprintf(h->err, sizeof h->err, "%s", msg);
+  return code;
+}
+
+static double *dup_d(const double *src, size_t n) {
+  double *p = (double *)malloc(n * sizeof(double) + 8);
+  if (src) memcpy(p, src, n * sizeof(double)); else memset(p, 0, n * sizeof(double));
+  return p;
+}
+
+int oracle_create(const sem2d_desc *s, oracle_solver **out) {
+  if (!s || !out) return 1;
+  if (s->ngll != NG) return 2;
+  oracle_solver *h = (oracle_solver *)calloc(1, sizeof *h);
+  h->ndof = s->ndof; h->nelem = s->nelem; h->npoin = s->npoin; h->nelast = s->nelast;
+  h->shared = s->coef_shared;
+  h->ibool = (int *)malloc(sizeof(int) * (size_t)s->nelem * NG * NG);
+  for (size_t k = 0; k < (size_t)s->nelem * NG * NG; ++k) h->ibool[k] = s->ibool[k] - 1;
+  size_t nc = (size_t)(s->coef_shared ? 1 : s->nelem) * s->nelast * NG * NG;
+  h->coefs = dup_d(s->coefs, nc);
+  for (int j = 0; j < NG; ++j)
+    for (int i = 0; i < NG; ++i) h->H[i][j] = s->hprime[j * NG + i];
+  size_t nf = (size_t)s->ndof * s->npoin;
+  h->rmass = dup_d(s->rmass, nf);
+  h->dt = s->dt; h->beta = s->beta; h->gamma = s->gamma;
+  h->d = dup_d(NULL, nf); h->v = dup_d(NULL, nf); h->a = dup_d(NULL, nf);
+  *out = h;
+  return 0;
+}
+
+int oracle_set_abso(oracle_solver *h, int n, const int *node, const double *C) {
+  if (h->nabso >= 16) return fail(h, 3, "too many absorbing boundaries");
+  abso_t *b = &h->abso[h->nabso++];
+  b->n = n;
+  b->node = (int *)malloc(sizeof(int) * n);
+  for (int k = 0; k < n; ++k) b->node[k] = node[k] - 1;
+  b->C = dup_d(C, (size_t)h->ndof * n);
+  return 0;
+}
+
+int oracle_set_force(oracle_solver *h, int iglob, const double *dir, int *isrc) {
+  source_t *s = &h->src[h->nsrc];
+  memset(s, 0, sizeof *s);
+  s->kind = 1; s->iglob = iglob - 1;
+  if (dir) { s->dir[0] = dir[0]; s->dir[1] = dir[1]; }
+  if (isrc) *isrc = h->nsrc;
+  h->nsrc++;
+  return 0;
+}
+
+int oracle_set_moment(oracle_solver *h, int nel, const int *ixi, const int *ieta,
+                      const double *cxi, const double *ceta, int *isrc) {
+  source_t *s = &h->src[h->nsrc];
+  memset(s, 0, sizeof *s);
+  s->kind = 2; s->nel = nel;
+  s->ixi = (int *)malloc(sizeof(int) * nel * NG);
+  s->ieta = (int *)malloc(sizeof(int) * nel * NG);
+  for (int k = 0; k < nel * NG; ++k) { s->ixi[k] = ixi[k] - 1; s->ieta[k] = ieta[k] - 1; }
+  s->cxi = dup_d(cxi, (size_t)nel * h->ndof * NG);
+  s->ceta = dup_d(ceta, (size_t)nel * h->ndof * NG);
+  if (isrc) *isrc = h->nsrc;
+  h->nsrc++;
+  return 0;
+}
+
+int oracle_set_stf(oracle_solver *h, int isrc, int nt, const double *ampl) {
+  if (isrc < 0 || isrc >= h->nsrc) return fail(h, 3, "bad source index");
+  h->src[isrc].nt = nt;
+  h->src[isrc].ampl = dup_d(ampl, nt);
+  return 0;
+}
+
+static const double *rec_field(oracle_solver *h) {
+  return h->field == 0 ? h->d : (h->field == 1 ? h->v : h->a);
+}
+
+static void rec_store(oracle_solver *h) {
+  if (!h->nrec || h->it % h->isamp) return;
+  int s = h->it / h->isamp;
+  if (s >= h->nsamp) return;
+  const double *f = rec_field(h);
+  for (int c = 0; c < h->ndof; ++c)
+    for (int r = 0; r < h->nrec; ++r)
+      h->seis[((size_t)s * h->ndof + c) * h->nrec + r] = f[(size_t)c * h->npoin + h->rnode[r]];
+  if (s + 1 > h->taken) h->taken = s + 1;
+}
+
+int oracle_set_receivers(oracle_solver *h, int nrec, const int *iglob, int isamp, int field,
+                         int nsamp) {
+  h->nrec = nrec; h->isamp = isamp; h->field = field; h->nsamp = nsamp;
+  h->rnode = (int *)malloc(sizeof(int) * nrec);
+  for (int r = 0; r < nrec; ++r) h->rnode[r] = iglob[r] - 1;
+  h->seis = dup_d(NULL, (size_t)nsamp * h->ndof * nrec);
+  h->taken = 0;
+  rec_store(h);
+  return 0;
+}
+
+/* ---------------- friction ---------------- */
+static double swf_mu1(const fault_t *f, int k) {
+  double th = f->theta[k], dc = f->p[0][k], mus = f->p[1][k], mud = f->p[2][k];
+  double mu;
+  if (f->kind == 1) mu = mus - (mus - mud) * fmin(th / dc, 1.0);
+  else if (f->kind == 2) mu = mud - (mud - mus) * exp(-th / dc);
+  else mu = mud + (mus - mud) / pow(1.0 + th / dc, f->p[4][k]);
+  return mu + f->p[3][k] * th;
+}
+
+static double rsf_mu1(const fault_t *f, int k, double v) {
+  double dc = f->p[0][k], mus = f->p[1][k], a = f->p[2][k], b = f->p[3][k], vs = f->p[4][k];
+  double th = f->theta[k];
+  switch (f->kind) {
+    case 1: return mus + a * fabs(v) / (fabs(v) + vs) - b * th / (th + dc);
+    case 2: case 3: return a * asinh(fabs(v) / (2.0 * vs) * exp((mus + b * log(vs * th / dc)) / a));
+    default: return a * asinh(fabs(v) / (2.0 * vs) * exp((mus + b * log(f->p[5][k] * th / dc + 1)) / a));
+  }
+}
+
+static double rsf_theta1(const fault_t *f, int k, double theta, double v, double dt) {
+  double dc = f->p[0][k];
+  switch (f->kind) {
+    case 1: {
+      double tn = theta * f->coeft[k] + f->Tc[k] * fabs(v) * (1.0 - f->coeft[k]);
+      return tn < 1.0e-12 ? 0.0 : tn;
+    }
+    case 2: case 4: {
+      double x = fabs(v) / dc;
+      double ex = exp(-dt * x);
+      if (dt * x > 1e-8) return theta * ex + (1.0 - ex) / x;
+      return theta * ex + dt * (1.0 - 0.5 * dt * x);
+    }
+    default: {
+      double tn = dc / fabs(v);
+      return tn * pow(theta / tn, exp(-dt / tn));
+    }
+  }
+}
+
+/* nr_fric_func_tau (SRC/bc_dynflt_rsf.f90:533-568) */
+static void fric_func(const fault_t *f, int k, double theta, double tau, double tstick,
+                      double sigma, double Z, double *fv, double *dfv, double *v) {
+  double dc = f->p[0][k], mus = f->p[1][k], a = f->p[2][k], b = f->p[3][k], vs = f->p[4][k];
+  double tmp;
+  if (f->kind == 4) tmp = mus + b * log(f->p[5][k] * theta / dc + 1.0);
+  else tmp = mus + b * log(vs * theta / dc);
+  tmp = 2.0 * vs * exp(-tmp / a);
+  *v = sinh(tau / (-sigma * a)) * tmp;
+  *fv = tstick - Z * (*v) - tau;
+  double dv = cosh(tau / (-sigma * a)) * tmp / (-sigma * a);
+  *dfv = -Z * dv - 1.0;
+}
+
+/* nr_solver (SRC/bc_dynflt_rsf.f90:369-468); returns v of the last evaluation */
+static double nr_solver(const fault_t *f, int k, double xL, double xR, double xacc, double theta,
+                        double tstick, double sigma, double Z, int *ok) {
+  double flow, fhigh, dfdx, v, xl, xr, xlow, xhigh, xest, dx, dxold, fx, tmp;
+  *ok = 1;
+  fric_func(f, k, theta, xL, tstick, sigma, Z, &flow, &dfdx, &v);
+  fric_func(f, k, theta, xR, tstick, sigma, Z, &fhigh, &dfdx, &v);
+  xl = xL; xr = xR;
+  while (flow * fhigh > 0) {
+    xl = xl / 2.0;
+    xr = xr * 2.0;
+    fric_func(f, k, theta, xl, tstick, sigma, Z, &flow, &dfdx, &v);
+    fric_func(f, k, theta, xr, tstick, sigma, Z, &fhigh, &dfdx, &v);
+  }
+  if (flow == 0) return v;
+  if (fhigh == 0) return v;
+  if (flow < 0) { xlow = xl; xhigh = xr; } else { xhigh = xl; xlow = xr; }
+  xest = 0.5 * (xl + xr);
+  dxold = fabs(xr - xl);
+  dx = dxold;
+  fric_func(f, k, theta, xest, tstick, sigma, Z, &fx, &dfdx, &v);
+  for (int is = 1; is <= 200; ++is) {
+    if (((xest - xhigh) * dfdx - fx) * ((xest - xlow) * dfdx - fx) > 0 ||
+        fabs(2 * fx) > fabs(dxold * dfdx)) {
+      dxold = dx;
+      dx = 0.5 * (xhigh - xlow);
+      xest = xlow + dx;
+      if (xlow == xest) return v;
+    } else {
+      dxold = dx;
+      dx = fx / dfdx;
+      tmp = xest;
+      xest = xest - dx;
+      if (tmp == xest) return v;
+    }
+    if (fabs(dx) < fabs(xacc)) {
+      fric_func(f, k, theta, xest, tstick, sigma, Z, &fx, &dfdx, &v);
+      return v;
+    }
+    fric_func(f, k, theta, xest, tstick, sigma, Z, &fx, &dfdx, &v);
+    if (fx < 0) xlow = xest; else xhigh = xest;
+  }
+  *ok = 0;
+  return v;
+}
+
+/* rsf_update_V (SRC/bc_dynflt_rsf.f90:325-358) for one node */
+static double rsf_v1(const fault_t *f, int k, double tstick, double sigma, double theta, double Z,
+                     int *ok) {
+  double dc = f->p[0][k], mus = f->p[1][k], a = f->p[2][k], b = f->p[3][k], vs = f->p[4][k];
+  *ok = 1;
+  if (f->kind == 1) {
+    double mnd = mus - b * theta / (theta + dc); /* rsf_mu_no_direct kind 1 */
+    double v = (tstick + sigma * mnd) / Z;
+    double tmp = v - vs + sigma * a / Z;
+    v = 0.5 * (tmp + sqrt(tmp * tmp + 4.0 * v * vs));
+    v = fmax(0.0, v);
+    return v < 1.0e-12 ? 0.0 : v;
+  }
+  double tol = -0.001 * a * sigma;
+  double lo = fmin(0.0, tstick), hi = fmax(0.0, tstick);
+  return nr_solver(f, k, lo, hi, tol, theta, tstick, sigma, Z, ok);
+}
+
+static void fault_record(oracle_solver *h) {
+  fault_t *f = h->flt;
+  if (h->it < f->oit) return;
+  if (f->taken < f->nsamp) {
+    double *r = f->rec + (size_t)f->taken * 6 * f->nout;
+    int o = 0;
+    for (int k = f->oix1 - 1; k <= f->oixn - 1; k += f->oixd, ++o) {
+      r[0 * f->nout + o] = f->D[k];
+      r[1 * f->nout + o] = f->V[k];
+      r[2 * f->nout + o] = f->T[k];
+      r[3 * f->nout + o] = f->T[f->np + k];
+      r[4 * f->nout + o] = f->MU[k];
+      r[5 * f->nout + o] = f->Tstick[k];
+    }
+    f->taken++;
+  }
+  f->oit += f->oitd;
+}
+
+int oracle_set_fault(oracle_solver *h, const sem2d_fault_desc *d, int nsamp) {
+  fault_t *f = (fault_t *)calloc(1, sizeof *f);
+  int np = d->npoin, nd = h->ndof;
+  f->np = np; f->ndof = nd;
+  f->n1 = (int *)malloc(sizeof(int) * np);
+  f->n2 = (int *)malloc(sizeof(int) * np);
+  for (int k = 0; k < np; ++k) { f->n1[k] = d->node1[k] - 1; f->n2[k] = d->node2[k] - 1; }
+  f->nrm = dup_d(d->n1, 2 * (size_t)np);
+  f->B = dup_d(d->B, (size_t)nd * np);
+  f->invM1 = dup_d(d->invM1, (size_t)nd * np);
+  f->invM2 = dup_d(d->invM2, (size_t)nd * np);
+  f->Z = dup_d(d->Z, (size_t)nd * np);
+  f->T0 = dup_d(d->T0, 2 * (size_t)np);
+  f->coh = dup_d(d->cohesion, np);
+  f->a2v = d->coefA2V; f->a2d = d->coefA2D;
+  f->opening = d->allow_opening; f->law = d->law; f->kind = d->kind; f->healing = d->healing;
+  for (int q = 0; q < 6; ++q) f->p[q] = d->p[q] ? dup_d(d->p[q], np) : dup_d(NULL, np);
+  f->theta = dup_d(d->theta0, np);
+  f->Tc = dup_d(NULL, np); f->coeft = dup_d(NULL, np);
+  if (f->law == 2)
+    for (int k = 0; k < np; ++k) {
+      f->Tc[k] = f->p[0][k] / f->p[4][k];
+      f->coeft[k] = exp(-h->dt / f->Tc[k]);
+    }
+  f->D = dup_d(NULL, (size_t)nd * np);
+  f->V = dup_d(NULL, (size_t)nd * np);
+  if (d->V0) memcpy(f->V, d->V0, sizeof(double) * np);
+  f->T = dup_d(NULL, 2 * (size_t)np);
+  f->Tstick = dup_d(NULL, 2 * (size_t)np);
+  f->MU = dup_d(NULL, np);
+  f->sigma = dup_d(NULL, np);
+  for (int k = 0; k < np; ++k) {
+    f->MU[k] = f->law == 1 ? swf_mu1(f, k) : rsf_mu1(f, k, f->V[k]);
+    f->sigma[k] = f->T0[np + k];
+  }
+  f->oit = d->oit; f->oitd = d->oitd; f->oix1 = d->oix1; f->oixn = d->oixn; f->oixd = d->oixd;
+  f->nout = (f->oixn - f->oix1) / f->oixd + 1;
+  f->nsamp = nsamp;
+  f->rec = dup_d(NULL, (size_t)nsamp * 6 * f->nout);
+  h->flt = f;
+  fault_record(h);
+  return 0;
+}
+
+/* ---------------- element forces ---------------- */
+static void mm(const double A[NG][NG], const double B[NG][NG], double C[NG][NG]) {
+  /* My_MATMUL, C(i,j) = sum_k A(i,k) B(k,j); arrays indexed [col][row] = Fortran (row,col) */
+  for (int j = 0; j < NG; ++j)
+    for (int i = 0; i < NG; ++i) {
+      double c = 0.0;
+      for (int k = 0; k < NG; ++k) c = c + A[k][i] * B[j][k];
+      C[j][i] = c;
+    }
+}
+
+/* element arrays are stored [j][i] (Fortran (i,j)); H as Fortran matrices below */
+static void kd2(const oracle_solver *h, const double *a, double U[2][NG][NG], double F[2][NG][NG]) {
+  double Hm[NG][NG], Ht[NG][NG]; /* [col][row] */
+  for (int i = 0; i < NG; ++i)
+    for (int j = 0; j < NG; ++j) { Hm[j][i] = h->H[i][j]; Ht[j][i] = h->H[j][i]; }
+#define A(k, i, j) a[((k) * NG + (j)) * NG + (i)]
+  double tmp[NG][NG], t2[NG][NG];
+  if (h->ndof == 2) {
+    double dxx[NG][NG], dzx[NG][NG], dxe[NG][NG], dze[NG][NG];
+    mm(Ht, U[0], dxx);
+    mm(Ht, U[1], dzx);
+    mm(U[0], Hm, dxe);
+    mm(U[1], Hm, dze);
+    if (h->nelast == 6) {
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
+        tmp[j][i] = A(0, i, j) * dxx[j][i] + A(1, i, j) * dze[j][i];
+      mm(Hm, tmp, F[0]);
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
+        tmp[j][i] = A(3, i, j) * (dxe[j][i] + dzx[j][i]);
+      mm(tmp, Ht, t2);
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) F[0][j][i] = F[0][j][i] + t2[j][i];
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
+        tmp[j][i] = A(4, i, j) * dxe[j][i] + A(5, i, j) * dzx[j][i];
+      mm(Hm, tmp, F[1]);
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
+        tmp[j][i] = A(1, i, j) * dxx[j][i] + A(2, i, j) * dze[j][i];
+      mm(tmp, Ht, t2);
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) F[1][j][i] = F[1][j][i] + t2[j][i];
+    } else {
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
+        tmp[j][i] = A(0, i, j) * dxx[j][i] + A(6, i, j) * dxe[j][i] + A(7, i, j) * dzx[j][i] +
+                    A(1, i, j) * dze[j][i];
+      mm(Hm, tmp, F[0]);
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
+        tmp[j][i] = A(6, i, j) * dxx[j][i] + A(3, i, j) * dxe[j][i] + A(4, i, j) * dzx[j][i] +
+                    A(8, i, j) * dze[j][i];
+      mm(tmp, Ht, t2);
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) F[0][j][i] = F[0][j][i] + t2[j][i];
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
+        tmp[j][i] = A(7, i, j) * dxx[j][i] + A(4, i, j) * dxe[j][i] + A(5, i, j) * dzx[j][i] +
+                    A(9, i, j) * dze[j][i];
+      mm(Hm, tmp, F[1]);
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
+        tmp[j][i] = A(1, i, j) * dxx[j][i] + A(8, i, j) * dxe[j][i] + A(9, i, j) * dzx[j][i] +
+                    A(2, i, j) * dze[j][i];
+      mm(tmp, Ht, t2);
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) F[1][j][i] = F[1][j][i] + t2[j][i];
+    }
+  } else {
+    double dx[NG][NG], de[NG][NG];
+    mm(Ht, U[0], dx);
+    mm(U[0], Hm, de);
+    if (h->nelast == 2) {
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) tmp[j][i] = A(0, i, j) * dx[j][i];
+      mm(Hm, tmp, F[0]);
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) tmp[j][i] = A(1, i, j) * de[j][i];
+    } else {
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
+        tmp[j][i] = A(0, i, j) * dx[j][i] + A(2, i, j) * de[j][i];
+      mm(Hm, tmp, F[0]);
+      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
+        tmp[j][i] = A(2, i, j) * dx[j][i] + A(1, i, j) * de[j][i];
+    }
+    mm(tmp, Ht, t2);
+    for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) F[0][j][i] = F[0][j][i] + t2[j][i];
+  }
+#undef A
+}
+
+static void fint(const oracle_solver *h, const double *d, double *f) {
+  size_t np = h->npoin;
+  memset(f, 0, sizeof(double) * np * h->ndof);
+  double U[2][NG][NG], F[2][NG][NG];
+  for (int e = 0; e < h->nelem; ++e) {
+    const int *ib = h->ibool + (size_t)e * NG * NG;
+    for (int c = 0; c < h->ndof; ++c)
+      for (int j = 0; j < NG; ++j)
+        for (int i = 0; i < NG; ++i) U[c][j][i] = d[c * np + ib[j * NG + i]];
+    const double *a = h->coefs + (h->shared ? 0 : (size_t)e * h->nelast * NG * NG);
+    kd2(h, a, U, F);
+    for (int j = 0; j < NG; ++j)
+      for (int i = 0; i < NG; ++i) {
+        int k = ib[j * NG + i];
+        for (int c = 0; c < h->ndof; ++c) f[c * np + k] = f[c * np + k] + F[c][j][i];
+      }
+  }
+}
+
+int oracle_compute_fint(oracle_solver *h, const double *d, double *f) {
+  fint(h, d, f);
+  return 0;
+}
+
+static void fault_apply(oracle_solver *h, double *MxA) {
+  fault_t *f = h->flt;
+  int np = f->np, nd = h->ndof;
+  size_t N = h->npoin;
+  for (int k = 0; k < np; ++k) {
+    int i1 = f->n1[k], i2 = f->n2[k];
+    double dD[2] = {0, 0}, dV[2] = {0, 0}, dA[2] = {0, 0}, T[2] = {0, 0}, Ts[2];
+    for (int c = 0; c < nd; ++c) {
+      dD[c] = h->d[c * N + i2] - h->d[c * N + i1];
+      dV[c] = h->v[c * N + i2] - h->v[c * N + i1];
+      dA[c] = f->invM2[c * np + k] * MxA[c * N + i2] - f->invM1[c * np + k] * MxA[c * N + i1];
+      T[c] = f->Z[c * np + k] * (dV[c] + f->a2v * dA[c]);
+    }
+    double nx = f->nrm[k], nz = f->nrm[np + k];
+    if (nd == 2) {
+      double r0, r1;
+      r0 = nz * dD[0] - nx * dD[1]; r1 = nx * dD[0] + nz * dD[1]; dD[0] = r0; dD[1] = r1;
+      r0 = nz * dV[0] - nx * dV[1]; r1 = nx * dV[0] + nz * dV[1]; dV[0] = r0; dV[1] = r1;
+      r0 = nz * dA[0] - nx * dA[1]; r1 = nx * dA[0] + nz * dA[1]; dA[0] = r0; dA[1] = r1;
+      r0 = nz * T[0] - nx * T[1]; r1 = nx * T[0] + nz * T[1]; T[0] = r0; T[1] = r1;
+    } else {
+      T[1] = 0.0;
+    }
+    T[0] = T[0] + f->T0[k];
+    T[1] = T[1] + f->T0[np + k];
+    if (f->opening) T[1] = fmin(T[1], 0.0);
+    f->sigma[k] = T[1]; /* normal_update kind 1 */
+    double sigma = f->sigma[k];
+    Ts[0] = T[0]; Ts[1] = T[1];
+    if (f->law == 2) {
+      double vold = f->V[k], Z1 = f->Z[k];
+      int ok;
+      double th = rsf_theta1(f, k, f->theta[k], vold, h->dt);
+      double vn = rsf_v1(f, k, T[0], sigma, th, Z1, &ok);
+      th = rsf_theta1(f, k, f->theta[k], 0.5 * (vold + vn), h->dt);
+      vn = rsf_v1(f, k, T[0], sigma, th, Z1, &ok);
+      f->theta[k] = th;
+      f->V[k] = vn;
+      f->MU[k] = rsf_mu1(f, k, f->V[k]);
+      double strength = -f->MU[k] * sigma;
+      T[0] = copysign(fabs(strength), T[0]);
+    } else {
+      if (f->a2d == 0.0) {
+        if (f->healing) {
+          f->theta[k] = f->theta[k] + fabs(dV[0]) * h->dt;
+          if (fabs(dV[0]) < 1e-14) f->theta[k] = 0.0;
+        } else {
+          f->theta[k] = fabs(dD[0]);
+        }
+      } else {
+        f->theta[k] = fabs(f->D[k]);
+      }
+      f->MU[k] = swf_mu1(f, k);
+      double strength = f->coh[k] - f->MU[k] * sigma;
+      T[0] = copysign(fmin(fabs(T[0]), strength), T[0]);
+    }
+    T[0] = T[0] - f->T0[k];
+    T[1] = T[1] - f->T0[np + k];
+    Ts[0] = Ts[0] - f->T0[k];
+    Ts[1] = Ts[1] - f->T0[np + k];
+    f->T[k] = T[0]; f->T[np + k] = T[1];
+    f->Tstick[k] = Ts[0]; f->Tstick[np + k] = Ts[1];
+    double Tx[2] = {T[0], T[1]};
+    if (nd == 2) {
+      Tx[0] = nz * T[0] + nx * T[1];
+      Tx[1] = -nx * T[0] + nz * T[1];
+    }
+    for (int c = 0; c < nd; ++c) {
+      MxA[c * N + i1] = MxA[c * N + i1] + f->B[c * np + k] * Tx[c];
+      MxA[c * N + i2] = MxA[c * N + i2] - f->B[c * np + k] * Tx[c];
+    }
+    for (int c = 0; c < nd; ++c) {
+      double tf = (c == 0) ? f->T[k] : f->T[np + k];
+      dA[c] = dA[c] - tf / (f->Z[c * np + k] * f->a2v);
+      f->D[c * np + k] = dD[c] + f->a2d * dA[c];
+      f->V[c * np + k] = dV[c] + f->a2v * dA[c];
+    }
+  }
+}
+
+static void one_step(oracle_solver *h) {
+  size_t n = (size_t)h->ndof * h->npoin, N = h->npoin;
+  double dt = h->dt, beta = h->beta, gam = h->gamma;
+  double *d = h->d, *v = h->v, *a = h->a;
+  double c1 = (0.5 - beta) * dt * dt, c2 = (1.0 - gam) * dt;
+  for (size_t k = 0; k < n; ++k) {
+    d[k] = d[k] + dt * v[k] + c1 * a[k];
+    v[k] = v[k] + c2 * a[k];
+  }
+  double *f = a; /* f => a (SRC/solver.f90:52) */
+  fint(h, d, f);
+  int it = h->it + 1;
+  for (int s = 0; s < h->nsrc; ++s) {
+    source_t *S = &h->src[s];
+    double amp = (it - 1 < S->nt) ? S->ampl[it - 1] : 0.0;
+    if (S->kind == 1) {
+      if (h->ndof == 1) f[S->iglob] = f[S->iglob] + amp;
+      else {
+        f[S->iglob] = f[S->iglob] + S->dir[0] * amp;
+        f[N + S->iglob] = f[N + S->iglob] + S->dir[1] * amp;
+      }
+    } else {
+      for (int k = 0; k < S->nel; ++k) {
+        for (int c = 0; c < h->ndof; ++c)
+          for (int q = 0; q < NG; ++q) {
+            int g = S->ixi[k * NG + q];
+            f[c * N + g] = f[c * N + g] + amp * S->cxi[((size_t)k * h->ndof + c) * NG + q];
+          }
+        for (int c = 0; c < h->ndof; ++c)
+          for (int q = 0; q < NG; ++q) {
+            int g = S->ieta[k * NG + q];
+            f[c * N + g] = f[c * N + g] + amp * S->ceta[((size_t)k * h->ndof + c) * NG + q];
+          }
+      }
+    }
+  }
+  for (int b = 0; b < h->nabso; ++b) {
+    abso_t *B = &h->abso[b];
+    for (int c = 0; c < h->ndof; ++c)
+      for (int k = 0; k < B->n; ++k) {
+        int g = B->node[k];
+        f[c * N + g] = f[c * N + g] - B->C[c * B->n + k] * v[c * N + g];
+      }
+  }
+  if (h->flt) fault_apply(h, f);
+  for (size_t k = 0; k < n; ++k) a[k] = f[k] * h->rmass[k];
+  double c3 = gam * dt, c4 = beta * dt * dt;
+  for (size_t k = 0; k < n; ++k) {
+    v[k] = v[k] + c3 * a[k];
+    d[k] = d[k] + c4 * a[k];
+  }
+  h->it = it;
+  rec_store(h);
+  if (h->flt) fault_record(h);
+}
+
+int oracle_step(oracle_solver *h, int nsteps) {
+  for (int s = 0; s < nsteps; ++s) one_step(h);
+  return 0;
+}
+
+int oracle_fetch_fields(oracle_solver *h, double *d, double *v, double *a) {
+  size_t n = sizeof(double) * h->ndof * h->npoin;
+  if (d) memcpy(d, h->d, n);
+  if (v) memcpy(v, h->v, n);
+  if (a) memcpy(a, h->a, n);
+  return 0;
+}
+
+int oracle_set_fields(oracle_solver *h, const double *d, const double *v, const double *a) {
+  size_t n = sizeof(double) * h->ndof * h->npoin;
+  if (d) memcpy(h->d, d, n);
+  if (v) memcpy(h->v, v, n);
+  if (a) memcpy(h->a, a, n);
+  return 0;
+}
+
+int oracle_fetch_seismo(oracle_solver *h, double *out, int *taken) {
+  if (h->nrec) memcpy(out, h->seis, sizeof(double) * h->taken * h->ndof * h->nrec);
+  if (taken) *taken = h->taken;
+  return 0;
+}
+
+int oracle_fetch_fault(oracle_solver *h, double *out, int *taken) {
+  if (!h->flt) return fail(h, 4, "no fault");
+  memcpy(out, h->flt->rec, sizeof(double) * h->flt->taken * 6 * h->flt->nout);
+  if (taken) *taken = h->flt->taken;
+  return 0;
+}
+
+int oracle_fetch_fault_state(oracle_solver *h, double *D, double *V, double *T, double *MU,
+                             double *Ts) {
+  fault_t *f = h->flt;
+  if (!f) return fail(h, 4, "no fault");
+  size_t nd = (size_t)f->ndof * f->np;
+  if (D) memcpy(D, f->D, nd * sizeof(double));
+  if (V) memcpy(V, f->V, nd * sizeof(double));
+  if (T) memcpy(T, f->T, 2 * f->np * sizeof(double));
+  if (MU) memcpy(MU, f->MU, f->np * sizeof(double));
+  if (Ts) memcpy(Ts, f->Tstick, 2 * f->np * sizeof(double));
+  return 0;
+}
+
+int oracle_get_step(oracle_solver *h, int *it) { *it = h->it; return 0; }
+const char *oracle_last_error(oracle_solver *h) { return h ? h->err : "null handle"; }
+
+void oracle_destroy(oracle_solver *h) {
+  if (!h) return;
+  free(h->ibool); free(h->coefs); free(h->rmass); free(h->d); free(h->v); free(h->a);
+  for (int b = 0; b < h->nabso; ++b) { free(h->abso[b].node); free(h->abso[b].C); }
+  for (int s = 0; s < h->nsrc; ++s) {
+    free(h->src[s].ixi); free(h->src[s].ieta); free(h->src[s].cxi); free(h->src[s].ceta);
+    free(h->src[s].ampl);
+  }
+  free(h->rnode); free(h->seis);
+  if (h->flt) {
+    fault_t *f = h->flt;
+    free(f->n1); free(f->n2); free(f->nrm); free(f->B); free(f->invM1); free(f->invM2);
+    free(f->Z); free(f->T0); free(f->coh);
+    for (int q = 0; q < 6; ++q) free(f->p[q]);
+    free(f->theta); free(f->Tc); free(f->coeft); free(f->D); free(f->V); free(f->T);
+    free(f->Tstick); free(f->MU); free(f->sigma); free(f->rec); free(f);
+  }
+  free(h);
+}

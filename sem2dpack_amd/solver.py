@@ -1,0 +1,182 @@
+"""Python driver of the C ABI (``include/sem2d.h``) for a built :class:`~builder.Problem`.
+
+:class:`Solver` plays the role of the reference's time loop (``SRC/main.f90:51-99``):
+it creates the device solver from the problem arrays, registers absorbing
+boundaries, sources, receivers and the fault in the reference's order, then
+advances and fetches.  It always drives the HIP library (``libsem2d_hip.so``); a
+missing library raises :class:`~_lib.NativeLibraryMissing`.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from ._lib import load_hip_lib
+
+
+class SolverError(RuntimeError):
+    pass
+
+
+def _p(a, t=np.float64):
+    return a.ctypes.data_as(C.POINTER(C.c_double if t == np.float64 else C.c_int))
+
+
+class Solver:
+    FIELD_CODE = {"D": 0, "V": 1, "A": 2}
+
+    def __init__(self, pb, bulk_fp32=False, api=None, structured=True):
+        self.api = api if api is not None else load_hip_lib()
+        self.pb = pb
+        self.ndof, self.npoin = pb.ndof, pb.npoin
+        self._keep = []
+        ibool = np.ascontiguousarray(pb.ibool, dtype=np.int32)
+        coefs = np.ascontiguousarray(pb.coefs, dtype=np.float64)
+        hprime = np.ascontiguousarray(pb.hprime.T, dtype=np.float64)  # [j][i]
+        rmass = np.ascontiguousarray(pb.rmass, dtype=np.float64)
+        d = capi.Desc()
+        d.ndof, d.ngll, d.nelem, d.npoin = pb.ndof, 5, pb.nelem, pb.npoin
+        d.nelast = pb.nelast
+        d.coef_shared = 1 if pb.coef_mode == "shared" else 0
+        d.ibool, d.coefs, d.hprime, d.rmass = _p(ibool, np.int32), _p(coefs), _p(hprime), _p(rmass)
+        d.dt, d.beta, d.gamma = pb.dt, pb.beta, pb.gamma
+        d.bulk_fp32 = 1 if bulk_fp32 else 0
+        cells = None
+        if structured and getattr(pb, "mesh", None) is not None:
+            m = pb.mesh
+            cells = np.empty(m["nx"] * m["nz"], dtype=np.int32)
+            cells[pb.elem_iz * m["nx"] + pb.elem_ix] = np.arange(1, pb.nelem + 1, dtype=np.int32)
+            d.nx, d.nz, d.ezflt = m["nx"], m["nz"], max(m["ezflt"], 0)
+            d.elem_of_cell = _p(cells, np.int32)
+        h = C.c_void_p()
+        rc = self.api.create(C.byref(d), C.byref(h))
+        if rc != 0:
+            raise SolverError(f"{self.api.prefix}create failed ({rc})")
+        self.h = h
+        self._setup()
+
+    # ------------------------------------------------------------------
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.api.last_error(self.h)
+            raise SolverError(f"{self.api.prefix}{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def _setup(self):
+        pb = self.pb
+        for ab in pb.abso:
+            node = np.ascontiguousarray(ab["node"], dtype=np.int32)
+            Cm = np.ascontiguousarray(ab["C"].T, dtype=np.float64)  # [ndof][n]
+            self._check(self.api.set_abso(self.h, len(node), _p(node, np.int32), _p(Cm)), "set_abso")
+        for s in pb.sources:
+            isrc = C.c_int()
+            if s["kind"] == "force":
+                dirv = np.ascontiguousarray(s["dir"], dtype=np.float64)
+                self._check(self.api.set_force(self.h, int(s["iglob"]), _p(dirv), C.byref(isrc)), "set_force")
+            else:
+                ixi = np.ascontiguousarray(s["iglob_xi"], dtype=np.int32)
+                ieta = np.ascontiguousarray(s["iglob_eta"], dtype=np.int32)
+                cxi = np.ascontiguousarray(s["coef_xi"], dtype=np.float64)
+                ceta = np.ascontiguousarray(s["coef_eta"], dtype=np.float64)
+                self._check(self.api.set_moment(self.h, ixi.shape[0], _p(ixi, np.int32), _p(ieta, np.int32),
+                                                _p(cxi), _p(ceta), C.byref(isrc)), "set_moment")
+            tab = np.ascontiguousarray(s["stf_table"], dtype=np.float64)
+            self._check(self.api.set_stf(self.h, isrc.value, len(tab), _p(tab)), "set_stf")
+        if pb.fault is not None:
+            self._set_fault(pb.fault)
+        r = pb.receivers
+        if r is not None:
+            node = np.ascontiguousarray(r["iglob"], dtype=np.int32)
+            self.nrec, self.nsamp = len(node), r["nsamp"]
+            self._check(self.api.set_receivers(self.h, len(node), _p(node, np.int32), int(r["isamp"]),
+                                               self.FIELD_CODE[r["field"]], int(r["nsamp"])), "set_receivers")
+
+    def _set_fault(self, f):
+        nd = self.ndof
+        fd = capi.FaultDesc()
+        keep = []
+
+        def arr(a, t=np.float64):
+            a = np.ascontiguousarray(a, dtype=t)
+            keep.append(a)
+            return _p(a, t)
+
+        npf = len(f["node1"])
+        fd.npoin = npf
+        fd.node1, fd.node2 = arr(f["node1"], np.int32), arr(f["node2"], np.int32)
+        fd.n1 = arr(f["n1"].T)
+        fd.B, fd.invM1, fd.invM2, fd.Z = arr(f["B"].T), arr(f["invM1"].T), arr(f["invM2"].T), arr(f["Z"].T)
+        fd.T0 = arr(f["T0"].T)
+        fd.cohesion = arr(f["cohesion"])
+        fd.coefA2V, fd.coefA2D = f["coefA2V"], f["coefA2D"]
+        fd.allow_opening = 1 if f["opening"] else 0
+        if f["law"] == "SWF":
+            fd.law, fd.kind, fd.healing = 1, f["swf_kind"], 1 if f["healing"] else 0
+            names = ("dc", "mus", "mud", "alpha", "p")
+        else:
+            fd.law, fd.kind, fd.healing = 2, f["rsf_kind"], 0
+            names = ("dc", "mus", "a", "b", "vstar", "vc")
+        for q, nme in enumerate(names):
+            fd.p[q] = arr(f[nme])
+        fd.theta0 = arr(f["theta"])
+        fd.V0 = arr(f["V0"][:, 0])
+        fd.oit, fd.oitd, fd.oix1, fd.oixn, fd.oixd = f["oit"], f["oitd"], f["oix1"], f["oixn"], f["oixd"]
+        self.nout = (f["oixn"] - f["oix1"]) // f["oixd"] + 1
+        self.fault_nsamp = (self.pb.nt - f["oit"]) // f["oitd"] + 1
+        self.npf = npf
+        self._check(self.api.set_fault(self.h, C.byref(fd), int(self.fault_nsamp)), "set_fault")
+
+    # ------------------------------------------------------------------
+    def step(self, n):
+        self._check(self.api.step(self.h, int(n)), "step")
+
+    def fields(self):
+        sh = (self.ndof, self.npoin)
+        d, v, a = np.empty(sh), np.empty(sh), np.empty(sh)
+        self._check(self.api.fetch_fields(self.h, _p(d), _p(v), _p(a)), "fetch_fields")
+        return d, v, a
+
+    def set_fields(self, d=None, v=None, a=None):
+        arrs = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (d, v, a)]
+        ptrs = [None if x is None else _p(x) for x in arrs]
+        self._check(self.api.set_fields(self.h, *ptrs), "set_fields")
+
+    def seismograms(self):
+        out = np.empty((self.nsamp, self.ndof, self.nrec))
+        n = C.c_int()
+        self._check(self.api.fetch_seismo(self.h, _p(out), C.byref(n)), "fetch_seismo")
+        return out[:n.value]
+
+    def fault_records(self):
+        out = np.empty((self.fault_nsamp, 6, self.nout))
+        n = C.c_int()
+        self._check(self.api.fetch_fault(self.h, _p(out), C.byref(n)), "fetch_fault")
+        return out[:n.value]
+
+    def fault_state(self):
+        nd, npf = self.ndof, self.npf
+        D, V = np.empty((nd, npf)), np.empty((nd, npf))
+        T, Ts, MU = np.empty((2, npf)), np.empty((2, npf)), np.empty(npf)
+        self._check(self.api.fetch_fault_state(self.h, _p(D), _p(V), _p(T), _p(MU), _p(Ts)), "fetch_fault_state")
+        return dict(D=D, V=V, T=T, MU=MU, Tstick=Ts)
+
+    def fint(self, d):
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        f = np.empty_like(d)
+        self._check(self.api.compute_fint(self.h, _p(d), _p(f)), "compute_fint")
+        return f
+
+    def last_step_ms(self):
+        ms = C.c_double()
+        self._check(self.api.last_step_ms(self.h, C.byref(ms)), "last_step_ms")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.api.destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
