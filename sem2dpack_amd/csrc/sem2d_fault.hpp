@@ -1,0 +1,255 @@
+// Dynamic fault boundary: pointwise traction solve on split-node pairs.
+//
+// One lane per fault node pair.  All fault arithmetic is FP64 whatever the bulk
+// type, and floating-point contraction is disabled so that, given the same
+// nodal inputs, every operation rounds as in the reference.
+//   BC_DYNFLT_apply      SRC/bc_dynflt.f90:569-739
+//   slip weakening       SRC/bc_dynflt_swf.f90:142-194
+//   rate and state       SRC/bc_dynflt_rsf.f90:165-568 (two-pass rsf_solver,
+//                        aging/slip/V-shape state updates, safeguarded
+//                        Newton-bisection nr_solver returning the slip rate of
+//                        its last function evaluation)
+//   Coulomb normal law   SRC/bc_dynflt_normal.f90:118-136 (kind 1)
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sem2d {
+
+struct FaultDev {
+  int np, ndof;
+  const int *n1, *n2;                    // 0-based node pairs
+  const double *nrm;                     // [2][np]
+  const double *B, *invM1, *invM2, *Z;   // [ndof][np]
+  const double *T0;                      // [2][np]
+  const double *coh;                     // [np]
+  double a2v, a2d, dt;
+  int opening, law, kind, healing;
+  const double *p0, *p1, *p2, *p3, *p4, *p5;  // friction parameters [np]
+  const double *Tc, *coeft;              // RSF kind 1 helpers
+  double *theta;                         // [np] state
+  double *D, *V;                         // [ndof][np]
+  double *T, *Tstick;                    // [2][np]
+  double *MU, *sigma;                    // [np]
+  int *nr_fail;                          // Newton iteration overflow flag
+};
+
+#pragma clang fp contract(off)
+
+__device__ inline double f_swf_mu(const FaultDev& F, int k) {
+  const double th = F.theta[k], dc = F.p0[k], mus = F.p1[k], mud = F.p2[k];
+  double mu;
+  if (F.kind == 1) mu = mus - (mus - mud) * fmin(th / dc, 1.0);
+  else if (F.kind == 2) mu = mud - (mud - mus) * exp(-th / dc);
+  else mu = mud + (mus - mud) / pow(1.0 + th / dc, F.p4[k]);
+  return mu + F.p3[k] * th;
+}
+
+__device__ inline double f_rsf_mu(const FaultDev& F, int k, double v) {
+  const double dc = F.p0[k], mus = F.p1[k], a = F.p2[k], b = F.p3[k], vs = F.p4[k], th = F.theta[k];
+  switch (F.kind) {
+    case 1: return mus + a * fabs(v) / (fabs(v) + vs) - b * th / (th + dc);
+    case 2:
+    case 3: return a * asinh(fabs(v) / (2.0 * vs) * exp((mus + b * log(vs * th / dc)) / a));
+    default: return a * asinh(fabs(v) / (2.0 * vs) * exp((mus + b * log(F.p5[k] * th / dc + 1)) / a));
+  }
+}
+
+__device__ inline double f_rsf_theta(const FaultDev& F, int k, double theta, double v) {
+  const double dc = F.p0[k], dt = F.dt;
+  switch (F.kind) {
+    case 1: {
+      double tn = theta * F.coeft[k] + F.Tc[k] * fabs(v) * (1.0 - F.coeft[k]);
+      return tn < 1.0e-12 ? 0.0 : tn;
+    }
+    case 2:
+    case 4: {
+      const double x = fabs(v) / dc;
+      const double ex = exp(-dt * x);
+      if (dt * x > 1e-8) return theta * ex + (1.0 - ex) / x;
+      return theta * ex + dt * (1.0 - 0.5 * dt * x);
+    }
+    default: {
+      const double tn = dc / fabs(v);
+      return tn * pow(theta / tn, exp(-dt / tn));
+    }
+  }
+}
+
+__device__ inline void f_fric(const FaultDev& F, int k, double theta, double tau, double tstick,
+                              double sigma, double Z, double& fv, double& dfv, double& v) {
+  const double dc = F.p0[k], mus = F.p1[k], a = F.p2[k], b = F.p3[k], vs = F.p4[k];
+  double tmp = (F.kind == 4) ? mus + b * log(F.p5[k] * theta / dc + 1.0) : mus + b * log(vs * theta / dc);
+  tmp = 2.0 * vs * exp(-tmp / a);
+  v = sinh(tau / (-sigma * a)) * tmp;
+  fv = tstick - Z * v - tau;
+  const double dv = cosh(tau / (-sigma * a)) * tmp / (-sigma * a);
+  dfv = -Z * dv - 1.0;
+}
+
+__device__ inline double f_nr_solver(const FaultDev& F, int k, double xL, double xR, double xacc,
+                                     double theta, double tstick, double sigma, double Z, bool& ok) {
+  double flow, fhigh, dfdx, v, xlow, xhigh, dx, dxold, fx;
+  ok = true;
+  f_fric(F, k, theta, xL, tstick, sigma, Z, flow, dfdx, v);
+  f_fric(F, k, theta, xR, tstick, sigma, Z, fhigh, dfdx, v);
+  double xl = xL, xr = xR;
+  int guard = 0;
+  while (flow * fhigh > 0 && guard++ < 2100) {
+    xl = xl / 2.0;
+    xr = xr * 2.0;
+    f_fric(F, k, theta, xl, tstick, sigma, Z, flow, dfdx, v);
+    f_fric(F, k, theta, xr, tstick, sigma, Z, fhigh, dfdx, v);
+  }
+  if (flow * fhigh > 0) { ok = false; return v; }
+  if (flow == 0) return v;
+  if (fhigh == 0) return v;
+  if (flow < 0) { xlow = xl; xhigh = xr; } else { xhigh = xl; xlow = xr; }
+  double xest = 0.5 * (xl + xr);
+  dxold = fabs(xr - xl);
+  dx = dxold;
+  f_fric(F, k, theta, xest, tstick, sigma, Z, fx, dfdx, v);
+  for (int is = 1; is <= 200; ++is) {
+    if (((xest - xhigh) * dfdx - fx) * ((xest - xlow) * dfdx - fx) > 0 ||
+        fabs(2 * fx) > fabs(dxold * dfdx)) {
+      dxold = dx;
+      dx = 0.5 * (xhigh - xlow);
+      xest = xlow + dx;
+      if (xlow == xest) return v;
+    } else {
+      dxold = dx;
+      dx = fx / dfdx;
+      const double tmp = xest;
+      xest = xest - dx;
+      if (tmp == xest) return v;
+    }
+    if (fabs(dx) < fabs(xacc)) {
+      f_fric(F, k, theta, xest, tstick, sigma, Z, fx, dfdx, v);
+      return v;
+    }
+    f_fric(F, k, theta, xest, tstick, sigma, Z, fx, dfdx, v);
+    if (fx < 0) xlow = xest; else xhigh = xest;
+  }
+  ok = false;
+  return v;
+}
+
+__device__ inline double f_rsf_v(const FaultDev& F, int k, double tstick, double sigma, double theta,
+                                 double Z, bool& ok) {
+  ok = true;
+  if (F.kind == 1) {
+    const double dc = F.p0[k], mus = F.p1[k], a = F.p2[k], b = F.p3[k], vs = F.p4[k];
+    const double mnd = mus - b * theta / (theta + dc);
+    double v = (tstick + sigma * mnd) / Z;
+    const double tmp = v - vs + sigma * a / Z;
+    v = 0.5 * (tmp + sqrt(tmp * tmp + 4.0 * v * vs));
+    v = fmax(0.0, v);
+    return v < 1.0e-12 ? 0.0 : v;
+  }
+  const double tol = -0.001 * F.p2[k] * sigma;
+  return f_nr_solver(F, k, fmin(0.0, tstick), fmax(0.0, tstick), tol, theta, tstick, sigma, Z, ok);
+}
+
+template <typename R>
+__global__ void k_fault(FaultDev F, const R* __restrict__ d, const R* __restrict__ v, R* __restrict__ f,
+                        int npoin) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= F.np) return;
+  const int np = F.np, nd = F.ndof;
+  const int i1 = F.n1[k], i2 = F.n2[k];
+  double dD[2] = {0, 0}, dV[2] = {0, 0}, dA[2] = {0, 0}, T[2] = {0, 0}, Ts[2];
+  for (int c = 0; c < nd; ++c) {
+    const size_t o = (size_t)c * npoin;
+    dD[c] = (double)d[o + i2] - (double)d[o + i1];
+    dV[c] = (double)v[o + i2] - (double)v[o + i1];
+    dA[c] = F.invM2[c * np + k] * (double)f[o + i2] - F.invM1[c * np + k] * (double)f[o + i1];
+    T[c] = F.Z[c * np + k] * (dV[c] + F.a2v * dA[c]);
+  }
+  const double nx = F.nrm[k], nz = F.nrm[np + k];
+  if (nd == 2) {
+    double r0, r1;
+    r0 = nz * dD[0] - nx * dD[1]; r1 = nx * dD[0] + nz * dD[1]; dD[0] = r0; dD[1] = r1;
+    r0 = nz * dV[0] - nx * dV[1]; r1 = nx * dV[0] + nz * dV[1]; dV[0] = r0; dV[1] = r1;
+    r0 = nz * dA[0] - nx * dA[1]; r1 = nx * dA[0] + nz * dA[1]; dA[0] = r0; dA[1] = r1;
+    r0 = nz * T[0] - nx * T[1];   r1 = nx * T[0] + nz * T[1];   T[0] = r0;  T[1] = r1;
+  } else {
+    T[1] = 0.0;
+  }
+  T[0] = T[0] + F.T0[k];
+  T[1] = T[1] + F.T0[np + k];
+  if (F.opening) T[1] = fmin(T[1], 0.0);
+  F.sigma[k] = T[1];
+  const double sigma = T[1];
+  Ts[0] = T[0];
+  Ts[1] = T[1];
+  if (F.law == 2) {
+    const double vold = F.V[k], Z1 = F.Z[k];
+    bool ok1, ok2;
+    double th = f_rsf_theta(F, k, F.theta[k], vold);
+    double vn = f_rsf_v(F, k, T[0], sigma, th, Z1, ok1);
+    th = f_rsf_theta(F, k, F.theta[k], 0.5 * (vold + vn));
+    vn = f_rsf_v(F, k, T[0], sigma, th, Z1, ok2);
+    if (!(ok1 && ok2)) atomicOr(F.nr_fail, 1);
+    F.theta[k] = th;
+    F.V[k] = vn;
+    F.MU[k] = f_rsf_mu(F, k, vn);
+    const double strength = -F.MU[k] * sigma;
+    T[0] = copysign(fabs(strength), T[0]);
+  } else {
+    if (F.a2d == 0.0) {
+      if (F.healing) {
+        double th = F.theta[k] + fabs(dV[0]) * F.dt;
+        if (fabs(dV[0]) < 1e-14) th = 0.0;
+        F.theta[k] = th;
+      } else {
+        F.theta[k] = fabs(dD[0]);
+      }
+    } else {
+      F.theta[k] = fabs(F.D[k]);
+    }
+    F.MU[k] = f_swf_mu(F, k);
+    const double strength = F.coh[k] - F.MU[k] * sigma;
+    T[0] = copysign(fmin(fabs(T[0]), strength), T[0]);
+  }
+  T[0] = T[0] - F.T0[k];
+  T[1] = T[1] - F.T0[np + k];
+  Ts[0] = Ts[0] - F.T0[k];
+  Ts[1] = Ts[1] - F.T0[np + k];
+  F.T[k] = T[0];
+  F.T[np + k] = T[1];
+  F.Tstick[k] = Ts[0];
+  F.Tstick[np + k] = Ts[1];
+  double Tx[2] = {T[0], T[1]};
+  if (nd == 2) {
+    Tx[0] = nz * T[0] + nx * T[1];
+    Tx[1] = -nx * T[0] + nz * T[1];
+  }
+  for (int c = 0; c < nd; ++c) {
+    const size_t o = (size_t)c * npoin;
+    const double bt = F.B[c * np + k] * Tx[c];
+    f[o + i1] = (R)((double)f[o + i1] + bt);
+    f[o + i2] = (R)((double)f[o + i2] - bt);
+  }
+  for (int c = 0; c < nd; ++c) {
+    const double tf = T[c];
+    const double da = dA[c] - tf / (F.Z[c * np + k] * F.a2v);
+    F.D[c * np + k] = dD[c] + F.a2d * da;
+    F.V[c * np + k] = dV[c] + F.a2v * da;
+  }
+}
+
+// fault record: out[6][nout] from the current state (SRC/bc_dynflt.f90:762-767)
+__global__ void k_fault_record(FaultDev F, int oix1, int oixd, int nout, double* __restrict__ out) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nout) return;
+  const int k = oix1 - 1 + o * oixd;
+  out[0 * nout + o] = F.D[k];
+  out[1 * nout + o] = F.V[k];
+  out[2 * nout + o] = F.T[k];
+  out[3 * nout + o] = F.T[F.np + k];
+  out[4 * nout + o] = F.MU[k];
+  out[5 * nout + o] = F.Tstick[k];
+}
+
+#pragma clang fp contract(on)
+
+}  // namespace sem2d

@@ -1,0 +1,188 @@
+// Device kernels of the explicit SEM time step (gfx950, wave64).
+//
+// Field layout: component-major SoA, f[c*npoin + node] (the reference's
+// field(npoin,ndof) column-major layout, SRC/fields.f90:7-11).  R is the bulk
+// type (double, or float for the FP32 bulk variant); the fault solve always
+// works in double.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sem2d {
+
+constexpr int NG = 5;
+constexpr int NP = NG * NG;
+
+// h'_i(x_j) stored at H[i*5+j] (set once per library load, it only depends on NGLL)
+__constant__ double c_H[NP];
+
+template <typename R>
+struct Coef {
+  static __device__ __forceinline__ R h(int a, int b) { return (R)c_H[a * NG + b]; }
+};
+
+// ---------------------------------------------------------------------------
+// Element internal forces, general (any ibool), one colour per launch.
+//
+// f = -K d for the elements e0 .. e0+ne-1 (consecutive in the colour-sorted
+// element order), assembled into f with plain read-modify-write: elements of
+// one colour share no node, so the update is race-free and deterministic.
+// Follows ELAST_KD2_PSV / ELAST_KD2_SH (SRC/mat_elastic.f90:589-641, 739-775):
+//   dU/dxi(i,j)  = sum_k H(k,i) U(k,j)      dU/deta(i,j) = sum_k U(i,k) H(k,j)
+//   f(i,j)       = sum_k H(i,k) t1(k,j) + sum_k t2(i,k) H(j,k)
+// One thread per GLL point, 10 elements (250 threads) per workgroup.
+// ---------------------------------------------------------------------------
+constexpr int FE_EPB = 10;
+constexpr int FE_THREADS = FE_EPB * NP;
+
+template <typename R, int NDOF, int NELAST, bool SHARED>
+__global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int* __restrict__ ibool,
+                                                      const R* __restrict__ coefs,
+                                                      const R* __restrict__ d, R* __restrict__ f,
+                                                      int npoin) {
+  __shared__ R U[FE_EPB][NDOF][NP];
+  __shared__ R T1[FE_EPB][NDOF][NP];
+  __shared__ R T2[FE_EPB][NDOF][NP];
+  const int t = threadIdx.x;
+  const int le = t / NP;
+  const int p = t - le * NP;
+  const int i = p % NG, j = p / NG;
+  const int el = blockIdx.x * FE_EPB + le;
+  const bool active = (t < FE_THREADS) && (el < ne);
+  const int e = e0 + el;
+  int node = 0;
+  if (active) {
+    node = ibool[(size_t)e * NP + p];
+#pragma unroll
+    for (int c = 0; c < NDOF; ++c) U[le][c][p] = d[(size_t)c * npoin + node];
+  }
+  __syncthreads();
+  if (active) {
+    const R* a = coefs + (SHARED ? 0 : (size_t)e * NELAST * NP) + p;
+    R dxi[NDOF], deta[NDOF];
+#pragma unroll
+    for (int c = 0; c < NDOF; ++c) {
+      R s1 = 0, s2 = 0;
+#pragma unroll
+      for (int k = 0; k < NG; ++k) {
+        s1 += Coef<R>::h(k, i) * U[le][c][j * NG + k];
+        s2 += U[le][c][k * NG + i] * Coef<R>::h(k, j);
+      }
+      dxi[c] = s1;
+      deta[c] = s2;
+    }
+    if constexpr (NDOF == 2) {
+      if constexpr (NELAST == 6) {
+        const R a1 = a[0], a2 = a[NP], a3 = a[2 * NP], a4 = a[3 * NP], a5 = a[4 * NP], a6 = a[5 * NP];
+        T1[le][0][p] = a1 * dxi[0] + a2 * deta[1];
+        T2[le][0][p] = a4 * (deta[0] + dxi[1]);
+        T1[le][1][p] = a5 * deta[0] + a6 * dxi[1];
+        T2[le][1][p] = a2 * dxi[0] + a3 * deta[1];
+      } else {
+        R A[10];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) A[k] = a[k * NP];
+        T1[le][0][p] = A[0] * dxi[0] + A[6] * deta[0] + A[7] * dxi[1] + A[1] * deta[1];
+        T2[le][0][p] = A[6] * dxi[0] + A[3] * deta[0] + A[4] * dxi[1] + A[8] * deta[1];
+        T1[le][1][p] = A[7] * dxi[0] + A[4] * deta[0] + A[5] * dxi[1] + A[9] * deta[1];
+        T2[le][1][p] = A[1] * dxi[0] + A[8] * deta[0] + A[9] * dxi[1] + A[2] * deta[1];
+      }
+    } else {
+      if constexpr (NELAST == 2) {
+        T1[le][0][p] = a[0] * dxi[0];
+        T2[le][0][p] = a[NP] * deta[0];
+      } else {
+        const R a1 = a[0], a2 = a[NP], a3 = a[2 * NP];
+        T1[le][0][p] = a1 * dxi[0] + a3 * deta[0];
+        T2[le][0][p] = a3 * dxi[0] + a2 * deta[0];
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < NDOF; ++c) {
+      R s1 = 0, s2 = 0;
+#pragma unroll
+      for (int k = 0; k < NG; ++k) {
+        s1 += Coef<R>::h(i, k) * T1[le][c][j * NG + k];
+        s2 += T2[le][c][k * NG + i] * Coef<R>::h(j, k);
+      }
+      R* fp = f + (size_t)c * npoin + node;
+      *fp = *fp + (s1 + s2);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Nodal kernels
+// ---------------------------------------------------------------------------
+// Newmark predictor (SRC/solver.f90:59-60) and f = 0 (f aliases a, SRC/solver.f90:52,286)
+template <typename R>
+__global__ void k_predict(size_t n, R* __restrict__ d, R* __restrict__ v, R* __restrict__ a, R dt,
+                          R c1, R c2) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const R ak = a[k];
+    const R vk = v[k];
+    d[k] = d[k] + dt * vk + c1 * ak;
+    v[k] = vk + c2 * ak;
+    a[k] = 0;
+  }
+}
+
+// a = f*rmass, corrector (SRC/solver.f90:78-82)
+template <typename R>
+__global__ void k_correct(size_t n, R* __restrict__ d, R* __restrict__ v, R* __restrict__ a,
+                          const R* __restrict__ rmass, R c3, R c4) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const R ak = a[k] * rmass[k];
+    a[k] = ak;
+    v[k] = v[k] + c3 * ak;
+    d[k] = d[k] + c4 * ak;
+  }
+}
+
+// Point sources at step it (SRC/src_gen.f90:290-317).  Entries are applied in
+// order by a single lane (a handful of nodes).
+struct SrcEntry {
+  int node;      // 0-based
+  int comp;      // component
+  int src;       // source index (row of the amplitude table)
+  double coef;   // dir(c) for forces, coef_xi / coef_eta for moment stencils
+};
+
+template <typename R>
+__global__ void k_sources(int nent, const SrcEntry* __restrict__ ent, const double* __restrict__ ampl,
+                          int nt, int it, R* __restrict__ f, int npoin) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int q = 0; q < nent; ++q) {
+    const SrcEntry s = ent[q];
+    const double am = (it >= 1 && it <= nt) ? ampl[(size_t)s.src * nt + (it - 1)] : 0.0;
+    R* fp = f + (size_t)s.comp * npoin + s.node;
+    *fp = (R)((double)*fp + s.coef * am);
+  }
+}
+
+// Absorbing boundary, one boundary per launch (SRC/bc_abso.f90:323-325)
+template <typename R>
+__global__ void k_abso(int n, int ndof, const int* __restrict__ node, const double* __restrict__ C,
+                       const R* __restrict__ v, R* __restrict__ f, int npoin) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int g = node[k];
+  for (int c = 0; c < ndof; ++c) {
+    R* fp = f + (size_t)c * npoin + g;
+    *fp = (R)((double)*fp - C[(size_t)c * n + k] * (double)v[(size_t)c * npoin + g]);
+  }
+}
+
+// receivers: out[s][c][r] = field[c][node[r]]
+template <typename R>
+__global__ void k_receivers(int nrec, int ndof, const int* __restrict__ node, const R* __restrict__ field,
+                            int npoin, double* __restrict__ out) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  for (int c = 0; c < ndof; ++c) out[(size_t)c * nrec + r] = (double)field[(size_t)c * npoin + node[r]];
+}
+
+}  // namespace sem2d
