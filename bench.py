@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the MI355X explicit SEM time step.
+
+Workload (the large in-plane throughput configuration): P-SV, 200 km x 100 km,
+2000 x 1000 elements, NGLL = 5 (32,012,001 global GLL nodes), 8-layer medium
+(vp 4000 -> 7000 m/s) with a seeded 10 % Gaussian random field (2 km correlation),
+vs = vp/sqrt(3), rho = 770 + 0.32 vp, explosive Ricker source (f0 = 1 Hz), four
+absorbing boundaries.  A "step" is one Newmark step of the whole mesh; inputs are
+resident in HBM before the timed region.
+
+Metric: GLL node-updates/s = npoin * steps / seconds (FP64 fields).  The same run is
+repeated with the FP32 bulk update (fault/time accumulators in FP64) and reported
+under "variants".
+
+Roofline: algorithmic bytes per step B = npoin*(6*ndof+1)*s + nelem*25*(4+2*s)
+(d, v, a read and written, one inverse-mass word per node, int32 ibool and two
+moduli per element-local point; s = 8 or 4), divided by the step time measured
+with HIP events on the solver's stream, against 8 TB/s.
+
+CPU baseline: the reference SEM2DPACK Fortran solver (built from source by
+oracle/build_ref.py) on one core, on a 500 x 250-element window of the same medium
+(same element size and statistics), timed with its own "CPU time per timestep".
+"""
+import argparse
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+CPU_SAMPLE = dict(nx=500, nz=250, lx=50e3, lz=25e3, steps=60)
+
+
+def alg_bytes(npoin, nelem, ndof, s, m=2):
+    return npoin * (6 * ndof + 1) * s + nelem * 25 * (4 + m * s)
+
+
+def cpu_baseline():
+    """Reference Fortran solver on a window of the medium; returns dict or None."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "sem2dsolve")
+    if not os.path.exists(exe):
+        return None
+    from sem2dpack_amd import workloads
+    S = CPU_SAMPLE
+    with tempfile.TemporaryDirectory(prefix="sem2d_cpu_") as tmp:
+        grid = workloads.layered_random_grid((0.0, S["lx"]), (0.0, S["lz"]), dx=100.0)
+        workloads.write_hete1(os.path.join(tmp, "c3_media.tab"), *grid)
+        with open(os.path.join(tmp, "Par.inp"), "w") as fh:
+            fh.write(workloads.par_throughput(S["nx"], S["nz"], S["lx"], S["lz"], S["steps"]))
+        cmd = f"ulimit -s unlimited; exec taskset -c 0 {exe}"
+        t0 = time.time()
+        r = subprocess.run(["bash", "-c", cmd], cwd=tmp, capture_output=True, text=True, timeout=600)
+        wall = time.time() - t0
+        out = r.stdout
+        m = re.findall(r"CPU time per timestep[ .]*([0-9.+\-Ee]+)", out)
+        if r.returncode != 0 or "FATAL" in out or not m:
+            return {"value": None, "error": (out + r.stderr)[-400:]}
+        per_step = float(m[-1])
+        npoin = (4 * S["nx"] + 1) * (4 * S["nz"] + 1)
+    return {"value": npoin / per_step, "unit": "GLL node-updates/s", "cores": 1, "kind": "reference",
+            "sample": f"{S['nx']}x{S['nz']}-element window (npoin={npoin}) of the same medium, {S['steps']} "
+                      f"steps, reference sem2dsolve flang -O2 on 1 pinned core; {per_step * 1e3:.2f} ms/step "
+                      f"(its own CPU-time-per-step report; wall {wall:.1f} s incl. init)"}
+
+
+def run_gpu(pb, steps, warmup, fp32):
+    from sem2dpack_amd import solver
+    s = solver.Solver(pb, bulk_fp32=fp32)
+    s.step(warmup)
+    s.fields()
+    t0 = time.perf_counter()
+    s.step(steps)
+    t1 = time.perf_counter()
+    ev_ms = s.last_step_ms()
+    d, v, a = s.fields()
+    s.close()
+    import numpy as np
+    finite = bool(np.isfinite(v).all())
+    return (t1 - t0), ev_ms, finite
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is a design note, not an implemented path.", file=sys.stderr)
+        sys.exit(2)
+
+    from sem2dpack_amd import workloads
+    t0 = time.time()
+    pb = workloads.build_throughput(nt=args.steps + args.warmup)
+    build_s = time.time() - t0
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+
+    npoin, nelem, ndof = pb.npoin, pb.nelem, pb.ndof
+    wall, ev_ms, finite = run_gpu(pb, args.steps, args.warmup, False)
+    B64 = alg_bytes(npoin, nelem, ndof, 8)
+    value = npoin * args.steps / wall
+    ach = B64 / (ev_ms * 1e-3)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "c3_fp64_hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_step")
+    line = {
+        "metric": "GLL node-updates/s",
+        "value": value,
+        "unit": "GLL node-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "P-SV 2000x1000 elements NGLL=5, layered random medium, explosion, 4 ABCs",
+                   "npoin": npoin, "nelem": nelem, "dt": pb.dt, "bulk": "fp64", "build_s": round(build_s, 1)},
+        "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK, "traffic": traffic,
+                     "alg_bytes_per_step": B64, "event_ms_per_step": ev_ms},
+        "cpu_baseline": cpu,
+        "finite": finite,
+    }
+    if not args.no_fp32:
+        wall32, ev32, fin32 = run_gpu(pb, args.steps, args.warmup, True)
+        B32 = alg_bytes(npoin, nelem, ndof, 4)
+        line["variants"] = {"fp32_bulk": {
+            "value": npoin * args.steps / wall32, "ms_per_step": 1e3 * wall32 / args.steps,
+            "roofline_frac": B32 / (ev32 * 1e-3) / HBM_PEAK, "achieved_GBs": B32 / (ev32 * 1e-3) / 1e9,
+            "alg_bytes_per_step": B32, "finite": fin32}}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -14,8 +14,9 @@
  *     so each component is contiguous;
  *   - ibool is [nelem][ngll][ngll] (Fortran ibool(i,j,e), i fastest);
  *   - element coefficients are [nelem][nelast][ngll][ngll] (Fortran a(i,j,k) of
- *     each element, SRC/mat_elastic.f90:11-14), or a single [nelast][ngll][ngll]
- *     set shared by all elements (SRC/mat_gen.f90:357-365);
+ *     each element, SRC/mat_elastic.f90:11-14), a single [nelast][ngll][ngll]
+ *     set shared by all elements (SRC/mat_gen.f90:357-365), or, on flat grids,
+ *     the moduli per GLL node with three geometric factors per element;
  *   - hprime is [ngll][ngll] column-major: hprime[j*ngll+i] = h'_i(x_j).
  * Every call returns 0 on success and a non-zero code otherwise; the message is
  * available from sem2d_last_error (the reference prints and stops instead,
@@ -38,9 +39,15 @@ typedef struct sem2d_desc {
   int ngll;              /* must be 5 */
   int nelem, npoin;
   int nelast;            /* 2, 3 (SH flat/general), 6, 10 (P-SV flat/general) */
-  int coef_shared;       /* 1: coefs holds one set for all elements */
+  int coef_mode;         /* 0: coefs per element; 1: one coefs set shared by all elements;
+                            2: compact flat-grid form (lam_mu, geom, wgll; nelast 2 or 6) */
   const int *ibool;      /* [nelem][ngll][ngll], 1-based */
-  const double *coefs;   /* [nelem or 1][nelast][ngll][ngll] */
+  const double *coefs;   /* modes 0, 1: [nelem or 1][nelast][ngll][ngll] */
+  const double *lam_mu;  /* mode 2: [nelem][ndof][ngll][ngll]: lambda, mu (P-SV) or mu (SH) */
+  const double *geom;    /* mode 2: [nelem][3]: -detJ*DxiDx^2, -detJ*DetaDz^2, -detJ*DxiDx*DetaDz,
+                            so that a(i,j,k) = wgll(i)*wgll(j) * modulus(i,j) * factor
+                            (SRC/mat_elastic.f90:334-340, 356-358) */
+  const double *wgll;    /* mode 2: [ngll] GLL weights */
   const double *hprime;  /* [ngll][ngll], hprime[j*ngll+i] = h'_i(x_j) */
   const double *rmass;   /* [ndof][npoin], inverse of the assembled diagonal mass */
   double dt, beta, gamma;/* Newmark time step and coefficients (SRC/time.f90:5-11) */

@@ -107,10 +107,11 @@ int oracle_create(const sem2d_desc *s, oracle_solver **out) {
   if (s->ngll != NG) return 2;
   oracle_solver *h = (oracle_solver *)calloc(1, sizeof *h);
   h->ndof = s->ndof; h->nelem = s->nelem; h->npoin = s->npoin; h->nelast = s->nelast;
-  h->shared = s->coef_shared;
+  if (s->coef_mode == 2) { free(h); return 2; } /* the oracle uses the reference's a() arrays */
+  h->shared = s->coef_mode == 1;
   h->ibool = (int *)malloc(sizeof(int) * (size_t)s->nelem * NG * NG);
   for (size_t k = 0; k < (size_t)s->nelem * NG * NG; ++k) h->ibool[k] = s->ibool[k] - 1;
-  size_t nc = (size_t)(s->coef_shared ? 1 : s->nelem) * s->nelast * NG * NG;
+  size_t nc = (size_t)(h->shared ? 1 : s->nelem) * s->nelast * NG * NG;
   h->coefs = dup_d(s->coefs, nc);
   for (int j = 0; j < NG; ++j)
     for (int i = 0; i < NG; ++i) h->H[i][j] = s->hprime[j * NG + i];

@@ -469,7 +469,44 @@ def _inv_jac(J11, J12, J21, J22):
 # ---------------------------------------------------------------------------
 # materials
 # ---------------------------------------------------------------------------
-def _materials(par):
+def _read_hete1(par, group, fname_dir):
+    """read_hete1_dist (SRC/distribution_hete1.f90): header 'ncol nx nz x0 z0 dx dz', then nx*nz rows."""
+    fname = group.get("file", "")
+    col = int(group.get("col", 1))
+    path = fname if os.path.isabs(fname) else os.path.join(fname_dir or ".", fname)
+    with open(path) as fh:
+        head = fh.readline().replace("d", "e").replace("D", "e").split()
+        ncol, nx, nz = int(float(head[0])), int(float(head[1])), int(float(head[2]))
+        x0, z0, dx, dz = (float(v) for v in head[3:7])
+        vals = np.loadtxt(fh, dtype=np.float64, ndmin=2, converters=None)
+    if vals.shape[0] < nx * nz:
+        raise InputError("read_hete1_dist: file too short")
+    val = vals[:nx * nz, col - 1].reshape(nz, nx)  # [j][i]
+    return ("hete1", dict(nx=nx, nz=nz, x0=x0, z0=z0, dx=dx, dz=dz, val=val))
+
+
+def _gen_hete1(p, x, z):
+    """generate_hete1_dist (SRC/distribution_hete1.f90): bilinear interpolation on a regular grid."""
+    nx, nz, x0, z0, dx, dz, val = p["nx"], p["nz"], p["x0"], p["z0"], p["dx"], p["dz"], p["val"]
+    if (x.min() < x0 - TINY_XABS or x.max() > x0 + (nx - 1) * dx + TINY_XABS or
+            z.min() < z0 - TINY_XABS or z.max() > z0 + (nz - 1) * dz + TINY_XABS):
+        raise InputError("generate_hete1_dist: input grid is too small")
+    xi = (x - x0) / dx
+    i = np.floor(xi).astype(np.int64)
+    i = np.where(i >= nx - 1, nx - 2, i)
+    i = np.where(i < 0, 0, i)
+    xi = xi - i.astype(np.float64)
+    eta = (z - z0) / dz
+    j = np.floor(eta).astype(np.int64)
+    j = np.where(j >= nz - 1, nz - 2, j)
+    j = np.where(j < 0, 0, j)
+    eta = eta - j.astype(np.float64)
+    return ((((1.0 - xi) * (1.0 - eta)) * val[j, i] + (xi * (1.0 - eta)) * val[j, i + 1])
+            + (xi * eta) * val[j + 1, i + 1]) + ((1.0 - xi) * eta) * val[j + 1, i]
+
+
+def _materials(par, fname_dir=None):
+    """MAT_read / MAT_ELAST_read (SRC/mat_gen.f90:101-200, SRC/mat_elastic.f90:69-188)."""
     mats = {}
     for g in par.all("MATERIAL"):
         tag = int(g.get("tag", 0))
@@ -483,22 +520,50 @@ def _materials(par):
         me = par.after(g, "MAT_ELASTIC")
         if me is None:
             raise InputError("MAT_ELAST_read: MAT_ELASTIC block not found")
-        for key in ("rhoh", "cph", "csh", "c11", "c11h"):
+        for key in ("c11", "c11h", "c13", "c13h"):
             if me.get(key):
-                raise InputError(f"MAT_ELASTIC {key} is not supported by this build")
+                raise InputError(f"MAT_ELASTIC {key} (anisotropy) is not supported by this build")
         rho, cp, cs = float(me.get("rho", 0)), float(me.get("cp", 0)), float(me.get("cs", 0))
-        if rho <= 0:
+        rhoH, cpH, csH = me.get("rhoh", ""), me.get("cph", ""), me.get("csh", "")
+        if rho <= 0 and not rhoH:
             raise InputError("MAT_ELAST_read: undefined density (rho)")
-        mu = rho * cs * cs
-        kmod = rho * cp * cp
-        lam = rho * (cp * cp - 2.0 * cs * cs)
-        poiss = 0.5 * lam / (kmod - mu)
-        if poiss < 0.0 or poiss > 0.5:
-            raise InputError("Poisson's ratio out of range !")
-        mats[tag] = dict(rho=rho, cp=cp, cs=cs, lam=lam, mu=mu)
+        if not ((cp > 0 or cpH) and (cs > 0 or csH)):
+            raise InputError("MAT_ELAST_read: cp and cs must be set")
+        if not (rhoH or cpH or csH):
+            mu = rho * cs * cs
+            kmod = rho * cp * cp
+            lam = rho * (cp * cp - 2.0 * cs * cs)
+            poiss = 0.5 * lam / (kmod - mu)
+            if poiss < 0.0 or poiss > 0.5:
+                raise InputError("Poisson's ratio out of range !")
+            mats[tag] = dict(homog=True, rho=rho, cp=cp, cs=cs, lam=lam, mu=mu)
+        else:
+            # distributions follow MAT_ELASTIC in the order rho, cp, cs (SRC/mat_elastic.f90:105-113)
+            it = iter(par.next_groups(me))
+            spec = {}
+            for nme, val, hname in (("rho", rho, rhoH), ("cp", cp, cpH), ("cs", cs, csH)):
+                if not hname:
+                    spec[nme] = ("const", val)
+                    continue
+                want = "DIST_" + str(hname).upper()
+                gg = next(it, None)
+                while gg is not None and gg.name != want:
+                    gg = next(it, None)
+                if gg is None:
+                    raise InputError(f"{want} block not found")
+                if want != "DIST_HETE1":
+                    raise InputError(f"distribution {want} for materials is not supported by this build")
+                spec[nme] = _read_hete1(par, gg, fname_dir)
+            mats[tag] = dict(homog=False, spec=spec)
     if not mats:
         raise InputError("MAT_read: MATERIAL block not found")
     return mats
+
+
+def _eval_dist(d, x, z):
+    if d[0] == "const":
+        return np.full(x.shape, d[1])
+    return _gen_hete1(d[1], x, z)
 
 
 def _elastic_coefs(DxiDx, DetaDz, w2J, lam, mu, ndof):
@@ -510,6 +575,17 @@ def _elastic_coefs(DxiDx, DetaDz, w2J, lam, mu, ndof):
         a = np.stack([Kx * DxiDx * DxiDx, lam * DxiDx * DetaDz, Kx * DetaDz * DetaDz,
                       mu * DetaDz * DetaDz, mu * DxiDx * DetaDz, mu * DxiDx * DxiDx])
     return (-w2J)[None] * a
+
+
+def _elastic_coefs_batch(DxiDx, DetaDz, w2J, lam, mu, ndof):
+    """_elastic_coefs for a batch of elements: arrays (n, 5, 5) -> (n, nelast, 5, 5)."""
+    if ndof == 1:
+        a = np.stack([mu * DxiDx * DxiDx, mu * DetaDz * DetaDz], axis=1)
+    else:
+        Kx = lam + 2.0 * mu
+        a = np.stack([Kx * DxiDx * DxiDx, lam * DxiDx * DetaDz, Kx * DetaDz * DetaDz,
+                      mu * DetaDz * DetaDz, mu * DxiDx * DetaDz, mu * DxiDx * DxiDx], axis=1)
+    return (-w2J)[:, None] * a
 
 
 # ---------------------------------------------------------------------------
@@ -607,7 +683,7 @@ def _normal_and_weights(t, jac, wgll):
     return NORM, W
 
 
-def _abso_init(t, jac, wgll, mat_of_elem, M, coefA2Vrhs, ndof, stacey):
+def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
     """BC_ABSO_init (SRC/bc_abso.f90:115-266), Clayton-Engquist on flat boundaries."""
     if stacey:
         raise InputError("BC_ABSORB stacey=T is not supported by this build")
@@ -622,14 +698,14 @@ def _abso_init(t, jac, wgll, mat_of_elem, M, coefA2Vrhs, ndof, stacey):
     dim_t = 0 if t.edge in (EDGE_D, EDGE_U) else 1
     ii, jj = _edge_ij(t.edge)
     C = np.zeros((len(t.node), ndof))
+    rho_p, cp_p, cs_p = props
     for n, e in enumerate(t.elem):
-        m = mat_of_elem[e]
-        rho = m["rho"]
-        c = [0.0, 0.0]
-        c[geo_nor] = m["cp"]
-        c[geo_tan] = m["cs"]
         for k in range(5):
             i, j = ii[k], jj[k]
+            rho = rho_p[e, j, i]
+            c = [0.0, 0.0]
+            c[geo_nor] = cp_p[e, j, i]
+            c[geo_tan] = cs_p[e, j, i]
             if dim_t == 0:
                 a, b = J11[e, j, i], J21[e, j, i]
             else:
@@ -999,9 +1075,19 @@ def _receivers(par, coord, nt):
 
 
 # ---------------------------------------------------------------------------
-def build(par, nsteps=None):
-    """Build a :class:`Problem` from a parsed Par.inp (:func:`parfile.read_par`)."""
+def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=True):
+    """Build a :class:`Problem` from a parsed Par.inp (:func:`parfile.read_par`).
+
+    ``material``, if given, is a callable ``(x, z) -> (rho, cp, cs)`` evaluated at the
+    GLL nodes of every element; it replaces the MAT_ELASTIC blocks (one tag), as a
+    distribution would.  ``fname_dir`` is where relative data file names are read
+    (default: the directory of the input file).  ``per_element_coefs=False`` skips
+    the reference's per-element coefficient arrays for heterogeneous media (only the
+    compact lambda/mu form is then built).
+    """
     if isinstance(par, (str, os.PathLike)):
+        if fname_dir is None:
+            fname_dir = os.path.dirname(os.path.abspath(par))
         par = _pf.read_par(par)
     gen = par.first("GENERAL")
     if gen is None:
@@ -1046,12 +1132,42 @@ def build(par, nsteps=None):
         j2 = min(mesh["ezflt"] + mesh["fznz"], nz)
         rows = num["iz"] + 1
         tags[(rows >= j1) & (rows <= j2)] = mesh["fztag"]
-    mats = _materials(par)
-    for t in np.unique(tags):
-        if int(t) not in mats:
-            raise InputError("MAT_init_prop: material tag missing")
-    mat_of_elem = [mats[int(t)] for t in tags]
     pb.tags = tags
+    ibool0 = pb.ibool - 1
+    ex = pb.coord[ibool0, 0]
+    ez = pb.coord[ibool0, 1]
+    # material properties at the GLL nodes of each element (MAT_init_prop,
+    # SRC/mat_gen.f90:204-303; MAT_ELAST_init_elem_prop, SRC/mat_elastic.f90:194-234)
+    shp = (nelem, 5, 5)
+    rho_p, cp_p, cs_p, lam_p, mu_p = (np.empty(shp) for _ in range(5))
+    homog_tag = {}
+    if material is not None:
+        r, c1, c2 = material(ex, ez)
+        rho_p[:], cp_p[:], cs_p[:] = r, c1, c2
+        lam_p[:] = rho_p * (cp_p * cp_p - 2.0 * cs_p * cs_p)
+        mu_p[:] = rho_p * cs_p * cs_p
+        homog_tag = {int(t): False for t in np.unique(tags)}
+    else:
+        mats = _materials(par, fname_dir)
+        for t in np.unique(tags):
+            if int(t) not in mats:
+                raise InputError("MAT_init_prop: material tag missing")
+            m = mats[int(t)]
+            sel = tags == t
+            homog_tag[int(t)] = m["homog"]
+            if m["homog"]:
+                rho_p[sel], cp_p[sel], cs_p[sel] = m["rho"], m["cp"], m["cs"]
+                lam_p[sel], mu_p[sel] = m["lam"], m["mu"]
+            else:
+                xs, zs = ex[sel], ez[sel]
+                r = _eval_dist(m["spec"]["rho"], xs, zs)
+                c1 = _eval_dist(m["spec"]["cp"], xs, zs)
+                c2 = _eval_dist(m["spec"]["cs"], xs, zs)
+                rho_p[sel], cp_p[sel], cs_p[sel] = r, c1, c2
+                lam_p[sel] = r * (c1 * c1 - 2.0 * c2 * c2)
+                mu_p[sel] = r * c2 * c2
+    pb.rho, pb.cp, pb.cs, pb.lam, pb.mu = rho_p, cp_p, cs_p, lam_p, mu_p
+    pb.homogeneous = all(homog_tag.values())
 
     jac = _jacobians(num["cx"], num["cz"], xgll)
     J11, J12, J21, J22 = jac
@@ -1060,13 +1176,10 @@ def build(par, nsteps=None):
     dvol = (J11 * J22 - J12 * J21) * wgll2[None]
 
     # CHECK_grid + TIME_init
-    ibool0 = pb.ibool - 1
-    ex = pb.coord[ibool0, 0]
-    ez = pb.coord[ibool0, 1]
     d1 = np.sqrt((ex[:, :-1, 1:] - ex[:, :-1, :-1]) ** 2 + (ez[:, :-1, 1:] - ez[:, :-1, :-1]) ** 2)
     d2 = np.sqrt((ex[:, 1:, :-1] - ex[:, :-1, :-1]) ** 2 + (ez[:, 1:, :-1] - ez[:, :-1, :-1]) ** 2)
-    cvel = np.array([m["cp"] if ndof == 2 else m["cs"] for m in mat_of_elem])
-    grid_cfl = float((cvel[:, None, None] / np.minimum(d1, d2)).max())
+    cvel = cp_p if ndof == 2 else cs_p
+    grid_cfl = float((cvel[:, :-1, :-1] / np.minimum(d1, d2)).max())
     tg = par.first("TIME")
     kind = tg.get("kind", "leapfrog") if tg is not None else "leapfrog"
     if kind != "newmark":
@@ -1100,27 +1213,44 @@ def build(par, nsteps=None):
     pb.grid_cfl = grid_cfl
 
     # MAT_init_work: flat grid, elastic.  Homogeneous materials share the
-    # array of the first element of their tag (SRC/mat_gen.f90:357-365).
+    # array of the first element of their tag (SRC/mat_gen.f90:357-365); other
+    # elements get their own array (SRC/mat_elastic.f90:242-360).
     pb.nelast = 2 if ndof == 1 else 6
     utags = np.unique(tags)
-    if len(utags) == 1:
+    first_of = {int(t): int(np.nonzero(tags == t)[0][0]) for t in utags}
+    if len(utags) == 1 and pb.homogeneous:
         e1 = 0
-        m = mat_of_elem[e1]
-        pb.coefs = _elastic_coefs(DxiDx[e1], DetaDz[e1], dvol[e1], m["lam"], m["mu"], ndof)
+        pb.coefs = _elastic_coefs(DxiDx[e1], DetaDz[e1], dvol[e1], lam_p[e1], mu_p[e1], ndof)
         pb.coef_mode = "shared"
     else:
-        coefs = np.empty((nelem, pb.nelast, 5, 5))
-        first_of = {int(t): int(np.nonzero(tags == t)[0][0]) for t in utags}
-        for t in utags:
-            e1 = first_of[int(t)]
-            m = mats[int(t)]
-            coefs[tags == t] = _elastic_coefs(DxiDx[e1], DetaDz[e1], dvol[e1], m["lam"], m["mu"], ndof)
-        pb.coefs = coefs
         pb.coef_mode = "per_element"
+        if per_element_coefs or pb.homogeneous:
+            coefs = np.empty((nelem, pb.nelast, 5, 5))
+            for t in utags:
+                sel = tags == t
+                if homog_tag[int(t)]:
+                    e1 = first_of[int(t)]
+                    coefs[sel] = _elastic_coefs(DxiDx[e1], DetaDz[e1], dvol[e1], lam_p[e1], mu_p[e1], ndof)
+                else:
+                    idx = np.nonzero(sel)[0]
+                    for c0 in range(0, len(idx), 65536):
+                        b = idx[c0:c0 + 65536]
+                        coefs[b] = _elastic_coefs_batch(DxiDx[b], DetaDz[b], dvol[b], lam_p[b], mu_p[b], ndof)
+            pb.coefs = coefs
+        else:
+            pb.coefs = None
+    # compact form for heterogeneous flat grids: lambda, mu per GLL node and the three
+    # geometric factors of each element, -detJ*DxiDx^2, -detJ*DetaDz^2, -detJ*DxiDx*DetaDz
+    # (taken at the element's first GLL node); a(:,:,k) = wgll2 * material * factor.
+    if not (len(utags) == 1 and pb.homogeneous):
+        detJ = J11[:, 0, 0] * J22[:, 0, 0] - J12[:, 0, 0] * J21[:, 0, 0]
+        gx, gz = DxiDx[:, 0, 0], DetaDz[:, 0, 0]
+        pb.geom = np.stack([-detJ * gx * gx, -detJ * gz * gz, -detJ * gx * gz], axis=1)
+        pb.lam_mu = np.stack([lam_p, mu_p], axis=1) if ndof == 2 else mu_p[:, None]
+    pb.wgll2 = wgll2
 
     # MAT_MASS_init: sequential element-order assembly (np.add.at is sequential)
-    rho = np.array([m["rho"] for m in mat_of_elem])
-    massloc = rho[:, None, None] * dvol
+    massloc = rho_p * dvol
     M = np.zeros((ndof, npoin))
     np.add.at(M[0], ibool0.ravel(), massloc.ravel())
     if ndof == 2:
@@ -1144,7 +1274,7 @@ def build(par, nsteps=None):
             ab = par.after(bd, "BC_ABSORB")
             stacey = bool(ab.get("stacey", False)) if ab is not None else False
             t = _bound_topo(t1, mesh, num, perm_inv)
-            pb.abso.append(_abso_init(t, jac, wgll, mat_of_elem, M, coefA2Vrhs, ndof, stacey))
+            pb.abso.append(_abso_init(t, jac, wgll, (rho_p, cp_p, cs_p), M, coefA2Vrhs, ndof, stacey))
         elif kind == "DYNFLT":
             if pb.fault is not None:
                 raise InputError("more than one DYNFLT boundary is not supported by this build")

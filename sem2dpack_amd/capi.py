@@ -13,8 +13,9 @@ ip = C.POINTER(C.c_int)
 class Desc(C.Structure):
     _fields_ = [
         ("ndof", C.c_int), ("ngll", C.c_int), ("nelem", C.c_int), ("npoin", C.c_int),
-        ("nelast", C.c_int), ("coef_shared", C.c_int),
-        ("ibool", ip), ("coefs", dp), ("hprime", dp), ("rmass", dp),
+        ("nelast", C.c_int), ("coef_mode", C.c_int),
+        ("ibool", ip), ("coefs", dp), ("lam_mu", dp), ("geom", dp), ("wgll", dp),
+        ("hprime", dp), ("rmass", dp),
         ("dt", C.c_double), ("beta", C.c_double), ("gamma", C.c_double),
         ("bulk_fp32", C.c_int),
         ("nx", C.c_int), ("nz", C.c_int), ("ezflt", C.c_int), ("elem_of_cell", ip),

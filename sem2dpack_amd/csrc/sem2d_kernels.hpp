@@ -13,8 +13,12 @@ namespace sem2d {
 constexpr int NG = 5;
 constexpr int NP = NG * NG;
 
-// h'_i(x_j) stored at H[i*5+j] (set once per library load, it only depends on NGLL)
+// h'_i(x_j) stored at H[i*5+j], and wgll(i)*wgll(j) at W2[j*5+i] (they only depend on NGLL)
 __constant__ double c_H[NP];
+__constant__ double c_W2[NP];
+
+// Coefficient storage modes (sem2d_desc.coef_mode)
+enum { COEF_PER_ELEM = 0, COEF_SHARED = 1, COEF_COMPACT = 2 };
 
 template <typename R>
 struct Coef {
@@ -35,9 +39,39 @@ struct Coef {
 constexpr int FE_EPB = 10;
 constexpr int FE_THREADS = FE_EPB * NP;
 
-template <typename R, int NDOF, int NELAST, bool SHARED>
+// Element coefficients a_k at point p of element e, for the three storage modes.
+// Compact flat form: a(i,j,k) = W2(i,j) * modulus(i,j) * geom_e (SRC/mat_elastic.f90:327-358
+// with -weights = -W2*detJ folded into geom_e).
+template <typename R, int NDOF, int NELAST, int MODE>
+__device__ __forceinline__ void load_coefs(const R* __restrict__ coefs, const R* __restrict__ geom, int e, int p,
+                                           R* A) {
+  if constexpr (MODE == COEF_COMPACT) {
+    const R w = (R)c_W2[p];
+    const R g0 = geom[(size_t)e * 3 + 0], g1 = geom[(size_t)e * 3 + 1], g2 = geom[(size_t)e * 3 + 2];
+    if constexpr (NDOF == 2) {
+      const R la = coefs[(size_t)e * 2 * NP + p], mu = coefs[(size_t)e * 2 * NP + NP + p];
+      const R kx = la + 2 * mu;
+      A[0] = w * kx * g0;
+      A[1] = w * la * g2;
+      A[2] = w * kx * g1;
+      A[3] = w * mu * g1;
+      A[4] = w * mu * g2;
+      A[5] = w * mu * g0;
+    } else {
+      const R mu = coefs[(size_t)e * NP + p];
+      A[0] = w * mu * g0;
+      A[1] = w * mu * g1;
+    }
+  } else {
+    const R* a = coefs + (MODE == COEF_SHARED ? 0 : (size_t)e * NELAST * NP) + p;
+#pragma unroll
+    for (int k = 0; k < NELAST; ++k) A[k] = a[k * NP];
+  }
+}
+
+template <typename R, int NDOF, int NELAST, int MODE>
 __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int* __restrict__ ibool,
-                                                      const R* __restrict__ coefs,
+                                                      const R* __restrict__ coefs, const R* __restrict__ geom,
                                                       const R* __restrict__ d, R* __restrict__ f,
                                                       int npoin) {
   __shared__ R U[FE_EPB][NDOF][NP];
@@ -58,7 +92,8 @@ __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int*
   }
   __syncthreads();
   if (active) {
-    const R* a = coefs + (SHARED ? 0 : (size_t)e * NELAST * NP) + p;
+    R A[NELAST];
+    load_coefs<R, NDOF, NELAST, MODE>(coefs, geom, e, p, A);
     R dxi[NDOF], deta[NDOF];
 #pragma unroll
     for (int c = 0; c < NDOF; ++c) {
@@ -73,15 +108,12 @@ __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int*
     }
     if constexpr (NDOF == 2) {
       if constexpr (NELAST == 6) {
-        const R a1 = a[0], a2 = a[NP], a3 = a[2 * NP], a4 = a[3 * NP], a5 = a[4 * NP], a6 = a[5 * NP];
+        const R a1 = A[0], a2 = A[1], a3 = A[2], a4 = A[3], a5 = A[4], a6 = A[5];
         T1[le][0][p] = a1 * dxi[0] + a2 * deta[1];
         T2[le][0][p] = a4 * (deta[0] + dxi[1]);
         T1[le][1][p] = a5 * deta[0] + a6 * dxi[1];
         T2[le][1][p] = a2 * dxi[0] + a3 * deta[1];
       } else {
-        R A[10];
-#pragma unroll
-        for (int k = 0; k < 10; ++k) A[k] = a[k * NP];
         T1[le][0][p] = A[0] * dxi[0] + A[6] * deta[0] + A[7] * dxi[1] + A[1] * deta[1];
         T2[le][0][p] = A[6] * dxi[0] + A[3] * deta[0] + A[4] * dxi[1] + A[8] * deta[1];
         T1[le][1][p] = A[7] * dxi[0] + A[4] * deta[0] + A[5] * dxi[1] + A[9] * deta[1];
@@ -89,10 +121,10 @@ __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int*
       }
     } else {
       if constexpr (NELAST == 2) {
-        T1[le][0][p] = a[0] * dxi[0];
-        T2[le][0][p] = a[NP] * deta[0];
+        T1[le][0][p] = A[0] * dxi[0];
+        T2[le][0][p] = A[1] * deta[0];
       } else {
-        const R a1 = a[0], a2 = a[NP], a3 = a[2 * NP];
+        const R a1 = A[0], a2 = A[1], a3 = A[2];
         T1[le][0][p] = a1 * dxi[0] + a3 * deta[0];
         T2[le][0][p] = a3 * dxi[0] + a2 * deta[0];
       }

@@ -46,12 +46,12 @@ struct FaultHost {
 }  // namespace
 
 struct sem2d_solver {
-  int ndof, nelem, npoin, nelast, shared, fp32;
+  int ndof, nelem, npoin, nelast, coef_mode, fp32;
   double dt, beta, gamma;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // bulk arrays (type R = double or float, stored as raw bytes)
-  void *d = nullptr, *v = nullptr, *a = nullptr, *rmass = nullptr, *coefs = nullptr;
+  void *d = nullptr, *v = nullptr, *a = nullptr, *rmass = nullptr, *coefs = nullptr, *geom = nullptr;
   int* ibool = nullptr;  // colour-sorted, 0-based
   std::vector<int> colour_start;  // ncolour+1 offsets into the sorted order
   std::vector<int> elem_order;    // sorted position -> original element (0-based)
@@ -202,30 +202,35 @@ int build_colours(sem2d_solver* h, const sem2d_desc* s, std::vector<int>& colour
   return 0;
 }
 
+template <typename R, int NDOF, int NELAST, int MODE>
+void launch_force_t(sem2d_solver* h, const R* d, R* f) {
+  for (size_t c = 0; c + 1 < h->colour_start.size(); ++c) {
+    const int e0 = h->colour_start[c], ne = h->colour_start[c + 1] - e0;
+    if (ne <= 0) continue;
+    k_force_colour<R, NDOF, NELAST, MODE><<<(ne + FE_EPB - 1) / FE_EPB, 256, 0, h->stream>>>(
+        e0, ne, h->ibool, (const R*)h->coefs, (const R*)h->geom, d, f, h->npoin);
+  }
+}
+
+template <typename R, int NDOF, int NELAST>
+void launch_force_m(sem2d_solver* h, const R* d, R* f) {
+  switch (h->coef_mode) {
+    case COEF_SHARED: launch_force_t<R, NDOF, NELAST, COEF_SHARED>(h, d, f); break;
+    case COEF_COMPACT:
+      if constexpr (NELAST == 2 || NELAST == 6) launch_force_t<R, NDOF, NELAST, COEF_COMPACT>(h, d, f);
+      break;
+    default: launch_force_t<R, NDOF, NELAST, COEF_PER_ELEM>(h, d, f);
+  }
+}
+
 template <typename R>
 void launch_force(sem2d_solver* h, const R* d, R* f) {
-  for (size_t c = 0; c + 1 < h->colour_start.size(); ++c) {
-    int e0 = h->colour_start[c], ne = h->colour_start[c + 1] - e0;
-    if (ne <= 0) continue;
-    dim3 grid((ne + FE_EPB - 1) / FE_EPB), block(256);
-    const R* co = (const R*)h->coefs;
-    if (h->ndof == 2) {
-      if (h->nelast == 6) {
-        if (h->shared) k_force_colour<R, 2, 6, true><<<grid, block, 0, h->stream>>>(e0, ne, h->ibool, co, d, f, h->npoin);
-        else k_force_colour<R, 2, 6, false><<<grid, block, 0, h->stream>>>(e0, ne, h->ibool, co, d, f, h->npoin);
-      } else {
-        if (h->shared) k_force_colour<R, 2, 10, true><<<grid, block, 0, h->stream>>>(e0, ne, h->ibool, co, d, f, h->npoin);
-        else k_force_colour<R, 2, 10, false><<<grid, block, 0, h->stream>>>(e0, ne, h->ibool, co, d, f, h->npoin);
-      }
-    } else {
-      if (h->nelast == 2) {
-        if (h->shared) k_force_colour<R, 1, 2, true><<<grid, block, 0, h->stream>>>(e0, ne, h->ibool, co, d, f, h->npoin);
-        else k_force_colour<R, 1, 2, false><<<grid, block, 0, h->stream>>>(e0, ne, h->ibool, co, d, f, h->npoin);
-      } else {
-        if (h->shared) k_force_colour<R, 1, 3, true><<<grid, block, 0, h->stream>>>(e0, ne, h->ibool, co, d, f, h->npoin);
-        else k_force_colour<R, 1, 3, false><<<grid, block, 0, h->stream>>>(e0, ne, h->ibool, co, d, f, h->npoin);
-      }
-    }
+  if (h->ndof == 2) {
+    if (h->nelast == 6) launch_force_m<R, 2, 6>(h, d, f);
+    else launch_force_m<R, 2, 10>(h, d, f);
+  } else {
+    if (h->nelast == 2) launch_force_m<R, 1, 2>(h, d, f);
+    else launch_force_m<R, 1, 3>(h, d, f);
   }
 }
 
@@ -334,14 +339,18 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   if (!(s->ndof == 1 || s->ndof == 2)) return 2;
   if (s->ndof == 2 && s->nelast != 6 && s->nelast != 10) return 2;
   if (s->ndof == 1 && s->nelast != 2 && s->nelast != 3) return 2;
-  if (s->nelem <= 0 || s->npoin <= 0 || !s->ibool || !s->coefs || !s->hprime || !s->rmass) return 2;
+  if (s->nelem <= 0 || s->npoin <= 0 || !s->ibool || !s->hprime || !s->rmass) return 2;
+  if (s->coef_mode < 0 || s->coef_mode > 2) return 2;
+  if (s->coef_mode == COEF_COMPACT && (!s->lam_mu || !s->geom || !s->wgll || (s->nelast != 2 && s->nelast != 6)))
+    return 2;
+  if (s->coef_mode != COEF_COMPACT && !s->coefs) return 2;
   sem2d_solver* h = new sem2d_solver();
   *out = h;
   h->ndof = s->ndof;
   h->nelem = s->nelem;
   h->npoin = s->npoin;
   h->nelast = s->nelast;
-  h->shared = s->coef_shared ? 1 : 0;
+  h->coef_mode = s->coef_mode;
   h->fp32 = s->bulk_fp32 ? 1 : 0;
   h->dt = s->dt;
   h->beta = s->beta;
@@ -355,6 +364,12 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   for (int i = 0; i < NG; ++i)
     for (int j = 0; j < NG; ++j) Hc[i * NG + j] = s->hprime[j * NG + i];
   HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_H), Hc, sizeof Hc, 0, hipMemcpyHostToDevice));
+  if (s->coef_mode == COEF_COMPACT) {
+    double W2[NP];
+    for (int j = 0; j < NG; ++j)
+      for (int i = 0; i < NG; ++i) W2[j * NG + i] = s->wgll[i] * s->wgll[j];
+    HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_W2), W2, sizeof W2, 0, hipMemcpyHostToDevice));
+  }
 
   std::vector<int> colour;
   int ncol = 0;
@@ -372,14 +387,22 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
     for (int p = 0; p < NP; ++p) ib[(size_t)k * NP + p] = s->ibool[(size_t)e * NP + p] - 1;
   }
   if ((rc = upload(h, &h->ibool, ib.data(), ib.size()))) return rc;
-  const size_t ncoef = (size_t)h->nelast * NP;
-  if (h->shared) {
-    if ((rc = upload_bulk(h, &h->coefs, s->coefs, ncoef))) return rc;
+  if (h->coef_mode == COEF_SHARED) {
+    if ((rc = upload_bulk(h, &h->coefs, s->coefs, (size_t)h->nelast * NP))) return rc;
   } else {
-    std::vector<double> co((size_t)s->nelem * ncoef);
+    // per-element arrays follow the colour-sorted element order
+    const size_t per = h->coef_mode == COEF_COMPACT ? (size_t)h->ndof * NP : (size_t)h->nelast * NP;
+    const double* src = h->coef_mode == COEF_COMPACT ? s->lam_mu : s->coefs;
+    std::vector<double> co((size_t)s->nelem * per);
     for (int k = 0; k < s->nelem; ++k)
-      std::memcpy(&co[(size_t)k * ncoef], s->coefs + (size_t)h->elem_order[k] * ncoef, ncoef * sizeof(double));
+      std::memcpy(&co[(size_t)k * per], src + (size_t)h->elem_order[k] * per, per * sizeof(double));
     if ((rc = upload_bulk(h, &h->coefs, co.data(), co.size()))) return rc;
+    if (h->coef_mode == COEF_COMPACT) {
+      std::vector<double> ge((size_t)s->nelem * 3);
+      for (int k = 0; k < s->nelem; ++k)
+        for (int q = 0; q < 3; ++q) ge[(size_t)k * 3 + q] = s->geom[(size_t)h->elem_order[k] * 3 + q];
+      if ((rc = upload_bulk(h, &h->geom, ge.data(), ge.size()))) return rc;
+    }
   }
   const size_t nf = (size_t)h->ndof * h->npoin;
   if ((rc = upload_bulk(h, &h->rmass, s->rmass, nf))) return rc;

@@ -25,20 +25,36 @@ def _p(a, t=np.float64):
 class Solver:
     FIELD_CODE = {"D": 0, "V": 1, "A": 2}
 
-    def __init__(self, pb, bulk_fp32=False, api=None, structured=True):
+    def __init__(self, pb, bulk_fp32=False, api=None, structured=True, compact=True):
         self.api = api if api is not None else load_hip_lib()
         self.pb = pb
         self.ndof, self.npoin = pb.ndof, pb.npoin
         self._keep = []
         ibool = np.ascontiguousarray(pb.ibool, dtype=np.int32)
-        coefs = np.ascontiguousarray(pb.coefs, dtype=np.float64)
         hprime = np.ascontiguousarray(pb.hprime.T, dtype=np.float64)  # [j][i]
         rmass = np.ascontiguousarray(pb.rmass, dtype=np.float64)
         d = capi.Desc()
         d.ndof, d.ngll, d.nelem, d.npoin = pb.ndof, 5, pb.nelem, pb.npoin
         d.nelast = pb.nelast
-        d.coef_shared = 1 if pb.coef_mode == "shared" else 0
-        d.ibool, d.coefs, d.hprime, d.rmass = _p(ibool, np.int32), _p(coefs), _p(hprime), _p(rmass)
+        keep = []
+        if pb.coef_mode == "shared":
+            d.coef_mode = 1
+        elif (compact or pb.coefs is None) and getattr(pb, "lam_mu", None) is not None:
+            d.coef_mode = 2
+        else:
+            d.coef_mode = 0
+        if d.coef_mode == 2:
+            lm = np.ascontiguousarray(pb.lam_mu, dtype=np.float64)
+            ge = np.ascontiguousarray(pb.geom, dtype=np.float64)
+            wg = np.ascontiguousarray(pb.wgll, dtype=np.float64)
+            keep += [lm, ge, wg]
+            d.lam_mu, d.geom, d.wgll = _p(lm), _p(ge), _p(wg)
+        else:
+            coefs = np.ascontiguousarray(pb.coefs, dtype=np.float64)
+            keep.append(coefs)
+            d.coefs = _p(coefs)
+        self.coef_mode = d.coef_mode
+        d.ibool, d.hprime, d.rmass = _p(ibool, np.int32), _p(hprime), _p(rmass)
         d.dt, d.beta, d.gamma = pb.dt, pb.beta, pb.gamma
         d.bulk_fp32 = 1 if bulk_fp32 else 0
         cells = None
