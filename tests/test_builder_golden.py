@@ -1,0 +1,56 @@
+"""Host-side problem set-up vs the reference (golden fixtures written by the reference solver).
+
+Numbering, coordinates and inverse mass are compared bit for bit (SHA-256 of the
+arrays the reference dumped); receivers and fault pairing node for node.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, case_par
+from sem2dpack_amd import builder
+
+CASES = ["small_psv", "c1_psv", "sh_swf", "psv_rsf", "hete_psv"]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module", params=CASES)
+def case(request):
+    g = np.load(os.path.join(GOLDEN, request.param + ".npz"))
+    pb = builder.build(case_par(request.param))
+    return request.param, g, pb
+
+
+def test_numbering_bit_exact(case):
+    name, g, pb = case
+    assert pb.npoin == int(g["npoin"]) and pb.nelem == int(g["nelem"])
+    if "ibool" in g:
+        assert np.array_equal(pb.ibool, g["ibool"])
+    assert sha(pb.ibool.astype(np.int32)) == str(g["ibool_sha256"])
+
+
+def test_coord_mass_dt_bit_exact(case):
+    name, g, pb = case
+    assert sha(pb.coord) == str(g["coord_sha256"])  # reference coord(2,npoin) memory order
+    assert sha(pb.rmass.copy()) == str(g["rmass_sha256"])
+    assert pb.dt == float(g["dt"])
+
+
+def test_receivers_and_fault_pairing(case):
+    name, g, pb = case
+    assert np.array_equal(pb.receivers["iglob"], g["rec_nodes"])
+    if "fault_node1" in g:
+        assert np.array_equal(pb.fault["node1"], g["fault_node1"])
+        assert np.array_equal(pb.fault["node2"], g["fault_node2"])
+
+
+def test_mass_is_rho_times_area():
+    """Sum of the assembled mass (before the absorbing-boundary term) = rho * area."""
+    pb = builder.build(case_par("small_psv"))
+    area = 5e3 * 4e3
+    assert abs(pb.mass_raw[0].sum() - 2500.0 * area) / (2500.0 * area) < 1e-13
