@@ -13,12 +13,15 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
 #include "sem2d.h"
 #include "sem2d_fault.hpp"
 #include "sem2d_kernels.hpp"
+#include "sem2d_tile.hpp"
 
 using namespace sem2d;
 
@@ -33,6 +36,37 @@ struct AbsoDev {
   int n;
   int* node;
   double* C;
+};
+
+struct AbsoHost {
+  std::vector<int> node;   // 0-based reference nodes
+  std::vector<double> C;   // [ndof][n]
+};
+
+// Structured-mesh fast path (node-grid field order, fused tile kernel)
+struct StructState {
+  bool on = false, ready = false;
+  int nx = 0, nz = 0, ezflt = 0, NX1 = 0, NZ1 = 0;
+  std::vector<int> gmap;          // reference node (0-based) -> grid index
+  std::vector<int4> tiles;
+  int ntiles = 0, per_xcd = 0;
+  int4* tiles_dev = nullptr;
+  int* spec_cnt = nullptr;
+  int* spec_idx = nullptr;
+  void* fpart = nullptr;
+  size_t FPS = 0;
+  DefNode* defs = nullptr;
+  int ndef = 0;
+  int* part_off = nullptr;
+  SrcEntry* dsrc = nullptr;
+  double* dabso = nullptr;
+  void* rmdef = nullptr;
+  void* rmass0 = nullptr;
+  void *coefs = nullptr, *geom = nullptr;  // cell-ordered per-element arrays
+  int* fault_g = nullptr;
+  void* fault_rm = nullptr;
+  int nfnodes = 0;
+  std::vector<void*> prep_allocs;  // freed when the deferred tables are rebuilt
 };
 
 struct FaultHost {
@@ -57,6 +91,10 @@ struct sem2d_solver {
   std::vector<int> elem_order;    // sorted position -> original element (0-based)
   // boundaries and sources
   std::vector<AbsoDev> abso;
+  std::vector<AbsoHost> abso_host;
+  std::vector<double> rmass_host;          // [ndof][npoin], reference order
+  std::vector<int> fault_n1, fault_n2;     // 0-based reference nodes
+  StructState st;
   std::vector<SrcEntry> src_host;
   std::vector<std::vector<double>> stf;
   int nsrc = 0;
@@ -286,6 +324,263 @@ int record_fault(sem2d_solver* h) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// structured path set-up
+// ---------------------------------------------------------------------------
+// Grid map from ibool and the cell -> element table; returns false if ibool does not
+// number a MESH_CART box the way the node grid does (the general path is used then).
+bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
+  StructState& S = h->st;
+  const int nx = s->nx, nz = s->nz, ez = s->ezflt;
+  if (nx <= 0 || nz <= 0 || !s->elem_of_cell || (long)nx * nz != s->nelem) return false;
+  if (nx > 0x7fff || ez < 0 || ez >= nz) return false;
+  S.nx = nx; S.nz = nz; S.ezflt = ez;
+  S.NX1 = 4 * nx + 1;
+  S.NZ1 = 4 * nz + 1 + (ez > 0 ? 1 : 0);
+  if ((long)S.NX1 * S.NZ1 != s->npoin) return false;
+  S.gmap.assign(s->npoin, -1);
+  std::vector<char> used((size_t)s->npoin, 0);
+  for (int iz = 0; iz < nz; ++iz) {
+    const int sh = (ez > 0 && iz >= ez) ? 1 : 0;
+    for (int ix = 0; ix < nx; ++ix) {
+      const int e = s->elem_of_cell[(size_t)iz * nx + ix] - 1;
+      if (e < 0 || e >= s->nelem) return false;
+      for (int j = 0; j < NG; ++j)
+        for (int i = 0; i < NG; ++i) {
+          const int ref = s->ibool[(size_t)e * NP + j * NG + i] - 1;
+          const int g = (4 * iz + j + sh) * S.NX1 + 4 * ix + i;
+          if (S.gmap[ref] == -1) {
+            if (used[g]) return false;
+            S.gmap[ref] = g;
+            used[g] = 1;
+          } else if (S.gmap[ref] != g) {
+            return false;
+          }
+        }
+    }
+  }
+  for (int k = 0; k < s->npoin; ++k)
+    if (S.gmap[k] < 0) return false;
+  // tiles: element-row bands split at the fault, TX x TZ tiles in each band
+  S.tiles.clear();
+  const int nb = ez > 0 ? 2 : 1;
+  for (int b = 0; b < nb; ++b) {
+    const int r0 = (b == 0) ? 0 : ez, r1 = (nb == 1) ? nz : (b == 0 ? ez : nz);
+    const int sh = b;
+    for (int tz0 = r0; tz0 < r1; tz0 += TZ)
+      for (int tx0 = 0; tx0 < nx; tx0 += TX) {
+        const int nxe = std::min(TX, nx - tx0), nze = std::min(TZ, r1 - tz0);
+        S.tiles.push_back(make_int4(tx0, tz0, nxe | (nze << 8), 4 * tz0 + sh));
+      }
+  }
+  S.ntiles = (int)S.tiles.size();
+  S.per_xcd = (S.ntiles + 7) / 8;
+  return true;
+}
+
+template <typename T>
+int prep_upload(sem2d_solver* h, T** p, const T* src, size_t n) {
+  void* q = nullptr;
+  HIPCHK(h, hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+  h->st.prep_allocs.push_back(q);
+  *p = (T*)q;
+  if (n) HIPCHK(h, hipMemcpyAsync(q, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  return 0;
+}
+
+int prep_upload_bulk(sem2d_solver* h, void** p, const std::vector<double>& src) {
+  if (!h->fp32) return prep_upload(h, (double**)p, src.data(), src.size());
+  std::vector<float> tmp(src.begin(), src.end());
+  int rc = prep_upload(h, (float**)p, tmp.data(), tmp.size());
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// Deferred-node tables: perimeter and special nodes of every tile, their partial-force
+// slots, sources, absorbing coefficients and inverse masses.
+int prepare_structured(sem2d_solver* h) {
+  StructState& S = h->st;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (void* q : S.prep_allocs) hipFree(q);
+  S.prep_allocs.clear();
+  const int nd = h->ndof;
+  // special nodes: point-source nodes inside a tile patch
+  std::vector<int> src_g;
+  for (const SrcEntry& e : h->src_host) src_g.push_back(S.gmap[e.node]);
+  std::sort(src_g.begin(), src_g.end());
+  src_g.erase(std::unique(src_g.begin(), src_g.end()), src_g.end());
+  std::vector<int> spec_cnt(S.ntiles, 0), spec_idx((size_t)S.ntiles * MAXSPECIAL, 0);
+  // entries (g, tile, slot)
+  struct Ent { int g, t, slot; };
+  std::vector<Ent> ents;
+  ents.reserve((size_t)S.ntiles * PERIM_MAX);
+  for (int t = 0; t < S.ntiles; ++t) {
+    const int4 tg = S.tiles[t];
+    const int nxe = tg.z & 0xff, nze = tg.z >> 8;
+    const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, X0 = 4 * tg.x, Z0 = tg.w;
+    auto gi = [&](int X, int Z) { return (Z0 + Z) * S.NX1 + X0 + X; };
+    for (int X = 0; X < PX; ++X) ents.push_back({gi(X, 0), t, X});
+    for (int X = 0; X < PX; ++X) ents.push_back({gi(X, PZ - 1), t, PX + X});
+    for (int Z = 1; Z < PZ - 1; ++Z) ents.push_back({gi(0, Z), t, 2 * PX + Z - 1});
+    for (int Z = 1; Z < PZ - 1; ++Z) ents.push_back({gi(PX - 1, Z), t, 2 * PX + (PZ - 2) + Z - 1});
+    for (int g : src_g) {
+      const int X = g % S.NX1 - X0, Z = g / S.NX1 - Z0;
+      if (X > 0 && X < PX - 1 && Z > 0 && Z < PZ - 1) {
+        if (spec_cnt[t] >= MAXSPECIAL) return set_err(h, 5, "too many source nodes in one tile");
+        spec_idx[(size_t)t * MAXSPECIAL + spec_cnt[t]] = Z * PX + X;
+        ents.push_back({g, t, PERIM_MAX + spec_cnt[t]});
+        spec_cnt[t]++;
+      }
+    }
+  }
+  std::sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) {
+    return a.g != b.g ? a.g < b.g : a.t < b.t;
+  });
+  // inverse map grid -> reference node, for sources / ABC / fault lookups
+  std::vector<int> ginv((size_t)h->npoin);
+  for (int k = 0; k < h->npoin; ++k) ginv[S.gmap[k]] = k;
+  std::unordered_map<int, std::vector<int>> src_of;  // ref node -> src_host indices
+  for (size_t q = 0; q < h->src_host.size(); ++q) src_of[h->src_host[q].node].push_back((int)q);
+  std::unordered_map<int, std::vector<std::pair<int, int>>> abso_of;  // ref node -> (bc, k)
+  for (size_t b = 0; b < h->abso_host.size(); ++b)
+    for (size_t k = 0; k < h->abso_host[b].node.size(); ++k)
+      abso_of[h->abso_host[b].node[k]].push_back({(int)b, (int)k});
+  std::vector<char> is_fault((size_t)h->npoin, 0);
+  for (int k : h->fault_n1) is_fault[k] = 1;
+  for (int k : h->fault_n2) is_fault[k] = 1;
+
+  std::vector<DefNode> defs;
+  std::vector<int> part_off;
+  std::vector<SrcEntry> dsrc;
+  std::vector<double> dabso;
+  std::vector<int> def_ref;
+  for (size_t i = 0; i < ents.size();) {
+    size_t j = i;
+    DefNode n{};
+    n.g = ents[i].g;
+    n.pstart = (int)part_off.size();
+    while (j < ents.size() && ents[j].g == n.g) {
+      part_off.push_back(ents[j].t * SLOTS + ents[j].slot);
+      ++j;
+    }
+    n.pcnt = (int)part_off.size() - n.pstart;
+    const int ref = ginv[n.g];
+    n.sstart = (int)dsrc.size();
+    auto its = src_of.find(ref);
+    if (its != src_of.end())
+      for (int q : its->second) dsrc.push_back(h->src_host[q]);
+    n.scnt = (int)dsrc.size() - n.sstart;
+    n.astart = (int)(dabso.size() / nd);
+    auto ita = abso_of.find(ref);
+    if (ita != abso_of.end())
+      for (auto& bk : ita->second) {
+        const AbsoHost& B = h->abso_host[bk.first];
+        const size_t nb = B.node.size();
+        for (int c = 0; c < nd; ++c) dabso.push_back(B.C[(size_t)c * nb + bk.second]);
+      }
+    n.acnt = (int)(dabso.size() / nd) - n.astart;
+    n.fault = is_fault[ref];
+    defs.push_back(n);
+    def_ref.push_back(ref);
+    i = j;
+  }
+  S.ndef = (int)defs.size();
+  std::vector<double> rmdef((size_t)nd * S.ndef);
+  for (int q = 0; q < S.ndef; ++q)
+    for (int c = 0; c < nd; ++c) rmdef[(size_t)c * S.ndef + q] = h->rmass_host[(size_t)c * h->npoin + def_ref[q]];
+  int rc;
+  if ((rc = prep_upload(h, &S.tiles_dev, S.tiles.data(), S.tiles.size()))) return rc;
+  if ((rc = prep_upload(h, &S.spec_cnt, spec_cnt.data(), spec_cnt.size()))) return rc;
+  if ((rc = prep_upload(h, &S.spec_idx, spec_idx.data(), spec_idx.size()))) return rc;
+  if ((rc = prep_upload(h, &S.defs, defs.data(), defs.size()))) return rc;
+  if ((rc = prep_upload(h, &S.part_off, part_off.data(), part_off.size()))) return rc;
+  if ((rc = prep_upload(h, &S.dsrc, dsrc.data(), dsrc.size()))) return rc;
+  if ((rc = prep_upload(h, &S.dabso, dabso.data(), dabso.size()))) return rc;
+  if ((rc = prep_upload_bulk(h, &S.rmdef, rmdef))) return rc;
+  S.FPS = (size_t)S.ntiles * SLOTS;
+  void* fp = nullptr;
+  HIPCHK(h, hipMalloc(&fp, std::max<size_t>(S.FPS * nd, 1) * rsize(h)));
+  S.prep_allocs.push_back(fp);
+  S.fpart = fp;
+  // split-fault nodes for the post-fault corrector
+  std::vector<int> fg;
+  std::vector<double> frm;
+  for (int k : h->fault_n1) fg.push_back(S.gmap[k]);
+  for (int k : h->fault_n2) fg.push_back(S.gmap[k]);
+  S.nfnodes = (int)fg.size();
+  frm.resize((size_t)nd * S.nfnodes);
+  for (int q = 0; q < S.nfnodes; ++q) {
+    const int ref = q < (int)h->fault_n1.size() ? h->fault_n1[q] : h->fault_n2[q - h->fault_n1.size()];
+    for (int c = 0; c < nd; ++c) frm[(size_t)c * S.nfnodes + q] = h->rmass_host[(size_t)c * h->npoin + ref];
+  }
+  if ((rc = prep_upload(h, &S.fault_g, fg.data(), fg.size()))) return rc;
+  if ((rc = prep_upload_bulk(h, &S.fault_rm, frm))) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  S.ready = true;
+  return 0;
+}
+
+template <typename R, int NDOF, int NELAST, int MODE>
+void launch_tile_t(sem2d_solver* h, R dt, R c1, R c2, R c3, R c4) {
+  StructState& S = h->st;
+  TileDev T;
+  T.NX1 = S.NX1;
+  T.nx = S.nx;
+  T.ntiles = S.ntiles;
+  T.per_xcd = S.per_xcd;
+  T.tiles = S.tiles_dev;
+  T.spec_cnt = S.spec_cnt;
+  T.spec_idx = S.spec_idx;
+  T.NGp = (size_t)h->npoin;
+  T.FPS = S.FPS;
+  k_tile<R, NDOF, NELAST, MODE><<<8 * S.per_xcd, TILE_THREADS, 0, h->stream>>>(
+      T, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom, (R*)S.fpart, dt,
+      c1, c2, c3, c4);
+}
+
+template <typename R, int NDOF, int NELAST>
+void launch_tile_m(sem2d_solver* h, R dt, R c1, R c2, R c3, R c4) {
+  switch (h->coef_mode) {
+    case COEF_SHARED: launch_tile_t<R, NDOF, NELAST, COEF_SHARED>(h, dt, c1, c2, c3, c4); break;
+    case COEF_COMPACT:
+      if constexpr (NELAST == 2 || NELAST == 6) launch_tile_t<R, NDOF, NELAST, COEF_COMPACT>(h, dt, c1, c2, c3, c4);
+      break;
+    default: launch_tile_t<R, NDOF, NELAST, COEF_PER_ELEM>(h, dt, c1, c2, c3, c4);
+  }
+}
+
+template <typename R>
+void launch_structured_step(sem2d_solver* h, int it, R dt, R c1, R c2, R c3, R c4) {
+  StructState& S = h->st;
+  if (h->ndof == 2) {
+    if (h->nelast == 6) launch_tile_m<R, 2, 6>(h, dt, c1, c2, c3, c4);
+    else launch_tile_m<R, 2, 10>(h, dt, c1, c2, c3, c4);
+  } else {
+    if (h->nelast == 2) launch_tile_m<R, 1, 2>(h, dt, c1, c2, c3, c4);
+    else launch_tile_m<R, 1, 3>(h, dt, c1, c2, c3, c4);
+  }
+  const int gd = (S.ndef + 255) / 256;
+  if (h->ndof == 2)
+    k_deferred<R, 2><<<gd, 256, 0, h->stream>>>(S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dsrc,
+                                                 h->ampl_dev, h->ampl_nt, it, S.dabso, (const R*)S.rmdef, (R*)h->d,
+                                                 (R*)h->v, (R*)h->a, (size_t)h->npoin, dt, c1, c2, c3, c4);
+  else
+    k_deferred<R, 1><<<gd, 256, 0, h->stream>>>(S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dsrc,
+                                                 h->ampl_dev, h->ampl_nt, it, S.dabso, (const R*)S.rmdef, (R*)h->d,
+                                                 (R*)h->v, (R*)h->a, (size_t)h->npoin, dt, c1, c2, c3, c4);
+  if (h->flt.on) {
+    k_fault<R><<<grid_for(h->flt.dev.np, 64), 64, 0, h->stream>>>(h->flt.dev, (const R*)h->d, (const R*)h->v,
+                                                                   (R*)h->a, h->npoin);
+    if (h->ndof == 2)
+      k_fault_correct<R, 2><<<grid_for(S.nfnodes, 64), 64, 0, h->stream>>>(
+          S.nfnodes, S.fault_g, (const R*)S.fault_rm, (R*)h->d, (R*)h->v, (R*)h->a, (size_t)h->npoin, c3, c4);
+    else
+      k_fault_correct<R, 1><<<grid_for(S.nfnodes, 64), 64, 0, h->stream>>>(
+          S.nfnodes, S.fault_g, (const R*)S.fault_rm, (R*)h->d, (R*)h->v, (R*)h->a, (size_t)h->npoin, c3, c4);
+  }
+}
+
 template <typename R>
 int do_steps(sem2d_solver* h, int nsteps) {
   int rc = sync_sources(h);
@@ -298,9 +593,17 @@ int do_steps(sem2d_solver* h, int nsteps) {
   const R c1 = (R)((0.5 - h->beta) * h->dt * h->dt), c2 = (R)((1.0 - h->gamma) * h->dt);
   const R c3 = (R)(h->gamma * h->dt), c4 = (R)(h->beta * h->dt * h->dt);
   const int gb = grid_for(n, 256);
+  if (h->st.on && !h->st.ready && (rc = prepare_structured(h))) return rc;
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   for (int s = 0; s < nsteps; ++s) {
     const int it = h->it + 1;
+    if (h->st.on) {
+      launch_structured_step<R>(h, it, dt, c1, c2, c3, c4);
+      h->it = it;
+      record_receivers<R>(h);
+      record_fault(h);
+      continue;
+    }
     k_predict<R><<<gb, 256, 0, h->stream>>>(n, d, v, a, dt, c1, c2);
     launch_force<R>(h, d, a);
     if (!h->src_host.empty())
@@ -406,6 +709,31 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   }
   const size_t nf = (size_t)h->ndof * h->npoin;
   if ((rc = upload_bulk(h, &h->rmass, s->rmass, nf))) return rc;
+  h->rmass_host.assign(s->rmass, s->rmass + nf);
+  if (structured_map(h, s)) {
+    StructState& S = h->st;
+    S.on = true;
+    std::vector<double> rm0((size_t)h->npoin);
+    for (int k = 0; k < h->npoin; ++k) rm0[S.gmap[k]] = s->rmass[k];
+    if ((rc = upload_bulk(h, &S.rmass0, rm0.data(), rm0.size()))) return rc;
+    if (h->coef_mode == COEF_SHARED) {
+      S.coefs = h->coefs;
+    } else {
+      // per-element arrays in cell order (iz*nx + ix)
+      const size_t per = h->coef_mode == COEF_COMPACT ? (size_t)h->ndof * NP : (size_t)h->nelast * NP;
+      const double* src = h->coef_mode == COEF_COMPACT ? s->lam_mu : s->coefs;
+      std::vector<double> co((size_t)s->nelem * per);
+      for (int cell = 0; cell < s->nelem; ++cell)
+        std::memcpy(&co[(size_t)cell * per], src + (size_t)(s->elem_of_cell[cell] - 1) * per, per * sizeof(double));
+      if ((rc = upload_bulk(h, &S.coefs, co.data(), co.size()))) return rc;
+      if (h->coef_mode == COEF_COMPACT) {
+        std::vector<double> ge((size_t)s->nelem * 3);
+        for (int cell = 0; cell < s->nelem; ++cell)
+          for (int q = 0; q < 3; ++q) ge[(size_t)cell * 3 + q] = s->geom[(size_t)(s->elem_of_cell[cell] - 1) * 3 + q];
+        if ((rc = upload_bulk(h, &S.geom, ge.data(), ge.size()))) return rc;
+      }
+    }
+  }
   if ((rc = dev_alloc(h, (char**)&h->d, nf * rsize(h)))) return rc;
   if ((rc = dev_alloc(h, (char**)&h->v, nf * rsize(h)))) return rc;
   if ((rc = dev_alloc(h, (char**)&h->a, nf * rsize(h)))) return rc;
@@ -432,6 +760,11 @@ int sem2d_set_abso(sem2d_solver* h, int n, const int* node, const double* C) {
   if (rc) return rc;
   if ((rc = upload(h, &b.C, C, (size_t)h->ndof * n))) return rc;
   h->abso.push_back(b);
+  AbsoHost ah;
+  ah.node = nd;
+  ah.C.assign(C, C + (size_t)h->ndof * n);
+  h->abso_host.push_back(ah);
+  h->st.ready = false;
   return 0;
 }
 
@@ -448,6 +781,7 @@ int sem2d_set_force(sem2d_solver* h, int iglob, const double* dir, int* isrc) {
   }
   h->stf.emplace_back();
   h->src_dirty = true;
+  h->st.ready = false;
   if (isrc) *isrc = s;
   return 0;
 }
@@ -474,6 +808,7 @@ int sem2d_set_moment(sem2d_solver* h, int nel, const int* ixi, const int* ieta, 
   }
   h->stf.emplace_back();
   h->src_dirty = true;
+  h->st.ready = false;
   if (isrc) *isrc = s;
   return 0;
 }
@@ -495,6 +830,7 @@ int sem2d_set_receivers(sem2d_solver* h, int nrec, const int* iglob, int isamp, 
   for (int r = 0; r < nrec; ++r) {
     if (iglob[r] < 1 || iglob[r] > h->npoin) return set_err(h, 3, "set_receivers: node out of range");
     nd[r] = iglob[r] - 1;
+    if (h->st.on) nd[r] = h->st.gmap[nd[r]];
   }
   int rc = upload(h, &h->rec_node, nd.data(), nd.size());
   if (rc) return rc;
@@ -531,6 +867,14 @@ int sem2d_set_fault(sem2d_solver* h, const sem2d_fault_desc* f, int nsamp) {
     n1[k] = f->node1[k] - 1;
     n2[k] = f->node2[k] - 1;
   }
+  h->fault_n1 = n1;
+  h->fault_n2 = n2;
+  h->st.ready = false;
+  if (h->st.on)
+    for (int k = 0; k < np; ++k) {
+      n1[k] = h->st.gmap[n1[k]];
+      n2[k] = h->st.gmap[n2[k]];
+    }
   int rc;
   int* ip;
   double* dp;
@@ -636,23 +980,44 @@ int sem2d_step(sem2d_solver* h, int nsteps) {
   return h->fp32 ? do_steps<float>(h, nsteps) : do_steps<double>(h, nsteps);
 }
 
+// device field (grid order when structured) -> host array in reference order
+static int fetch_field(sem2d_solver* h, double* out, const void* dev) {
+  const size_t n = (size_t)h->ndof * h->npoin;
+  if (!h->st.on) return read_bulk(h, out, dev, n);
+  std::vector<double> tmp(n);
+  int rc = read_bulk(h, tmp.data(), dev, n);
+  if (rc) return rc;
+  for (int c = 0; c < h->ndof; ++c)
+    for (int k = 0; k < h->npoin; ++k)
+      out[(size_t)c * h->npoin + k] = tmp[(size_t)c * h->npoin + h->st.gmap[k]];
+  return 0;
+}
+
+static int store_field(sem2d_solver* h, void* dev, const double* in) {
+  const size_t n = (size_t)h->ndof * h->npoin;
+  if (!h->st.on) return write_bulk(h, dev, in, n);
+  std::vector<double> tmp(n);
+  for (int c = 0; c < h->ndof; ++c)
+    for (int k = 0; k < h->npoin; ++k)
+      tmp[(size_t)c * h->npoin + h->st.gmap[k]] = in[(size_t)c * h->npoin + k];
+  return write_bulk(h, dev, tmp.data(), n);
+}
+
 int sem2d_fetch_fields(sem2d_solver* h, double* d, double* v, double* a) {
   if (!h) return 1;
-  const size_t n = (size_t)h->ndof * h->npoin;
   int rc = 0;
-  if (d && (rc = read_bulk(h, d, h->d, n))) return rc;
-  if (v && (rc = read_bulk(h, v, h->v, n))) return rc;
-  if (a && (rc = read_bulk(h, a, h->a, n))) return rc;
+  if (d && (rc = fetch_field(h, d, h->d))) return rc;
+  if (v && (rc = fetch_field(h, v, h->v))) return rc;
+  if (a && (rc = fetch_field(h, a, h->a))) return rc;
   return 0;
 }
 
 int sem2d_set_fields(sem2d_solver* h, const double* d, const double* v, const double* a) {
   if (!h) return 1;
-  const size_t n = (size_t)h->ndof * h->npoin;
   int rc = 0;
-  if (d && (rc = write_bulk(h, h->d, d, n))) return rc;
-  if (v && (rc = write_bulk(h, h->v, v, n))) return rc;
-  if (a && (rc = write_bulk(h, h->a, a, n))) return rc;
+  if (d && (rc = store_field(h, h->d, d))) return rc;
+  if (v && (rc = store_field(h, h->v, v))) return rc;
+  if (a && (rc = store_field(h, h->a, a))) return rc;
   return 0;
 }
 
@@ -728,6 +1093,7 @@ void sem2d_destroy(sem2d_solver* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
   for (void* p : h->allocs) hipFree(p);
+  for (void* p : h->st.prep_allocs) hipFree(p);
   for (auto& b : h->abso) { (void)b; }
   if (h->src_dev) hipFree(h->src_dev);
   if (h->ampl_dev) hipFree(h->ampl_dev);

@@ -1,0 +1,280 @@
+// Fused tile kernels for structured (MESH_CART) meshes.
+//
+// Fields live in node-grid order: node (X, Z) of the (4nx+1) x (4nz+1[+1]) GLL grid
+// at index Z*NX1 + X (a split fault duplicates one grid row).  The element grid is
+// cut into tiles of up to TX x TZ elements; one workgroup owns one tile and, in a
+// single pass over HBM, does for the tile's patch of (4*nxe+1) x (4*nze+1) nodes:
+//
+//   1. load d, v, a; Newmark predictor (SRC/solver.f90:59-60); predicted d in LDS;
+//   2. element forces f = -K d (ELAST_KD2_*, SRC/mat_elastic.f90:589-641,739-775)
+//      assembled into an LDS node array, colour by colour (2x2 checkerboard of the
+//      tile's elements: elements of one colour share no node, so the LDS updates
+//      need no atomics and the summation order is fixed);
+//   3. interior patch nodes: a = f*rmass, corrector (SRC/solver.f90:78-82), store;
+//      patch-perimeter nodes and "special" nodes (point sources): store the
+//      partial force of this tile to fpart and leave the node untouched.
+//
+// k_deferred then finishes every perimeter / special node: it sums the partial
+// forces of the tiles that share it (fixed tile order), adds sources and absorbing
+// boundaries (in BC order) and applies the corrector; split-fault nodes are handed
+// to the fault kernel instead (predicted d, v and the assembled force written to
+// d, v, a).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sem2d_kernels.hpp"
+
+namespace sem2d {
+
+constexpr int TX = 8, TZ = 8;
+constexpr int PXM = 4 * TX + 1, PZM = 4 * TZ + 1;
+constexpr int PMAX = PXM * PZM;
+constexpr int TILE_THREADS = 256;
+constexpr int NODE_ITERS = (PMAX + TILE_THREADS - 1) / TILE_THREADS;
+constexpr int PERIM_MAX = 2 * PXM + 2 * (PZM - 2);
+constexpr int MAXSPECIAL = 16;
+constexpr int SLOTS = PERIM_MAX + MAXSPECIAL;
+
+struct TileDev {
+  int NX1;            // grid row length (4*nx+1)
+  int nx;             // elements per row (cell index = iz*nx + ix)
+  int ntiles;
+  int per_xcd;        // tiles per XCD slice of the launch
+  const int4* tiles;  // (tx0, tz0, nxe | nze << 8, Z0)
+  const int* spec_cnt;
+  const int* spec_idx;  // [ntiles][MAXSPECIAL] local patch indices
+  size_t NGp;           // component stride of the field arrays
+  size_t FPS;           // component stride of fpart (ntiles*SLOTS)
+};
+
+__device__ __forceinline__ int perim_slot(int X, int Z, int PX, int PZ) {
+  if (Z == 0) return X;
+  if (Z == PZ - 1) return PX + X;
+  if (X == 0) return 2 * PX + (Z - 1);
+  if (X == PX - 1) return 2 * PX + (PZ - 2) + (Z - 1);
+  return -1;
+}
+
+template <typename R, int NDOF, int NELAST, int MODE>
+__global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict__ d, R* __restrict__ v,
+                                                       R* __restrict__ a, const R* __restrict__ rmass0,
+                                                       const R* __restrict__ coefs, const R* __restrict__ geom,
+                                                       R* __restrict__ fpart, R dt, R c1, R c2, R c3, R c4) {
+  __shared__ R Ud[NDOF][PMAX];
+  __shared__ R Fn[NDOF][PMAX];
+  __shared__ R T1[FE_EPB][NDOF][NP];
+  __shared__ R T2[FE_EPB][NDOF][NP];
+  __shared__ short sflag[PMAX];
+
+  // XCD-aware order: the 8 XCDs receive workgroups round-robin, so give each XCD a
+  // contiguous range of tiles (neighbouring tiles share patch rows in its L2).
+  const int bid = blockIdx.x;
+  const int t = (bid & 7) * T.per_xcd + (bid >> 3);
+  if (t >= T.ntiles) return;
+  const int tid = threadIdx.x;
+  const int4 tg = T.tiles[t];
+  const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
+  const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
+  const int X0 = 4 * tx0;
+  const size_t S = T.NGp;
+
+  R vp[NODE_ITERS][NDOF];
+#pragma unroll
+  for (int r = 0; r < NODE_ITERS; ++r) {
+    const int p = tid + TILE_THREADS * r;
+    if (p < NPT) {
+      const int Z = p / PX, X = p - Z * PX;
+      const size_t g = (size_t)(Z0 + Z) * T.NX1 + X0 + X;
+#pragma unroll
+      for (int c = 0; c < NDOF; ++c) {
+        const R dk = d[c * S + g], vk = v[c * S + g], ak = a[c * S + g];
+        Ud[c][p] = dk + dt * vk + c1 * ak;
+        vp[r][c] = vk + c2 * ak;
+        Fn[c][p] = 0;
+      }
+      sflag[p] = -1;
+    }
+  }
+  __syncthreads();
+  const int nspec = T.spec_cnt[t];
+  if (tid < nspec) sflag[T.spec_idx[t * MAXSPECIAL + tid]] = (short)tid;
+
+  // element forces, colour by colour
+  const int le = tid / NP;
+  const int pe = tid - le * NP;
+  const int ii = pe % NG, jj = pe / NG;
+#pragma unroll 1
+  for (int cc = 0; cc < 4; ++cc) {
+    const int cx = cc & 1, cz = cc >> 1;
+    const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
+    const int nce = ncx * ncz;
+#pragma unroll 1
+    for (int b0 = 0; b0 < nce; b0 += FE_EPB) {
+      const int ce = b0 + le;
+      const bool active = (tid < FE_THREADS) && (ce < nce);
+      int base = 0;
+      if (active) {
+        const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
+        base = 4 * lz * PX + 4 * lx;
+        const int cell = (tz0 + lz) * T.nx + tx0 + lx;
+        R A[NELAST];
+        load_coefs<R, NDOF, NELAST, MODE>(coefs, geom, cell, pe, A);
+        R dxi[NDOF], deta[NDOF];
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          R s1 = 0, s2 = 0;
+#pragma unroll
+          for (int k = 0; k < NG; ++k) {
+            s1 += Coef<R>::h(k, ii) * Ud[c][base + jj * PX + k];
+            s2 += Ud[c][base + k * PX + ii] * Coef<R>::h(k, jj);
+          }
+          dxi[c] = s1;
+          deta[c] = s2;
+        }
+        if constexpr (NDOF == 2) {
+          if constexpr (NELAST == 6) {
+            T1[le][0][pe] = A[0] * dxi[0] + A[1] * deta[1];
+            T2[le][0][pe] = A[3] * (deta[0] + dxi[1]);
+            T1[le][1][pe] = A[4] * deta[0] + A[5] * dxi[1];
+            T2[le][1][pe] = A[1] * dxi[0] + A[2] * deta[1];
+          } else {
+            T1[le][0][pe] = A[0] * dxi[0] + A[6] * deta[0] + A[7] * dxi[1] + A[1] * deta[1];
+            T2[le][0][pe] = A[6] * dxi[0] + A[3] * deta[0] + A[4] * dxi[1] + A[8] * deta[1];
+            T1[le][1][pe] = A[7] * dxi[0] + A[4] * deta[0] + A[5] * dxi[1] + A[9] * deta[1];
+            T2[le][1][pe] = A[1] * dxi[0] + A[8] * deta[0] + A[9] * dxi[1] + A[2] * deta[1];
+          }
+        } else {
+          if constexpr (NELAST == 2) {
+            T1[le][0][pe] = A[0] * dxi[0];
+            T2[le][0][pe] = A[1] * deta[0];
+          } else {
+            T1[le][0][pe] = A[0] * dxi[0] + A[2] * deta[0];
+            T2[le][0][pe] = A[2] * dxi[0] + A[1] * deta[0];
+          }
+        }
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          R s1 = 0, s2 = 0;
+#pragma unroll
+          for (int k = 0; k < NG; ++k) {
+            s1 += Coef<R>::h(ii, k) * T1[le][c][jj * NG + k];
+            s2 += T2[le][c][k * NG + ii] * Coef<R>::h(jj, k);
+          }
+          R* fp = &Fn[c][base + jj * PX + ii];
+          *fp = *fp + (s1 + s2);
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // finish interior nodes, hand perimeter / special nodes over
+  const size_t fbase = (size_t)t * SLOTS;
+#pragma unroll
+  for (int r = 0; r < NODE_ITERS; ++r) {
+    const int p = tid + TILE_THREADS * r;
+    if (p < NPT) {
+      const int Z = p / PX, X = p - Z * PX;
+      int slot = perim_slot(X, Z, PX, PZ);
+      if (slot < 0 && sflag[p] >= 0) slot = PERIM_MAX + sflag[p];
+      if (slot >= 0) {
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = Fn[c][p];
+      } else {
+        const size_t g = (size_t)(Z0 + Z) * T.NX1 + X0 + X;
+        const R rm = rmass0[g];
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          const R ak = Fn[c][p] * rm;
+          a[c * S + g] = ak;
+          v[c * S + g] = vp[r][c] + c3 * ak;
+          d[c * S + g] = Ud[c][p] + c4 * ak;
+        }
+      }
+    }
+  }
+}
+
+// One deferred node: tile-perimeter, special or split-fault node.
+struct DefNode {
+  int g;        // grid index
+  int pstart;   // partial offsets part_off[pstart .. pstart+pcnt)
+  int pcnt;
+  int sstart;   // sources src[sstart .. sstart+scnt)
+  int scnt;
+  int astart;   // absorbing coefficients abso[astart .. astart+acnt), [ndof] each
+  int acnt;
+  int fault;    // 1: split-fault node (finished by the fault kernels)
+};
+
+template <typename R, int NDOF>
+__global__ __launch_bounds__(256) void k_deferred(int ndef, const DefNode* __restrict__ defs,
+                                                  const int* __restrict__ part_off, const R* __restrict__ fpart,
+                                                  size_t FPS, const SrcEntry* __restrict__ src,
+                                                  const double* __restrict__ ampl, int nt, int it,
+                                                  const double* __restrict__ abso, const R* __restrict__ rmdef,
+                                                  R* __restrict__ d, R* __restrict__ v, R* __restrict__ a, size_t GS,
+                                                  R dt, R c1, R c2, R c3, R c4) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ndef) return;
+  const DefNode n = defs[q];
+  const size_t g = n.g;
+  R dp[NDOF], vpp[NDOF], f[NDOF];
+#pragma unroll
+  for (int c = 0; c < NDOF; ++c) {
+    const R dk = d[c * GS + g], vk = v[c * GS + g], ak = a[c * GS + g];
+    dp[c] = dk + dt * vk + c1 * ak;
+    vpp[c] = vk + c2 * ak;
+    R s = 0;
+    for (int k = 0; k < n.pcnt; ++k) s = s + fpart[c * FPS + part_off[n.pstart + k]];
+    f[c] = s;
+  }
+  for (int k = 0; k < n.scnt; ++k) {
+    const SrcEntry e = src[n.sstart + k];
+    const double am = (it >= 1 && it <= nt) ? ampl[(size_t)e.src * nt + (it - 1)] : 0.0;
+#pragma unroll
+    for (int c = 0; c < NDOF; ++c)
+      if (c == e.comp) f[c] = (R)((double)f[c] + e.coef * am);
+  }
+  for (int k = 0; k < n.acnt; ++k) {
+#pragma unroll
+    for (int c = 0; c < NDOF; ++c) f[c] = (R)((double)f[c] - abso[(size_t)(n.astart + k) * NDOF + c] * (double)vpp[c]);
+  }
+  if (n.fault) {
+#pragma unroll
+    for (int c = 0; c < NDOF; ++c) {
+      d[c * GS + g] = dp[c];
+      v[c * GS + g] = vpp[c];
+      a[c * GS + g] = f[c];
+    }
+    return;
+  }
+#pragma unroll
+  for (int c = 0; c < NDOF; ++c) {
+    const R ak = f[c] * rmdef[(size_t)c * ndef + q];
+    a[c * GS + g] = ak;
+    v[c * GS + g] = vpp[c] + c3 * ak;
+    d[c * GS + g] = dp[c] + c4 * ak;
+  }
+}
+
+// corrector on the split-fault nodes after the fault solve: a = f*rmass, v, d update
+template <typename R, int NDOF>
+__global__ void k_fault_correct(int n, const int* __restrict__ g, const R* __restrict__ rm, R* __restrict__ d,
+                                R* __restrict__ v, R* __restrict__ a, size_t GS, R c3, R c4) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const size_t k = g[q];
+#pragma unroll
+  for (int c = 0; c < NDOF; ++c) {
+    const R ak = a[c * GS + k] * rm[(size_t)c * n + q];
+    a[c * GS + k] = ak;
+    v[c * GS + k] = v[c * GS + k] + c3 * ak;
+    d[c * GS + k] = d[c * GS + k] + c4 * ak;
+  }
+}
+
+}  // namespace sem2d
