@@ -1,0 +1,186 @@
+! sem2d_hip.f90 -- ISO_C_BINDING interface to libsem2d_hip.so (include/sem2d.h).
+!
+! This is the thin Fortran layer through which the SEM2DPACK host code calls the
+! MI355X time step.  It only declares interoperable types and interfaces; it
+! depends on no other module, so it compiles on its own and can be dropped into the
+! reference's SRC/ directory unchanged.  The derived types mirror sem2d_desc and
+! sem2d_fault_desc field by field (same order, C-interoperable kinds).
+module sem2d_hip
+
+  use iso_c_binding
+  implicit none
+  private
+
+  type, bind(C), public :: sem2d_desc
+    integer(c_int) :: ndof = 0, ngll = 5, nelem = 0, npoin = 0
+    integer(c_int) :: nelast = 0, coef_mode = 0
+    type(c_ptr) :: ibool = c_null_ptr, coefs = c_null_ptr, lam_mu = c_null_ptr, &
+                   geom = c_null_ptr, wgll = c_null_ptr, hprime = c_null_ptr, rmass = c_null_ptr
+    real(c_double) :: dt = 0d0, beta = 0d0, gamma = 0.5d0
+    integer(c_int) :: bulk_fp32 = 0
+    integer(c_int) :: nx = 0, nz = 0, ezflt = 0
+    type(c_ptr) :: elem_of_cell = c_null_ptr
+  end type sem2d_desc
+
+  type, bind(C), public :: sem2d_fault_desc
+    integer(c_int) :: npoin = 0
+    type(c_ptr) :: node1 = c_null_ptr, node2 = c_null_ptr, n1 = c_null_ptr
+    type(c_ptr) :: B = c_null_ptr, invM1 = c_null_ptr, invM2 = c_null_ptr, Z = c_null_ptr
+    type(c_ptr) :: T0 = c_null_ptr, cohesion = c_null_ptr
+    real(c_double) :: coefA2V = 0d0, coefA2D = 0d0
+    integer(c_int) :: allow_opening = 1, law = 1, kind = 1, healing = 0
+    type(c_ptr) :: p(6) = c_null_ptr
+    type(c_ptr) :: theta0 = c_null_ptr, V0 = c_null_ptr
+    integer(c_int) :: oit = 0, oitd = 1, oix1 = 1, oixn = 1, oixd = 1
+  end type sem2d_fault_desc
+
+  public :: sem2d_create, sem2d_set_abso, sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
+            sem2d_set_receivers, sem2d_set_fault, sem2d_step, sem2d_fetch_fields, &
+            sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
+            sem2d_compute_fint, sem2d_get_step, sem2d_last_step_ms, sem2d_last_error, &
+            sem2d_destroy, sem2d_error_message
+
+  interface
+    integer(c_int) function sem2d_create(desc, h) bind(C, name='sem2d_create')
+      import :: c_int, c_ptr, sem2d_desc
+      type(sem2d_desc), intent(in) :: desc
+      type(c_ptr), intent(out) :: h
+    end function
+
+    integer(c_int) function sem2d_set_abso(h, n, node, C) bind(C, name='sem2d_set_abso')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: n
+      integer(c_int), intent(in) :: node(*)
+      real(c_double), intent(in) :: C(*)
+    end function
+
+    integer(c_int) function sem2d_set_force(h, iglob, dir, isrc) bind(C, name='sem2d_set_force')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: iglob
+      real(c_double), intent(in) :: dir(*)
+      integer(c_int), intent(out) :: isrc
+    end function
+
+    integer(c_int) function sem2d_set_moment(h, nel, iglob_xi, iglob_eta, coef_xi, coef_eta, isrc) &
+        bind(C, name='sem2d_set_moment')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: nel
+      integer(c_int), intent(in) :: iglob_xi(*), iglob_eta(*)
+      real(c_double), intent(in) :: coef_xi(*), coef_eta(*)
+      integer(c_int), intent(out) :: isrc
+    end function
+
+    integer(c_int) function sem2d_set_stf(h, isrc, nt, ampl) bind(C, name='sem2d_set_stf')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: isrc, nt
+      real(c_double), intent(in) :: ampl(*)
+    end function
+
+    integer(c_int) function sem2d_set_receivers(h, nrec, iglob, isamp, field, nsamp) &
+        bind(C, name='sem2d_set_receivers')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nrec, isamp, field, nsamp
+      integer(c_int), intent(in) :: iglob(*)
+    end function
+
+    integer(c_int) function sem2d_set_fault(h, f, nsamp) bind(C, name='sem2d_set_fault')
+      import :: c_int, c_ptr, sem2d_fault_desc
+      type(c_ptr), value :: h
+      type(sem2d_fault_desc), intent(in) :: f
+      integer(c_int), value :: nsamp
+    end function
+
+    integer(c_int) function sem2d_step(h, nsteps) bind(C, name='sem2d_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+    end function
+
+    integer(c_int) function sem2d_fetch_fields(h, d, v, a) bind(C, name='sem2d_fetch_fields')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: d(*), v(*), a(*)
+    end function
+
+    integer(c_int) function sem2d_set_fields(h, d, v, a) bind(C, name='sem2d_set_fields')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: d(*), v(*), a(*)
+    end function
+
+    integer(c_int) function sem2d_fetch_seismo(h, out, nsamp_taken) bind(C, name='sem2d_fetch_seismo')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), intent(out) :: nsamp_taken
+    end function
+
+    integer(c_int) function sem2d_fetch_fault(h, out, nsamp_taken) bind(C, name='sem2d_fetch_fault')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), intent(out) :: nsamp_taken
+    end function
+
+    integer(c_int) function sem2d_fetch_fault_state(h, D, V, T, MU, Tstick) &
+        bind(C, name='sem2d_fetch_fault_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: D(*), V(*), T(*), MU(*), Tstick(*)
+    end function
+
+    integer(c_int) function sem2d_compute_fint(h, d, f) bind(C, name='sem2d_compute_fint')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: d(*)
+      real(c_double), intent(out) :: f(*)
+    end function
+
+    integer(c_int) function sem2d_get_step(h, it) bind(C, name='sem2d_get_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: it
+    end function
+
+    integer(c_int) function sem2d_last_step_ms(h, ms) bind(C, name='sem2d_last_step_ms')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: ms
+    end function
+
+    type(c_ptr) function sem2d_last_error(h) bind(C, name='sem2d_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    subroutine sem2d_destroy(h) bind(C, name='sem2d_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end subroutine
+  end interface
+
+contains
+
+  ! Message of the last failed call, as a Fortran string (for IO_abort).
+  function sem2d_error_message(h) result(msg)
+    type(c_ptr), intent(in) :: h
+    character(len=512) :: msg
+    character(kind=c_char), pointer :: cs(:)
+    type(c_ptr) :: p
+    integer :: k
+    msg = ''
+    p = sem2d_last_error(h)
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, cs, [512])
+    do k = 1, 512
+      if (cs(k) == c_null_char) exit
+      msg(k:k) = cs(k)
+    enddo
+  end function sem2d_error_message
+
+end module sem2d_hip

@@ -15,6 +15,8 @@ directory under tests/golden/cases/ and stores compact FP64 vectors:
                                                         fields, every fault_every steps
   flt_f32 (fault cases)                                 the reference's own float32
                                                         FltXX_sem2d.dat records
+  seis_f32                                              the reference's own float32
+                                                        U?_sem2d.dat traces
 
 Usage: python tests/golden/make_golden.py [case ...]   (needs /root/reference)
 """
@@ -81,6 +83,12 @@ def run_case(name):
         isamp = int(rl.get("isamp", 1)) if rl is not None else 1
         series = {"D": nd["d"], "V": nd["v"], "A": nd["a"]}[field]
         out["seismo"] = series[::isamp, :, :nrec]
+        # the reference's own float32 seismogram files (REC_write, direct access)
+        names = ["Uy_sem2d.dat"] if g["ndof"] == 1 else ["Ux_sem2d.dat", "Uz_sem2d.dat"]
+        if all(os.path.exists(os.path.join(tmp, n)) for n in names):
+            nsamp = out["seismo"].shape[0]
+            out["seis_f32"] = np.stack([np.fromfile(os.path.join(tmp, n), np.float32).reshape(nrec, nsamp).T
+                                        for n in names], axis=1)
         if 5 in g["bounds"] and 6 in g["bounds"]:
             n1, n2 = g["bounds"][5]["node"], g["bounds"][6]["node"]
             keep = n1 != n2

@@ -63,3 +63,24 @@ def test_oracle_matches_reference(name, oracle_api):
             ours = rec[::st_every].astype(np.float32)
             cols = [0, 1, 2, 4] + ([5] if pb.fault["law"] == "SWF" else [])
             assert np.array_equal(ours[:, cols], ref32[:, cols])
+
+
+@pytest.mark.parametrize("name", ["small_psv", "sh_swf"])
+def test_output_files_match_reference(name, oracle_api, tmp_path):
+    """Seismogram and fault files written from the oracle's FP64 series are byte-compatible
+    with the reference's float32 files."""
+    from sem2dpack_amd import outputs
+    g = np.load(os.path.join(GOLDEN, name + ".npz"))
+    nt = int(g["nt"])
+    pb = builder.build(case_par(name), nsteps=nt)
+    s = solver.Solver(pb, api=oracle_api, compact=False)
+    s.step(nt)
+    r = pb.receivers
+    outputs.write_seismograms(str(tmp_path), s.seismograms(), pb.dt, r["isamp"], r["coord"])
+    back = outputs.read_seismograms(str(tmp_path), r["nsamp"], len(r["iglob"]), pb.ndof)
+    assert np.array_equal(back, g["seis_f32"])
+    if pb.fault is not None:
+        outputs.write_fault(str(tmp_path), pb.fault, s.fault_records(), pb.dt, nt)
+        nout = (pb.fault["oixn"] - pb.fault["oix1"]) // pb.fault["oixd"] + 1
+        f32 = outputs.read_fault(str(tmp_path / "Flt05_sem2d.dat"), nout)
+        assert np.array_equal(f32[::int(g["fault_every"])], g["flt_f32"])
