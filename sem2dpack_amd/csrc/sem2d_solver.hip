@@ -47,7 +47,10 @@ struct AbsoHost {
 struct StructState {
   bool on = false, ready = false;
   int nx = 0, nz = 0, ezflt = 0, NX1 = 0, NZ1 = 0;
-  std::vector<int> gmap;          // reference node (0-based) -> grid index
+  std::vector<int> gmap;          // reference node (0-based) -> storage index
+  std::vector<int> gridx;         // reference node (0-based) -> grid index Z*NX1 + X
+  int nvl = 0;                    // vertical tile-edge lines (stored column-contiguous)
+  size_t NS = 0;                  // field component stride: NX1*NZ1 + nvl*NZ1
   std::vector<int4> tiles;
   int ntiles = 0, per_xcd = 0;
   int4* tiles_dev = nullptr;
@@ -81,6 +84,7 @@ struct FaultHost {
 
 struct sem2d_solver {
   int ndof, nelem, npoin, nelast, coef_mode, fp32;
+  size_t fstride = 0;  // component stride of the device field arrays
   double dt, beta, gamma;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -306,7 +310,7 @@ int record_receivers(sem2d_solver* h) {
   int s = h->it / h->isamp;
   if (s >= h->nsamp) return 0;
   const R* fld = (const R*)(h->field == 0 ? h->d : (h->field == 1 ? h->v : h->a));
-  k_receivers<R><<<grid_for(h->nrec, 64), 64, 0, h->stream>>>(h->nrec, h->ndof, h->rec_node, fld, h->npoin,
+  k_receivers<R><<<grid_for(h->nrec, 64), 64, 0, h->stream>>>(h->nrec, h->ndof, h->rec_node, fld, h->fstride,
                                                               h->seis + (size_t)s * h->ndof * h->nrec);
   h->taken = std::max(h->taken, s + 1);
   return 0;
@@ -329,6 +333,18 @@ int record_fault(sem2d_solver* h) {
 // ---------------------------------------------------------------------------
 // Grid map from ibool and the cell -> element table; returns false if ibool does not
 // number a MESH_CART box the way the node grid does (the general path is used then).
+// Storage index of grid node (X, Z): nodes on vertical tile-edge lines (X a multiple
+// of 4*TX, or the right domain edge) live column-contiguous after the grid block, so
+// that the column edges of the tiles are read and written with unit stride.
+static size_t storage_index(const StructState& S, int X, int Z) {
+  const int W = 4 * TX;
+  int k = -1;
+  if (X % W == 0) k = X / W;
+  else if (X == S.NX1 - 1) k = S.nvl - 1;
+  if (k >= 0) return (size_t)S.NX1 * S.NZ1 + (size_t)k * S.NZ1 + Z;
+  return (size_t)Z * S.NX1 + X;
+}
+
 bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
   StructState& S = h->st;
   const int nx = s->nx, nz = s->nz, ez = s->ezflt;
@@ -338,6 +354,8 @@ bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
   S.NX1 = 4 * nx + 1;
   S.NZ1 = 4 * nz + 1 + (ez > 0 ? 1 : 0);
   if ((long)S.NX1 * S.NZ1 != s->npoin) return false;
+  S.nvl = (nx + TX - 1) / TX + 1;
+  S.NS = (size_t)S.NX1 * S.NZ1 + (size_t)S.nvl * S.NZ1;
   S.gmap.assign(s->npoin, -1);
   std::vector<char> used((size_t)s->npoin, 0);
   for (int iz = 0; iz < nz; ++iz) {
@@ -361,6 +379,8 @@ bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
   }
   for (int k = 0; k < s->npoin; ++k)
     if (S.gmap[k] < 0) return false;
+  S.gridx = S.gmap;
+  for (int k = 0; k < s->npoin; ++k) S.gmap[k] = (int)storage_index(S, S.gridx[k] % S.NX1, S.gridx[k] / S.NX1);
   // tiles: element-row bands split at the fault, TX x TZ tiles in each band
   S.tiles.clear();
   const int nb = ez > 0 ? 2 : 1;
@@ -406,8 +426,8 @@ int prepare_structured(sem2d_solver* h) {
   S.prep_allocs.clear();
   const int nd = h->ndof;
   // special nodes: point-source nodes inside a tile patch
-  std::vector<int> src_g;
-  for (const SrcEntry& e : h->src_host) src_g.push_back(S.gmap[e.node]);
+  std::vector<int> src_g;  // grid indices of source nodes
+  for (const SrcEntry& e : h->src_host) src_g.push_back(S.gridx[e.node]);
   std::sort(src_g.begin(), src_g.end());
   src_g.erase(std::unique(src_g.begin(), src_g.end()), src_g.end());
   std::vector<int> spec_cnt(S.ntiles, 0), spec_idx((size_t)S.ntiles * MAXSPECIAL, 0);
@@ -419,7 +439,7 @@ int prepare_structured(sem2d_solver* h) {
     const int4 tg = S.tiles[t];
     const int nxe = tg.z & 0xff, nze = tg.z >> 8;
     const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, X0 = 4 * tg.x, Z0 = tg.w;
-    auto gi = [&](int X, int Z) { return (Z0 + Z) * S.NX1 + X0 + X; };
+    auto gi = [&](int X, int Z) { return (int)storage_index(S, X0 + X, Z0 + Z); };
     for (int X = 0; X < PX; ++X) ents.push_back({gi(X, 0), t, X});
     for (int X = 0; X < PX; ++X) ents.push_back({gi(X, PZ - 1), t, PX + X});
     for (int Z = 1; Z < PZ - 1; ++Z) ents.push_back({gi(0, Z), t, 2 * PX + Z - 1});
@@ -429,7 +449,7 @@ int prepare_structured(sem2d_solver* h) {
       if (X > 0 && X < PX - 1 && Z > 0 && Z < PZ - 1) {
         if (spec_cnt[t] >= MAXSPECIAL) return set_err(h, 5, "too many source nodes in one tile");
         spec_idx[(size_t)t * MAXSPECIAL + spec_cnt[t]] = Z * PX + X;
-        ents.push_back({g, t, PERIM_MAX + spec_cnt[t]});
+        ents.push_back({(int)storage_index(S, X0 + X, Z0 + Z), t, PERIM_MAX + spec_cnt[t]});
         spec_cnt[t]++;
       }
     }
@@ -437,8 +457,8 @@ int prepare_structured(sem2d_solver* h) {
   std::sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) {
     return a.g != b.g ? a.g < b.g : a.t < b.t;
   });
-  // inverse map grid -> reference node, for sources / ABC / fault lookups
-  std::vector<int> ginv((size_t)h->npoin);
+  // inverse map storage -> reference node, for sources / ABC / fault lookups
+  std::vector<int> ginv(S.NS, -1);
   for (int k = 0; k < h->npoin; ++k) ginv[S.gmap[k]] = k;
   std::unordered_map<int, std::vector<int>> src_of;  // ref node -> src_host indices
   for (size_t q = 0; q < h->src_host.size(); ++q) src_of[h->src_host[q].node].push_back((int)q);
@@ -532,7 +552,9 @@ void launch_tile_t(sem2d_solver* h, R dt, R c1, R c2, R c3, R c4) {
   T.tiles = S.tiles_dev;
   T.spec_cnt = S.spec_cnt;
   T.spec_idx = S.spec_idx;
-  T.NGp = (size_t)h->npoin;
+  T.NGp = h->fstride;
+  T.NZ1 = S.NZ1;
+  T.nvl = S.nvl;
   T.FPS = S.FPS;
   k_tile<R, NDOF, NELAST, MODE><<<8 * S.per_xcd, TILE_THREADS, 0, h->stream>>>(
       T, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom, (R*)S.fpart, dt,
@@ -564,20 +586,20 @@ void launch_structured_step(sem2d_solver* h, int it, R dt, R c1, R c2, R c3, R c
   if (h->ndof == 2)
     k_deferred<R, 2><<<gd, 256, 0, h->stream>>>(S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dsrc,
                                                  h->ampl_dev, h->ampl_nt, it, S.dabso, (const R*)S.rmdef, (R*)h->d,
-                                                 (R*)h->v, (R*)h->a, (size_t)h->npoin, dt, c1, c2, c3, c4);
+                                                 (R*)h->v, (R*)h->a, h->fstride, dt, c1, c2, c3, c4);
   else
     k_deferred<R, 1><<<gd, 256, 0, h->stream>>>(S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dsrc,
                                                  h->ampl_dev, h->ampl_nt, it, S.dabso, (const R*)S.rmdef, (R*)h->d,
-                                                 (R*)h->v, (R*)h->a, (size_t)h->npoin, dt, c1, c2, c3, c4);
+                                                 (R*)h->v, (R*)h->a, h->fstride, dt, c1, c2, c3, c4);
   if (h->flt.on) {
     k_fault<R><<<grid_for(h->flt.dev.np, 64), 64, 0, h->stream>>>(h->flt.dev, (const R*)h->d, (const R*)h->v,
-                                                                   (R*)h->a, h->npoin);
+                                                                   (R*)h->a, h->fstride);
     if (h->ndof == 2)
       k_fault_correct<R, 2><<<grid_for(S.nfnodes, 64), 64, 0, h->stream>>>(
-          S.nfnodes, S.fault_g, (const R*)S.fault_rm, (R*)h->d, (R*)h->v, (R*)h->a, (size_t)h->npoin, c3, c4);
+          S.nfnodes, S.fault_g, (const R*)S.fault_rm, (R*)h->d, (R*)h->v, (R*)h->a, h->fstride, c3, c4);
     else
       k_fault_correct<R, 1><<<grid_for(S.nfnodes, 64), 64, 0, h->stream>>>(
-          S.nfnodes, S.fault_g, (const R*)S.fault_rm, (R*)h->d, (R*)h->v, (R*)h->a, (size_t)h->npoin, c3, c4);
+          S.nfnodes, S.fault_g, (const R*)S.fault_rm, (R*)h->d, (R*)h->v, (R*)h->a, h->fstride, c3, c4);
   }
 }
 
@@ -710,10 +732,12 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   const size_t nf = (size_t)h->ndof * h->npoin;
   if ((rc = upload_bulk(h, &h->rmass, s->rmass, nf))) return rc;
   h->rmass_host.assign(s->rmass, s->rmass + nf);
+  h->fstride = (size_t)h->npoin;
   if (structured_map(h, s)) {
     StructState& S = h->st;
     S.on = true;
-    std::vector<double> rm0((size_t)h->npoin);
+    h->fstride = S.NS;
+    std::vector<double> rm0(S.NS, 0.0);
     for (int k = 0; k < h->npoin; ++k) rm0[S.gmap[k]] = s->rmass[k];
     if ((rc = upload_bulk(h, &S.rmass0, rm0.data(), rm0.size()))) return rc;
     if (h->coef_mode == COEF_SHARED) {
@@ -734,13 +758,14 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
       }
     }
   }
-  if ((rc = dev_alloc(h, (char**)&h->d, nf * rsize(h)))) return rc;
-  if ((rc = dev_alloc(h, (char**)&h->v, nf * rsize(h)))) return rc;
-  if ((rc = dev_alloc(h, (char**)&h->a, nf * rsize(h)))) return rc;
+  const size_t nfs = (size_t)h->ndof * h->fstride;
+  if ((rc = dev_alloc(h, (char**)&h->d, nfs * rsize(h)))) return rc;
+  if ((rc = dev_alloc(h, (char**)&h->v, nfs * rsize(h)))) return rc;
+  if ((rc = dev_alloc(h, (char**)&h->a, nfs * rsize(h)))) return rc;
   if ((rc = dev_alloc(h, &h->flags, 4))) return rc;
-  HIPCHK(h, hipMemsetAsync(h->d, 0, nf * rsize(h), h->stream));
-  HIPCHK(h, hipMemsetAsync(h->v, 0, nf * rsize(h), h->stream));
-  HIPCHK(h, hipMemsetAsync(h->a, 0, nf * rsize(h), h->stream));
+  HIPCHK(h, hipMemsetAsync(h->d, 0, nfs * rsize(h), h->stream));
+  HIPCHK(h, hipMemsetAsync(h->v, 0, nfs * rsize(h), h->stream));
+  HIPCHK(h, hipMemsetAsync(h->a, 0, nfs * rsize(h), h->stream));
   HIPCHK(h, hipMemsetAsync(h->flags, 0, 4 * sizeof(int), h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -984,23 +1009,23 @@ int sem2d_step(sem2d_solver* h, int nsteps) {
 static int fetch_field(sem2d_solver* h, double* out, const void* dev) {
   const size_t n = (size_t)h->ndof * h->npoin;
   if (!h->st.on) return read_bulk(h, out, dev, n);
-  std::vector<double> tmp(n);
-  int rc = read_bulk(h, tmp.data(), dev, n);
+  const size_t S = h->fstride;
+  std::vector<double> tmp((size_t)h->ndof * S);
+  int rc = read_bulk(h, tmp.data(), dev, tmp.size());
   if (rc) return rc;
   for (int c = 0; c < h->ndof; ++c)
-    for (int k = 0; k < h->npoin; ++k)
-      out[(size_t)c * h->npoin + k] = tmp[(size_t)c * h->npoin + h->st.gmap[k]];
+    for (int k = 0; k < h->npoin; ++k) out[(size_t)c * h->npoin + k] = tmp[(size_t)c * S + h->st.gmap[k]];
   return 0;
 }
 
 static int store_field(sem2d_solver* h, void* dev, const double* in) {
   const size_t n = (size_t)h->ndof * h->npoin;
   if (!h->st.on) return write_bulk(h, dev, in, n);
-  std::vector<double> tmp(n);
+  const size_t S = h->fstride;
+  std::vector<double> tmp((size_t)h->ndof * S, 0.0);
   for (int c = 0; c < h->ndof; ++c)
-    for (int k = 0; k < h->npoin; ++k)
-      tmp[(size_t)c * h->npoin + h->st.gmap[k]] = in[(size_t)c * h->npoin + k];
-  return write_bulk(h, dev, tmp.data(), n);
+    for (int k = 0; k < h->npoin; ++k) tmp[(size_t)c * S + h->st.gmap[k]] = in[(size_t)c * h->npoin + k];
+  return write_bulk(h, dev, tmp.data(), tmp.size());
 }
 
 int sem2d_fetch_fields(sem2d_solver* h, double* d, double* v, double* a) {

@@ -44,8 +44,21 @@ struct TileDev {
   const int* spec_cnt;
   const int* spec_idx;  // [ntiles][MAXSPECIAL] local patch indices
   size_t NGp;           // component stride of the field arrays
+  int NZ1;              // grid rows
+  int nvl;              // vertical tile-edge lines stored after the grid block
   size_t FPS;           // component stride of fpart (ntiles*SLOTS)
 };
+
+// Storage index of patch node (X, Z) of the tile at (X0, Z0) whose left and right
+// columns lie on vertical tile-edge lines kL and kR (stored column-contiguous
+// after the NX1*NZ1 grid block); other nodes use grid order Z*NX1 + X.
+__device__ __forceinline__ size_t node_index(const TileDev& T, int X, int Z, int PX, int X0, int Z0, int kL,
+                                             int kR) {
+  const size_t grid = (size_t)T.NX1 * T.NZ1;
+  if (X == 0) return grid + (size_t)kL * T.NZ1 + (Z0 + Z);
+  if (X == PX - 1) return grid + (size_t)kR * T.NZ1 + (Z0 + Z);
+  return (size_t)(Z0 + Z) * T.NX1 + X0 + X;
+}
 
 __device__ __forceinline__ int perim_slot(int X, int Z, int PX, int PZ) {
   if (Z == 0) return X;
@@ -77,6 +90,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
   const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
   const int X0 = 4 * tx0;
   const size_t S = T.NGp;
+  const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
 
   R vp[NODE_ITERS][NDOF];
 #pragma unroll
@@ -84,7 +98,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
     const int p = tid + TILE_THREADS * r;
     if (p < NPT) {
       const int Z = p / PX, X = p - Z * PX;
-      const size_t g = (size_t)(Z0 + Z) * T.NX1 + X0 + X;
+      const size_t g = node_index(T, X, Z, PX, X0, Z0, kL, kR);
 #pragma unroll
       for (int c = 0; c < NDOF; ++c) {
         const R dk = d[c * S + g], vk = v[c * S + g], ak = a[c * S + g];
@@ -184,7 +198,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = Fn[c][p];
       } else {
-        const size_t g = (size_t)(Z0 + Z) * T.NX1 + X0 + X;
+        const size_t g = node_index(T, X, Z, PX, X0, Z0, kL, kR);
         const R rm = rmass0[g];
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
