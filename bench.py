@@ -93,7 +93,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="time a 1 GiB device-to-device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 27  # 2^27 doubles = 1 GiB
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / 10
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "ms": 1e3 * t,
+                          "GBps_rw": 16 * n / t / 1e9}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is a design note, not an implemented path.", file=sys.stderr)
