@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <map>
 #include <unordered_map>
 #include <string>
@@ -53,6 +54,7 @@ struct StructState {
   size_t NS = 0;                  // field component stride: NX1*NZ1 + nvl*NZ1
   std::vector<int4> tiles;
   int ntiles = 0, per_xcd = 0;
+  int pipe_wgs = 0;  // > 0: persistent pipelined tile kernel with this many workgroups
   int4* tiles_dev = nullptr;
   int* spec_cnt = nullptr;
   int* spec_idx = nullptr;
@@ -395,6 +397,16 @@ bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
   }
   S.ntiles = (int)S.tiles.size();
   S.per_xcd = (S.ntiles + 7) / 8;
+  {
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    int per_cu = 2;
+    const char* env = getenv("SEM2D_TILE_WG_PER_CU");
+    if (env) per_cu = atoi(env);
+    // the persistent kernel pays off once there are several tiles per workgroup
+    S.pipe_wgs = per_cu > 0 && S.ntiles >= 4 * ncu * per_cu ? ncu * per_cu : 0;
+    S.pipe_wgs = ((S.pipe_wgs + 7) / 8) * 8;
+  }
   return true;
 }
 
@@ -556,9 +568,17 @@ void launch_tile_t(sem2d_solver* h, R dt, R c1, R c2, R c3, R c4) {
   T.NZ1 = S.NZ1;
   T.nvl = S.nvl;
   T.FPS = S.FPS;
-  k_tile<R, NDOF, NELAST, MODE><<<8 * S.per_xcd, TILE_THREADS, 0, h->stream>>>(
-      T, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom, (R*)S.fpart, dt,
-      c1, c2, c3, c4);
+  if (S.pipe_wgs > 0) {
+    // persistent pipelined kernel: pipe_wgs workgroups, each on a run of tiles
+    const int per = (S.ntiles + S.pipe_wgs - 1) / S.pipe_wgs;
+    k_tile_pipe<R, NDOF, NELAST, MODE><<<S.pipe_wgs, TILE_THREADS, 0, h->stream>>>(
+        T, per, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom,
+        (R*)S.fpart, dt, c1, c2, c3, c4);
+  } else {
+    k_tile<R, NDOF, NELAST, MODE><<<8 * S.per_xcd, TILE_THREADS, 0, h->stream>>>(
+        T, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom, (R*)S.fpart,
+        dt, c1, c2, c3, c4);
+  }
 }
 
 template <typename R, int NDOF, int NELAST>

@@ -212,6 +212,239 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
   }
 }
 
+// Persistent, software-pipelined variant of k_tile.  Each workgroup walks a
+// contiguous run of tiles [t0, t1) (runs are grouped per XCD).  For the current tile
+// it first loads its per-element moduli (compact mode) into registers, then issues
+// the global loads of the next tile's d, v, a into registers and only then runs the
+// LDS element phase of the current tile, so that HBM transfers of tile i+1 overlap
+// the element computation of tile i.  Results are identical to k_tile: same
+// predictor, same colour order and same summation order.
+constexpr int PB_MAXB = 8;  // point tasks per thread and tile: 4 colours x 2 batches of FE_EPB elements
+
+template <typename R, int NDOF, int NELAST, int MODE>
+__global__ __launch_bounds__(TILE_THREADS) void k_tile_pipe(TileDev T, int tiles_per_wg, R* __restrict__ d,
+                                                            R* __restrict__ v, R* __restrict__ a,
+                                                            const R* __restrict__ rmass0,
+                                                            const R* __restrict__ coefs, const R* __restrict__ geom,
+                                                            R* __restrict__ fpart, R dt, R c1, R c2, R c3, R c4) {
+  __shared__ R Ud[NDOF][PMAX];
+  __shared__ R Fn[NDOF][PMAX];
+  __shared__ R T1[FE_EPB][NDOF][NP];
+  __shared__ R T2[FE_EPB][NDOF][NP];
+  __shared__ short sflag[PMAX];
+  // H, GLL weight products and per-element geometric factors live in LDS: every
+  // global (vector) load issued during the element phase would otherwise wait for
+  // the outstanding prefetch of the next tile (vector loads retire in order).
+  __shared__ double sH[NP];
+  __shared__ double sW2[NP];
+  __shared__ R Gm[TX * TZ][3];
+
+  const int nwg = gridDim.x;
+  const int bid = blockIdx.x;
+  // workgroup -> run of tiles: XCD x = bid & 7 owns the x-th eighth of the tiles
+  const int per_x = (nwg + 7) >> 3;
+  const int w = (bid & 7) * per_x + (bid >> 3);
+  const int t0 = w * tiles_per_wg;
+  const int t1 = min(t0 + tiles_per_wg, T.ntiles);
+  if (t0 >= t1) return;
+  const int tid = threadIdx.x;
+  const size_t S = T.NGp;
+  const int le = tid / NP;
+  const int pe = tid - le * NP;
+  const int ii = pe % NG, jj = pe / NG;
+  if (tid < NP) {
+    sH[tid] = c_H[tid];
+    sW2[tid] = c_W2[tid];
+  }
+
+  // prefetch registers: d, v, a of this thread's patch nodes of the next tile
+  R pd[NODE_ITERS][NDOF], pv[NODE_ITERS][NDOF], pa[NODE_ITERS][NDOF];
+  auto issue_loads = [&](int t) {
+    const int4 tg = T.tiles[t];
+    const int tx0 = tg.x, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
+    const int PX = 4 * nxe + 1, NPT = PX * (4 * nze + 1), X0 = 4 * tx0;
+    const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
+#pragma unroll
+    for (int r = 0; r < NODE_ITERS; ++r) {
+      const int p = tid + TILE_THREADS * r;
+      if (p < NPT) {
+        const int Z = p / PX, X = p - Z * PX;
+        const size_t g = node_index(T, X, Z, PX, X0, Z0, kL, kR);
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          pd[r][c] = d[c * S + g];
+          pv[r][c] = v[c * S + g];
+          pa[r][c] = a[c * S + g];
+        }
+      }
+    }
+  };
+  issue_loads(t0);
+
+#pragma unroll 1
+  for (int t = t0; t < t1; ++t) {
+    const int4 tg = T.tiles[t];
+    const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
+    const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
+    const int X0 = 4 * tx0;
+    const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
+
+    // predictor from the prefetched values
+    R vp[NODE_ITERS][NDOF];
+#pragma unroll
+    for (int r = 0; r < NODE_ITERS; ++r) {
+      const int p = tid + TILE_THREADS * r;
+      if (p < NPT) {
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          Ud[c][p] = pd[r][c] + dt * pv[r][c] + c1 * pa[r][c];
+          vp[r][c] = pv[r][c] + c2 * pa[r][c];
+          Fn[c][p] = 0;
+        }
+        sflag[p] = -1;
+      }
+    }
+    // moduli of this thread's point tasks (compact mode), before the next prefetch
+    R lm[PB_MAXB][NDOF];
+    if constexpr (MODE == COEF_COMPACT) {
+      if (tid < nxe * nze * 3) {
+        const int e = tid / 3, k = tid - 3 * (tid / 3);
+        const int lx = e % nxe, lz = e / nxe;
+        Gm[e][k] = geom[((size_t)(tz0 + lz) * T.nx + tx0 + lx) * 3 + k];
+      }
+#pragma unroll
+      for (int q = 0; q < PB_MAXB; ++q) {
+        const int cc = q >> 1, b0 = (q & 1) * FE_EPB;
+        const int cx = cc & 1, cz = cc >> 1;
+        const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
+        const int ce = b0 + le;
+        if (tid < FE_THREADS && ce < ncx * ncz) {
+          const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
+          const size_t cell = (size_t)(tz0 + lz) * T.nx + tx0 + lx;
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) lm[q][c] = coefs[cell * NDOF * NP + c * NP + pe];
+        }
+      }
+    }
+    __syncthreads();
+    const int nspec = T.spec_cnt[t];
+    if (tid < nspec) sflag[T.spec_idx[t * MAXSPECIAL + tid]] = (short)tid;
+    // overlap: loads of the next tile fly during this tile's element phase
+    if (t + 1 < t1) issue_loads(t + 1);
+
+#pragma unroll
+    for (int q = 0; q < PB_MAXB; ++q) {
+      const int cc = q >> 1, b0 = (q & 1) * FE_EPB;
+      const int cx = cc & 1, cz = cc >> 1;
+      const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
+      const int nce = ncx * ncz;
+      if (b0 >= nce) continue;  // uniform across the workgroup
+      const int ce = b0 + le;
+      const bool active = (tid < FE_THREADS) && (ce < nce);
+      int base = 0;
+      if (active) {
+        const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
+        base = 4 * lz * PX + 4 * lx;
+        R A[NELAST];
+        if constexpr (MODE == COEF_COMPACT) {
+          const R w2 = (R)sW2[pe];
+          const int el = lz * nxe + lx;
+          const R g0 = Gm[el][0], g1 = Gm[el][1], g2 = Gm[el][2];
+          if constexpr (NDOF == 2) {
+            const R la = lm[q][0], mu = lm[q][1], kx = la + 2 * mu;
+            A[0] = w2 * kx * g0;
+            A[1] = w2 * la * g2;
+            A[2] = w2 * kx * g1;
+            A[3] = w2 * mu * g1;
+            A[4] = w2 * mu * g2;
+            A[5] = w2 * mu * g0;
+          } else {
+            A[0] = w2 * lm[q][0] * g0;
+            A[1] = w2 * lm[q][0] * g1;
+          }
+        } else {
+          const int cell = (tz0 + lz) * T.nx + tx0 + lx;
+          load_coefs<R, NDOF, NELAST, MODE>(coefs, geom, cell, pe, A);
+        }
+        R dxi[NDOF], deta[NDOF];
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          R s1 = 0, s2 = 0;
+#pragma unroll
+          for (int k = 0; k < NG; ++k) {
+            s1 += (R)sH[k * NG + ii] * Ud[c][base + jj * PX + k];
+            s2 += Ud[c][base + k * PX + ii] * (R)sH[k * NG + jj];
+          }
+          dxi[c] = s1;
+          deta[c] = s2;
+        }
+        if constexpr (NDOF == 2) {
+          if constexpr (NELAST == 6) {
+            T1[le][0][pe] = A[0] * dxi[0] + A[1] * deta[1];
+            T2[le][0][pe] = A[3] * (deta[0] + dxi[1]);
+            T1[le][1][pe] = A[4] * deta[0] + A[5] * dxi[1];
+            T2[le][1][pe] = A[1] * dxi[0] + A[2] * deta[1];
+          } else {
+            T1[le][0][pe] = A[0] * dxi[0] + A[6] * deta[0] + A[7] * dxi[1] + A[1] * deta[1];
+            T2[le][0][pe] = A[6] * dxi[0] + A[3] * deta[0] + A[4] * dxi[1] + A[8] * deta[1];
+            T1[le][1][pe] = A[7] * dxi[0] + A[4] * deta[0] + A[5] * dxi[1] + A[9] * deta[1];
+            T2[le][1][pe] = A[1] * dxi[0] + A[8] * deta[0] + A[9] * dxi[1] + A[2] * deta[1];
+          }
+        } else {
+          if constexpr (NELAST == 2) {
+            T1[le][0][pe] = A[0] * dxi[0];
+            T2[le][0][pe] = A[1] * deta[0];
+          } else {
+            T1[le][0][pe] = A[0] * dxi[0] + A[2] * deta[0];
+            T2[le][0][pe] = A[2] * dxi[0] + A[1] * deta[0];
+          }
+        }
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          R s1 = 0, s2 = 0;
+#pragma unroll
+          for (int k = 0; k < NG; ++k) {
+            s1 += (R)sH[ii * NG + k] * T1[le][c][jj * NG + k];
+            s2 += T2[le][c][k * NG + ii] * (R)sH[jj * NG + k];
+          }
+          R* fp = &Fn[c][base + jj * PX + ii];
+          *fp = *fp + (s1 + s2);
+        }
+      }
+      __syncthreads();
+    }
+
+    const size_t fbase = (size_t)t * SLOTS;
+#pragma unroll
+    for (int r = 0; r < NODE_ITERS; ++r) {
+      const int p = tid + TILE_THREADS * r;
+      if (p < NPT) {
+        const int Z = p / PX, X = p - Z * PX;
+        int slot = perim_slot(X, Z, PX, PZ);
+        if (slot < 0 && sflag[p] >= 0) slot = PERIM_MAX + sflag[p];
+        if (slot >= 0) {
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = Fn[c][p];
+        } else {
+          const size_t g = node_index(T, X, Z, PX, X0, Z0, kL, kR);
+          const R rm = rmass0[g];
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) {
+            const R ak = Fn[c][p] * rm;
+            a[c * S + g] = ak;
+            v[c * S + g] = vp[r][c] + c3 * ak;
+            d[c * S + g] = Ud[c][p] + c4 * ak;
+          }
+        }
+      }
+    }
+    __syncthreads();  // LDS reuse by the next tile
+  }
+}
+
 // One deferred node: tile-perimeter, special or split-fault node.
 struct DefNode {
   int g;        // grid index
