@@ -744,6 +744,18 @@ def _read_cd(par, group_iter, value, name_h):
             raise InputError("read_pwconr_dist: needs more than 2 zones (num)")
         return ("pwconr", dict(num=num, ref=[float(ref[0]), float(ref[1])], rad=vals[:num - 1],
                                val=vals[num - 1:2 * num - 1]))
+    if want == "DIST_LINEAR":  # SRC/distribution_linear.f90 read_linear_dist (no smoothing)
+        if float(g.get("length", 0.0)) > 0:
+            raise InputError("DIST_LINEAR with smoothing length is not supported by this build")
+        dim = int(g.get("dim", 1))
+        fname = g.get("file", "")
+        if fname:
+            tab = np.loadtxt(fname, ndmin=2)
+        else:
+            n = int(g.get("n", 0))
+            vals = " ".join(g.data_lines).replace("d", "e").replace("D", "e").split()
+            tab = np.array([float(v) for v in vals[:2 * n]]).reshape(n, 2)
+        return ("linear", dict(dim=dim, x=tab[:, 0].copy(), val=tab[:, 1].copy()))
     raise InputError(f"distribution {want} is not supported by this build")
 
 
@@ -762,6 +774,14 @@ def _gen_cd(cd, coord):
                     break
             out[n] = p["val"][z]
         return out
+    if kind == "linear":  # interpol of SRC/distribution_linear.f90
+        xi, yi = p["x"], p["val"]
+        xo = coord[:, p["dim"] - 1]
+        ni = len(xi)
+        ipos = np.searchsorted(xi, xo, side="right")  # number of knots <= x (1-based position)
+        ipos = np.clip(ipos, 1, ni - 1) - 1
+        eta = (xo - xi[ipos]) / (xi[ipos + 1] - xi[ipos])
+        return yi[ipos] + eta * (yi[ipos + 1] - yi[ipos])
     raise ValueError(kind)
 
 

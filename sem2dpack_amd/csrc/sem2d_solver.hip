@@ -55,6 +55,7 @@ struct StructState {
   std::vector<int4> tiles;
   int ntiles = 0, per_xcd = 0;
   int pipe_wgs = 0;  // > 0: persistent pipelined tile kernel with this many workgroups
+  int pipe_threads = 512;
   int4* tiles_dev = nullptr;
   int* spec_cnt = nullptr;
   int* spec_idx = nullptr;
@@ -403,6 +404,8 @@ bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
     int per_cu = 2;
     const char* env = getenv("SEM2D_TILE_WG_PER_CU");
     if (env) per_cu = atoi(env);
+    const char* envt = getenv("SEM2D_TILE_THREADS");
+    if (envt) S.pipe_threads = atoi(envt) == 256 ? 256 : 512;
     // the persistent kernel pays off once there are several tiles per workgroup
     S.pipe_wgs = per_cu > 0 && S.ntiles >= 4 * ncu * per_cu ? ncu * per_cu : 0;
     S.pipe_wgs = ((S.pipe_wgs + 7) / 8) * 8;
@@ -571,9 +574,14 @@ void launch_tile_t(sem2d_solver* h, R dt, R c1, R c2, R c3, R c4) {
   if (S.pipe_wgs > 0) {
     // persistent pipelined kernel: pipe_wgs workgroups, each on a run of tiles
     const int per = (S.ntiles + S.pipe_wgs - 1) / S.pipe_wgs;
-    k_tile_pipe<R, NDOF, NELAST, MODE><<<S.pipe_wgs, TILE_THREADS, 0, h->stream>>>(
-        T, per, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom,
-        (R*)S.fpart, dt, c1, c2, c3, c4);
+    if (S.pipe_threads == 256)
+      k_tile_pipe<R, NDOF, NELAST, MODE, 256, 10><<<S.pipe_wgs, 256, 0, h->stream>>>(
+          T, per, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom,
+          (R*)S.fpart, dt, c1, c2, c3, c4);
+    else
+      k_tile_pipe<R, NDOF, NELAST, MODE, 512, 20><<<S.pipe_wgs, 512, 0, h->stream>>>(
+          T, per, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom,
+          (R*)S.fpart, dt, c1, c2, c3, c4);
   } else {
     k_tile<R, NDOF, NELAST, MODE><<<8 * S.per_xcd, TILE_THREADS, 0, h->stream>>>(
         T, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom, (R*)S.fpart,

@@ -219,18 +219,23 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
 // LDS element phase of the current tile, so that HBM transfers of tile i+1 overlap
 // the element computation of tile i.  Results are identical to k_tile: same
 // predictor, same colour order and same summation order.
-constexpr int PB_MAXB = 8;  // point tasks per thread and tile: 4 colours x 2 batches of FE_EPB elements
-
-template <typename R, int NDOF, int NELAST, int MODE>
-__global__ __launch_bounds__(TILE_THREADS) void k_tile_pipe(TileDev T, int tiles_per_wg, R* __restrict__ d,
+// NT threads per workgroup, EPB elements per element batch (EPB*25 <= NT).
+template <typename R, int NDOF, int NELAST, int MODE, int NT, int EPB>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 4 : 2))) void k_tile_pipe(TileDev T, int tiles_per_wg, R* __restrict__ d,
                                                             R* __restrict__ v, R* __restrict__ a,
                                                             const R* __restrict__ rmass0,
                                                             const R* __restrict__ coefs, const R* __restrict__ geom,
                                                             R* __restrict__ fpart, R dt, R c1, R c2, R c3, R c4) {
   __shared__ R Ud[NDOF][PMAX];
   __shared__ R Fn[NDOF][PMAX];
-  __shared__ R T1[FE_EPB][NDOF][NP];
-  __shared__ R T2[FE_EPB][NDOF][NP];
+  constexpr int NITER = (PMAX + NT - 1) / NT;                       // patch nodes per thread
+  constexpr int NBATCH = (TX * TZ / 4 + EPB - 1) / EPB;              // batches per colour
+  constexpr int PB_MAXB = 4 * NBATCH;                               // point tasks per thread
+  constexpr int ETHREADS = EPB * NP;
+  static_assert(ETHREADS <= NT, "element batch larger than the workgroup");
+  __shared__ R Vp[NDOF][PMAX];  // predicted velocity (kept in LDS to free registers)
+  __shared__ R T1[EPB][NDOF][NP];
+  __shared__ R T2[EPB][NDOF][NP];
   __shared__ short sflag[PMAX];
   // H, GLL weight products and per-element geometric factors live in LDS: every
   // global (vector) load issued during the element phase would otherwise wait for
@@ -258,15 +263,15 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_pipe(TileDev T, int tiles
   }
 
   // prefetch registers: d, v, a of this thread's patch nodes of the next tile
-  R pd[NODE_ITERS][NDOF], pv[NODE_ITERS][NDOF], pa[NODE_ITERS][NDOF];
+  R pd[NITER][NDOF], pv[NITER][NDOF], pa[NITER][NDOF];
   auto issue_loads = [&](int t) {
     const int4 tg = T.tiles[t];
     const int tx0 = tg.x, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
     const int PX = 4 * nxe + 1, NPT = PX * (4 * nze + 1), X0 = 4 * tx0;
     const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
 #pragma unroll
-    for (int r = 0; r < NODE_ITERS; ++r) {
-      const int p = tid + TILE_THREADS * r;
+    for (int r = 0; r < NITER; ++r) {
+      const int p = tid + NT * r;
       if (p < NPT) {
         const int Z = p / PX, X = p - Z * PX;
         const size_t g = node_index(T, X, Z, PX, X0, Z0, kL, kR);
@@ -290,15 +295,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_pipe(TileDev T, int tiles
     const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
 
     // predictor from the prefetched values
-    R vp[NODE_ITERS][NDOF];
 #pragma unroll
-    for (int r = 0; r < NODE_ITERS; ++r) {
-      const int p = tid + TILE_THREADS * r;
+    for (int r = 0; r < NITER; ++r) {
+      const int p = tid + NT * r;
       if (p < NPT) {
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
           Ud[c][p] = pd[r][c] + dt * pv[r][c] + c1 * pa[r][c];
-          vp[r][c] = pv[r][c] + c2 * pa[r][c];
+          Vp[c][p] = pv[r][c] + c2 * pa[r][c];
           Fn[c][p] = 0;
         }
         sflag[p] = -1;
@@ -314,11 +318,11 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_pipe(TileDev T, int tiles
       }
 #pragma unroll
       for (int q = 0; q < PB_MAXB; ++q) {
-        const int cc = q >> 1, b0 = (q & 1) * FE_EPB;
+        const int cc = q / NBATCH, b0 = (q % NBATCH) * EPB;
         const int cx = cc & 1, cz = cc >> 1;
         const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
         const int ce = b0 + le;
-        if (tid < FE_THREADS && ce < ncx * ncz) {
+        if (tid < ETHREADS && ce < ncx * ncz) {
           const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
           const size_t cell = (size_t)(tz0 + lz) * T.nx + tx0 + lx;
 #pragma unroll
@@ -334,13 +338,13 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_pipe(TileDev T, int tiles
 
 #pragma unroll
     for (int q = 0; q < PB_MAXB; ++q) {
-      const int cc = q >> 1, b0 = (q & 1) * FE_EPB;
+      const int cc = q / NBATCH, b0 = (q % NBATCH) * EPB;
       const int cx = cc & 1, cz = cc >> 1;
       const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
       const int nce = ncx * ncz;
       if (b0 >= nce) continue;  // uniform across the workgroup
       const int ce = b0 + le;
-      const bool active = (tid < FE_THREADS) && (ce < nce);
+      const bool active = (tid < ETHREADS) && (ce < nce);
       int base = 0;
       if (active) {
         const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
@@ -419,8 +423,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_pipe(TileDev T, int tiles
 
     const size_t fbase = (size_t)t * SLOTS;
 #pragma unroll
-    for (int r = 0; r < NODE_ITERS; ++r) {
-      const int p = tid + TILE_THREADS * r;
+    for (int r = 0; r < NITER; ++r) {
+      const int p = tid + NT * r;
       if (p < NPT) {
         const int Z = p / PX, X = p - Z * PX;
         int slot = perim_slot(X, Z, PX, PZ);
@@ -435,7 +439,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_pipe(TileDev T, int tiles
           for (int c = 0; c < NDOF; ++c) {
             const R ak = Fn[c][p] * rm;
             a[c * S + g] = ak;
-            v[c * S + g] = vp[r][c] + c3 * ak;
+            v[c * S + g] = Vp[c][p] + c3 * ak;
             d[c * S + g] = Ud[c][p] + c4 * ak;
           }
         }
