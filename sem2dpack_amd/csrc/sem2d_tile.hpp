@@ -213,13 +213,20 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
 }
 
 // Persistent, software-pipelined variant of k_tile.  Each workgroup walks a
-// contiguous run of tiles [t0, t1) (runs are grouped per XCD).  For the current tile
-// it first loads its per-element moduli (compact mode) into registers, then issues
-// the global loads of the next tile's d, v, a into registers and only then runs the
-// LDS element phase of the current tile, so that HBM transfers of tile i+1 overlap
-// the element computation of tile i.  Results are identical to k_tile: same
-// predictor, same colour order and same summation order.
-// NT threads per workgroup, EPB elements per element batch (EPB*25 <= NT).
+// contiguous run of tiles [t0, t1) (runs are grouped per XCD).  No tile waits on a
+// global load issued during that tile: everything a tile reads comes from registers
+// filled while the previous tile was being processed.
+//   * d, v, a of the next tile's patch nodes are issued at the start of the current
+//     tile's element phase;
+//   * the next tile's inverse mass, moduli, geometric factors and special-node list
+//     are issued at the end of the current element phase, before the current tile's
+//     stores (these registers are free then).
+// Vector memory operations retire in issue order (loads and stores share one
+// counter), so a load issued after the stores of a tile could only be consumed once
+// those stores had completed; the order above keeps every wait off the stores.
+// Results are identical to k_tile: same predictor, same colour order and same
+// summation order.  NT threads per workgroup, EPB elements per element batch
+// (EPB*25 <= NT).
 template <typename R, int NDOF, int NELAST, int MODE, int NT, int EPB>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 4 : 2))) void k_tile_pipe(TileDev T, int tiles_per_wg, R* __restrict__ d,
                                                             R* __restrict__ v, R* __restrict__ a,
@@ -233,13 +240,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   constexpr int PB_MAXB = 4 * NBATCH;                               // point tasks per thread
   constexpr int ETHREADS = EPB * NP;
   static_assert(ETHREADS <= NT, "element batch larger than the workgroup");
-  __shared__ R Vp[NDOF][PMAX];  // predicted velocity (kept in LDS to free registers)
+  static_assert(3 * TX * TZ <= NT && MAXSPECIAL <= NT, "per-tile tables larger than the workgroup");
+  __shared__ R Vp[NDOF][PMAX];  // predicted velocity
+  __shared__ R Rm[PMAX];        // inverse mass of the patch nodes
   __shared__ R T1[EPB][NDOF][NP];
   __shared__ R T2[EPB][NDOF][NP];
   __shared__ short sflag[PMAX];
-  // H, GLL weight products and per-element geometric factors live in LDS: every
-  // global (vector) load issued during the element phase would otherwise wait for
-  // the outstanding prefetch of the next tile (vector loads retire in order).
+  // H, GLL weight products and per-element geometric factors live in LDS (no global
+  // load inside the element phase).
   __shared__ double sH[NP];
   __shared__ double sW2[NP];
   __shared__ R Gm[TX * TZ][3];
@@ -262,9 +270,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
     sW2[tid] = c_W2[tid];
   }
 
-  // prefetch registers: d, v, a of this thread's patch nodes of the next tile
+  // prefetch registers of the next tile: node values ...
   R pd[NITER][NDOF], pv[NITER][NDOF], pa[NITER][NDOF];
-  auto issue_loads = [&](int t) {
+  auto issue_nodes = [&](int t) {
     const int4 tg = T.tiles[t];
     const int tx0 = tg.x, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
     const int PX = 4 * nxe + 1, NPT = PX * (4 * nze + 1), X0 = 4 * tx0;
@@ -284,37 +292,32 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
       }
     }
   };
-  issue_loads(t0);
-
-#pragma unroll 1
-  for (int t = t0; t < t1; ++t) {
+  // ... and the late group: inverse mass of the patch nodes, moduli of this thread's
+  // point tasks (compact mode), one geometric factor, one special-node index
+  R pm[NITER];
+  R lm[PB_MAXB][NDOF];
+  R pg = 0;
+  int psp = -1;
+  auto issue_elems = [&](int t) {
     const int4 tg = T.tiles[t];
     const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
-    const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
-    const int X0 = 4 * tx0;
-    const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
-
-    // predictor from the prefetched values
+    {
+      const int PX = 4 * nxe + 1, NPT = PX * (4 * nze + 1), X0 = 4 * tx0;
+      const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
 #pragma unroll
-    for (int r = 0; r < NITER; ++r) {
-      const int p = tid + NT * r;
-      if (p < NPT) {
-#pragma unroll
-        for (int c = 0; c < NDOF; ++c) {
-          Ud[c][p] = pd[r][c] + dt * pv[r][c] + c1 * pa[r][c];
-          Vp[c][p] = pv[r][c] + c2 * pa[r][c];
-          Fn[c][p] = 0;
+      for (int r = 0; r < NITER; ++r) {
+        const int p = tid + NT * r;
+        if (p < NPT) {
+          const int Z = p / PX, X = p - Z * PX;
+          pm[r] = rmass0[node_index(T, X, Z, PX, X0, Z0, kL, kR)];
         }
-        sflag[p] = -1;
       }
     }
-    // moduli of this thread's point tasks (compact mode), before the next prefetch
-    R lm[PB_MAXB][NDOF];
     if constexpr (MODE == COEF_COMPACT) {
       if (tid < nxe * nze * 3) {
         const int e = tid / 3, k = tid - 3 * (tid / 3);
         const int lx = e % nxe, lz = e / nxe;
-        Gm[e][k] = geom[((size_t)(tz0 + lz) * T.nx + tx0 + lx) * 3 + k];
+        pg = geom[((size_t)(tz0 + lz) * T.nx + tx0 + lx) * 3 + k];
       }
 #pragma unroll
       for (int q = 0; q < PB_MAXB; ++q) {
@@ -330,11 +333,43 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
         }
       }
     }
+    if (tid < T.spec_cnt[t]) psp = T.spec_idx[t * MAXSPECIAL + tid];
+  };
+  issue_nodes(t0);
+  issue_elems(t0);
+
+#pragma unroll 1
+  for (int t = t0; t < t1; ++t) {
+    const int4 tg = T.tiles[t];
+    const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
+    const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
+    const int X0 = 4 * tx0;
+    const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
+    (void)tz0;
+
+    // predictor from the prefetched values
+#pragma unroll
+    for (int r = 0; r < NITER; ++r) {
+      const int p = tid + NT * r;
+      if (p < NPT) {
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          Ud[c][p] = pd[r][c] + dt * pv[r][c] + c1 * pa[r][c];
+          Vp[c][p] = pv[r][c] + c2 * pa[r][c];
+          Fn[c][p] = 0;
+        }
+        Rm[p] = pm[r];
+        sflag[p] = -1;
+      }
+    }
+    if constexpr (MODE == COEF_COMPACT) {
+      if (tid < nxe * nze * 3) Gm[tid / 3][tid - 3 * (tid / 3)] = pg;
+    }
     __syncthreads();
-    const int nspec = T.spec_cnt[t];
-    if (tid < nspec) sflag[T.spec_idx[t * MAXSPECIAL + tid]] = (short)tid;
-    // overlap: loads of the next tile fly during this tile's element phase
-    if (t + 1 < t1) issue_loads(t + 1);
+    if (psp >= 0) sflag[psp] = (short)tid;
+    psp = -1;
+    // overlap: node loads of the next tile fly during this tile's element phase
+    if (t + 1 < t1) issue_nodes(t + 1);
 
 #pragma unroll
     for (int q = 0; q < PB_MAXB; ++q) {
@@ -420,6 +455,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
       }
       __syncthreads();
     }
+    // element loads of the next tile, ahead of this tile's stores
+    if (t + 1 < t1) issue_elems(t + 1);
 
     const size_t fbase = (size_t)t * SLOTS;
 #pragma unroll
@@ -434,7 +471,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
           for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = Fn[c][p];
         } else {
           const size_t g = node_index(T, X, Z, PX, X0, Z0, kL, kR);
-          const R rm = rmass0[g];
+          const R rm = Rm[p];
 #pragma unroll
           for (int c = 0; c < NDOF; ++c) {
             const R ak = Fn[c][p] * rm;
