@@ -143,6 +143,13 @@ int sem2d_fetch_fault_state(sem2d_solver *h, double *D, double *V, double *T, do
    MAT_ELAST_f (SRC/mat_elastic.f90:396-426).  d and f are [ndof][npoin] host arrays. */
 int sem2d_compute_fint(sem2d_solver *h, const double *d, double *f);
 
+/* Launch configuration of the structured-grid tile kernel (no counterpart in the
+   reference, whose loop is serial): threads per workgroup (256 = 10-element batches,
+   512 = 20-element batches) and persistent workgroups per CU (0: one workgroup per
+   tile; < 0: automatic, the persistent kernel once there are >= 4 tiles per
+   workgroup).  Results do not depend on it (same summation order). */
+int sem2d_set_tile_config(sem2d_solver *h, int threads, int wg_per_cu);
+
 /* Current step index (number of steps taken). */
 int sem2d_get_step(sem2d_solver *h, int *it);
 
@@ -152,6 +159,53 @@ int sem2d_last_step_ms(sem2d_solver *h, double *ms_per_step);
 
 const char *sem2d_last_error(sem2d_solver *h);
 void sem2d_destroy(sem2d_solver *h);
+
+/* ------------------------------------------------------------------------------
+   Batched ensemble: many independent scenarios advanced together.
+
+   The reference runs one scenario per process (SRC/main.f90:51-99, one time loop per
+   problem_type); a batch replaces N such time loops.  Each member is an ordinary
+   handle, fully set up (sem2d_create, boundaries, sources, receivers, fault) and not
+   yet stepped.  The members must be MESH_CART grids (sem2d_desc.nx > 0) with the same
+   ndof, element kernel, coefficient mode, bulk precision, dt and Newmark parameters;
+   with faults, the same friction law and kind, opening and healing flags and output
+   times.  Mesh sizes, materials, boundaries, sources, receivers and fault parameters
+   may differ.  The batch copies the members' state at creation and then evolves on
+   its own; every member computes exactly what its own sem2d_step would (bitwise).
+   The members must outlive the batch (their node maps are used by the fetches).
+   ------------------------------------------------------------------------------ */
+typedef struct sem2d_batch sem2d_batch;
+
+int sem2d_batch_create(int n, sem2d_solver *const *members, sem2d_batch **out);
+
+/* Advance every member by nsteps.  The loop runs as replays of a captured hipGraph of
+   a chunk of up to 64 steps (the step index is kept in device memory). */
+int sem2d_batch_step(sem2d_batch *b, int nsteps);
+
+/* Autotuner: time trial_steps steps of each tile-kernel launch configuration
+   (threads per workgroup = elements per batch x 25, workgroups per CU, persistent or
+   one workgroup per tile), keep the fastest and restore the state.  *choice receives
+   the index of the chosen configuration. */
+int sem2d_batch_tune(sem2d_batch *b, int trial_steps, int *choice);
+
+/* Set the tile-kernel launch configuration explicitly (as sem2d_set_tile_config;
+   0 <= wg_per_cu <= 4). */
+int sem2d_batch_set_config(sem2d_batch *b, int threads, int wg_per_cu);
+
+/* Current tile-kernel launch configuration and total tile count. */
+int sem2d_batch_config(sem2d_batch *b, int *threads, int *workgroups, int *ntiles);
+
+/* Per-member outputs, in the layouts of the single-scenario fetches. */
+int sem2d_batch_fetch_fields(sem2d_batch *b, int member, double *d, double *v, double *a);
+int sem2d_batch_fetch_seismo(sem2d_batch *b, int member, double *out, int *nsamp_taken);
+int sem2d_batch_fetch_fault(sem2d_batch *b, int member, double *out, int *nsamp_taken);
+int sem2d_batch_fetch_fault_state(sem2d_batch *b, int member, double *D, double *V, double *T,
+                                  double *MU, double *Tstick);
+int sem2d_batch_get_step(sem2d_batch *b, int *it);
+/* ms per step of the last sem2d_batch_step (HIP events on the batch stream) */
+int sem2d_batch_last_step_ms(sem2d_batch *b, double *ms_per_step);
+const char *sem2d_batch_last_error(sem2d_batch *b);
+void sem2d_batch_destroy(sem2d_batch *b);
 
 #ifdef __cplusplus
 }

@@ -905,10 +905,15 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
     oix1 = max(int(oxi[0]), 1)
     oixn = min(int(oxi[1]), npf)
     oixd = int(oxi[2])
-    oitd = max(1, int(round(float(g.get("otd", 0.0)) / dt)))
+    oitd = max(1, _nint(float(g.get("otd", 0.0)) / dt))
     f.update(oix1=oix1, oixn=oixn, oixd=oixd, oitd=oitd, odt=dt * oitd,
-             oit=int(round(float(g.get("ot1", 0.0)) / dt)), osides=bool(g.get("osides", False)))
+             oit=_nint(float(g.get("ot1", 0.0)) / dt), osides=bool(g.get("osides", False)))
     return f
+
+
+def _nint(x):
+    """Fortran NINT: nearest integer, halves rounded away from zero."""
+    return int(math.copysign(math.floor(abs(x) + 0.5), x))
 
 
 def swf_mu(kind, theta, dc, mus, mud, p, alpha):

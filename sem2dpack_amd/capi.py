@@ -48,14 +48,31 @@ _SIGS = {
     "fetch_fault": ([C.c_void_p, dp, ip], C.c_int),
     "fetch_fault_state": ([C.c_void_p, dp, dp, dp, dp, dp], C.c_int),
     "compute_fint": ([C.c_void_p, dp, dp], C.c_int),
+    "set_tile_config": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "get_step": ([C.c_void_p, ip], C.c_int),
     "last_step_ms": ([C.c_void_p, dp], C.c_int),
     "last_error": ([C.c_void_p], C.c_char_p),
     "destroy": ([C.c_void_p], None),
+    # batched ensemble (HIP library only)
+    "batch_create": ([C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)], C.c_int),
+    "batch_step": ([C.c_void_p, C.c_int], C.c_int),
+    "batch_tune": ([C.c_void_p, C.c_int, ip], C.c_int),
+    "batch_set_config": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "batch_config": ([C.c_void_p, ip, ip, ip], C.c_int),
+    "batch_fetch_fields": ([C.c_void_p, C.c_int, dp, dp, dp], C.c_int),
+    "batch_fetch_seismo": ([C.c_void_p, C.c_int, dp, ip], C.c_int),
+    "batch_fetch_fault": ([C.c_void_p, C.c_int, dp, ip], C.c_int),
+    "batch_fetch_fault_state": ([C.c_void_p, C.c_int, dp, dp, dp, dp, dp], C.c_int),
+    "batch_get_step": ([C.c_void_p, ip], C.c_int),
+    "batch_last_step_ms": ([C.c_void_p, dp], C.c_int),
+    "batch_last_error": ([C.c_void_p], C.c_char_p),
+    "batch_destroy": ([C.c_void_p], None),
 }
 
-# entry points every implementation of the ABI must export
-REQUIRED = [k for k in _SIGS if k != "last_step_ms"]
+# entry points every implementation of the ABI must export (the CPU oracle has no
+# event timing, no kernel launch configuration and no batched ensemble)
+GPU_ONLY = ("last_step_ms", "set_tile_config")
+REQUIRED = [k for k in _SIGS if k not in GPU_ONLY and not k.startswith("batch_")]
 
 
 class Api:

@@ -43,7 +43,7 @@ constexpr int FE_THREADS = FE_EPB * NP;
 // Compact flat form: a(i,j,k) = W2(i,j) * modulus(i,j) * geom_e (SRC/mat_elastic.f90:327-358
 // with -weights = -W2*detJ folded into geom_e).
 template <typename R, int NDOF, int NELAST, int MODE>
-__device__ __forceinline__ void load_coefs(const R* __restrict__ coefs, const R* __restrict__ geom, int e, int p,
+__device__ __forceinline__ void load_coefs(const R* __restrict__ coefs, const R* __restrict__ geom, size_t e, int p,
                                            R* A) {
   if constexpr (MODE == COEF_COMPACT) {
     const R w = (R)c_W2[p];

@@ -57,6 +57,7 @@ struct StructState {
   int pipe_wgs = 0;  // > 0: persistent pipelined tile kernel with this many workgroups
   int pipe_threads = 512;
   int4* tiles_dev = nullptr;
+  int4* grids_dev = nullptr;
   int* spec_cnt = nullptr;
   int* spec_idx = nullptr;
   void* fpart = nullptr;
@@ -404,6 +405,9 @@ bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
     int per_cu = 2;
     const char* env = getenv("SEM2D_TILE_WG_PER_CU");
     if (env) per_cu = atoi(env);
+    // measured on MI355X: 256 threads (10-element batches) for FP64 fields, 512
+    // threads (20-element batches) for FP32 fields
+    S.pipe_threads = h->fp32 ? 512 : 256;
     const char* envt = getenv("SEM2D_TILE_THREADS");
     if (envt) S.pipe_threads = atoi(envt) == 256 ? 256 : 512;
     // the persistent kernel pays off once there are several tiles per workgroup
@@ -432,20 +436,34 @@ int prep_upload_bulk(sem2d_solver* h, void** p, const std::vector<double>& src) 
   return 0;
 }
 
-// Deferred-node tables: perimeter and special nodes of every tile, their partial-force
-// slots, sources, absorbing coefficients and inverse masses.
-int prepare_structured(sem2d_solver* h) {
+// Host tables of the structured step of one grid, with local indices (grid node
+// storage from 0, tiles from 0, cells from 0): tiles and their grid descriptors,
+// special nodes, deferred nodes with their partial-force slots, sources, absorbing
+// coefficients and inverse masses, and the split-fault nodes.
+struct StructTables {
+  std::vector<int4> tiles, grids;
+  std::vector<int> spec_cnt, spec_idx;
+  std::vector<DefNode> defs;
+  std::vector<int> part_off;
+  std::vector<SrcEntry> dsrc;
+  std::vector<double> dabso;
+  std::vector<double> rmdef;     // [ndof][ndef]
+  std::vector<int> fault_g;
+  std::vector<double> fault_rm;  // [ndof][nf]
+};
+
+int struct_tables(sem2d_solver* h, StructTables& X) {
   StructState& S = h->st;
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  for (void* q : S.prep_allocs) hipFree(q);
-  S.prep_allocs.clear();
   const int nd = h->ndof;
+  X.tiles = S.tiles;
+  X.grids.assign(S.ntiles, make_int4(0, S.NX1, S.NZ1, 0));
   // special nodes: point-source nodes inside a tile patch
   std::vector<int> src_g;  // grid indices of source nodes
   for (const SrcEntry& e : h->src_host) src_g.push_back(S.gridx[e.node]);
   std::sort(src_g.begin(), src_g.end());
   src_g.erase(std::unique(src_g.begin(), src_g.end()), src_g.end());
-  std::vector<int> spec_cnt(S.ntiles, 0), spec_idx((size_t)S.ntiles * MAXSPECIAL, 0);
+  X.spec_cnt.assign(S.ntiles, 0);
+  X.spec_idx.assign((size_t)S.ntiles * MAXSPECIAL, 0);
   // entries (g, tile, slot)
   struct Ent { int g, t, slot; };
   std::vector<Ent> ents;
@@ -454,18 +472,18 @@ int prepare_structured(sem2d_solver* h) {
     const int4 tg = S.tiles[t];
     const int nxe = tg.z & 0xff, nze = tg.z >> 8;
     const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, X0 = 4 * tg.x, Z0 = tg.w;
-    auto gi = [&](int X, int Z) { return (int)storage_index(S, X0 + X, Z0 + Z); };
-    for (int X = 0; X < PX; ++X) ents.push_back({gi(X, 0), t, X});
-    for (int X = 0; X < PX; ++X) ents.push_back({gi(X, PZ - 1), t, PX + X});
-    for (int Z = 1; Z < PZ - 1; ++Z) ents.push_back({gi(0, Z), t, 2 * PX + Z - 1});
-    for (int Z = 1; Z < PZ - 1; ++Z) ents.push_back({gi(PX - 1, Z), t, 2 * PX + (PZ - 2) + Z - 1});
+    auto gi = [&](int Xn, int Zn) { return (int)storage_index(S, X0 + Xn, Z0 + Zn); };
+    for (int Xn = 0; Xn < PX; ++Xn) ents.push_back({gi(Xn, 0), t, Xn});
+    for (int Xn = 0; Xn < PX; ++Xn) ents.push_back({gi(Xn, PZ - 1), t, PX + Xn});
+    for (int Zn = 1; Zn < PZ - 1; ++Zn) ents.push_back({gi(0, Zn), t, 2 * PX + Zn - 1});
+    for (int Zn = 1; Zn < PZ - 1; ++Zn) ents.push_back({gi(PX - 1, Zn), t, 2 * PX + (PZ - 2) + Zn - 1});
     for (int g : src_g) {
-      const int X = g % S.NX1 - X0, Z = g / S.NX1 - Z0;
-      if (X > 0 && X < PX - 1 && Z > 0 && Z < PZ - 1) {
-        if (spec_cnt[t] >= MAXSPECIAL) return set_err(h, 5, "too many source nodes in one tile");
-        spec_idx[(size_t)t * MAXSPECIAL + spec_cnt[t]] = Z * PX + X;
-        ents.push_back({(int)storage_index(S, X0 + X, Z0 + Z), t, PERIM_MAX + spec_cnt[t]});
-        spec_cnt[t]++;
+      const int Xn = g % S.NX1 - X0, Zn = g / S.NX1 - Z0;
+      if (Xn > 0 && Xn < PX - 1 && Zn > 0 && Zn < PZ - 1) {
+        if (X.spec_cnt[t] >= MAXSPECIAL) return set_err(h, 5, "too many source nodes in one tile");
+        X.spec_idx[(size_t)t * MAXSPECIAL + X.spec_cnt[t]] = Zn * PX + Xn;
+        ents.push_back({(int)storage_index(S, X0 + Xn, Z0 + Zn), t, PERIM_MAX + X.spec_cnt[t]});
+        X.spec_cnt[t]++;
       }
     }
   }
@@ -485,149 +503,215 @@ int prepare_structured(sem2d_solver* h) {
   for (int k : h->fault_n1) is_fault[k] = 1;
   for (int k : h->fault_n2) is_fault[k] = 1;
 
-  std::vector<DefNode> defs;
-  std::vector<int> part_off;
-  std::vector<SrcEntry> dsrc;
-  std::vector<double> dabso;
+  X.defs.clear();
+  X.part_off.clear();
+  X.dsrc.clear();
+  X.dabso.clear();
   std::vector<int> def_ref;
   for (size_t i = 0; i < ents.size();) {
     size_t j = i;
     DefNode n{};
     n.g = ents[i].g;
-    n.pstart = (int)part_off.size();
+    n.pstart = (int)X.part_off.size();
     while (j < ents.size() && ents[j].g == n.g) {
-      part_off.push_back(ents[j].t * SLOTS + ents[j].slot);
+      X.part_off.push_back(ents[j].t * SLOTS + ents[j].slot);
       ++j;
     }
-    n.pcnt = (int)part_off.size() - n.pstart;
+    n.pcnt = (int)X.part_off.size() - n.pstart;
     const int ref = ginv[n.g];
-    n.sstart = (int)dsrc.size();
+    n.sstart = (int)X.dsrc.size();
     auto its = src_of.find(ref);
     if (its != src_of.end())
-      for (int q : its->second) dsrc.push_back(h->src_host[q]);
-    n.scnt = (int)dsrc.size() - n.sstart;
-    n.astart = (int)(dabso.size() / nd);
+      for (int q : its->second) X.dsrc.push_back(h->src_host[q]);
+    n.scnt = (int)X.dsrc.size() - n.sstart;
+    n.astart = (int)(X.dabso.size() / nd);
     auto ita = abso_of.find(ref);
     if (ita != abso_of.end())
       for (auto& bk : ita->second) {
         const AbsoHost& B = h->abso_host[bk.first];
         const size_t nb = B.node.size();
-        for (int c = 0; c < nd; ++c) dabso.push_back(B.C[(size_t)c * nb + bk.second]);
+        for (int c = 0; c < nd; ++c) X.dabso.push_back(B.C[(size_t)c * nb + bk.second]);
       }
-    n.acnt = (int)(dabso.size() / nd) - n.astart;
+    n.acnt = (int)(X.dabso.size() / nd) - n.astart;
     n.fault = is_fault[ref];
-    defs.push_back(n);
+    X.defs.push_back(n);
     def_ref.push_back(ref);
     i = j;
   }
-  S.ndef = (int)defs.size();
-  std::vector<double> rmdef((size_t)nd * S.ndef);
-  for (int q = 0; q < S.ndef; ++q)
-    for (int c = 0; c < nd; ++c) rmdef[(size_t)c * S.ndef + q] = h->rmass_host[(size_t)c * h->npoin + def_ref[q]];
-  int rc;
-  if ((rc = prep_upload(h, &S.tiles_dev, S.tiles.data(), S.tiles.size()))) return rc;
-  if ((rc = prep_upload(h, &S.spec_cnt, spec_cnt.data(), spec_cnt.size()))) return rc;
-  if ((rc = prep_upload(h, &S.spec_idx, spec_idx.data(), spec_idx.size()))) return rc;
-  if ((rc = prep_upload(h, &S.defs, defs.data(), defs.size()))) return rc;
-  if ((rc = prep_upload(h, &S.part_off, part_off.data(), part_off.size()))) return rc;
-  if ((rc = prep_upload(h, &S.dsrc, dsrc.data(), dsrc.size()))) return rc;
-  if ((rc = prep_upload(h, &S.dabso, dabso.data(), dabso.size()))) return rc;
-  if ((rc = prep_upload_bulk(h, &S.rmdef, rmdef))) return rc;
+  const size_t ndef = X.defs.size();
+  X.rmdef.assign((size_t)nd * ndef, 0.0);
+  for (size_t q = 0; q < ndef; ++q)
+    for (int c = 0; c < nd; ++c) X.rmdef[(size_t)c * ndef + q] = h->rmass_host[(size_t)c * h->npoin + def_ref[q]];
+  // split-fault nodes for the post-fault corrector
+  X.fault_g.clear();
+  for (int k : h->fault_n1) X.fault_g.push_back(S.gmap[k]);
+  for (int k : h->fault_n2) X.fault_g.push_back(S.gmap[k]);
+  const size_t nf = X.fault_g.size();
+  X.fault_rm.assign((size_t)nd * nf, 0.0);
+  for (size_t q = 0; q < nf; ++q) {
+    const int ref = q < h->fault_n1.size() ? h->fault_n1[q] : h->fault_n2[q - h->fault_n1.size()];
+    for (int c = 0; c < nd; ++c) X.fault_rm[(size_t)c * nf + q] = h->rmass_host[(size_t)c * h->npoin + ref];
+  }
+  return 0;
+}
+
+// Upload the structured tables of the solver's own grid.
+int prepare_structured(sem2d_solver* h) {
+  StructState& S = h->st;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (void* q : S.prep_allocs) hipFree(q);
+  S.prep_allocs.clear();
+  StructTables X;
+  int rc = struct_tables(h, X);
+  if (rc) return rc;
+  S.ndef = (int)X.defs.size();
+  if ((rc = prep_upload(h, &S.tiles_dev, X.tiles.data(), X.tiles.size()))) return rc;
+  if ((rc = prep_upload(h, &S.grids_dev, X.grids.data(), X.grids.size()))) return rc;
+  if ((rc = prep_upload(h, &S.spec_cnt, X.spec_cnt.data(), X.spec_cnt.size()))) return rc;
+  if ((rc = prep_upload(h, &S.spec_idx, X.spec_idx.data(), X.spec_idx.size()))) return rc;
+  if ((rc = prep_upload(h, &S.defs, X.defs.data(), X.defs.size()))) return rc;
+  if ((rc = prep_upload(h, &S.part_off, X.part_off.data(), X.part_off.size()))) return rc;
+  if ((rc = prep_upload(h, &S.dsrc, X.dsrc.data(), X.dsrc.size()))) return rc;
+  if ((rc = prep_upload(h, &S.dabso, X.dabso.data(), X.dabso.size()))) return rc;
+  if ((rc = prep_upload_bulk(h, &S.rmdef, X.rmdef))) return rc;
   S.FPS = (size_t)S.ntiles * SLOTS;
   void* fp = nullptr;
-  HIPCHK(h, hipMalloc(&fp, std::max<size_t>(S.FPS * nd, 1) * rsize(h)));
+  HIPCHK(h, hipMalloc(&fp, std::max<size_t>(S.FPS * h->ndof, 1) * rsize(h)));
   S.prep_allocs.push_back(fp);
   S.fpart = fp;
-  // split-fault nodes for the post-fault corrector
-  std::vector<int> fg;
-  std::vector<double> frm;
-  for (int k : h->fault_n1) fg.push_back(S.gmap[k]);
-  for (int k : h->fault_n2) fg.push_back(S.gmap[k]);
-  S.nfnodes = (int)fg.size();
-  frm.resize((size_t)nd * S.nfnodes);
-  for (int q = 0; q < S.nfnodes; ++q) {
-    const int ref = q < (int)h->fault_n1.size() ? h->fault_n1[q] : h->fault_n2[q - h->fault_n1.size()];
-    for (int c = 0; c < nd; ++c) frm[(size_t)c * S.nfnodes + q] = h->rmass_host[(size_t)c * h->npoin + ref];
-  }
-  if ((rc = prep_upload(h, &S.fault_g, fg.data(), fg.size()))) return rc;
-  if ((rc = prep_upload_bulk(h, &S.fault_rm, frm))) return rc;
+  S.nfnodes = (int)X.fault_g.size();
+  if ((rc = prep_upload(h, &S.fault_g, X.fault_g.data(), X.fault_g.size()))) return rc;
+  if ((rc = prep_upload_bulk(h, &S.fault_rm, X.fault_rm))) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   S.ready = true;
   return 0;
 }
 
-template <typename R, int NDOF, int NELAST, int MODE>
-void launch_tile_t(sem2d_solver* h, R dt, R c1, R c2, R c3, R c4) {
+// Everything one structured step launches on: the device tables of one grid (a
+// solver) or of many grids sharing the field arrays (a batch).
+struct StepDev {
+  int ndof = 0, nelast = 0, coef_mode = 0;
+  size_t fstride = 0;
+  void *d = nullptr, *v = nullptr, *a = nullptr;
+  TileDev T{};
+  int pipe_wgs = 0, pipe_threads = 512;
+  const void *rmass0 = nullptr, *coefs = nullptr, *geom = nullptr;
+  void* fpart = nullptr;
+  int ndef = 0;
+  const DefNode* defs = nullptr;
+  const int* part_off = nullptr;
+  const SrcEntry* dsrc = nullptr;
+  const double* dabso = nullptr;
+  const void* rmdef = nullptr;
+  const double* ampl = nullptr;
+  int ampl_nt = 0;
+  bool fault_on = false;
+  FaultDev F{};
+  int nfnodes = 0;
+  const int* fault_g = nullptr;
+  const void* fault_rm = nullptr;
+  const int* itp = nullptr;  // device step counter (graph replay) or null: step index by value
+  hipStream_t stream = nullptr;
+};
+
+StepDev step_dev(sem2d_solver* h) {
   StructState& S = h->st;
-  TileDev T;
-  T.NX1 = S.NX1;
-  T.nx = S.nx;
-  T.ntiles = S.ntiles;
-  T.per_xcd = S.per_xcd;
-  T.tiles = S.tiles_dev;
-  T.spec_cnt = S.spec_cnt;
-  T.spec_idx = S.spec_idx;
-  T.NGp = h->fstride;
-  T.NZ1 = S.NZ1;
-  T.nvl = S.nvl;
-  T.FPS = S.FPS;
-  if (S.pipe_wgs > 0) {
+  StepDev X;
+  X.ndof = h->ndof;
+  X.nelast = h->nelast;
+  X.coef_mode = h->coef_mode;
+  X.fstride = h->fstride;
+  X.d = h->d;
+  X.v = h->v;
+  X.a = h->a;
+  X.T.ntiles = S.ntiles;
+  X.T.per_xcd = S.per_xcd;
+  X.T.tiles = S.tiles_dev;
+  X.T.grids = S.grids_dev;
+  X.T.spec_cnt = S.spec_cnt;
+  X.T.spec_idx = S.spec_idx;
+  X.T.NGp = h->fstride;
+  X.T.FPS = S.FPS;
+  X.pipe_wgs = S.pipe_wgs;
+  X.pipe_threads = S.pipe_threads;
+  X.rmass0 = S.rmass0;
+  X.coefs = S.coefs;
+  X.geom = S.geom;
+  X.fpart = S.fpart;
+  X.ndef = S.ndef;
+  X.defs = S.defs;
+  X.part_off = S.part_off;
+  X.dsrc = S.dsrc;
+  X.dabso = S.dabso;
+  X.rmdef = S.rmdef;
+  X.ampl = h->ampl_dev;
+  X.ampl_nt = h->ampl_nt;
+  X.fault_on = h->flt.on;
+  if (h->flt.on) X.F = h->flt.dev;
+  X.nfnodes = S.nfnodes;
+  X.fault_g = S.fault_g;
+  X.fault_rm = S.fault_rm;
+  X.stream = h->stream;
+  return X;
+}
+
+template <typename R, int NDOF, int NELAST, int MODE>
+void launch_tile_t(const StepDev& X, R dt, R c1, R c2, R c3, R c4) {
+  if (X.pipe_wgs > 0) {
     // persistent pipelined kernel: pipe_wgs workgroups, each on a run of tiles
-    const int per = (S.ntiles + S.pipe_wgs - 1) / S.pipe_wgs;
-    if (S.pipe_threads == 256)
-      k_tile_pipe<R, NDOF, NELAST, MODE, 256, 10><<<S.pipe_wgs, 256, 0, h->stream>>>(
-          T, per, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom,
-          (R*)S.fpart, dt, c1, c2, c3, c4);
+    const int per = (X.T.ntiles + X.pipe_wgs - 1) / X.pipe_wgs;
+    if (X.pipe_threads == 256)
+      k_tile_pipe<R, NDOF, NELAST, MODE, 256, 10><<<X.pipe_wgs, 256, 0, X.stream>>>(
+          X.T, per, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
+          (R*)X.fpart, dt, c1, c2, c3, c4);
     else
-      k_tile_pipe<R, NDOF, NELAST, MODE, 512, 20><<<S.pipe_wgs, 512, 0, h->stream>>>(
-          T, per, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom,
-          (R*)S.fpart, dt, c1, c2, c3, c4);
+      k_tile_pipe<R, NDOF, NELAST, MODE, 512, 20><<<X.pipe_wgs, 512, 0, X.stream>>>(
+          X.T, per, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
+          (R*)X.fpart, dt, c1, c2, c3, c4);
   } else {
-    k_tile<R, NDOF, NELAST, MODE><<<8 * S.per_xcd, TILE_THREADS, 0, h->stream>>>(
-        T, (R*)h->d, (R*)h->v, (R*)h->a, (const R*)S.rmass0, (const R*)S.coefs, (const R*)S.geom, (R*)S.fpart,
+    k_tile<R, NDOF, NELAST, MODE><<<8 * X.T.per_xcd, TILE_THREADS, 0, X.stream>>>(
+        X.T, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom, (R*)X.fpart,
         dt, c1, c2, c3, c4);
   }
 }
 
 template <typename R, int NDOF, int NELAST>
-void launch_tile_m(sem2d_solver* h, R dt, R c1, R c2, R c3, R c4) {
-  switch (h->coef_mode) {
-    case COEF_SHARED: launch_tile_t<R, NDOF, NELAST, COEF_SHARED>(h, dt, c1, c2, c3, c4); break;
+void launch_tile_m(const StepDev& X, R dt, R c1, R c2, R c3, R c4) {
+  switch (X.coef_mode) {
+    case COEF_SHARED: launch_tile_t<R, NDOF, NELAST, COEF_SHARED>(X, dt, c1, c2, c3, c4); break;
     case COEF_COMPACT:
-      if constexpr (NELAST == 2 || NELAST == 6) launch_tile_t<R, NDOF, NELAST, COEF_COMPACT>(h, dt, c1, c2, c3, c4);
+      if constexpr (NELAST == 2 || NELAST == 6) launch_tile_t<R, NDOF, NELAST, COEF_COMPACT>(X, dt, c1, c2, c3, c4);
       break;
-    default: launch_tile_t<R, NDOF, NELAST, COEF_PER_ELEM>(h, dt, c1, c2, c3, c4);
+    default: launch_tile_t<R, NDOF, NELAST, COEF_PER_ELEM>(X, dt, c1, c2, c3, c4);
   }
 }
 
 template <typename R>
-void launch_structured_step(sem2d_solver* h, int it, R dt, R c1, R c2, R c3, R c4) {
-  StructState& S = h->st;
-  if (h->ndof == 2) {
-    if (h->nelast == 6) launch_tile_m<R, 2, 6>(h, dt, c1, c2, c3, c4);
-    else launch_tile_m<R, 2, 10>(h, dt, c1, c2, c3, c4);
+void launch_structured_step(const StepDev& X, int it, R dt, R c1, R c2, R c3, R c4) {
+  if (X.ndof == 2) {
+    if (X.nelast == 6) launch_tile_m<R, 2, 6>(X, dt, c1, c2, c3, c4);
+    else launch_tile_m<R, 2, 10>(X, dt, c1, c2, c3, c4);
   } else {
-    if (h->nelast == 2) launch_tile_m<R, 1, 2>(h, dt, c1, c2, c3, c4);
-    else launch_tile_m<R, 1, 3>(h, dt, c1, c2, c3, c4);
+    if (X.nelast == 2) launch_tile_m<R, 1, 2>(X, dt, c1, c2, c3, c4);
+    else launch_tile_m<R, 1, 3>(X, dt, c1, c2, c3, c4);
   }
-  const int gd = (S.ndef + 255) / 256;
-  if (h->ndof == 2)
-    k_deferred<R, 2><<<gd, 256, 0, h->stream>>>(S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dsrc,
-                                                 h->ampl_dev, h->ampl_nt, it, S.dabso, (const R*)S.rmdef, (R*)h->d,
-                                                 (R*)h->v, (R*)h->a, h->fstride, dt, c1, c2, c3, c4);
+  const int gd = (X.ndef + 255) / 256;
+  if (X.ndof == 2)
+    k_deferred<R, 2><<<gd, 256, 0, X.stream>>>(X.ndef, X.defs, X.part_off, (const R*)X.fpart, X.T.FPS, X.dsrc,
+                                                X.ampl, X.ampl_nt, it, X.itp, X.dabso, (const R*)X.rmdef, (R*)X.d,
+                                                (R*)X.v, (R*)X.a, X.fstride, dt, c1, c2, c3, c4);
   else
-    k_deferred<R, 1><<<gd, 256, 0, h->stream>>>(S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dsrc,
-                                                 h->ampl_dev, h->ampl_nt, it, S.dabso, (const R*)S.rmdef, (R*)h->d,
-                                                 (R*)h->v, (R*)h->a, h->fstride, dt, c1, c2, c3, c4);
-  if (h->flt.on) {
-    k_fault<R><<<grid_for(h->flt.dev.np, 64), 64, 0, h->stream>>>(h->flt.dev, (const R*)h->d, (const R*)h->v,
-                                                                   (R*)h->a, h->fstride);
-    if (h->ndof == 2)
-      k_fault_correct<R, 2><<<grid_for(S.nfnodes, 64), 64, 0, h->stream>>>(
-          S.nfnodes, S.fault_g, (const R*)S.fault_rm, (R*)h->d, (R*)h->v, (R*)h->a, h->fstride, c3, c4);
+    k_deferred<R, 1><<<gd, 256, 0, X.stream>>>(X.ndef, X.defs, X.part_off, (const R*)X.fpart, X.T.FPS, X.dsrc,
+                                                X.ampl, X.ampl_nt, it, X.itp, X.dabso, (const R*)X.rmdef, (R*)X.d,
+                                                (R*)X.v, (R*)X.a, X.fstride, dt, c1, c2, c3, c4);
+  if (X.fault_on) {
+    k_fault<R><<<grid_for(X.F.np, 64), 64, 0, X.stream>>>(X.F, (const R*)X.d, (const R*)X.v, (R*)X.a, X.fstride);
+    if (X.ndof == 2)
+      k_fault_correct<R, 2><<<grid_for(X.nfnodes, 64), 64, 0, X.stream>>>(
+          X.nfnodes, X.fault_g, (const R*)X.fault_rm, (R*)X.d, (R*)X.v, (R*)X.a, X.fstride, c3, c4);
     else
-      k_fault_correct<R, 1><<<grid_for(S.nfnodes, 64), 64, 0, h->stream>>>(
-          S.nfnodes, S.fault_g, (const R*)S.fault_rm, (R*)h->d, (R*)h->v, (R*)h->a, h->fstride, c3, c4);
+      k_fault_correct<R, 1><<<grid_for(X.nfnodes, 64), 64, 0, X.stream>>>(
+          X.nfnodes, X.fault_g, (const R*)X.fault_rm, (R*)X.d, (R*)X.v, (R*)X.a, X.fstride, c3, c4);
   }
 }
 
@@ -644,11 +728,13 @@ int do_steps(sem2d_solver* h, int nsteps) {
   const R c3 = (R)(h->gamma * h->dt), c4 = (R)(h->beta * h->dt * h->dt);
   const int gb = grid_for(n, 256);
   if (h->st.on && !h->st.ready && (rc = prepare_structured(h))) return rc;
+  StepDev X;
+  if (h->st.on) X = step_dev(h);
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   for (int s = 0; s < nsteps; ++s) {
     const int it = h->it + 1;
     if (h->st.on) {
-      launch_structured_step<R>(h, it, dt, c1, c2, c3, c4);
+      launch_structured_step<R>(X, it, dt, c1, c2, c3, c4);
       h->it = it;
       record_receivers<R>(h);
       record_fault(h);
@@ -1128,6 +1214,21 @@ int sem2d_compute_fint(sem2d_solver* h, const double* d, double* f) {
   return rc;
 }
 
+int sem2d_set_tile_config(sem2d_solver* h, int threads, int wg_per_cu) {
+  if (!h) return 1;
+  if (!h->st.on) return set_err(h, 4, "set_tile_config: not a structured (MESH_CART) grid");
+  if ((threads != 256 && threads != 512) || wg_per_cu > 4)
+    return set_err(h, 3, "set_tile_config: threads must be 256 or 512 and wg_per_cu <= 4");
+  StructState& S = h->st;
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  S.pipe_threads = threads;
+  if (wg_per_cu < 0) S.pipe_wgs = S.ntiles >= 4 * ncu * 2 ? ncu * 2 : 0;  // automatic
+  else S.pipe_wgs = ncu * wg_per_cu;
+  S.pipe_wgs = ((S.pipe_wgs + 7) / 8) * 8;
+  return 0;
+}
+
 int sem2d_get_step(sem2d_solver* h, int* it) {
   if (!h || !it) return 1;
   *it = h->it;
@@ -1154,6 +1255,727 @@ void sem2d_destroy(sem2d_solver* h) {
   if (h->ev1) hipEventDestroy(h->ev1);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched ensemble: many independent structured scenarios advanced together.
+//
+// The member solvers' grids are laid side by side in one set of field arrays
+// (member m's storage block starts at node_base[m]; component stride = the sum of
+// the members' strides).  Tiles, deferred nodes, sources, absorbing coefficients
+// and fault pairs of all members are concatenated with their indices offset, so
+// one persistent tile launch and one deferred launch cover every scenario.  The
+// step index lives in device memory (it_dev), which lets a captured hipGraph of a
+// chunk of steps be replayed for the whole time loop.  Every member performs
+// exactly the arithmetic of its own solver, so batch and one-by-one results are
+// bitwise identical.
+// ===========================================================================
+namespace {
+
+template <typename R>
+__global__ void k_batch_record(const int* __restrict__ itp, int nrec, int ndof, const int* __restrict__ rec_node,
+                               const R* __restrict__ field, size_t fstride, int isamp, int nsamp,
+                               double* __restrict__ seis, FaultDev F, const int* __restrict__ fidx, int nout,
+                               int oit0, int oitd, int fnsamp, double* __restrict__ frec) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int it = *itp;
+  if (q < nrec && it % isamp == 0 && it / isamp < nsamp) {
+    double* out = seis + (size_t)(it / isamp) * ndof * nrec;
+    for (int c = 0; c < ndof; ++c) out[(size_t)c * nrec + q] = (double)field[(size_t)c * fstride + rec_node[q]];
+  }
+  if (q < nout && it >= oit0 && (it - oit0) % oitd == 0 && (it - oit0) / oitd < fnsamp) {
+    double* out = frec + (size_t)((it - oit0) / oitd) * 6 * nout;
+    const int k = fidx[q];
+    out[0 * nout + q] = F.D[k];
+    out[1 * nout + q] = F.V[k];
+    out[2 * nout + q] = F.T[k];
+    out[3 * nout + q] = F.T[F.np + k];
+    out[4 * nout + q] = F.MU[k];
+    out[5 * nout + q] = F.Tstick[k];
+  }
+}
+
+__global__ void k_tick(int* itp) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *itp = *itp + 1;
+}
+
+}  // namespace
+
+struct sem2d_batch {
+  int n = 0;
+  std::vector<sem2d_solver*> mem;
+  int ndof = 0, nelast = 0, coef_mode = 0, fp32 = 0;
+  double dt = 0, beta = 0, gamma = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  size_t fstride = 0;
+  std::vector<size_t> node_base;
+  std::vector<int> fault_base, rec_base, fout_base, src_base;
+  StepDev X;
+  int ncu = 256;
+  // records
+  int nrec = 0, isamp = 1, field = 1, nsamp = 0;
+  int* rec_node = nullptr;
+  double* seis = nullptr;
+  int fnout = 0, oit0 = 0, oitd = 1, fnsamp = 0;
+  int* fidx = nullptr;
+  double* frec = nullptr;
+  int* it_dev = nullptr;
+  int* flags = nullptr;
+  int it = 0;
+  // graph of a chunk of steps
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int gsteps = 0;
+  double last_ms = 0.0;
+  std::string err;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int berr(sem2d_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  return code;
+}
+
+#define BHIPCHK(b, call)                                                                  \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return berr(b, 10, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+template <typename T>
+int balloc(sem2d_batch* b, T** p, size_t n) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) return berr(b, 11, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+  b->allocs.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+template <typename T>
+int bupload(sem2d_batch* b, T** p, const std::vector<T>& src) {
+  int rc = balloc(b, p, src.size());
+  if (rc) return rc;
+  if (!src.empty())
+    BHIPCHK(b, hipMemcpyAsync(*p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, b->stream));
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int bupload_bulk(sem2d_batch* b, void** p, const std::vector<double>& src) {
+  if (!b->fp32) return bupload(b, (double**)p, src);
+  std::vector<float> tmp(src.begin(), src.end());
+  return bupload(b, (float**)p, tmp);
+}
+
+// device -> host copy of n doubles
+int bdown(sem2d_batch* b, std::vector<double>& dst, const double* src, size_t n) {
+  dst.resize(n);
+  if (n) BHIPCHK(b, hipMemcpy(dst.data(), src, n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int check_members(sem2d_batch* b) {
+  sem2d_solver* h0 = b->mem[0];
+  for (int m = 0; m < b->n; ++m) {
+    sem2d_solver* h = b->mem[m];
+    if (!h) return berr(b, 3, "member %d is null", m);
+    if (!h->st.on) return berr(b, 3, "member %d is not a structured (MESH_CART) grid", m);
+    if (h->it != 0) return berr(b, 3, "member %d has already been stepped", m);
+    if (h->ndof != h0->ndof || h->nelast != h0->nelast || h->coef_mode != h0->coef_mode || h->fp32 != h0->fp32)
+      return berr(b, 3, "member %d: ndof, element kernel, coefficient mode and bulk precision must match", m);
+    if (h->dt != h0->dt || h->beta != h0->beta || h->gamma != h0->gamma)
+      return berr(b, 3, "member %d: time step and Newmark parameters must match", m);
+    if (h->flt.on != h0->flt.on) return berr(b, 3, "member %d: either every member has a fault or none", m);
+    if (h->flt.on) {
+      const FaultDev &F = h->flt.dev, &F0 = h0->flt.dev;
+      if (F.law != F0.law || F.kind != F0.kind || F.opening != F0.opening || F.healing != F0.healing ||
+          F.a2v != F0.a2v || F.a2d != F0.a2d)
+        return berr(b, 3, "member %d: fault law, kind, opening, healing and time coefficients must match", m);
+      const int oit0 = h->flt.oit - h->flt.taken * h->flt.oitd;
+      const int oit00 = h0->flt.oit - h0->flt.taken * h0->flt.oitd;
+      if (h->flt.oitd != h0->flt.oitd || oit0 != oit00)
+        return berr(b, 3, "member %d: fault output times must match", m);
+    }
+  }
+  return 0;
+}
+
+// concatenate the device-resident per-member arrays that only exist on the device
+int gather_fault(sem2d_batch* b) {
+  const int nd = b->ndof;
+  int np = 0;
+  b->fault_base.assign(b->n, 0);
+  for (int m = 0; m < b->n; ++m) {
+    b->fault_base[m] = np;
+    np += b->mem[m]->flt.dev.np;
+  }
+  FaultDev F = b->mem[0]->flt.dev;
+  F.np = np;
+  std::vector<int> n1, n2;
+  std::vector<double> cat[22];
+  // arrays and their row counts ([rows][np] layout)
+  struct Arr { const double* FaultDev::*p; int rows; };
+  const Arr arrs[] = {{&FaultDev::nrm, 2}, {&FaultDev::B, nd},    {&FaultDev::invM1, nd}, {&FaultDev::invM2, nd},
+                      {&FaultDev::Z, nd},  {&FaultDev::T0, 2},    {&FaultDev::coh, 1},    {&FaultDev::p0, 1},
+                      {&FaultDev::p1, 1},  {&FaultDev::p2, 1},    {&FaultDev::p3, 1},     {&FaultDev::p4, 1},
+                      {&FaultDev::p5, 1},  {&FaultDev::Tc, 1},    {&FaultDev::coeft, 1}};
+  struct State { double* FaultDev::*p; int rows; };
+  const State states[] = {{&FaultDev::theta, 1}, {&FaultDev::D, nd},     {&FaultDev::V, nd}, {&FaultDev::T, 2},
+                          {&FaultDev::Tstick, 2}, {&FaultDev::MU, 1},    {&FaultDev::sigma, 1}};
+  const int na = sizeof(arrs) / sizeof(arrs[0]), ns = sizeof(states) / sizeof(states[0]);
+  for (int q = 0; q < na; ++q) cat[q].assign((size_t)arrs[q].rows * np, 0.0);
+  for (int q = 0; q < ns; ++q) cat[na + q].assign((size_t)states[q].rows * np, 0.0);
+  for (int m = 0; m < b->n; ++m) {
+    const FaultDev& Fm = b->mem[m]->flt.dev;
+    const int nm = Fm.np, o = b->fault_base[m];
+    std::vector<int> t1(nm), t2(nm);
+    BHIPCHK(b, hipMemcpy(t1.data(), Fm.n1, nm * sizeof(int), hipMemcpyDeviceToHost));
+    BHIPCHK(b, hipMemcpy(t2.data(), Fm.n2, nm * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nm; ++k) {
+      n1.push_back((int)(t1[k] + b->node_base[m]));
+      n2.push_back((int)(t2[k] + b->node_base[m]));
+    }
+    std::vector<double> tmp;
+    for (int q = 0; q < na + ns; ++q) {
+      const int rows = q < na ? arrs[q].rows : states[q - na].rows;
+      const double* src = q < na ? Fm.*(arrs[q].p) : Fm.*(states[q - na].p);
+      int rc = bdown(b, tmp, src, (size_t)rows * nm);
+      if (rc) return rc;
+      for (int r = 0; r < rows; ++r)
+        std::copy(tmp.begin() + (size_t)r * nm, tmp.begin() + (size_t)(r + 1) * nm, cat[q].begin() + (size_t)r * np + o);
+    }
+  }
+  int rc;
+  int* ip;
+  if ((rc = bupload(b, &ip, n1))) return rc;
+  F.n1 = ip;
+  if ((rc = bupload(b, &ip, n2))) return rc;
+  F.n2 = ip;
+  for (int q = 0; q < na + ns; ++q) {
+    double* dp;
+    if ((rc = bupload(b, &dp, cat[q]))) return rc;
+    if (q < na) F.*(arrs[q].p) = dp;
+    else F.*(states[q - na].p) = dp;
+  }
+  F.nr_fail = b->flags;
+  b->X.fault_on = true;
+  b->X.F = F;
+  // fault records: output nodes oix1:oixn:oixd of every member, member after member
+  std::vector<int> fidx;
+  b->fout_base.assign(b->n, 0);
+  b->fnsamp = 0;
+  for (int m = 0; m < b->n; ++m) {
+    const FaultHost& H = b->mem[m]->flt;
+    b->fout_base[m] = (int)fidx.size();
+    for (int o = 0; o < H.nout; ++o) fidx.push_back(b->fault_base[m] + H.oix1 - 1 + o * H.oixd);
+    b->fnsamp = std::max(b->fnsamp, H.nsamp);
+  }
+  b->fnout = (int)fidx.size();
+  const FaultHost& H0 = b->mem[0]->flt;
+  b->oitd = H0.oitd;
+  b->oit0 = H0.oit - H0.taken * H0.oitd;
+  if ((rc = bupload(b, &b->fidx, fidx))) return rc;
+  if ((rc = balloc(b, &b->frec, (size_t)b->fnsamp * 6 * std::max(b->fnout, 1)))) return rc;
+  return 0;
+}
+
+template <typename R>
+void launch_batch_record(sem2d_batch* b) {
+  const int nq = std::max(std::max(b->nrec, b->fnout), 1);
+  const R* fld = (const R*)(b->field == 0 ? b->X.d : (b->field == 1 ? b->X.v : b->X.a));
+  k_batch_record<R><<<grid_for(nq, 64), 64, 0, b->stream>>>(b->it_dev, b->nrec, b->ndof, b->rec_node, fld,
+                                                            b->fstride, b->isamp, b->nsamp, b->seis, b->X.F,
+                                                            b->fidx, b->fnout, b->oit0, b->oitd, b->fnsamp,
+                                                            b->frec);
+}
+
+template <typename R>
+void launch_batch_steps(sem2d_batch* b, int nsteps) {
+  const R dt = (R)b->dt;
+  const R c1 = (R)((0.5 - b->beta) * b->dt * b->dt), c2 = (R)((1.0 - b->gamma) * b->dt);
+  const R c3 = (R)(b->gamma * b->dt), c4 = (R)(b->beta * b->dt * b->dt);
+  for (int s = 0; s < nsteps; ++s) {
+    launch_structured_step<R>(b->X, 0, dt, c1, c2, c3, c4);
+    launch_batch_record<R>(b);
+    k_tick<<<1, 64, 0, b->stream>>>(b->it_dev);
+  }
+}
+
+void batch_drop_graph(sem2d_batch* b) {
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->gsteps = 0;
+}
+
+int batch_graph(sem2d_batch* b, int chunk) {
+  if (b->gexec && b->gsteps == chunk) return 0;
+  batch_drop_graph(b);
+  BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  if (b->fp32) launch_batch_steps<float>(b, chunk);
+  else launch_batch_steps<double>(b, chunk);
+  BHIPCHK(b, hipStreamEndCapture(b->stream, &b->graph));
+  BHIPCHK(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  b->gsteps = chunk;
+  return 0;
+}
+
+// launch configuration of the tile kernel: threads per workgroup (256: 10-element
+// batches, 512: 20-element batches) and workgroups per CU (0: one workgroup per tile)
+void batch_set_config(sem2d_batch* b, int threads, int per_cu) {
+  b->X.pipe_threads = threads;
+  b->X.pipe_wgs = per_cu > 0 ? ((b->ncu * per_cu + 7) / 8) * 8 : 0;
+  batch_drop_graph(b);
+}
+
+int batch_run(sem2d_batch* b, int nsteps, bool use_graph) {
+  const int chunk = std::min(nsteps, 64);
+  BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
+  int done = 0;
+  if (use_graph && chunk > 0) {
+    int rc = batch_graph(b, chunk);
+    if (rc) return rc;
+    for (; done + chunk <= nsteps; done += chunk) BHIPCHK(b, hipGraphLaunch(b->gexec, b->stream));
+  }
+  if (done < nsteps) {
+    if (b->fp32) launch_batch_steps<float>(b, nsteps - done);
+    else launch_batch_steps<double>(b, nsteps - done);
+  }
+  BHIPCHK(b, hipEventRecord(b->ev1, b->stream));
+  BHIPCHK(b, hipGetLastError());
+  BHIPCHK(b, hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  BHIPCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = nsteps > 0 ? ms / nsteps : 0.0;
+  b->it += nsteps;
+  if (b->X.fault_on) {
+    int flag = 0;
+    BHIPCHK(b, hipMemcpy(&flag, b->flags, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) return berr(b, 20, "NR_Solver has exceeded the maximum iterations (200)");
+  }
+  return 0;
+}
+
+int rec_taken(const sem2d_batch* b) { return b->nrec ? std::min(b->nsamp, b->it / b->isamp + 1) : 0; }
+int flt_taken(const sem2d_batch* b) {
+  if (!b->X.fault_on || b->it < b->oit0) return 0;
+  return std::min(b->fnsamp, (b->it - b->oit0) / b->oitd + 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
+  if (!out) return 1;
+  *out = nullptr;
+  if (n <= 0 || !members) return 2;
+  sem2d_batch* b = new sem2d_batch();
+  *out = b;
+  b->n = n;
+  b->mem.assign(members, members + n);
+  int rc = check_members(b);
+  if (rc) return rc;
+  sem2d_solver* h0 = b->mem[0];
+  b->ndof = h0->ndof;
+  b->nelast = h0->nelast;
+  b->coef_mode = h0->coef_mode;
+  b->fp32 = h0->fp32;
+  b->dt = h0->dt;
+  b->beta = h0->beta;
+  b->gamma = h0->gamma;
+  const int nd = b->ndof;
+  const size_t rs = b->fp32 ? sizeof(float) : sizeof(double);
+  BHIPCHK(b, hipStreamCreate(&b->stream));
+  BHIPCHK(b, hipEventCreate(&b->ev0));
+  BHIPCHK(b, hipEventCreate(&b->ev1));
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  }
+  // member tables, concatenated
+  StructTables A;
+  std::vector<double> rmdef_c[2], frm_c[2];
+  std::vector<SrcEntry> dsrc;
+  std::vector<double> ampl;
+  int ampl_nt = 0, nsrc = 0;
+  for (int m = 0; m < n; ++m) {
+    sem2d_solver* h = b->mem[m];
+    if ((rc = sync_sources(h))) return berr(b, rc, "member %d: %s", m, h->err.c_str());
+    ampl_nt = std::max(ampl_nt, h->ampl_nt);
+  }
+  ampl_nt = std::max(ampl_nt, 1);  // every source owns a row of the table
+  b->node_base.assign(n, 0);
+  b->src_base.assign(n, 0);
+  size_t nb = 0, cb = 0, shared_off = 0;
+  for (int m = 0; m < n; ++m) {
+    sem2d_solver* h = b->mem[m];
+    StructTables X;
+    if ((rc = struct_tables(h, X))) return berr(b, rc, "member %d: %s", m, h->err.c_str());
+    b->node_base[m] = nb;
+    const int tb = (int)A.tiles.size(), pb = (int)A.part_off.size(), sb = (int)A.dsrc.size();
+    const int ab = (int)(A.dabso.size() / nd);
+    const int cellw = b->coef_mode == COEF_SHARED ? (int)shared_off : (int)cb;
+    if (nb + h->st.NS > 0x7fffffffULL || cb + h->nelem > 0x7fffffffULL) return berr(b, 5, "batch too large");
+    for (size_t t = 0; t < X.tiles.size(); ++t) {
+      A.tiles.push_back(X.tiles[t]);
+      A.grids.push_back(make_int4((int)nb, X.grids[t].y, X.grids[t].z, cellw));
+    }
+    A.spec_cnt.insert(A.spec_cnt.end(), X.spec_cnt.begin(), X.spec_cnt.end());
+    A.spec_idx.insert(A.spec_idx.end(), X.spec_idx.begin(), X.spec_idx.end());
+    for (DefNode d : X.defs) {
+      d.g += (int)nb;
+      d.pstart += pb;
+      d.sstart += sb;
+      d.astart += ab;
+      A.defs.push_back(d);
+    }
+    for (int po : X.part_off) A.part_off.push_back(po + tb * SLOTS);
+    for (SrcEntry e : X.dsrc) {
+      e.src += nsrc;
+      A.dsrc.push_back(e);
+    }
+    A.dabso.insert(A.dabso.end(), X.dabso.begin(), X.dabso.end());
+    const size_t ndef = X.defs.size(), nf = X.fault_g.size();
+    for (int c = 0; c < nd; ++c) {
+      rmdef_c[c].insert(rmdef_c[c].end(), X.rmdef.begin() + c * ndef, X.rmdef.begin() + (c + 1) * ndef);
+      frm_c[c].insert(frm_c[c].end(), X.fault_rm.begin() + c * nf, X.fault_rm.begin() + (c + 1) * nf);
+    }
+    for (int g : X.fault_g) A.fault_g.push_back(g + (int)nb);
+    // source amplitude rows, padded to the common length
+    b->src_base[m] = nsrc;
+    for (int s = 0; s < h->nsrc; ++s) {
+      std::vector<double> row(ampl_nt, 0.0);
+      std::copy(h->stf[s].begin(), h->stf[s].end(), row.begin());
+      ampl.insert(ampl.end(), row.begin(), row.end());
+    }
+    nsrc += h->nsrc;
+    nb += h->st.NS;
+    cb += h->nelem;
+    shared_off += (size_t)h->nelast * NP;
+  }
+  b->fstride = nb;
+  for (int c = 0; c < nd; ++c) {
+    A.rmdef.insert(A.rmdef.end(), rmdef_c[c].begin(), rmdef_c[c].end());
+    A.fault_rm.insert(A.fault_rm.end(), frm_c[c].begin(), frm_c[c].end());
+  }
+  StepDev& X = b->X;
+  X.ndof = nd;
+  X.nelast = b->nelast;
+  X.coef_mode = b->coef_mode;
+  X.fstride = nb;
+  X.stream = b->stream;
+  const int ntiles = (int)A.tiles.size();
+  int4 *tiles, *grids;
+  int *sc, *si, *po;
+  DefNode* defs;
+  SrcEntry* ds;
+  double *dab, *amp;
+  int* fg;
+  if ((rc = bupload(b, &tiles, A.tiles))) return rc;
+  if ((rc = bupload(b, &grids, A.grids))) return rc;
+  if ((rc = bupload(b, &sc, A.spec_cnt))) return rc;
+  if ((rc = bupload(b, &si, A.spec_idx))) return rc;
+  if ((rc = bupload(b, &defs, A.defs))) return rc;
+  if ((rc = bupload(b, &po, A.part_off))) return rc;
+  if ((rc = bupload(b, &ds, A.dsrc))) return rc;
+  if ((rc = bupload(b, &dab, A.dabso))) return rc;
+  if (ampl.empty()) ampl.assign(1, 0.0);
+  if ((rc = bupload(b, &amp, ampl))) return rc;
+  if ((rc = bupload(b, &fg, A.fault_g))) return rc;
+  void *rmdef, *frm;
+  if ((rc = bupload_bulk(b, &rmdef, A.rmdef))) return rc;
+  if ((rc = bupload_bulk(b, &frm, A.fault_rm))) return rc;
+  X.T.ntiles = ntiles;
+  X.T.per_xcd = (ntiles + 7) / 8;
+  X.T.tiles = tiles;
+  X.T.grids = grids;
+  X.T.spec_cnt = sc;
+  X.T.spec_idx = si;
+  X.T.NGp = nb;
+  X.T.FPS = (size_t)ntiles * SLOTS;
+  X.ndef = (int)A.defs.size();
+  X.defs = defs;
+  X.part_off = po;
+  X.dsrc = ds;
+  X.dabso = dab;
+  X.rmdef = rmdef;
+  X.ampl = amp;
+  X.ampl_nt = ampl_nt;
+  X.nfnodes = (int)A.fault_g.size();
+  X.fault_g = fg;
+  X.fault_rm = frm;
+  // bulk arrays: fields, inverse mass, coefficients, partial forces
+  char *d, *v, *a, *rm0, *co, *ge = nullptr, *fp;
+  if ((rc = balloc(b, &d, nd * nb * rs))) return rc;
+  if ((rc = balloc(b, &v, nd * nb * rs))) return rc;
+  if ((rc = balloc(b, &a, nd * nb * rs))) return rc;
+  if ((rc = balloc(b, &rm0, nb * rs))) return rc;
+  if ((rc = balloc(b, &fp, (size_t)nd * X.T.FPS * rs))) return rc;
+  const size_t co_per = b->coef_mode == COEF_SHARED ? 0
+                        : (b->coef_mode == COEF_COMPACT ? (size_t)nd * NP : (size_t)b->nelast * NP);
+  const size_t co_n = b->coef_mode == COEF_SHARED ? shared_off : cb * co_per;
+  if ((rc = balloc(b, &co, co_n * rs))) return rc;
+  if (b->coef_mode == COEF_COMPACT && (rc = balloc(b, &ge, cb * 3 * rs))) return rc;
+  size_t cofs = 0, gofs = 0;
+  for (int m = 0; m < n; ++m) {
+    sem2d_solver* h = b->mem[m];
+    const size_t NS = h->st.NS, o = b->node_base[m];
+    for (int c = 0; c < nd; ++c) {
+      BHIPCHK(b, hipMemcpyAsync(d + (c * nb + o) * rs, (char*)h->d + c * NS * rs, NS * rs, hipMemcpyDeviceToDevice, b->stream));
+      BHIPCHK(b, hipMemcpyAsync(v + (c * nb + o) * rs, (char*)h->v + c * NS * rs, NS * rs, hipMemcpyDeviceToDevice, b->stream));
+      BHIPCHK(b, hipMemcpyAsync(a + (c * nb + o) * rs, (char*)h->a + c * NS * rs, NS * rs, hipMemcpyDeviceToDevice, b->stream));
+    }
+    BHIPCHK(b, hipMemcpyAsync(rm0 + o * rs, h->st.rmass0, NS * rs, hipMemcpyDeviceToDevice, b->stream));
+    const size_t ncf = b->coef_mode == COEF_SHARED ? (size_t)h->nelast * NP : (size_t)h->nelem * co_per;
+    BHIPCHK(b, hipMemcpyAsync(co + cofs * rs, h->st.coefs, ncf * rs, hipMemcpyDeviceToDevice, b->stream));
+    cofs += ncf;
+    if (ge) {
+      BHIPCHK(b, hipMemcpyAsync(ge + gofs * rs, h->st.geom, (size_t)h->nelem * 3 * rs, hipMemcpyDeviceToDevice, b->stream));
+      gofs += (size_t)h->nelem * 3;
+    }
+  }
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  X.d = d;
+  X.v = v;
+  X.a = a;
+  X.rmass0 = rm0;
+  X.coefs = co;
+  X.geom = ge;
+  X.fpart = fp;
+  if ((rc = balloc(b, &b->it_dev, 1))) return rc;
+  if ((rc = balloc(b, &b->flags, 4))) return rc;
+  const int zero4[4] = {0, 0, 0, 0};
+  BHIPCHK(b, hipMemcpy(b->flags, zero4, sizeof zero4, hipMemcpyHostToDevice));
+  BHIPCHK(b, hipMemcpy(b->it_dev, zero4, sizeof(int), hipMemcpyHostToDevice));
+  X.itp = b->it_dev;
+  if (h0->flt.on && (rc = gather_fault(b))) return rc;
+  // receivers: members with stations, member after member
+  std::vector<int> rn;
+  b->rec_base.assign(n, 0);
+  for (int m = 0; m < n; ++m) {
+    sem2d_solver* h = b->mem[m];
+    b->rec_base[m] = (int)rn.size();
+    if (!h->nrec) continue;
+    if (!rn.empty() && (h->isamp != b->isamp || h->field != b->field))
+      return berr(b, 3, "member %d: receiver sampling and field must match", m);
+    b->isamp = h->isamp;
+    b->field = h->field;
+    b->nsamp = std::max(b->nsamp, h->nsamp);
+    std::vector<int> t(h->nrec);
+    BHIPCHK(b, hipMemcpy(t.data(), h->rec_node, h->nrec * sizeof(int), hipMemcpyDeviceToHost));
+    for (int g : t) rn.push_back(g + (int)b->node_base[m]);
+  }
+  b->nrec = (int)rn.size();
+  if ((rc = bupload(b, &b->rec_node, rn))) return rc;
+  if ((rc = balloc(b, &b->seis, (size_t)std::max(b->nsamp, 1) * nd * std::max(b->nrec, 1)))) return rc;
+  // default launch configuration (sem2d_batch_tune measures the alternatives)
+  batch_set_config(b, b->fp32 ? 512 : 256, ntiles >= 8 * b->ncu ? 2 : 0);
+  // records of step 0, then the counter moves to step 1
+  if (b->fp32) launch_batch_record<float>(b);
+  else launch_batch_record<double>(b);
+  k_tick<<<1, 64, 0, b->stream>>>(b->it_dev);
+  BHIPCHK(b, hipGetLastError());
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int sem2d_batch_step(sem2d_batch* b, int nsteps) {
+  if (!b) return 1;
+  if (nsteps < 0) return berr(b, 3, "batch_step: negative nsteps");
+  return batch_run(b, nsteps, true);
+}
+
+int sem2d_batch_tune(sem2d_batch* b, int trial_steps, int* choice) {
+  if (!b) return 1;
+  if (trial_steps < 1) return berr(b, 3, "batch_tune: trial_steps must be >= 1");
+  // save the state the trials advance
+  const size_t rs = b->fp32 ? sizeof(float) : sizeof(double);
+  const size_t nf = (size_t)b->ndof * b->fstride * rs;
+  std::vector<void*> saved;
+  auto keep = [&](const void* src, size_t bytes) -> int {
+    void* q = nullptr;
+    BHIPCHK(b, hipMalloc(&q, std::max<size_t>(bytes, 1)));
+    saved.push_back(q);
+    BHIPCHK(b, hipMemcpyAsync(q, src, bytes, hipMemcpyDeviceToDevice, b->stream));
+    return 0;
+  };
+  struct Blk { void* p; size_t bytes; };
+  std::vector<Blk> blks = {{b->X.d, nf}, {b->X.v, nf}, {b->X.a, nf}, {b->it_dev, sizeof(int)}, {b->flags, 4 * sizeof(int)}};
+  if (b->X.fault_on) {
+    const FaultDev& F = b->X.F;
+    const size_t np = F.np, ndp = (size_t)F.ndof * np;
+    blks.push_back({F.theta, np * 8});
+    blks.push_back({F.D, ndp * 8});
+    blks.push_back({F.V, ndp * 8});
+    blks.push_back({F.T, 2 * np * 8});
+    blks.push_back({F.Tstick, 2 * np * 8});
+    blks.push_back({F.MU, np * 8});
+    blks.push_back({F.sigma, np * 8});
+  }
+  int rc = 0;
+  for (auto& k : blks)
+    if ((rc = keep(k.p, k.bytes))) break;
+  const int it0 = b->it;
+  const int cand[][2] = {{512, 2}, {256, 2}, {512, 1}, {256, 1}, {256, 0}};
+  const int ncand = sizeof(cand) / sizeof(cand[0]);
+  double best = 1e30;
+  int ibest = 0;
+  for (int q = 0; q < ncand && !rc; ++q) {
+    batch_set_config(b, cand[q][0], cand[q][1]);
+    rc = batch_run(b, 1, false);  // warm-up
+    if (!rc) rc = batch_run(b, trial_steps, false);
+    if (!rc && b->last_ms < best) {
+      best = b->last_ms;
+      ibest = q;
+    }
+    // restore
+    for (size_t k = 0; k < blks.size() && !rc; ++k)
+      if (hipMemcpyAsync(blks[k].p, saved[k], blks[k].bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
+        rc = berr(b, 10, "batch_tune: restore failed");
+    b->it = it0;
+  }
+  hipStreamSynchronize(b->stream);
+  for (void* q : saved) hipFree(q);
+  if (rc) return rc;
+  batch_set_config(b, cand[ibest][0], cand[ibest][1]);
+  if (choice) *choice = ibest;
+  return 0;
+}
+
+int sem2d_batch_set_config(sem2d_batch* b, int threads, int wg_per_cu) {
+  if (!b) return 1;
+  if ((threads != 256 && threads != 512) || wg_per_cu < 0 || wg_per_cu > 4)
+    return berr(b, 3, "batch_set_config: threads must be 256 or 512 and 0 <= wg_per_cu <= 4");
+  batch_set_config(b, threads, wg_per_cu);
+  return 0;
+}
+
+int sem2d_batch_config(sem2d_batch* b, int* threads, int* wgs, int* ntiles) {
+  if (!b) return 1;
+  if (threads) *threads = b->X.pipe_wgs > 0 ? b->X.pipe_threads : TILE_THREADS;
+  if (wgs) *wgs = b->X.pipe_wgs > 0 ? b->X.pipe_wgs : 8 * b->X.T.per_xcd;
+  if (ntiles) *ntiles = b->X.T.ntiles;
+  return 0;
+}
+
+int sem2d_batch_fetch_fields(sem2d_batch* b, int m, double* d, double* v, double* a) {
+  if (!b) return 1;
+  if (m < 0 || m >= b->n) return berr(b, 3, "batch_fetch_fields: bad member %d", m);
+  sem2d_solver* h = b->mem[m];
+  const size_t NS = h->st.NS, o = b->node_base[m], rs = b->fp32 ? 4 : 8;
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  double* outs[3] = {d, v, a};
+  void* srcs[3] = {b->X.d, b->X.v, b->X.a};
+  std::vector<char> raw(NS * rs);
+  for (int q = 0; q < 3; ++q) {
+    if (!outs[q]) continue;
+    for (int c = 0; c < b->ndof; ++c) {
+      BHIPCHK(b, hipMemcpy(raw.data(), (char*)srcs[q] + (c * b->fstride + o) * rs, NS * rs, hipMemcpyDeviceToHost));
+      for (int k = 0; k < h->npoin; ++k) {
+        const size_t g = h->st.gmap[k];
+        outs[q][(size_t)c * h->npoin + k] = b->fp32 ? (double)((float*)raw.data())[g] : ((double*)raw.data())[g];
+      }
+    }
+  }
+  return 0;
+}
+
+int sem2d_batch_fetch_seismo(sem2d_batch* b, int m, double* out, int* taken) {
+  if (!b) return 1;
+  if (m < 0 || m >= b->n) return berr(b, 3, "batch_fetch_seismo: bad member %d", m);
+  sem2d_solver* h = b->mem[m];
+  const int tk = h->nrec ? std::min(rec_taken(b), h->nsamp) : 0;
+  if (out && tk) {
+    std::vector<double> all;
+    int rc = bdown(b, all, b->seis, (size_t)tk * b->ndof * b->nrec);
+    if (rc) return rc;
+    for (int s = 0; s < tk; ++s)
+      for (int c = 0; c < b->ndof; ++c)
+        for (int r = 0; r < h->nrec; ++r)
+          out[((size_t)s * b->ndof + c) * h->nrec + r] = all[((size_t)s * b->ndof + c) * b->nrec + b->rec_base[m] + r];
+  }
+  if (taken) *taken = tk;
+  return 0;
+}
+
+int sem2d_batch_fetch_fault(sem2d_batch* b, int m, double* out, int* taken) {
+  if (!b) return 1;
+  if (m < 0 || m >= b->n) return berr(b, 3, "batch_fetch_fault: bad member %d", m);
+  if (!b->X.fault_on) return berr(b, 4, "batch_fetch_fault: no fault");
+  const FaultHost& H = b->mem[m]->flt;
+  const int tk = std::min(flt_taken(b), H.nsamp);
+  if (out && tk) {
+    std::vector<double> all;
+    int rc = bdown(b, all, b->frec, (size_t)tk * 6 * b->fnout);
+    if (rc) return rc;
+    for (int s = 0; s < tk; ++s)
+      for (int q = 0; q < 6; ++q)
+        for (int o = 0; o < H.nout; ++o)
+          out[((size_t)s * 6 + q) * H.nout + o] = all[((size_t)s * 6 + q) * b->fnout + b->fout_base[m] + o];
+  }
+  if (taken) *taken = tk;
+  return 0;
+}
+
+int sem2d_batch_fetch_fault_state(sem2d_batch* b, int m, double* D, double* V, double* T, double* MU, double* Ts) {
+  if (!b) return 1;
+  if (m < 0 || m >= b->n) return berr(b, 3, "batch_fetch_fault_state: bad member %d", m);
+  if (!b->X.fault_on) return berr(b, 4, "batch_fetch_fault_state: no fault");
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  const FaultDev& F = b->X.F;
+  const int nm = b->mem[m]->flt.dev.np, o = b->fault_base[m], np = F.np;
+  auto rows = [&](double* dst, const double* src, int nr) -> int {
+    if (!dst) return 0;
+    for (int r = 0; r < nr; ++r)
+      BHIPCHK(b, hipMemcpy(dst + (size_t)r * nm, src + (size_t)r * np + o, nm * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  };
+  int rc;
+  if ((rc = rows(D, F.D, F.ndof))) return rc;
+  if ((rc = rows(V, F.V, F.ndof))) return rc;
+  if ((rc = rows(T, F.T, 2))) return rc;
+  if ((rc = rows(MU, F.MU, 1))) return rc;
+  if ((rc = rows(Ts, F.Tstick, 2))) return rc;
+  return 0;
+}
+
+int sem2d_batch_get_step(sem2d_batch* b, int* it) {
+  if (!b || !it) return 1;
+  *it = b->it;
+  return 0;
+}
+
+int sem2d_batch_last_step_ms(sem2d_batch* b, double* ms) {
+  if (!b || !ms) return 1;
+  *ms = b->last_ms;
+  return 0;
+}
+
+const char* sem2d_batch_last_error(sem2d_batch* b) { return b ? b->err.c_str() : "null handle"; }
+
+void sem2d_batch_destroy(sem2d_batch* b) {
+  if (!b) return;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  for (void* p : b->allocs) hipFree(p);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
 }
 
 }  // extern "C"

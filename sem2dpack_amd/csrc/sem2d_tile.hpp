@@ -36,28 +36,41 @@ constexpr int MAXSPECIAL = 16;
 constexpr int SLOTS = PERIM_MAX + MAXSPECIAL;
 
 struct TileDev {
-  int NX1;            // grid row length (4*nx+1)
-  int nx;             // elements per row (cell index = iz*nx + ix)
   int ntiles;
   int per_xcd;        // tiles per XCD slice of the launch
   const int4* tiles;  // (tx0, tz0, nxe | nze << 8, Z0)
+  const int4* grids;  // per tile: (node_base, NX1, NZ1, cell_base) of the grid the tile lies in
   const int* spec_cnt;
   const int* spec_idx;  // [ntiles][MAXSPECIAL] local patch indices
   size_t NGp;           // component stride of the field arrays
-  int NZ1;              // grid rows
-  int nvl;              // vertical tile-edge lines stored after the grid block
   size_t FPS;           // component stride of fpart (ntiles*SLOTS)
 };
 
 // Storage index of patch node (X, Z) of the tile at (X0, Z0) whose left and right
-// columns lie on vertical tile-edge lines kL and kR (stored column-contiguous
-// after the NX1*NZ1 grid block); other nodes use grid order Z*NX1 + X.
-__device__ __forceinline__ size_t node_index(const TileDev& T, int X, int Z, int PX, int X0, int Z0, int kL,
+// columns lie on vertical tile-edge lines kL and kR.  A grid of NX1 x NZ1 nodes
+// starting at node_base keeps the nodes of those lines column-contiguous after its
+// NX1*NZ1 grid block; other nodes use grid order Z*NX1 + X.  Several grids (one per
+// scenario of a batch) can share the field arrays, each at its own node_base.
+__device__ __forceinline__ size_t node_index(const int4 gr, int X, int Z, int PX, int X0, int Z0, int kL,
                                              int kR) {
-  const size_t grid = (size_t)T.NX1 * T.NZ1;
-  if (X == 0) return grid + (size_t)kL * T.NZ1 + (Z0 + Z);
-  if (X == PX - 1) return grid + (size_t)kR * T.NZ1 + (Z0 + Z);
-  return (size_t)(Z0 + Z) * T.NX1 + X0 + X;
+  const size_t base = (size_t)gr.x, grid = (size_t)gr.y * gr.z;
+  if (X == 0) return base + grid + (size_t)kL * gr.z + (Z0 + Z);
+  if (X == PX - 1) return base + grid + (size_t)kR * gr.z + (Z0 + Z);
+  return base + (size_t)(Z0 + Z) * gr.y + X0 + X;
+}
+
+// element (cell) index of local element (lx, lz) of a tile at (tx0, tz0).  The
+// fourth descriptor field is the grid's first cell for per-cell coefficients; with
+// shared coefficients (one element array per grid) it is the offset of the grid's
+// array instead and the cell index is not used.
+__device__ __forceinline__ size_t cell_index(const int4 gr, int tx0, int tz0, int lx, int lz) {
+  const int nx = (gr.y - 1) >> 2;
+  return (size_t)gr.w + (size_t)(tz0 + lz) * nx + tx0 + lx;
+}
+
+template <typename R, int MODE>
+__device__ __forceinline__ const R* grid_coefs(const R* coefs, const int4 gr) {
+  return MODE == COEF_SHARED ? coefs + gr.w : coefs;
 }
 
 __device__ __forceinline__ int perim_slot(int X, int Z, int PX, int PZ) {
@@ -86,6 +99,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
   if (t >= T.ntiles) return;
   const int tid = threadIdx.x;
   const int4 tg = T.tiles[t];
+  const int4 gr = T.grids[t];
   const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
   const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
   const int X0 = 4 * tx0;
@@ -98,7 +112,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
     const int p = tid + TILE_THREADS * r;
     if (p < NPT) {
       const int Z = p / PX, X = p - Z * PX;
-      const size_t g = node_index(T, X, Z, PX, X0, Z0, kL, kR);
+      const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
 #pragma unroll
       for (int c = 0; c < NDOF; ++c) {
         const R dk = d[c * S + g], vk = v[c * S + g], ak = a[c * S + g];
@@ -130,9 +144,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
       if (active) {
         const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
         base = 4 * lz * PX + 4 * lx;
-        const int cell = (tz0 + lz) * T.nx + tx0 + lx;
+        const size_t cell = cell_index(gr, tx0, tz0, lx, lz);
         R A[NELAST];
-        load_coefs<R, NDOF, NELAST, MODE>(coefs, geom, cell, pe, A);
+        load_coefs<R, NDOF, NELAST, MODE>(grid_coefs<R, MODE>(coefs, gr), geom, cell, pe, A);
         R dxi[NDOF], deta[NDOF];
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
@@ -198,7 +212,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = Fn[c][p];
       } else {
-        const size_t g = node_index(T, X, Z, PX, X0, Z0, kL, kR);
+        const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
         const R rm = rmass0[g];
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
@@ -274,6 +288,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   R pd[NITER][NDOF], pv[NITER][NDOF], pa[NITER][NDOF];
   auto issue_nodes = [&](int t) {
     const int4 tg = T.tiles[t];
+    const int4 gr = T.grids[t];
     const int tx0 = tg.x, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
     const int PX = 4 * nxe + 1, NPT = PX * (4 * nze + 1), X0 = 4 * tx0;
     const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
@@ -282,7 +297,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
       const int p = tid + NT * r;
       if (p < NPT) {
         const int Z = p / PX, X = p - Z * PX;
-        const size_t g = node_index(T, X, Z, PX, X0, Z0, kL, kR);
+        const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
           pd[r][c] = d[c * S + g];
@@ -300,6 +315,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   int psp = -1;
   auto issue_elems = [&](int t) {
     const int4 tg = T.tiles[t];
+    const int4 gr = T.grids[t];
     const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
     {
       const int PX = 4 * nxe + 1, NPT = PX * (4 * nze + 1), X0 = 4 * tx0;
@@ -309,7 +325,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
         const int p = tid + NT * r;
         if (p < NPT) {
           const int Z = p / PX, X = p - Z * PX;
-          pm[r] = rmass0[node_index(T, X, Z, PX, X0, Z0, kL, kR)];
+          pm[r] = rmass0[node_index(gr, X, Z, PX, X0, Z0, kL, kR)];
         }
       }
     }
@@ -317,7 +333,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
       if (tid < nxe * nze * 3) {
         const int e = tid / 3, k = tid - 3 * (tid / 3);
         const int lx = e % nxe, lz = e / nxe;
-        pg = geom[((size_t)(tz0 + lz) * T.nx + tx0 + lx) * 3 + k];
+        pg = geom[cell_index(gr, tx0, tz0, lx, lz) * 3 + k];
       }
 #pragma unroll
       for (int q = 0; q < PB_MAXB; ++q) {
@@ -327,7 +343,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
         const int ce = b0 + le;
         if (tid < ETHREADS && ce < ncx * ncz) {
           const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
-          const size_t cell = (size_t)(tz0 + lz) * T.nx + tx0 + lx;
+          const size_t cell = cell_index(gr, tx0, tz0, lx, lz);
 #pragma unroll
           for (int c = 0; c < NDOF; ++c) lm[q][c] = coefs[cell * NDOF * NP + c * NP + pe];
         }
@@ -341,11 +357,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
 #pragma unroll 1
   for (int t = t0; t < t1; ++t) {
     const int4 tg = T.tiles[t];
+    const int4 gr = T.grids[t];
     const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
     const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
     const int X0 = 4 * tx0;
     const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
-    (void)tz0;
 
     // predictor from the prefetched values
 #pragma unroll
@@ -402,8 +418,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
             A[1] = w2 * lm[q][0] * g1;
           }
         } else {
-          const int cell = (tz0 + lz) * T.nx + tx0 + lx;
-          load_coefs<R, NDOF, NELAST, MODE>(coefs, geom, cell, pe, A);
+          const size_t cell = cell_index(gr, tx0, tz0, lx, lz);
+          load_coefs<R, NDOF, NELAST, MODE>(grid_coefs<R, MODE>(coefs, gr), geom, cell, pe, A);
         }
         R dxi[NDOF], deta[NDOF];
 #pragma unroll
@@ -470,7 +486,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
 #pragma unroll
           for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = Fn[c][p];
         } else {
-          const size_t g = node_index(T, X, Z, PX, X0, Z0, kL, kR);
+          const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
           const R rm = Rm[p];
 #pragma unroll
           for (int c = 0; c < NDOF; ++c) {
@@ -502,7 +518,8 @@ template <typename R, int NDOF>
 __global__ __launch_bounds__(256) void k_deferred(int ndef, const DefNode* __restrict__ defs,
                                                   const int* __restrict__ part_off, const R* __restrict__ fpart,
                                                   size_t FPS, const SrcEntry* __restrict__ src,
-                                                  const double* __restrict__ ampl, int nt, int it,
+                                                  const double* __restrict__ ampl, int nt, int it_arg,
+                                                  const int* __restrict__ itp,
                                                   const double* __restrict__ abso, const R* __restrict__ rmdef,
                                                   R* __restrict__ d, R* __restrict__ v, R* __restrict__ a, size_t GS,
                                                   R dt, R c1, R c2, R c3, R c4) {
@@ -510,6 +527,7 @@ __global__ __launch_bounds__(256) void k_deferred(int ndef, const DefNode* __res
   if (q >= ndef) return;
   const DefNode n = defs[q];
   const size_t g = n.g;
+  const int it = itp ? *itp : it_arg;  // step index: device counter under graph replay
   R dp[NDOF], vpp[NDOF], f[NDOF];
 #pragma unroll
   for (int c = 0; c < NDOF; ++c) {

@@ -186,10 +186,102 @@ class Solver:
         self._check(self.api.last_step_ms(self.h, C.byref(ms)), "last_step_ms")
         return ms.value
 
+    def set_tile_config(self, threads, wg_per_cu):
+        """Tile-kernel launch configuration (structured grids); results do not depend on it."""
+        self._check(self.api.set_tile_config(self.h, int(threads), int(wg_per_cu)), "set_tile_config")
+
     def close(self):
         if getattr(self, "h", None):
             self.api.destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many scenarios advanced together (``sem2d_batch_*`` of ``include/sem2d.h``).
+
+    ``members`` are :class:`Solver` objects, set up and not yet stepped; they are kept
+    alive by the batch.  Every member's results are bitwise those of its own solver.
+    """
+
+    def __init__(self, members):
+        self.members = list(members)
+        self.api = self.members[0].api
+        n = len(self.members)
+        arr = (C.c_void_p * n)(*[m.h.value for m in self.members])
+        h = C.c_void_p()
+        rc = self.api.batch_create(n, arr, C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.api.batch_last_error(h) if h.value else b""
+            self.close()
+            raise SolverError(f"batch_create failed ({rc}): {msg.decode() if msg else ''}")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.api.batch_last_error(self.h)
+            raise SolverError(f"batch_{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def tune(self, trial_steps=5):
+        choice = C.c_int()
+        self._check(self.api.batch_tune(self.h, int(trial_steps), C.byref(choice)), "tune")
+        return choice.value
+
+    def set_config(self, threads, wg_per_cu):
+        self._check(self.api.batch_set_config(self.h, int(threads), int(wg_per_cu)), "set_config")
+
+    def config(self):
+        t, w, n = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.api.batch_config(self.h, C.byref(t), C.byref(w), C.byref(n)), "config")
+        return dict(threads=t.value, workgroups=w.value, ntiles=n.value)
+
+    def step(self, n):
+        self._check(self.api.batch_step(self.h, int(n)), "step")
+
+    def fields(self, m):
+        s = self.members[m]
+        sh = (s.ndof, s.npoin)
+        d, v, a = np.empty(sh), np.empty(sh), np.empty(sh)
+        self._check(self.api.batch_fetch_fields(self.h, m, _p(d), _p(v), _p(a)), "fetch_fields")
+        return d, v, a
+
+    def seismograms(self, m):
+        s = self.members[m]
+        out = np.empty((s.nsamp, s.ndof, s.nrec))
+        n = C.c_int()
+        self._check(self.api.batch_fetch_seismo(self.h, m, _p(out), C.byref(n)), "fetch_seismo")
+        return out[:n.value]
+
+    def fault_records(self, m):
+        s = self.members[m]
+        out = np.empty((s.fault_nsamp, 6, s.nout))
+        n = C.c_int()
+        self._check(self.api.batch_fetch_fault(self.h, m, _p(out), C.byref(n)), "fetch_fault")
+        return out[:n.value]
+
+    def fault_state(self, m):
+        s = self.members[m]
+        nd, npf = s.ndof, s.npf
+        D, V = np.empty((nd, npf)), np.empty((nd, npf))
+        T, Ts, MU = np.empty((2, npf)), np.empty((2, npf)), np.empty(npf)
+        self._check(self.api.batch_fetch_fault_state(self.h, m, _p(D), _p(V), _p(T), _p(MU), _p(Ts)),
+                    "fetch_fault_state")
+        return dict(D=D, V=V, T=T, MU=MU, Tstick=Ts)
+
+    def last_step_ms(self):
+        ms = C.c_double()
+        self._check(self.api.batch_last_step_ms(self.h, C.byref(ms)), "last_step_ms")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.api.batch_destroy(self.h)
+        self.h = None
 
     def __del__(self):
         try:

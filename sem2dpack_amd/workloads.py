@@ -106,3 +106,69 @@ def build_throughput(nx=2000, nz=1000, lx=200e3, lz=100e3, nt=5000, seed=2026020
     grid = layered_random_grid((0.0, lx), (0.0, lz), dx=dx, seed=seed)
     par = parfile.parse_par(par_throughput(nx, nz, lx, lz, nt))
     return builder.build(par, nsteps=nt, material=hete1_material(*grid), per_element_coefs=per_element_coefs)
+
+
+# ---------------------------------------------------------------------------
+# Batched rupture ensemble
+# ---------------------------------------------------------------------------
+# Each scenario: P-SV box of nx x nz elements of 100 m (nz = nx/2), homogeneous
+# medium (rho 2670, cp 6000, cs 3464), a horizontal split-node fault through the
+# middle (ezflt = -1), rate-and-state aging law (kind 2) with a ~ U(0.008, 0.016),
+# b = 0.012, Dc = 0.02 m, V* = 1e-6 m/s, f* = 0.6, normal stress -100 MPa, initial
+# slip rate 1e-9 m/s and state Dc/V, four absorbing boundaries, and an initial shear
+# stress 60 MPa * (1 + 0.05 g(x)), g a unit-variance 1D Gaussian random field of
+# correlation length 1 km, given to the solver as a DIST_LINEAR table sampled every
+# 100 m.  A fixed time step of 0.86 ms (Courant 0.299) is shared by all scenarios
+# (a Courant number would give each mesh a dt differing in the last bits).  Fault
+# outputs every 10 steps at every 4th fault node.
+RUPTURE_DX = 100.0
+RUPTURE_SEED = 20260401
+
+
+def gaussian_random_field_1d(n, dx, corr_len, seed):
+    """Unit-variance 1D field with correlation exp(-r^2/corr_len^2), by FFT filtering."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = int(2 ** math.ceil(math.log2(n + 8 * corr_len / dx)))
+    w = rng.standard_normal(p)
+    k = np.fft.fftfreq(p, d=dx) * 2 * np.pi
+    g = np.real(np.fft.ifft(np.fft.fft(w) * np.exp(-k ** 2 * corr_len ** 2 / 8.0)))[:n]
+    return (g - g.mean()) / g.std()
+
+
+def rupture_ensemble(n=256, seed=RUPTURE_SEED, nx_min=100, nx_max=400):
+    """Scenario parameters: list of dicts (nx, nz, a, seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nxs = rng.integers(nx_min, nx_max + 1, size=n)
+    avals = rng.uniform(0.008, 0.016, size=n)
+    return [dict(nx=int(nx), nz=int(nx) // 2, a=float(a), seed=seed + 1 + s)
+            for s, (nx, a) in enumerate(zip(nxs, avals))]
+
+
+def par_rupture(nx, nz, a, seed, nt=3000, dx=RUPTURE_DX, tt_mean=60e6, tt_std=0.05, corr_len=1000.0,
+                dt="8.6d-4", otd="8.6d-3", oxd=4):
+    """Par.inp text of one ensemble scenario (see the block comment above)."""
+    lx, lz = nx * dx, nz * dx
+    xs = -0.5 * lx + dx * np.arange(nx + 1)
+    tt = tt_mean * (1.0 + tt_std * gaussian_random_field_1d(nx + 1, dx, corr_len, seed))
+    table = "\n".join(f" {float(x)!r} {float(t)!r}" for x, t in zip(xs, tt))
+    bcs = "".join(f"&BC_DEF tag={t}, kind='ABSORB' /\n" for t in (1, 2, 3, 4))
+    return (f"&GENERAL iexec=1, ngll=5, fmax=1d0, ndof=2, title='RSF ensemble scenario {seed}', "
+            f"verbose='0000', ItInfo=100000 /\n"
+            f"&MESH_DEF method='CARTESIAN' /\n"
+            f"&MESH_CART xlim={-0.5 * lx!r},{0.5 * lx!r}, zlim={-0.5 * lz!r},{0.5 * lz!r}, "
+            f"nelem={nx},{nz}, ezflt=-1 /\n"
+            f"&MATERIAL tag=1, kind='ELAST' /\n"
+            f"&MAT_ELASTIC rho=2670d0, cp=6000d0, cs=3464d0 /\n"
+            + bcs +
+            f"&BC_DEF tags=5,6, kind='DYNFLT' /\n"
+            f"&BC_DYNFLT friction='RSF', Tn=-100d6, TtH='LINEAR', V=1d-9, otd={otd}, oxi=1,{2 ** 31 - 1},{oxd} /\n"
+            f"&DIST_LINEAR n={nx + 1}, dim=1 /\n{table}\n"
+            f"&BC_DYNFLT_RSF kind=2, Dc=0.02d0, MuS=0.6d0, a={a!r}, b=0.012d0, Vstar=1d-6, theta=2d7 /\n"
+            f"&TIME kind='newmark', NbSteps={nt}, Dt={dt} /\n")
+
+
+def build_rupture(sc, nt=3000):
+    """Problem of one ensemble scenario (parameters from :func:`rupture_ensemble`)."""
+    from . import builder, parfile
+    par = parfile.parse_par(par_rupture(sc["nx"], sc["nz"], sc["a"], sc["seed"], nt=nt))
+    return builder.build(par)

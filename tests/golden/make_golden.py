@@ -103,8 +103,17 @@ def run_case(name):
             out["fault_dd"] = nd["d"][sel][:, :, k2] - nd["d"][sel][:, :, k1]
             out["fault_dv"] = nd["v"][sel][:, :, k2] - nd["v"][sel][:, :, k1]
             flt = os.path.join(tmp, "Flt05_sem2d.dat")
-            if os.path.exists(flt):
+            bd = pf.read_par(os.path.join(src, "Par.inp")).first("BC_DYNFLT")
+            oxi = [int(x) for x in bd.get("oxi", [1, 2 ** 31 - 1, 1])]
+            otd = float(bd.get("otd", 0.0))
+            oitd = max(1, int(np.floor(otd / g["dt"] + 0.5)))
+            if os.path.exists(flt) and oitd == 1 and oxi[2] == 1:
                 out["flt_f32"] = read_flt(flt, npf)[::every]
+            elif os.path.exists(flt):
+                # decimated output (every oitd steps at nodes oix1:oixn:oixd): keep all records
+                nout = (min(oxi[1], npf) - oxi[0]) // oxi[2] + 1
+                out["flt_f32_all"] = read_flt(flt, nout)
+                out["flt_oitd"], out["flt_oix1"], out["flt_oixd"] = oitd, oxi[0], oxi[2]
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
         print(name, "ok", os.path.getsize(os.path.join(HERE, name + ".npz")), "bytes")
     finally:

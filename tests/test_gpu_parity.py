@@ -44,7 +44,7 @@ def test_fint(name, oracle_api):
 
 
 @pytest.mark.parametrize("name,nsteps", [("small_psv", 300), ("c1_psv", 600), ("sh_swf", 600),
-                                         ("psv_rsf", 500), ("hete_psv", 400)])
+                                         ("psv_rsf", 500), ("hete_psv", 400), ("rsf_ens", 400)])
 def test_time_series(name, nsteps, oracle_api):
     print(f"\n[case {name} nsteps {nsteps}]", flush=True)
     pb, ref, gpu = _pair(name, nsteps, oracle_api)
@@ -96,3 +96,33 @@ def test_fp32_bulk_error(oracle_api):
     err = rel(g32.seismograms(), ref.seismograms())
     print(f"\nFP32 bulk seismogram rel-L2 error vs FP64 oracle: {err:.3e}")
     assert err < 1e-3
+
+
+def test_pipelined_tile_kernel(oracle_api):
+    """The persistent pipelined tile kernel (runs of several tiles per workgroup, with
+    the next tile's loads in flight) on a 400x200-element window of the heterogeneous
+    throughput medium with its explosion source: FP64 vs the oracle at the parity bar,
+    and bitwise equal to the one-workgroup-per-tile kernel for both launch widths and
+    both bulk precisions (every variant sums each node's forces in the same order)."""
+    from sem2dpack_amd import workloads
+    nt = 40
+    pb = workloads.build_throughput(nx=400, nz=200, lx=40e3, lz=20e3, nt=nt, per_element_coefs=True)
+    ref = solver.Solver(pb, api=oracle_api, compact=False)
+    ref.step(nt)
+    fr = ref.fields()
+    for fp32 in (False, True):
+        outs = {}
+        for threads, per_cu in ((256, 0), (256, 1), (512, 1), (512, 2)):
+            print(f"\n[pipe fp32={fp32} threads={threads} wg_per_cu={per_cu}]", flush=True)
+            g = solver.Solver(pb, bulk_fp32=fp32)
+            g.set_tile_config(threads, per_cu)
+            g.step(nt)
+            outs[(threads, per_cu)] = g.fields()
+            g.close()
+        base = outs[(256, 0)]
+        for key, f in outs.items():
+            for x, y in zip(f, base):
+                assert np.array_equal(x, y), key
+        if not fp32:
+            for nm, x, y in zip("dva", base, fr):
+                assert rel(x, y) < TOL, nm
