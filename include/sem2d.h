@@ -161,6 +161,33 @@ const char *sem2d_last_error(sem2d_solver *h);
 void sem2d_destroy(sem2d_solver *h);
 
 /* ------------------------------------------------------------------------------
+   Adjoint of the time loop (no counterpart in the reference, which has no
+   differentiation).  Structured P-SV grids with compact (lambda, mu) coefficients,
+   beta = 0, no fault.  J is any function of the recorded seismograms.
+
+   sem2d_adjoint_init      allocate sqrt(N) checkpointing for nsteps steps: the state
+                           before every segment of seg steps (seg <= 0: ceil(sqrt(N)))
+                           plus seg+1 replay slots
+   sem2d_adjoint_forward   from the zero state, nsteps steps storing the checkpoints;
+                           seismograms are recorded as by sem2d_step
+   sem2d_adjoint_backward  gseis = dJ/d(seismograms) [nsamp][ndof][nrec]; replays each
+                           segment from its checkpoint and runs the adjoint (transposed)
+                           steps backwards through it
+   sem2d_adjoint_fetch     the sensitivities: glam, gmu [nelem] = dJ/d(lambda), dJ/d(mu)
+                           per element (summed over its GLL points), gmass [npoin] =
+                           dJ/dM (assembled mass, both components), gabso [ndof][npoin] =
+                           dJ/dC at absorbing-boundary nodes (0 elsewhere), C the
+                           Clayton-Engquist coefficient including its share of the
+                           modified mass
+   ------------------------------------------------------------------------------ */
+int sem2d_adjoint_init(sem2d_solver *h, int nsteps, int seg);
+int sem2d_adjoint_forward(sem2d_solver *h);
+int sem2d_adjoint_backward(sem2d_solver *h, const double *gseis);
+int sem2d_adjoint_fetch(sem2d_solver *h, double *glam, double *gmu, double *gmass, double *gabso);
+/* device time (ms) of the last forward and backward sweeps */
+int sem2d_adjoint_timing(sem2d_solver *h, double *fwd_ms, double *bwd_ms);
+
+/* ------------------------------------------------------------------------------
    Batched ensemble: many independent scenarios advanced together.
 
    The reference runs one scenario per process (SRC/main.f90:51-99, one time loop per

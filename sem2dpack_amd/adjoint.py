@@ -1,0 +1,143 @@
+"""Gradients of seismogram functionals with respect to per-element rho, lambda, mu.
+
+The reference has no differentiation; this module adds it on top of the drop-in
+time step.  :class:`SeismogramModel` maps a model -- constant density and Lame
+parameters in every element, arrays over the cells (index iz*nx + ix) -- to the
+seismograms of a structured P-SV input (its sources, receivers and boundaries),
+and back-propagates dJ/d(seismograms) to dJ/d(model) with the hand-written
+adjoint of the HIP library (``sem2d_adjoint_*``: sqrt(N) checkpointing, replayed
+segments, transposed element operator).
+
+The library returns the sensitivities to the quantities the time step consumes:
+lambda and mu of every element (through the element coefficients), the assembled
+mass M and the Clayton-Engquist coefficients C (the latter including their share of
+the modified mass).  The chain rule to rho, lambda, mu is done here, mirroring how
+the builder forms them (``SRC/mat_mass.f90:29-61``, ``SRC/bc_abso.f90:195-207``):
+
+  M(node)   = sum over element points at the node of rho_e * w_i w_j detJ
+  C(node,c) = sum over boundary-edge points at the node of z_e,c * w_k |J1D|,
+              z = rho cp = sqrt(rho (lambda + 2 mu)) along the normal,
+              z = rho cs = sqrt(rho mu) along the boundary.
+
+:func:`seismograms` exposes the map as a ``torch.autograd.Function``.
+"""
+import numpy as np
+
+from . import builder, parfile, solver
+
+
+class SeismogramModel:
+    """seismograms(rho, lam, mu) of an input file, differentiable by the adjoint.
+
+    ``par``: Par.inp path or text (structured P-SV, fixed ``Dt`` so that every model
+    shares the time step); ``fname_dir``: where its data files are read.
+    """
+
+    def __init__(self, par, nsteps=None, fname_dir=None, bulk_fp32=False, seg=0):
+        if isinstance(par, str) and "\n" in par:
+            par = parfile.parse_par(par)
+        elif isinstance(par, str):
+            import os
+            fname_dir = fname_dir or os.path.dirname(os.path.abspath(par))
+            par = parfile.read_par(par)
+        self.par, self.fname_dir = par, fname_dir
+        self.nsteps, self.bulk_fp32, self.seg = nsteps, bulk_fp32, seg
+        self._state = None
+        self.last_timing = None
+
+    def problem(self, rho, lam, mu):
+        return builder.build(self.par, nsteps=self.nsteps, fname_dir=self.fname_dir, per_element_coefs=False,
+                             elem_props=(rho, lam, mu))
+
+    def forward(self, rho, lam, mu):
+        """Seismograms [nsamp][ndof][nrec] (FP64) of the model."""
+        rho, lam, mu = (np.ascontiguousarray(a, dtype=np.float64) for a in (rho, lam, mu))
+        pb = self.problem(rho, lam, mu)
+        if pb.ndof != 2 or pb.receivers is None:
+            raise ValueError("SeismogramModel needs a P-SV input with receivers")
+        s = solver.Solver(pb, bulk_fp32=self.bulk_fp32)
+        s.adjoint_init(pb.nt, self.seg)
+        s.adjoint_forward()
+        seis = s.seismograms()
+        if self._state is not None:
+            self._state[1].close()
+        self._state = (pb, s, (rho, lam, mu))
+        return seis
+
+    def backward(self, gseis):
+        """(grho, glam, gmu) over the cells, for dJ/d(seismograms) = gseis, at the model
+        of the last :meth:`forward`."""
+        if self._state is None:
+            raise RuntimeError("backward before forward")
+        pb, s, (rho, lam, mu) = self._state
+        s.adjoint_backward(gseis)
+        self.last_timing = s.adjoint_timing()
+        glam_e, gmu_e, gmass, gabso = s.adjoint_fetch()
+        return self.chain(pb, rho, lam, mu, glam_e, gmu_e, gmass, gabso)
+
+    @staticmethod
+    def chain(pb, rho, lam, mu, glam_e, gmu_e, gmass, gabso):
+        """Per-cell gradients from the library's sensitivities (see the module text)."""
+        nx = pb.mesh["nx"]
+        cell = pb.elem_iz * nx + pb.elem_ix  # builder element -> cell
+        r, la, m = rho[cell], lam[cell], mu[cell]
+        grho_e = (pb.dvol * gmass[pb.ibool - 1]).sum(axis=(1, 2))
+        glam_e = glam_e.copy()
+        gmu_e = gmu_e.copy()
+        for ab in pb.abso:
+            bidx, elem, wk = ab["contrib"]
+            node = ab["node"][bidx] - 1
+            re, le, me = r[elem], la[elem], m[elem]
+            zp = np.sqrt(re * (le + 2.0 * me))
+            zs = np.sqrt(re * me)
+            for c in range(pb.ndof):
+                g = gabso[c, node] * wk
+                if c == ab["geo_nor"]:
+                    np.add.at(grho_e, elem, g * (le + 2.0 * me) / (2.0 * zp))
+                    np.add.at(glam_e, elem, g * re / (2.0 * zp))
+                    np.add.at(gmu_e, elem, g * re / zp)
+                else:
+                    np.add.at(grho_e, elem, g * me / (2.0 * zs))
+                    np.add.at(gmu_e, elem, g * re / (2.0 * zs))
+        out = [np.zeros(pb.nelem) for _ in range(3)]
+        for o, g in zip(out, (grho_e, glam_e, gmu_e)):
+            o[cell] = g
+        return tuple(out)
+
+    def close(self):
+        if self._state is not None:
+            self._state[1].close()
+            self._state = None
+
+
+def _make_function():
+    import torch
+
+    class _Seismograms(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, rho, lam, mu, model):
+            arrs = [t.detach().to("cpu", torch.float64).numpy() for t in (rho, lam, mu)]
+            seis = model.forward(*arrs)
+            ctx.model = model
+            ctx.like = rho
+            return torch.as_tensor(seis, dtype=torch.float64, device=rho.device)
+
+        @staticmethod
+        def backward(ctx, gseis):
+            g = gseis.detach().to("cpu", torch.float64).numpy()
+            grads = ctx.model.backward(g)
+            like = ctx.like
+            return tuple(torch.as_tensor(x, dtype=like.dtype, device=like.device) for x in grads) + (None,)
+
+    return _Seismograms
+
+
+_FN = None
+
+
+def seismograms(rho, lam, mu, model):
+    """torch: seismograms of the per-cell model tensors (rho, lam, mu), differentiable."""
+    global _FN
+    if _FN is None:
+        _FN = _make_function()
+    return _FN.apply(rho, lam, mu, model)

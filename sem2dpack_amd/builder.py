@@ -699,6 +699,7 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
     ii, jj = _edge_ij(t.edge)
     C = np.zeros((len(t.node), ndof))
     rho_p, cp_p, cs_p = props
+    cb, ce, cf = [], [], []  # contributions: boundary node, element, weight w_k |J1D|
     for n, e in enumerate(t.elem):
         for k in range(5):
             i, j = ii[k], jj[k]
@@ -712,6 +713,9 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
                 a, b = J12[e, j, i], J22[e, j, i]
             coef = wgll[k] * math.sqrt(a * a + b * b)
             bck = t.ibool[n, k]
+            cb.append(bck)
+            ce.append(e)
+            cf.append(coef)
             if ndof == 1:
                 C[bck, 0] = C[bck, 0] + rho * c[geo_tan] * coef
             else:
@@ -719,7 +723,10 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
                 C[bck, 1] = C[bck, 1] + rho * c[1] * coef
     nodes0 = t.node - 1
     M[:, nodes0] = M[:, nodes0] + coefA2Vrhs * C.T
-    return dict(tag=t.tag, node=t.node.astype(np.int32), C=C)
+    # C[node, c] = sum over contributions of rho * c_c * weight, c_c = cp along the
+    # normal (component geo_nor), cs along the boundary
+    return dict(tag=t.tag, node=t.node.astype(np.int32), C=C, geo_nor=geo_nor,
+                contrib=(np.array(cb, dtype=np.int64), np.array(ce, dtype=np.int64), np.array(cf)))
 
 
 # ---------------------------------------------------------------------------
@@ -959,7 +966,9 @@ def _f32(x):
     return float(np.float32(x))
 
 
-def _sources(par, num, jac, hprime, ndof, dt, nt):
+def _sources(par, num, jac, hprime, ndof, dt, nt, fname_dir=None):
+    """SO_read / SO_init (SRC/src_gen.f90:96-215): one source at ``coord``, or one per
+    line of ``file`` (x z [tdelay [ampli]]), all with the same STF and mechanism."""
     g = par.first("SRC_DEF")
     if g is None:
         return []
@@ -967,9 +976,23 @@ def _sources(par, num, jac, hprime, ndof, dt, nt):
     mech = g.get("mechanism", " ")
     if str(stf).strip() == "" or str(mech).strip() == "":
         raise InputError("SO_read: stf/mechanism not set")
-    if g.get("file", "none") != "none":
-        raise InputError("SRC_DEF file= is not supported by this build")
-    coord_in = g.get("coord")
+    fname = str(g.get("file", "none")).strip()
+    if fname != "none":
+        path = fname if os.path.isabs(fname) else os.path.join(fname_dir or ".", fname)
+        tab = np.loadtxt(path, ndmin=2)
+        if tab.shape[1] not in (2, 3, 4):
+            raise InputError("SO_read: source file must have 2, 3 or 4 columns")
+        rows = [(tab[i, :2], tab[i, 2] if tab.shape[1] > 2 else 0.0, tab[i, 3] if tab.shape[1] > 3 else 1.0)
+                for i in range(tab.shape[0])]
+    else:
+        coord_in = g.get("coord")
+        if coord_in is None:
+            raise InputError("SO_read: coord not set")
+        rows = [(coord_in, 0.0, 1.0)]
+    return [_one_source(par, g, stf, mech, c, td, am, num, jac, hprime, ndof, dt, nt) for c, td, am in rows]
+
+
+def _one_source(par, g, stf, mech, coord_in, tdelay, ampli, num, jac, hprime, ndof, dt, nt):
     if stf != "RICKER":
         raise InputError(f"stf {stf!r} is not supported by this build")
     # STF_RICKER reals are single precision (SRC/stf_ricker.f90:53-73)
@@ -986,7 +1009,8 @@ def _sources(par, num, jac, hprime, ndof, dt, nt):
     coord = num["coord"]
     ibool = num["ibool"]
     iglob, _ = _nearest_node(coord, coord_in)
-    src = dict(iglob=iglob, f0=f0, t0=onset, stf_ampli=amp, tdelay=0.0, ampli=1.0, mechanism=mech)
+    src = dict(iglob=iglob, f0=f0, t0=onset, stf_ampli=amp, tdelay=float(tdelay), ampli=float(ampli),
+               mechanism=mech)
     if mech == "FORCE":
         fg = par.after(g, "SRC_FORCE")
         angle = float(fg.get("angle", 0.0)) if fg is not None else 0.0
@@ -1030,7 +1054,7 @@ def _sources(par, num, jac, hprime, ndof, dt, nt):
     # amplitude table at t = it*dt, it = 1..nt (SRC/src_gen.f90:290-317)
     src["stf_table"] = np.array([ricker(it * dt - src["tdelay"], f0, onset, amp) * src["ampli"]
                                  for it in range(1, nt + 1)])
-    return [src]
+    return src
 
 
 def _moment_stencil(iglob, Mt, ibool, jac, hprime, ndof):
@@ -1100,12 +1124,14 @@ def _receivers(par, coord, nt):
 
 
 # ---------------------------------------------------------------------------
-def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=True):
+def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=True, elem_props=None):
     """Build a :class:`Problem` from a parsed Par.inp (:func:`parfile.read_par`).
 
     ``material``, if given, is a callable ``(x, z) -> (rho, cp, cs)`` evaluated at the
     GLL nodes of every element; it replaces the MAT_ELASTIC blocks (one tag), as a
-    distribution would.  ``fname_dir`` is where relative data file names are read
+    distribution would.  ``elem_props = (rho, lam, mu)``, arrays over the cells
+    (index iz*nx + ix), gives every element constant properties instead (the model
+    space of the adjoint).  ``fname_dir`` is where relative data file names are read
     (default: the directory of the input file).  ``per_element_coefs=False`` skips
     the reference's per-element coefficient arrays for heterogeneous media (only the
     compact lambda/mu form is then built).
@@ -1166,7 +1192,14 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     shp = (nelem, 5, 5)
     rho_p, cp_p, cs_p, lam_p, mu_p = (np.empty(shp) for _ in range(5))
     homog_tag = {}
-    if material is not None:
+    if elem_props is not None:
+        cell = num["iz"] * nx + num["ix"]
+        r, la, m = (np.asarray(a, dtype=np.float64)[cell][:, None, None] for a in elem_props)
+        rho_p[:], lam_p[:], mu_p[:] = r, la, m
+        cp_p[:] = np.sqrt((lam_p + 2.0 * mu_p) / rho_p)
+        cs_p[:] = np.sqrt(mu_p / rho_p)
+        homog_tag = {int(t): False for t in np.unique(tags)}
+    elif material is not None:
         r, c1, c2 = material(ex, ez)
         rho_p[:], cp_p[:], cs_p[:] = r, c1, c2
         lam_p[:] = rho_p * (cp_p * cp_p - 2.0 * cs_p * cs_p)
@@ -1275,6 +1308,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     pb.wgll2 = wgll2
 
     # MAT_MASS_init: sequential element-order assembly (np.add.at is sequential)
+    pb.dvol = dvol  # w_i w_j detJ at every element-local point (dM/drho)
     massloc = rho_p * dvol
     M = np.zeros((ndof, npoin))
     np.add.at(M[0], ibool0.ravel(), massloc.ravel())
@@ -1308,7 +1342,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
             raise InputError(f"BC kind {kind!r} is not supported by this build")
 
     pb.receivers = _receivers(par, pb.coord, pb.nt)
-    pb.sources = _sources(par, num, jac, H, ndof, dt, pb.nt)
+    pb.sources = _sources(par, num, jac, H, ndof, dt, pb.nt, fname_dir)
     pb.rmass = 1.0 / M
     pb.mass = M
     return pb

@@ -53,6 +53,12 @@ _SIGS = {
     "last_step_ms": ([C.c_void_p, dp], C.c_int),
     "last_error": ([C.c_void_p], C.c_char_p),
     "destroy": ([C.c_void_p], None),
+    # adjoint (HIP library only)
+    "adjoint_init": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "adjoint_forward": ([C.c_void_p], C.c_int),
+    "adjoint_backward": ([C.c_void_p, dp], C.c_int),
+    "adjoint_fetch": ([C.c_void_p, dp, dp, dp, dp], C.c_int),
+    "adjoint_timing": ([C.c_void_p, dp, dp], C.c_int),
     # batched ensemble (HIP library only)
     "batch_create": ([C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)], C.c_int),
     "batch_step": ([C.c_void_p, C.c_int], C.c_int),
@@ -70,9 +76,10 @@ _SIGS = {
 }
 
 # entry points every implementation of the ABI must export (the CPU oracle has no
-# event timing, no kernel launch configuration and no batched ensemble)
+# event timing, no kernel launch configuration, no adjoint and no batched ensemble)
 GPU_ONLY = ("last_step_ms", "set_tile_config")
-REQUIRED = [k for k in _SIGS if k not in GPU_ONLY and not k.startswith("batch_")]
+GPU_ONLY_PREFIXES = ("batch_", "adjoint_")
+REQUIRED = [k for k in _SIGS if k not in GPU_ONLY and not k.startswith(GPU_ONLY_PREFIXES)]
 
 
 class Api:

@@ -74,6 +74,8 @@ struct StructState {
   void* fault_rm = nullptr;
   int nfnodes = 0;
   std::vector<void*> prep_allocs;  // freed when the deferred tables are rebuilt
+  std::vector<int> cell_elem;      // cell (iz*nx + ix) -> element (0-based)
+  std::vector<int> def_ref;        // deferred node -> reference node (0-based)
 };
 
 struct FaultHost {
@@ -85,6 +87,9 @@ struct FaultHost {
 };
 
 }  // namespace
+
+struct AdjState;
+void adj_free(AdjState* A);
 
 struct sem2d_solver {
   int ndof, nelem, npoin, nelast, coef_mode, fp32;
@@ -120,6 +125,7 @@ struct sem2d_solver {
   double last_ms = 0.0;
   std::string err;
   std::vector<void*> allocs;
+  AdjState* adj = nullptr;
 };
 
 namespace {
@@ -565,6 +571,16 @@ int prepare_structured(sem2d_solver* h) {
   int rc = struct_tables(h, X);
   if (rc) return rc;
   S.ndef = (int)X.defs.size();
+  S.def_ref.resize(S.ndef);
+  for (int q = 0; q < S.ndef; ++q) {
+    // storage index -> reference node (inverse of gmap)
+    S.def_ref[q] = X.defs[q].g;
+  }
+  {
+    std::vector<int> ginv(S.NS, -1);
+    for (int k = 0; k < h->npoin; ++k) ginv[S.gmap[k]] = k;
+    for (int q = 0; q < S.ndef; ++q) S.def_ref[q] = ginv[S.def_ref[q]];
+  }
   if ((rc = prep_upload(h, &S.tiles_dev, X.tiles.data(), X.tiles.size()))) return rc;
   if ((rc = prep_upload(h, &S.grids_dev, X.grids.data(), X.grids.size()))) return rc;
   if ((rc = prep_upload(h, &S.spec_cnt, X.spec_cnt.data(), X.spec_cnt.size()))) return rc;
@@ -850,6 +866,8 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   if (structured_map(h, s)) {
     StructState& S = h->st;
     S.on = true;
+    S.cell_elem.resize(s->nelem);
+    for (int cell = 0; cell < s->nelem; ++cell) S.cell_elem[cell] = s->elem_of_cell[cell] - 1;
     h->fstride = S.NS;
     std::vector<double> rm0(S.NS, 0.0);
     for (int k = 0; k < h->npoin; ++k) rm0[S.gmap[k]] = s->rmass[k];
@@ -1246,6 +1264,7 @@ const char* sem2d_last_error(sem2d_solver* h) { return h ? h->err.c_str() : "nul
 void sem2d_destroy(sem2d_solver* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->adj) adj_free(h->adj);
   for (void* p : h->allocs) hipFree(p);
   for (void* p : h->st.prep_allocs) hipFree(p);
   for (auto& b : h->abso) { (void)b; }
@@ -1976,6 +1995,259 @@ void sem2d_batch_destroy(sem2d_batch* b) {
   if (b->ev1) hipEventDestroy(b->ev1);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Adjoint of the time loop with sqrt(N) checkpointing (structured P-SV grids,
+// compact coefficients, beta = 0, no fault).
+//
+// forward:  from the zero state, nsteps steps; the state before every segment of
+//           seg steps is kept (checkpoints); seismograms are recorded as usual.
+// backward: for each segment, last first: the segment is replayed from its
+//           checkpoint into seg+1 state slots, then the adjoint steps run from the
+//           segment's last step down to its first, injecting dJ/d(seismogram) at the
+//           sample steps and accumulating the sensitivities:
+//             glam, gmu [cell]   dJ/d(lambda), dJ/d(mu) of each element (summed over
+//                                its GLL points)
+//             mbar [node]        dJ/dM (M the assembled mass, same for both components)
+//             gabs [c][node]     dJ/dC_c at absorbing-boundary nodes (C the
+//                                Clayton-Engquist coefficient, incl. its share c3*C
+//                                of the modified mass)
+// The host turns these into gradients with respect to per-element rho, lambda, mu
+// (the chain rule through M = sum rho w J and C = sum rho c w |J1D|).
+// ===========================================================================
+struct AdjState {
+  int nsteps = 0, seg = 0, nck = 0;
+  size_t state_bytes = 0;  // d, v, a of one state
+  std::vector<char*> ck, slot;
+  char *ad = nullptr, *av = nullptr, *aa = nullptr, *rmass = nullptr;
+  double *glam = nullptr, *gmu = nullptr, *mbar = nullptr, *gabs = nullptr, *gseis = nullptr;
+  std::vector<void*> allocs;
+  double fwd_ms = 0, bwd_ms = 0;
+};
+
+void adj_free(AdjState* A) {
+  for (void* p : A->allocs) hipFree(p);
+  delete A;
+}
+
+namespace {
+
+int adj_alloc(sem2d_solver* h, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
+  if (e != hipSuccess) return set_err(h, 11, "adjoint: hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+  h->adj->allocs.push_back(*p);
+  return 0;
+}
+
+// copy a whole state (d, v, a) between two state buffers or the solver's fields
+int copy_state(sem2d_solver* h, char* dst_d, char* dst_v, char* dst_a, const char* d, const char* v, const char* a) {
+  const size_t n = (size_t)h->ndof * h->fstride * rsize(h);
+  HIPCHK(h, hipMemcpyAsync(dst_d, d, n, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dst_v, v, n, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(dst_a, a, n, hipMemcpyDeviceToDevice, h->stream));
+  return 0;
+}
+
+inline char* st_d(sem2d_solver* h, char* s) { (void)h; return s; }
+inline char* st_v(sem2d_solver* h, char* s) { return s + (size_t)h->ndof * h->fstride * rsize(h); }
+inline char* st_a(sem2d_solver* h, char* s) { return s + 2 * (size_t)h->ndof * h->fstride * rsize(h); }
+
+template <typename R>
+int adj_backward(sem2d_solver* h, const double* gseis_host) {
+  AdjState& A = *h->adj;
+  StructState& S = h->st;
+  const size_t GS = h->fstride, nf = (size_t)h->ndof * GS;
+  const R dt = (R)h->dt;
+  const R c1 = (R)((0.5 - h->beta) * h->dt * h->dt), c2 = (R)((1.0 - h->gamma) * h->dt);
+  const R c3 = (R)(h->gamma * h->dt), c4 = (R)(h->beta * h->dt * h->dt);
+  const size_t ns = (size_t)h->nsamp * h->ndof * h->nrec;
+  if (h->nrec && gseis_host)
+    HIPCHK(h, hipMemcpyAsync(A.gseis, gseis_host, ns * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(A.ad, 0, nf * sizeof(R), h->stream));
+  HIPCHK(h, hipMemsetAsync(A.av, 0, nf * sizeof(R), h->stream));
+  HIPCHK(h, hipMemsetAsync(A.aa, 0, nf * sizeof(R), h->stream));
+  HIPCHK(h, hipMemsetAsync(A.glam, 0, (size_t)h->nelem * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(A.gmu, 0, (size_t)h->nelem * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(A.mbar, 0, GS * sizeof(double), h->stream));
+  HIPCHK(h, hipMemsetAsync(A.gabs, 0, (size_t)h->ndof * std::max(S.ndef, 1) * sizeof(double), h->stream));
+  StepDev X = step_dev(h);
+  R* inj = (R*)(h->field == 0 ? A.ad : (h->field == 1 ? A.av : A.aa));
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  for (int k = A.nck - 1; k >= 0; --k) {
+    const int s0 = k * A.seg, s1 = std::min(A.nsteps, s0 + A.seg), L = s1 - s0;
+    // replay: slot[j] = state after step s0 + j
+    char* s = A.slot[0];
+    int rc = copy_state(h, st_d(h, s), st_v(h, s), st_a(h, s), st_d(h, A.ck[k]), st_v(h, A.ck[k]), st_a(h, A.ck[k]));
+    if (rc) return rc;
+    for (int j = 1; j <= L; ++j) {
+      char* p = A.slot[j - 1];
+      char* q = A.slot[j];
+      if ((rc = copy_state(h, st_d(h, q), st_v(h, q), st_a(h, q), st_d(h, p), st_v(h, p), st_a(h, p)))) return rc;
+      X.d = st_d(h, q);
+      X.v = st_v(h, q);
+      X.a = st_a(h, q);
+      launch_structured_step<R>(X, s0 + j, dt, c1, c2, c3, c4);
+    }
+    // adjoint steps n = s1 .. s0+1
+    for (int n = s1; n > s0; --n) {
+      const int j = n - s0;
+      if (h->nrec && n % h->isamp == 0 && n / h->isamp < h->nsamp)
+        k_adj_inject<R><<<1, 64, 0, h->stream>>>(h->nrec, h->rec_node,
+                                                 A.gseis + (size_t)(n / h->isamp) * h->ndof * h->nrec, inj, GS);
+      const R* dfw = (const R*)st_d(h, A.slot[j]);
+      const R* afw = (const R*)st_a(h, A.slot[j]);
+      k_adj_tile<R><<<8 * S.per_xcd, TILE_THREADS, 0, h->stream>>>(
+          X.T, (R*)A.ad, (R*)A.av, (R*)A.aa, (const R*)A.rmass, dfw, afw, (const R*)S.coefs, (const R*)S.geom,
+          (R*)S.fpart, A.glam, A.gmu, A.mbar, dt, c1, c2, c3);
+      k_adj_deferred<R><<<(S.ndef + 255) / 256, 256, 0, h->stream>>>(
+          S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dabso, (const R*)S.rmdef, (R*)A.ad, (R*)A.av,
+          (R*)A.aa, GS, afw, (const R*)st_v(h, A.slot[j - 1]), (const R*)st_a(h, A.slot[j - 1]), A.mbar, A.gabs,
+          dt, c1, c2, c3);
+    }
+  }
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  A.bwd_ms = ms;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
+  if (!h) return 1;
+  if (!h->st.on || h->ndof != 2 || h->coef_mode != COEF_COMPACT || h->nelast != 6)
+    return set_err(h, 4, "adjoint: needs a structured P-SV grid with compact (lambda, mu) coefficients");
+  if (h->beta != 0.0) return set_err(h, 4, "adjoint: needs the explicit scheme (beta = 0)");
+  if (h->flt.on) return set_err(h, 4, "adjoint: faults are not differentiated");
+  if (nsteps < 1) return set_err(h, 3, "adjoint_init: nsteps must be >= 1");
+  if (seg <= 0) seg = (int)std::ceil(std::sqrt((double)nsteps));
+  int rc;
+  if (!h->st.ready && (rc = sync_sources(h))) return rc;
+  if (!h->st.ready && (rc = prepare_structured(h))) return rc;
+  if (h->adj) adj_free(h->adj);
+  h->adj = new AdjState();
+  AdjState& A = *h->adj;
+  A.nsteps = nsteps;
+  A.seg = std::min(seg, nsteps);
+  A.nck = (nsteps + A.seg - 1) / A.seg;
+  const size_t nf = (size_t)h->ndof * h->fstride * rsize(h);
+  A.state_bytes = 3 * nf;
+  for (int k = 0; k < A.nck; ++k) {
+    void* p;
+    if ((rc = adj_alloc(h, &p, A.state_bytes))) return rc;
+    A.ck.push_back((char*)p);
+  }
+  for (int k = 0; k <= A.seg; ++k) {
+    void* p;
+    if ((rc = adj_alloc(h, &p, A.state_bytes))) return rc;
+    A.slot.push_back((char*)p);
+  }
+  void* p;
+  if ((rc = adj_alloc(h, &p, nf))) return rc;
+  A.ad = (char*)p;
+  if ((rc = adj_alloc(h, &p, nf))) return rc;
+  A.av = (char*)p;
+  if ((rc = adj_alloc(h, &p, nf))) return rc;
+  A.aa = (char*)p;
+  if ((rc = adj_alloc(h, &p, (size_t)h->nelem * sizeof(double)))) return rc;
+  A.glam = (double*)p;
+  if ((rc = adj_alloc(h, &p, (size_t)h->nelem * sizeof(double)))) return rc;
+  A.gmu = (double*)p;
+  if ((rc = adj_alloc(h, &p, h->fstride * sizeof(double)))) return rc;
+  A.mbar = (double*)p;
+  if ((rc = adj_alloc(h, &p, (size_t)h->ndof * std::max(h->st.ndef, 1) * sizeof(double)))) return rc;
+  A.gabs = (double*)p;
+  if ((rc = adj_alloc(h, &p, (size_t)std::max(h->nsamp, 1) * h->ndof * std::max(h->nrec, 1) * sizeof(double))))
+    return rc;
+  A.gseis = (double*)p;
+  // inverse mass of both components in storage order
+  std::vector<double> rm((size_t)h->ndof * h->fstride, 0.0);
+  for (int c = 0; c < h->ndof; ++c)
+    for (int k = 0; k < h->npoin; ++k)
+      rm[(size_t)c * h->fstride + h->st.gmap[k]] = h->rmass_host[(size_t)c * h->npoin + k];
+  if ((rc = adj_alloc(h, &p, nf))) return rc;
+  A.rmass = (char*)p;
+  if ((rc = write_bulk(h, A.rmass, rm.data(), rm.size()))) return rc;
+  return 0;
+}
+
+int sem2d_adjoint_forward(sem2d_solver* h) {
+  if (!h) return 1;
+  if (!h->adj) return set_err(h, 4, "adjoint_forward: call sem2d_adjoint_init first");
+  AdjState& A = *h->adj;
+  const size_t nf = (size_t)h->ndof * h->fstride * rsize(h);
+  HIPCHK(h, hipMemsetAsync(h->d, 0, nf, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->v, 0, nf, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->a, 0, nf, h->stream));
+  h->it = 0;
+  h->taken = 0;
+  int rc = h->fp32 ? record_receivers<float>(h) : record_receivers<double>(h);
+  if (rc) return rc;
+  double ms = 0;
+  for (int k = 0; k < A.nck; ++k) {
+    char* c = A.ck[k];
+    if ((rc = copy_state(h, st_d(h, c), st_v(h, c), st_a(h, c), (char*)h->d, (char*)h->v, (char*)h->a))) return rc;
+    const int L = std::min(A.seg, A.nsteps - k * A.seg);
+    rc = h->fp32 ? do_steps<float>(h, L) : do_steps<double>(h, L);
+    if (rc) return rc;
+    ms += h->last_ms * L;
+  }
+  A.fwd_ms = ms;
+  return 0;
+}
+
+int sem2d_adjoint_backward(sem2d_solver* h, const double* gseis) {
+  if (!h) return 1;
+  if (!h->adj) return set_err(h, 4, "adjoint_backward: call sem2d_adjoint_init first");
+  if (h->it != h->adj->nsteps) return set_err(h, 4, "adjoint_backward: run sem2d_adjoint_forward first");
+  return h->fp32 ? adj_backward<float>(h, gseis) : adj_backward<double>(h, gseis);
+}
+
+int sem2d_adjoint_fetch(sem2d_solver* h, double* glam, double* gmu, double* gmass, double* gabso) {
+  if (!h) return 1;
+  if (!h->adj) return set_err(h, 4, "adjoint_fetch: call sem2d_adjoint_init first");
+  AdjState& A = *h->adj;
+  StructState& S = h->st;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::vector<double> tmp;
+  if (glam || gmu) {
+    tmp.resize(h->nelem);
+    for (int q = 0; q < 2; ++q) {
+      double* out = q == 0 ? glam : gmu;
+      if (!out) continue;
+      HIPCHK(h, hipMemcpy(tmp.data(), q == 0 ? A.glam : A.gmu, h->nelem * sizeof(double), hipMemcpyDeviceToHost));
+      for (int cell = 0; cell < h->nelem; ++cell) out[S.cell_elem[cell]] = tmp[cell];
+    }
+  }
+  if (gmass) {
+    tmp.resize(h->fstride);
+    HIPCHK(h, hipMemcpy(tmp.data(), A.mbar, h->fstride * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < h->npoin; ++k) gmass[k] = tmp[S.gmap[k]];
+  }
+  if (gabso) {
+    std::fill(gabso, gabso + (size_t)h->ndof * h->npoin, 0.0);
+    tmp.resize((size_t)h->ndof * std::max(S.ndef, 1));
+    HIPCHK(h, hipMemcpy(tmp.data(), A.gabs, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int c = 0; c < h->ndof; ++c)
+      for (int q = 0; q < S.ndef; ++q) gabso[(size_t)c * h->npoin + S.def_ref[q]] = tmp[(size_t)c * S.ndef + q];
+  }
+  return 0;
+}
+
+int sem2d_adjoint_timing(sem2d_solver* h, double* fwd_ms, double* bwd_ms) {
+  if (!h) return 1;
+  if (!h->adj) return set_err(h, 4, "adjoint_timing: call sem2d_adjoint_init first");
+  if (fwd_ms) *fwd_ms = h->adj->fwd_ms;
+  if (bwd_ms) *bwd_ms = h->adj->bwd_ms;
+  return 0;
 }
 
 }  // extern "C"

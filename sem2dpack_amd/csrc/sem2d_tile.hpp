@@ -583,4 +583,239 @@ __global__ void k_fault_correct(int n, const int* __restrict__ g, const R* __res
   }
 }
 
+
+// ===========================================================================
+// Adjoint (reverse-mode) step on structured grids, P-SV with compact coefficients.
+//
+// Forward step n (SRC/solver.f90:42-84 with beta = 0):
+//   dp = d + dt v + c1 a,  vp = v + c2 a,  a' = (-K dp - C vp + s) / Mt,
+//   v' = vp + c3 a',  d' = dp,                       Mt = M + c3 C (ABC-modified mass)
+// Its transpose, from the adjoint (ad, av, aa) of step n to that of step n-1:
+//   fb = (aa + c3 av) / Mt
+//   ad <- ad - K^T fb,  vbp = av - C fb,  av <- vbp + dt ad,  aa <- c2 vbp + c1 ad
+// and the sensitivities of J to the step's parameters:
+//   dJ/dMt = -fb * a_n,  dJ/dC = -fb * vp_n,
+//   dJ/dA_k = (B fb)^T dPhi/dA_k (B d_n)   per GLL point, A_k the KD2 coefficients
+// with Phi the point flux matrix of ELAST_KD2_PSV (SRC/mat_elastic.f90:589-641):
+//   flux(xi,x) = A0 gxx + A1 gez,  flux(eta,x) = A3 (gex + gxz),
+//   flux(xi,z) = A4 gex + A5 gxz,  flux(eta,z) = A1 gxx + A2 gez,
+// (g.. the xi/eta derivatives of the x/z components).  -K^T uses Phi^T:
+//   flux(eta,x) = A3 gex + A4 gxz,  flux(xi,z) = A3 gex + A5 gxz  (others unchanged).
+// A = W2 * modulus * geometric factor (A0 = W2 (l+2m) g0, A1 = W2 l g2, A2 = W2 (l+2m) g1,
+// A3 = W2 m g1, A4 = W2 m g2, A5 = W2 m g0), so per point
+//   dJ/dl = W2 [g0 pxx gxx + g2 (pxx gez + pez gxx) + g1 pez gez]
+//   dJ/dm = W2 [2 g0 pxx gxx + 2 g1 pez gez + g1 pex (gex + gxz) + g2 pxz gex + g0 pxz gxz]
+// (p.. the derivatives of fb, g.. those of d_n), summed over the element's points.
+// One workgroup per tile; interior nodes finish here, perimeter nodes in k_adj_deferred.
+// ===========================================================================
+template <typename R>
+__global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
+    TileDev T, R* __restrict__ ad, R* __restrict__ av, R* __restrict__ aa, const R* __restrict__ rmass,
+    const R* __restrict__ dfw, const R* __restrict__ afw, const R* __restrict__ coefs, const R* __restrict__ geom,
+    R* __restrict__ fpart, double* __restrict__ glam, double* __restrict__ gmu, double* __restrict__ mbar,
+    R dt, R c1, R c2, R c3) {
+  constexpr int NDOF = 2;
+  __shared__ R Uf[NDOF][PMAX];  // fb
+  __shared__ R Ud[NDOF][PMAX];  // forward d_n
+  __shared__ R Fn[NDOF][PMAX];
+  __shared__ R T1[FE_EPB][NDOF][NP];
+  __shared__ R T2[FE_EPB][NDOF][NP];
+  __shared__ double Gr[FE_EPB][2][NP];
+  __shared__ short sflag[PMAX];
+
+  const int bid = blockIdx.x;
+  const int t = (bid & 7) * T.per_xcd + (bid >> 3);
+  if (t >= T.ntiles) return;
+  const int tid = threadIdx.x;
+  const int4 tg = T.tiles[t];
+  const int4 gr = T.grids[t];
+  const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
+  const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
+  const int X0 = 4 * tx0;
+  const size_t S = T.NGp;
+  const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
+
+  R adr[NODE_ITERS][NDOF], avr[NODE_ITERS][NDOF];
+#pragma unroll
+  for (int r = 0; r < NODE_ITERS; ++r) {
+    const int p = tid + TILE_THREADS * r;
+    if (p < NPT) {
+      const int Z = p / PX, X = p - Z * PX;
+      const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
+#pragma unroll
+      for (int c = 0; c < NDOF; ++c) {
+        const R dk = ad[c * S + g], vk = av[c * S + g], ak = aa[c * S + g];
+        Uf[c][p] = (ak + c3 * vk) * rmass[c * S + g];
+        Ud[c][p] = dfw[c * S + g];
+        Fn[c][p] = 0;
+        adr[r][c] = dk;
+        avr[r][c] = vk;
+      }
+      sflag[p] = -1;
+    }
+  }
+  __syncthreads();
+  const int nspec = T.spec_cnt[t];
+  if (tid < nspec) sflag[T.spec_idx[t * MAXSPECIAL + tid]] = (short)tid;
+
+  const int le = tid / NP;
+  const int pe = tid - le * NP;
+  const int ii = pe % NG, jj = pe / NG;
+#pragma unroll 1
+  for (int cc = 0; cc < 4; ++cc) {
+    const int cx = cc & 1, cz = cc >> 1;
+    const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
+    const int nce = ncx * ncz;
+#pragma unroll 1
+    for (int b0 = 0; b0 < nce; b0 += FE_EPB) {
+      const int ce = b0 + le;
+      const bool active = (tid < FE_THREADS) && (ce < nce);
+      int base = 0;
+      size_t cell = 0;
+      if (active) {
+        const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
+        base = 4 * lz * PX + 4 * lx;
+        cell = cell_index(gr, tx0, tz0, lx, lz);
+        const R w = (R)c_W2[pe];
+        const R g0 = geom[cell * 3 + 0], g1 = geom[cell * 3 + 1], g2 = geom[cell * 3 + 2];
+        const R la = coefs[cell * 2 * NP + pe], mu = coefs[cell * 2 * NP + NP + pe];
+        const R kx = la + 2 * mu;
+        const R A0 = w * kx * g0, A1 = w * la * g2, A2 = w * kx * g1;
+        const R A3 = w * mu * g1, A4 = w * mu * g2, A5 = w * mu * g0;
+        R pxi[NDOF], peta[NDOF], gxi[NDOF], geta[NDOF];
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          R s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+#pragma unroll
+          for (int k = 0; k < NG; ++k) {
+            const R hx = Coef<R>::h(k, ii), he = Coef<R>::h(k, jj);
+            s1 += hx * Uf[c][base + jj * PX + k];
+            s2 += Uf[c][base + k * PX + ii] * he;
+            s3 += hx * Ud[c][base + jj * PX + k];
+            s4 += Ud[c][base + k * PX + ii] * he;
+          }
+          pxi[c] = s1;
+          peta[c] = s2;
+          gxi[c] = s3;
+          geta[c] = s4;
+        }
+        const R pxx = pxi[0], pex = peta[0], pxz = pxi[1], pez = peta[1];
+        const R gxx = gxi[0], gex = geta[0], gxz = gxi[1], gez = geta[1];
+        T1[le][0][pe] = A0 * pxx + A1 * pez;
+        T2[le][0][pe] = A3 * pex + A4 * pxz;
+        T1[le][1][pe] = A3 * pex + A5 * pxz;
+        T2[le][1][pe] = A1 * pxx + A2 * pez;
+        const double wd = (double)w;
+        Gr[le][0][pe] = wd * ((double)g0 * pxx * gxx + (double)g2 * (pxx * gez + pez * gxx) + (double)g1 * pez * gez);
+        Gr[le][1][pe] = wd * (2.0 * g0 * pxx * gxx + 2.0 * g1 * pez * gez + (double)g1 * pex * (gex + gxz) +
+                              (double)g2 * pxz * gex + (double)g0 * pxz * gxz);
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          R s1 = 0, s2 = 0;
+#pragma unroll
+          for (int k = 0; k < NG; ++k) {
+            s1 += Coef<R>::h(ii, k) * T1[le][c][jj * NG + k];
+            s2 += T2[le][c][k * NG + ii] * Coef<R>::h(jj, k);
+          }
+          R* fp = &Fn[c][base + jj * PX + ii];
+          *fp = *fp + (s1 + s2);
+        }
+        if (pe == 0) {  // fixed-order sum over the element's points
+          double sl = 0, sm = 0;
+          for (int q = 0; q < NP; ++q) {
+            sl += Gr[le][0][q];
+            sm += Gr[le][1][q];
+          }
+          glam[cell] += sl;
+          gmu[cell] += sm;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  const size_t fbase = (size_t)t * SLOTS;
+#pragma unroll
+  for (int r = 0; r < NODE_ITERS; ++r) {
+    const int p = tid + TILE_THREADS * r;
+    if (p < NPT) {
+      const int Z = p / PX, X = p - Z * PX;
+      int slot = perim_slot(X, Z, PX, PZ);
+      if (slot < 0 && sflag[p] >= 0) slot = PERIM_MAX + sflag[p];
+      if (slot >= 0) {
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = Fn[c][p];
+      } else {
+        const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
+        double mb = 0;
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          const R dn = adr[r][c] + Fn[c][p];
+          const R vbp = avr[r][c];
+          ad[c * S + g] = dn;
+          av[c * S + g] = vbp + dt * dn;
+          aa[c * S + g] = c2 * vbp + c1 * dn;
+          mb -= (double)Uf[c][p] * (double)afw[c * S + g];
+        }
+        mbar[g] += mb;
+      }
+    }
+  }
+}
+
+// Adjoint step at the deferred nodes (tile perimeters, source nodes): the partial
+// forces -K^T fb of the tiles, the absorbing term, the mass and ABC sensitivities.
+// gabs[c][q] accumulates dJ/dC_c of the node (for nodes with absorbing entries):
+// -fb_c (vp_c + c3 a_c), vp = v_{n-1} + c2 a_{n-1} the forward predicted velocity.
+template <typename R>
+__global__ __launch_bounds__(256) void k_adj_deferred(int ndef, const DefNode* __restrict__ defs,
+                                                      const int* __restrict__ part_off, const R* __restrict__ fpart,
+                                                      size_t FPS, const double* __restrict__ abso,
+                                                      const R* __restrict__ rmdef, R* __restrict__ ad,
+                                                      R* __restrict__ av, R* __restrict__ aa, size_t GS,
+                                                      const R* __restrict__ afw, const R* __restrict__ vprev,
+                                                      const R* __restrict__ aprev, double* __restrict__ mbar,
+                                                      double* __restrict__ gabs, R dt, R c1, R c2, R c3) {
+  constexpr int NDOF = 2;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ndef) return;
+  const DefNode n = defs[q];
+  const size_t g = n.g;
+  double mb = 0;
+#pragma unroll
+  for (int c = 0; c < NDOF; ++c) {
+    const R dk = ad[c * GS + g], vk = av[c * GS + g], ak = aa[c * GS + g];
+    const R fb = (ak + c3 * vk) * rmdef[(size_t)c * ndef + q];
+    R F = 0;
+    for (int k = 0; k < n.pcnt; ++k) F = F + fpart[c * FPS + part_off[n.pstart + k]];
+    double Cc = 0;
+    for (int k = 0; k < n.acnt; ++k) Cc += abso[(size_t)(n.astart + k) * NDOF + c];
+    const R vbp = (R)((double)vk - Cc * (double)fb);
+    const R dn = dk + F;
+    ad[c * GS + g] = dn;
+    av[c * GS + g] = vbp + dt * dn;
+    aa[c * GS + g] = c2 * vbp + c1 * dn;
+    const double an = (double)afw[c * GS + g];
+    mb -= (double)fb * an;
+    if (n.acnt > 0) {
+      const double vp = (double)vprev[c * GS + g] + (double)c2 * (double)aprev[c * GS + g];
+      gabs[(size_t)c * ndef + q] -= (double)fb * (vp + (double)c3 * an);
+    }
+  }
+  mbar[g] += mb;
+}
+
+// dJ/d(recorded field) of one sample added at the receiver nodes (one thread: nodes
+// may repeat)
+template <typename R>
+__global__ void k_adj_inject(int nrec, const int* __restrict__ node, const double* __restrict__ g, R* __restrict__ fld,
+                             size_t GS) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int r = 0; r < nrec; ++r)
+    for (int c = 0; c < 2; ++c) fld[c * GS + node[r]] = (R)((double)fld[c * GS + node[r]] + g[(size_t)c * nrec + r]);
+}
+
 }  // namespace sem2d

@@ -25,10 +25,10 @@ def _p(a, t=np.float64):
 class Solver:
     FIELD_CODE = {"D": 0, "V": 1, "A": 2}
 
-    def __init__(self, pb, bulk_fp32=False, api=None, structured=True, compact=True):
+    def __init__(self, pb, bulk_fp32=False, api=None, structured=True, compact=True, keep_problem=True):
         self.api = api if api is not None else load_hip_lib()
         self.pb = pb
-        self.ndof, self.npoin = pb.ndof, pb.npoin
+        self.ndof, self.npoin, self.nelem = pb.ndof, pb.npoin, pb.nelem
         self._keep = []
         ibool = np.ascontiguousarray(pb.ibool, dtype=np.int32)
         hprime = np.ascontiguousarray(pb.hprime.T, dtype=np.float64)  # [j][i]
@@ -70,6 +70,8 @@ class Solver:
             raise SolverError(f"{self.api.prefix}create failed ({rc})")
         self.h = h
         self._setup()
+        if not keep_problem:
+            self.pb = None  # the device solver holds what it needs; free the host problem
 
     # ------------------------------------------------------------------
     def _check(self, rc, what):
@@ -185,6 +187,33 @@ class Solver:
         ms = C.c_double()
         self._check(self.api.last_step_ms(self.h, C.byref(ms)), "last_step_ms")
         return ms.value
+
+    # -- adjoint (sem2d_adjoint_*) ---------------------------------------
+    def adjoint_init(self, nsteps, seg=0):
+        self.adj_nsteps = int(nsteps)
+        self._check(self.api.adjoint_init(self.h, int(nsteps), int(seg)), "adjoint_init")
+
+    def adjoint_forward(self):
+        self._check(self.api.adjoint_forward(self.h), "adjoint_forward")
+
+    def adjoint_backward(self, gseis):
+        g = np.ascontiguousarray(gseis, dtype=np.float64)
+        if g.shape != (self.nsamp, self.ndof, self.nrec):
+            raise ValueError(f"gseis must have shape {(self.nsamp, self.ndof, self.nrec)}")
+        self._check(self.api.adjoint_backward(self.h, _p(g)), "adjoint_backward")
+
+    def adjoint_fetch(self):
+        """(glam, gmu) per element, gmass per node, gabso [ndof][npoin]."""
+        ne = self.nelem
+        glam, gmu = np.empty(ne), np.empty(ne)
+        gmass, gabso = np.empty(self.npoin), np.empty((self.ndof, self.npoin))
+        self._check(self.api.adjoint_fetch(self.h, _p(glam), _p(gmu), _p(gmass), _p(gabso)), "adjoint_fetch")
+        return glam, gmu, gmass, gabso
+
+    def adjoint_timing(self):
+        f, b = C.c_double(), C.c_double()
+        self._check(self.api.adjoint_timing(self.h, C.byref(f), C.byref(b)), "adjoint_timing")
+        return f.value, b.value
 
     def set_tile_config(self, threads, wg_per_cu):
         """Tile-kernel launch configuration (structured grids); results do not depend on it."""

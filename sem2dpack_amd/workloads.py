@@ -172,3 +172,81 @@ def build_rupture(sc, nt=3000):
     from . import builder, parfile
     par = parfile.parse_par(par_rupture(sc["nx"], sc["nz"], sc["a"], sc["seed"], nt=nt))
     return builder.build(par)
+
+
+# ---------------------------------------------------------------------------
+# Adjoint (waveform gradient) problem
+# ---------------------------------------------------------------------------
+# nx x nz elements of 100 m, free surface on top, absorbing bottom, left and right
+# boundaries; nsrc vertical point forces (Ricker f0, fired together, one source file)
+# 1 km below the surface (a quarter of the depth on small boxes); nrec velocity
+# receivers along the surface; fixed Dt.  Models are per-element (rho, lambda, mu):
+# the smooth background rho 2670, cp 6000, cs 3464 and a true model with seeded
+# Gaussian anomalies of up to +-5 % in rho, cp and cs.
+ADJ_BACKGROUND = (2670.0, 6000.0, 3464.0)
+
+
+def par_adjoint(nx, nz, nt, nrec, f0=3.0, onset=None, dx=100.0, dt="8.6d-4", src_file="sources.tab"):
+    lx, lz = nx * dx, nz * dx
+    onset = 1.2 / f0 if onset is None else onset
+    rho, cp, cs = ADJ_BACKGROUND
+    return (f"&GENERAL iexec=1, ngll=5, fmax={f0 * 2.5!r}, ndof=2, title='adjoint problem {nx}x{nz}', "
+            f"verbose='0000', ItInfo=100000 /\n"
+            f"&MESH_DEF method='CARTESIAN' /\n"
+            f"&MESH_CART xlim=0d0,{lx!r}, zlim=0d0,{lz!r}, nelem={nx},{nz} /\n"
+            f"&MATERIAL tag=1, kind='ELAST' /\n"
+            f"&MAT_ELASTIC rho={rho!r}, cp={cp!r}, cs={cs!r} /\n"
+            f"&BC_DEF tag=1, kind='ABSORB' /\n"
+            f"&BC_DEF tag=2, kind='ABSORB' /\n"
+            f"&BC_DEF tag=4, kind='ABSORB' /\n"
+            f"&TIME kind='newmark', NbSteps={nt}, Dt={dt} /\n"
+            f"&SRC_DEF stf='RICKER', mechanism='FORCE', file='{src_file}' /\n"
+            f"&STF_RICKER f0={f0!r}, onset={onset!r}, ampli=1d0 /\n"
+            f"&SRC_FORCE angle=0d0 /\n"
+            f"&REC_LINE number={nrec}, first={2 * dx!r},{lz!r}, last={lx - 2 * dx!r},{lz!r}, field='V' /\n")
+
+
+def write_adjoint_sources(path, nx, nz, nsrc, dx=100.0):
+    lx, lz = nx * dx, nz * dx
+    depth = min(1000.0, 0.25 * lz)
+    with open(path, "w") as fh:
+        for k in range(nsrc):
+            fh.write(f"{(k + 0.5) * lx / nsrc!r} {lz - depth!r}\n")
+
+
+def adjoint_models(nx, nz, seed=20260501, amp=0.05, nblob=8, dx=100.0):
+    """(background, true) models, each (rho, lam, mu) over the cells (iz*nx + ix)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lx, lz = nx * dx, nz * dx
+    xc = (np.arange(nx) + 0.5) * dx
+    zc = (np.arange(nz) + 0.5) * dx
+    X, Z = np.meshgrid(xc, zc)
+    fields = []
+    for _ in range(3):
+        g = np.zeros_like(X)
+        for _b in range(nblob):
+            cx, cz = rng.uniform(0, lx), rng.uniform(0, lz)
+            w = rng.uniform(0.04, 0.08) * max(lx, lz)
+            g += rng.uniform(-1.0, 1.0) * np.exp(-((X - cx) ** 2 + (Z - cz) ** 2) / w ** 2)
+        fields.append(amp * g / np.abs(g).max())
+    rho0, cp0, cs0 = ADJ_BACKGROUND
+
+    def lame(rho, cp, cs):
+        return rho.ravel(), (rho * (cp * cp - 2.0 * cs * cs)).ravel(), (rho * cs * cs).ravel()
+
+    one = np.ones_like(X)
+    back = lame(rho0 * one, cp0 * one, cs0 * one)
+    true = lame(rho0 * (1 + fields[0]), cp0 * (1 + fields[1]), cs0 * (1 + fields[2]))
+    return back, true
+
+
+def adjoint_setup(workdir, nx=400, nz=400, nt=3000, nsrc=16, nrec=64, f0=3.0, onset=None, seed=20260501):
+    """Write the input of the adjoint problem to workdir; returns (par_path, background, true)."""
+    import os
+    os.makedirs(workdir, exist_ok=True)
+    write_adjoint_sources(os.path.join(workdir, "sources.tab"), nx, nz, nsrc)
+    path = os.path.join(workdir, "Par.inp")
+    with open(path, "w") as fh:
+        fh.write(par_adjoint(nx, nz, nt, nrec, f0=f0, onset=onset))
+    back, true = adjoint_models(nx, nz, seed=seed)
+    return path, back, true

@@ -31,9 +31,9 @@ def test_hip_library_exports_all():
 
 def test_oracle_exports_all(oracle_api):
     """The oracle restates the single-scenario path; event timing, the kernel launch
-    configuration and the batched ensemble exist only in the HIP library."""
+    configuration, the adjoint and the batched ensemble exist only in the HIP library."""
     for n in declared():
-        if n[len("sem2d_"):] in capi.GPU_ONLY or n.startswith("sem2d_batch_"):
+        if n[len("sem2d_"):] in capi.GPU_ONLY or n[len("sem2d_"):].startswith(capi.GPU_ONLY_PREFIXES):
             continue
         assert hasattr(oracle_api.lib, "oracle_" + n[len("sem2d_"):]), n
 

@@ -14,7 +14,7 @@ from conftest import GOLDEN, case_par
 from sem2dpack_amd import builder, solver
 import refdump  # noqa: F401  (fixture readers live beside the tests)
 
-CASES = ["small_psv", "c1_psv", "sh_swf", "psv_rsf", "hete_psv", "rsf_ens"]
+CASES = ["small_psv", "c1_psv", "sh_swf", "psv_rsf", "hete_psv", "rsf_ens", "multi_src"]
 
 
 def sha(a):
