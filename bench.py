@@ -163,14 +163,93 @@ def run_batch(args):
     print(json.dumps(line))
 
 
+def run_adjoint(args):
+    """Waveform gradient with respect to per-element (rho, lambda, mu): one gradient =
+    checkpointed forward + segment replays + adjoint sweep.  Reported against a plain
+    forward run of the same problem, and checked against central differences along
+    random directions (FP64 and FP32 bulk)."""
+    import numpy as np
+    from sem2dpack_amd import adjoint, solver, workloads
+    nt = args.steps
+    work = tempfile.mkdtemp(prefix="sem2d_adj_")
+    path, back, true = workloads.adjoint_setup(work, nx=400, nz=400, nt=nt, nsrc=16, nrec=64, f0=3.0)
+    t0 = time.time()
+    model = adjoint.SeismogramModel(path)
+    obs = model.forward(*true)
+    pb = model.problem(*back)
+    build_s = time.time() - t0
+    print(f"[adjoint] problem: npoin={pb.npoin} nelem={pb.nelem} nt={nt}", file=sys.stderr, flush=True)
+    # plain forward (the reference's time loop) on the background model
+    s = solver.Solver(pb)
+    s.step(5)
+    s.close()
+    s = solver.Solver(pb)
+    t1 = time.perf_counter()
+    s.step(nt)
+    t_fwd = time.perf_counter() - t1
+    fwd_ev_ms = s.last_step_ms() * nt
+    s.close()
+    out = {}
+    for fp32 in (False, True):
+        m = adjoint.SeismogramModel(path, bulk_fp32=fp32)
+        g_s = solver.Solver(pb, bulk_fp32=fp32)
+        m._state = (pb, g_s, tuple(np.ascontiguousarray(a) for a in back))
+        t2 = time.perf_counter()
+        g_s.adjoint_init(nt, 0)
+        g_s.adjoint_forward()
+        seis = g_s.seismograms()
+        grad = m.backward(seis - obs)
+        t_grad = time.perf_counter() - t2
+        out[fp32] = dict(grad=grad, t=t_grad, ev=m.last_timing, J=0.5 * float(np.sum((seis - obs) ** 2)))
+        m.close()
+        print(f"[adjoint] fp32={fp32}: gradient in {t_grad:.2f} s (forward {t_fwd:.2f} s)", file=sys.stderr,
+              flush=True)
+    # central differences along random directions (FP64 forward)
+    rng = np.random.Generator(np.random.PCG64(20260502))
+    fdm = adjoint.SeismogramModel(path)
+    errs64, errs32 = [], []
+    eps = 1e-2
+    for k in range(args.fd_dirs):
+        dm = [rng.standard_normal(p.size) * p * 1e-2 for p in back]
+        Jp = 0.5 * float(np.sum((fdm.forward(*[p + eps * q for p, q in zip(back, dm)]) - obs) ** 2))
+        Jm = 0.5 * float(np.sum((fdm.forward(*[p - eps * q for p, q in zip(back, dm)]) - obs) ** 2))
+        fd = (Jp - Jm) / (2 * eps)
+        for fp32, errs in ((False, errs64), (True, errs32)):
+            an = sum(np.dot(a, b) for a, b in zip(out[fp32]["grad"], dm))
+            errs.append(abs(an - fd) / abs(fd))
+        print(f"[adjoint] direction {k}: FD {fd:.6e} rel err fp64 {errs64[-1]:.2e} fp32 {errs32[-1]:.2e}",
+              file=sys.stderr, flush=True)
+    fdm.close()
+    t_grad = out[False]["t"]
+    ev_f, ev_b = out[False]["ev"]
+    line = {
+        "metric": "adjoint gradients/s", "value": 1.0 / t_grad, "unit": "gradients/s",
+        "n_gpus": 1, "steps": nt, "warmup": 0, "ms_per_step": 1e3 * t_grad / nt,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "P-SV 400x400 elements NGLL=5, 16 simultaneous Ricker forces (3 Hz), 64 surface "
+                               "receivers, L2 waveform misfit, gradient w.r.t. per-element rho/lambda/mu",
+                   "npoin": pb.npoin, "nelem": pb.nelem, "checkpoint_segment": int(np.ceil(np.sqrt(nt))),
+                   "build_s": round(build_s, 1)},
+        "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
+        "device_ms": {"forward_plain": fwd_ev_ms, "checkpointed_forward": ev_f, "replay_and_adjoint": ev_b},
+        "fp32_bulk": {"gradient_s": out[True]["t"], "gradient_over_forward": out[True]["t"] / t_fwd},
+        "fd_check": {"directions": args.fd_dirs, "eps_rel": eps * 1e-2,
+                     "max_rel_err_fp64": max(errs64) if errs64 else None,
+                     "max_rel_err_fp32": max(errs32) if errs32 else None},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["throughput", "batch"], default="throughput",
+    ap.add_argument("--workload", choices=["throughput", "batch", "adjoint"], default="throughput",
                     help="throughput: the 2000x1000-element P-SV case (default); batch: the "
-                         "rupture ensemble, batched vs one by one")
+                         "rupture ensemble, batched vs one by one; adjoint: waveform gradients")
+    ap.add_argument("--fd-dirs", type=int, default=20, help="adjoint: finite-difference directions")
     ap.add_argument("--scenarios", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
@@ -199,6 +278,10 @@ def main():
     if args.workload == "batch":
         args.steps = 3000 if args.steps is None else args.steps
         run_batch(args)
+        return
+    if args.workload == "adjoint":
+        args.steps = 3000 if args.steps is None else args.steps
+        run_adjoint(args)
         return
     args.steps = 1000 if args.steps is None else args.steps
 

@@ -135,6 +135,11 @@ int sem2d_fetch_seismo(sem2d_solver *h, double *out, int *nsamp_taken);
    with nout = (oixn-oix1)/oixd + 1. */
 int sem2d_fetch_fault(sem2d_solver *h, double *out, int *nsamp_taken);
 
+/* Fault potency rows [nrows][2*(ndof+1)], one per step from it = 0: P-SV p11, p22, p12
+   of the displacement jump then of the velocity jump, SH p13, p23 of each -- the
+   FltXX_potency_sem2d.tab lines of BC_DYNFLT_write (SRC/bc_dynflt.f90:757,832-855). */
+int sem2d_fetch_potency(sem2d_solver *h, double *out, int *nrows);
+
 /* Current fault state of all pairs: D, V [ndof][npoin], T, Tstick [2][npoin], MU [npoin]. */
 int sem2d_fetch_fault_state(sem2d_solver *h, double *D, double *V, double *T, double *MU,
                             double *Tstick);

@@ -23,6 +23,7 @@
  *     normal             normal_update kind 1  SRC/bc_dynflt_normal.f90:118-136
  *   receivers            REC_store (AtNode)   SRC/receivers.f90:309-344
  *   fault records        BC_DYNFLT_write      SRC/bc_dynflt.f90:751-778
+ *   potency              BC_DYNFLT_potency    SRC/bc_dynflt.f90:832-855
  *
  * Known deviation: in the rate-and-state branch the reference leaves the local
  * Tstick array unset (SRC/bc_dynflt.f90:620-636 never assigns it); here Tstick is
@@ -68,6 +69,8 @@ typedef struct {
   int oit, oitd, oix1, oixn, oixd, nout;
   int nsamp, taken;
   double *rec; /* [nsamp][6][nout] */
+  int pot_rows, pot_cap;
+  double *pot; /* [rows][2*(ndof+1)] potency of d then v, one row per step from it = 0 */
 } fault_t;
 
 struct oracle_solver {
@@ -309,8 +312,44 @@ static double rsf_v1(const fault_t *f, int k, double tstick, double sigma, doubl
   return nr_solver(f, k, lo, hi, tol, theta, tstick, sigma, Z, ok);
 }
 
+/* BC_DYNFLT_potency (SRC/bc_dynflt.f90:832-855) of the field x, jump = x(node2) - x(node1):
+   P-SV p11, p22, p12; SH p13, p23 (sequential sums, as Fortran's SUM) */
+static void potency(const oracle_solver *h, const double *x, double *p) {
+  const fault_t *f = h->flt;
+  const int np = f->np, N = h->npoin;
+  const double *n1x = f->nrm, *n1z = f->nrm + np, *B = f->B;
+  if (h->ndof == 2) {
+    double s1 = 0, s2 = 0, s3 = 0;
+    for (int k = 0; k < np; ++k) s1 += n1x[k] * (x[f->n2[k]] - x[f->n1[k]]) * B[k];
+    for (int k = 0; k < np; ++k) s2 += n1z[k] * (x[N + f->n2[k]] - x[N + f->n1[k]]) * B[k];
+    for (int k = 0; k < np; ++k)
+      s3 += (n1x[k] * (x[N + f->n2[k]] - x[N + f->n1[k]]) + n1z[k] * (x[f->n2[k]] - x[f->n1[k]])) * B[k];
+    p[0] = s1; p[1] = s2; p[2] = 0.5 * s3;
+  } else {
+    double s1 = 0, s2 = 0;
+    for (int k = 0; k < np; ++k) s1 += n1x[k] * (x[f->n2[k]] - x[f->n1[k]]) * B[k];
+    for (int k = 0; k < np; ++k) s2 += n1z[k] * (x[f->n2[k]] - x[f->n1[k]]) * B[k];
+    p[0] = 0.5 * s1; p[1] = 0.5 * s2;
+  }
+}
+
+/* one potency row per call (BC_DYNFLT_write, SRC/bc_dynflt.f90:757) */
+static void potency_row(oracle_solver *h) {
+  fault_t *f = h->flt;
+  const int nc = 2 * (h->ndof + 1);
+  if (f->pot_rows == f->pot_cap) {
+    f->pot_cap = f->pot_cap ? 2 * f->pot_cap : 1024;
+    f->pot = (double *)realloc(f->pot, sizeof(double) * nc * f->pot_cap);
+  }
+  double *r = f->pot + (size_t)f->pot_rows * nc;
+  potency(h, h->d, r);
+  potency(h, h->v, r + h->ndof + 1);
+  f->pot_rows++;
+}
+
 static void fault_record(oracle_solver *h) {
   fault_t *f = h->flt;
+  potency_row(h);
   if (h->it < f->oit) return;
   if (f->taken < f->nsamp) {
     double *r = f->rec + (size_t)f->taken * 6 * f->nout;
@@ -659,6 +698,13 @@ int oracle_fetch_fault_state(oracle_solver *h, double *D, double *V, double *T, 
   return 0;
 }
 
+int oracle_fetch_potency(oracle_solver *h, double *out, int *nrows) {
+  if (!h->flt) return fail(h, 4, "no fault");
+  if (out) memcpy(out, h->flt->pot, sizeof(double) * h->flt->pot_rows * 2 * (h->ndof + 1));
+  if (nrows) *nrows = h->flt->pot_rows;
+  return 0;
+}
+
 int oracle_get_step(oracle_solver *h, int *it) { *it = h->it; return 0; }
 const char *oracle_last_error(oracle_solver *h) { return h ? h->err : "null handle"; }
 
@@ -677,7 +723,7 @@ void oracle_destroy(oracle_solver *h) {
     free(f->Z); free(f->T0); free(f->coh);
     for (int q = 0; q < 6; ++q) free(f->p[q]);
     free(f->theta); free(f->Tc); free(f->coeft); free(f->D); free(f->V); free(f->T);
-    free(f->Tstick); free(f->MU); free(f->sigma); free(f->rec); free(f);
+    free(f->Tstick); free(f->MU); free(f->sigma); free(f->rec); free(f->pot); free(f);
   }
   free(h);
 }

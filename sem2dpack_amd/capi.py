@@ -47,6 +47,7 @@ _SIGS = {
     "fetch_seismo": ([C.c_void_p, dp, ip], C.c_int),
     "fetch_fault": ([C.c_void_p, dp, ip], C.c_int),
     "fetch_fault_state": ([C.c_void_p, dp, dp, dp, dp, dp], C.c_int),
+    "fetch_potency": ([C.c_void_p, dp, ip], C.c_int),
     "compute_fint": ([C.c_void_p, dp, dp], C.c_int),
     "set_tile_config": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "get_step": ([C.c_void_p, ip], C.c_int),

@@ -250,6 +250,60 @@ __global__ void k_fault_record(FaultDev F, int oix1, int oixd, int nout, double*
   out[5 * nout + o] = F.Tstick[k];
 }
 
+// Potency of the displacement and velocity jumps (BC_DYNFLT_potency,
+// SRC/bc_dynflt.f90:832-855): P-SV p11, p22, p12 of d then of v, SH p13, p23 of d
+// then of v, one row of 2*(ndof+1) values.  One workgroup of 256 lanes: strided
+// partial sums, then a fixed-shape tree, so the result is run-to-run identical.
+template <typename R>
+__global__ __launch_bounds__(256) void k_fault_potency(FaultDev F, const R* __restrict__ d, const R* __restrict__ v,
+                                                       size_t stride, double* __restrict__ out) {
+  __shared__ double red[6][256];
+  const int t = threadIdx.x, np = F.np;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = t; k < np; k += 256) {
+    const int i1 = F.n1[k], i2 = F.n2[k];
+    const double nx = F.nrm[k], nz = F.nrm[np + k], b = F.B[k];
+    const double jd0 = (double)d[i2] - (double)d[i1], jv0 = (double)v[i2] - (double)v[i1];
+    if (F.ndof == 2) {
+      const double jd1 = (double)d[stride + i2] - (double)d[stride + i1];
+      const double jv1 = (double)v[stride + i2] - (double)v[stride + i1];
+      s[0] += nx * jd0 * b;
+      s[1] += nz * jd1 * b;
+      s[2] += (nx * jd1 + nz * jd0) * b;
+      s[3] += nx * jv0 * b;
+      s[4] += nz * jv1 * b;
+      s[5] += (nx * jv1 + nz * jv0) * b;
+    } else {
+      s[0] += nx * jd0 * b;
+      s[1] += nz * jd0 * b;
+      s[2] += nx * jv0 * b;
+      s[3] += nz * jv0 * b;
+    }
+  }
+  for (int q = 0; q < 6; ++q) red[q][t] = s[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+      for (int q = 0; q < 6; ++q) red[q][t] = red[q][t] + red[q][t + w];
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (F.ndof == 2) {
+      out[0] = red[0][0];
+      out[1] = red[1][0];
+      out[2] = 0.5 * red[2][0];
+      out[3] = red[3][0];
+      out[4] = red[4][0];
+      out[5] = 0.5 * red[5][0];
+    } else {
+      out[0] = 0.5 * red[0][0];
+      out[1] = 0.5 * red[1][0];
+      out[2] = 0.5 * red[2][0];
+      out[3] = 0.5 * red[3][0];
+    }
+  }
+}
+
 #pragma clang fp contract(on)
 
 }  // namespace sem2d

@@ -84,6 +84,8 @@ struct FaultHost {
   std::vector<void*> allocs;
   int oit, oitd, oix1, oixn, oixd, nout, nsamp, taken;
   double* rec;  // device [nsamp][6][nout]
+  double* pot = nullptr;  // device [pot_cap][2*(ndof+1)] potency rows, one per step from it = 0
+  int pot_rows = 0, pot_cap = 0;
 };
 
 }  // namespace
@@ -326,9 +328,38 @@ int record_receivers(sem2d_solver* h) {
   return 0;
 }
 
+// room for `rows` potency rows (grown outside any step loop)
+int potency_reserve(sem2d_solver* h, int rows) {
+  FaultHost& F = h->flt;
+  if (rows <= F.pot_cap) return 0;
+  const int cap = std::max(rows, 2 * F.pot_cap);
+  const size_t nc = 2 * (h->ndof + 1);
+  double* p = nullptr;
+  HIPCHK(h, hipMalloc(&p, (size_t)cap * nc * sizeof(double)));
+  if (F.pot && F.pot_rows)
+    HIPCHK(h, hipMemcpyAsync(p, F.pot, (size_t)F.pot_rows * nc * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (F.pot) hipFree(F.pot);
+  F.pot = p;
+  F.pot_cap = cap;
+  return 0;
+}
+
+template <typename R>
+void record_potency(sem2d_solver* h) {
+  FaultHost& F = h->flt;
+  if (F.pot_rows >= F.pot_cap) return;  // reserved by the caller
+  k_fault_potency<R><<<1, 256, 0, h->stream>>>(F.dev, (const R*)h->d, (const R*)h->v, h->fstride,
+                                               F.pot + (size_t)F.pot_rows * 2 * (h->ndof + 1));
+  F.pot_rows++;
+}
+
 int record_fault(sem2d_solver* h) {
   FaultHost& F = h->flt;
-  if (!F.on || h->it < F.oit) return 0;
+  if (!F.on) return 0;
+  if (h->fp32) record_potency<float>(h);
+  else record_potency<double>(h);
+  if (h->it < F.oit) return 0;
   if (F.taken < F.nsamp) {
     k_fault_record<<<grid_for(F.nout, 64), 64, 0, h->stream>>>(F.dev, F.oix1, F.oixd, F.nout,
                                                                F.rec + (size_t)F.taken * 6 * F.nout);
@@ -744,6 +775,7 @@ int do_steps(sem2d_solver* h, int nsteps) {
   const R c3 = (R)(h->gamma * h->dt), c4 = (R)(h->beta * h->dt * h->dt);
   const int gb = grid_for(n, 256);
   if (h->st.on && !h->st.ready && (rc = prepare_structured(h))) return rc;
+  if (h->flt.on && (rc = potency_reserve(h, h->flt.pot_rows + nsteps))) return rc;
   StepDev X;
   if (h->st.on) X = step_dev(h);
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
@@ -1126,6 +1158,7 @@ int sem2d_set_fault(sem2d_solver* h, const sem2d_fault_desc* f, int nsamp) {
   F.taken = 0;
   if ((rc = dev_alloc(h, &F.rec, (size_t)nsamp * 6 * F.nout))) return rc;
   F.on = true;
+  if ((rc = potency_reserve(h, 1024))) return rc;
   record_fault(h);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -1193,6 +1226,16 @@ int sem2d_fetch_fault(sem2d_solver* h, double* out, int* taken) {
     HIPCHK(h, hipMemcpy(out, h->flt.rec, (size_t)h->flt.taken * 6 * h->flt.nout * sizeof(double),
                         hipMemcpyDeviceToHost));
   if (taken) *taken = h->flt.taken;
+  return 0;
+}
+
+int sem2d_fetch_potency(sem2d_solver* h, double* out, int* nrows) {
+  if (!h) return 1;
+  if (!h->flt.on) return set_err(h, 4, "fetch_potency: no fault");
+  if (out && h->flt.pot_rows)
+    HIPCHK(h, hipMemcpy(out, h->flt.pot, (size_t)h->flt.pot_rows * 2 * (h->ndof + 1) * sizeof(double),
+                        hipMemcpyDeviceToHost));
+  if (nrows) *nrows = h->flt.pot_rows;
   return 0;
 }
 
@@ -1265,6 +1308,7 @@ void sem2d_destroy(sem2d_solver* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
   if (h->adj) adj_free(h->adj);
+  if (h->flt.pot) hipFree(h->flt.pot);
   for (void* p : h->allocs) hipFree(p);
   for (void* p : h->st.prep_allocs) hipFree(p);
   for (auto& b : h->abso) { (void)b; }
@@ -2133,6 +2177,7 @@ int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
   if (!h->st.ready && (rc = sync_sources(h))) return rc;
   if (!h->st.ready && (rc = prepare_structured(h))) return rc;
   if (h->adj) adj_free(h->adj);
+  if (h->flt.pot) hipFree(h->flt.pot);
   h->adj = new AdjState();
   AdjState& A = *h->adj;
   A.nsteps = nsteps;

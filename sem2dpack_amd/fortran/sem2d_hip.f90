@@ -37,8 +37,15 @@ module sem2d_hip
   public :: sem2d_create, sem2d_set_abso, sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
             sem2d_set_receivers, sem2d_set_fault, sem2d_step, sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
-            sem2d_compute_fint, sem2d_get_step, sem2d_last_step_ms, sem2d_last_error, &
-            sem2d_destroy, sem2d_error_message
+            sem2d_fetch_potency, &
+            sem2d_compute_fint, sem2d_set_tile_config, sem2d_get_step, sem2d_last_step_ms, &
+            sem2d_last_error, sem2d_destroy, sem2d_error_message, &
+            sem2d_adjoint_init, sem2d_adjoint_forward, sem2d_adjoint_backward, sem2d_adjoint_fetch, &
+            sem2d_adjoint_timing, &
+            sem2d_batch_create, sem2d_batch_step, sem2d_batch_tune, sem2d_batch_set_config, &
+            sem2d_batch_config, sem2d_batch_fetch_fields, sem2d_batch_fetch_seismo, &
+            sem2d_batch_fetch_fault, sem2d_batch_fetch_fault_state, sem2d_batch_get_step, &
+            sem2d_batch_last_step_ms, sem2d_batch_last_error, sem2d_batch_destroy
 
   interface
     integer(c_int) function sem2d_create(desc, h) bind(C, name='sem2d_create')
@@ -134,11 +141,24 @@ module sem2d_hip
       real(c_double), intent(out) :: D(*), V(*), T(*), MU(*), Tstick(*)
     end function
 
+    integer(c_int) function sem2d_fetch_potency(h, out, nrows) bind(C, name='sem2d_fetch_potency')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), intent(out) :: nrows
+    end function
+
     integer(c_int) function sem2d_compute_fint(h, d, f) bind(C, name='sem2d_compute_fint')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(in) :: d(*)
       real(c_double), intent(out) :: f(*)
+    end function
+
+    integer(c_int) function sem2d_set_tile_config(h, threads, wg_per_cu) bind(C, name='sem2d_set_tile_config')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: threads, wg_per_cu
     end function
 
     integer(c_int) function sem2d_get_step(h, it) bind(C, name='sem2d_get_step')
@@ -161,6 +181,122 @@ module sem2d_hip
     subroutine sem2d_destroy(h) bind(C, name='sem2d_destroy')
       import :: c_ptr
       type(c_ptr), value :: h
+    end subroutine
+
+    ! adjoint of the time loop
+    integer(c_int) function sem2d_adjoint_init(h, nsteps, seg) bind(C, name='sem2d_adjoint_init')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps, seg
+    end function
+
+    integer(c_int) function sem2d_adjoint_forward(h) bind(C, name='sem2d_adjoint_forward')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    integer(c_int) function sem2d_adjoint_backward(h, gseis) bind(C, name='sem2d_adjoint_backward')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: gseis(*)
+    end function
+
+    integer(c_int) function sem2d_adjoint_fetch(h, glam, gmu, gmass, gabso) bind(C, name='sem2d_adjoint_fetch')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: glam(*), gmu(*), gmass(*), gabso(*)
+    end function
+
+    integer(c_int) function sem2d_adjoint_timing(h, fwd_ms, bwd_ms) bind(C, name='sem2d_adjoint_timing')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: fwd_ms, bwd_ms
+    end function
+
+    ! batched ensemble
+    integer(c_int) function sem2d_batch_create(n, members, b) bind(C, name='sem2d_batch_create')
+      import :: c_int, c_ptr
+      integer(c_int), value :: n
+      type(c_ptr), intent(in) :: members(*)
+      type(c_ptr), intent(out) :: b
+    end function
+
+    integer(c_int) function sem2d_batch_step(b, nsteps) bind(C, name='sem2d_batch_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: nsteps
+    end function
+
+    integer(c_int) function sem2d_batch_tune(b, trial_steps, choice) bind(C, name='sem2d_batch_tune')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: trial_steps
+      integer(c_int), intent(out) :: choice
+    end function
+
+    integer(c_int) function sem2d_batch_set_config(b, threads, wg_per_cu) bind(C, name='sem2d_batch_set_config')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: threads, wg_per_cu
+    end function
+
+    integer(c_int) function sem2d_batch_config(b, threads, workgroups, ntiles) bind(C, name='sem2d_batch_config')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), intent(out) :: threads, workgroups, ntiles
+    end function
+
+    integer(c_int) function sem2d_batch_fetch_fields(b, m, d, v, a) bind(C, name='sem2d_batch_fetch_fields')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: m
+      real(c_double), intent(out) :: d(*), v(*), a(*)
+    end function
+
+    integer(c_int) function sem2d_batch_fetch_seismo(b, m, out, nsamp_taken) bind(C, name='sem2d_batch_fetch_seismo')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: m
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), intent(out) :: nsamp_taken
+    end function
+
+    integer(c_int) function sem2d_batch_fetch_fault(b, m, out, nsamp_taken) bind(C, name='sem2d_batch_fetch_fault')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: m
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), intent(out) :: nsamp_taken
+    end function
+
+    integer(c_int) function sem2d_batch_fetch_fault_state(b, m, D, V, T, MU, Tstick) &
+        bind(C, name='sem2d_batch_fetch_fault_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: m
+      real(c_double), intent(out) :: D(*), V(*), T(*), MU(*), Tstick(*)
+    end function
+
+    integer(c_int) function sem2d_batch_get_step(b, it) bind(C, name='sem2d_batch_get_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), intent(out) :: it
+    end function
+
+    integer(c_int) function sem2d_batch_last_step_ms(b, ms) bind(C, name='sem2d_batch_last_step_ms')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      real(c_double), intent(out) :: ms
+    end function
+
+    type(c_ptr) function sem2d_batch_last_error(b) bind(C, name='sem2d_batch_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: b
+    end function
+
+    subroutine sem2d_batch_destroy(b) bind(C, name='sem2d_batch_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: b
     end subroutine
   end interface
 

@@ -11,6 +11,8 @@
   stick traction of nodes ``oix1:oixn:oixd`` in single precision
   (``POST/python/sem2d_read_fault.py`` reads it); ``FltXX_sem2d.hdr`` and
   ``FltXX_init_sem2d.tab``.
+* Potency (``BC_DYNFLT_write`` / ``BC_DYNFLT_potency``, ``SRC/bc_dynflt.f90:757,
+  832-855``): ``FltXX_potency_sem2d.tab``, one formatted ``(6D24.16)`` line per step.
 
 Numbers in the text headers use Python formatting; the binary files are byte
 compatible with the reference's.
@@ -69,6 +71,37 @@ def write_fault(outdir, fault, records, dt, nt):
     with open(os.path.join(outdir, f"Flt{tag:02d}_init_sem2d.tab"), "w") as fh:
         for t0, tn, mu, b in zip(fault["T0"][sel, 0], fault["T0"][sel, 1], fault["MU0"][sel], fault["B"][sel, 0]):
             fh.write(f" {t0!r} {tn!r} {mu!r} {b!r}\n")
+
+
+def fortran_d(x, w=24, d=16):
+    """Fortran Dw.d edit descriptor: 0.ddd...D+ee (sign, leading zero, d digits)."""
+    if x == 0.0:
+        body = "0." + "0" * d + "D+00"
+    else:
+        m, e = f"{abs(x):.{d - 1}e}".split("e")
+        digits = m.replace(".", "")
+        ex = int(e) + 1
+        exp = f"D{ex:+03d}" if abs(ex) < 100 else f"{ex:+04d}"
+        body = ("-" if x < 0 else "") + "0." + digits + exp
+    return body.rjust(w)
+
+
+def write_potency(outdir, tag, pot):
+    """FltXX_potency_sem2d.tab: one line per row, format (6D24.16) as BC_DYNFLT_write
+    (SRC/bc_dynflt.f90:757); a row of 4 values (SH) fills one line of 4 fields."""
+    with open(os.path.join(outdir, f"Flt{tag:02d}_potency_sem2d.tab"), "w") as fh:
+        for row in pot:
+            for k in range(0, len(row), 6):
+                fh.write("".join(fortran_d(float(x)) for x in row[k:k + 6]) + "\n")
+
+
+def read_potency(path):
+    """Values of a potency file (D exponents), one row per line."""
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            rows.append([float(line[i:i + 24].replace("D", "E")) for i in range(0, len(line.rstrip("\n")), 24)])
+    return rows
 
 
 def read_fault(path, nout, ndat=6):

@@ -170,6 +170,14 @@ class Solver:
         self._check(self.api.fetch_fault(self.h, _p(out), C.byref(n)), "fetch_fault")
         return out[:n.value]
 
+    def potency(self):
+        """[nrows][2*(ndof+1)] potency rows, one per step from it = 0."""
+        n = C.c_int()
+        self._check(self.api.fetch_potency(self.h, None, C.byref(n)), "fetch_potency")
+        out = np.empty((max(n.value, 1), 2 * (self.ndof + 1)))
+        self._check(self.api.fetch_potency(self.h, _p(out), C.byref(n)), "fetch_potency")
+        return out[:n.value]
+
     def fault_state(self):
         nd, npf = self.ndof, self.npf
         D, V = np.empty((nd, npf)), np.empty((nd, npf))

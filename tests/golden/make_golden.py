@@ -15,6 +15,8 @@ directory under tests/golden/cases/ and stores compact FP64 vectors:
                                                         fields, every fault_every steps
   flt_f32 (fault cases)                                 the reference's own float32
                                                         FltXX_sem2d.dat records
+  potency_txt (fault cases)                             the reference's own
+                                                        FltXX_potency_sem2d.tab text
   seis_f32                                              the reference's own float32
                                                         U?_sem2d.dat traces
 
@@ -102,6 +104,10 @@ def run_case(name):
             # displacement and velocity jumps d(node2) - d(node1), xz frame
             out["fault_dd"] = nd["d"][sel][:, :, k2] - nd["d"][sel][:, :, k1]
             out["fault_dv"] = nd["v"][sel][:, :, k2] - nd["v"][sel][:, :, k1]
+            pot = os.path.join(tmp, "Flt05_potency_sem2d.tab")
+            if os.path.exists(pot):
+                with open(pot) as fh:
+                    out["potency_txt"] = np.array(fh.read())
             flt = os.path.join(tmp, "Flt05_sem2d.dat")
             bd = pf.read_par(os.path.join(src, "Par.inp")).first("BC_DYNFLT")
             oxi = [int(x) for x in bd.get("oxi", [1, 2 ** 31 - 1, 1])]

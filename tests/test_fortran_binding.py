@@ -35,3 +35,13 @@ def test_module_compiles_and_layout_matches(tmp_path):
     out = subprocess.run([str(tmp_path / "probe")], capture_output=True, text=True, check=True).stdout.split()
     assert int(out[0]) == ctypes.sizeof(capi.Desc)
     assert int(out[1]) == ctypes.sizeof(capi.FaultDesc)
+
+
+def test_module_declares_every_entry_point():
+    """Every function of include/sem2d.h has a bind(C) interface in the Fortran module."""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "sem2d.h")).read()
+    names = set(re.findall(r"\b(sem2d_[a-z_0-9]+)\s*\(", hdr))
+    mod = open(SRC).read()
+    bound = set(re.findall(r"bind\(C, name='(sem2d_[a-z_0-9]+)'\)", mod))
+    assert names == bound, names ^ bound

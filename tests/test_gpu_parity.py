@@ -70,6 +70,13 @@ def test_time_series(name, nsteps, oracle_api):
                 errs[nm], tols[nm] = np.linalg.norm(r_gpu[:, q] - r_ref[:, q]) / scale, TOL
             else:
                 errs[nm], tols[nm] = rel(r_gpu[:, q], r_ref[:, q]), TOL
+    if pb.fault is not None:
+        p_ref, p_gpu = ref.potency(), gpu.potency()
+        assert p_gpu.shape == p_ref.shape
+        # parallel (fixed tree) vs sequential sums: absolute error per column against the
+        # column's largest magnitude
+        scale = np.maximum(np.abs(p_ref).max(axis=0), 1e-300)
+        errs["potency"], tols["potency"] = float((np.abs(p_gpu - p_ref) / scale).max()), 1e-10
     print("  " + "  ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
     bad = {k: v for k, v in errs.items() if not v < tols[k]}
     assert not bad, bad

@@ -77,6 +77,23 @@ def test_oracle_matches_reference(name, oracle_api):
             assert np.array_equal(ours[:, cols], ref32[:, cols])
 
 
+@pytest.mark.parametrize("name", ["sh_swf", "psv_rsf", "rsf_ens"])
+def test_potency_file_matches_reference(name, oracle_api, tmp_path):
+    """FltXX_potency_sem2d.tab written from the oracle's potency rows is byte-identical to
+    the reference's file (BC_DYNFLT_potency sums, (6D24.16) formatting)."""
+    from sem2dpack_amd import outputs
+    g = np.load(os.path.join(GOLDEN, name + ".npz"))
+    nt = int(g["nt"])
+    pb = builder.build(case_par(name), nsteps=nt)
+    s = solver.Solver(pb, api=oracle_api, compact=False)
+    s.step(nt)
+    pot = s.potency()
+    assert pot.shape == (nt + 1, 2 * (pb.ndof + 1))
+    outputs.write_potency(str(tmp_path), int(pb.fault["tags"][0]), pot)
+    ours = open(tmp_path / "Flt05_potency_sem2d.tab").read()
+    assert ours == str(g["potency_txt"])
+
+
 @pytest.mark.parametrize("name", ["small_psv", "sh_swf"])
 def test_output_files_match_reference(name, oracle_api, tmp_path):
     """Seismogram and fault files written from the oracle's FP64 series are byte-compatible
