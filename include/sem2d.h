@@ -118,6 +118,14 @@ int sem2d_set_receivers(sem2d_solver *h, int nrec, const int *iglob, int isamp, 
    kept; a record is taken at registration when the current step >= oit. */
 int sem2d_set_fault(sem2d_solver *h, const sem2d_fault_desc *f, int nsamp);
 
+/* Receivers inside elements (REC_LINE AtNode=F; REC_posit/REC_store,
+   SRC/receivers.f90:274-285,326-341): element elem[r] (1-based) and its 25 weights
+   interp[r][k] = h_i(xi) h_j(eta), k = (j-1)*5 + i, as SE_init_interpol
+   (SRC/spec_grid.f90:385-406) computes them; a sample is sum_k interp[r][k] *
+   field(ibool(k, elem[r])).  Replaces any receivers set before. */
+int sem2d_set_receivers_interp(sem2d_solver *h, int nrec, const int *elem, const double *interp,
+                               int isamp, int field, int nsamp);
+
 /* Advance nsteps Newmark steps: replaces the loop body of SRC/main.f90:51-99
    (solve -> solve_Newmark, SRC/solver.f90:42-84, plus REC_store and BC_write records).
    May return before the device finishes; fetch calls synchronise. */

@@ -111,7 +111,9 @@ program ref_harness
   enddo
   allocate(recnodes(n))
   do k = 1, nrec
-    recnodes(k) = 0
+    ! receivers inside elements (AtNode=F) sit on no node: dump node 1 for them (their
+    ! series are taken from the reference's own seismogram files instead)
+    recnodes(k) = 1
     do i = 1, npoin
       if (pb%grid%coord(1,i) == rcoord(1,k) .and. pb%grid%coord(2,i) == rcoord(2,k)) then
         recnodes(k) = i

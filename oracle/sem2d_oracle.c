@@ -86,7 +86,8 @@ struct oracle_solver {
   int nsrc;
   source_t src[64];
   int nrec, isamp, field, nsamp, taken;
-  int *rnode;
+  int *rnode;        /* [nrec], or [nrec][25] element nodes when interpolated */
+  double *rw;        /* [nrec][25] interpolation weights, NULL for receivers at nodes */
   double *seis;
   fault_t *flt;
   int it;
@@ -178,9 +179,31 @@ static void rec_store(oracle_solver *h) {
   if (s >= h->nsamp) return;
   const double *f = rec_field(h);
   for (int c = 0; c < h->ndof; ++c)
-    for (int r = 0; r < h->nrec; ++r)
-      h->seis[((size_t)s * h->ndof + c) * h->nrec + r] = f[(size_t)c * h->npoin + h->rnode[r]];
+    for (int r = 0; r < h->nrec; ++r) {
+      double v;
+      if (h->rw) { /* matmul(interp, vloc): SRC/receivers.f90:329-339 */
+        v = 0.0;
+        for (int k = 0; k < NG * NG; ++k)
+          v += h->rw[(size_t)r * NG * NG + k] * f[(size_t)c * h->npoin + h->rnode[(size_t)r * NG * NG + k]];
+      } else {
+        v = f[(size_t)c * h->npoin + h->rnode[r]];
+      }
+      h->seis[((size_t)s * h->ndof + c) * h->nrec + r] = v;
+    }
   if (s + 1 > h->taken) h->taken = s + 1;
+}
+
+int oracle_set_receivers_interp(oracle_solver *h, int nrec, const int *elem, const double *interp, int isamp,
+                                int field, int nsamp) {
+  h->nrec = nrec; h->isamp = isamp; h->field = field; h->nsamp = nsamp;
+  h->rnode = (int *)malloc(sizeof(int) * nrec * NG * NG);
+  for (int r = 0; r < nrec; ++r)
+    for (int k = 0; k < NG * NG; ++k) h->rnode[r * NG * NG + k] = h->ibool[(size_t)(elem[r] - 1) * NG * NG + k];
+  h->rw = dup_d(interp, (size_t)nrec * NG * NG);
+  h->seis = dup_d(NULL, (size_t)nsamp * h->ndof * nrec);
+  h->taken = 0;
+  rec_store(h);
+  return 0;
 }
 
 int oracle_set_receivers(oracle_solver *h, int nrec, const int *iglob, int isamp, int field,
@@ -716,7 +739,7 @@ void oracle_destroy(oracle_solver *h) {
     free(h->src[s].ixi); free(h->src[s].ieta); free(h->src[s].cxi); free(h->src[s].ceta);
     free(h->src[s].ampl);
   }
-  free(h->rnode); free(h->seis);
+  free(h->rnode); free(h->rw); free(h->seis);
   if (h->flt) {
     fault_t *f = h->flt;
     free(f->n1); free(f->n2); free(f->nrm); free(f->B); free(f->invM1); free(f->invM2);

@@ -1091,7 +1091,61 @@ def _moment_stencil(iglob, Mt, ibool, jac, hprime, ndof):
                 coef_xi=np.array(coef_xi), coef_eta=np.array(coef_eta))
 
 
-def _receivers(par, coord, nt):
+def _hgll(i, z, xgll, nz=5):
+    """HGLL (SRC/gll.f90:270-296): Lagrange interpolant through the GLL points."""
+    dz = z - xgll[i]
+    if abs(dz) < 1.0e-5:
+        return 1.0
+    n = nz - 1
+    alfan = float(n) * (float(n) + 1.0)
+    return -(((1.0 - z * z) * _pndleg(z, n)) / ((alfan * _pnleg(xgll[i], n)) * (z - xgll[i])))
+
+
+def _q4(s, t):
+    """Q4_getshape / Q4_getdershape (SRC/elem_q4.f90:37-78)."""
+    q = 0.25
+    sp, sm, tp, tm = s + 1.0, s - 1.0, t + 1.0, t - 1.0
+    shape = [(q * sm) * tm, -((q * sp) * tm), (q * sp) * tp, -((q * sm) * tp)]
+    der = [[q * tm, q * sm], [-(q * tm), -(q * sp)], [q * tp, q * sp], [-(q * tp), -(q * sm)]]
+    return shape, der
+
+
+def _find_point(p, cxe, cze, xi, eta):
+    """FE_find_point (SRC/fem_grid.f90:443-500) on a Q4 element: Newton iterations for
+    the local coordinates of point p from the initial guess (xi, eta)."""
+    coorg = (cxe, cze)
+    x1, x2 = cxe[0] - cxe[2], cze[0] - cze[2]
+    y1, y2 = cxe[1] - cxe[3], cze[1] - cze[3]
+    esize = math.sqrt((0.5 * abs(x1 * y2 - x2 * y1)) / PI)
+    tolx, tolf = TINY_XABS, TINY_XABS * esize
+    x = [xi, eta]
+    n = 1
+    while n <= 100:
+        shap, dshap = _q4(x[0], x[1])
+        fvec = [(((c[0] * shap[0] + c[1] * shap[1]) + c[2] * shap[2]) + c[3] * shap[3]) - q
+                for c, q in zip(coorg, p)]
+        fjac = [[((c[0] * dshap[0][k] + c[1] * dshap[1][k]) + c[2] * dshap[2][k]) + c[3] * dshap[3][k]
+                 for k in range(2)] for c in coorg]
+        if abs(fvec[0]) + abs(fvec[1]) <= tolf:
+            break
+        pv = [-fvec[0], -fvec[1]]
+        det = fjac[0][0] * fjac[1][1] - fjac[0][1] * fjac[1][0]
+        if det <= 0.0:
+            raise InputError("SE_InverseJacobian: undefined Jacobian")
+        B = [[fjac[1][1] / det, (-fjac[0][1]) / det], [(-fjac[1][0]) / det, fjac[0][0] / det]]
+        pv = [B[0][0] * pv[0] + B[0][1] * pv[1], B[1][0] * pv[0] + B[1][1] * pv[1]]
+        x = [x[0] + pv[0], x[1] + pv[1]]
+        if abs(pv[0]) + abs(pv[1]) <= tolx:
+            break
+        n += 1
+    if n >= 100:
+        raise InputError("FE_find_point: did not converge")
+    shap, _ = _q4(x[0], x[1])
+    new = [((c[0] * shap[0] + c[1] * shap[1]) + c[2] * shap[2]) + c[3] * shap[3] for c in coorg]
+    return x[0], x[1], new
+
+
+def _receivers(par, coord, nt, num=None, xgll=None):
     g = par.first("REC_LINE")
     if g is None:
         return None
@@ -1100,8 +1154,7 @@ def _receivers(par, coord, nt):
     field = str(g.get("field", "V")).strip()
     if field not in ("D", "V", "A"):
         raise InputError('REC_read: parameter field has wrong value [D,V,A]')
-    if not g.get("atnode", True):
-        raise InputError("REC_LINE AtNode=F is not supported by this build")
+    atnode = bool(g.get("atnode", True))
     if g.get("file", "none") != "none":
         raise InputError("REC_LINE file= is not supported by this build")
     first = [float(v) for v in g.get("first")]
@@ -1113,13 +1166,35 @@ def _receivers(par, coord, nt):
             pts.append((first[0] + f * (last[0] - first[0]), first[1] + f * (last[1] - first[1])))
         else:
             pts.append((first[0], first[1]))
+    if not atnode:
+        # REC_posit, AtNode = F (SRC/receivers.f90:274-285): SE_find_point takes the first
+        # element (element order, then j, then i) holding the nearest GLL node, starts
+        # Newton from that node's (xi, eta), and -- FE_find_point tests the initial guess,
+        # which is inside [-1,1] -- always keeps that element, extrapolating if needed.
+        ibool = num["ibool"]
+        elems, interp, rc = [], [], []
+        for p in pts:
+            ip, _ = _nearest_node(coord, p)
+            e, j, i = (int(v) for v in np.argwhere(ibool == ip)[0])
+            xi, eta, new = _find_point(p, num["cx"][e], num["cz"][e], xgll[i], xgll[j])
+            w = np.empty(25)
+            for jj in range(5):
+                fj = _hgll(jj, eta, xgll)
+                for ii in range(5):
+                    w[jj * 5 + ii] = _hgll(ii, xi, xgll) * fj
+            elems.append(e + 1)
+            interp.append(w)
+            rc.append(new)
+        return dict(atnode=False, elem=np.array(elems, dtype=np.int32), interp=np.array(interp), isamp=isamp,
+                    field=field, nsamp=nt // isamp + 1, coord=np.array(rc),
+                    iglob=np.zeros(len(pts), dtype=np.int32))
     nodes = []
     for p in pts:  # REC_posit dedup (SRC/receivers.f90:247-272), incl. its "irec > 1" test
         ip, _ = _nearest_node(coord, p)
         if len(nodes) > 1 and ip in nodes:
             continue
         nodes.append(ip)
-    return dict(iglob=np.array(nodes, dtype=np.int32), isamp=isamp, field=field,
+    return dict(atnode=True, iglob=np.array(nodes, dtype=np.int32), isamp=isamp, field=field,
                 nsamp=nt // isamp + 1, coord=coord[np.array(nodes) - 1])
 
 
@@ -1341,7 +1416,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         else:
             raise InputError(f"BC kind {kind!r} is not supported by this build")
 
-    pb.receivers = _receivers(par, pb.coord, pb.nt)
+    pb.receivers = _receivers(par, pb.coord, pb.nt, num, xgll)
     pb.sources = _sources(par, num, jac, H, ndof, dt, pb.nt, fname_dir)
     pb.rmass = 1.0 / M
     pb.mass = M

@@ -41,6 +41,7 @@ _SIGS = {
     "set_stf": ([C.c_void_p, C.c_int, C.c_int, dp], C.c_int),
     "set_receivers": ([C.c_void_p, C.c_int, ip, C.c_int, C.c_int, C.c_int], C.c_int),
     "set_fault": ([C.c_void_p, C.POINTER(FaultDesc), C.c_int], C.c_int),
+    "set_receivers_interp": ([C.c_void_p, C.c_int, ip, dp, C.c_int, C.c_int, C.c_int], C.c_int),
     "step": ([C.c_void_p, C.c_int], C.c_int),
     "fetch_fields": ([C.c_void_p, dp, dp, dp], C.c_int),
     "set_fields": ([C.c_void_p, dp, dp, dp], C.c_int),

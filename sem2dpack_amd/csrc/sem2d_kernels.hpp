@@ -217,4 +217,21 @@ __global__ void k_receivers(int nrec, int ndof, const int* __restrict__ node, co
   for (int c = 0; c < ndof; ++c) out[(size_t)c * nrec + r] = (double)field[(size_t)c * npoin + node[r]];
 }
 
+// Receivers inside elements (REC_store with AtNode = F, SRC/receivers.f90:326-341): the
+// 25 weights h_i(xi) h_j(eta) times the field at the element's nodes, summed in the
+// element's (j, i) order without contraction.
+#pragma clang fp contract(off)
+template <typename R>
+__global__ void k_receivers_interp(int nrec, int ndof, const int* __restrict__ node, const double* __restrict__ w,
+                                   const R* __restrict__ field, size_t npoin, double* __restrict__ out) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  for (int c = 0; c < ndof; ++c) {
+    double s = 0.0;
+    for (int k = 0; k < NP; ++k) s += w[(size_t)r * NP + k] * (double)field[(size_t)c * npoin + node[(size_t)r * NP + k]];
+    out[(size_t)c * nrec + r] = s;
+  }
+}
+#pragma clang fp contract(on)
+
 }  // namespace sem2d

@@ -119,7 +119,8 @@ struct sem2d_solver {
   bool src_dirty = false;
   // receivers
   int nrec = 0, isamp = 1, field = 1, nsamp = 0, taken = 0;
-  int* rec_node = nullptr;
+  int* rec_node = nullptr;       // [nrec] node, or [nrec][25] element nodes when interpolated
+  double* rec_w = nullptr;       // [nrec][25] interpolation weights (nullptr: receivers at nodes)
   double* seis = nullptr;
   FaultHost flt;
   int* flags = nullptr;  // [0] Newton failure
@@ -128,6 +129,7 @@ struct sem2d_solver {
   std::string err;
   std::vector<void*> allocs;
   AdjState* adj = nullptr;
+  std::vector<int> elem_ibool;  // [nelem][25] 0-based reference nodes (interpolated receivers)
 };
 
 namespace {
@@ -322,8 +324,12 @@ int record_receivers(sem2d_solver* h) {
   int s = h->it / h->isamp;
   if (s >= h->nsamp) return 0;
   const R* fld = (const R*)(h->field == 0 ? h->d : (h->field == 1 ? h->v : h->a));
-  k_receivers<R><<<grid_for(h->nrec, 64), 64, 0, h->stream>>>(h->nrec, h->ndof, h->rec_node, fld, h->fstride,
-                                                              h->seis + (size_t)s * h->ndof * h->nrec);
+  if (h->rec_w)
+    k_receivers_interp<R><<<grid_for(h->nrec, 64), 64, 0, h->stream>>>(
+        h->nrec, h->ndof, h->rec_node, h->rec_w, fld, h->fstride, h->seis + (size_t)s * h->ndof * h->nrec);
+  else
+    k_receivers<R><<<grid_for(h->nrec, 64), 64, 0, h->stream>>>(h->nrec, h->ndof, h->rec_node, fld, h->fstride,
+                                                                h->seis + (size_t)s * h->ndof * h->nrec);
   h->taken = std::max(h->taken, s + 1);
   return 0;
 }
@@ -844,6 +850,8 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   h->gamma = s->gamma;
   for (size_t k = 0; k < (size_t)s->nelem * NP; ++k)
     if (s->ibool[k] < 1 || s->ibool[k] > s->npoin) return set_err(h, 3, "ibool entry %zu out of range", k);
+  h->elem_ibool.resize((size_t)s->nelem * NP);
+  for (size_t k = 0; k < (size_t)s->nelem * NP; ++k) h->elem_ibool[k] = s->ibool[k] - 1;
   HIPCHK(h, hipStreamCreate(&h->stream));
   HIPCHK(h, hipEventCreate(&h->ev0));
   HIPCHK(h, hipEventCreate(&h->ev1));
@@ -1023,6 +1031,36 @@ int sem2d_set_receivers(sem2d_solver* h, int nrec, const int* iglob, int isamp, 
   }
   int rc = upload(h, &h->rec_node, nd.data(), nd.size());
   if (rc) return rc;
+  h->rec_w = nullptr;
+  if ((rc = dev_alloc(h, &h->seis, (size_t)nsamp * h->ndof * nrec))) return rc;
+  h->nrec = nrec;
+  h->isamp = isamp;
+  h->field = field;
+  h->nsamp = nsamp;
+  h->taken = 0;
+  if (h->fp32) rc = record_receivers<float>(h);
+  else rc = record_receivers<double>(h);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return rc;
+}
+
+int sem2d_set_receivers_interp(sem2d_solver* h, int nrec, const int* elem, const double* interp, int isamp,
+                               int field, int nsamp) {
+  if (!h) return 1;
+  if (nrec <= 0 || !elem || !interp || isamp < 1 || field < 0 || field > 2 || nsamp < 1)
+    return set_err(h, 3, "set_receivers_interp: bad arguments");
+  if (h->elem_ibool.empty()) return set_err(h, 4, "set_receivers_interp: element table not kept");
+  std::vector<int> nd((size_t)nrec * NP);
+  for (int r = 0; r < nrec; ++r) {
+    if (elem[r] < 1 || elem[r] > h->nelem) return set_err(h, 3, "set_receivers_interp: element out of range");
+    for (int k = 0; k < NP; ++k) {
+      int g = h->elem_ibool[(size_t)(elem[r] - 1) * NP + k];
+      nd[(size_t)r * NP + k] = h->st.on ? h->st.gmap[g] : g;
+    }
+  }
+  int rc = upload(h, &h->rec_node, nd.data(), nd.size());
+  if (rc) return rc;
+  if ((rc = upload(h, &h->rec_w, interp, (size_t)nrec * NP))) return rc;
   if ((rc = dev_alloc(h, &h->seis, (size_t)nsamp * h->ndof * nrec))) return rc;
   h->nrec = nrec;
   h->isamp = isamp;
@@ -1833,6 +1871,7 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
     sem2d_solver* h = b->mem[m];
     b->rec_base[m] = (int)rn.size();
     if (!h->nrec) continue;
+    if (h->rec_w) return berr(b, 3, "member %d: receivers inside elements (AtNode=F) are not batched", m);
     if (!rn.empty() && (h->isamp != b->isamp || h->field != b->field))
       return berr(b, 3, "member %d: receiver sampling and field must match", m);
     b->isamp = h->isamp;
@@ -2139,7 +2178,7 @@ int adj_backward(sem2d_solver* h, const double* gseis_host) {
     for (int n = s1; n > s0; --n) {
       const int j = n - s0;
       if (h->nrec && n % h->isamp == 0 && n / h->isamp < h->nsamp)
-        k_adj_inject<R><<<1, 64, 0, h->stream>>>(h->nrec, h->rec_node,
+        k_adj_inject<R><<<1, 64, 0, h->stream>>>(h->nrec, h->rec_node, h->rec_w,
                                                  A.gseis + (size_t)(n / h->isamp) * h->ndof * h->nrec, inj, GS);
       const R* dfw = (const R*)st_d(h, A.slot[j]);
       const R* afw = (const R*)st_a(h, A.slot[j]);

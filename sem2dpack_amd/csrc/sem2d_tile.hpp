@@ -808,14 +808,25 @@ __global__ __launch_bounds__(256) void k_adj_deferred(int ndef, const DefNode* _
   mbar[g] += mb;
 }
 
-// dJ/d(recorded field) of one sample added at the receiver nodes (one thread: nodes
-// may repeat)
+// dJ/d(recorded field) of one sample added at the receiver nodes, or spread over the
+// element nodes with the interpolation weights w (the transpose of the interpolation);
+// one thread, since nodes may repeat
 template <typename R>
-__global__ void k_adj_inject(int nrec, const int* __restrict__ node, const double* __restrict__ g, R* __restrict__ fld,
-                             size_t GS) {
+__global__ void k_adj_inject(int nrec, const int* __restrict__ node, const double* __restrict__ w,
+                             const double* __restrict__ g, R* __restrict__ fld, size_t GS) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   for (int r = 0; r < nrec; ++r)
-    for (int c = 0; c < 2; ++c) fld[c * GS + node[r]] = (R)((double)fld[c * GS + node[r]] + g[(size_t)c * nrec + r]);
+    for (int c = 0; c < 2; ++c) {
+      const double gr = g[(size_t)c * nrec + r];
+      if (w) {
+        for (int k = 0; k < NP; ++k) {
+          const size_t i = c * GS + node[(size_t)r * NP + k];
+          fld[i] = (R)((double)fld[i] + w[(size_t)r * NP + k] * gr);
+        }
+      } else {
+        fld[c * GS + node[r]] = (R)((double)fld[c * GS + node[r]] + gr);
+      }
+    }
 }
 
 }  // namespace sem2d

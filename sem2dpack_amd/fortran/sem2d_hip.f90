@@ -35,7 +35,8 @@ module sem2d_hip
   end type sem2d_fault_desc
 
   public :: sem2d_create, sem2d_set_abso, sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
-            sem2d_set_receivers, sem2d_set_fault, sem2d_step, sem2d_fetch_fields, &
+            sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_step, &
+            sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
             sem2d_fetch_potency, &
             sem2d_compute_fint, sem2d_set_tile_config, sem2d_get_step, sem2d_last_step_ms, &
@@ -93,6 +94,15 @@ module sem2d_hip
       type(c_ptr), value :: h
       integer(c_int), value :: nrec, isamp, field, nsamp
       integer(c_int), intent(in) :: iglob(*)
+    end function
+
+    integer(c_int) function sem2d_set_receivers_interp(h, nrec, elem, interp, isamp, field, nsamp) &
+        bind(C, name='sem2d_set_receivers_interp')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: nrec, isamp, field, nsamp
+      integer(c_int), intent(in) :: elem(*)
+      real(c_double), intent(in) :: interp(*)
     end function
 
     integer(c_int) function sem2d_set_fault(h, f, nsamp) bind(C, name='sem2d_set_fault')

@@ -102,7 +102,14 @@ class Solver:
         if pb.fault is not None:
             self._set_fault(pb.fault)
         r = pb.receivers
-        if r is not None:
+        if r is not None and not r.get("atnode", True):
+            el = np.ascontiguousarray(r["elem"], dtype=np.int32)
+            w = np.ascontiguousarray(r["interp"], dtype=np.float64)
+            self.nrec, self.nsamp = len(el), r["nsamp"]
+            self._check(self.api.set_receivers_interp(self.h, len(el), _p(el, np.int32), _p(w), int(r["isamp"]),
+                                                      self.FIELD_CODE[r["field"]], int(r["nsamp"])),
+                        "set_receivers_interp")
+        elif r is not None:
             node = np.ascontiguousarray(r["iglob"], dtype=np.int32)
             self.nrec, self.nsamp = len(node), r["nsamp"]
             self._check(self.api.set_receivers(self.h, len(node), _p(node, np.int32), int(r["isamp"]),

@@ -12,7 +12,7 @@ import pytest
 from conftest import GOLDEN, case_par
 from sem2dpack_amd import builder
 
-CASES = ["small_psv", "c1_psv", "sh_swf", "psv_rsf", "hete_psv", "rsf_ens", "multi_src"]
+CASES = ["small_psv", "c1_psv", "sh_swf", "psv_rsf", "hete_psv", "rsf_ens", "multi_src", "interp_rec"]
 
 
 def sha(a):
@@ -45,6 +45,8 @@ def test_receivers_and_fault_pairing(case):
     name, g, pb = case
     if pb.receivers is None:
         assert g["rec_nodes"].size == 0
+    elif not pb.receivers["atnode"]:
+        pass  # receivers inside elements: pinned through their float32 series (oracle tests)
     else:
         assert np.array_equal(pb.receivers["iglob"], g["rec_nodes"])
     if "fault_node1" in g:
