@@ -118,6 +118,12 @@ int sem2d_set_receivers(sem2d_solver *h, int nrec, const int *iglob, int isamp, 
    kept; a record is taken at registration when the current step >= oit. */
 int sem2d_set_fault(sem2d_solver *h, const sem2d_fault_desc *f, int nsamp);
 
+/* Kelvin-Voigt damping: eta [nelem][ngll][ngll] per element point, already scaled by dt
+   when ETAxDT (matwrk%eta of MAT_KV_init_elem_work, SRC/mat_kelvin_voigt.f90:121-135),
+   0 outside Kelvin-Voigt elements; the element force becomes -K (d + eta v)
+   (SRC/mat_gen.f90:435).  Call right after sem2d_create. */
+int sem2d_set_kelvin_voigt(sem2d_solver *h, const double *eta);
+
 /* Receivers inside elements (REC_LINE AtNode=F; REC_posit/REC_store,
    SRC/receivers.f90:274-285,326-341): element elem[r] (1-based) and its 25 weights
    interp[r][k] = h_i(xi) h_j(eta), k = (j-1)*5 + i, as SE_init_interpol

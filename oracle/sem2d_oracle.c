@@ -24,6 +24,7 @@
  *   receivers            REC_store (AtNode)   SRC/receivers.f90:309-344
  *   fault records        BC_DYNFLT_write      SRC/bc_dynflt.f90:751-778
  *   potency              BC_DYNFLT_potency    SRC/bc_dynflt.f90:832-855
+ *   Kelvin-Voigt         MAT_KV_add_etav      SRC/mat_kelvin_voigt.f90:139-150
  *
  * Known deviation: in the rate-and-state branch the reference leaves the local
  * Tstick array unset (SRC/bc_dynflt.f90:620-636 never assigns it); here Tstick is
@@ -88,6 +89,7 @@ struct oracle_solver {
   int nrec, isamp, field, nsamp, taken;
   int *rnode;        /* [nrec], or [nrec][25] element nodes when interpolated */
   double *rw;        /* [nrec][25] interpolation weights, NULL for receivers at nodes */
+  double *kv_eta;    /* [nelem][25] Kelvin-Voigt eta (dt-scaled), NULL if none */
   double *seis;
   fault_t *flt;
   int it;
@@ -514,7 +516,9 @@ static void kd2(const oracle_solver *h, const double *a, double U[2][NG][NG], do
 #undef A
 }
 
-static void fint(const oracle_solver *h, const double *d, double *f) {
+/* compute_Fint (SRC/solver.f90:273-320); v != NULL adds the Kelvin-Voigt term
+   d + eta v in the viscous elements (MAT_KV_add_etav, SRC/mat_kelvin_voigt.f90:139-150) */
+static void fint(const oracle_solver *h, const double *d, double *f, const double *v) {
   size_t np = h->npoin;
   memset(f, 0, sizeof(double) * np * h->ndof);
   double U[2][NG][NG], F[2][NG][NG];
@@ -523,6 +527,12 @@ static void fint(const oracle_solver *h, const double *d, double *f) {
     for (int c = 0; c < h->ndof; ++c)
       for (int j = 0; j < NG; ++j)
         for (int i = 0; i < NG; ++i) U[c][j][i] = d[c * np + ib[j * NG + i]];
+    if (v && h->kv_eta) {
+      const double *eta = h->kv_eta + (size_t)e * NG * NG;
+      for (int c = 0; c < h->ndof; ++c)
+        for (int j = 0; j < NG; ++j)
+          for (int i = 0; i < NG; ++i) U[c][j][i] = U[c][j][i] + eta[j * NG + i] * v[c * np + ib[j * NG + i]];
+    }
     const double *a = h->coefs + (h->shared ? 0 : (size_t)e * h->nelast * NG * NG);
     kd2(h, a, U, F);
     for (int j = 0; j < NG; ++j)
@@ -534,7 +544,12 @@ static void fint(const oracle_solver *h, const double *d, double *f) {
 }
 
 int oracle_compute_fint(oracle_solver *h, const double *d, double *f) {
-  fint(h, d, f);
+  fint(h, d, f, NULL);
+  return 0;
+}
+
+int oracle_set_kelvin_voigt(oracle_solver *h, const double *eta) {
+  h->kv_eta = dup_d(eta, (size_t)h->nelem * NG * NG);
   return 0;
 }
 
@@ -628,7 +643,7 @@ static void one_step(oracle_solver *h) {
     v[k] = v[k] + c2 * a[k];
   }
   double *f = a; /* f => a (SRC/solver.f90:52) */
-  fint(h, d, f);
+  fint(h, d, f, v);
   int it = h->it + 1;
   for (int s = 0; s < h->nsrc; ++s) {
     source_t *S = &h->src[s];
@@ -739,7 +754,7 @@ void oracle_destroy(oracle_solver *h) {
     free(h->src[s].ixi); free(h->src[s].ieta); free(h->src[s].cxi); free(h->src[s].ceta);
     free(h->src[s].ampl);
   }
-  free(h->rnode); free(h->rw); free(h->seis);
+  free(h->rnode); free(h->rw); free(h->seis); free(h->kv_eta);
   if (h->flt) {
     fault_t *f = h->flt;
     free(f->n1); free(f->n2); free(f->nrm); free(f->B); free(f->invM1); free(f->invM2);

@@ -515,8 +515,10 @@ def _materials(par, fname_dir=None):
         if tag <= 0:
             raise InputError("MAT_read: tag must be positive")
         for k in kinds:
-            if k not in ("ELAST",):
+            if k not in ("ELAST", "KV"):
                 raise InputError(f"material kind {k!r} is not supported by this build")
+        if "ELAST" not in kinds:
+            raise InputError("this build needs an ELAST material (optionally with KV)")
         me = par.after(g, "MAT_ELASTIC")
         if me is None:
             raise InputError("MAT_ELAST_read: MAT_ELASTIC block not found")
@@ -555,6 +557,15 @@ def _materials(par, fname_dir=None):
                     raise InputError(f"distribution {want} for materials is not supported by this build")
                 spec[nme] = _read_hete1(par, gg, fname_dir)
             mats[tag] = dict(homog=False, spec=spec)
+        if "KV" in kinds:  # MAT_KV_read (SRC/mat_kelvin_voigt.f90:72-100)
+            kv = par.after(g, "MAT_KV")
+            if kv is None:
+                raise InputError("MAT_KV_read: MAT_KV input block not found")
+            if str(kv.get("etah", "")).strip():
+                raise InputError("MAT_KV etaH (distribution) is not supported by this build")
+            mats[tag]["kv_eta"] = float(kv.get("eta", 0.0))
+            # NormalizeByDT is module-level: the last MAT_KV block read decides for all
+            mats["_kv_by_dt"] = bool(kv.get("etaxdt", True))
     if not mats:
         raise InputError("MAT_read: MATERIAL block not found")
     return mats
@@ -1282,6 +1293,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         homog_tag = {int(t): False for t in np.unique(tags)}
     else:
         mats = _materials(par, fname_dir)
+        kv_tags = {t: m["kv_eta"] for t, m in mats.items() if isinstance(t, int) and "kv_eta" in m}
         for t in np.unique(tags):
             if int(t) not in mats:
                 raise InputError("MAT_init_prop: material tag missing")
@@ -1300,6 +1312,11 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
                 lam_p[sel] = r * (c1 * c1 - 2.0 * c2 * c2)
                 mu_p[sel] = r * c2 * c2
     pb.rho, pb.cp, pb.cs, pb.lam, pb.mu = rho_p, cp_p, cs_p, lam_p, mu_p
+    kv_eta0 = None
+    if material is None and elem_props is None and kv_tags:
+        kv_eta0 = np.zeros(nelem)
+        for t, eta in kv_tags.items():
+            kv_eta0[tags == t] = eta
     pb.homogeneous = all(homog_tag.values())
 
     jac = _jacobians(num["cx"], num["cz"], xgll)
@@ -1342,6 +1359,12 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     if courant > critical:
         raise InputError("TIME_init: Courant number too high (IO_warning aborts)")
     pb.dt, pb.beta, pb.gamma, pb.alpha = dt, beta, gamma, 1.0
+    # Kelvin-Voigt viscosity per element point (MAT_KV_init_elem_work,
+    # SRC/mat_kelvin_voigt.f90:121-135): eta, times dt when ETAxDT
+    pb.kv_eta = None
+    if kv_eta0 is not None:
+        eta = np.broadcast_to(kv_eta0[:, None, None], (nelem, 5, 5)).copy()
+        pb.kv_eta = dt * eta if mats.get("_kv_by_dt", True) else eta
     pb.nt = int(nbsteps if nsteps is None else nsteps)
     pb.grid_cfl = grid_cfl
 

@@ -69,11 +69,15 @@ __device__ __forceinline__ void load_coefs(const R* __restrict__ coefs, const R*
   }
 }
 
+// Element forces of one colour.  With Kelvin-Voigt elements (kv != nullptr, [element
+// in colour order][25]) the element displacement is d + eta v (MAT_KV_add_etav,
+// SRC/mat_kelvin_voigt.f90:139-150, applied by MAT_Fint, SRC/mat_gen.f90:435).
 template <typename R, int NDOF, int NELAST, int MODE>
 __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int* __restrict__ ibool,
                                                       const R* __restrict__ coefs, const R* __restrict__ geom,
                                                       const R* __restrict__ d, R* __restrict__ f,
-                                                      int npoin) {
+                                                      int npoin, const R* __restrict__ v = nullptr,
+                                                      const R* __restrict__ kv = nullptr) {
   __shared__ R U[FE_EPB][NDOF][NP];
   __shared__ R T1[FE_EPB][NDOF][NP];
   __shared__ R T2[FE_EPB][NDOF][NP];
@@ -87,8 +91,14 @@ __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int*
   int node = 0;
   if (active) {
     node = ibool[(size_t)e * NP + p];
+    if (kv) {
+      const R eta = kv[(size_t)e * NP + p];
 #pragma unroll
-    for (int c = 0; c < NDOF; ++c) U[le][c][p] = d[(size_t)c * npoin + node];
+      for (int c = 0; c < NDOF; ++c) U[le][c][p] = d[(size_t)c * npoin + node] + eta * v[(size_t)c * npoin + node];
+    } else {
+#pragma unroll
+      for (int c = 0; c < NDOF; ++c) U[le][c][p] = d[(size_t)c * npoin + node];
+    }
   }
   __syncthreads();
   if (active) {

@@ -130,6 +130,7 @@ struct sem2d_solver {
   std::vector<void*> allocs;
   AdjState* adj = nullptr;
   std::vector<int> elem_ibool;  // [nelem][25] 0-based reference nodes (interpolated receivers)
+  void* kv_eta = nullptr;       // Kelvin-Voigt viscosity [element in colour order][25] (bulk type)
 };
 
 namespace {
@@ -259,34 +260,36 @@ int build_colours(sem2d_solver* h, const sem2d_desc* s, std::vector<int>& colour
 }
 
 template <typename R, int NDOF, int NELAST, int MODE>
-void launch_force_t(sem2d_solver* h, const R* d, R* f) {
+void launch_force_t(sem2d_solver* h, const R* d, R* f, const R* v) {
+  const R* kv = v ? (const R*)h->kv_eta : nullptr;
   for (size_t c = 0; c + 1 < h->colour_start.size(); ++c) {
     const int e0 = h->colour_start[c], ne = h->colour_start[c + 1] - e0;
     if (ne <= 0) continue;
     k_force_colour<R, NDOF, NELAST, MODE><<<(ne + FE_EPB - 1) / FE_EPB, 256, 0, h->stream>>>(
-        e0, ne, h->ibool, (const R*)h->coefs, (const R*)h->geom, d, f, h->npoin);
+        e0, ne, h->ibool, (const R*)h->coefs, (const R*)h->geom, d, f, h->npoin, v, kv);
   }
 }
 
 template <typename R, int NDOF, int NELAST>
-void launch_force_m(sem2d_solver* h, const R* d, R* f) {
+void launch_force_m(sem2d_solver* h, const R* d, R* f, const R* v) {
   switch (h->coef_mode) {
-    case COEF_SHARED: launch_force_t<R, NDOF, NELAST, COEF_SHARED>(h, d, f); break;
+    case COEF_SHARED: launch_force_t<R, NDOF, NELAST, COEF_SHARED>(h, d, f, v); break;
     case COEF_COMPACT:
-      if constexpr (NELAST == 2 || NELAST == 6) launch_force_t<R, NDOF, NELAST, COEF_COMPACT>(h, d, f);
+      if constexpr (NELAST == 2 || NELAST == 6) launch_force_t<R, NDOF, NELAST, COEF_COMPACT>(h, d, f, v);
       break;
-    default: launch_force_t<R, NDOF, NELAST, COEF_PER_ELEM>(h, d, f);
+    default: launch_force_t<R, NDOF, NELAST, COEF_PER_ELEM>(h, d, f, v);
   }
 }
 
+// f = -K d, or -K (d + eta v) in Kelvin-Voigt elements when v is given
 template <typename R>
-void launch_force(sem2d_solver* h, const R* d, R* f) {
+void launch_force(sem2d_solver* h, const R* d, R* f, const R* v = nullptr) {
   if (h->ndof == 2) {
-    if (h->nelast == 6) launch_force_m<R, 2, 6>(h, d, f);
-    else launch_force_m<R, 2, 10>(h, d, f);
+    if (h->nelast == 6) launch_force_m<R, 2, 6>(h, d, f, v);
+    else launch_force_m<R, 2, 10>(h, d, f, v);
   } else {
-    if (h->nelast == 2) launch_force_m<R, 1, 2>(h, d, f);
-    else launch_force_m<R, 1, 3>(h, d, f);
+    if (h->nelast == 2) launch_force_m<R, 1, 2>(h, d, f, v);
+    else launch_force_m<R, 1, 3>(h, d, f, v);
   }
 }
 
@@ -795,7 +798,7 @@ int do_steps(sem2d_solver* h, int nsteps) {
       continue;
     }
     k_predict<R><<<gb, 256, 0, h->stream>>>(n, d, v, a, dt, c1, c2);
-    launch_force<R>(h, d, a);
+    launch_force<R>(h, d, a, h->kv_eta ? v : nullptr);
     if (!h->src_host.empty())
       k_sources<R><<<1, 64, 0, h->stream>>>((int)h->src_host.size(), h->src_dev, h->ampl_dev, h->ampl_nt, it, a,
                                            h->npoin);
@@ -1042,6 +1045,27 @@ int sem2d_set_receivers(sem2d_solver* h, int nrec, const int* iglob, int isamp, 
   else rc = record_receivers<double>(h);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return rc;
+}
+
+int sem2d_set_kelvin_voigt(sem2d_solver* h, const double* eta) {
+  if (!h) return 1;
+  if (!eta) return set_err(h, 3, "set_kelvin_voigt: null eta");
+  if (h->it != 0 || h->nrec || h->flt.on || h->nsrc || !h->abso.empty())
+    return set_err(h, 4, "set_kelvin_voigt: call it right after sem2d_create");
+  if (h->st.on) {
+    // the fused tile kernels have no viscous term: use the general element path, with
+    // the fields in the reference numbering
+    h->st.on = false;
+    h->fstride = (size_t)h->npoin;
+    const size_t nfs = (size_t)h->ndof * h->fstride * rsize(h);
+    HIPCHK(h, hipMemsetAsync(h->d, 0, nfs, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->v, 0, nfs, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->a, 0, nfs, h->stream));
+  }
+  std::vector<double> e((size_t)h->nelem * NP);
+  for (int k = 0; k < h->nelem; ++k)
+    std::memcpy(&e[(size_t)k * NP], eta + (size_t)h->elem_order[k] * NP, NP * sizeof(double));
+  return upload_bulk(h, &h->kv_eta, e.data(), e.size());
 }
 
 int sem2d_set_receivers_interp(sem2d_solver* h, int nrec, const int* elem, const double* interp, int isamp,

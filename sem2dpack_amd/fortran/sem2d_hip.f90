@@ -34,7 +34,7 @@ module sem2d_hip
     integer(c_int) :: oit = 0, oitd = 1, oix1 = 1, oixn = 1, oixd = 1
   end type sem2d_fault_desc
 
-  public :: sem2d_create, sem2d_set_abso, sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
+  public :: sem2d_create, sem2d_set_kelvin_voigt, sem2d_set_abso, sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
             sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_step, &
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
@@ -53,6 +53,12 @@ module sem2d_hip
       import :: c_int, c_ptr, sem2d_desc
       type(sem2d_desc), intent(in) :: desc
       type(c_ptr), intent(out) :: h
+    end function
+
+    integer(c_int) function sem2d_set_kelvin_voigt(h, eta) bind(C, name='sem2d_set_kelvin_voigt')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: eta(*)
     end function
 
     integer(c_int) function sem2d_set_abso(h, n, node, C) bind(C, name='sem2d_set_abso')

@@ -81,6 +81,9 @@ class Solver:
 
     def _setup(self):
         pb = self.pb
+        if getattr(pb, "kv_eta", None) is not None:
+            eta = np.ascontiguousarray(pb.kv_eta, dtype=np.float64)
+            self._check(self.api.set_kelvin_voigt(self.h, _p(eta)), "set_kelvin_voigt")
         for ab in pb.abso:
             node = np.ascontiguousarray(ab["node"], dtype=np.int32)
             Cm = np.ascontiguousarray(ab["C"].T, dtype=np.float64)  # [ndof][n]

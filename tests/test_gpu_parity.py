@@ -44,6 +44,7 @@ def test_fint(name, oracle_api):
 
 
 @pytest.mark.parametrize("name,nsteps", [("small_psv", 300), ("c1_psv", 600), ("sh_swf", 600), ("multi_src", 300), ("interp_rec", 300),
+                                         ("sh_kv", 600), ("psv_kv", 500),
                                          ("psv_rsf", 500), ("hete_psv", 400), ("rsf_ens", 400)])
 def test_time_series(name, nsteps, oracle_api):
     print(f"\n[case {name} nsteps {nsteps}]", flush=True)

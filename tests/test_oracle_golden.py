@@ -14,7 +14,7 @@ from conftest import GOLDEN, case_par
 from sem2dpack_amd import builder, solver
 import refdump  # noqa: F401  (fixture readers live beside the tests)
 
-CASES = ["small_psv", "c1_psv", "sh_swf", "psv_rsf", "hete_psv", "rsf_ens", "multi_src", "interp_rec"]
+CASES = ["small_psv", "c1_psv", "sh_swf", "psv_rsf", "hete_psv", "rsf_ens", "multi_src", "interp_rec", "sh_kv", "psv_kv"]
 
 
 def sha(a):
@@ -81,7 +81,7 @@ def test_oracle_matches_reference(name, oracle_api):
             assert np.array_equal(ours[:, cols], ref32[:, cols])
 
 
-@pytest.mark.parametrize("name", ["sh_swf", "psv_rsf", "rsf_ens"])
+@pytest.mark.parametrize("name", ["sh_swf", "psv_rsf", "rsf_ens", "sh_kv"])
 def test_potency_file_matches_reference(name, oracle_api, tmp_path):
     """FltXX_potency_sem2d.tab written from the oracle's potency rows is byte-identical to
     the reference's file (BC_DYNFLT_potency sums, (6D24.16) formatting)."""
