@@ -668,6 +668,8 @@ struct StepDev {
   const void* fault_rm = nullptr;
   const int* itp = nullptr;  // device step counter (graph replay) or null: step index by value
   hipStream_t stream = nullptr;
+  // out-of-place step (adjoint replay): read the state here, write it to d, v, a
+  const void *din = nullptr, *vin = nullptr, *ain = nullptr;
 };
 
 StepDev step_dev(sem2d_solver* h) {
@@ -713,6 +715,27 @@ StepDev step_dev(sem2d_solver* h) {
 
 template <typename R, int NDOF, int NELAST, int MODE>
 void launch_tile_t(const StepDev& X, R dt, R c1, R c2, R c3, R c4) {
+  if constexpr (MODE == COEF_COMPACT && NDOF == 2 && NELAST == 6) {
+    if (X.din) {  // out of place (only the adjoint replay, P-SV compact)
+      const R *di = (const R*)X.din, *vi = (const R*)X.vin, *ai = (const R*)X.ain;
+      if (X.pipe_wgs > 0) {
+        const int per = (X.T.ntiles + X.pipe_wgs - 1) / X.pipe_wgs;
+        if (X.pipe_threads == 256)
+          k_tile_pipe<R, NDOF, NELAST, MODE, 256, 10, true><<<X.pipe_wgs, 256, 0, X.stream>>>(
+              X.T, per, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
+              (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
+        else
+          k_tile_pipe<R, NDOF, NELAST, MODE, 512, 20, true><<<X.pipe_wgs, 512, 0, X.stream>>>(
+              X.T, per, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
+              (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
+      } else {
+        k_tile<R, NDOF, NELAST, MODE, true><<<8 * X.T.per_xcd, TILE_THREADS, 0, X.stream>>>(
+            X.T, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
+            (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
+      }
+      return;
+    }
+  }
   if (X.pipe_wgs > 0) {
     // persistent pipelined kernel: pipe_wgs workgroups, each on a run of tiles
     const int per = (X.T.ntiles + X.pipe_wgs - 1) / X.pipe_wgs;
@@ -752,7 +775,13 @@ void launch_structured_step(const StepDev& X, int it, R dt, R c1, R c2, R c3, R 
     else launch_tile_m<R, 1, 3>(X, dt, c1, c2, c3, c4);
   }
   const int gd = (X.ndef + 255) / 256;
-  if (X.ndof == 2)
+  if (X.ndof == 2 && X.din)
+    k_deferred<R, 2, true><<<gd, 256, 0, X.stream>>>(X.ndef, X.defs, X.part_off, (const R*)X.fpart, X.T.FPS,
+                                                      X.dsrc, X.ampl, X.ampl_nt, it, X.itp, X.dabso,
+                                                      (const R*)X.rmdef, (R*)X.d, (R*)X.v, (R*)X.a, X.fstride, dt,
+                                                      c1, c2, c3, c4, (const R*)X.din, (const R*)X.vin,
+                                                      (const R*)X.ain);
+  else if (X.ndof == 2)
     k_deferred<R, 2><<<gd, 256, 0, X.stream>>>(X.ndef, X.defs, X.part_off, (const R*)X.fpart, X.T.FPS, X.dsrc,
                                                 X.ampl, X.ampl_nt, it, X.itp, X.dabso, (const R*)X.rmdef, (R*)X.d,
                                                 (R*)X.v, (R*)X.a, X.fstride, dt, c1, c2, c3, c4);
@@ -2189,15 +2218,18 @@ int adj_backward(sem2d_solver* h, const double* gseis_host) {
     char* s = A.slot[0];
     int rc = copy_state(h, st_d(h, s), st_v(h, s), st_a(h, s), st_d(h, A.ck[k]), st_v(h, A.ck[k]), st_a(h, A.ck[k]));
     if (rc) return rc;
-    for (int j = 1; j <= L; ++j) {
+    for (int j = 1; j <= L; ++j) {  // out of place: slot[j-1] -> slot[j]
       char* p = A.slot[j - 1];
       char* q = A.slot[j];
-      if ((rc = copy_state(h, st_d(h, q), st_v(h, q), st_a(h, q), st_d(h, p), st_v(h, p), st_a(h, p)))) return rc;
+      X.din = st_d(h, p);
+      X.vin = st_v(h, p);
+      X.ain = st_a(h, p);
       X.d = st_d(h, q);
       X.v = st_v(h, q);
       X.a = st_a(h, q);
       launch_structured_step<R>(X, s0 + j, dt, c1, c2, c3, c4);
     }
+    X.din = X.vin = X.ain = nullptr;
     // adjoint steps n = s1 .. s0+1
     for (int n = s1; n > s0; --n) {
       const int j = n - s0;

@@ -81,11 +81,17 @@ __device__ __forceinline__ int perim_slot(int X, int Z, int PX, int PZ) {
   return -1;
 }
 
-template <typename R, int NDOF, int NELAST, int MODE>
+template <typename R, int NDOF, int NELAST, int MODE, bool OOP = false>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict__ d, R* __restrict__ v,
                                                        R* __restrict__ a, const R* __restrict__ rmass0,
                                                        const R* __restrict__ coefs, const R* __restrict__ geom,
-                                                       R* __restrict__ fpart, R dt, R c1, R c2, R c3, R c4) {
+                                                       R* __restrict__ fpart, R dt, R c1, R c2, R c3, R c4,
+                                                       const R* __restrict__ din = nullptr,
+                                                       const R* __restrict__ vin = nullptr,
+                                                       const R* __restrict__ ain = nullptr) {
+  const R* __restrict__ dsrc = OOP ? din : d;
+  const R* __restrict__ vsrc = OOP ? vin : v;
+  const R* __restrict__ asrc = OOP ? ain : a;
   __shared__ R Ud[NDOF][PMAX];
   __shared__ R Fn[NDOF][PMAX];
   __shared__ R T1[FE_EPB][NDOF][NP];
@@ -115,7 +121,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
       const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
 #pragma unroll
       for (int c = 0; c < NDOF; ++c) {
-        const R dk = d[c * S + g], vk = v[c * S + g], ak = a[c * S + g];
+        const R dk = dsrc[c * S + g], vk = vsrc[c * S + g], ak = asrc[c * S + g];
         Ud[c][p] = dk + dt * vk + c1 * ak;
         vp[r][c] = vk + c2 * ak;
         Fn[c][p] = 0;
@@ -241,12 +247,19 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
 // Results are identical to k_tile: same predictor, same colour order and same
 // summation order.  NT threads per workgroup, EPB elements per element batch
 // (EPB*25 <= NT).
-template <typename R, int NDOF, int NELAST, int MODE, int NT, int EPB>
+template <typename R, int NDOF, int NELAST, int MODE, int NT, int EPB, bool OOP = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 4 : 2))) void k_tile_pipe(TileDev T, int tiles_per_wg, R* __restrict__ d,
                                                             R* __restrict__ v, R* __restrict__ a,
                                                             const R* __restrict__ rmass0,
                                                             const R* __restrict__ coefs, const R* __restrict__ geom,
-                                                            R* __restrict__ fpart, R dt, R c1, R c2, R c3, R c4) {
+                                                            R* __restrict__ fpart, R dt, R c1, R c2, R c3, R c4,
+                                                            const R* __restrict__ din = nullptr,
+                                                            const R* __restrict__ vin = nullptr,
+                                                            const R* __restrict__ ain = nullptr) {
+  // OOP: the step reads the state (din, vin, ain) and writes (d, v, a); otherwise in place
+  const R* __restrict__ dsrc = OOP ? din : d;
+  const R* __restrict__ vsrc = OOP ? vin : v;
+  const R* __restrict__ asrc = OOP ? ain : a;
   __shared__ R Ud[NDOF][PMAX];
   __shared__ R Fn[NDOF][PMAX];
   constexpr int NITER = (PMAX + NT - 1) / NT;                       // patch nodes per thread
@@ -300,9 +313,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
         const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
-          pd[r][c] = d[c * S + g];
-          pv[r][c] = v[c * S + g];
-          pa[r][c] = a[c * S + g];
+          pd[r][c] = dsrc[c * S + g];
+          pv[r][c] = vsrc[c * S + g];
+          pa[r][c] = asrc[c * S + g];
         }
       }
     }
@@ -514,7 +527,7 @@ struct DefNode {
   int fault;    // 1: split-fault node (finished by the fault kernels)
 };
 
-template <typename R, int NDOF>
+template <typename R, int NDOF, bool OOP = false>
 __global__ __launch_bounds__(256) void k_deferred(int ndef, const DefNode* __restrict__ defs,
                                                   const int* __restrict__ part_off, const R* __restrict__ fpart,
                                                   size_t FPS, const SrcEntry* __restrict__ src,
@@ -522,16 +535,22 @@ __global__ __launch_bounds__(256) void k_deferred(int ndef, const DefNode* __res
                                                   const int* __restrict__ itp,
                                                   const double* __restrict__ abso, const R* __restrict__ rmdef,
                                                   R* __restrict__ d, R* __restrict__ v, R* __restrict__ a, size_t GS,
-                                                  R dt, R c1, R c2, R c3, R c4) {
+                                                  R dt, R c1, R c2, R c3, R c4,
+                                                  const R* __restrict__ din = nullptr,
+                                                  const R* __restrict__ vin = nullptr,
+                                                  const R* __restrict__ ain = nullptr) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= ndef) return;
   const DefNode n = defs[q];
   const size_t g = n.g;
   const int it = itp ? *itp : it_arg;  // step index: device counter under graph replay
+  const R* __restrict__ dsrc = OOP ? din : d;
+  const R* __restrict__ vsrc = OOP ? vin : v;
+  const R* __restrict__ asrc = OOP ? ain : a;
   R dp[NDOF], vpp[NDOF], f[NDOF];
 #pragma unroll
   for (int c = 0; c < NDOF; ++c) {
-    const R dk = d[c * GS + g], vk = v[c * GS + g], ak = a[c * GS + g];
+    const R dk = dsrc[c * GS + g], vk = vsrc[c * GS + g], ak = asrc[c * GS + g];
     dp[c] = dk + dt * vk + c1 * ak;
     vpp[c] = vk + c2 * ak;
     R s = 0;
