@@ -12,9 +12,15 @@ LIB = os.path.join(HERE, "lib", "libsem2d_oracle.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
 
-def load():
+LIB_ULP = os.path.join(HERE, "lib", "libsem2d_oracle_ulp.so")
+
+
+def load(ulp_perturbed=False):
+    """The oracle; ``ulp_perturbed=True``: the build whose libm results are moved by one
+    ulp (to measure the reference arithmetic's conditioning)."""
     from sem2dpack_amd import capi
-    if not os.path.exists(LIB):
+    lib = LIB_ULP if ulp_perturbed else LIB
+    if not os.path.exists(lib):
         from . import build_oracle
         build_oracle.build()
-    return capi.bind(ctypes.CDLL(LIB), prefix="oracle_")
+    return capi.bind(ctypes.CDLL(lib), prefix="oracle_")

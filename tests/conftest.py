@@ -26,3 +26,10 @@ def case_par(name):
 def oracle_api():
     import oracle
     return oracle.load()
+
+
+@pytest.fixture(scope="session")
+def oracle_ulp_api():
+    """the oracle with every libm result moved by one ulp (conditioning probe)"""
+    import oracle
+    return oracle.load(ulp_perturbed=True)

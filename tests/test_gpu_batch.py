@@ -75,19 +75,26 @@ def test_rupture_batch_bitwise(fp32):
         assert_same(b, solos, m, True)
 
 
-def test_rupture_batch_matches_oracle(oracle_api):
+def test_rupture_batch_matches_oracle(oracle_api, oracle_ulp_api):
+    """member vs the oracle, with the bar of the single-scenario parity tests: 1e-10, or
+    10x the oracle's own movement under one-ulp libm changes where the rate-and-state
+    solve makes that larger"""
     pbs, members = rupture_members((24, 30), 200, False)
     b = solver.Batch(members)
     b.step(200)
-    ref = solver.Solver(pbs[1], api=oracle_api, compact=False)
-    ref.step(200)
-    d, v, a = b.fields(1)
-    dr, vr, ar = ref.fields()
-    for x, y, tol in ((d, dr, 1e-10), (v, vr, 1e-10), (a, ar, 1e-8)):
-        assert np.linalg.norm(x - y) <= tol * np.linalg.norm(y)
-    rb, rr = b.fault_records(1), ref.fault_records()
+    outs = []
+    for api in (oracle_api, oracle_ulp_api):
+        ref = solver.Solver(pbs[1], api=api, compact=False)
+        ref.step(200)
+        outs.append((ref.fields(), ref.fault_records()))
+    (fr, rr), (fu, ru) = outs
+    rel = lambda x, y: np.linalg.norm(x - y) / np.linalg.norm(y)
+    bar = lambda u, y: max(1e-10, 10 * rel(u, y))
+    for x, y, u in zip(b.fields(1), fr, fu):
+        assert rel(x, y) < bar(u, y)
+    rb = b.fault_records(1)
     for q in (0, 1, 2, 4):
-        assert np.linalg.norm(rb[:, q] - rr[:, q]) <= 1e-10 * np.linalg.norm(rr[:, q])
+        assert rel(rb[:, q], rr[:, q]) < bar(ru[:, q], rr[:, q])
 
 
 def test_source_batch_bitwise_and_tuned():

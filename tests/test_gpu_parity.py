@@ -1,14 +1,16 @@
 """HIP time step vs the oracle (C restatement of the reference) on the same inputs.
 
-FP64 bar: relative L2 error <= 1e-10 on displacement, velocity, seismograms and the
-fault slip, slip rate, shear traction and friction series; element forces <= 1e-13
-(one application of -K).  Two quantities get a documented floor instead:
-
-* the acceleration field of the rate-and-state case (1e-8): the oracle itself moves
-  by 2.3e-10 there when its exp/log/sinh/cosh/asinh results are perturbed by one ulp,
-  i.e. it is the conditioning of the reference's Newton solve, not a kernel error;
-* the fault normal traction change, which is zero up to round-off in these
-  symmetric set-ups: compared in absolute terms against 1e-10 of |T0_n|.
+FP64 bar: relative L2 error <= 1e-10 on displacement, velocity, acceleration,
+seismograms, fault slip, slip rate, shear traction, friction and potency; element
+forces <= 1e-13 (one application of -K).  Where the reference arithmetic is itself
+ill-conditioned the bar is widened to 10x its measured sensitivity: every case is
+also run on the oracle build whose exp/log/sinh/cosh/asinh results are moved by one
+ulp, and a quantity whose oracle output then moves by more than 1e-11 gets the bar
+max(1e-10, 10 x that movement).  This happens only with rate-and-state friction,
+whose Newton solve (SRC/bc_dynflt_rsf.f90:369-468, tolerance 0.001*a*|sigma|,
+slip rate of the last evaluation) amplifies round-off.  The normal traction change,
+zero up to round-off in these symmetric set-ups, is compared in absolute terms
+against |T0_n|.
 """
 import numpy as np
 import pytest
@@ -46,41 +48,49 @@ def test_fint(name, oracle_api):
 @pytest.mark.parametrize("name,nsteps", [("small_psv", 300), ("c1_psv", 600), ("sh_swf", 600), ("multi_src", 300), ("interp_rec", 300),
                                          ("sh_kv", 600), ("psv_kv", 500),
                                          ("psv_rsf", 500), ("hete_psv", 400), ("rsf_ens", 400)])
-def test_time_series(name, nsteps, oracle_api):
+def test_time_series(name, nsteps, oracle_api, oracle_ulp_api):
     print(f"\n[case {name} nsteps {nsteps}]", flush=True)
     pb, ref, gpu = _pair(name, nsteps, oracle_api)
-    ref.step(nsteps)
-    gpu.step(nsteps)
-    errs, tols = {}, {}
-    rsf = pb.fault is not None and pb.fault["law"] == "RSF"
-    for nm, a, b in zip("dva", gpu.fields(), ref.fields()):
-        errs[nm] = rel(a, b)
-        tols[nm] = 1e-8 if (nm == "a" and rsf) else TOL
+    ulp = solver.Solver(pb, api=oracle_ulp_api, compact=False)
+    for s in (ref, gpu, ulp):
+        s.step(nsteps)
+    # each quantity: (gpu, oracle, perturbed oracle, error function)
+    rows = {}
+    for nm, a, b, c in zip("dva", gpu.fields(), ref.fields(), ulp.fields()):
+        rows[nm] = (a, b, c, rel)
     if pb.receivers is not None:
-        s_ref, s_gpu = ref.seismograms(), gpu.seismograms()
-        assert s_gpu.shape == s_ref.shape
-        errs["seismo"], tols["seismo"] = rel(s_gpu, s_ref), TOL
+        rows["seismo"] = (gpu.seismograms(), ref.seismograms(), ulp.seismograms(), rel)
     if pb.fault is not None:
-        r_ref, r_gpu = ref.fault_records(), gpu.fault_records()
+        r_gpu, r_ref, r_ulp = gpu.fault_records(), ref.fault_records(), ulp.fault_records()
         assert r_gpu.shape == r_ref.shape
+        rsf = pb.fault["law"] == "RSF"
         for q, nm in enumerate(("slip", "slip_rate", "shear", "normal", "mu", "tstick")):
             if rsf and nm == "tstick":
                 continue  # unset in the reference's RSF branch
             if nm == "normal":
                 scale = np.abs(pb.fault["T0"][:, 1]).max() * np.sqrt(r_ref[:, q].size)
-                errs[nm], tols[nm] = np.linalg.norm(r_gpu[:, q] - r_ref[:, q]) / scale, TOL
+                err = (lambda x, y, sc=scale: np.linalg.norm(x - y) / sc)
             else:
-                errs[nm], tols[nm] = rel(r_gpu[:, q], r_ref[:, q]), TOL
-    if pb.fault is not None:
-        p_ref, p_gpu = ref.potency(), gpu.potency()
+                err = rel
+            rows[nm] = (r_gpu[:, q], r_ref[:, q], r_ulp[:, q], err)
+        p_gpu, p_ref, p_ulp = gpu.potency(), ref.potency(), ulp.potency()
         assert p_gpu.shape == p_ref.shape
-        # parallel (fixed tree) vs sequential sums: absolute error per column against the
-        # column's largest magnitude
-        scale = np.maximum(np.abs(p_ref).max(axis=0), 1e-300)
-        errs["potency"], tols["potency"] = float((np.abs(p_gpu - p_ref) / scale).max()), 1e-10
-    print("  " + "  ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
-    bad = {k: v for k, v in errs.items() if not v < tols[k]}
+        # against the largest potency of the same field (displacement, velocity columns):
+        # single columns can be pure round-off (no opening on a planar fault)
+        h = p_ref.shape[1] // 2
+        scale = np.repeat([max(np.abs(p_ref[:, :h]).max(), 1e-300), max(np.abs(p_ref[:, h:]).max(), 1e-300)], h)
+        rows["potency"] = (p_gpu, p_ref, p_ulp, lambda x, y, sc=scale: float((np.abs(x - y) / sc).max()))
+    errs, tols, sens = {}, {}, {}
+    for nm, (x, y, z, err) in rows.items():
+        errs[nm] = err(x, y)
+        sens[nm] = err(z, y)
+        tols[nm] = TOL if sens[nm] <= 1e-11 else max(TOL, 10.0 * sens[nm])
+    print("  error " + "  ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+    print("  bar   " + "  ".join(f"{k}={v:.1e}" for k, v in tols.items()), flush=True)
+    bad = {k: (v, tols[k]) for k, v in errs.items() if not v < tols[k]}
     assert not bad, bad
+    if pb.fault is None or pb.fault["law"] != "RSF":
+        assert all(v == TOL for v in tols.values())  # the widening is for friction only
 
 
 def test_deterministic(oracle_api):
