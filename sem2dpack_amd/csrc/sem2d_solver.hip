@@ -48,10 +48,11 @@ struct AbsoHost {
 struct StructState {
   bool on = false, ready = false;
   int nx = 0, nz = 0, ezflt = 0, NX1 = 0, NZ1 = 0;
+  int ROW = 0;                    // storage slots per node row (4*TX per tile column)
   std::vector<int> gmap;          // reference node (0-based) -> storage index
   std::vector<int> gridx;         // reference node (0-based) -> grid index Z*NX1 + X
   int nvl = 0;                    // vertical tile-edge lines (stored column-contiguous)
-  size_t NS = 0;                  // field component stride: NX1*NZ1 + nvl*NZ1
+  size_t NS = 0;                  // field component stride: ROW*NZ1 + nvl*NZ1
   std::vector<int4> tiles;
   int ntiles = 0, per_xcd = 0;
   int pipe_wgs = 0;  // > 0: persistent pipelined tile kernel with this many workgroups
@@ -386,13 +387,16 @@ int record_fault(sem2d_solver* h) {
 // Storage index of grid node (X, Z): nodes on vertical tile-edge lines (X a multiple
 // of 4*TX, or the right domain edge) live column-contiguous after the grid block, so
 // that the column edges of the tiles are read and written with unit stride.
+// The other nodes are stored row by row, each tile column's 4*TX-1 nodes in a segment
+// of 4*TX slots (ROW = 4*TX * tile columns slots per row), so every segment starts on
+// a cache-line boundary.
 static size_t storage_index(const StructState& S, int X, int Z) {
   const int W = 4 * TX;
   int k = -1;
   if (X % W == 0) k = X / W;
   else if (X == S.NX1 - 1) k = S.nvl - 1;
-  if (k >= 0) return (size_t)S.NX1 * S.NZ1 + (size_t)k * S.NZ1 + Z;
-  return (size_t)Z * S.NX1 + X;
+  if (k >= 0) return (size_t)S.ROW * S.NZ1 + (size_t)k * S.NZ1 + Z;
+  return (size_t)Z * S.ROW + (size_t)(X / W) * W + (X % W) - 1;
 }
 
 bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
@@ -405,7 +409,11 @@ bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
   S.NZ1 = 4 * nz + 1 + (ez > 0 ? 1 : 0);
   if ((long)S.NX1 * S.NZ1 != s->npoin) return false;
   S.nvl = (nx + TX - 1) / TX + 1;
-  S.NS = (size_t)S.NX1 * S.NZ1 + (size_t)S.nvl * S.NZ1;
+  S.ROW = 4 * TX * (S.nvl - 1);
+  // component stride rounded to 4*TX slots so that every component (and every batch
+  // member placed at a multiple of it) keeps the segments line-aligned
+  S.NS = (size_t)S.ROW * S.NZ1 + (size_t)S.nvl * S.NZ1;
+  S.NS = (S.NS + 4 * TX - 1) / (4 * TX) * (4 * TX);
   S.gmap.assign(s->npoin, -1);
   std::vector<char> used((size_t)s->npoin, 0);
   for (int iz = 0; iz < nz; ++iz) {
@@ -440,7 +448,7 @@ bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
     for (int tz0 = r0; tz0 < r1; tz0 += TZ)
       for (int tx0 = 0; tx0 < nx; tx0 += TX) {
         const int nxe = std::min(TX, nx - tx0), nze = std::min(TZ, r1 - tz0);
-        S.tiles.push_back(make_int4(tx0, tz0, nxe | (nze << 8), 4 * tz0 + sh));
+        S.tiles.push_back(make_int4(tx0, tz0, nxe | (nze << 8) | (nx << 16), 4 * tz0 + sh));
       }
   }
   S.ntiles = (int)S.tiles.size();
@@ -502,7 +510,7 @@ int struct_tables(sem2d_solver* h, StructTables& X) {
   StructState& S = h->st;
   const int nd = h->ndof;
   X.tiles = S.tiles;
-  X.grids.assign(S.ntiles, make_int4(0, S.NX1, S.NZ1, 0));
+  X.grids.assign(S.ntiles, make_int4(0, S.ROW, S.NZ1, 0));
   // special nodes: point-source nodes inside a tile patch
   std::vector<int> src_g;  // grid indices of source nodes
   for (const SrcEntry& e : h->src_host) src_g.push_back(S.gridx[e.node]);
@@ -516,7 +524,7 @@ int struct_tables(sem2d_solver* h, StructTables& X) {
   ents.reserve((size_t)S.ntiles * PERIM_MAX);
   for (int t = 0; t < S.ntiles; ++t) {
     const int4 tg = S.tiles[t];
-    const int nxe = tg.z & 0xff, nze = tg.z >> 8;
+    const int nxe = tg.z & 0xff, nze = (tg.z >> 8) & 0xff;
     const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, X0 = 4 * tg.x, Z0 = tg.w;
     auto gi = [&](int Xn, int Zn) { return (int)storage_index(S, X0 + Xn, Z0 + Zn); };
     for (int Xn = 0; Xn < PX; ++Xn) ents.push_back({gi(Xn, 0), t, Xn});
@@ -2156,6 +2164,10 @@ void sem2d_batch_destroy(sem2d_batch* b) {
 // ===========================================================================
 struct AdjState {
   int nsteps = 0, seg = 0, nck = 0;
+  // injection plan of dJ/d(seismogram): distinct nodes, their contributions
+  int nu = 0;
+  int *unode = nullptr, *uoff = nullptr, *urec = nullptr;
+  double* uw = nullptr;
   size_t state_bytes = 0;  // d, v, a of one state
   std::vector<char*> ck, slot;
   char *ad = nullptr, *av = nullptr, *aa = nullptr, *rmass = nullptr;
@@ -2234,8 +2246,9 @@ int adj_backward(sem2d_solver* h, const double* gseis_host) {
     for (int n = s1; n > s0; --n) {
       const int j = n - s0;
       if (h->nrec && n % h->isamp == 0 && n / h->isamp < h->nsamp)
-        k_adj_inject<R><<<1, 64, 0, h->stream>>>(h->nrec, h->rec_node, h->rec_w,
-                                                 A.gseis + (size_t)(n / h->isamp) * h->ndof * h->nrec, inj, GS);
+        k_adj_inject<R><<<grid_for(A.nu, 64), 64, 0, h->stream>>>(
+            A.nu, A.unode, A.uoff, A.urec, A.uw, h->nrec, A.gseis + (size_t)(n / h->isamp) * h->ndof * h->nrec,
+            inj, GS);
       const R* dfw = (const R*)st_d(h, A.slot[j]);
       const R* afw = (const R*)st_a(h, A.slot[j]);
       k_adj_tile<R><<<8 * S.per_xcd, TILE_THREADS, 0, h->stream>>>(
@@ -2308,6 +2321,42 @@ int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
   if ((rc = adj_alloc(h, &p, (size_t)std::max(h->nsamp, 1) * h->ndof * std::max(h->nrec, 1) * sizeof(double))))
     return rc;
   A.gseis = (double*)p;
+  // injection plan: the receivers' nodes (or element nodes with weights), grouped by
+  // node, contributions in receiver order
+  if (h->nrec) {
+    const int per = h->rec_w ? NP : 1;
+    std::vector<int> rn((size_t)h->nrec * per);
+    std::vector<double> rw((size_t)h->nrec * per, 1.0);
+    HIPCHK(h, hipMemcpy(rn.data(), h->rec_node, rn.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (h->rec_w) HIPCHK(h, hipMemcpy(rw.data(), h->rec_w, rw.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::map<int, std::vector<std::pair<int, double>>> by;  // node -> (receiver, weight)
+    for (int r = 0; r < h->nrec; ++r)
+      for (int k = 0; k < per; ++k) by[rn[(size_t)r * per + k]].push_back({r, rw[(size_t)r * per + k]});
+    std::vector<int> un, off(1, 0), rec;
+    std::vector<double> w;
+    for (auto& kv : by) {
+      un.push_back(kv.first);
+      for (auto& e : kv.second) {
+        rec.push_back(e.first);
+        w.push_back(e.second);
+      }
+      off.push_back((int)rec.size());
+    }
+    A.nu = (int)un.size();
+    void* q;
+    if ((rc = adj_alloc(h, &q, un.size() * sizeof(int)))) return rc;
+    A.unode = (int*)q;
+    if ((rc = adj_alloc(h, &q, off.size() * sizeof(int)))) return rc;
+    A.uoff = (int*)q;
+    if ((rc = adj_alloc(h, &q, rec.size() * sizeof(int)))) return rc;
+    A.urec = (int*)q;
+    if ((rc = adj_alloc(h, &q, w.size() * sizeof(double)))) return rc;
+    A.uw = (double*)q;
+    HIPCHK(h, hipMemcpy(A.unode, un.data(), un.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(A.uoff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(A.urec, rec.data(), rec.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(A.uw, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   // inverse mass of both components in storage order
   std::vector<double> rm((size_t)h->ndof * h->fstride, 0.0);
   for (int c = 0; c < h->ndof; ++c)

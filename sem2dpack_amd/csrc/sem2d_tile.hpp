@@ -38,8 +38,8 @@ constexpr int SLOTS = PERIM_MAX + MAXSPECIAL;
 struct TileDev {
   int ntiles;
   int per_xcd;        // tiles per XCD slice of the launch
-  const int4* tiles;  // (tx0, tz0, nxe | nze << 8, Z0)
-  const int4* grids;  // per tile: (node_base, NX1, NZ1, cell_base) of the grid the tile lies in
+  const int4* tiles;  // (tx0, tz0, nxe | nze << 8 | nx << 16, Z0), nx = elements per grid row
+  const int4* grids;  // per tile: (node_base, ROW, NZ1, cell_base) of the grid the tile lies in
   const int* spec_cnt;
   const int* spec_idx;  // [ntiles][MAXSPECIAL] local patch indices
   size_t NGp;           // component stride of the field arrays
@@ -47,24 +47,26 @@ struct TileDev {
 };
 
 // Storage index of patch node (X, Z) of the tile at (X0, Z0) whose left and right
-// columns lie on vertical tile-edge lines kL and kR.  A grid of NX1 x NZ1 nodes
-// starting at node_base keeps the nodes of those lines column-contiguous after its
-// NX1*NZ1 grid block; other nodes use grid order Z*NX1 + X.  Several grids (one per
-// scenario of a batch) can share the field arrays, each at its own node_base.
+// columns lie on vertical tile-edge lines kL and kR.  A grid of NZ1 node rows starting
+// at node_base stores, row after row, the nodes strictly between its vertical
+// tile-edge lines: each tile column's 4*TX-1 nodes in a segment of 4*TX slots (ROW
+// slots per row), so that the segments start on cache-line boundaries; the nodes of
+// the tile-edge lines follow column by column (NZ1 each, unit stride along Z).
+// Several grids (one per scenario of a batch) can share the field arrays, each at its
+// own node_base.
 __device__ __forceinline__ size_t node_index(const int4 gr, int X, int Z, int PX, int X0, int Z0, int kL,
                                              int kR) {
   const size_t base = (size_t)gr.x, grid = (size_t)gr.y * gr.z;
   if (X == 0) return base + grid + (size_t)kL * gr.z + (Z0 + Z);
   if (X == PX - 1) return base + grid + (size_t)kR * gr.z + (Z0 + Z);
-  return base + (size_t)(Z0 + Z) * gr.y + X0 + X;
+  return base + (size_t)(Z0 + Z) * gr.y + X0 + X - 1;
 }
 
 // element (cell) index of local element (lx, lz) of a tile at (tx0, tz0).  The
 // fourth descriptor field is the grid's first cell for per-cell coefficients; with
 // shared coefficients (one element array per grid) it is the offset of the grid's
 // array instead and the cell index is not used.
-__device__ __forceinline__ size_t cell_index(const int4 gr, int tx0, int tz0, int lx, int lz) {
-  const int nx = (gr.y - 1) >> 2;
+__device__ __forceinline__ size_t cell_index(const int4 gr, int nx, int tx0, int tz0, int lx, int lz) {
   return (size_t)gr.w + (size_t)(tz0 + lz) * nx + tx0 + lx;
 }
 
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
   const int tid = threadIdx.x;
   const int4 tg = T.tiles[t];
   const int4 gr = T.grids[t];
-  const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
+  const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = (tg.z >> 8) & 0xff, Z0 = tg.w;
   const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
   const int X0 = 4 * tx0;
   const size_t S = T.NGp;
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
       if (active) {
         const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
         base = 4 * lz * PX + 4 * lx;
-        const size_t cell = cell_index(gr, tx0, tz0, lx, lz);
+        const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
         R A[NELAST];
         load_coefs<R, NDOF, NELAST, MODE>(grid_coefs<R, MODE>(coefs, gr), geom, cell, pe, A);
         R dxi[NDOF], deta[NDOF];
@@ -232,8 +234,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
   }
 }
 
-// Persistent, software-pipelined variant of k_tile.  Each workgroup walks a
-// contiguous run of tiles [t0, t1) (runs are grouped per XCD).  No tile waits on a
+// Persistent, software-pipelined variant of k_tile.  Each workgroup walks a strided
+// sequence of tiles inside its XCD's share (see below).  No tile waits on a
 // global load issued during that tile: everything a tile reads comes from registers
 // filled while the previous tile was being processed.
 //   * d, v, a of the next tile's patch nodes are issued at the start of the current
@@ -281,12 +283,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
 
   const int nwg = gridDim.x;
   const int bid = blockIdx.x;
-  // workgroup -> run of tiles: XCD x = bid & 7 owns the x-th eighth of the tiles
-  const int per_x = (nwg + 7) >> 3;
-  const int w = (bid & 7) * per_x + (bid >> 3);
-  const int t0 = w * tiles_per_wg;
-  const int t1 = min(t0 + tiles_per_wg, T.ntiles);
-  if (t0 >= t1) return;
+  // Workgroups reach the 8 XCDs round-robin (XCD = bid & 7).  XCD x owns the x-th
+  // eighth of the tiles [s0, s1); its W workgroups take them interleaved (workgroup k:
+  // s0 + k, s0 + k + W, ...), so at any time one XCD works on consecutive tiles and the
+  // cache lines and halo nodes shared by x-neighbours are fetched once into its L2.
+  (void)tiles_per_wg;
+  const int W = nwg >> 3;  // nwg is a multiple of 8
+  const int slice = (T.ntiles + 7) >> 3;
+  const int s0 = (bid & 7) * slice;
+  const int s1 = min(s0 + slice, T.ntiles);
+  const int t0 = s0 + (bid >> 3);
+  if (t0 >= s1) return;
   const int tid = threadIdx.x;
   const size_t S = T.NGp;
   const int le = tid / NP;
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   auto issue_nodes = [&](int t) {
     const int4 tg = T.tiles[t];
     const int4 gr = T.grids[t];
-    const int tx0 = tg.x, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
+    const int tx0 = tg.x, nxe = tg.z & 0xff, nze = (tg.z >> 8) & 0xff, Z0 = tg.w;
     const int PX = 4 * nxe + 1, NPT = PX * (4 * nze + 1), X0 = 4 * tx0;
     const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
 #pragma unroll
@@ -329,7 +336,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   auto issue_elems = [&](int t) {
     const int4 tg = T.tiles[t];
     const int4 gr = T.grids[t];
-    const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
+    const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = (tg.z >> 8) & 0xff, Z0 = tg.w;
     {
       const int PX = 4 * nxe + 1, NPT = PX * (4 * nze + 1), X0 = 4 * tx0;
       const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
@@ -346,7 +353,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
       if (tid < nxe * nze * 3) {
         const int e = tid / 3, k = tid - 3 * (tid / 3);
         const int lx = e % nxe, lz = e / nxe;
-        pg = geom[cell_index(gr, tx0, tz0, lx, lz) * 3 + k];
+        pg = geom[cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz) * 3 + k];
       }
 #pragma unroll
       for (int q = 0; q < PB_MAXB; ++q) {
@@ -356,7 +363,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
         const int ce = b0 + le;
         if (tid < ETHREADS && ce < ncx * ncz) {
           const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
-          const size_t cell = cell_index(gr, tx0, tz0, lx, lz);
+          const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
 #pragma unroll
           for (int c = 0; c < NDOF; ++c) lm[q][c] = coefs[cell * NDOF * NP + c * NP + pe];
         }
@@ -368,10 +375,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   issue_elems(t0);
 
 #pragma unroll 1
-  for (int t = t0; t < t1; ++t) {
+  for (int t = t0; t < s1; t += W) {
     const int4 tg = T.tiles[t];
     const int4 gr = T.grids[t];
-    const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
+    const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = (tg.z >> 8) & 0xff, Z0 = tg.w;
     const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
     const int X0 = 4 * tx0;
     const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
@@ -398,7 +405,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
     if (psp >= 0) sflag[psp] = (short)tid;
     psp = -1;
     // overlap: node loads of the next tile fly during this tile's element phase
-    if (t + 1 < t1) issue_nodes(t + 1);
+    if (t + W < s1) issue_nodes(t + W);
 
 #pragma unroll
     for (int q = 0; q < PB_MAXB; ++q) {
@@ -431,7 +438,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
             A[1] = w2 * lm[q][0] * g1;
           }
         } else {
-          const size_t cell = cell_index(gr, tx0, tz0, lx, lz);
+          const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
           load_coefs<R, NDOF, NELAST, MODE>(grid_coefs<R, MODE>(coefs, gr), geom, cell, pe, A);
         }
         R dxi[NDOF], deta[NDOF];
@@ -485,7 +492,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
       __syncthreads();
     }
     // element loads of the next tile, ahead of this tile's stores
-    if (t + 1 < t1) issue_elems(t + 1);
+    if (t + W < s1) issue_elems(t + W);
 
     const size_t fbase = (size_t)t * SLOTS;
 #pragma unroll
@@ -648,7 +655,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
   const int tid = threadIdx.x;
   const int4 tg = T.tiles[t];
   const int4 gr = T.grids[t];
-  const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = tg.z >> 8, Z0 = tg.w;
+  const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = (tg.z >> 8) & 0xff, Z0 = tg.w;
   const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
   const int X0 = 4 * tx0;
   const size_t S = T.NGp;
@@ -694,7 +701,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
       if (active) {
         const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
         base = 4 * lz * PX + 4 * lx;
-        cell = cell_index(gr, tx0, tz0, lx, lz);
+        cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
         const R w = (R)c_W2[pe];
         const R g0 = geom[cell * 3 + 0], g1 = geom[cell * 3 + 1], g2 = geom[cell * 3 + 2];
         const R la = coefs[cell * 2 * NP + pe], mu = coefs[cell * 2 * NP + NP + pe];
@@ -828,24 +835,22 @@ __global__ __launch_bounds__(256) void k_adj_deferred(int ndef, const DefNode* _
 }
 
 // dJ/d(recorded field) of one sample added at the receiver nodes, or spread over the
-// element nodes with the interpolation weights w (the transpose of the interpolation);
-// one thread, since nodes may repeat
+// element nodes with the interpolation weights (the transpose of the interpolation).
+// One lane per distinct node u: its contributions (receiver rec[k], weight w[k], k in
+// [off[u], off[u+1]), in receiver order) are added one after the other, which is the
+// order a receiver-by-receiver loop would use.
 template <typename R>
-__global__ void k_adj_inject(int nrec, const int* __restrict__ node, const double* __restrict__ w,
+__global__ void k_adj_inject(int nu, const int* __restrict__ unode, const int* __restrict__ off,
+                             const int* __restrict__ rec, const double* __restrict__ w, int nrec,
                              const double* __restrict__ g, R* __restrict__ fld, size_t GS) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int r = 0; r < nrec; ++r)
-    for (int c = 0; c < 2; ++c) {
-      const double gr = g[(size_t)c * nrec + r];
-      if (w) {
-        for (int k = 0; k < NP; ++k) {
-          const size_t i = c * GS + node[(size_t)r * NP + k];
-          fld[i] = (R)((double)fld[i] + w[(size_t)r * NP + k] * gr);
-        }
-      } else {
-        fld[c * GS + node[r]] = (R)((double)fld[c * GS + node[r]] + gr);
-      }
-    }
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nu) return;
+  for (int c = 0; c < 2; ++c) {
+    const size_t i = c * GS + unode[u];
+    R x = fld[i];
+    for (int k = off[u]; k < off[u + 1]; ++k) x = (R)((double)x + w[k] * g[(size_t)c * nrec + rec[k]]);
+    fld[i] = x;
+  }
 }
 
 }  // namespace sem2d
