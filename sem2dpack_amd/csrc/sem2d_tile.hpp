@@ -272,8 +272,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   static_assert(3 * TX * TZ <= NT && MAXSPECIAL <= NT, "per-tile tables larger than the workgroup");
   __shared__ R Vp[NDOF][PMAX];  // predicted velocity
   __shared__ R Rm[PMAX];        // inverse mass of the patch nodes
-  __shared__ R T1[EPB][NDOF][NP];
-  __shared__ R T2[EPB][NDOF][NP];
+  __shared__ R T1b[2][EPB][NDOF][NP];  // double-buffered element stresses
+  __shared__ R T2b[2][EPB][NDOF][NP];
   __shared__ short sflag[PMAX];
   // H, GLL weight products and per-element geometric factors live in LDS (no global
   // load inside the element phase).
@@ -302,6 +302,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   if (tid < NP) {
     sH[tid] = c_H[tid];
     sW2[tid] = c_W2[tid];
+  }
+  // this lane's element point (ii, jj) is the same for every tile: keep the H entries
+  // its contractions use in registers
+  const R w2 = (R)c_W2[pe];
+  R hA[NG], hB[NG], hC[NG], hD[NG];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) {
+    hA[k] = (R)c_H[k * NG + ii];
+    hB[k] = (R)c_H[k * NG + jj];
+    hC[k] = (R)c_H[ii * NG + k];
+    hD[k] = (R)c_H[jj * NG + k];
   }
 
   // prefetch registers of the next tile: node values ...
@@ -407,89 +418,105 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
     // overlap: node loads of the next tile fly during this tile's element phase
     if (t + W < s1) issue_nodes(t + W);
 
+    // Element batches, software-pipelined over one barrier per batch: in phase q the
+    // lanes compute the stresses of batch q (into T[q & 1]) and assemble batch q-1 (from
+    // T[(q-1) & 1]) into Fn.  Batch q-1's nodes belong to elements of one colour, and
+    // the elements of one colour share no node, so the per-node summation order is the
+    // colour order, as in the two-barrier form.
+    bool pv_valid = false, pv_active = false;
+    int pv_base = 0, nvb = 0;
 #pragma unroll
-    for (int q = 0; q < PB_MAXB; ++q) {
-      const int cc = q / NBATCH, b0 = (q % NBATCH) * EPB;
-      const int cx = cc & 1, cz = cc >> 1;
-      const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
-      const int nce = ncx * ncz;
-      if (b0 >= nce) continue;  // uniform across the workgroup
-      const int ce = b0 + le;
-      const bool active = (tid < ETHREADS) && (ce < nce);
+    for (int q = 0; q <= PB_MAXB; ++q) {
+      bool has = false, active = false;
       int base = 0;
-      if (active) {
-        const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
-        base = 4 * lz * PX + 4 * lx;
-        R A[NELAST];
-        if constexpr (MODE == COEF_COMPACT) {
-          const R w2 = (R)sW2[pe];
-          const int el = lz * nxe + lx;
-          const R g0 = Gm[el][0], g1 = Gm[el][1], g2 = Gm[el][2];
+      if (q < PB_MAXB) {
+        const int cc = q / NBATCH, b0 = (q % NBATCH) * EPB;
+        const int cx = cc & 1, cz = cc >> 1;
+        const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
+        const int nce = ncx * ncz;
+        has = b0 < nce;  // uniform across the workgroup
+        const int ce = b0 + le;
+        active = has && (tid < ETHREADS) && (ce < nce);
+        if (active) {
+          const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
+          base = 4 * lz * PX + 4 * lx;
+          R A[NELAST];
+          if constexpr (MODE == COEF_COMPACT) {
+            const int el = lz * nxe + lx;
+            const R g0 = Gm[el][0], g1 = Gm[el][1], g2 = Gm[el][2];
+            if constexpr (NDOF == 2) {
+              const R la = lm[q][0], mu = lm[q][1], kx = la + 2 * mu;
+              A[0] = w2 * kx * g0;
+              A[1] = w2 * la * g2;
+              A[2] = w2 * kx * g1;
+              A[3] = w2 * mu * g1;
+              A[4] = w2 * mu * g2;
+              A[5] = w2 * mu * g0;
+            } else {
+              A[0] = w2 * lm[q][0] * g0;
+              A[1] = w2 * lm[q][0] * g1;
+            }
+          } else {
+            const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
+            load_coefs<R, NDOF, NELAST, MODE>(grid_coefs<R, MODE>(coefs, gr), geom, cell, pe, A);
+          }
+          R dxi[NDOF], deta[NDOF];
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) {
+            R s1 = 0, s2 = 0;
+#pragma unroll
+            for (int k = 0; k < NG; ++k) {
+              s1 += hA[k] * Ud[c][base + jj * PX + k];
+              s2 += Ud[c][base + k * PX + ii] * hB[k];
+            }
+            dxi[c] = s1;
+            deta[c] = s2;
+          }
+          R(*T1)[NDOF][NP] = T1b[nvb & 1];
+          R(*T2)[NDOF][NP] = T2b[nvb & 1];
           if constexpr (NDOF == 2) {
-            const R la = lm[q][0], mu = lm[q][1], kx = la + 2 * mu;
-            A[0] = w2 * kx * g0;
-            A[1] = w2 * la * g2;
-            A[2] = w2 * kx * g1;
-            A[3] = w2 * mu * g1;
-            A[4] = w2 * mu * g2;
-            A[5] = w2 * mu * g0;
+            if constexpr (NELAST == 6) {
+              T1[le][0][pe] = A[0] * dxi[0] + A[1] * deta[1];
+              T2[le][0][pe] = A[3] * (deta[0] + dxi[1]);
+              T1[le][1][pe] = A[4] * deta[0] + A[5] * dxi[1];
+              T2[le][1][pe] = A[1] * dxi[0] + A[2] * deta[1];
+            } else {
+              T1[le][0][pe] = A[0] * dxi[0] + A[6] * deta[0] + A[7] * dxi[1] + A[1] * deta[1];
+              T2[le][0][pe] = A[6] * dxi[0] + A[3] * deta[0] + A[4] * dxi[1] + A[8] * deta[1];
+              T1[le][1][pe] = A[7] * dxi[0] + A[4] * deta[0] + A[5] * dxi[1] + A[9] * deta[1];
+              T2[le][1][pe] = A[1] * dxi[0] + A[8] * deta[0] + A[9] * dxi[1] + A[2] * deta[1];
+            }
           } else {
-            A[0] = w2 * lm[q][0] * g0;
-            A[1] = w2 * lm[q][0] * g1;
-          }
-        } else {
-          const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
-          load_coefs<R, NDOF, NELAST, MODE>(grid_coefs<R, MODE>(coefs, gr), geom, cell, pe, A);
-        }
-        R dxi[NDOF], deta[NDOF];
-#pragma unroll
-        for (int c = 0; c < NDOF; ++c) {
-          R s1 = 0, s2 = 0;
-#pragma unroll
-          for (int k = 0; k < NG; ++k) {
-            s1 += (R)sH[k * NG + ii] * Ud[c][base + jj * PX + k];
-            s2 += Ud[c][base + k * PX + ii] * (R)sH[k * NG + jj];
-          }
-          dxi[c] = s1;
-          deta[c] = s2;
-        }
-        if constexpr (NDOF == 2) {
-          if constexpr (NELAST == 6) {
-            T1[le][0][pe] = A[0] * dxi[0] + A[1] * deta[1];
-            T2[le][0][pe] = A[3] * (deta[0] + dxi[1]);
-            T1[le][1][pe] = A[4] * deta[0] + A[5] * dxi[1];
-            T2[le][1][pe] = A[1] * dxi[0] + A[2] * deta[1];
-          } else {
-            T1[le][0][pe] = A[0] * dxi[0] + A[6] * deta[0] + A[7] * dxi[1] + A[1] * deta[1];
-            T2[le][0][pe] = A[6] * dxi[0] + A[3] * deta[0] + A[4] * dxi[1] + A[8] * deta[1];
-            T1[le][1][pe] = A[7] * dxi[0] + A[4] * deta[0] + A[5] * dxi[1] + A[9] * deta[1];
-            T2[le][1][pe] = A[1] * dxi[0] + A[8] * deta[0] + A[9] * dxi[1] + A[2] * deta[1];
-          }
-        } else {
-          if constexpr (NELAST == 2) {
-            T1[le][0][pe] = A[0] * dxi[0];
-            T2[le][0][pe] = A[1] * deta[0];
-          } else {
-            T1[le][0][pe] = A[0] * dxi[0] + A[2] * deta[0];
-            T2[le][0][pe] = A[2] * dxi[0] + A[1] * deta[0];
+            if constexpr (NELAST == 2) {
+              T1[le][0][pe] = A[0] * dxi[0];
+              T2[le][0][pe] = A[1] * deta[0];
+            } else {
+              T1[le][0][pe] = A[0] * dxi[0] + A[2] * deta[0];
+              T2[le][0][pe] = A[2] * dxi[0] + A[1] * deta[0];
+            }
           }
         }
       }
-      __syncthreads();
-      if (active) {
+      if (pv_active) {  // assemble the previous batch
+        R(*T1)[NDOF][NP] = T1b[(nvb - 1) & 1];
+        R(*T2)[NDOF][NP] = T2b[(nvb - 1) & 1];
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
           R s1 = 0, s2 = 0;
 #pragma unroll
           for (int k = 0; k < NG; ++k) {
-            s1 += (R)sH[ii * NG + k] * T1[le][c][jj * NG + k];
-            s2 += T2[le][c][k * NG + ii] * (R)sH[jj * NG + k];
+            s1 += hC[k] * T1[le][c][jj * NG + k];
+            s2 += T2[le][c][k * NG + ii] * hD[k];
           }
-          R* fp = &Fn[c][base + jj * PX + ii];
+          R* fp = &Fn[c][pv_base + jj * PX + ii];
           *fp = *fp + (s1 + s2);
         }
       }
-      __syncthreads();
+      if (has || pv_valid) __syncthreads();
+      pv_valid = has;
+      pv_active = active;
+      pv_base = base;
+      if (has) ++nvb;
     }
     // element loads of the next tile, ahead of this tile's stores
     if (t + W < s1) issue_elems(t + W);
