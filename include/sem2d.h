@@ -165,8 +165,9 @@ int sem2d_compute_fint(sem2d_solver *h, const double *d, double *f);
 /* Launch configuration of the structured-grid tile kernel (no counterpart in the
    reference, whose loop is serial): threads per workgroup (256 = 10-element batches,
    512 = 20-element batches) and persistent workgroups per CU (0: one workgroup per
-   tile; < 0: automatic, the persistent kernel once there are >= 4 tiles per
-   workgroup).  Results do not depend on it (same summation order). */
+   tile; < 0: automatic, the persistent kernel with 2 (FP64) or 3 (FP32) workgroups
+   per CU once there are >= 4 tiles per workgroup).  Results do not depend on it (same
+   summation order). */
 int sem2d_set_tile_config(sem2d_solver *h, int threads, int wg_per_cu);
 
 /* Current step index (number of steps taken). */

@@ -456,12 +456,12 @@ bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
   {
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    int per_cu = 2;
+    // measured on MI355X (2000x1000-element P-SV): 256 threads (10-element batches)
+    // with 2 workgroups per CU for FP64 fields (LDS and VGPRs allow 2), 3 for FP32
+    int per_cu = h->fp32 ? 3 : 2;
     const char* env = getenv("SEM2D_TILE_WG_PER_CU");
     if (env) per_cu = atoi(env);
-    // measured on MI355X: 256 threads (10-element batches) for FP64 fields, 512
-    // threads (20-element batches) for FP32 fields
-    S.pipe_threads = h->fp32 ? 512 : 256;
+    S.pipe_threads = 256;
     const char* envt = getenv("SEM2D_TILE_THREADS");
     if (envt) S.pipe_threads = atoi(envt) == 256 ? 256 : 512;
     // the persistent kernel pays off once there are several tiles per workgroup
@@ -1383,7 +1383,10 @@ int sem2d_set_tile_config(sem2d_solver* h, int threads, int wg_per_cu) {
   int dev = 0, ncu = 256;
   if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   S.pipe_threads = threads;
-  if (wg_per_cu < 0) S.pipe_wgs = S.ntiles >= 4 * ncu * 2 ? ncu * 2 : 0;  // automatic
+  if (wg_per_cu < 0) {  // automatic: the measured defaults
+    const int pc = h->fp32 ? 3 : 2;
+    S.pipe_wgs = S.ntiles >= 4 * ncu * pc ? ncu * pc : 0;
+  }
   else S.pipe_wgs = ncu * wg_per_cu;
   S.pipe_wgs = ((S.pipe_wgs + 7) / 8) * 8;
   return 0;
@@ -1946,7 +1949,7 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   if ((rc = bupload(b, &b->rec_node, rn))) return rc;
   if ((rc = balloc(b, &b->seis, (size_t)std::max(b->nsamp, 1) * nd * std::max(b->nrec, 1)))) return rc;
   // default launch configuration (sem2d_batch_tune measures the alternatives)
-  batch_set_config(b, b->fp32 ? 512 : 256, ntiles >= 8 * b->ncu ? 2 : 0);
+  batch_set_config(b, 256, ntiles >= 8 * b->ncu ? (b->fp32 ? 3 : 2) : 0);
   // records of step 0, then the counter moves to step 1
   if (b->fp32) launch_batch_record<float>(b);
   else launch_batch_record<double>(b);
@@ -1993,7 +1996,7 @@ int sem2d_batch_tune(sem2d_batch* b, int trial_steps, int* choice) {
   for (auto& k : blks)
     if ((rc = keep(k.p, k.bytes))) break;
   const int it0 = b->it;
-  const int cand[][2] = {{512, 2}, {256, 2}, {512, 1}, {256, 1}, {256, 0}};
+  const int cand[][2] = {{256, 3}, {512, 2}, {256, 2}, {512, 1}, {256, 1}, {256, 0}};
   const int ncand = sizeof(cand) / sizeof(cand[0]);
   double best = 1e30;
   int ibest = 0;
