@@ -234,6 +234,26 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
   }
 }
 
+// LDS node value of NDOF components stored together (one vector access for P-SV)
+template <typename R, int N>
+struct LVec {
+  using T = R;
+};
+template <typename R>
+struct LVec<R, 2> {
+  typedef R T __attribute__((ext_vector_type(2)));
+};
+template <typename R, int N>
+__device__ __forceinline__ typename LVec<R, N>::T lv_make(const R* x) {
+  if constexpr (N == 2) return typename LVec<R, N>::T{x[0], x[1]};
+  else return x[0];
+}
+template <typename R, int N>
+__device__ __forceinline__ R lv_get(const typename LVec<R, N>::T& v, int c) {
+  if constexpr (N == 2) return c == 0 ? v.x : v.y;
+  else return v;
+}
+
 // Persistent, software-pipelined variant of k_tile.  Each workgroup walks a strided
 // sequence of tiles inside its XCD's share (see below).  No tile waits on a
 // global load issued during that tile: everything a tile reads comes from registers
@@ -262,18 +282,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   const R* __restrict__ dsrc = OOP ? din : d;
   const R* __restrict__ vsrc = OOP ? vin : v;
   const R* __restrict__ asrc = OOP ? ain : a;
-  __shared__ R Ud[NDOF][PMAX];
-  __shared__ R Fn[NDOF][PMAX];
+  // node values with both components side by side: one LDS access moves both
+  using LV = typename LVec<R, NDOF>::T;
+  __shared__ LV Ud[PMAX];
+  __shared__ LV Fn[PMAX];
   constexpr int NITER = (PMAX + NT - 1) / NT;                       // patch nodes per thread
   constexpr int NBATCH = (TX * TZ / 4 + EPB - 1) / EPB;              // batches per colour
   constexpr int PB_MAXB = 4 * NBATCH;                               // point tasks per thread
   constexpr int ETHREADS = EPB * NP;
   static_assert(ETHREADS <= NT, "element batch larger than the workgroup");
   static_assert(3 * TX * TZ <= NT && MAXSPECIAL <= NT, "per-tile tables larger than the workgroup");
-  __shared__ R Vp[NDOF][PMAX];  // predicted velocity
+  __shared__ LV Vp[PMAX];  // predicted velocity
   __shared__ R Rm[PMAX];        // inverse mass of the patch nodes
-  __shared__ R T1b[2][EPB][NDOF][NP];  // double-buffered element stresses
-  __shared__ R T2b[2][EPB][NDOF][NP];
+  __shared__ LV T1b[2][EPB][NP];  // double-buffered element stresses
+  __shared__ LV T2b[2][EPB][NP];
   __shared__ short sflag[PMAX];
   // H, GLL weight products and per-element geometric factors live in LDS (no global
   // load inside the element phase).
@@ -399,12 +421,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
     for (int r = 0; r < NITER; ++r) {
       const int p = tid + NT * r;
       if (p < NPT) {
+        R u[NDOF], w[NDOF];
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
-          Ud[c][p] = pd[r][c] + dt * pv[r][c] + c1 * pa[r][c];
-          Vp[c][p] = pv[r][c] + c2 * pa[r][c];
-          Fn[c][p] = 0;
+          u[c] = pd[r][c] + dt * pv[r][c] + c1 * pa[r][c];
+          w[c] = pv[r][c] + c2 * pa[r][c];
         }
+        Ud[p] = lv_make<R, NDOF>(u);
+        Vp[p] = lv_make<R, NDOF>(w);
+        Fn[p] = LV{};
         Rm[p] = pm[r];
         sflag[p] = -1;
       }
@@ -462,55 +487,64 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
           }
           R dxi[NDOF], deta[NDOF];
 #pragma unroll
-          for (int c = 0; c < NDOF; ++c) {
-            R s1 = 0, s2 = 0;
+          for (int c = 0; c < NDOF; ++c) dxi[c] = deta[c] = 0;
 #pragma unroll
-            for (int k = 0; k < NG; ++k) {
-              s1 += hA[k] * Ud[c][base + jj * PX + k];
-              s2 += Ud[c][base + k * PX + ii] * hB[k];
+          for (int k = 0; k < NG; ++k) {
+            const LV u1 = Ud[base + jj * PX + k], u2 = Ud[base + k * PX + ii];
+#pragma unroll
+            for (int c = 0; c < NDOF; ++c) {
+              dxi[c] += hA[k] * lv_get<R, NDOF>(u1, c);
+              deta[c] += lv_get<R, NDOF>(u2, c) * hB[k];
             }
-            dxi[c] = s1;
-            deta[c] = s2;
           }
-          R(*T1)[NDOF][NP] = T1b[nvb & 1];
-          R(*T2)[NDOF][NP] = T2b[nvb & 1];
+          LV(*T1) = T1b[nvb & 1][le];
+          LV(*T2) = T2b[nvb & 1][le];
+          R t1[NDOF], t2[NDOF];
           if constexpr (NDOF == 2) {
             if constexpr (NELAST == 6) {
-              T1[le][0][pe] = A[0] * dxi[0] + A[1] * deta[1];
-              T2[le][0][pe] = A[3] * (deta[0] + dxi[1]);
-              T1[le][1][pe] = A[4] * deta[0] + A[5] * dxi[1];
-              T2[le][1][pe] = A[1] * dxi[0] + A[2] * deta[1];
+              t1[0] = A[0] * dxi[0] + A[1] * deta[1];
+              t2[0] = A[3] * (deta[0] + dxi[1]);
+              t1[1] = A[4] * deta[0] + A[5] * dxi[1];
+              t2[1] = A[1] * dxi[0] + A[2] * deta[1];
             } else {
-              T1[le][0][pe] = A[0] * dxi[0] + A[6] * deta[0] + A[7] * dxi[1] + A[1] * deta[1];
-              T2[le][0][pe] = A[6] * dxi[0] + A[3] * deta[0] + A[4] * dxi[1] + A[8] * deta[1];
-              T1[le][1][pe] = A[7] * dxi[0] + A[4] * deta[0] + A[5] * dxi[1] + A[9] * deta[1];
-              T2[le][1][pe] = A[1] * dxi[0] + A[8] * deta[0] + A[9] * dxi[1] + A[2] * deta[1];
+              t1[0] = A[0] * dxi[0] + A[6] * deta[0] + A[7] * dxi[1] + A[1] * deta[1];
+              t2[0] = A[6] * dxi[0] + A[3] * deta[0] + A[4] * dxi[1] + A[8] * deta[1];
+              t1[1] = A[7] * dxi[0] + A[4] * deta[0] + A[5] * dxi[1] + A[9] * deta[1];
+              t2[1] = A[1] * dxi[0] + A[8] * deta[0] + A[9] * dxi[1] + A[2] * deta[1];
             }
           } else {
             if constexpr (NELAST == 2) {
-              T1[le][0][pe] = A[0] * dxi[0];
-              T2[le][0][pe] = A[1] * deta[0];
+              t1[0] = A[0] * dxi[0];
+              t2[0] = A[1] * deta[0];
             } else {
-              T1[le][0][pe] = A[0] * dxi[0] + A[2] * deta[0];
-              T2[le][0][pe] = A[2] * dxi[0] + A[1] * deta[0];
+              t1[0] = A[0] * dxi[0] + A[2] * deta[0];
+              t2[0] = A[2] * dxi[0] + A[1] * deta[0];
             }
           }
+          T1[pe] = lv_make<R, NDOF>(t1);
+          T2[pe] = lv_make<R, NDOF>(t2);
         }
       }
       if (pv_active) {  // assemble the previous batch
-        R(*T1)[NDOF][NP] = T1b[(nvb - 1) & 1];
-        R(*T2)[NDOF][NP] = T2b[(nvb - 1) & 1];
+        const LV* T1 = T1b[(nvb - 1) & 1][le];
+        const LV* T2 = T2b[(nvb - 1) & 1][le];
+        R s1[NDOF], s2[NDOF], f[NDOF];
 #pragma unroll
-        for (int c = 0; c < NDOF; ++c) {
-          R s1 = 0, s2 = 0;
+        for (int c = 0; c < NDOF; ++c) s1[c] = s2[c] = 0;
 #pragma unroll
-          for (int k = 0; k < NG; ++k) {
-            s1 += hC[k] * T1[le][c][jj * NG + k];
-            s2 += T2[le][c][k * NG + ii] * hD[k];
+        for (int k = 0; k < NG; ++k) {
+          const LV a1 = T1[jj * NG + k], a2 = T2[k * NG + ii];
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) {
+            s1[c] += hC[k] * lv_get<R, NDOF>(a1, c);
+            s2[c] += lv_get<R, NDOF>(a2, c) * hD[k];
           }
-          R* fp = &Fn[c][pv_base + jj * PX + ii];
-          *fp = *fp + (s1 + s2);
         }
+        LV* fp = &Fn[pv_base + jj * PX + ii];
+        const LV f0 = *fp;
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) f[c] = lv_get<R, NDOF>(f0, c) + (s1[c] + s2[c]);
+        *fp = lv_make<R, NDOF>(f);
       }
       if (has || pv_valid) __syncthreads();
       pv_valid = has;
@@ -531,16 +565,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
         if (slot < 0 && sflag[p] >= 0) slot = PERIM_MAX + sflag[p];
         if (slot >= 0) {
 #pragma unroll
-          for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = Fn[c][p];
+          for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = lv_get<R, NDOF>(Fn[p], c);
         } else {
           const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
           const R rm = Rm[p];
 #pragma unroll
           for (int c = 0; c < NDOF; ++c) {
-            const R ak = Fn[c][p] * rm;
+            const R ak = lv_get<R, NDOF>(Fn[p], c) * rm;
             a[c * S + g] = ak;
-            v[c * S + g] = Vp[c][p] + c3 * ak;
-            d[c * S + g] = Ud[c][p] + c4 * ak;
+            v[c * S + g] = lv_get<R, NDOF>(Vp[p], c) + c3 * ak;
+            d[c * S + g] = lv_get<R, NDOF>(Ud[p], c) + c4 * ak;
           }
         }
       }
