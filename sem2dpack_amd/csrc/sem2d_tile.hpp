@@ -702,12 +702,13 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
     R* __restrict__ fpart, double* __restrict__ glam, double* __restrict__ gmu, double* __restrict__ mbar,
     R dt, R c1, R c2, R c3) {
   constexpr int NDOF = 2;
-  __shared__ R Uf[NDOF][PMAX];  // fb
-  __shared__ R Ud[NDOF][PMAX];  // forward d_n
-  __shared__ R Fn[NDOF][PMAX];
-  __shared__ R T1[FE_EPB][NDOF][NP];
-  __shared__ R T2[FE_EPB][NDOF][NP];
-  __shared__ double Gr[FE_EPB][2][NP];
+  using LV = typename LVec<R, NDOF>::T;  // both components of a node side by side
+  __shared__ LV Uf[PMAX];  // fb
+  __shared__ LV Ud[PMAX];  // forward d_n
+  __shared__ LV Fn[PMAX];
+  __shared__ LV T1[FE_EPB][NP];
+  __shared__ LV T2[FE_EPB][NP];
+  __shared__ double Gr[FE_EPB][NP][2];
   __shared__ short sflag[PMAX];
 
   const int bid = blockIdx.x;
@@ -721,6 +722,19 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
   const int X0 = 4 * tx0;
   const size_t S = T.NGp;
   const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
+  const int le = tid / NP;
+  const int pe = tid - le * NP;
+  const int ii = pe % NG, jj = pe / NG;
+  // the H entries and GLL weight of this lane's element point
+  const R w = (R)c_W2[pe];
+  R hA[NG], hB[NG], hC[NG], hD[NG];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) {
+    hA[k] = (R)c_H[k * NG + ii];
+    hB[k] = (R)c_H[k * NG + jj];
+    hC[k] = (R)c_H[ii * NG + k];
+    hD[k] = (R)c_H[jj * NG + k];
+  }
 
   R adr[NODE_ITERS][NDOF], avr[NODE_ITERS][NDOF];
 #pragma unroll
@@ -729,15 +743,18 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
     if (p < NPT) {
       const int Z = p / PX, X = p - Z * PX;
       const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
+      R f[NDOF], u[NDOF];
 #pragma unroll
       for (int c = 0; c < NDOF; ++c) {
         const R dk = ad[c * S + g], vk = av[c * S + g], ak = aa[c * S + g];
-        Uf[c][p] = (ak + c3 * vk) * rmass[c * S + g];
-        Ud[c][p] = dfw[c * S + g];
-        Fn[c][p] = 0;
+        f[c] = (ak + c3 * vk) * rmass[c * S + g];
+        u[c] = dfw[c * S + g];
         adr[r][c] = dk;
         avr[r][c] = vk;
       }
+      Uf[p] = lv_make<R, NDOF>(f);
+      Ud[p] = lv_make<R, NDOF>(u);
+      Fn[p] = LV{};
       sflag[p] = -1;
     }
   }
@@ -745,9 +762,6 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
   const int nspec = T.spec_cnt[t];
   if (tid < nspec) sflag[T.spec_idx[t * MAXSPECIAL + tid]] = (short)tid;
 
-  const int le = tid / NP;
-  const int pe = tid - le * NP;
-  const int ii = pe % NG, jj = pe / NG;
 #pragma unroll 1
   for (int cc = 0; cc < 4; ++cc) {
     const int cx = cc & 1, cz = cc >> 1;
@@ -763,58 +777,55 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
         const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
         base = 4 * lz * PX + 4 * lx;
         cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
-        const R w = (R)c_W2[pe];
         const R g0 = geom[cell * 3 + 0], g1 = geom[cell * 3 + 1], g2 = geom[cell * 3 + 2];
         const R la = coefs[cell * 2 * NP + pe], mu = coefs[cell * 2 * NP + NP + pe];
         const R kx = la + 2 * mu;
         const R A0 = w * kx * g0, A1 = w * la * g2, A2 = w * kx * g1;
         const R A3 = w * mu * g1, A4 = w * mu * g2, A5 = w * mu * g0;
-        R pxi[NDOF], peta[NDOF], gxi[NDOF], geta[NDOF];
+        R pxi[NDOF] = {0, 0}, peta[NDOF] = {0, 0}, gxi[NDOF] = {0, 0}, geta[NDOF] = {0, 0};
 #pragma unroll
-        for (int c = 0; c < NDOF; ++c) {
-          R s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+        for (int k = 0; k < NG; ++k) {
+          const LV f1 = Uf[base + jj * PX + k], f2 = Uf[base + k * PX + ii];
+          const LV u1 = Ud[base + jj * PX + k], u2 = Ud[base + k * PX + ii];
 #pragma unroll
-          for (int k = 0; k < NG; ++k) {
-            const R hx = Coef<R>::h(k, ii), he = Coef<R>::h(k, jj);
-            s1 += hx * Uf[c][base + jj * PX + k];
-            s2 += Uf[c][base + k * PX + ii] * he;
-            s3 += hx * Ud[c][base + jj * PX + k];
-            s4 += Ud[c][base + k * PX + ii] * he;
+          for (int c = 0; c < NDOF; ++c) {
+            pxi[c] += hA[k] * lv_get<R, NDOF>(f1, c);
+            peta[c] += lv_get<R, NDOF>(f2, c) * hB[k];
+            gxi[c] += hA[k] * lv_get<R, NDOF>(u1, c);
+            geta[c] += lv_get<R, NDOF>(u2, c) * hB[k];
           }
-          pxi[c] = s1;
-          peta[c] = s2;
-          gxi[c] = s3;
-          geta[c] = s4;
         }
         const R pxx = pxi[0], pex = peta[0], pxz = pxi[1], pez = peta[1];
         const R gxx = gxi[0], gex = geta[0], gxz = gxi[1], gez = geta[1];
-        T1[le][0][pe] = A0 * pxx + A1 * pez;
-        T2[le][0][pe] = A3 * pex + A4 * pxz;
-        T1[le][1][pe] = A3 * pex + A5 * pxz;
-        T2[le][1][pe] = A1 * pxx + A2 * pez;
+        T1[le][pe] = LV{A0 * pxx + A1 * pez, A3 * pex + A5 * pxz};
+        T2[le][pe] = LV{A3 * pex + A4 * pxz, A1 * pxx + A2 * pez};
         const double wd = (double)w;
-        Gr[le][0][pe] = wd * ((double)g0 * pxx * gxx + (double)g2 * (pxx * gez + pez * gxx) + (double)g1 * pez * gez);
-        Gr[le][1][pe] = wd * (2.0 * g0 * pxx * gxx + 2.0 * g1 * pez * gez + (double)g1 * pex * (gex + gxz) +
+        Gr[le][pe][0] = wd * ((double)g0 * pxx * gxx + (double)g2 * (pxx * gez + pez * gxx) + (double)g1 * pez * gez);
+        Gr[le][pe][1] = wd * (2.0 * g0 * pxx * gxx + 2.0 * g1 * pez * gez + (double)g1 * pex * (gex + gxz) +
                               (double)g2 * pxz * gex + (double)g0 * pxz * gxz);
       }
       __syncthreads();
       if (active) {
+        R s1[NDOF] = {0, 0}, s2[NDOF] = {0, 0}, f[NDOF];
 #pragma unroll
-        for (int c = 0; c < NDOF; ++c) {
-          R s1 = 0, s2 = 0;
+        for (int k = 0; k < NG; ++k) {
+          const LV a1 = T1[le][jj * NG + k], a2 = T2[le][k * NG + ii];
 #pragma unroll
-          for (int k = 0; k < NG; ++k) {
-            s1 += Coef<R>::h(ii, k) * T1[le][c][jj * NG + k];
-            s2 += T2[le][c][k * NG + ii] * Coef<R>::h(jj, k);
+          for (int c = 0; c < NDOF; ++c) {
+            s1[c] += hC[k] * lv_get<R, NDOF>(a1, c);
+            s2[c] += lv_get<R, NDOF>(a2, c) * hD[k];
           }
-          R* fp = &Fn[c][base + jj * PX + ii];
-          *fp = *fp + (s1 + s2);
         }
+        LV* fp = &Fn[base + jj * PX + ii];
+        const LV f0 = *fp;
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) f[c] = lv_get<R, NDOF>(f0, c) + (s1[c] + s2[c]);
+        *fp = lv_make<R, NDOF>(f);
         if (pe == 0) {  // fixed-order sum over the element's points
           double sl = 0, sm = 0;
           for (int q = 0; q < NP; ++q) {
-            sl += Gr[le][0][q];
-            sm += Gr[le][1][q];
+            sl += Gr[le][q][0];
+            sm += Gr[le][q][1];
           }
           glam[cell] += sl;
           gmu[cell] += sm;
@@ -832,20 +843,22 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
       const int Z = p / PX, X = p - Z * PX;
       int slot = perim_slot(X, Z, PX, PZ);
       if (slot < 0 && sflag[p] >= 0) slot = PERIM_MAX + sflag[p];
+      const LV fn = Fn[p];
       if (slot >= 0) {
 #pragma unroll
-        for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = Fn[c][p];
+        for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = lv_get<R, NDOF>(fn, c);
       } else {
         const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
+        const LV fb = Uf[p];
         double mb = 0;
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
-          const R dn = adr[r][c] + Fn[c][p];
+          const R dn = adr[r][c] + lv_get<R, NDOF>(fn, c);
           const R vbp = avr[r][c];
           ad[c * S + g] = dn;
           av[c * S + g] = vbp + dt * dn;
           aa[c * S + g] = c2 * vbp + c1 * dn;
-          mb -= (double)Uf[c][p] * (double)afw[c * S + g];
+          mb -= (double)lv_get<R, NDOF>(fb, c) * (double)afw[c * S + g];
         }
         mbar[g] += mb;
       }
