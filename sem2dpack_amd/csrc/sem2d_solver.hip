@@ -59,6 +59,8 @@ struct StructState {
   int pipe_threads = 512;
   int4* tiles_dev = nullptr;
   int4* grids_dev = nullptr;
+  int* tlist_dev = nullptr;  // launch list of the pipelined kernel: full tiles, then the rest
+  int nfull = 0, npart = 0;
   int* spec_cnt = nullptr;
   int* spec_idx = nullptr;
   void* fpart = nullptr;
@@ -471,6 +473,28 @@ bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
   return true;
 }
 
+// Launch list of the pipelined tile kernel: the full TX x TZ tiles (its constant-shape,
+// 32-bit-offset variant) first, then the others.  idx32: the field and partial-force
+// arrays are below 4 GiB per component, as that variant requires; otherwise every tile
+// goes to the general variant.
+static std::vector<int> tile_list(const std::vector<int4>& tiles, bool idx32, int& nfull) {
+  std::vector<int> list;
+  list.reserve(tiles.size());
+  auto full = [&](const int4& tg) { return idx32 && (tg.z & 0xffff) == (TX | (TZ << 8)); };
+  for (int t = 0; t < (int)tiles.size(); ++t)
+    if (full(tiles[t])) list.push_back(t);
+  nfull = (int)list.size();
+  for (int t = 0; t < (int)tiles.size(); ++t)
+    if (!full(tiles[t])) list.push_back(t);
+  return list;
+}
+
+// ngp, fps: field and partial-force component strides; ncoef: entries of the per-cell
+// coefficient array
+static bool fits32(size_t ngp, size_t fps, size_t ncoef, size_t rs) {
+  return std::max(std::max(ngp, fps), ncoef) * rs < (size_t(1) << 32);
+}
+
 template <typename T>
 int prep_upload(sem2d_solver* h, T** p, const T* src, size_t n) {
   void* q = nullptr;
@@ -639,6 +663,15 @@ int prepare_structured(sem2d_solver* h) {
   if ((rc = prep_upload(h, &S.dabso, X.dabso.data(), X.dabso.size()))) return rc;
   if ((rc = prep_upload_bulk(h, &S.rmdef, X.rmdef))) return rc;
   S.FPS = (size_t)S.ntiles * SLOTS;
+  {
+    const std::vector<int> list = tile_list(X.tiles, fits32(h->fstride, S.FPS,
+                                                          h->coef_mode == COEF_SHARED
+                                                              ? (size_t)h->nelast * NP
+                                                              : (size_t)h->nelem * std::max(h->ndof, h->nelast) * NP,
+                                                          rsize(h)), S.nfull);
+    S.npart = S.ntiles - S.nfull;
+    if ((rc = prep_upload(h, &S.tlist_dev, list.data(), list.size()))) return rc;
+  }
   void* fp = nullptr;
   HIPCHK(h, hipMalloc(&fp, std::max<size_t>(S.FPS * h->ndof, 1) * rsize(h)));
   S.prep_allocs.push_back(fp);
@@ -658,6 +691,8 @@ struct StepDev {
   size_t fstride = 0;
   void *d = nullptr, *v = nullptr, *a = nullptr;
   TileDev T{};
+  const int* tlist = nullptr;  // pipelined kernel: tlist[0 .. nfull) full tiles, then npart others
+  int nfull = 0, npart = 0;
   int pipe_wgs = 0, pipe_threads = 512;
   const void *rmass0 = nullptr, *coefs = nullptr, *geom = nullptr;
   void* fpart = nullptr;
@@ -694,6 +729,9 @@ StepDev step_dev(sem2d_solver* h) {
   X.T.per_xcd = S.per_xcd;
   X.T.tiles = S.tiles_dev;
   X.T.grids = S.grids_dev;
+  X.tlist = S.tlist_dev;
+  X.nfull = S.nfull;
+  X.npart = S.npart;
   X.T.spec_cnt = S.spec_cnt;
   X.T.spec_idx = S.spec_idx;
   X.T.NGp = h->fstride;
@@ -721,21 +759,31 @@ StepDev step_dev(sem2d_solver* h) {
   return X;
 }
 
+// Pipelined tile kernel over n listed tiles (FULL: all of them full tiles)
+template <typename R, int NDOF, int NELAST, int MODE, bool OOP, bool FULL>
+void launch_pipe(const StepDev& X, const int* list, int n, R dt, R c1, R c2, R c3, R c4, const R* di,
+                 const R* vi, const R* ai) {
+  if (n <= 0) return;
+  const int wgs = std::min(X.pipe_wgs, (n + 7) / 8 * 8);
+  if (X.pipe_threads == 256)
+    k_tile_pipe<R, NDOF, NELAST, MODE, 256, 10, OOP, FULL><<<wgs, 256, 0, X.stream>>>(
+        X.T, list, n, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
+        (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
+  else
+    k_tile_pipe<R, NDOF, NELAST, MODE, 512, 20, OOP, FULL><<<wgs, 512, 0, X.stream>>>(
+        X.T, list, n, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
+        (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
+}
+
 template <typename R, int NDOF, int NELAST, int MODE>
 void launch_tile_t(const StepDev& X, R dt, R c1, R c2, R c3, R c4) {
   if constexpr (MODE == COEF_COMPACT && NDOF == 2 && NELAST == 6) {
     if (X.din) {  // out of place (only the adjoint replay, P-SV compact)
       const R *di = (const R*)X.din, *vi = (const R*)X.vin, *ai = (const R*)X.ain;
       if (X.pipe_wgs > 0) {
-        const int per = (X.T.ntiles + X.pipe_wgs - 1) / X.pipe_wgs;
-        if (X.pipe_threads == 256)
-          k_tile_pipe<R, NDOF, NELAST, MODE, 256, 10, true><<<X.pipe_wgs, 256, 0, X.stream>>>(
-              X.T, per, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
-              (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
-        else
-          k_tile_pipe<R, NDOF, NELAST, MODE, 512, 20, true><<<X.pipe_wgs, 512, 0, X.stream>>>(
-              X.T, per, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
-              (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
+        launch_pipe<R, NDOF, NELAST, MODE, true, true>(X, X.tlist, X.nfull, dt, c1, c2, c3, c4, di, vi, ai);
+        launch_pipe<R, NDOF, NELAST, MODE, true, false>(X, X.tlist + X.nfull, X.npart, dt, c1, c2, c3, c4, di,
+                                                        vi, ai);
       } else {
         k_tile<R, NDOF, NELAST, MODE, true><<<8 * X.T.per_xcd, TILE_THREADS, 0, X.stream>>>(
             X.T, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
@@ -745,16 +793,12 @@ void launch_tile_t(const StepDev& X, R dt, R c1, R c2, R c3, R c4) {
     }
   }
   if (X.pipe_wgs > 0) {
-    // persistent pipelined kernel: pipe_wgs workgroups, each on a run of tiles
-    const int per = (X.T.ntiles + X.pipe_wgs - 1) / X.pipe_wgs;
-    if (X.pipe_threads == 256)
-      k_tile_pipe<R, NDOF, NELAST, MODE, 256, 10><<<X.pipe_wgs, 256, 0, X.stream>>>(
-          X.T, per, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
-          (R*)X.fpart, dt, c1, c2, c3, c4);
-    else
-      k_tile_pipe<R, NDOF, NELAST, MODE, 512, 20><<<X.pipe_wgs, 512, 0, X.stream>>>(
-          X.T, per, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
-          (R*)X.fpart, dt, c1, c2, c3, c4);
+    // persistent pipelined kernel: up to pipe_wgs workgroups, each on a run of tiles;
+    // the full tiles and the others (grid edges) in two launches
+    launch_pipe<R, NDOF, NELAST, MODE, false, true>(X, X.tlist, X.nfull, dt, c1, c2, c3, c4, nullptr, nullptr,
+                                                     nullptr);
+    launch_pipe<R, NDOF, NELAST, MODE, false, false>(X, X.tlist + X.nfull, X.npart, dt, c1, c2, c3, c4, nullptr,
+                                                      nullptr, nullptr);
   } else {
     k_tile<R, NDOF, NELAST, MODE><<<8 * X.T.per_xcd, TILE_THREADS, 0, X.stream>>>(
         X.T, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom, (R*)X.fpart,
@@ -1864,6 +1908,19 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   void *rmdef, *frm;
   if ((rc = bupload_bulk(b, &rmdef, A.rmdef))) return rc;
   if ((rc = bupload_bulk(b, &frm, A.fault_rm))) return rc;
+  {
+    int nfull = 0;
+    const std::vector<int> list = tile_list(A.tiles, fits32(nb, (size_t)ntiles * SLOTS,
+                                                      b->coef_mode == COEF_SHARED
+                                                          ? shared_off
+                                                          : cb * std::max(nd, b->nelast) * NP,
+                                                      rs), nfull);
+    int* tl = nullptr;
+    if ((rc = bupload(b, &tl, list))) return rc;
+    X.tlist = tl;
+    X.nfull = nfull;
+    X.npart = ntiles - nfull;
+  }
   X.T.ntiles = ntiles;
   X.T.per_xcd = (ntiles + 7) / 8;
   X.T.tiles = tiles;

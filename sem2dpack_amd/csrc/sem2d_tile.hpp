@@ -22,6 +22,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sem2d_kernels.hpp"
 
 namespace sem2d {
@@ -76,11 +78,13 @@ __device__ __forceinline__ const R* grid_coefs(const R* coefs, const int4 gr) {
 }
 
 __device__ __forceinline__ int perim_slot(int X, int Z, int PX, int PZ) {
-  if (Z == 0) return X;
-  if (Z == PZ - 1) return PX + X;
-  if (X == 0) return 2 * PX + (Z - 1);
-  if (X == PX - 1) return 2 * PX + (PZ - 2) + (Z - 1);
-  return -1;
+  // branch-free form of: Z == 0 -> X; Z == PZ-1 -> PX + X; X == 0 -> 2PX + Z-1;
+  // X == PX-1 -> 2PX + PZ-2 + Z-1; otherwise -1
+  int s = X == PX - 1 ? 2 * PX + (PZ - 2) + (Z - 1) : -1;
+  s = X == 0 ? 2 * PX + (Z - 1) : s;
+  s = Z == PZ - 1 ? PX + X : s;
+  s = Z == 0 ? X : s;
+  return s;
 }
 
 template <typename R, int NDOF, int NELAST, int MODE, bool OOP = false>
@@ -254,6 +258,35 @@ __device__ __forceinline__ R lv_get(const typename LVec<R, N>::T& v, int c) {
   else return v;
 }
 
+// Element i of an array below 4 GiB, addressed as a uniform base plus a 32-bit byte
+// offset (the scalar-base / vector-offset form of the memory instructions, with no
+// 64-bit address arithmetic per access).
+template <typename T>
+__device__ __forceinline__ T& at32(T* base, uint32_t i) {
+  using C = std::conditional_t<std::is_const<T>::value, const char, char>;
+  return *reinterpret_cast<T*>(reinterpret_cast<C*>(base) + i * (uint32_t)sizeof(T));
+}
+
+// p, as a value the compiler cannot look through: a uniform base kept in scalar
+// registers, so that base + 32-bit offset is not re-associated into 64-bit vector
+// address arithmetic
+template <typename T>
+__device__ __forceinline__ T* sgpr_base(T* p) {
+  using G = __attribute__((address_space(1))) T*;  // global memory
+  G g = (G)p;
+  asm("" : "+s"(g));
+  return (T*)g;
+}
+
+// node_index for a full tile (PX = 4*TX+1, right edge on line kL+1), in 32 bits and
+// branch-free (both forms computed, one selected)
+__device__ __forceinline__ uint32_t node_index_full(const int4 gr, int X, int Z, int X0, int Z0, int kL) {
+  const uint32_t edge = (uint32_t)gr.x + (uint32_t)gr.y * (uint32_t)gr.z + (uint32_t)(Z0 + Z) +
+                        (uint32_t)(X == 0 ? kL : kL + 1) * (uint32_t)gr.z;
+  const uint32_t inner = (uint32_t)gr.x + (uint32_t)(Z0 + Z) * (uint32_t)gr.y + (uint32_t)(X0 + X - 1);
+  return (X == 0 || X == 4 * TX) ? edge : inner;
+}
+
 // Persistent, software-pipelined variant of k_tile.  Each workgroup walks a strided
 // sequence of tiles inside its XCD's share (see below).  No tile waits on a
 // global load issued during that tile: everything a tile reads comes from registers
@@ -268,9 +301,15 @@ __device__ __forceinline__ R lv_get(const typename LVec<R, N>::T& v, int c) {
 // those stores had completed; the order above keeps every wait off the stores.
 // Results are identical to k_tile: same predictor, same colour order and same
 // summation order.  NT threads per workgroup, EPB elements per element batch
-// (EPB*25 <= NT).
-template <typename R, int NDOF, int NELAST, int MODE, int NT, int EPB, bool OOP = false>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 4 : 2))) void k_tile_pipe(TileDev T, int tiles_per_wg, R* __restrict__ d,
+// (EPB*25 <= NT).  The launch covers the tiles tlist[0 .. nlist).
+//
+// FULL: every listed tile has TX x TZ elements, so patch and colour shapes are
+// compile-time constants (no run-time divisions in the index arithmetic), and the
+// field, inverse-mass, partial-force and per-cell arrays are addressed with 32-bit
+// offsets from per-component bases (the caller guarantees that every component of
+// these arrays is below 4 GiB).
+template <typename R, int NDOF, int NELAST, int MODE, int NT, int EPB, bool OOP = false, bool FULL = false>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 4 : 2))) void k_tile_pipe(TileDev T, const int* __restrict__ tlist, int nlist, R* __restrict__ d,
                                                             R* __restrict__ v, R* __restrict__ a,
                                                             const R* __restrict__ rmass0,
                                                             const R* __restrict__ coefs, const R* __restrict__ geom,
@@ -306,17 +345,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   const int nwg = gridDim.x;
   const int bid = blockIdx.x;
   // Workgroups reach the 8 XCDs round-robin (XCD = bid & 7).  XCD x owns the x-th
-  // eighth of the tiles [s0, s1); its W workgroups take them interleaved (workgroup k:
-  // s0 + k, s0 + k + W, ...), so at any time one XCD works on consecutive tiles and the
-  // cache lines and halo nodes shared by x-neighbours are fetched once into its L2.
-  (void)tiles_per_wg;
+  // eighth of the listed tiles [s0, s1); its W workgroups take them interleaved
+  // (workgroup k: s0 + k, s0 + k + W, ...), so at any time one XCD works on consecutive
+  // tiles and the cache lines and halo nodes shared by x-neighbours are fetched once
+  // into its L2.
   const int W = nwg >> 3;  // nwg is a multiple of 8
-  const int slice = (T.ntiles + 7) >> 3;
+  const int slice = (nlist + 7) >> 3;
   const int s0 = (bid & 7) * slice;
-  const int s1 = min(s0 + slice, T.ntiles);
-  const int t0 = s0 + (bid >> 3);
-  if (t0 >= s1) return;
+  const int s1 = min(s0 + slice, nlist);
+  const int q0 = s0 + (bid >> 3);
+  if (q0 >= s1) return;
   const int tid = threadIdx.x;
+  // lane index of the node loops, re-materialised per full tile (below) so that their
+  // per-lane index arithmetic is recomputed rather than held in registers across tiles
+  int tnode = tid;
   const size_t S = T.NGp;
   const int le = tid / NP;
   const int pe = tid - le * NP;
@@ -336,26 +378,83 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
     hC[k] = (R)c_H[ii * NG + k];
     hD[k] = (R)c_H[jj * NG + k];
   }
+  // tile geometry: compile-time constants for full tiles
+  struct Geo {
+    int tx0, tz0, nxe, nze, Z0, PX, PZ, NPT, X0, kL, kR;
+  };
+  auto geo = [&](const int4 tg) {
+    Geo g;
+    g.tx0 = tg.x;
+    g.tz0 = tg.y;
+    g.nxe = FULL ? TX : (tg.z & 0xff);
+    g.nze = FULL ? TZ : ((tg.z >> 8) & 0xff);
+    g.Z0 = tg.w;
+    g.PX = 4 * g.nxe + 1;
+    g.PZ = 4 * g.nze + 1;
+    g.NPT = g.PX * g.PZ;
+    g.X0 = 4 * g.tx0;
+    g.kL = g.tx0 / TX;
+    g.kR = FULL ? g.kL + 1 : (g.tx0 + g.nxe + TX - 1) / TX;
+    return g;
+  };
+  // storage index of patch node (X, Z) and field access (32-bit offsets for full tiles)
+  auto nidx = [&](const int4 gr, const Geo& G, int X, int Z) {
+    if constexpr (FULL) return node_index_full(gr, X, Z, G.X0, G.Z0, G.kL);
+    else return node_index(gr, X, Z, G.PX, G.X0, G.Z0, G.kL, G.kR);
+  };
+  auto fld = [&](auto* cbase, auto g) -> auto& {
+    if constexpr (FULL) return at32(cbase, (uint32_t)g);
+    else return cbase[g];
+  };
+  auto ubase = [&](auto* p) {
+    if constexpr (FULL) return sgpr_base(p);
+    else return p;
+  };
+  // per-component array bases, formed once
+  const R *bD[NDOF], *bV[NDOF], *bA[NDOF];
+  R *oD[NDOF], *oV[NDOF], *oA[NDOF], *bF[NDOF];
+#pragma unroll
+  for (int c = 0; c < NDOF; ++c) {
+    bD[c] = ubase(dsrc + c * S);
+    bV[c] = ubase(vsrc + c * S);
+    bA[c] = ubase(asrc + c * S);
+    if constexpr (OOP) {
+      oD[c] = ubase(d + c * S);
+      oV[c] = ubase(v + c * S);
+      oA[c] = ubase(a + c * S);
+    } else {
+      oD[c] = const_cast<R*>(bD[c]);
+      oV[c] = const_cast<R*>(bV[c]);
+      oA[c] = const_cast<R*>(bA[c]);
+    }
+    bF[c] = ubase(fpart + c * T.FPS);
+  }
+  const R* bRm = ubase(rmass0);
+  const R* bCo = ubase(coefs);
+  const R* bGe = ubase(geom);
+  // cell index in the element arrays (32 bits for full tiles)
+  auto cidx = [&](const int4 gr, int nx, const Geo& G, int lx, int lz) {
+    if constexpr (FULL) return (uint32_t)gr.w + (uint32_t)(G.tz0 + lz) * (uint32_t)nx + (uint32_t)(G.tx0 + lx);
+    else return cell_index(gr, nx, G.tx0, G.tz0, lx, lz);
+  };
 
   // prefetch registers of the next tile: node values ...
   R pd[NITER][NDOF], pv[NITER][NDOF], pa[NITER][NDOF];
   auto issue_nodes = [&](int t) {
     const int4 tg = T.tiles[t];
     const int4 gr = T.grids[t];
-    const int tx0 = tg.x, nxe = tg.z & 0xff, nze = (tg.z >> 8) & 0xff, Z0 = tg.w;
-    const int PX = 4 * nxe + 1, NPT = PX * (4 * nze + 1), X0 = 4 * tx0;
-    const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
+    const Geo G = geo(tg);
 #pragma unroll
     for (int r = 0; r < NITER; ++r) {
-      const int p = tid + NT * r;
-      if (p < NPT) {
-        const int Z = p / PX, X = p - Z * PX;
-        const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
+      const int p = tnode + NT * r;
+      if (p < G.NPT) {
+        const int Z = p / G.PX, X = p - Z * G.PX;
+        const auto g = nidx(gr, G, X, Z);
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
-          pd[r][c] = dsrc[c * S + g];
-          pv[r][c] = vsrc[c * S + g];
-          pa[r][c] = asrc[c * S + g];
+          pd[r][c] = fld(bD[c], g);
+          pv[r][c] = fld(bV[c], g);
+          pa[r][c] = fld(bA[c], g);
         }
       }
     }
@@ -369,57 +468,62 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   auto issue_elems = [&](int t) {
     const int4 tg = T.tiles[t];
     const int4 gr = T.grids[t];
-    const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = (tg.z >> 8) & 0xff, Z0 = tg.w;
-    {
-      const int PX = 4 * nxe + 1, NPT = PX * (4 * nze + 1), X0 = 4 * tx0;
-      const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
+    const Geo G = geo(tg);
 #pragma unroll
-      for (int r = 0; r < NITER; ++r) {
-        const int p = tid + NT * r;
-        if (p < NPT) {
-          const int Z = p / PX, X = p - Z * PX;
-          pm[r] = rmass0[node_index(gr, X, Z, PX, X0, Z0, kL, kR)];
-        }
+    for (int r = 0; r < NITER; ++r) {
+      const int p = tnode + NT * r;
+      if (p < G.NPT) {
+        const int Z = p / G.PX, X = p - Z * G.PX;
+        pm[r] = fld(bRm, nidx(gr, G, X, Z));
       }
     }
     if constexpr (MODE == COEF_COMPACT) {
-      if (tid < nxe * nze * 3) {
+      if (tid < G.nxe * G.nze * 3) {
         const int e = tid / 3, k = tid - 3 * (tid / 3);
-        const int lx = e % nxe, lz = e / nxe;
-        pg = geom[cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz) * 3 + k];
+        const int lx = e % G.nxe, lz = e / G.nxe;
+        pg = fld(bGe, cidx(gr, tg.z >> 16, G, lx, lz) * 3 + k);
       }
 #pragma unroll
       for (int q = 0; q < PB_MAXB; ++q) {
         const int cc = q / NBATCH, b0 = (q % NBATCH) * EPB;
         const int cx = cc & 1, cz = cc >> 1;
-        const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
+        const int ncx = (G.nxe - cx + 1) >> 1, ncz = (G.nze - cz + 1) >> 1;
         const int ce = b0 + le;
         if (tid < ETHREADS && ce < ncx * ncz) {
           const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
-          const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
+          const auto cell = cidx(gr, tg.z >> 16, G, lx, lz);
 #pragma unroll
-          for (int c = 0; c < NDOF; ++c) lm[q][c] = coefs[cell * NDOF * NP + c * NP + pe];
+          for (int c = 0; c < NDOF; ++c) lm[q][c] = fld(bCo, cell * NDOF * NP + c * NP + pe);
         }
       }
     }
     if (tid < T.spec_cnt[t]) psp = T.spec_idx[t * MAXSPECIAL + tid];
   };
-  issue_nodes(t0);
-  issue_elems(t0);
+  issue_nodes(tlist[q0]);
+  issue_elems(tlist[q0]);
 
 #pragma unroll 1
-  for (int t = t0; t < s1; t += W) {
+  for (int q0i = q0; q0i < s1; q0i += W) {
+    int lelem = le;  // likewise for the element-batch lane index (full tiles)
+    if constexpr (FULL) {
+      tnode = tid;
+      asm volatile("" : "+v"(tnode));
+      asm volatile("" : "+v"(lelem));
+      // keep the ranges known, so that loop bounds that hold for every lane fold away
+      __builtin_assume(tnode >= 0 && tnode < NT);
+      __builtin_assume(lelem >= 0 && lelem <= (NT - 1) / NP);
+    }
+    const int t = tlist[q0i];
+    const int tn = q0i + W < s1 ? tlist[q0i + W] : -1;  // next tile of this workgroup
     const int4 tg = T.tiles[t];
     const int4 gr = T.grids[t];
-    const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = (tg.z >> 8) & 0xff, Z0 = tg.w;
-    const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
-    const int X0 = 4 * tx0;
-    const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
+    const Geo G = geo(tg);
+    const int PX = G.PX, PZ = G.PZ, NPT = G.NPT;
 
     // predictor from the prefetched values
 #pragma unroll
     for (int r = 0; r < NITER; ++r) {
-      const int p = tid + NT * r;
+      const int p = tnode + NT * r;
       if (p < NPT) {
         R u[NDOF], w[NDOF];
 #pragma unroll
@@ -435,13 +539,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
       }
     }
     if constexpr (MODE == COEF_COMPACT) {
-      if (tid < nxe * nze * 3) Gm[tid / 3][tid - 3 * (tid / 3)] = pg;
+      if (tid < G.nxe * G.nze * 3) Gm[tid / 3][tid - 3 * (tid / 3)] = pg;
     }
     __syncthreads();
     if (psp >= 0) sflag[psp] = (short)tid;
     psp = -1;
     // overlap: node loads of the next tile fly during this tile's element phase
-    if (t + W < s1) issue_nodes(t + W);
+    if (tn >= 0) issue_nodes(tn);
 
     // Element batches, software-pipelined over one barrier per batch: in phase q the
     // lanes compute the stresses of batch q (into T[q & 1]) and assemble batch q-1 (from
@@ -457,17 +561,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
       if (q < PB_MAXB) {
         const int cc = q / NBATCH, b0 = (q % NBATCH) * EPB;
         const int cx = cc & 1, cz = cc >> 1;
-        const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
+        const int ncx = (G.nxe - cx + 1) >> 1, ncz = (G.nze - cz + 1) >> 1;
         const int nce = ncx * ncz;
         has = b0 < nce;  // uniform across the workgroup
-        const int ce = b0 + le;
+        const int ce = b0 + lelem;
         active = has && (tid < ETHREADS) && (ce < nce);
         if (active) {
           const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
           base = 4 * lz * PX + 4 * lx;
           R A[NELAST];
           if constexpr (MODE == COEF_COMPACT) {
-            const int el = lz * nxe + lx;
+            const int el = lz * G.nxe + lx;
             const R g0 = Gm[el][0], g1 = Gm[el][1], g2 = Gm[el][2];
             if constexpr (NDOF == 2) {
               const R la = lm[q][0], mu = lm[q][1], kx = la + 2 * mu;
@@ -482,7 +586,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
               A[1] = w2 * lm[q][0] * g1;
             }
           } else {
-            const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
+            const size_t cell = cell_index(gr, tg.z >> 16, G.tx0, G.tz0, lx, lz);
             load_coefs<R, NDOF, NELAST, MODE>(grid_coefs<R, MODE>(coefs, gr), geom, cell, pe, A);
           }
           R dxi[NDOF], deta[NDOF];
@@ -553,28 +657,28 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
       if (has) ++nvb;
     }
     // element loads of the next tile, ahead of this tile's stores
-    if (t + W < s1) issue_elems(t + W);
+    if (tn >= 0) issue_elems(tn);
 
-    const size_t fbase = (size_t)t * SLOTS;
+    const uint32_t fbase = (uint32_t)t * SLOTS;
 #pragma unroll
     for (int r = 0; r < NITER; ++r) {
-      const int p = tid + NT * r;
+      const int p = tnode + NT * r;
       if (p < NPT) {
         const int Z = p / PX, X = p - Z * PX;
         int slot = perim_slot(X, Z, PX, PZ);
         if (slot < 0 && sflag[p] >= 0) slot = PERIM_MAX + sflag[p];
         if (slot >= 0) {
 #pragma unroll
-          for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = lv_get<R, NDOF>(Fn[p], c);
+          for (int c = 0; c < NDOF; ++c) fld(bF[c], fbase + slot) = lv_get<R, NDOF>(Fn[p], c);
         } else {
-          const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
+          const auto g = nidx(gr, G, X, Z);
           const R rm = Rm[p];
 #pragma unroll
           for (int c = 0; c < NDOF; ++c) {
             const R ak = lv_get<R, NDOF>(Fn[p], c) * rm;
-            a[c * S + g] = ak;
-            v[c * S + g] = lv_get<R, NDOF>(Vp[p], c) + c3 * ak;
-            d[c * S + g] = lv_get<R, NDOF>(Ud[p], c) + c4 * ak;
+            fld(oA[c], g) = ak;
+            fld(oV[c], g) = lv_get<R, NDOF>(Vp[p], c) + c3 * ak;
+            fld(oD[c], g) = lv_get<R, NDOF>(Ud[p], c) + c4 * ak;
           }
         }
       }
