@@ -177,6 +177,7 @@ def run_adjoint(args):
     model = adjoint.SeismogramModel(path)
     obs = model.forward(*true)
     pb = model.problem(*back)
+    model.close()  # its stored forward states are not needed any more
     build_s = time.time() - t0
     print(f"[adjoint] problem: npoin={pb.npoin} nelem={pb.nelem} nt={nt}", file=sys.stderr, flush=True)
     # plain forward (the reference's time loop) on the background model
@@ -196,14 +197,19 @@ def run_adjoint(args):
         m._state = (pb, g_s, tuple(np.ascontiguousarray(a) for a in back))
         t2 = time.perf_counter()
         g_s.adjoint_init(nt, 0)
+        plan = g_s.adjoint_plan()
+        t_init = time.perf_counter() - t2
         g_s.adjoint_forward()
         seis = g_s.seismograms()
+        t_fw = time.perf_counter() - t2 - t_init
         grad = m.backward(seis - obs)
         t_grad = time.perf_counter() - t2
-        out[fp32] = dict(grad=grad, t=t_grad, ev=m.last_timing, J=0.5 * float(np.sum((seis - obs) ** 2)))
+        out[fp32] = dict(grad=grad, t=t_grad, ev=m.last_timing, J=0.5 * float(np.sum((seis - obs) ** 2)),
+                         plan={"history": plan[0], "segment": plan[1]},
+                         host_s={"init": t_init, "forward": t_fw, "backward": t_grad - t_init - t_fw})
         m.close()
-        print(f"[adjoint] fp32={fp32}: gradient in {t_grad:.2f} s (forward {t_fwd:.2f} s)", file=sys.stderr,
-              flush=True)
+        print(f"[adjoint] fp32={fp32}: gradient in {t_grad:.2f} s (forward {t_fwd:.2f} s), plan {plan}, "
+              f"device ms {m.last_timing}, wall s {out[fp32]['host_s']}", file=sys.stderr, flush=True)
     # central differences along random directions (FP64 forward)
     rng = np.random.Generator(np.random.PCG64(20260502))
     fdm = adjoint.SeismogramModel(path)
@@ -229,11 +235,13 @@ def run_adjoint(args):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "P-SV 400x400 elements NGLL=5, 16 simultaneous Ricker forces (3 Hz), 64 surface "
                                "receivers, L2 waveform misfit, gradient w.r.t. per-element rho/lambda/mu",
-                   "npoin": pb.npoin, "nelem": pb.nelem, "checkpoint_segment": int(np.ceil(np.sqrt(nt))),
+                   "npoin": pb.npoin, "nelem": pb.nelem, "storage_plan": out[False]["plan"],
                    "build_s": round(build_s, 1)},
         "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
-        "device_ms": {"forward_plain": fwd_ev_ms, "checkpointed_forward": ev_f, "replay_and_adjoint": ev_b},
-        "fp32_bulk": {"gradient_s": out[True]["t"], "gradient_over_forward": out[True]["t"] / t_fwd},
+        "device_ms": {"forward_plain": fwd_ev_ms, "forward_storing": ev_f, "backward": ev_b},
+        "fp32_bulk": {"gradient_s": out[True]["t"], "gradient_over_forward": out[True]["t"] / t_fwd,
+                      "device_ms": {"forward_storing": out[True]["ev"][0], "backward": out[True]["ev"][1]},
+                      "storage_plan": out[True]["plan"]},
         "fd_check": {"directions": args.fd_dirs, "eps_rel": eps * 1e-2,
                      "max_rel_err_fp64": max(errs64) if errs64 else None,
                      "max_rel_err_fp32": max(errs32) if errs32 else None},

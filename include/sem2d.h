@@ -185,14 +185,19 @@ void sem2d_destroy(sem2d_solver *h);
    differentiation).  Structured P-SV grids with compact (lambda, mu) coefficients,
    beta = 0, no fault.  J is any function of the recorded seismograms.
 
-   sem2d_adjoint_init      allocate sqrt(N) checkpointing for nsteps steps: the state
-                           before every segment of seg steps (seg <= 0: ceil(sqrt(N)))
-                           plus seg+1 replay slots
-   sem2d_adjoint_forward   from the zero state, nsteps steps storing the checkpoints;
-                           seismograms are recorded as by sem2d_step
-   sem2d_adjoint_backward  gseis = dJ/d(seismograms) [nsamp][ndof][nrec]; replays each
-                           segment from its checkpoint and runs the adjoint (transposed)
-                           steps backwards through it
+   sem2d_adjoint_init      plan the storage of the forward states for nsteps steps.
+                           seg > 0: sqrt(N)-style checkpointing, the state before every
+                           segment of seg steps plus seg+1 replay slots.  seg <= 0: the
+                           displacement history of every step (FP32 fields: with the
+                           acceleration) if it fits in free device memory, else
+                           checkpoints with seg = ceil(sqrt(N))
+   sem2d_adjoint_forward   from the zero state, nsteps steps storing the checkpoints or
+                           the history; seismograms are recorded as by sem2d_step
+   sem2d_adjoint_backward  gseis = dJ/d(seismograms) [nsamp][ndof][nrec]; runs the
+                           adjoint (transposed) steps backwards, replaying each segment
+                           from its checkpoint first (checkpoint plan) or reading the
+                           history (history plan: a_n and the predicted velocity are
+                           differences of the stored displacements)
    sem2d_adjoint_fetch     the sensitivities: glam, gmu [nelem] = dJ/d(lambda), dJ/d(mu)
                            per element (summed over its GLL points), gmass [npoin] =
                            dJ/dM (assembled mass, both components), gabso [ndof][npoin] =
@@ -206,6 +211,13 @@ int sem2d_adjoint_backward(sem2d_solver *h, const double *gseis);
 int sem2d_adjoint_fetch(sem2d_solver *h, double *glam, double *gmu, double *gmass, double *gabso);
 /* device time (ms) of the last forward and backward sweeps */
 int sem2d_adjoint_timing(sem2d_solver *h, double *fwd_ms, double *bwd_ms);
+/* the plan sem2d_adjoint_init chose: history = 1 for the history plan, seg = the
+   checkpoint segment length (nsteps with the history plan) */
+int sem2d_adjoint_plan(sem2d_solver *h, int *history, int *seg);
+/* The history plan's device block outlives its solver: it is kept (one block per
+   process) for the next history plan that fits in it, since releasing and mapping a
+   block of that size costs seconds.  This returns the kept block to the device. */
+int sem2d_adjoint_release(void);
 
 /* ------------------------------------------------------------------------------
    Batched ensemble: many independent scenarios advanced together.

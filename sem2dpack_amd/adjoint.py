@@ -5,8 +5,9 @@ time step.  :class:`SeismogramModel` maps a model -- constant density and Lame
 parameters in every element, arrays over the cells (index iz*nx + ix) -- to the
 seismograms of a structured P-SV input (its sources, receivers and boundaries),
 and back-propagates dJ/d(seismograms) to dJ/d(model) with the hand-written
-adjoint of the HIP library (``sem2d_adjoint_*``: sqrt(N) checkpointing, replayed
-segments, transposed element operator).
+adjoint of the HIP library (``sem2d_adjoint_*``: the forward displacement history
+kept in device memory when it fits, else sqrt(N) checkpoints and replayed segments;
+transposed element operator).
 
 The library returns the sensitivities to the quantities the time step consumes:
 lambda and mu of every element (through the element coefficients), the assembled
@@ -30,7 +31,9 @@ class SeismogramModel:
     """seismograms(rho, lam, mu) of an input file, differentiable by the adjoint.
 
     ``par``: Par.inp path or text (structured P-SV, fixed ``Dt`` so that every model
-    shares the time step); ``fname_dir``: where its data files are read.
+    shares the time step); ``fname_dir``: where its data files are read.  ``seg``: 0
+    lets the library keep every step's displacement when it fits in device memory
+    (else checkpoints every ceil(sqrt(N)) steps); > 0 forces checkpoints every seg steps.
     """
 
     def __init__(self, par, nsteps=None, fname_dir=None, bulk_fp32=False, seg=0):
@@ -55,12 +58,11 @@ class SeismogramModel:
         pb = self.problem(rho, lam, mu)
         if pb.ndof != 2 or pb.receivers is None:
             raise ValueError("SeismogramModel needs a P-SV input with receivers")
+        self.close()  # the previous model's stored states go before the new ones are planned
         s = solver.Solver(pb, bulk_fp32=self.bulk_fp32)
         s.adjoint_init(pb.nt, self.seg)
         s.adjoint_forward()
         seis = s.seismograms()
-        if self._state is not None:
-            self._state[1].close()
         self._state = (pb, s, (rho, lam, mu))
         return seis
 
@@ -133,6 +135,15 @@ def _make_function():
 
 
 _FN = None
+
+
+def release():
+    """Return the device block the history plan keeps for the next gradient (the library
+    keeps one block per process, so that a sequence of gradients does not pay for
+    releasing and mapping it each time)."""
+    api = solver.load_hip_lib()
+    if api.adjoint_release() != 0:
+        raise RuntimeError("sem2d_adjoint_release failed")
 
 
 def seismograms(rho, lam, mu, model):

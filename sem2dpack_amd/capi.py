@@ -62,6 +62,8 @@ _SIGS = {
     "adjoint_backward": ([C.c_void_p, dp], C.c_int),
     "adjoint_fetch": ([C.c_void_p, dp, dp, dp, dp], C.c_int),
     "adjoint_timing": ([C.c_void_p, dp, dp], C.c_int),
+    "adjoint_plan": ([C.c_void_p, ip, ip], C.c_int),
+    "adjoint_release": ([], C.c_int),
     # batched ensemble (HIP library only)
     "batch_create": ([C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)], C.c_int),
     "batch_step": ([C.c_void_p, C.c_int], C.c_int),

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <map>
+#include <mutex>
 #include <unordered_map>
 #include <string>
 #include <vector>
@@ -2224,6 +2225,12 @@ void sem2d_batch_destroy(sem2d_batch* b) {
 // ===========================================================================
 struct AdjState {
   int nsteps = 0, seg = 0, nck = 0;
+  // history mode: the displacement after every step (FP32 bulk: and the acceleration)
+  // kept in device memory, so that the backward sweep needs no replay
+  bool hist = false;
+  char *H = nullptr, *AH = nullptr, *aN = nullptr;  // [nsteps+1] states; a after the last step
+  void* hblock = nullptr;  // the history's device block (from or back to the history cache)
+  size_t hbytes = 0;
   // injection plan of dJ/d(seismogram): distinct nodes, their contributions
   int nu = 0;
   int *unode = nullptr, *uoff = nullptr, *urec = nullptr;
@@ -2236,8 +2243,77 @@ struct AdjState {
   double fwd_ms = 0, bwd_ms = 0;
 };
 
+// One reusable history block per process.  Releasing a block of this size (a hundred
+// GB) and mapping a new one costs seconds, while a sequence of gradients (one adjoint
+// state each) needs the same block again and again: a freed state's history block is
+// kept here and handed to the next history plan that fits in it, until
+// sem2d_adjoint_release.  A block in use by a live state is not shared.
+struct HistCache {
+  std::mutex m;
+  void* p = nullptr;
+  size_t bytes = 0;
+  int dev = -1;
+};
+HistCache& hist_cache() {
+  static HistCache c;
+  return c;
+}
+
+// a block of at least `bytes` on the current device: the cached one if it fits, else a new
+// one (the cached block released first); nullptr if the device has no room
+void* hist_acquire(size_t bytes, size_t* got) {
+  HistCache& c = hist_cache();
+  std::lock_guard<std::mutex> lk(c.m);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (c.p && c.dev == dev && c.bytes >= bytes) {
+    void* p = c.p;
+    *got = c.bytes;
+    c.p = nullptr;
+    c.bytes = 0;
+    return p;
+  }
+  if (c.p) {
+    (void)hipFree(c.p);
+    c.p = nullptr;
+    c.bytes = 0;
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  *got = bytes;
+  return p;
+}
+
+// size of the cached block usable on the current device (0: none)
+size_t hist_cached_bytes() {
+  HistCache& c = hist_cache();
+  std::lock_guard<std::mutex> lk(c.m);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return c.p && c.dev == dev ? c.bytes : 0;
+}
+
+void hist_return(void* p, size_t bytes) {
+  HistCache& c = hist_cache();
+  std::lock_guard<std::mutex> lk(c.m);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (c.p && c.bytes >= bytes) {  // keep the larger block
+    (void)hipFree(p);
+    return;
+  }
+  if (c.p) (void)hipFree(c.p);
+  c.p = p;
+  c.bytes = bytes;
+  c.dev = dev;
+}
+
 void adj_free(AdjState* A) {
   for (void* p : A->allocs) hipFree(p);
+  if (A->hblock) hist_return(A->hblock, A->hbytes);
   delete A;
 }
 
@@ -2284,7 +2360,29 @@ int adj_backward(sem2d_solver* h, const double* gseis_host) {
   StepDev X = step_dev(h);
   R* inj = (R*)(h->field == 0 ? A.ad : (h->field == 1 ? A.av : A.aa));
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-  for (int k = A.nck - 1; k >= 0; --k) {
+  if (A.hist) {
+    // no replay: the forward displacements are all in memory
+    const size_t nfb = nf * sizeof(R);
+    const double idt = 1.0 / h->dt, idt2 = idt * idt;
+    const int N = A.nsteps;
+    for (int n = N; n >= 1; --n) {
+      if (h->nrec && n % h->isamp == 0 && n / h->isamp < h->nsamp)
+        k_adj_inject<R><<<grid_for(A.nu, 64), 64, 0, h->stream>>>(
+            A.nu, A.unode, A.uoff, A.urec, A.uw, h->nrec, A.gseis + (size_t)(n / h->isamp) * h->ndof * h->nrec,
+            inj, GS);
+      const R* dc = (const R*)(A.H + (size_t)n * nfb);
+      const R* dp = (const R*)(A.H + (size_t)(n - 1) * nfb);
+      const R* dn = n < N ? (const R*)(A.H + (size_t)(n + 1) * nfb) : nullptr;
+      const R* af = A.AH ? (const R*)(A.AH + (size_t)n * nfb) : (n == N ? (const R*)A.aN : nullptr);
+      k_adj_tile<R><<<8 * S.per_xcd, TILE_THREADS, 0, h->stream>>>(
+          X.T, (R*)A.ad, (R*)A.av, (R*)A.aa, (const R*)A.rmass, dc, af, (const R*)S.coefs, (const R*)S.geom,
+          (R*)S.fpart, A.glam, A.gmu, A.mbar, dt, c1, c2, c3, dp, dn, idt2);
+      k_adj_deferred<R><<<(S.ndef + 255) / 256, 256, 0, h->stream>>>(
+          S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dabso, (const R*)S.rmdef, (R*)A.ad, (R*)A.av,
+          (R*)A.aa, GS, af, nullptr, nullptr, A.mbar, A.gabs, dt, c1, c2, c3, dc, dp, dn, idt, idt2);
+    }
+  }
+  for (int k = A.hist ? -1 : A.nck - 1; k >= 0; --k) {
     const int s0 = k * A.seg, s1 = std::min(A.nsteps, s0 + A.seg), L = s1 - s0;
     // replay: slot[j] = state after step s0 + j
     char* s = A.slot[0];
@@ -2329,6 +2427,38 @@ int adj_backward(sem2d_solver* h, const double* gseis_host) {
   return 0;
 }
 
+// Forward sweep of the history mode: every step's displacement (and, for FP32 fields,
+// acceleration) copied into the history; the acceleration after the last step kept.
+template <typename R>
+int adj_forward_hist(sem2d_solver* h) {
+  AdjState& A = *h->adj;
+  const size_t nfb = (size_t)h->ndof * h->fstride * sizeof(R);
+  const R dt = (R)h->dt;
+  const R c1 = (R)((0.5 - h->beta) * h->dt * h->dt), c2 = (R)((1.0 - h->gamma) * h->dt);
+  const R c3 = (R)(h->gamma * h->dt), c4 = (R)(h->beta * h->dt * h->dt);
+  int rc = sync_sources(h);
+  if (rc) return rc;
+  StepDev X = step_dev(h);
+  HIPCHK(h, hipMemsetAsync(A.H, 0, nfb, h->stream));
+  if (A.AH) HIPCHK(h, hipMemsetAsync(A.AH, 0, nfb, h->stream));
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  for (int n = 1; n <= A.nsteps; ++n) {
+    launch_structured_step<R>(X, n, dt, c1, c2, c3, c4);
+    h->it = n;
+    if ((rc = record_receivers<R>(h))) return rc;
+    HIPCHK(h, hipMemcpyAsync(A.H + (size_t)n * nfb, h->d, nfb, hipMemcpyDeviceToDevice, h->stream));
+    if (A.AH) HIPCHK(h, hipMemcpyAsync(A.AH + (size_t)n * nfb, h->a, nfb, hipMemcpyDeviceToDevice, h->stream));
+  }
+  if (A.aN) HIPCHK(h, hipMemcpyAsync(A.aN, h->a, nfb, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  A.fwd_ms = ms;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2340,6 +2470,7 @@ int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
   if (h->beta != 0.0) return set_err(h, 4, "adjoint: needs the explicit scheme (beta = 0)");
   if (h->flt.on) return set_err(h, 4, "adjoint: faults are not differentiated");
   if (nsteps < 1) return set_err(h, 3, "adjoint_init: nsteps must be >= 1");
+  const bool autoplan = seg <= 0;
   if (seg <= 0) seg = (int)std::ceil(std::sqrt((double)nsteps));
   int rc;
   if (!h->st.ready && (rc = sync_sources(h))) return rc;
@@ -2353,12 +2484,38 @@ int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
   A.nck = (nsteps + A.seg - 1) / A.seg;
   const size_t nf = (size_t)h->ndof * h->fstride * rsize(h);
   A.state_bytes = 3 * nf;
-  for (int k = 0; k < A.nck; ++k) {
+  if (autoplan) {
+    // history mode when every step's displacement (FP32 fields: and acceleration) fits in
+    // free device memory, next to the adjoint's own arrays and a 4 GiB reserve
+    const size_t hbytes = (size_t)(nsteps + 1) * nf * (h->fp32 ? 2 : 1);
+    size_t fr = 0, tot = 0;
+    const size_t cached = hist_cached_bytes();  // reusable, so counted as free
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess &&
+        hbytes + 9 * nf + ((size_t)4 << 30) < fr + (cached >= hbytes ? hbytes : cached)) {
+      size_t got = 0;
+      void* p = hist_acquire(hbytes, &got);
+      if (p) {
+        A.hblock = p;
+        A.hbytes = got;
+        A.H = (char*)p;
+        A.hist = true;
+        if (h->fp32) {
+          A.AH = A.H + (size_t)(nsteps + 1) * nf;
+        } else {
+          if ((rc = adj_alloc(h, &p, nf))) return rc;
+          A.aN = (char*)p;
+        }
+        A.seg = nsteps;
+        A.nck = 1;
+      }  // else: checkpoints
+    }
+  }
+  for (int k = 0; !A.hist && k < A.nck; ++k) {
     void* p;
     if ((rc = adj_alloc(h, &p, A.state_bytes))) return rc;
     A.ck.push_back((char*)p);
   }
-  for (int k = 0; k <= A.seg; ++k) {
+  for (int k = 0; !A.hist && k <= A.seg; ++k) {
     void* p;
     if ((rc = adj_alloc(h, &p, A.state_bytes))) return rc;
     A.slot.push_back((char*)p);
@@ -2440,6 +2597,7 @@ int sem2d_adjoint_forward(sem2d_solver* h) {
   h->taken = 0;
   int rc = h->fp32 ? record_receivers<float>(h) : record_receivers<double>(h);
   if (rc) return rc;
+  if (A.hist) return h->fp32 ? adj_forward_hist<float>(h) : adj_forward_hist<double>(h);
   double ms = 0;
   for (int k = 0; k < A.nck; ++k) {
     char* c = A.ck[k];
@@ -2488,6 +2646,26 @@ int sem2d_adjoint_fetch(sem2d_solver* h, double* glam, double* gmu, double* gmas
     for (int c = 0; c < h->ndof; ++c)
       for (int q = 0; q < S.ndef; ++q) gabso[(size_t)c * h->npoin + S.def_ref[q]] = tmp[(size_t)c * S.ndef + q];
   }
+  return 0;
+}
+
+int sem2d_adjoint_release(void) {
+  HistCache& c = hist_cache();
+  std::lock_guard<std::mutex> lk(c.m);
+  if (c.p) {
+    hipError_t e = hipFree(c.p);
+    c.p = nullptr;
+    c.bytes = 0;
+    if (e != hipSuccess) return 10;
+  }
+  return 0;
+}
+
+int sem2d_adjoint_plan(sem2d_solver* h, int* history, int* seg) {
+  if (!h) return 1;
+  if (!h->adj) return set_err(h, 4, "adjoint_plan: call sem2d_adjoint_init first");
+  if (history) *history = h->adj->hist ? 1 : 0;
+  if (seg) *seg = h->adj->seg;
   return 0;
 }
 

@@ -799,12 +799,25 @@ __global__ void k_fault_correct(int n, const int* __restrict__ g, const R* __res
 // (p.. the derivatives of fb, g.. those of d_n), summed over the element's points.
 // One workgroup per tile; interior nodes finish here, perimeter nodes in k_adj_deferred.
 // ===========================================================================
+// Forward acceleration a_n at field index i: stored (a != nullptr), or, from a history
+// of displacements, the second difference (d_{n+1} - 2 d_n + d_{n-1}) / dt^2, which equals
+// a_n for the explicit Newmark scheme (beta = 0, gamma = 1/2): d_{n+1} - d_n = dt v_n +
+// dt^2/2 a_n and v_n - v_{n-1} = dt/2 (a_{n-1} + a_n).
+template <typename R>
+__device__ __forceinline__ double fwd_accel(const R* __restrict__ a, const R* __restrict__ dprev,
+                                            const R* __restrict__ dnext, size_t i, R dcur, double idt2) {
+  if (a) return (double)a[i];
+  const double dc = (double)dcur;
+  return (((double)dnext[i] - dc) - (dc - (double)dprev[i])) * idt2;
+}
+
 template <typename R>
 __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
     TileDev T, R* __restrict__ ad, R* __restrict__ av, R* __restrict__ aa, const R* __restrict__ rmass,
     const R* __restrict__ dfw, const R* __restrict__ afw, const R* __restrict__ coefs, const R* __restrict__ geom,
     R* __restrict__ fpart, double* __restrict__ glam, double* __restrict__ gmu, double* __restrict__ mbar,
-    R dt, R c1, R c2, R c3) {
+    R dt, R c1, R c2, R c3, const R* __restrict__ dprev = nullptr, const R* __restrict__ dnext = nullptr,
+    double idt2 = 0.0) {
   constexpr int NDOF = 2;
   using LV = typename LVec<R, NDOF>::T;  // both components of a node side by side
   __shared__ LV Uf[PMAX];  // fb
@@ -813,6 +826,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
   __shared__ LV T1[FE_EPB][NP];
   __shared__ LV T2[FE_EPB][NP];
   __shared__ double Gr[FE_EPB][NP][2];
+  __shared__ double Gs[FE_EPB][NG][2];  // row sums of Gr
   __shared__ short sflag[PMAX];
 
   const int bid = blockIdx.x;
@@ -841,6 +855,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
   }
 
   R adr[NODE_ITERS][NDOF], avr[NODE_ITERS][NDOF];
+  double mbr[NODE_ITERS];  // mass sensitivity of the node, -fb . a_n (used at interior nodes)
 #pragma unroll
   for (int r = 0; r < NODE_ITERS; ++r) {
     const int p = tid + TILE_THREADS * r;
@@ -848,6 +863,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
       const int Z = p / PX, X = p - Z * PX;
       const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
       R f[NDOF], u[NDOF];
+      double mb = 0;
 #pragma unroll
       for (int c = 0; c < NDOF; ++c) {
         const R dk = ad[c * S + g], vk = av[c * S + g], ak = aa[c * S + g];
@@ -855,7 +871,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
         u[c] = dfw[c * S + g];
         adr[r][c] = dk;
         avr[r][c] = vk;
+        mb -= (double)f[c] * fwd_accel(afw, dprev, dnext, c * S + g, u[c], idt2);
       }
+      mbr[r] = mb;
       Uf[p] = lv_make<R, NDOF>(f);
       Ud[p] = lv_make<R, NDOF>(u);
       Fn[p] = LV{};
@@ -925,17 +943,33 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) f[c] = lv_get<R, NDOF>(f0, c) + (s1[c] + s2[c]);
         *fp = lv_make<R, NDOF>(f);
-        if (pe == 0) {  // fixed-order sum over the element's points
-          double sl = 0, sm = 0;
-          for (int q = 0; q < NP; ++q) {
-            sl += Gr[le][q][0];
-            sm += Gr[le][q][1];
-          }
-          glam[cell] += sl;
-          gmu[cell] += sm;
+      }
+      // fixed-order sum over the element's points in two levels: lanes pe < 5 sum the
+      // rows jj = pe into Gs, then (after the barrier) lane pe = 0 sums the five row
+      // sums.  Gs is written again only after the next batch's first barrier, which
+      // lane pe = 0 reaches after its reads; the barrier also orders this batch's Fn
+      // updates and Gr, T1, T2 reads before the next batch's writes.
+      if (active && pe < NG) {
+        double rl = 0, rm = 0;
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+          rl += Gr[le][pe * NG + q][0];
+          rm += Gr[le][pe * NG + q][1];
         }
+        Gs[le][pe][0] = rl;
+        Gs[le][pe][1] = rm;
       }
       __syncthreads();
+      if (active && pe == 0) {
+        double sl = 0, sm = 0;
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+          sl += Gs[le][q][0];
+          sm += Gs[le][q][1];
+        }
+        glam[cell] += sl;
+        gmu[cell] += sm;
+      }
     }
   }
 
@@ -953,8 +987,6 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
         for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = lv_get<R, NDOF>(fn, c);
       } else {
         const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
-        const LV fb = Uf[p];
-        double mb = 0;
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
           const R dn = adr[r][c] + lv_get<R, NDOF>(fn, c);
@@ -962,9 +994,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
           ad[c * S + g] = dn;
           av[c * S + g] = vbp + dt * dn;
           aa[c * S + g] = c2 * vbp + c1 * dn;
-          mb -= (double)lv_get<R, NDOF>(fb, c) * (double)afw[c * S + g];
         }
-        mbar[g] += mb;
+        mbar[g] += mbr[r];
       }
     }
   }
@@ -982,7 +1013,11 @@ __global__ __launch_bounds__(256) void k_adj_deferred(int ndef, const DefNode* _
                                                       R* __restrict__ av, R* __restrict__ aa, size_t GS,
                                                       const R* __restrict__ afw, const R* __restrict__ vprev,
                                                       const R* __restrict__ aprev, double* __restrict__ mbar,
-                                                      double* __restrict__ gabs, R dt, R c1, R c2, R c3) {
+                                                      double* __restrict__ gabs, R dt, R c1, R c2, R c3,
+                                                      const R* __restrict__ dcur = nullptr,
+                                                      const R* __restrict__ dprev = nullptr,
+                                                      const R* __restrict__ dnext = nullptr, double idt = 0.0,
+                                                      double idt2 = 0.0) {
   constexpr int NDOF = 2;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= ndef) return;
@@ -1002,10 +1037,14 @@ __global__ __launch_bounds__(256) void k_adj_deferred(int ndef, const DefNode* _
     ad[c * GS + g] = dn;
     av[c * GS + g] = vbp + dt * dn;
     aa[c * GS + g] = c2 * vbp + c1 * dn;
-    const double an = (double)afw[c * GS + g];
+    // with a displacement history (dcur set): a_n as in fwd_accel, and the predicted
+    // velocity v_{n-1} + dt/2 a_{n-1} = (d_n - d_{n-1}) / dt
+    const double an = dcur ? fwd_accel(afw, dprev, dnext, c * GS + g, dcur[c * GS + g], idt2)
+                           : (double)afw[c * GS + g];
     mb -= (double)fb * an;
     if (n.acnt > 0) {
-      const double vp = (double)vprev[c * GS + g] + (double)c2 * (double)aprev[c * GS + g];
+      const double vp = dcur ? ((double)dcur[c * GS + g] - (double)dprev[c * GS + g]) * idt
+                             : (double)vprev[c * GS + g] + (double)c2 * (double)aprev[c * GS + g];
       gabs[(size_t)c * ndef + q] -= (double)fb * (vp + (double)c3 * an);
     }
   }

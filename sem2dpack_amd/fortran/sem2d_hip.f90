@@ -42,7 +42,7 @@ module sem2d_hip
             sem2d_compute_fint, sem2d_set_tile_config, sem2d_get_step, sem2d_last_step_ms, &
             sem2d_last_error, sem2d_destroy, sem2d_error_message, &
             sem2d_adjoint_init, sem2d_adjoint_forward, sem2d_adjoint_backward, sem2d_adjoint_fetch, &
-            sem2d_adjoint_timing, &
+            sem2d_adjoint_timing, sem2d_adjoint_plan, sem2d_adjoint_release, &
             sem2d_batch_create, sem2d_batch_step, sem2d_batch_tune, sem2d_batch_set_config, &
             sem2d_batch_config, sem2d_batch_fetch_fields, sem2d_batch_fetch_seismo, &
             sem2d_batch_fetch_fault, sem2d_batch_fetch_fault_state, sem2d_batch_get_step, &
@@ -227,6 +227,14 @@ module sem2d_hip
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(out) :: fwd_ms, bwd_ms
+    end function
+    integer(c_int) function sem2d_adjoint_plan(h, history, seg) bind(C, name='sem2d_adjoint_plan')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: history, seg
+    end function
+    integer(c_int) function sem2d_adjoint_release() bind(C, name='sem2d_adjoint_release')
+      import :: c_int
     end function
 
     ! batched ensemble

@@ -233,6 +233,13 @@ class Solver:
         self._check(self.api.adjoint_timing(self.h, C.byref(f), C.byref(b)), "adjoint_timing")
         return f.value, b.value
 
+    def adjoint_plan(self):
+        """(history, seg): history True when the forward displacements of every step are
+        kept (no replay), seg the checkpoint segment length."""
+        hi, sg = C.c_int(), C.c_int()
+        self._check(self.api.adjoint_plan(self.h, C.byref(hi), C.byref(sg)), "adjoint_plan")
+        return bool(hi.value), sg.value
+
     def set_tile_config(self, threads, wg_per_cu):
         """Tile-kernel launch configuration (structured grids); results do not depend on it."""
         self._check(self.api.set_tile_config(self.h, int(threads), int(wg_per_cu)), "set_tile_config")

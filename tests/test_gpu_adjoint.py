@@ -8,8 +8,9 @@ m = per-element (rho, lambda, mu), evaluated at the smooth background.
 Bars: the directional derivative g.dm of the FP64 adjoint gradient agrees with the
 central difference (J(m + e dm) - J(m - e dm)) / 2e within 1e-3 (relative), both for
 the GPU forward and for the oracle (the C restatement of the reference, on the CPU);
-the gradient does not depend on the checkpoint segment length (bitwise); the FP32
-bulk gradient is reported against the FP64 one.
+the gradient does not depend on the checkpoint segment length (bitwise); the default
+plan (displacement history, no replay) agrees with the checkpoint plan to round-off; the
+FP32 bulk gradient is reported against the FP64 one.
 """
 import numpy as np
 import pytest
@@ -103,6 +104,25 @@ def test_checkpoint_segments_do_not_change_the_gradient(prob):
     for g in grads[1:]:
         for a, b in zip(g, grads[0]):
             assert np.array_equal(a, b)
+
+
+def test_history_plan_matches_checkpoints(prob):
+    """The default plan keeps every step's forward displacement (no replay) and takes a_n
+    and the predicted velocity from differences of it; it agrees with the checkpoint
+    plan (stored a_n and v_n) to round-off, in FP64 and with FP32 fields."""
+    back, obs = prob["back"], prob["obs"]
+    for fp32, tol in ((False, 1e-9), (True, 1e-3)):
+        grads = []
+        for seg, history in ((0, True), (20, False)):
+            model = adjoint.SeismogramModel(prob["path"], seg=seg, bulk_fp32=fp32)
+            seis = model.forward(*back)
+            assert model._state[1].adjoint_plan()[0] == history
+            grads.append(model.backward(seis - obs))
+            model.close()
+        errs = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(*grads)]
+        print(f"\n  fp32={fp32}: history vs checkpoint plan rel-L2 (rho, lam, mu): {errs}", flush=True)
+        assert max(errs) < tol
+    adjoint.release()  # the kept history block goes back to the device
 
 
 def test_fp32_bulk_gradient(prob):
