@@ -827,6 +827,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
   __shared__ LV T2[FE_EPB][NP];
   __shared__ double Gr[FE_EPB][NP][2];
   __shared__ double Gs[FE_EPB][NG][2];  // row sums of Gr
+  __shared__ double Ge[TX * TZ][2];     // dJ/dlambda, dJ/dmu of the tile's elements, accumulated
+  __shared__ R Gm[TX * TZ][3];          // geometric factors of the tile's elements
   __shared__ short sflag[PMAX];
 
   const int bid = blockIdx.x;
@@ -873,34 +875,72 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
         avr[r][c] = vk;
         mb -= (double)f[c] * fwd_accel(afw, dprev, dnext, c * S + g, u[c], idt2);
       }
-      mbr[r] = mb;
+      // interior nodes: the running sum is read here, with the other loads, and written
+      // back in the store phase (perimeter and special nodes are the deferred kernel's)
+      mbr[r] = perim_slot(X, Z, PX, PZ) < 0 ? mbar[g] + mb : 0.0;
       Uf[p] = lv_make<R, NDOF>(f);
       Ud[p] = lv_make<R, NDOF>(u);
       Fn[p] = LV{};
       sflag[p] = -1;
     }
   }
+  // FP64 fields: moduli of this lane's point in each batch loaded with the node values
+  // (no global load inside the element batches; with FP32 fields the registers are
+  // better spent on occupancy, and the batches load their own); geometric factors of the
+  // elements in LDS
+  constexpr bool PREF = sizeof(R) == 8;
+  R lm[4 * ((TX * TZ / 4 + FE_EPB - 1) / FE_EPB)][2];
+#pragma unroll
+  for (int q = 0; q < (PREF ? 4 * ((TX * TZ / 4 + FE_EPB - 1) / FE_EPB) : 0); ++q) {
+    constexpr int NB = (TX * TZ / 4 + FE_EPB - 1) / FE_EPB;
+    const int cc = q / NB, b0 = (q % NB) * FE_EPB;
+    const int cx = cc & 1, cz = cc >> 1;
+    const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
+    const int ce = b0 + le;
+    if (tid < FE_THREADS && ce < ncx * ncz) {
+      const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, cx + 2 * (ce % ncx), cz + 2 * (ce / ncx));
+      lm[q][0] = coefs[cell * 2 * NP + pe];
+      lm[q][1] = coefs[cell * 2 * NP + NP + pe];
+    }
+  }
+  if (tid < nxe * nze * 3) {
+    const int e = tid / 3, k = tid - 3 * (tid / 3);
+    Gm[e][k] = geom[cell_index(gr, tg.z >> 16, tx0, tz0, e % nxe, e / nxe) * 3 + k];
+  }
+  if (tid < nxe * nze) {  // the elements' running sensitivities (local index lz*nxe + lx)
+    const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, tid % nxe, tid / nxe);
+    Ge[tid][0] = glam[cell];
+    Ge[tid][1] = gmu[cell];
+  }
   __syncthreads();
   const int nspec = T.spec_cnt[t];
   if (tid < nspec) sflag[T.spec_idx[t * MAXSPECIAL + tid]] = (short)tid;
 
-#pragma unroll 1
-  for (int cc = 0; cc < 4; ++cc) {
+  // element batch q: colour cc = q / NBATCH, elements b0 .. b0+FE_EPB-1 of that colour
+  constexpr int NBATCH = (TX * TZ / 4 + FE_EPB - 1) / FE_EPB;
+  auto batch = [&](const int q) {
+    const int cc = q / NBATCH, b0 = (q % NBATCH) * FE_EPB;
     const int cx = cc & 1, cz = cc >> 1;
     const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
     const int nce = ncx * ncz;
-#pragma unroll 1
-    for (int b0 = 0; b0 < nce; b0 += FE_EPB) {
+    if (b0 < nce) {  // uniform
       const int ce = b0 + le;
       const bool active = (tid < FE_THREADS) && (ce < nce);
-      int base = 0;
-      size_t cell = 0;
+      int base = 0, el = 0;
       if (active) {
         const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
         base = 4 * lz * PX + 4 * lx;
-        cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
-        const R g0 = geom[cell * 3 + 0], g1 = geom[cell * 3 + 1], g2 = geom[cell * 3 + 2];
-        const R la = coefs[cell * 2 * NP + pe], mu = coefs[cell * 2 * NP + NP + pe];
+        el = lz * nxe + lx;
+        const R g0 = Gm[el][0], g1 = Gm[el][1], g2 = Gm[el][2];
+        R la, mu;
+        if constexpr (PREF) {
+          la = lm[q][0];
+          mu = lm[q][1];
+        } else {
+          const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
+          la = coefs[cell * 2 * NP + pe];
+          mu = coefs[cell * 2 * NP + NP + pe];
+        }
         const R kx = la + 2 * mu;
         const R A0 = w * kx * g0, A1 = w * la * g2, A2 = w * kx * g1;
         const R A3 = w * mu * g1, A4 = w * mu * g2, A5 = w * mu * g0;
@@ -967,12 +1007,24 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
           sl += Gs[le][q][0];
           sm += Gs[le][q][1];
         }
-        glam[cell] += sl;
-        gmu[cell] += sm;
+        Ge[el][0] += sl;
+        Ge[el][1] += sm;
       }
     }
+  };
+  if constexpr (PREF) {  // moduli in registers: batches unrolled
+#pragma unroll
+    for (int q = 0; q < 4 * NBATCH; ++q) batch(q);
+  } else {
+#pragma unroll 1
+    for (int q = 0; q < 4 * NBATCH; ++q) batch(q);
   }
-
+  __syncthreads();  // every batch's element sums are in Ge
+  if (tid < nxe * nze) {
+    const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, tid % nxe, tid / nxe);
+    glam[cell] = Ge[tid][0];
+    gmu[cell] = Ge[tid][1];
+  }
   const size_t fbase = (size_t)t * SLOTS;
 #pragma unroll
   for (int r = 0; r < NODE_ITERS; ++r) {
@@ -995,7 +1047,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
           av[c * S + g] = vbp + dt * dn;
           aa[c * S + g] = c2 * vbp + c1 * dn;
         }
-        mbar[g] += mbr[r];
+        mbar[g] = mbr[r];
       }
     }
   }
