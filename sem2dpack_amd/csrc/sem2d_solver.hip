@@ -459,12 +459,13 @@ bool structured_map(sem2d_solver* h, const sem2d_desc* s) {
   {
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    // measured on MI355X (2000x1000-element P-SV): 256 threads (10-element batches)
-    // with 2 workgroups per CU for FP64 fields (LDS and VGPRs allow 2), 3 for FP32
-    int per_cu = h->fp32 ? 3 : 2;
+    // measured on MI355X (2000x1000-element P-SV, full-tile variant): 512 threads
+    // (20-element batches, one batch per colour) with 1 workgroup per CU for FP64 fields
+    // (VGPRs allow 8 waves per CU), 2 for FP32
+    int per_cu = h->fp32 ? 2 : 1;
     const char* env = getenv("SEM2D_TILE_WG_PER_CU");
     if (env) per_cu = atoi(env);
-    S.pipe_threads = 256;
+    S.pipe_threads = 512;
     const char* envt = getenv("SEM2D_TILE_THREADS");
     if (envt) S.pipe_threads = atoi(envt) == 256 ? 256 : 512;
     // the persistent kernel pays off once there are several tiles per workgroup
