@@ -190,6 +190,16 @@ def run_adjoint(args):
     t_fwd = time.perf_counter() - t1
     fwd_ev_ms = s.last_step_ms() * nt
     s.close()
+    # the same with FP32 fields, the reference point of the FP32 gradient
+    s = solver.Solver(pb, bulk_fp32=True)
+    s.step(5)
+    s.close()
+    s = solver.Solver(pb, bulk_fp32=True)
+    t1 = time.perf_counter()
+    s.step(nt)
+    t_fwd32 = time.perf_counter() - t1
+    fwd32_ev_ms = s.last_step_ms() * nt
+    s.close()
     out = {}
     for fp32 in (False, True):
         m = adjoint.SeismogramModel(path, bulk_fp32=fp32)
@@ -239,8 +249,11 @@ def run_adjoint(args):
                    "build_s": round(build_s, 1)},
         "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
         "device_ms": {"forward_plain": fwd_ev_ms, "forward_storing": ev_f, "backward": ev_b},
-        "fp32_bulk": {"gradient_s": out[True]["t"], "gradient_over_forward": out[True]["t"] / t_fwd,
-                      "device_ms": {"forward_storing": out[True]["ev"][0], "backward": out[True]["ev"][1]},
+        "fp32_bulk": {"gradient_s": out[True]["t"], "forward_s": t_fwd32,
+                      "gradient_over_forward": out[True]["t"] / t_fwd32,
+                      "gradient_over_fp64_forward": out[True]["t"] / t_fwd,
+                      "device_ms": {"forward_plain": fwd32_ev_ms, "forward_storing": out[True]["ev"][0],
+                                    "backward": out[True]["ev"][1]},
                       "storage_plan": out[True]["plan"]},
         "fd_check": {"directions": args.fd_dirs, "eps_rel": eps * 1e-2,
                      "max_rel_err_fp64": max(errs64) if errs64 else None,

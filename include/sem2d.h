@@ -93,6 +93,15 @@ int sem2d_create(const sem2d_desc *desc, sem2d_solver **out);
    replaces BC_ABSO_apply (SRC/bc_abso.f90:286-336).  C is [ndof][n]. */
 int sem2d_set_abso(sem2d_solver *h, int n, const int *node, const double *C);
 
+/* Stacey term of the absorbing boundary registered last (P-SV, flat; BC_ABSORB
+   stacey = T): replaces bc%K and the KxD part of BC_ABSO_apply (SRC/bc_abso.f90:176-224,
+   321-333).  For boundary element e (boundary order) and edge point k (the order of the
+   boundary's element edges), enode[e][k] is the 0-based index into that boundary's node
+   array and K[e][k][c] the coefficient of component c; the force at the boundary nodes
+   loses sum_e K[e][k][c] * sum_j h'_j(x_k) D_c(node[enode[e][j]]), D the predicted
+   displacement, after the boundary's -C v term. */
+int sem2d_set_abso_stacey(sem2d_solver *h, int nbe, const int *enode, const double *K);
+
 /* Point force at a node: replaces FORCE_add (SRC/src_force.f90:77-90).  dir[2] is used
    when ndof = 2.  Returns the source index through *isrc. */
 int sem2d_set_force(sem2d_solver *h, int iglob, const double *dir, int *isrc);

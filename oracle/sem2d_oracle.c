@@ -15,7 +15,7 @@
  *     kd2_psv, kd2_sh    ELAST_KD2_PSV/SH, My_MATMUL  SRC/mat_elastic.f90:589-641,739-799
  *   sources              SO_add, FORCE_add, SRC_MOMENT_add  SRC/src_gen.f90:290-317,
  *                        SRC/src_force.f90:77-90, SRC/src_moment.f90:217-235
- *   abso_apply           BC_ABSO_apply (flat, no Stacey)  SRC/bc_abso.f90:286-336
+ *   abso_apply           BC_ABSO_apply (flat; Stacey P-SV)  SRC/bc_abso.f90:286-336
  *   fault_apply          BC_DYNFLT_apply      SRC/bc_dynflt.f90:569-739
  *     swf                swf_mu, swf_update_state  SRC/bc_dynflt_swf.f90:142-194
  *     rsf                rsf_solver, rsf_update_theta, rsf_update_V, nr_solver,
@@ -60,6 +60,9 @@ typedef struct {
   int n;
   int *node;    /* 0-based */
   double *C;    /* [ndof][n] */
+  int nbe;      /* Stacey: boundary elements (0: none) */
+  int *enode;   /* [nbe][NG] boundary-local node of each edge point */
+  double *K;    /* [nbe][NG][2] */
 } abso_t;
 
 typedef struct {
@@ -153,6 +156,20 @@ int oracle_set_abso(oracle_solver *h, int n, const int *node, const double *C) {
   b->node = (int *)malloc(sizeof(int) * n);
   for (int k = 0; k < n; ++k) b->node[k] = node[k] - 1;
   b->C = dup_d(C, (size_t)h->ndof * n);
+  return 0;
+}
+
+int oracle_set_abso_stacey(oracle_solver *h, int nbe, const int *enode, const double *K) {
+  if (h->nabso < 1) return fail(h, 3, "set_abso_stacey: no absorbing boundary");
+  if (h->ndof != 2) return fail(h, 3, "set_abso_stacey: P-SV only");
+  abso_t *b = &h->abso[h->nabso - 1];
+  if (nbe < 1 || !enode || !K) return fail(h, 3, "set_abso_stacey: empty");
+  for (int k = 0; k < nbe * NG; ++k)
+    if (enode[k] < 0 || enode[k] >= b->n) return fail(h, 3, "set_abso_stacey: node out of range");
+  b->nbe = nbe;
+  b->enode = (int *)malloc(sizeof(int) * (size_t)nbe * NG);
+  memcpy(b->enode, enode, sizeof(int) * (size_t)nbe * NG);
+  b->K = dup_d(K, (size_t)nbe * NG * 2);
   return 0;
 }
 
@@ -693,6 +710,27 @@ static void one_step(oracle_solver *h) {
         int g = B->node[k];
         f[c * N + g] = f[c * N + g] - B->C[c * B->n + k] * v[c * N + g];
       }
+    if (B->nbe) {
+      /* Stacey (SRC/bc_abso.f90:321-333): KxD(k(:),:) += K(:,:,e) * matmul(Ht, D(nodes(k),:))
+         over the boundary elements in order, then MxA(nodes,:) -= KxD; D is the predicted
+         displacement, Ht(k,j) = h'_j(x_k) */
+      double *kxd = (double *)calloc((size_t)B->n * 2, sizeof(double));
+      for (int e = 0; e < B->nbe; ++e) {
+        const int *en = &B->enode[e * NG];
+        for (int c = 0; c < 2; ++c)
+          for (int kk = 0; kk < NG; ++kk) {
+            double s = 0.0;
+            for (int j = 0; j < NG; ++j) s = s + h->H[j][kk] * d[c * N + B->node[en[j]]];
+            kxd[en[kk] * 2 + c] = kxd[en[kk] * 2 + c] + B->K[((size_t)e * NG + kk) * 2 + c] * s;
+          }
+      }
+      for (int c = 0; c < 2; ++c)
+        for (int k = 0; k < B->n; ++k) {
+          int g = B->node[k];
+          f[c * N + g] = f[c * N + g] - kxd[k * 2 + c];
+        }
+      free(kxd);
+    }
   }
   if (h->flt) fault_apply(h, f);
   for (size_t k = 0; k < n; ++k) a[k] = f[k] * h->rmass[k];
@@ -766,7 +804,9 @@ const char *oracle_last_error(oracle_solver *h) { return h ? h->err : "null hand
 void oracle_destroy(oracle_solver *h) {
   if (!h) return;
   free(h->ibool); free(h->coefs); free(h->rmass); free(h->d); free(h->v); free(h->a);
-  for (int b = 0; b < h->nabso; ++b) { free(h->abso[b].node); free(h->abso[b].C); }
+  for (int b = 0; b < h->nabso; ++b) {
+    free(h->abso[b].node); free(h->abso[b].C); free(h->abso[b].enode); free(h->abso[b].K);
+  }
   for (int s = 0; s < h->nsrc; ++s) {
     free(h->src[s].ixi); free(h->src[s].ieta); free(h->src[s].cxi); free(h->src[s].ceta);
     free(h->src[s].ampl);

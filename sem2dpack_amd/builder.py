@@ -695,9 +695,9 @@ def _normal_and_weights(t, jac, wgll):
 
 
 def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
-    """BC_ABSO_init (SRC/bc_abso.f90:115-266), Clayton-Engquist on flat boundaries."""
-    if stacey:
-        raise InputError("BC_ABSORB stacey=T is not supported by this build")
+    """BC_ABSO_init (SRC/bc_abso.f90:115-266), Clayton-Engquist on flat boundaries, with the
+    Stacey term for P-SV when ``stacey`` (SRC/bc_abso.f90:176-180,209-224):
+    K[e, k, geo_tan] = -K[e, k, geo_nor] = w_k |J1D| * DLocDGlob(tan, tan) * rho cs (2 cs - cp)."""
     J11, J12, J21, J22 = jac
     NORM, _ = _normal_and_weights(t, jac, wgll)
     if np.all(np.abs(NORM[:, 0]) < TINY_XABS):
@@ -710,6 +710,8 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
     ii, jj = _edge_ij(t.edge)
     C = np.zeros((len(t.node), ndof))
     rho_p, cp_p, cs_p = props
+    stacey = stacey and ndof == 2  # SRC/bc_abso.f90:176 (flat boundaries only, checked above)
+    K = np.zeros((len(t.elem), 5, 2)) if stacey else None
     cb, ce, cf = [], [], []  # contributions: boundary node, element, weight w_k |J1D|
     for n, e in enumerate(t.elem):
         for k in range(5):
@@ -732,12 +734,20 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
             else:
                 C[bck, 0] = C[bck, 0] + rho * c[0] * coef
                 C[bck, 1] = C[bck, 1] + rho * c[1] * coef
+            if stacey:
+                inv = _inv_jac(J11[e, j, i], J12[e, j, i], J21[e, j, i], J22[e, j, i])
+                xji = inv[2 * dim_t + geo_tan]  # DLocDGlob(LocDimTan, GeoDimTan)
+                K[n, k, 0] = coef * xji * rho * c[geo_tan] * (2.0 * c[geo_tan] - c[geo_nor])
+    if stacey:
+        K[:, :, 1] = K[:, :, 0]
+        K[:, :, geo_tan] = -K[:, :, geo_tan]
     nodes0 = t.node - 1
     M[:, nodes0] = M[:, nodes0] + coefA2Vrhs * C.T
     # C[node, c] = sum over contributions of rho * c_c * weight, c_c = cp along the
     # normal (component geo_nor), cs along the boundary
     return dict(tag=t.tag, node=t.node.astype(np.int32), C=C, geo_nor=geo_nor,
-                contrib=(np.array(cb, dtype=np.int64), np.array(ce, dtype=np.int64), np.array(cf)))
+                contrib=(np.array(cb, dtype=np.int64), np.array(ce, dtype=np.int64), np.array(cf)),
+                stacey=(np.ascontiguousarray(t.ibool[:, :5], dtype=np.int32), K) if stacey else None)
 
 
 # ---------------------------------------------------------------------------

@@ -218,6 +218,30 @@ __global__ void k_abso(int n, int ndof, const int* __restrict__ node, const doub
   }
 }
 
+// Stacey term of one flat P-SV absorbing boundary (SRC/bc_abso.f90:321-333), general path:
+// node q of the boundary (global node node[q]) subtracts KxD = sum over its boundary
+// elements (t = off[q] .. off[q+1]-1, element order; idx[t] = e * NG + k) of
+// K[e][k][c] * sum_j h'_j(x_k) d_c(enode[e][j]), d the predicted displacement.
+template <typename R>
+__global__ void k_stacey(int n, const int* __restrict__ node, const int* __restrict__ off,
+                         const int* __restrict__ idx, const int* __restrict__ enode, const double* __restrict__ K,
+                         const R* __restrict__ d, R* __restrict__ f, int npoin) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int g = node[q];
+  for (int c = 0; c < 2; ++c) {
+    double s = 0.0;
+    for (int t = off[q]; t < off[q + 1]; ++t) {
+      const int ek = idx[t], e = ek / NG, k = ek - NG * e;
+      double hd = 0.0;
+      for (int j = 0; j < NG; ++j) hd += c_H[j * NG + k] * (double)d[(size_t)c * npoin + enode[e * NG + j]];
+      s += K[(size_t)ek * 2 + c] * hd;
+    }
+    R* fp = f + (size_t)c * npoin + g;
+    *fp = (R)((double)*fp - s);
+  }
+}
+
 // receivers: out[s][c][r] = field[c][node[r]]
 template <typename R>
 __global__ void k_receivers(int nrec, int ndof, const int* __restrict__ node, const R* __restrict__ field,

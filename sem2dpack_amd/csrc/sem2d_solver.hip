@@ -38,11 +38,18 @@ struct AbsoDev {
   int n;
   int* node;
   double* C;
+  // Stacey (P-SV): boundary elements, their edge points' global nodes [nbe][NG] and K
+  // [nbe][NG][2]; per boundary node the contributions e*NG+k (element order) soff/sidx
+  int nbe = 0;
+  int *enode = nullptr, *soff = nullptr, *sidx = nullptr;
+  double* K = nullptr;
 };
 
 struct AbsoHost {
   std::vector<int> node;   // 0-based reference nodes
   std::vector<double> C;   // [ndof][n]
+  std::vector<int> senode;  // Stacey: [nbe][NG] boundary-local node of each edge point
+  std::vector<double> sK;   // [nbe][NG][2]
 };
 
 // Structured-mesh fast path (node-grid field order, fused tile kernel)
@@ -62,6 +69,11 @@ struct StructState {
   int4* grids_dev = nullptr;
   int* tlist_dev = nullptr;  // launch list of the pipelined kernel: full tiles, then the rest
   int nfull = 0, npart = 0;
+  // Stacey terms: boundary elements (storage-index edge nodes, K), per absorbing entry
+  // of the deferred nodes the element points it subtracts, and their values per step
+  int s_nbe = 0;
+  int *s_enode = nullptr, *ak_off = nullptr, *ak_idx = nullptr;
+  double *s_K = nullptr, *kxd = nullptr;
   int* spec_cnt = nullptr;
   int* spec_idx = nullptr;
   void* fpart = nullptr;
@@ -530,6 +542,8 @@ struct StructTables {
   std::vector<double> rmdef;     // [ndof][ndef]
   std::vector<int> fault_g;
   std::vector<double> fault_rm;  // [ndof][nf]
+  std::vector<int> s_enode, ak_off, ak_idx;  // Stacey (see StructState)
+  std::vector<double> s_K;
 };
 
 int struct_tables(sem2d_solver* h, StructTables& X) {
@@ -582,6 +596,26 @@ int struct_tables(sem2d_solver* h, StructTables& X) {
   std::vector<char> is_fault((size_t)h->npoin, 0);
   for (int k : h->fault_n1) is_fault[k] = 1;
   for (int k : h->fault_n2) is_fault[k] = 1;
+  // Stacey: global list of boundary elements (in boundary order) and, per boundary and
+  // boundary-local node, its element points e*NG+k in element order
+  X.s_enode.clear();
+  X.s_K.clear();
+  std::vector<std::vector<std::vector<int>>> scon(h->abso_host.size());
+  for (size_t b = 0; b < h->abso_host.size(); ++b) {
+    const AbsoHost& B = h->abso_host[b];
+    if (B.senode.empty()) continue;
+    const int base = (int)(X.s_enode.size() / NG), nbe = (int)(B.senode.size() / NG);
+    scon[b].assign(B.node.size(), {});
+    for (int e = 0; e < nbe; ++e)
+      for (int k = 0; k < NG; ++k) {
+        const int bn = B.senode[(size_t)e * NG + k];
+        X.s_enode.push_back(S.gmap[B.node[bn]]);
+        scon[b][bn].push_back((base + e) * NG + k);
+      }
+    X.s_K.insert(X.s_K.end(), B.sK.begin(), B.sK.end());
+  }
+  X.ak_off.assign(1, 0);
+  X.ak_idx.clear();
 
   X.defs.clear();
   X.part_off.clear();
@@ -611,6 +645,9 @@ int struct_tables(sem2d_solver* h, StructTables& X) {
         const AbsoHost& B = h->abso_host[bk.first];
         const size_t nb = B.node.size();
         for (int c = 0; c < nd; ++c) X.dabso.push_back(B.C[(size_t)c * nb + bk.second]);
+        if (!scon[bk.first].empty())
+          for (int t : scon[bk.first][bk.second]) X.ak_idx.push_back(t);
+        X.ak_off.push_back((int)X.ak_idx.size());
       }
     n.acnt = (int)(X.dabso.size() / nd) - n.astart;
     n.fault = is_fault[ref];
@@ -678,6 +715,17 @@ int prepare_structured(sem2d_solver* h) {
   HIPCHK(h, hipMalloc(&fp, std::max<size_t>(S.FPS * h->ndof, 1) * rsize(h)));
   S.prep_allocs.push_back(fp);
   S.fpart = fp;
+  S.s_nbe = (int)(X.s_enode.size() / NG);
+  if (S.s_nbe > 0) {
+    if ((rc = prep_upload(h, &S.s_enode, X.s_enode.data(), X.s_enode.size()))) return rc;
+    if ((rc = prep_upload(h, &S.s_K, X.s_K.data(), X.s_K.size()))) return rc;
+    if ((rc = prep_upload(h, &S.ak_off, X.ak_off.data(), X.ak_off.size()))) return rc;
+    if ((rc = prep_upload(h, &S.ak_idx, X.ak_idx.data(), X.ak_idx.size()))) return rc;
+    void* q = nullptr;
+    HIPCHK(h, hipMalloc(&q, X.s_enode.size() * 2 * sizeof(double)));
+    S.prep_allocs.push_back(q);
+    S.kxd = (double*)q;
+  }
   S.nfnodes = (int)X.fault_g.size();
   if ((rc = prep_upload(h, &S.fault_g, X.fault_g.data(), X.fault_g.size()))) return rc;
   if ((rc = prep_upload_bulk(h, &S.fault_rm, X.fault_rm))) return rc;
@@ -695,6 +743,10 @@ struct StepDev {
   TileDev T{};
   const int* tlist = nullptr;  // pipelined kernel: tlist[0 .. nfull) full tiles, then npart others
   int nfull = 0, npart = 0;
+  int s_nbe = 0;  // Stacey boundary elements (k_stacey_pre), their values passed to k_deferred
+  const int* s_enode = nullptr;
+  const double* s_K = nullptr;
+  StaceyRefs SR{};
   int pipe_wgs = 0, pipe_threads = 512;
   const void *rmass0 = nullptr, *coefs = nullptr, *geom = nullptr;
   void* fpart = nullptr;
@@ -734,6 +786,10 @@ StepDev step_dev(sem2d_solver* h) {
   X.tlist = S.tlist_dev;
   X.nfull = S.nfull;
   X.npart = S.npart;
+  X.s_nbe = S.s_nbe;
+  X.s_enode = S.s_enode;
+  X.s_K = S.s_K;
+  if (S.s_nbe > 0) X.SR = StaceyRefs{S.ak_off, S.ak_idx, S.kxd};
   X.T.spec_cnt = S.spec_cnt;
   X.T.spec_idx = S.spec_idx;
   X.T.NGp = h->fstride;
@@ -829,16 +885,21 @@ void launch_structured_step(const StepDev& X, int it, R dt, R c1, R c2, R c3, R 
     else launch_tile_m<R, 1, 3>(X, dt, c1, c2, c3, c4);
   }
   const int gd = (X.ndef + 255) / 256;
+  if (X.s_nbe > 0)  // Stacey terms from the previous step's state (boundary nodes are deferred)
+    k_stacey_pre<R><<<(X.s_nbe * NG + 127) / 128, 128, 0, X.stream>>>(
+        X.s_nbe, X.s_enode, X.s_K, (const R*)(X.din ? X.din : X.d), (const R*)(X.din ? X.vin : X.v),
+        (const R*)(X.din ? X.ain : X.a), X.fstride, dt, c1, (double*)X.SR.kxd);
   if (X.ndof == 2 && X.din)
     k_deferred<R, 2, true><<<gd, 256, 0, X.stream>>>(X.ndef, X.defs, X.part_off, (const R*)X.fpart, X.T.FPS,
                                                       X.dsrc, X.ampl, X.ampl_nt, it, X.itp, X.dabso,
                                                       (const R*)X.rmdef, (R*)X.d, (R*)X.v, (R*)X.a, X.fstride, dt,
                                                       c1, c2, c3, c4, (const R*)X.din, (const R*)X.vin,
-                                                      (const R*)X.ain);
+                                                      (const R*)X.ain, X.SR);
   else if (X.ndof == 2)
     k_deferred<R, 2><<<gd, 256, 0, X.stream>>>(X.ndef, X.defs, X.part_off, (const R*)X.fpart, X.T.FPS, X.dsrc,
                                                 X.ampl, X.ampl_nt, it, X.itp, X.dabso, (const R*)X.rmdef, (R*)X.d,
-                                                (R*)X.v, (R*)X.a, X.fstride, dt, c1, c2, c3, c4);
+                                                (R*)X.v, (R*)X.a, X.fstride, dt, c1, c2, c3, c4, nullptr, nullptr,
+                                                nullptr, X.SR);
   else
     k_deferred<R, 1><<<gd, 256, 0, X.stream>>>(X.ndef, X.defs, X.part_off, (const R*)X.fpart, X.T.FPS, X.dsrc,
                                                 X.ampl, X.ampl_nt, it, X.itp, X.dabso, (const R*)X.rmdef, (R*)X.d,
@@ -885,8 +946,12 @@ int do_steps(sem2d_solver* h, int nsteps) {
     if (!h->src_host.empty())
       k_sources<R><<<1, 64, 0, h->stream>>>((int)h->src_host.size(), h->src_dev, h->ampl_dev, h->ampl_nt, it, a,
                                            h->npoin);
-    for (auto& b : h->abso)
+    for (auto& b : h->abso) {
       k_abso<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, h->ndof, b.node, b.C, v, a, h->npoin);
+      if (b.nbe)
+        k_stacey<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, b.node, b.soff, b.sidx, b.enode, b.K, d, a,
+                                                                h->npoin);
+    }
     if (h->flt.on) k_fault<R><<<grid_for(h->flt.dev.np, 64), 64, 0, h->stream>>>(h->flt.dev, d, v, a, h->npoin);
     k_correct<R><<<gb, 256, 0, h->stream>>>(n, d, v, a, (const R*)h->rmass, c3, c4);
     h->it = it;
@@ -1047,6 +1112,38 @@ int sem2d_set_abso(sem2d_solver* h, int n, const int* node, const double* C) {
   ah.node = nd;
   ah.C.assign(C, C + (size_t)h->ndof * n);
   h->abso_host.push_back(ah);
+  h->st.ready = false;
+  return 0;
+}
+
+int sem2d_set_abso_stacey(sem2d_solver* h, int nbe, const int* enode, const double* K) {
+  if (!h) return 1;
+  if (h->abso.empty()) return set_err(h, 3, "set_abso_stacey: call sem2d_set_abso first");
+  if (h->ndof != 2) return set_err(h, 3, "set_abso_stacey: P-SV only");
+  if (nbe < 1 || !enode || !K) return set_err(h, 3, "set_abso_stacey: empty");
+  AbsoHost& ah = h->abso_host.back();
+  AbsoDev& ad = h->abso.back();
+  if (ad.nbe) return set_err(h, 3, "set_abso_stacey: this boundary already has its Stacey term");
+  const int n = (int)ah.node.size();
+  for (int k = 0; k < nbe * NG; ++k)
+    if (enode[k] < 0 || enode[k] >= n) return set_err(h, 3, "set_abso_stacey: node out of range");
+  ah.senode.assign(enode, enode + (size_t)nbe * NG);
+  ah.sK.assign(K, K + (size_t)nbe * NG * 2);
+  // general path: global nodes of the edge points, per boundary node its element points
+  std::vector<int> eg((size_t)nbe * NG), off(n + 1, 0), idx((size_t)nbe * NG);
+  for (int k = 0; k < nbe * NG; ++k) {
+    eg[k] = ah.node[enode[k]];
+    off[enode[k] + 1]++;
+  }
+  for (int q = 0; q < n; ++q) off[q + 1] += off[q];
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int k = 0; k < nbe * NG; ++k) idx[fill[enode[k]]++] = k;  // element order
+  int rc;
+  if ((rc = upload(h, &ad.enode, eg.data(), eg.size()))) return rc;
+  if ((rc = upload(h, &ad.K, K, (size_t)nbe * NG * 2))) return rc;
+  if ((rc = upload(h, &ad.soff, off.data(), off.size()))) return rc;
+  if ((rc = upload(h, &ad.sidx, idx.data(), idx.size()))) return rc;
+  ad.nbe = nbe;
   h->st.ready = false;
   return 0;
 }
@@ -1609,6 +1706,8 @@ int check_members(sem2d_batch* b) {
     if (h->dt != h0->dt || h->beta != h0->beta || h->gamma != h0->gamma)
       return berr(b, 3, "member %d: time step and Newmark parameters must match", m);
     if (h->flt.on != h0->flt.on) return berr(b, 3, "member %d: either every member has a fault or none", m);
+    for (const AbsoHost& B : h->abso_host)
+      if (!B.senode.empty()) return berr(b, 3, "member %d: Stacey boundaries are not batched", m);
     if (h->flt.on) {
       const FaultDev &F = h->flt.dev, &F0 = h0->flt.dev;
       if (F.law != F0.law || F.kind != F0.kind || F.opening != F0.opening || F.healing != F0.healing ||
@@ -2470,6 +2569,8 @@ int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
     return set_err(h, 4, "adjoint: needs a structured P-SV grid with compact (lambda, mu) coefficients");
   if (h->beta != 0.0) return set_err(h, 4, "adjoint: needs the explicit scheme (beta = 0)");
   if (h->flt.on) return set_err(h, 4, "adjoint: faults are not differentiated");
+  for (const AbsoHost& B : h->abso_host)
+    if (!B.senode.empty()) return set_err(h, 4, "adjoint: Stacey boundaries are not differentiated");
   if (nsteps < 1) return set_err(h, 3, "adjoint_init: nsteps must be >= 1");
   const bool autoplan = seg <= 0;
   if (seg <= 0) seg = (int)std::ceil(std::sqrt((double)nsteps));

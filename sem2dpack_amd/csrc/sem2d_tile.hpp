@@ -699,6 +699,40 @@ struct DefNode {
   int fault;    // 1: split-fault node (finished by the fault kernels)
 };
 
+// Stacey terms of the absorbing entries of the deferred nodes: entry q (the q-th
+// absorbing coefficient set of the step, in boundary order) subtracts kxd[idx[t]],
+// t = off[q] .. off[q+1]-1 (its boundary elements in order), after its -C v term.
+struct StaceyRefs {
+  const int* off = nullptr;
+  const int* idx = nullptr;
+  const double* kxd = nullptr;  // [boundary element][k][2], from k_stacey_pre
+};
+
+// Stacey term of flat P-SV absorbing boundaries (BC_ABSO_apply, SRC/bc_abso.f90:321-333)
+// per boundary element e and edge point k: kxd[e][k][c] = K[e][k][c] * sum_j h'_j(x_k)
+// D_c(node[e][j]), D the predicted displacement d + dt v + c1 a.  The boundary nodes are
+// deferred nodes, which k_deferred predicts later in the step, so the prediction is formed
+// here from the state of the previous step (the same expression as in k_deferred).
+template <typename R>
+__global__ void k_stacey_pre(int nbe, const int* __restrict__ enode, const double* __restrict__ K,
+                             const R* __restrict__ d, const R* __restrict__ v, const R* __restrict__ a, size_t GS,
+                             R dt, R c1, double* __restrict__ kxd) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;  // e * NG + k
+  if (q >= nbe * NG) return;
+  const int e = q / NG, k = q - NG * e;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      const size_t g = (size_t)c * GS + enode[e * NG + j];
+      const R dp = d[g] + dt * v[g] + c1 * a[g];
+      s += c_H[j * NG + k] * (double)dp;
+    }
+    kxd[(size_t)q * 2 + c] = K[(size_t)q * 2 + c] * s;
+  }
+}
+
 template <typename R, int NDOF, bool OOP = false>
 __global__ __launch_bounds__(256) void k_deferred(int ndef, const DefNode* __restrict__ defs,
                                                   const int* __restrict__ part_off, const R* __restrict__ fpart,
@@ -710,7 +744,7 @@ __global__ __launch_bounds__(256) void k_deferred(int ndef, const DefNode* __res
                                                   R dt, R c1, R c2, R c3, R c4,
                                                   const R* __restrict__ din = nullptr,
                                                   const R* __restrict__ vin = nullptr,
-                                                  const R* __restrict__ ain = nullptr) {
+                                                  const R* __restrict__ ain = nullptr, StaceyRefs SR = {}) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= ndef) return;
   const DefNode n = defs[q];
@@ -739,6 +773,17 @@ __global__ __launch_bounds__(256) void k_deferred(int ndef, const DefNode* __res
   for (int k = 0; k < n.acnt; ++k) {
 #pragma unroll
     for (int c = 0; c < NDOF; ++c) f[c] = (R)((double)f[c] - abso[(size_t)(n.astart + k) * NDOF + c] * (double)vpp[c]);
+    if (SR.off) {  // Stacey (P-SV)
+      const int t0 = SR.off[n.astart + k], t1 = SR.off[n.astart + k + 1];
+      if (t1 > t0) {
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          double s = 0.0;
+          for (int t = t0; t < t1; ++t) s += SR.kxd[(size_t)SR.idx[t] * 2 + c];
+          f[c] = (R)((double)f[c] - s);
+        }
+      }
+    }
   }
   if (n.fault) {
 #pragma unroll

@@ -34,7 +34,8 @@ module sem2d_hip
     integer(c_int) :: oit = 0, oitd = 1, oix1 = 1, oixn = 1, oixd = 1
   end type sem2d_fault_desc
 
-  public :: sem2d_create, sem2d_set_kelvin_voigt, sem2d_set_abso, sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
+  public :: sem2d_create, sem2d_set_kelvin_voigt, sem2d_set_abso, sem2d_set_abso_stacey, &
+            sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
             sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_step, &
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
@@ -67,6 +68,13 @@ module sem2d_hip
       integer(c_int), value :: n
       integer(c_int), intent(in) :: node(*)
       real(c_double), intent(in) :: C(*)
+    end function
+    integer(c_int) function sem2d_set_abso_stacey(h, nbe, enode, K) bind(C, name='sem2d_set_abso_stacey')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: nbe
+      integer(c_int), intent(in) :: enode(*)
+      real(c_double), intent(in) :: K(*)
     end function
 
     integer(c_int) function sem2d_set_force(h, iglob, dir, isrc) bind(C, name='sem2d_set_force')

@@ -88,6 +88,10 @@ class Solver:
             node = np.ascontiguousarray(ab["node"], dtype=np.int32)
             Cm = np.ascontiguousarray(ab["C"].T, dtype=np.float64)  # [ndof][n]
             self._check(self.api.set_abso(self.h, len(node), _p(node, np.int32), _p(Cm)), "set_abso")
+            if ab.get("stacey") is not None:
+                enode, K = ab["stacey"]
+                self._check(self.api.set_abso_stacey(self.h, len(enode), _p(enode, np.int32), _p(K)),
+                            "set_abso_stacey")
         for s in pb.sources:
             isrc = C.c_int()
             if s["kind"] == "force":

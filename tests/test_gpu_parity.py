@@ -47,7 +47,8 @@ def test_fint(name, oracle_api):
 
 @pytest.mark.parametrize("name,nsteps", [("small_psv", 300), ("c1_psv", 600), ("sh_swf", 600), ("multi_src", 300), ("interp_rec", 300),
                                          ("sh_kv", 600), ("psv_kv", 500),
-                                         ("psv_rsf", 500), ("hete_psv", 400), ("rsf_ens", 400)])
+                                         ("psv_rsf", 500), ("hete_psv", 400), ("rsf_ens", 400),
+                                         ("psv_stacey", 400)])
 def test_time_series(name, nsteps, oracle_api, oracle_ulp_api):
     print(f"\n[case {name} nsteps {nsteps}]", flush=True)
     pb, ref, gpu = _pair(name, nsteps, oracle_api)
@@ -91,6 +92,27 @@ def test_time_series(name, nsteps, oracle_api, oracle_ulp_api):
     assert not bad, bad
     if pb.fault is None or pb.fault["law"] != "RSF":
         assert all(v == TOL for v in tols.values())  # the widening is for friction only
+
+
+def test_stacey_general_path_and_fp32(oracle_api):
+    """Stacey boundaries (BC_ABSORB stacey=T) through the general path (colour-sorted
+    element kernel, k_abso then k_stacey per boundary) at the parity bar, and with FP32
+    fields (structured path) against the FP64 oracle."""
+    nt = 400
+    pb = builder.build(case_par("psv_stacey"), nsteps=nt)
+    assert all(ab["stacey"] is not None for ab in pb.abso)
+    ref = solver.Solver(pb, api=oracle_api, compact=False)
+    gen = solver.Solver(pb, structured=False)
+    g32 = solver.Solver(pb, bulk_fp32=True)
+    for s in (ref, gen, g32):
+        s.step(nt)
+    e_gen = rel(gen.seismograms(), ref.seismograms())
+    e_32 = rel(g32.seismograms(), ref.seismograms())
+    print(f"\n  Stacey general path rel-L2 {e_gen:.2e}, FP32 structured {e_32:.2e}", flush=True)
+    assert e_gen < TOL
+    for x, y in zip(gen.fields(), ref.fields()):
+        assert rel(x, y) < TOL
+    assert e_32 < 1e-3
 
 
 def test_deterministic(oracle_api):
