@@ -415,6 +415,130 @@ def _numbering(mesh, perm, xgll):
                 G=G, iglob_of_grid=iglob, NX1=NX1, NZ1=NZ1)
 
 
+def _read_mesh2d(g, fname_dir):
+    """MESH2D_read (SRC/mesh_mesh2d.f90:45-111): a header line before each group, then
+    "nel npel nnod nbc", nnod lines "id x z", nel lines "id n1 .. n_npel tag", and per
+    boundary "tag nbel" and nbel lines "id element edge"."""
+    if g is None:
+        raise InputError("MESH2D_read: MESH_MESH2D input block not found")
+    fname = str(g.get("file", "")).strip()
+    if not fname:
+        raise InputError("MESH2D_read: you must provide a mesh2d file name")
+    path = fname if os.path.isabs(fname) else os.path.join(fname_dir or ".", fname)
+    with open(path) as fh:
+        lines = [ln for ln in fh.read().splitlines() if ln.strip()]
+    vals = lambda ln: ln.replace("d", "e").replace("D", "e").split()  # noqa: E731
+    pos = 1
+    nel, npel, nnod, nb = (int(v) for v in vals(lines[pos])[:4])
+    if npel != 4:
+        raise InputError("MESH2D: only 4-node quadrilaterals are supported by this build")
+    pos += 2
+    coorg = np.array([[float(v) for v in vals(lines[pos + i])[1:3]] for i in range(nnod)])
+    pos += nnod + 1
+    knods = np.empty((nel, 4), dtype=np.int64)
+    tag = np.empty(nel, dtype=np.int64)
+    for e in range(nel):
+        v = [int(float(x)) for x in vals(lines[pos + e])]
+        knods[e] = v[1:5]
+        tag[e] = v[5]
+    pos += nel
+    bounds = {}
+    for _ in range(nb):
+        pos += 1
+        btag, nbel = (int(v) for v in vals(lines[pos])[:2])
+        pos += 2
+        rows = [[int(x) for x in vals(lines[pos + k])[:3]] for k in range(nbel)]
+        pos += nbel
+        bounds[btag] = ([r[1] for r in rows], [r[2] for r in rows])
+    return dict(kind="mesh2d", nelem=nel, coorg=coorg, knods=knods - 1, tag=tag, bounds=bounds, nx=0, nz=0,
+                ezflt=-1, fztag=0, fznz=0)
+
+
+def _numbering_fem(mesh, xgll):
+    """SE_init_numbering and SE_init_coord (SRC/spec_grid.f90:198-349) for an unstructured
+    4-node mesh in element order: interior points first, then each edge not yet numbered
+    (shared with the element across it, traversed in reverse there), then each vertex not
+    yet numbered (shared with every element holding that mesh node).  Coordinates are
+    written by every element touching a point, the last one in element order standing."""
+    knods = mesh["knods"]
+    nelem = len(knods)
+    ngll = 5
+    ivtx, jvtx = (0, ngll - 1, ngll - 1, 0), (0, 0, ngll - 1, ngll - 1)
+    iedg = [_edge_ij(n) for n in (EDGE_D, EDGE_R, EDGE_U, EDGE_L)]  # (i[k], j[k]) counterclockwise
+    # FE_SetConnectivity (SRC/fem_grid.f90:103-204): edge n joins vertices n and n+1
+    edge_of = {}
+    for e in range(nelem):
+        for n in range(4):
+            a, b = int(knods[e, n]), int(knods[e, (n + 1) % 4])
+            edge_of.setdefault((min(a, b), max(a, b)), []).append((e, n))
+    vconn = {}
+    for e in range(nelem):
+        for n in range(4):
+            vconn.setdefault(int(knods[e, n]), []).append((e, n))
+    ibool = np.zeros((nelem, ngll, ngll), dtype=np.int64)  # [e, j, i]
+    npoin = 0
+    for e in range(nelem):
+        for j in range(1, ngll - 1):
+            for i in range(1, ngll - 1):
+                npoin += 1
+                ibool[e, j, i] = npoin
+        for n in range(4):
+            ii, jj = iedg[n]
+            if ibool[e, jj[1], ii[1]] > 0:
+                continue
+            a, b = int(knods[e, n]), int(knods[e, (n + 1) % 4])
+            other = [x for x in edge_of[(min(a, b), max(a, b))] if x[0] != e]
+            ee, nn = other[0] if other else (-1, -1)
+            for k in range(1, ngll - 1):
+                npoin += 1
+                ibool[e, jj[k], ii[k]] = npoin
+                if ee >= 0:
+                    iR, jR = iedg[nn]
+                    ibool[ee, jR[ngll - 1 - k], iR[ngll - 1 - k]] = npoin
+        for n in range(4):
+            if ibool[e, jvtx[n], ivtx[n]] > 0:
+                continue
+            npoin += 1
+            for ee, nn in vconn[int(knods[e, n])]:
+                ibool[ee, jvtx[nn], ivtx[nn]] = npoin
+    cx = mesh["coorg"][knods, 0]
+    cz = mesh["coorg"][knods, 1]
+    shp = _shape_q4(xgll)
+    coord = np.empty((npoin, 2))
+    for e in range(nelem):
+        for j in range(ngll):
+            for i in range(ngll):
+                sv = shp[j, i]
+                g = ibool[e, j, i] - 1
+                coord[g, 0] = ((cx[e, 0] * sv[0] + cx[e, 1] * sv[1]) + cx[e, 2] * sv[2]) + cx[e, 3] * sv[3]
+                coord[g, 1] = ((cz[e, 0] * sv[0] + cz[e, 1] * sv[1]) + cz[e, 2] * sv[2]) + cz[e, 3] * sv[3]
+    return dict(ibool=ibool.astype(np.int32), coord=coord, npoin=npoin, cx=cx, cz=cz,
+                ix=np.zeros(nelem, dtype=np.int64), iz=np.zeros(nelem, dtype=np.int64))
+
+
+def _elastic_coefs_general(DxiDx, DxiDz, DetaDx, DetaDz, w2J, lam, mu, ndof):
+    """General-geometry coefficients a(:,:,k) of SRC/mat_elastic.f90:327-358 (nelast 3 and
+    10, isotropic), arrays (n, 5, 5) -> (n, nelast, 5, 5)."""
+    if ndof == 1:
+        a = np.stack([mu * DxiDx * DxiDx + mu * DxiDz * DxiDz,
+                      mu * DetaDx * DetaDx + mu * DetaDz * DetaDz,
+                      mu * DxiDx * DetaDx + mu * DxiDz * DetaDz], axis=1)
+    else:
+        Kx = lam + 2.0 * mu
+        Kz = Kx
+        a = np.stack([Kx * DxiDx * DxiDx + mu * DxiDz * DxiDz,
+                      lam * DxiDx * DetaDz + mu * DxiDz * DetaDx,
+                      Kz * DetaDz * DetaDz + mu * DetaDx * DetaDx,
+                      Kx * DetaDx * DetaDx + mu * DetaDz * DetaDz,
+                      lam * DxiDz * DetaDx + mu * DxiDx * DetaDz,
+                      Kz * DxiDz * DxiDz + mu * DxiDx * DxiDx,
+                      Kx * DxiDx * DetaDx + mu * DxiDz * DetaDz,
+                      (lam + mu) * DxiDx * DxiDz,
+                      (lam + mu) * DetaDx * DetaDz,
+                      Kz * DxiDz * DetaDz + mu * DxiDx * DetaDx], axis=1)
+    return (-w2J)[:, None] * a
+
+
 def _shape_q4(xgll):
     """Q4 shape functions at the GLL points (SRC/elem_q4.f90:37-53), [j, i, k]."""
     q = 0.25
@@ -603,7 +727,10 @@ def _elastic_coefs_batch(DxiDx, DetaDz, w2J, lam, mu, ndof):
 # boundaries
 # ---------------------------------------------------------------------------
 def _bound_elements(mesh):
-    """Boundary element lists (old element indices, 1-based) of SRC/mesh_structured.f90:101-196."""
+    """Boundary element lists (old element indices, 1-based) of SRC/mesh_structured.f90:101-196,
+    or, for a MESH2D mesh, the file's boundaries (element and edge of each boundary element)."""
+    if "bounds" in mesh:
+        return mesh["bounds"]
     nx, nz, ezflt = mesh["nx"], mesh["nz"], mesh["ezflt"]
     b = {}
     b[SIDE_D] = ([i + 1 for i in range(nx)], EDGE_D)
@@ -635,13 +762,14 @@ def _bound_topo(tag, mesh, num, perm_inv):
     """BC_set_bulk_node (SRC/spec_grid.f90:676-822): node list sorted by coordinate."""
     elems_old, edge = _bound_elements(mesh)[tag]
     elem = perm_inv[np.array(elems_old) - 1]  # 0-based new element index
+    edges = np.broadcast_to(np.asarray(edge, dtype=np.int64), (len(elem),)).copy()  # one per element
     ibool = num["ibool"]
     coord = num["coord"]
-    ii, jj = _edge_ij(edge)
     nodes = []
     bibool = np.empty((len(elem), 5), dtype=np.int64)
     index = {}
     for n, e in enumerate(elem):
+        ii, jj = _edge_ij(edges[n])
         for kloc in range(5):
             g = int(ibool[e, jj[kloc], ii[kloc]])
             if g not in index:
@@ -659,7 +787,7 @@ def _bound_topo(tag, mesh, num, perm_inv):
     t = _Bound()
     t.tag = tag
     t.elem = elem
-    t.edge = edge
+    t.edges = edges
     t.node = nodes[isort]              # 1-based global nodes
     t.ibool = back[bibool]             # (nelem_b, 5) -> index into t.node
     return t
@@ -671,10 +799,11 @@ def _normal_and_weights(t, jac, wgll):
     npb = len(t.node)
     NORM = np.zeros((npb, 2))
     W = np.zeros(npb)
-    ii, jj = _edge_ij(t.edge)
-    dim_t = 0 if t.edge in (EDGE_D, EDGE_U) else 1
-    sign = -1.0 if t.edge in (EDGE_U, EDGE_L) else 1.0
     for n, e in enumerate(t.elem):
+        edge = t.edges[n]
+        ii, jj = _edge_ij(edge)
+        dim_t = 0 if edge in (EDGE_D, EDGE_U) else 1
+        sign = -1.0 if edge in (EDGE_U, EDGE_L) else 1.0
         for k in range(5):
             i, j = ii[k], jj[k]
             if dim_t == 0:
@@ -706,14 +835,14 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
         geo_tan, geo_nor = 1, 0
     else:
         raise InputError("BC_ABSO_init: only flat boundaries are supported by this build")
-    dim_t = 0 if t.edge in (EDGE_D, EDGE_U) else 1
-    ii, jj = _edge_ij(t.edge)
     C = np.zeros((len(t.node), ndof))
     rho_p, cp_p, cs_p = props
     stacey = stacey and ndof == 2  # SRC/bc_abso.f90:176 (flat boundaries only, checked above)
     K = np.zeros((len(t.elem), 5, 2)) if stacey else None
     cb, ce, cf = [], [], []  # contributions: boundary node, element, weight w_k |J1D|
     for n, e in enumerate(t.elem):
+        dim_t = 0 if t.edges[n] in (EDGE_D, EDGE_U) else 1
+        ii, jj = _edge_ij(t.edges[n])
         for k in range(5):
             i, j = ii[k], jj[k]
             rho = rho_p[e, j, i]
@@ -1249,11 +1378,18 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     if "w" in gen.values:
         raise InputError("2.5D (W) is not supported by this build")
     md = par.first("MESH_DEF")
-    if md is None or str(md.get("method", "")).upper() != "CARTESIAN":
-        raise InputError("this build supports MESH_DEF method='CARTESIAN' only")
-    mesh = _cart_mesh(par.first("MESH_CART"))
-    for d in par.all("MESH_CART_DOMAIN"):
-        raise InputError("MESH_CART_DOMAIN is not supported by this build")
+    method = str(md.get("method", "")).upper() if md is not None else ""
+    if method == "CARTESIAN":
+        mesh = _cart_mesh(par.first("MESH_CART"))
+        for d in par.all("MESH_CART_DOMAIN"):
+            raise InputError("MESH_CART_DOMAIN is not supported by this build")
+    elif method == "MESH2D":
+        mesh = _read_mesh2d(par.first("MESH_MESH2D"), fname_dir)
+    else:
+        raise InputError("this build supports MESH_DEF method='CARTESIAN' or 'MESH2D'")
+    # a MESH_CART box is a flat grid (SRC/mesh_cartesian.f90:242); MESH2D meshes are not
+    # (fem_grid_type default), which selects the general element coefficients
+    flat = mesh.get("kind") != "mesh2d"
 
     pb = Problem()
     pb.ngll, pb.ndof = ngll, ndof
@@ -1261,20 +1397,28 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     xgll, wgll, H = gll5()
     pb.xgll, pb.wgll, pb.hprime = xgll, wgll, H
     nx, nz = mesh["nx"], mesh["nz"]
-    pb.nelem = nelem = nx * nz
-    perm = _rcm_perm(nx, nz)
-    perm_inv = np.empty(nelem, dtype=np.int64)
-    perm_inv[perm - 1] = np.arange(nelem)
+    if flat:
+        pb.nelem = nelem = nx * nz
+        perm = _rcm_perm(nx, nz)
+        perm_inv = np.empty(nelem, dtype=np.int64)
+        perm_inv[perm - 1] = np.arange(nelem)
+        num = _numbering(mesh, perm, xgll)
+    else:  # MESH2D: the file's element order (no renumbering)
+        if elem_props is not None:
+            raise InputError("per-element model arrays need a MESH_CART grid")
+        pb.nelem = nelem = mesh["nelem"]
+        perm = np.arange(1, nelem + 1, dtype=np.int32)
+        perm_inv = np.arange(nelem, dtype=np.int64)
+        num = _numbering_fem(mesh, xgll)
     pb.perm = perm
-    num = _numbering(mesh, perm, xgll)
     pb.ibool = num["ibool"]
     pb.coord = num["coord"]
     pb.npoin = npoin = num["npoin"]
     pb.elem_ix, pb.elem_iz = num["ix"], num["iz"]
 
-    # element tags (single domain tag 1, optional fault zone tag)
-    tags = np.ones(nelem, dtype=np.int64)
-    if mesh["fztag"] > 0:
+    # element tags (MESH_CART: single domain tag 1, optional fault zone tag)
+    tags = np.ones(nelem, dtype=np.int64) if flat else mesh["tag"].copy()
+    if flat and mesh["fztag"] > 0:
         j1 = max(mesh["ezflt"] + 1 - mesh["fznz"], 1)
         j2 = min(mesh["ezflt"] + mesh["fznz"], nz)
         rows = num["iz"] + 1
@@ -1381,10 +1525,20 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     # MAT_init_work: flat grid, elastic.  Homogeneous materials share the
     # array of the first element of their tag (SRC/mat_gen.f90:357-365); other
     # elements get their own array (SRC/mat_elastic.f90:242-360).
-    pb.nelast = 2 if ndof == 1 else 6
+    pb.nelast = (2 if ndof == 1 else 6) if flat else (3 if ndof == 1 else 10)
     utags = np.unique(tags)
     first_of = {int(t): int(np.nonzero(tags == t)[0][0]) for t in utags}
-    if len(utags) == 1 and pb.homogeneous:
+    if not flat:
+        # every element its own general-geometry array (SRC/mat_gen.f90:360-364 shares
+        # arrays only on flat grids)
+        pb.coef_mode = "per_element"
+        coefs = np.empty((nelem, pb.nelast, 5, 5))
+        for c0 in range(0, nelem, 65536):
+            b = slice(c0, min(c0 + 65536, nelem))
+            coefs[b] = _elastic_coefs_general(DxiDx[b], DxiDz[b], DetaDx[b], DetaDz[b], dvol[b], lam_p[b],
+                                              mu_p[b], ndof)
+        pb.coefs = coefs
+    elif len(utags) == 1 and pb.homogeneous:
         e1 = 0
         pb.coefs = _elastic_coefs(DxiDx[e1], DetaDz[e1], dvol[e1], lam_p[e1], mu_p[e1], ndof)
         pb.coef_mode = "shared"
@@ -1408,7 +1562,8 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     # compact form for heterogeneous flat grids: lambda, mu per GLL node and the three
     # geometric factors of each element, -detJ*DxiDx^2, -detJ*DetaDz^2, -detJ*DxiDx*DetaDz
     # (taken at the element's first GLL node); a(:,:,k) = wgll2 * material * factor.
-    if not (len(utags) == 1 and pb.homogeneous):
+    pb.geom = pb.lam_mu = None
+    if flat and not (len(utags) == 1 and pb.homogeneous):
         detJ = J11[:, 0, 0] * J22[:, 0, 0] - J12[:, 0, 0] * J21[:, 0, 0]
         gx, gz = DxiDx[:, 0, 0], DetaDz[:, 0, 0]
         pb.geom = np.stack([-detJ * gx * gx, -detJ * gz * gz, -detJ * gx * gz], axis=1)
@@ -1443,6 +1598,8 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
             t = _bound_topo(t1, mesh, num, perm_inv)
             pb.abso.append(_abso_init(t, jac, wgll, (rho_p, cp_p, cs_p), M, coefA2Vrhs, ndof, stacey))
         elif kind == "DYNFLT":
+            if not flat:
+                raise InputError("DYNFLT on a MESH2D mesh is not supported by this build")
             if pb.fault is not None:
                 raise InputError("more than one DYNFLT boundary is not supported by this build")
             pb.fault = _dynflt_init(par, bd, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma, pb.nt, ndof)

@@ -58,7 +58,7 @@ class Solver:
         d.dt, d.beta, d.gamma = pb.dt, pb.beta, pb.gamma
         d.bulk_fp32 = 1 if bulk_fp32 else 0
         cells = None
-        if structured and getattr(pb, "mesh", None) is not None:
+        if structured and getattr(pb, "mesh", None) is not None and pb.mesh.get("nx", 0) > 0:
             m = pb.mesh
             cells = np.empty(m["nx"] * m["nz"], dtype=np.int32)
             cells[pb.elem_iz * m["nx"] + pb.elem_ix] = np.arange(1, pb.nelem + 1, dtype=np.int32)

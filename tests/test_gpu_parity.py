@@ -35,7 +35,7 @@ def _pair(name, nsteps, oracle_api, **kw):
     return pb, ref, gpu
 
 
-@pytest.mark.parametrize("name", ["small_psv", "sh_swf", "psv_rsf", "c1_psv"])
+@pytest.mark.parametrize("name", ["small_psv", "sh_swf", "psv_rsf", "c1_psv", "mesh2d_psv"])
 def test_fint(name, oracle_api):
     pb, ref, gpu = _pair(name, 1, oracle_api)
     rng = np.random.default_rng(7)
@@ -48,7 +48,7 @@ def test_fint(name, oracle_api):
 @pytest.mark.parametrize("name,nsteps", [("small_psv", 300), ("c1_psv", 600), ("sh_swf", 600), ("multi_src", 300), ("interp_rec", 300),
                                          ("sh_kv", 600), ("psv_kv", 500),
                                          ("psv_rsf", 500), ("hete_psv", 400), ("rsf_ens", 400),
-                                         ("psv_stacey", 400)])
+                                         ("psv_stacey", 400), ("mesh2d_psv", 500)])
 def test_time_series(name, nsteps, oracle_api, oracle_ulp_api):
     print(f"\n[case {name} nsteps {nsteps}]", flush=True)
     pb, ref, gpu = _pair(name, nsteps, oracle_api)
@@ -113,6 +113,21 @@ def test_stacey_general_path_and_fp32(oracle_api):
     for x, y in zip(gen.fields(), ref.fields()):
         assert rel(x, y) < TOL
     assert e_32 < 1e-3
+
+
+def test_mesh2d_fp32(oracle_api):
+    """Unstructured MESH2D mesh (general-geometry elements, 10 coefficients, general path)
+    with FP32 fields against the FP64 oracle."""
+    nt = 500
+    pb = builder.build(case_par("mesh2d_psv"), nsteps=nt)
+    assert pb.nelast == 10
+    ref = solver.Solver(pb, api=oracle_api, compact=False)
+    g32 = solver.Solver(pb, bulk_fp32=True)
+    ref.step(nt)
+    g32.step(nt)
+    err = rel(g32.seismograms(), ref.seismograms())
+    print(f"\n  MESH2D FP32 seismogram rel-L2 error vs FP64 oracle: {err:.3e}", flush=True)
+    assert err < 1e-3
 
 
 def test_deterministic(oracle_api):
