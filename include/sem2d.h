@@ -102,6 +102,13 @@ int sem2d_set_abso(sem2d_solver *h, int n, const int *node, const double *C);
    displacement, after the boundary's -C v term. */
 int sem2d_set_abso_stacey(sem2d_solver *h, int nbe, const int *enode, const double *K);
 
+/* The absorbing boundary registered last is not flat (its normals are not all along x
+   or z; P-SV): replaces the non-flat branch of BC_ABSO_apply (SRC/bc_abso.f90:313-318).
+   nrm is [2][n], the unit normal at each of its n nodes; its C is then [rho cp w|J|,
+   rho cs w|J|] and the force loses C[0] (v.n) n + C[1] (v - (v.n) n).  Only for solvers
+   on the general path (not a MESH_CART grid); n must be that boundary's node count. */
+int sem2d_set_abso_normals(sem2d_solver *h, int n, const double *nrm);
+
 /* Point force at a node: replaces FORCE_add (SRC/src_force.f90:77-90).  dir[2] is used
    when ndof = 2.  Returns the source index through *isrc. */
 int sem2d_set_force(sem2d_solver *h, int iglob, const double *dir, int *isrc);

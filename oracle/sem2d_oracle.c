@@ -15,7 +15,7 @@
  *     kd2_psv, kd2_sh    ELAST_KD2_PSV/SH, My_MATMUL  SRC/mat_elastic.f90:589-641,739-799
  *   sources              SO_add, FORCE_add, SRC_MOMENT_add  SRC/src_gen.f90:290-317,
  *                        SRC/src_force.f90:77-90, SRC/src_moment.f90:217-235
- *   abso_apply           BC_ABSO_apply (flat; Stacey P-SV)  SRC/bc_abso.f90:286-336
+ *   abso_apply           BC_ABSO_apply (flat and not flat; Stacey P-SV)  SRC/bc_abso.f90:286-336
  *   fault_apply          BC_DYNFLT_apply      SRC/bc_dynflt.f90:569-739
  *     swf                swf_mu, swf_update_state  SRC/bc_dynflt_swf.f90:142-194
  *     rsf                rsf_solver, rsf_update_theta, rsf_update_V, nr_solver,
@@ -60,6 +60,7 @@ typedef struct {
   int n;
   int *node;    /* 0-based */
   double *C;    /* [ndof][n] */
+  double *nrm;  /* [2][n] node normals of a boundary that is not flat, NULL if flat */
   int nbe;      /* Stacey: boundary elements (0: none) */
   int *enode;   /* [nbe][NG] boundary-local node of each edge point */
   double *K;    /* [nbe][NG][2] */
@@ -156,6 +157,15 @@ int oracle_set_abso(oracle_solver *h, int n, const int *node, const double *C) {
   b->node = (int *)malloc(sizeof(int) * n);
   for (int k = 0; k < n; ++k) b->node[k] = node[k] - 1;
   b->C = dup_d(C, (size_t)h->ndof * n);
+  return 0;
+}
+
+int oracle_set_abso_normals(oracle_solver *h, int n, const double *nrm) {
+  if (h->nabso < 1) return fail(h, 3, "set_abso_normals: no absorbing boundary");
+  if (h->ndof != 2) return fail(h, 3, "set_abso_normals: P-SV only");
+  abso_t *b = &h->abso[h->nabso - 1];
+  if (n != b->n || !nrm) return fail(h, 3, "set_abso_normals: size differs from the boundary's");
+  b->nrm = dup_d(nrm, (size_t)2 * n);
   return 0;
 }
 
@@ -705,11 +715,23 @@ static void one_step(oracle_solver *h) {
   }
   for (int b = 0; b < h->nabso; ++b) {
     abso_t *B = &h->abso[b];
-    for (int c = 0; c < h->ndof; ++c)
+    if (B->nrm) {
+      /* not flat (SRC/bc_abso.f90:313-318): Vn = (V.n) n, MxA -= C1 Vn + C2 (V - Vn) */
       for (int k = 0; k < B->n; ++k) {
         int g = B->node[k];
-        f[c * N + g] = f[c * N + g] - B->C[c * B->n + k] * v[c * N + g];
+        double n1 = B->nrm[k], n2 = B->nrm[B->n + k];
+        double dot = v[g] * n1 + v[N + g] * n2;
+        double vn1 = dot * n1, vn2 = dot * n2;
+        f[g] = f[g] - B->C[k] * vn1 - B->C[B->n + k] * (v[g] - vn1);
+        f[N + g] = f[N + g] - B->C[k] * vn2 - B->C[B->n + k] * (v[N + g] - vn2);
       }
+    } else {
+      for (int c = 0; c < h->ndof; ++c)
+        for (int k = 0; k < B->n; ++k) {
+          int g = B->node[k];
+          f[c * N + g] = f[c * N + g] - B->C[c * B->n + k] * v[c * N + g];
+        }
+    }
     if (B->nbe) {
       /* Stacey (SRC/bc_abso.f90:321-333): KxD(k(:),:) += K(:,:,e) * matmul(Ht, D(nodes(k),:))
          over the boundary elements in order, then MxA(nodes,:) -= KxD; D is the predicted
@@ -806,6 +828,7 @@ void oracle_destroy(oracle_solver *h) {
   free(h->ibool); free(h->coefs); free(h->rmass); free(h->d); free(h->v); free(h->a);
   for (int b = 0; b < h->nabso; ++b) {
     free(h->abso[b].node); free(h->abso[b].C); free(h->abso[b].enode); free(h->abso[b].K);
+    free(h->abso[b].nrm);
   }
   for (int s = 0; s < h->nsrc; ++s) {
     free(h->src[s].ixi); free(h->src[s].ieta); free(h->src[s].cxi); free(h->src[s].ceta);

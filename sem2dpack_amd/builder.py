@@ -824,20 +824,27 @@ def _normal_and_weights(t, jac, wgll):
 
 
 def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
-    """BC_ABSO_init (SRC/bc_abso.f90:115-266), Clayton-Engquist on flat boundaries, with the
-    Stacey term for P-SV when ``stacey`` (SRC/bc_abso.f90:176-180,209-224):
+    """BC_ABSO_init (SRC/bc_abso.f90:115-266), Clayton-Engquist; on a boundary that is not
+    flat (normals not all along x or z) the P-SV coefficients are rho cp w|J| along the
+    normal and rho cs w|J| along the boundary, applied with the node normals, and the mass
+    is not modified.  Stacey term for flat P-SV boundaries when ``stacey``
+    (SRC/bc_abso.f90:176-180,209-224):
     K[e, k, geo_tan] = -K[e, k, geo_nor] = w_k |J1D| * DLocDGlob(tan, tan) * rho cs (2 cs - cp)."""
     J11, J12, J21, J22 = jac
     NORM, _ = _normal_and_weights(t, jac, wgll)
+    is_flat = True
     if np.all(np.abs(NORM[:, 0]) < TINY_XABS):
         geo_tan, geo_nor = 0, 1
     elif np.all(np.abs(NORM[:, 1]) < TINY_XABS):
         geo_tan, geo_nor = 1, 0
     else:
-        raise InputError("BC_ABSO_init: only flat boundaries are supported by this build")
+        # not flat (SRC/bc_abso.f90:163-166): cp along the local normal, cs along the
+        # boundary, no implicit treatment in the mass, normals kept for BC_ABSO_apply
+        is_flat = False
+        geo_tan, geo_nor = 1, 0
     C = np.zeros((len(t.node), ndof))
     rho_p, cp_p, cs_p = props
-    stacey = stacey and ndof == 2  # SRC/bc_abso.f90:176 (flat boundaries only, checked above)
+    stacey = stacey and ndof == 2 and is_flat  # SRC/bc_abso.f90:176
     K = np.zeros((len(t.elem), 5, 2)) if stacey else None
     cb, ce, cf = [], [], []  # contributions: boundary node, element, weight w_k |J1D|
     for n, e in enumerate(t.elem):
@@ -871,12 +878,14 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
         K[:, :, 1] = K[:, :, 0]
         K[:, :, geo_tan] = -K[:, :, geo_tan]
     nodes0 = t.node - 1
-    M[:, nodes0] = M[:, nodes0] + coefA2Vrhs * C.T
+    if is_flat:  # SRC/bc_abso.f90:243
+        M[:, nodes0] = M[:, nodes0] + coefA2Vrhs * C.T
     # C[node, c] = sum over contributions of rho * c_c * weight, c_c = cp along the
     # normal (component geo_nor), cs along the boundary
     return dict(tag=t.tag, node=t.node.astype(np.int32), C=C, geo_nor=geo_nor,
                 contrib=(np.array(cb, dtype=np.int64), np.array(ce, dtype=np.int64), np.array(cf)),
-                stacey=(np.ascontiguousarray(t.ibool[:, :5], dtype=np.int32), K) if stacey else None)
+                stacey=(np.ascontiguousarray(t.ibool[:, :5], dtype=np.int32), K) if stacey else None,
+                normals=None if (is_flat or ndof == 1) else NORM)
 
 
 # ---------------------------------------------------------------------------

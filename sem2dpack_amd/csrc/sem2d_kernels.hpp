@@ -218,6 +218,23 @@ __global__ void k_abso(int n, int ndof, const int* __restrict__ node, const doub
   }
 }
 
+// Absorbing boundary that is not flat, P-SV (SRC/bc_abso.f90:313-318): with the node normal
+// n, Vn = (v.n) n and the force loses C0 Vn + C1 (v - Vn), C = [rho cp, rho cs] w|J|.
+template <typename R>
+__global__ void k_abso_normal(int n, const int* __restrict__ node, const double* __restrict__ C,
+                              const double* __restrict__ nrm, const R* __restrict__ v, R* __restrict__ f,
+                              int npoin) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int g = node[k];
+  const double n1 = nrm[k], n2 = nrm[n + k];
+  const double v1 = (double)v[g], v2 = (double)v[npoin + g];
+  const double dot = v1 * n1 + v2 * n2;
+  const double vn1 = dot * n1, vn2 = dot * n2;
+  f[g] = (R)((double)f[g] - C[k] * vn1 - C[n + k] * (v1 - vn1));
+  f[npoin + g] = (R)((double)f[npoin + g] - C[k] * vn2 - C[n + k] * (v2 - vn2));
+}
+
 // Stacey term of one flat P-SV absorbing boundary (SRC/bc_abso.f90:321-333), general path:
 // node q of the boundary (global node node[q]) subtracts KxD = sum over its boundary
 // elements (t = off[q] .. off[q+1]-1, element order; idx[t] = e * NG + k) of

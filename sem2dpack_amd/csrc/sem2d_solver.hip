@@ -43,6 +43,7 @@ struct AbsoDev {
   int nbe = 0;
   int *enode = nullptr, *soff = nullptr, *sidx = nullptr;
   double* K = nullptr;
+  double* nrm = nullptr;  // [2][n] node normals of a boundary that is not flat (general path)
 };
 
 struct AbsoHost {
@@ -947,7 +948,10 @@ int do_steps(sem2d_solver* h, int nsteps) {
       k_sources<R><<<1, 64, 0, h->stream>>>((int)h->src_host.size(), h->src_dev, h->ampl_dev, h->ampl_nt, it, a,
                                            h->npoin);
     for (auto& b : h->abso) {
-      k_abso<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, h->ndof, b.node, b.C, v, a, h->npoin);
+      if (b.nrm)
+        k_abso_normal<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, b.node, b.C, b.nrm, v, a, h->npoin);
+      else
+        k_abso<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, h->ndof, b.node, b.C, v, a, h->npoin);
       if (b.nbe)
         k_stacey<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, b.node, b.soff, b.sidx, b.enode, b.K, d, a,
                                                                 h->npoin);
@@ -1114,6 +1118,17 @@ int sem2d_set_abso(sem2d_solver* h, int n, const int* node, const double* C) {
   h->abso_host.push_back(ah);
   h->st.ready = false;
   return 0;
+}
+
+int sem2d_set_abso_normals(sem2d_solver* h, int n, const double* nrm) {
+  if (!h) return 1;
+  if (h->abso.empty()) return set_err(h, 3, "set_abso_normals: call sem2d_set_abso first");
+  if (h->ndof != 2) return set_err(h, 3, "set_abso_normals: P-SV only");
+  if (h->st.on)
+    return set_err(h, 3, "set_abso_normals: a boundary that is not flat needs a solver on the general path");
+  AbsoDev& ad = h->abso.back();
+  if (!nrm || n != ad.n) return set_err(h, 3, "set_abso_normals: size differs from the boundary's");
+  return upload(h, &ad.nrm, nrm, (size_t)2 * n);
 }
 
 int sem2d_set_abso_stacey(sem2d_solver* h, int nbe, const int* enode, const double* K) {

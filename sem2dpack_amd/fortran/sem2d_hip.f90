@@ -35,6 +35,7 @@ module sem2d_hip
   end type sem2d_fault_desc
 
   public :: sem2d_create, sem2d_set_kelvin_voigt, sem2d_set_abso, sem2d_set_abso_stacey, &
+            sem2d_set_abso_normals, &
             sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
             sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_step, &
             sem2d_fetch_fields, &
@@ -75,6 +76,12 @@ module sem2d_hip
       integer(c_int), value :: nbe
       integer(c_int), intent(in) :: enode(*)
       real(c_double), intent(in) :: K(*)
+    end function
+    integer(c_int) function sem2d_set_abso_normals(h, n, nrm) bind(C, name='sem2d_set_abso_normals')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: nrm(*)
     end function
 
     integer(c_int) function sem2d_set_force(h, iglob, dir, isrc) bind(C, name='sem2d_set_force')

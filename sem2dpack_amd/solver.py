@@ -88,6 +88,9 @@ class Solver:
             node = np.ascontiguousarray(ab["node"], dtype=np.int32)
             Cm = np.ascontiguousarray(ab["C"].T, dtype=np.float64)  # [ndof][n]
             self._check(self.api.set_abso(self.h, len(node), _p(node, np.int32), _p(Cm)), "set_abso")
+            if ab.get("normals") is not None:
+                nrm = np.ascontiguousarray(ab["normals"].T, dtype=np.float64)  # [2][n]
+                self._check(self.api.set_abso_normals(self.h, len(node), _p(nrm)), "set_abso_normals")
             if ab.get("stacey") is not None:
                 enode, K = ab["stacey"]
                 self._check(self.api.set_abso_stacey(self.h, len(enode), _p(enode, np.int32), _p(K)),

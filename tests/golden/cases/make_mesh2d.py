@@ -1,9 +1,12 @@
-"""Write box.mesh2d: a 6 km x 4 km box of 30 x 20 four-node quadrilaterals in the MESH2D
-format (SRC/mesh_mesh2d.f90:17-36), with the interior mesh nodes moved off the lines
-of a regular grid (smooth sine displacements, so that no element is a rectangle),
+"""Write the MESH2D meshes of the golden cases mesh2d_psv/box.mesh2d and
+mesh2d_curved/curved.mesh2d: a 6 km x 4 km domain of 30 x 20 four-node quadrilaterals in
+the MESH2D format (SRC/mesh_mesh2d.f90:17-36), with the interior mesh nodes moved off the
+lines of a regular grid (smooth sine displacements, so that no element is a rectangle),
 elements and nodes listed in a shuffled order, two material tags (1 below z = 1.6 km
 before the distortion, 2 above) and four boundaries: 1 bottom, 2 right, 3 top, 4 left
-(edges 1..4 of their elements, counterclockwise from the bottom)."""
+(edges 1..4 of their elements, counterclockwise from the bottom).  In the curved mesh the
+right side bulges out by up to 400 m (x += 400 sin(pi z / 4 km) x / 6 km at every node),
+so that boundary 2 is not flat."""
 import os
 
 import numpy as np
@@ -11,7 +14,7 @@ import numpy as np
 NX, NZ, LX, LZ = 30, 20, 6000.0, 4000.0
 
 
-def main(path):
+def main(path, bulge=0.0):
     rng = np.random.default_rng(20261202)
     xs, zs = np.linspace(0.0, LX, NX + 1), np.linspace(0.0, LZ, NZ + 1)
     X, Z = np.meshgrid(xs, zs)  # [iz, ix]
@@ -20,6 +23,7 @@ def main(path):
     dx, dz = LX / NX, LZ / NZ
     X2 = X + inner * 0.18 * dx * np.sin(2 * np.pi * Z / LZ * 1.5) * np.sin(np.pi * X / LX * 3)
     Z2 = Z + inner * 0.18 * dz * np.sin(2 * np.pi * X / LX * 2.0) * np.cos(np.pi * Z / LZ * 2)
+    X2 = X2 + bulge * np.sin(np.pi * Z / LZ) * X / LX
     nnod = (NX + 1) * (NZ + 1)
     nid = rng.permutation(nnod) + 1  # shuffled node ids: nid[grid node] = file id
     order = np.argsort(nid)         # file id - 1 -> grid node
@@ -64,4 +68,6 @@ def main(path):
 
 
 if __name__ == "__main__":
-    main(os.path.join(os.path.dirname(os.path.abspath(__file__)), "box.mesh2d"))
+    here = os.path.dirname(os.path.abspath(__file__))
+    main(os.path.join(here, "mesh2d_psv", "box.mesh2d"))
+    main(os.path.join(here, "mesh2d_curved", "curved.mesh2d"), bulge=400.0)
