@@ -1,5 +1,5 @@
-"""Write the MESH2D meshes of the golden cases mesh2d_psv/box.mesh2d and
-mesh2d_curved/curved.mesh2d: a 6 km x 4 km domain of 30 x 20 four-node quadrilaterals in
+"""Write the MESH2D meshes of the golden cases mesh2d_psv/box.mesh2d (also used by
+mesh2d_sh) and mesh2d_curved/curved.mesh2d: a 6 km x 4 km domain of 30 x 20 four-node quadrilaterals in
 the MESH2D format (SRC/mesh_mesh2d.f90:17-36), with the interior mesh nodes moved off the
 lines of a regular grid (smooth sine displacements, so that no element is a rectangle),
 elements and nodes listed in a shuffled order, two material tags (1 below z = 1.6 km
@@ -70,4 +70,5 @@ def main(path, bulge=0.0):
 if __name__ == "__main__":
     here = os.path.dirname(os.path.abspath(__file__))
     main(os.path.join(here, "mesh2d_psv", "box.mesh2d"))
+    main(os.path.join(here, "mesh2d_sh", "box.mesh2d"))
     main(os.path.join(here, "mesh2d_curved", "curved.mesh2d"), bulge=400.0)
