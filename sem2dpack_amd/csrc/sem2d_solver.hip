@@ -2454,6 +2454,26 @@ inline char* st_d(sem2d_solver* h, char* s) { (void)h; return s; }
 inline char* st_v(sem2d_solver* h, char* s) { return s + (size_t)h->ndof * h->fstride * rsize(h); }
 inline char* st_a(sem2d_solver* h, char* s) { return s + 2 * (size_t)h->ndof * h->fstride * rsize(h); }
 
+// One adjoint tile step: the full tiles of the launch list with the constant-shape
+// variant (every array it addresses with 32-bit offsets below 4 GiB, the double
+// mass-sensitivity array included), the others with the general one.
+template <typename R>
+void launch_adj_tile(sem2d_solver* h, const StepDev& X, const R* dfw, const R* afw, const R* dp, const R* dn,
+                     R dt, R c1, R c2, R c3, double idt2) {
+  AdjState& A = *h->adj;
+  StructState& S = h->st;
+  const int nf = (h->fstride * sizeof(double) < (size_t(1) << 32)) ? S.nfull : 0;
+  const int np = S.ntiles - nf;
+  if (nf > 0)
+    k_adj_tile<R, true><<<8 * ((nf + 7) / 8), TILE_THREADS, 0, h->stream>>>(
+        X.T, S.tlist_dev, nf, (R*)A.ad, (R*)A.av, (R*)A.aa, (const R*)A.rmass, dfw, afw, (const R*)S.coefs,
+        (const R*)S.geom, (R*)S.fpart, A.glam, A.gmu, A.mbar, dt, c1, c2, c3, dp, dn, idt2);
+  if (np > 0)
+    k_adj_tile<R, false><<<8 * ((np + 7) / 8), TILE_THREADS, 0, h->stream>>>(
+        X.T, S.tlist_dev + nf, np, (R*)A.ad, (R*)A.av, (R*)A.aa, (const R*)A.rmass, dfw, afw,
+        (const R*)S.coefs, (const R*)S.geom, (R*)S.fpart, A.glam, A.gmu, A.mbar, dt, c1, c2, c3, dp, dn, idt2);
+}
+
 template <typename R>
 int adj_backward(sem2d_solver* h, const double* gseis_host) {
   AdjState& A = *h->adj;
@@ -2489,9 +2509,7 @@ int adj_backward(sem2d_solver* h, const double* gseis_host) {
       const R* dp = (const R*)(A.H + (size_t)(n - 1) * nfb);
       const R* dn = n < N ? (const R*)(A.H + (size_t)(n + 1) * nfb) : nullptr;
       const R* af = A.AH ? (const R*)(A.AH + (size_t)n * nfb) : (n == N ? (const R*)A.aN : nullptr);
-      k_adj_tile<R><<<8 * S.per_xcd, TILE_THREADS, 0, h->stream>>>(
-          X.T, (R*)A.ad, (R*)A.av, (R*)A.aa, (const R*)A.rmass, dc, af, (const R*)S.coefs, (const R*)S.geom,
-          (R*)S.fpart, A.glam, A.gmu, A.mbar, dt, c1, c2, c3, dp, dn, idt2);
+      launch_adj_tile<R>(h, X, dc, af, dp, dn, dt, c1, c2, c3, idt2);
       k_adj_deferred<R><<<(S.ndef + 255) / 256, 256, 0, h->stream>>>(
           S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dabso, (const R*)S.rmdef, (R*)A.ad, (R*)A.av,
           (R*)A.aa, GS, af, nullptr, nullptr, A.mbar, A.gabs, dt, c1, c2, c3, dc, dp, dn, idt, idt2);
@@ -2524,9 +2542,7 @@ int adj_backward(sem2d_solver* h, const double* gseis_host) {
             inj, GS);
       const R* dfw = (const R*)st_d(h, A.slot[j]);
       const R* afw = (const R*)st_a(h, A.slot[j]);
-      k_adj_tile<R><<<8 * S.per_xcd, TILE_THREADS, 0, h->stream>>>(
-          X.T, (R*)A.ad, (R*)A.av, (R*)A.aa, (const R*)A.rmass, dfw, afw, (const R*)S.coefs, (const R*)S.geom,
-          (R*)S.fpart, A.glam, A.gmu, A.mbar, dt, c1, c2, c3);
+      launch_adj_tile<R>(h, X, dfw, afw, nullptr, nullptr, dt, c1, c2, c3, 0.0);
       k_adj_deferred<R><<<(S.ndef + 255) / 256, 256, 0, h->stream>>>(
           S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dabso, (const R*)S.rmdef, (R*)A.ad, (R*)A.av,
           (R*)A.aa, GS, afw, (const R*)st_v(h, A.slot[j - 1]), (const R*)st_a(h, A.slot[j - 1]), A.mbar, A.gabs,

@@ -856,9 +856,10 @@ __device__ __forceinline__ double fwd_accel(const R* __restrict__ a, const R* __
   return (((double)dnext[i] - dc) - (dc - (double)dprev[i])) * idt2;
 }
 
-template <typename R>
-__global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
-    TileDev T, R* __restrict__ ad, R* __restrict__ av, R* __restrict__ aa, const R* __restrict__ rmass,
+template <typename R, bool FULL = false>
+__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) void k_adj_tile(
+    TileDev T, const int* __restrict__ tlist, int nlist, R* __restrict__ ad, R* __restrict__ av,
+    R* __restrict__ aa, const R* __restrict__ rmass,
     const R* __restrict__ dfw, const R* __restrict__ afw, const R* __restrict__ coefs, const R* __restrict__ geom,
     R* __restrict__ fpart, double* __restrict__ glam, double* __restrict__ gmu, double* __restrict__ mbar,
     R dt, R c1, R c2, R c3, const R* __restrict__ dprev = nullptr, const R* __restrict__ dnext = nullptr,
@@ -876,17 +877,65 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
   __shared__ R Gm[TX * TZ][3];          // geometric factors of the tile's elements
   __shared__ short sflag[PMAX];
 
+  // workgroup -> listed tile: each XCD (bid & 7) takes a contiguous eighth of the list
   const int bid = blockIdx.x;
-  const int t = (bid & 7) * T.per_xcd + (bid >> 3);
-  if (t >= T.ntiles) return;
+  const int per = (nlist + 7) >> 3;
+  const int qt = (bid & 7) * per + (bid >> 3);
+  if (qt >= nlist) return;
+  const int t = tlist[qt];
   const int tid = threadIdx.x;
   const int4 tg = T.tiles[t];
   const int4 gr = T.grids[t];
-  const int tx0 = tg.x, tz0 = tg.y, nxe = tg.z & 0xff, nze = (tg.z >> 8) & 0xff, Z0 = tg.w;
+  const int tx0 = tg.x, tz0 = tg.y, Z0 = tg.w;
+  const int nxe = FULL ? TX : (tg.z & 0xff), nze = FULL ? TZ : ((tg.z >> 8) & 0xff);
   const int PX = 4 * nxe + 1, PZ = 4 * nze + 1, NPT = PX * PZ;
   const int X0 = 4 * tx0;
   const size_t S = T.NGp;
-  const int kL = tx0 / TX, kR = (tx0 + nxe + TX - 1) / TX;
+  const int kL = tx0 / TX, kR = FULL ? kL + 1 : (tx0 + nxe + TX - 1) / TX;
+  // full tiles: compile-time shapes, 32-bit offsets from per-component bases held in
+  // scalar registers (every array below 4 GiB, checked by the caller)
+  auto nidx = [&](int X, int Z) {
+    if constexpr (FULL) return node_index_full(gr, X, Z, X0, Z0, kL);
+    else return node_index(gr, X, Z, PX, X0, Z0, kL, kR);
+  };
+  auto cidx = [&](int lx, int lz) {
+    if constexpr (FULL)
+      return (uint32_t)gr.w + (uint32_t)(tz0 + lz) * (uint32_t)(tg.z >> 16) + (uint32_t)(tx0 + lx);
+    else return cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
+  };
+  auto ub = [&](auto* p) {
+    if constexpr (FULL) return sgpr_base(p);
+    else return p;
+  };
+  auto at = [&](auto* base, auto g) -> auto& {
+    if constexpr (FULL) return at32(base, (uint32_t)g);
+    else return base[g];
+  };
+  R *bad[NDOF], *bav[NDOF], *baa[NDOF], *bfp[NDOF];
+  const R *brm[NDOF], *bdf[NDOF], *baf[NDOF], *bdp[NDOF], *bdn[NDOF];
+#pragma unroll
+  for (int c = 0; c < NDOF; ++c) {
+    bad[c] = ub(ad + c * S);
+    bav[c] = ub(av + c * S);
+    baa[c] = ub(aa + c * S);
+    brm[c] = ub(rmass + c * S);
+    bdf[c] = ub(dfw + c * S);
+    baf[c] = afw ? ub(afw + c * S) : nullptr;
+    bdp[c] = dprev ? ub(dprev + c * S) : nullptr;
+    bdn[c] = dnext ? ub(dnext + c * S) : nullptr;
+    bfp[c] = ub(fpart + c * T.FPS);
+  }
+  double* bmb = ub(mbar);
+  double* bgl = ub(glam);
+  double* bgm = ub(gmu);
+  const R* bco = ub(coefs);
+  const R* bge = ub(geom);
+  // forward acceleration at node g (as fwd_accel)
+  auto accel = [&](int c, auto g, R dcur) -> double {
+    if (afw) return (double)at(baf[c], g);
+    const double dc = (double)dcur;
+    return (((double)at(bdn[c], g) - dc) - (dc - (double)at(bdp[c], g))) * idt2;
+  };
   const int le = tid / NP;
   const int pe = tid - le * NP;
   const int ii = pe % NG, jj = pe / NG;
@@ -908,21 +957,21 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
     const int p = tid + TILE_THREADS * r;
     if (p < NPT) {
       const int Z = p / PX, X = p - Z * PX;
-      const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
+      const auto g = nidx(X, Z);
       R f[NDOF], u[NDOF];
       double mb = 0;
 #pragma unroll
       for (int c = 0; c < NDOF; ++c) {
-        const R dk = ad[c * S + g], vk = av[c * S + g], ak = aa[c * S + g];
-        f[c] = (ak + c3 * vk) * rmass[c * S + g];
-        u[c] = dfw[c * S + g];
+        const R dk = at(bad[c], g), vk = at(bav[c], g), ak = at(baa[c], g);
+        f[c] = (ak + c3 * vk) * at(brm[c], g);
+        u[c] = at(bdf[c], g);
         adr[r][c] = dk;
         avr[r][c] = vk;
-        mb -= (double)f[c] * fwd_accel(afw, dprev, dnext, c * S + g, u[c], idt2);
+        mb -= (double)f[c] * accel(c, g, u[c]);
       }
       // interior nodes: the running sum is read here, with the other loads, and written
       // back in the store phase (perimeter and special nodes are the deferred kernel's)
-      mbr[r] = perim_slot(X, Z, PX, PZ) < 0 ? mbar[g] + mb : 0.0;
+      mbr[r] = perim_slot(X, Z, PX, PZ) < 0 ? at(bmb, g) + mb : 0.0;
       Uf[p] = lv_make<R, NDOF>(f);
       Ud[p] = lv_make<R, NDOF>(u);
       Fn[p] = LV{};
@@ -943,19 +992,19 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
     const int ncx = (nxe - cx + 1) >> 1, ncz = (nze - cz + 1) >> 1;
     const int ce = b0 + le;
     if (tid < FE_THREADS && ce < ncx * ncz) {
-      const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, cx + 2 * (ce % ncx), cz + 2 * (ce / ncx));
-      lm[q][0] = coefs[cell * 2 * NP + pe];
-      lm[q][1] = coefs[cell * 2 * NP + NP + pe];
+      const auto cell = cidx(cx + 2 * (ce % ncx), cz + 2 * (ce / ncx));
+      lm[q][0] = at(bco, cell * 2 * NP + pe);
+      lm[q][1] = at(bco, cell * 2 * NP + NP + pe);
     }
   }
   if (tid < nxe * nze * 3) {
     const int e = tid / 3, k = tid - 3 * (tid / 3);
-    Gm[e][k] = geom[cell_index(gr, tg.z >> 16, tx0, tz0, e % nxe, e / nxe) * 3 + k];
+    Gm[e][k] = at(bge, cidx(e % nxe, e / nxe) * 3 + k);
   }
   if (tid < nxe * nze) {  // the elements' running sensitivities (local index lz*nxe + lx)
-    const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, tid % nxe, tid / nxe);
-    Ge[tid][0] = glam[cell];
-    Ge[tid][1] = gmu[cell];
+    const auto cell = cidx(tid % nxe, tid / nxe);
+    Ge[tid][0] = at(bgl, cell);
+    Ge[tid][1] = at(bgm, cell);
   }
   __syncthreads();
   const int nspec = T.spec_cnt[t];
@@ -982,9 +1031,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
           la = lm[q][0];
           mu = lm[q][1];
         } else {
-          const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, lx, lz);
-          la = coefs[cell * 2 * NP + pe];
-          mu = coefs[cell * 2 * NP + NP + pe];
+          const auto cell = cidx(lx, lz);
+          la = at(bco, cell * 2 * NP + pe);
+          mu = at(bco, cell * 2 * NP + NP + pe);
         }
         const R kx = la + 2 * mu;
         const R A0 = w * kx * g0, A1 = w * la * g2, A2 = w * kx * g1;
@@ -1066,11 +1115,11 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
   }
   __syncthreads();  // every batch's element sums are in Ge
   if (tid < nxe * nze) {
-    const size_t cell = cell_index(gr, tg.z >> 16, tx0, tz0, tid % nxe, tid / nxe);
-    glam[cell] = Ge[tid][0];
-    gmu[cell] = Ge[tid][1];
+    const auto cell = cidx(tid % nxe, tid / nxe);
+    at(bgl, cell) = Ge[tid][0];
+    at(bgm, cell) = Ge[tid][1];
   }
-  const size_t fbase = (size_t)t * SLOTS;
+  const uint32_t fbase = (uint32_t)t * SLOTS;
 #pragma unroll
   for (int r = 0; r < NODE_ITERS; ++r) {
     const int p = tid + TILE_THREADS * r;
@@ -1081,18 +1130,18 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
       const LV fn = Fn[p];
       if (slot >= 0) {
 #pragma unroll
-        for (int c = 0; c < NDOF; ++c) fpart[c * T.FPS + fbase + slot] = lv_get<R, NDOF>(fn, c);
+        for (int c = 0; c < NDOF; ++c) at(bfp[c], fbase + slot) = lv_get<R, NDOF>(fn, c);
       } else {
-        const size_t g = node_index(gr, X, Z, PX, X0, Z0, kL, kR);
+        const auto g = nidx(X, Z);
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
           const R dn = adr[r][c] + lv_get<R, NDOF>(fn, c);
           const R vbp = avr[r][c];
-          ad[c * S + g] = dn;
-          av[c * S + g] = vbp + dt * dn;
-          aa[c * S + g] = c2 * vbp + c1 * dn;
+          at(bad[c], g) = dn;
+          at(bav[c], g) = vbp + dt * dn;
+          at(baa[c], g) = c2 * vbp + c1 * dn;
         }
-        mbar[g] = mbr[r];
+        at(bmb, g) = mbr[r];
       }
     }
   }
