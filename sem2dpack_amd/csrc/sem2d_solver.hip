@@ -888,8 +888,8 @@ void launch_structured_step(const StepDev& X, int it, R dt, R c1, R c2, R c3, R 
   const int gd = (X.ndef + 255) / 256;
   if (X.s_nbe > 0)  // Stacey terms from the previous step's state (boundary nodes are deferred)
     k_stacey_pre<R><<<(X.s_nbe * NG + 127) / 128, 128, 0, X.stream>>>(
-        X.s_nbe, X.s_enode, X.s_K, (const R*)(X.din ? X.din : X.d), (const R*)(X.din ? X.vin : X.v),
-        (const R*)(X.din ? X.ain : X.a), X.fstride, dt, c1, (double*)X.SR.kxd);
+        X.s_nbe, X.s_enode, X.s_K, (const R*)(X.din ? X.din : X.d), (const R*)(X.din && X.vin ? X.vin : X.v),
+        (const R*)(X.din && X.ain ? X.ain : X.a), X.fstride, dt, c1, (double*)X.SR.kxd);
   if (X.ndof == 2 && X.din)
     k_deferred<R, 2, true><<<gd, 256, 0, X.stream>>>(X.ndef, X.defs, X.part_off, (const R*)X.fpart, X.T.FPS,
                                                       X.dsrc, X.ampl, X.ampl_nt, it, X.itp, X.dabso,
@@ -2558,8 +2558,11 @@ int adj_backward(sem2d_solver* h, const double* gseis_host) {
   return 0;
 }
 
-// Forward sweep of the history mode: every step's displacement (and, for FP32 fields,
-// acceleration) copied into the history; the acceleration after the last step kept.
+// Forward sweep of the history mode.  Step n reads the displacement of history slot n-1
+// and writes it into slot n (out of place, with v, and a unless it has a history, in
+// place), so the history needs no copies; for FP32 fields the acceleration goes through
+// its own history the same way.  The receivers read the current slots; the solver's own
+// arrays receive the final state, and the acceleration after the last step is kept.
 template <typename R>
 int adj_forward_hist(sem2d_solver* h) {
   AdjState& A = *h->adj;
@@ -2570,17 +2573,30 @@ int adj_forward_hist(sem2d_solver* h) {
   int rc = sync_sources(h);
   if (rc) return rc;
   StepDev X = step_dev(h);
+  void *d0 = h->d, *a0 = h->a;
+  auto Hn = [&](int n) { return (void*)(A.H + (size_t)n * nfb); };
+  auto AHn = [&](int n) { return (void*)(A.AH + (size_t)n * nfb); };
   HIPCHK(h, hipMemsetAsync(A.H, 0, nfb, h->stream));
   if (A.AH) HIPCHK(h, hipMemsetAsync(A.AH, 0, nfb, h->stream));
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   for (int n = 1; n <= A.nsteps; ++n) {
+    X.din = Hn(n - 1);
+    X.vin = nullptr;  // in place
+    X.ain = A.AH ? AHn(n - 1) : nullptr;
+    X.d = Hn(n);
+    X.a = A.AH ? AHn(n) : a0;
     launch_structured_step<R>(X, n, dt, c1, c2, c3, c4);
     h->it = n;
-    if ((rc = record_receivers<R>(h))) return rc;
-    HIPCHK(h, hipMemcpyAsync(A.H + (size_t)n * nfb, h->d, nfb, hipMemcpyDeviceToDevice, h->stream));
-    if (A.AH) HIPCHK(h, hipMemcpyAsync(A.AH + (size_t)n * nfb, h->a, nfb, hipMemcpyDeviceToDevice, h->stream));
+    h->d = X.d;
+    h->a = X.a;
+    rc = record_receivers<R>(h);
+    h->d = d0;
+    h->a = a0;
+    if (rc) return rc;
   }
-  if (A.aN) HIPCHK(h, hipMemcpyAsync(A.aN, h->a, nfb, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d0, Hn(A.nsteps), nfb, hipMemcpyDeviceToDevice, h->stream));
+  if (A.AH) HIPCHK(h, hipMemcpyAsync(a0, AHn(A.nsteps), nfb, hipMemcpyDeviceToDevice, h->stream));
+  if (A.aN) HIPCHK(h, hipMemcpyAsync(A.aN, a0, nfb, hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipEventSynchronize(h->ev1));

@@ -95,9 +95,10 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(TileDev T, R* __restrict_
                                                        const R* __restrict__ din = nullptr,
                                                        const R* __restrict__ vin = nullptr,
                                                        const R* __restrict__ ain = nullptr) {
+  // out of place: a null vin / ain reads v / a (their step is then in place)
   const R* __restrict__ dsrc = OOP ? din : d;
-  const R* __restrict__ vsrc = OOP ? vin : v;
-  const R* __restrict__ asrc = OOP ? ain : a;
+  const R* vsrc = OOP && vin ? vin : v;
+  const R* asrc = OOP && ain ? ain : a;
   __shared__ R Ud[NDOF][PMAX];
   __shared__ R Fn[NDOF][PMAX];
   __shared__ R T1[FE_EPB][NDOF][NP];
@@ -318,9 +319,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
                                                             const R* __restrict__ vin = nullptr,
                                                             const R* __restrict__ ain = nullptr) {
   // OOP: the step reads the state (din, vin, ain) and writes (d, v, a); otherwise in place
+  // out of place: a null vin / ain reads v / a (their step is then in place)
   const R* __restrict__ dsrc = OOP ? din : d;
-  const R* __restrict__ vsrc = OOP ? vin : v;
-  const R* __restrict__ asrc = OOP ? ain : a;
+  const R* vsrc = OOP && vin ? vin : v;
+  const R* asrc = OOP && ain ? ain : a;
   // node values with both components side by side: one LDS access moves both
   using LV = typename LVec<R, NDOF>::T;
   __shared__ LV Ud[PMAX];
@@ -750,9 +752,10 @@ __global__ __launch_bounds__(256) void k_deferred(int ndef, const DefNode* __res
   const DefNode n = defs[q];
   const size_t g = n.g;
   const int it = itp ? *itp : it_arg;  // step index: device counter under graph replay
+  // out of place: a null vin / ain reads v / a (their step is then in place)
   const R* __restrict__ dsrc = OOP ? din : d;
-  const R* __restrict__ vsrc = OOP ? vin : v;
-  const R* __restrict__ asrc = OOP ? ain : a;
+  const R* vsrc = OOP && vin ? vin : v;
+  const R* asrc = OOP && ain ? ain : a;
   R dp[NDOF], vpp[NDOF], f[NDOF];
 #pragma unroll
   for (int c = 0; c < NDOF; ++c) {
