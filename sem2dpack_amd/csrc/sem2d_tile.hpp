@@ -860,7 +860,7 @@ __device__ __forceinline__ double fwd_accel(const R* __restrict__ a, const R* __
 }
 
 template <typename R, bool FULL = false>
-__global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) void k_adj_tile(
+__global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
     TileDev T, const int* __restrict__ tlist, int nlist, R* __restrict__ ad, R* __restrict__ av,
     R* __restrict__ aa, const R* __restrict__ rmass,
     const R* __restrict__ dfw, const R* __restrict__ afw, const R* __restrict__ coefs, const R* __restrict__ geom,
@@ -1123,9 +1123,15 @@ __global__ __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(si
     at(bgm, cell) = Ge[tid][1];
   }
   const uint32_t fbase = (uint32_t)t * SLOTS;
+  // lane index re-materialised for the store loop, so that the node-index arithmetic of
+  // the load loop is recomputed here instead of being held in registers through the
+  // element batches
+  int tst = tid;
+  asm volatile("" : "+v"(tst));
+  __builtin_assume(tst >= 0 && tst < TILE_THREADS);
 #pragma unroll
   for (int r = 0; r < NODE_ITERS; ++r) {
-    const int p = tid + TILE_THREADS * r;
+    const int p = tst + TILE_THREADS * r;
     if (p < NPT) {
       const int Z = p / PX, X = p - Z * PX;
       int slot = perim_slot(X, Z, PX, PZ);
