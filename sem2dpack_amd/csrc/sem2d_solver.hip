@@ -2350,6 +2350,10 @@ struct AdjState {
   int nu = 0;
   int *unode = nullptr, *uoff = nullptr, *urec = nullptr;
   double* uw = nullptr;
+  // the injection of step n-1 done by the deferred kernel of step n: per deferred node the
+  // injection node it finishes (or -1), and the injection nodes that are not deferred
+  int *udef = nullptr, *uext = nullptr;
+  int next = 0;
   size_t state_bytes = 0;  // d, v, a of one state
   std::vector<char*> ck, slot;
   char *ad = nullptr, *av = nullptr, *aa = nullptr, *rmass = nullptr;
@@ -2494,25 +2498,47 @@ int adj_backward(sem2d_solver* h, const double* gseis_host) {
   HIPCHK(h, hipMemsetAsync(A.gabs, 0, (size_t)h->ndof * std::max(S.ndef, 1) * sizeof(double), h->stream));
   StepDev X = step_dev(h);
   R* inj = (R*)(h->field == 0 ? A.ad : (h->field == 1 ? A.av : A.aa));
+  auto sampled = [&](int n) { return h->nrec && n >= 1 && n % h->isamp == 0 && n / h->isamp < h->nsamp; };
+  auto gs_of = [&](int n) { return A.gseis + (size_t)(n / h->isamp) * h->ndof * h->nrec; };
+  // the deferred kernel of step n also adds the sample of step n-1 (after its own
+  // updates, before the tile kernel of step n-1 reads the field); the first step's
+  // sample gets its own launch
+  auto plan_for = [&](int n) {
+    InjectPlan P;
+    if (!sampled(n - 1)) return P;
+    P.nu = A.nu;
+    P.nrec = h->nrec;
+    P.unode = A.unode;
+    P.off = A.uoff;
+    P.rec = A.urec;
+    P.w = A.uw;
+    P.g = gs_of(n - 1);
+    P.fld = inj;
+    P.udef = A.udef;
+    P.uext = A.uext;
+    P.next = A.next;
+    return P;
+  };
+  auto deferred_grid = [&](const InjectPlan& P) { return (S.ndef + (P.g ? P.next : 0) + 255) / 256; };
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  if (sampled(A.nsteps))
+    k_adj_inject<R><<<grid_for(A.nu, 64), 64, 0, h->stream>>>(A.nu, A.unode, A.uoff, A.urec, A.uw, h->nrec,
+                                                             gs_of(A.nsteps), inj, GS);
   if (A.hist) {
     // no replay: the forward displacements are all in memory
     const size_t nfb = nf * sizeof(R);
     const double idt = 1.0 / h->dt, idt2 = idt * idt;
     const int N = A.nsteps;
     for (int n = N; n >= 1; --n) {
-      if (h->nrec && n % h->isamp == 0 && n / h->isamp < h->nsamp)
-        k_adj_inject<R><<<grid_for(A.nu, 64), 64, 0, h->stream>>>(
-            A.nu, A.unode, A.uoff, A.urec, A.uw, h->nrec, A.gseis + (size_t)(n / h->isamp) * h->ndof * h->nrec,
-            inj, GS);
       const R* dc = (const R*)(A.H + (size_t)n * nfb);
       const R* dp = (const R*)(A.H + (size_t)(n - 1) * nfb);
       const R* dn = n < N ? (const R*)(A.H + (size_t)(n + 1) * nfb) : nullptr;
       const R* af = A.AH ? (const R*)(A.AH + (size_t)n * nfb) : (n == N ? (const R*)A.aN : nullptr);
       launch_adj_tile<R>(h, X, dc, af, dp, dn, dt, c1, c2, c3, idt2);
-      k_adj_deferred<R><<<(S.ndef + 255) / 256, 256, 0, h->stream>>>(
+      const InjectPlan P = plan_for(n);
+      k_adj_deferred<R><<<deferred_grid(P), 256, 0, h->stream>>>(
           S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dabso, (const R*)S.rmdef, (R*)A.ad, (R*)A.av,
-          (R*)A.aa, GS, af, nullptr, nullptr, A.mbar, A.gabs, dt, c1, c2, c3, dc, dp, dn, idt, idt2);
+          (R*)A.aa, GS, af, nullptr, nullptr, A.mbar, A.gabs, dt, c1, c2, c3, dc, dp, dn, idt, idt2, P);
     }
   }
   for (int k = A.hist ? -1 : A.nck - 1; k >= 0; --k) {
@@ -2536,17 +2562,14 @@ int adj_backward(sem2d_solver* h, const double* gseis_host) {
     // adjoint steps n = s1 .. s0+1
     for (int n = s1; n > s0; --n) {
       const int j = n - s0;
-      if (h->nrec && n % h->isamp == 0 && n / h->isamp < h->nsamp)
-        k_adj_inject<R><<<grid_for(A.nu, 64), 64, 0, h->stream>>>(
-            A.nu, A.unode, A.uoff, A.urec, A.uw, h->nrec, A.gseis + (size_t)(n / h->isamp) * h->ndof * h->nrec,
-            inj, GS);
       const R* dfw = (const R*)st_d(h, A.slot[j]);
       const R* afw = (const R*)st_a(h, A.slot[j]);
       launch_adj_tile<R>(h, X, dfw, afw, nullptr, nullptr, dt, c1, c2, c3, 0.0);
-      k_adj_deferred<R><<<(S.ndef + 255) / 256, 256, 0, h->stream>>>(
+      const InjectPlan P = plan_for(n);  // the sample of step n-1 (the replay leaves the adjoint alone)
+      k_adj_deferred<R><<<deferred_grid(P), 256, 0, h->stream>>>(
           S.ndef, S.defs, S.part_off, (const R*)S.fpart, S.FPS, S.dabso, (const R*)S.rmdef, (R*)A.ad, (R*)A.av,
           (R*)A.aa, GS, afw, (const R*)st_v(h, A.slot[j - 1]), (const R*)st_a(h, A.slot[j - 1]), A.mbar, A.gabs,
-          dt, c1, c2, c3);
+          dt, c1, c2, c3, nullptr, nullptr, nullptr, 0.0, 0.0, P);
     }
   }
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
@@ -2722,6 +2745,30 @@ int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
     HIPCHK(h, hipMemcpy(A.uoff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(A.urec, rec.data(), rec.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(A.uw, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+    // injection nodes that are deferred nodes (storage index of deferred node q:
+    // gmap[def_ref[q]]) are finished by that deferred lane, the others by extra lanes
+    const StructState& S = h->st;
+    std::unordered_map<int, int> u_at;  // injection node -> u (a few entries)
+    for (int u = 0; u < (int)un.size(); ++u) u_at[un[u]] = u;
+    std::vector<int> udef(std::max(S.ndef, 1), -1), uext;
+    std::vector<char> taken(un.size(), 0);
+    for (int q = 0; q < S.ndef; ++q) {
+      auto it = u_at.find(S.gmap[S.def_ref[q]]);
+      if (it != u_at.end()) {
+        udef[q] = it->second;
+        taken[it->second] = 1;
+      }
+    }
+    for (int u = 0; u < (int)un.size(); ++u)
+      if (!taken[u]) uext.push_back(u);
+    A.next = (int)uext.size();
+    if ((rc = adj_alloc(h, &q, udef.size() * sizeof(int)))) return rc;
+    A.udef = (int*)q;
+    if ((rc = adj_alloc(h, &q, std::max<size_t>(uext.size(), 1) * sizeof(int)))) return rc;
+    A.uext = (int*)q;
+    HIPCHK(h, hipMemcpy(A.udef, udef.data(), udef.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!uext.empty())
+      HIPCHK(h, hipMemcpy(A.uext, uext.data(), uext.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   // inverse mass of both components in storage order
   std::vector<double> rm((size_t)h->ndof * h->fstride, 0.0);

@@ -1160,6 +1160,31 @@ __global__ __launch_bounds__(TILE_THREADS) void k_adj_tile(
 // forces -K^T fb of the tiles, the absorbing term, the mass and ABC sensitivities.
 // gabs[c][q] accumulates dJ/dC_c of the node (for nodes with absorbing entries):
 // -fb_c (vp_c + c3 a_c), vp = v_{n-1} + c2 a_{n-1} the forward predicted velocity.
+// dJ/d(recorded field) of one sample at the receiver nodes (see k_adj_inject); the
+// injection of step n-1 can ride on the deferred kernel of step n: udef[q] is the
+// injection node handled by deferred node q after its own update (or -1), uext the
+// injection nodes that are not deferred nodes, done by extra lanes.
+struct InjectPlan {
+  int nu = 0, nrec = 0;
+  const int *unode = nullptr, *off = nullptr, *rec = nullptr;
+  const double* w = nullptr;
+  const double* g = nullptr;  // the sample, [ndof][nrec]; null: no injection
+  void* fld = nullptr;        // the adjoint field it is added to (bulk type)
+  const int* udef = nullptr;
+  const int* uext = nullptr;
+  int next = 0;
+};
+
+template <typename R>
+__device__ __forceinline__ void inject_node(const InjectPlan& P, int u, R* __restrict__ fld, size_t GS) {
+  for (int c = 0; c < 2; ++c) {
+    const size_t i = c * GS + P.unode[u];
+    R x = fld[i];
+    for (int k = P.off[u]; k < P.off[u + 1]; ++k) x = (R)((double)x + P.w[k] * P.g[(size_t)c * P.nrec + P.rec[k]]);
+    fld[i] = x;
+  }
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_adj_deferred(int ndef, const DefNode* __restrict__ defs,
                                                       const int* __restrict__ part_off, const R* __restrict__ fpart,
@@ -1172,10 +1197,13 @@ __global__ __launch_bounds__(256) void k_adj_deferred(int ndef, const DefNode* _
                                                       const R* __restrict__ dcur = nullptr,
                                                       const R* __restrict__ dprev = nullptr,
                                                       const R* __restrict__ dnext = nullptr, double idt = 0.0,
-                                                      double idt2 = 0.0) {
+                                                      double idt2 = 0.0, InjectPlan IP = {}) {
   constexpr int NDOF = 2;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ndef) return;
+  if (q >= ndef) {  // injection lanes (nodes that are not deferred)
+    if (IP.g && q < ndef + IP.next) inject_node<R>(IP, IP.uext[q - ndef], (R*)IP.fld, GS);
+    return;
+  }
   const DefNode n = defs[q];
   const size_t g = n.g;
   double mb = 0;
@@ -1204,6 +1232,7 @@ __global__ __launch_bounds__(256) void k_adj_deferred(int ndef, const DefNode* _
     }
   }
   mbar[g] += mb;
+  if (IP.g && IP.udef[q] >= 0) inject_node<R>(IP, IP.udef[q], (R*)IP.fld, GS);  // after the update
 }
 
 // dJ/d(recorded field) of one sample added at the receiver nodes, or spread over the
@@ -1217,12 +1246,14 @@ __global__ void k_adj_inject(int nu, const int* __restrict__ unode, const int* _
                              const double* __restrict__ g, R* __restrict__ fld, size_t GS) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= nu) return;
-  for (int c = 0; c < 2; ++c) {
-    const size_t i = c * GS + unode[u];
-    R x = fld[i];
-    for (int k = off[u]; k < off[u + 1]; ++k) x = (R)((double)x + w[k] * g[(size_t)c * nrec + rec[k]]);
-    fld[i] = x;
-  }
+  InjectPlan P;
+  P.nrec = nrec;
+  P.unode = unode;
+  P.off = off;
+  P.rec = rec;
+  P.w = w;
+  P.g = g;
+  inject_node<R>(P, u, fld, GS);
 }
 
 }  // namespace sem2d
