@@ -1607,8 +1607,6 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
             t = _bound_topo(t1, mesh, num, perm_inv)
             pb.abso.append(_abso_init(t, jac, wgll, (rho_p, cp_p, cs_p), M, coefA2Vrhs, ndof, stacey))
         elif kind == "DYNFLT":
-            if not flat:
-                raise InputError("DYNFLT on a MESH2D mesh is not supported by this build")
             if pb.fault is not None:
                 raise InputError("more than one DYNFLT boundary is not supported by this build")
             pb.fault = _dynflt_init(par, bd, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma, pb.nt, ndof)

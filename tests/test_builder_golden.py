@@ -13,7 +13,7 @@ from conftest import GOLDEN, case_par
 from sem2dpack_amd import builder
 
 CASES = ["small_psv", "c1_psv", "sh_swf", "psv_rsf", "hete_psv", "rsf_ens", "multi_src", "interp_rec", "sh_kv", "psv_kv",
-         "psv_stacey", "mesh2d_psv", "mesh2d_curved", "mesh2d_sh"]
+         "psv_stacey", "mesh2d_psv", "mesh2d_curved", "mesh2d_sh", "mesh2d_flt"]
 
 
 def sha(a):

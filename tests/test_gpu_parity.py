@@ -49,7 +49,7 @@ def test_fint(name, oracle_api):
                                          ("sh_kv", 600), ("psv_kv", 500),
                                          ("psv_rsf", 500), ("hete_psv", 400), ("rsf_ens", 400),
                                          ("psv_stacey", 400), ("mesh2d_psv", 500), ("mesh2d_curved", 500),
-                                         ("mesh2d_sh", 500)])
+                                         ("mesh2d_sh", 500), ("mesh2d_flt", 500)])
 def test_time_series(name, nsteps, oracle_api, oracle_ulp_api):
     print(f"\n[case {name} nsteps {nsteps}]", flush=True)
     pb, ref, gpu = _pair(name, nsteps, oracle_api)
