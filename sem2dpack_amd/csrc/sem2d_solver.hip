@@ -826,11 +826,11 @@ void launch_pipe(const StepDev& X, const int* list, int n, R dt, R c1, R c2, R c
   const int wgs = std::min(X.pipe_wgs, (n + 7) / 8 * 8);
   if (X.pipe_threads == 256)
     k_tile_pipe<R, NDOF, NELAST, MODE, 256, 10, OOP, FULL><<<wgs, 256, 0, X.stream>>>(
-        X.T, list, n, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
+        X.T, list, n, X.T.tiles, X.T.grids, X.T.spec_cnt, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
         (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
   else
     k_tile_pipe<R, NDOF, NELAST, MODE, 512, 20, OOP, FULL><<<wgs, 512, 0, X.stream>>>(
-        X.T, list, n, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
+        X.T, list, n, X.T.tiles, X.T.grids, X.T.spec_cnt, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
         (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
 }
 

@@ -310,7 +310,10 @@ __device__ __forceinline__ uint32_t node_index_full(const int4 gr, int X, int Z,
 // offsets from per-component bases (the caller guarantees that every component of
 // these arrays is below 4 GiB).
 template <typename R, int NDOF, int NELAST, int MODE, int NT, int EPB, bool OOP = false, bool FULL = false>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 4 : 2))) void k_tile_pipe(TileDev T, const int* __restrict__ tlist, int nlist, R* __restrict__ d,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 4 : 2))) void k_tile_pipe(TileDev T, const int* __restrict__ tlist, int nlist,
+                                                            const int4* __restrict__ ttiles,
+                                                            const int4* __restrict__ tgrids,
+                                                            const int* __restrict__ tspc, R* __restrict__ d,
                                                             R* __restrict__ v, R* __restrict__ a,
                                                             const R* __restrict__ rmass0,
                                                             const R* __restrict__ coefs, const R* __restrict__ geom,
@@ -343,6 +346,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   __shared__ double sH[NP];
   __shared__ double sW2[NP];
   __shared__ R Gm[TX * TZ][3];
+  // moduli of each thread's point tasks (compact mode), staged from the prefetch
+  // registers by the predictor
+  __shared__ LV ML[MODE == COEF_COMPACT ? PB_MAXB : 1][ETHREADS];
 
   const int nwg = gridDim.x;
   const int bid = blockIdx.x;
@@ -440,11 +446,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
     else return cell_index(gr, nx, G.tx0, G.tz0, lx, lz);
   };
 
-  // prefetch registers of the next tile: node values ...
+  // Registers of the next tile, all issued in one group at the start of the current
+  // tile's element phase: node values, inverse mass of the patch nodes, the moduli of
+  // this thread's point tasks (compact mode) or its point coefficients (shared mode: the
+  // same for every element of the tile's grid), one geometric factor and one
+  // special-node index.
   R pd[NITER][NDOF], pv[NITER][NDOF], pa[NITER][NDOF];
-  auto issue_nodes = [&](int t) {
-    const int4 tg = T.tiles[t];
-    const int4 gr = T.grids[t];
+  R pm[NITER];
+  R lm[PB_MAXB][NDOF] = {};
+  R sa[NELAST] = {}, sac[NELAST] = {};
+  R pg = 0;
+  int psp = -1;
+  auto issue = [&](int t) {
+    const int4 tg = ttiles[t];
+    const int4 gr = tgrids[t];
     const Geo G = geo(tg);
 #pragma unroll
     for (int r = 0; r < NITER; ++r) {
@@ -458,25 +473,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
           pv[r][c] = fld(bV[c], g);
           pa[r][c] = fld(bA[c], g);
         }
-      }
-    }
-  };
-  // ... and the late group: inverse mass of the patch nodes, moduli of this thread's
-  // point tasks (compact mode), one geometric factor, one special-node index
-  R pm[NITER];
-  R lm[PB_MAXB][NDOF];
-  R pg = 0;
-  int psp = -1;
-  auto issue_elems = [&](int t) {
-    const int4 tg = T.tiles[t];
-    const int4 gr = T.grids[t];
-    const Geo G = geo(tg);
-#pragma unroll
-    for (int r = 0; r < NITER; ++r) {
-      const int p = tnode + NT * r;
-      if (p < G.NPT) {
-        const int Z = p / G.PX, X = p - Z * G.PX;
-        pm[r] = fld(bRm, nidx(gr, G, X, Z));
+        pm[r] = fld(bRm, g);
       }
     }
     if constexpr (MODE == COEF_COMPACT) {
@@ -498,11 +495,49 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
           for (int c = 0; c < NDOF; ++c) lm[q][c] = fld(bCo, cell * NDOF * NP + c * NP + pe);
         }
       }
+    } else if constexpr (MODE == COEF_SHARED) {
+      if (tid < ETHREADS) load_coefs<R, NDOF, NELAST, MODE>(grid_coefs<R, MODE>(coefs, gr), geom, 0, pe, sa);
     }
-    if (tid < T.spec_cnt[t]) psp = T.spec_idx[t * MAXSPECIAL + tid];
+    if (tid < tspc[t]) psp = T.spec_idx[t * MAXSPECIAL + tid];
   };
-  issue_nodes(tlist[q0]);
-  issue_elems(tlist[q0]);
+
+  // Stores of a finished tile (tile tp, descriptors tgp, grp) from LDS: a thread
+  // stores the patch nodes it owns in the node loops (p = tnode + NT r), so the next
+  // tile's predictor, run by the same thread on the same slots after this, needs no
+  // barrier in between.  Partial forces for perimeter and special nodes, new a, v, d
+  // for the others.
+  auto store_tile = [&](const int tp, const int4 tgp, const int4 grp) {
+    const Geo G = geo(tgp);
+    const uint32_t fbase = (uint32_t)tp * SLOTS;
+#pragma unroll
+    for (int r = 0; r < NITER; ++r) {
+      const int p = tnode + NT * r;
+      if (p < G.NPT) {
+        const int Z = p / G.PX, X = p - Z * G.PX;
+        int slot = perim_slot(X, Z, G.PX, G.PZ);
+        if (slot < 0 && sflag[p] >= 0) slot = PERIM_MAX + sflag[p];
+        if (slot >= 0) {
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) fld(bF[c], fbase + slot) = lv_get<R, NDOF>(Fn[p], c);
+        } else {
+          const auto g = nidx(grp, G, X, Z);
+          const R rm = Rm[p];
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) {
+            const R ak = lv_get<R, NDOF>(Fn[p], c) * rm;
+            fld(oA[c], g) = ak;
+            fld(oV[c], g) = lv_get<R, NDOF>(Vp[p], c) + c3 * ak;
+            fld(oD[c], g) = lv_get<R, NDOF>(Ud[p], c) + c4 * ak;
+          }
+        }
+      }
+    }
+  };
+
+  issue(tlist[q0]);
+  // the previous tile of this workgroup, whose results are still in LDS
+  int tp = -1;
+  int4 tgp = {}, grp = {};
 
 #pragma unroll 1
   for (int q0i = q0; q0i < s1; q0i += W) {
@@ -517,24 +552,35 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
     }
     const int t = tlist[q0i];
     const int tn = q0i + W < s1 ? tlist[q0i + W] : -1;  // next tile of this workgroup
-    const int4 tg = T.tiles[t];
-    const int4 gr = T.grids[t];
+    const int4 tg = ttiles[t];
+    const int4 gr = tgrids[t];
     const Geo G = geo(tg);
     const int PX = G.PX, PZ = G.PZ, NPT = G.NPT;
 
-    // predictor from the prefetched values
+    // predictor from the prefetched values (every wait on a global load of this tile
+    // is here, ahead of any store) ...
+    R u[NITER][NDOF], w[NITER][NDOF];
 #pragma unroll
     for (int r = 0; r < NITER; ++r) {
       const int p = tnode + NT * r;
       if (p < NPT) {
-        R u[NDOF], w[NDOF];
 #pragma unroll
         for (int c = 0; c < NDOF; ++c) {
-          u[c] = pd[r][c] + dt * pv[r][c] + c1 * pa[r][c];
-          w[c] = pv[r][c] + c2 * pa[r][c];
+          u[r][c] = pd[r][c] + dt * pv[r][c] + c1 * pa[r][c];
+          w[r][c] = pv[r][c] + c2 * pa[r][c];
         }
-        Ud[p] = lv_make<R, NDOF>(u);
-        Vp[p] = lv_make<R, NDOF>(w);
+      }
+    }
+    // ... then the previous tile's stores (this tile's element phase covers their
+    // completion, and the next wait on a load is for loads issued after them) ...
+    if (tp >= 0) store_tile(tp, tgp, grp);
+    // ... then this tile's node values into LDS
+#pragma unroll
+    for (int r = 0; r < NITER; ++r) {
+      const int p = tnode + NT * r;
+      if (p < NPT) {
+        Ud[p] = lv_make<R, NDOF>(u[r]);
+        Vp[p] = lv_make<R, NDOF>(w[r]);
         Fn[p] = LV{};
         Rm[p] = pm[r];
         sflag[p] = -1;
@@ -542,12 +588,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
     }
     if constexpr (MODE == COEF_COMPACT) {
       if (tid < G.nxe * G.nze * 3) Gm[tid / 3][tid - 3 * (tid / 3)] = pg;
+      if (tid < ETHREADS) {
+#pragma unroll
+        for (int q = 0; q < PB_MAXB; ++q) ML[q][tid] = lv_make<R, NDOF>(lm[q]);
+      }
+    } else if constexpr (MODE == COEF_SHARED) {
+#pragma unroll
+      for (int k = 0; k < NELAST; ++k) sac[k] = sa[k];
     }
     __syncthreads();
     if (psp >= 0) sflag[psp] = (short)tid;
     psp = -1;
-    // overlap: node loads of the next tile fly during this tile's element phase
-    if (tn >= 0) issue_nodes(tn);
+    // overlap: all loads of the next tile fly during this tile's element phase
+    if (tn >= 0) issue(tn);
 
     // Element batches, software-pipelined over one barrier per batch: in phase q the
     // lanes compute the stresses of batch q (into T[q & 1]) and assemble batch q-1 (from
@@ -575,8 +628,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
           if constexpr (MODE == COEF_COMPACT) {
             const int el = lz * G.nxe + lx;
             const R g0 = Gm[el][0], g1 = Gm[el][1], g2 = Gm[el][2];
+            const LV mm = ML[q][tid];
             if constexpr (NDOF == 2) {
-              const R la = lm[q][0], mu = lm[q][1], kx = la + 2 * mu;
+              const R la = lv_get<R, NDOF>(mm, 0), mu = lv_get<R, NDOF>(mm, 1), kx = la + 2 * mu;
               A[0] = w2 * kx * g0;
               A[1] = w2 * la * g2;
               A[2] = w2 * kx * g1;
@@ -584,9 +638,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
               A[4] = w2 * mu * g2;
               A[5] = w2 * mu * g0;
             } else {
-              A[0] = w2 * lm[q][0] * g0;
-              A[1] = w2 * lm[q][0] * g1;
+              A[0] = w2 * lv_get<R, NDOF>(mm, 0) * g0;
+              A[1] = w2 * lv_get<R, NDOF>(mm, 0) * g1;
             }
+          } else if constexpr (MODE == COEF_SHARED) {
+#pragma unroll
+            for (int k = 0; k < NELAST; ++k) A[k] = sac[k];
           } else {
             const size_t cell = cell_index(gr, tg.z >> 16, G.tx0, G.tz0, lx, lz);
             load_coefs<R, NDOF, NELAST, MODE>(grid_coefs<R, MODE>(coefs, gr), geom, cell, pe, A);
@@ -658,35 +715,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
       pv_base = base;
       if (has) ++nvb;
     }
-    // element loads of the next tile, ahead of this tile's stores
-    if (tn >= 0) issue_elems(tn);
-
-    const uint32_t fbase = (uint32_t)t * SLOTS;
-#pragma unroll
-    for (int r = 0; r < NITER; ++r) {
-      const int p = tnode + NT * r;
-      if (p < NPT) {
-        const int Z = p / PX, X = p - Z * PX;
-        int slot = perim_slot(X, Z, PX, PZ);
-        if (slot < 0 && sflag[p] >= 0) slot = PERIM_MAX + sflag[p];
-        if (slot >= 0) {
-#pragma unroll
-          for (int c = 0; c < NDOF; ++c) fld(bF[c], fbase + slot) = lv_get<R, NDOF>(Fn[p], c);
-        } else {
-          const auto g = nidx(gr, G, X, Z);
-          const R rm = Rm[p];
-#pragma unroll
-          for (int c = 0; c < NDOF; ++c) {
-            const R ak = lv_get<R, NDOF>(Fn[p], c) * rm;
-            fld(oA[c], g) = ak;
-            fld(oV[c], g) = lv_get<R, NDOF>(Vp[p], c) + c3 * ak;
-            fld(oD[c], g) = lv_get<R, NDOF>(Ud[p], c) + c4 * ak;
-          }
-        }
-      }
-    }
-    __syncthreads();  // LDS reuse by the next tile
+    // the last element batch ended with a barrier: Fn is complete
+    tp = t;
+    tgp = tg;
+    grp = gr;
   }
+  if (tp >= 0) store_tile(tp, tgp, grp);
 }
 
 // One deferred node: tile-perimeter, special or split-fault node.
