@@ -292,17 +292,23 @@ __device__ __forceinline__ uint32_t node_index_full(const int4 gr, int X, int Z,
 // sequence of tiles inside its XCD's share (see below).  No tile waits on a
 // global load issued during that tile: everything a tile reads comes from registers
 // filled while the previous tile was being processed.
-//   * d, v, a of the next tile's patch nodes are issued at the start of the current
-//     tile's element phase;
-//   * the next tile's inverse mass, moduli, geometric factors and special-node list
-//     are issued at the end of the current element phase, before the current tile's
-//     stores (these registers are free then).
 // Vector memory operations retire in issue order (loads and stores share one
-// counter), so a load issued after the stores of a tile could only be consumed once
-// those stores had completed; the order above keeps every wait off the stores.
+// counter), so a wait for a load also waits for every store issued before it, and the
+// compiler counts the stores issued after a load as the minimum over the branches.
+// Per tile, therefore:
+//   1. the predictor takes every wait, on loads issued a whole element phase earlier;
+//   2. the previous tile's results are stored from LDS (each thread stores the patch
+//      slots it overwrites next, so no barrier separates 2 and 3);
+//   3. the predicted values go to LDS;
+//   4. all loads of the next tile are issued in one group (d, v, a, inverse mass,
+//      moduli or point coefficients, geometric factors, special-node index);
+//   5. the element phase runs while they and the stores of 2 complete.
+// Tile descriptors are scalar loads, counted apart from vector memory.
 // Results are identical to k_tile: same predictor, same colour order and same
 // summation order.  NT threads per workgroup, EPB elements per element batch
-// (EPB*25 <= NT).  The launch covers the tiles tlist[0 .. nlist).
+// (EPB*25 <= NT).  The launch covers the tiles tlist[0 .. nlist).  ttiles, tgrids and
+// tspc are T.tiles, T.grids and T.spec_cnt, passed as restricted arguments so that the
+// compiler may read them through the scalar cache.
 //
 // FULL: every listed tile has TX x TZ elements, so patch and colour shapes are
 // compile-time constants (no run-time divisions in the index arithmetic), and the
