@@ -137,6 +137,7 @@ def run_batch(args):
     for k in range(n):
         fb, fs = b.fault_state(k), members[k].fault_state()
         same &= all(np.array_equal(fb[q], fs[q]) for q in fs)
+        same &= np.array_equal(b.potency(k), members[k].potency())
         if k % 16 == 0:
             same &= all(np.array_equal(x, y) for x, y in zip(b.fields(k), members[k].fields()))
     B = batch_bytes(npoin, nelem, nflt)

@@ -76,6 +76,7 @@ _SIGS = {
     "batch_fetch_seismo": ([C.c_void_p, C.c_int, dp, ip], C.c_int),
     "batch_fetch_fault": ([C.c_void_p, C.c_int, dp, ip], C.c_int),
     "batch_fetch_fault_state": ([C.c_void_p, C.c_int, dp, dp, dp, dp, dp], C.c_int),
+    "batch_fetch_potency": ([C.c_void_p, C.c_int, dp, ip], C.c_int),
     "batch_get_step": ([C.c_void_p, ip], C.c_int),
     "batch_last_step_ms": ([C.c_void_p, dp], C.c_int),
     "batch_last_error": ([C.c_void_p], C.c_char_p),

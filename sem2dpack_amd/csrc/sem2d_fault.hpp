@@ -252,15 +252,18 @@ __global__ void k_fault_record(FaultDev F, int oix1, int oixd, int nout, double*
 
 // Potency of the displacement and velocity jumps (BC_DYNFLT_potency,
 // SRC/bc_dynflt.f90:832-855): P-SV p11, p22, p12 of d then of v, SH p13, p23 of d
-// then of v, one row of 2*(ndof+1) values.  One workgroup of 256 lanes: strided
-// partial sums, then a fixed-shape tree, so the result is run-to-run identical.
+// then of v, one row of 2*(ndof+1) values, over the fault pairs k0 .. k0+n-1.  One
+// workgroup of 256 lanes: lane t sums pairs k0+t, k0+t+256, ..., then a fixed-shape
+// tree, so the result is run-to-run identical and the same for a fault alone (k0 = 0)
+// and inside a batch (its pairs at k0 of the concatenated arrays).
 template <typename R>
-__global__ __launch_bounds__(256) void k_fault_potency(FaultDev F, const R* __restrict__ d, const R* __restrict__ v,
-                                                       size_t stride, double* __restrict__ out) {
-  __shared__ double red[6][256];
+__device__ __forceinline__ void potency_row(const FaultDev& F, int k0, int n, const R* __restrict__ d,
+                                            const R* __restrict__ v, size_t stride, double* __restrict__ out,
+                                            double (*red)[256]) {
   const int t = threadIdx.x, np = F.np;
   double s[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = t; k < np; k += 256) {
+  for (int kl = t; kl < n; kl += 256) {
+    const int k = k0 + kl;
     const int i1 = F.n1[k], i2 = F.n2[k];
     const double nx = F.nrm[k], nz = F.nrm[np + k], b = F.B[k];
     const double jd0 = (double)d[i2] - (double)d[i1], jv0 = (double)v[i2] - (double)v[i1];
@@ -302,6 +305,30 @@ __global__ __launch_bounds__(256) void k_fault_potency(FaultDev F, const R* __re
       out[3] = 0.5 * red[3][0];
     }
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_fault_potency(FaultDev F, const R* __restrict__ d, const R* __restrict__ v,
+                                                       size_t stride, double* __restrict__ out) {
+  __shared__ double red[6][256];
+  potency_row<R>(F, 0, F.np, d, v, stride, out, red);
+}
+
+// Batched potency: workgroup m writes member m's row for the state after the current
+// step, row row_shift + *itp of pot[m][cap][2*(ndof+1)] (itp: the batch's step counter
+// in device memory, which a replayed graph advances); fbase[m] .. fbase[m+1] are member
+// m's pairs in the concatenated fault arrays.
+template <typename R>
+__global__ __launch_bounds__(256) void k_batch_potency(FaultDev F, const int* __restrict__ fbase,
+                                                       const R* __restrict__ d, const R* __restrict__ v,
+                                                       size_t stride, const int* __restrict__ itp, int row_shift,
+                                                       int cap, double* __restrict__ pot) {
+  __shared__ double red[6][256];
+  const int m = blockIdx.x;
+  const int row = row_shift + *itp;
+  if (row < 0 || row >= cap) return;  // uniform over the workgroup; the host reserves the rows
+  const int k0 = fbase[m];
+  potency_row<R>(F, k0, fbase[m + 1] - k0, d, v, stride, pot + ((size_t)m * cap + row) * (2 * (F.ndof + 1)), red);
 }
 
 #pragma clang fp contract(on)

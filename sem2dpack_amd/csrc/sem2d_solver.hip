@@ -1648,6 +1648,11 @@ struct sem2d_batch {
   int* it_dev = nullptr;
   int* flags = nullptr;
   int it = 0;
+  // potency rows: pot[member][pot_cap][2*(ndof+1)]; the row of the state after step s
+  // is pot_shift + s (pot_shift + 1 rows precede the batch's first step)
+  int* fbase_dev = nullptr;  // [n+1] first fault pair of each member
+  double* pot = nullptr;
+  int pot_cap = 0, pot_shift = 0;
   // graph of a chunk of steps
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1737,6 +1742,32 @@ int check_members(sem2d_batch* b) {
   return 0;
 }
 
+void batch_drop_graph(sem2d_batch* b);
+
+// room for `rows` potency rows per member (outside any captured graph: a new buffer
+// drops the graph, whose launches hold the old address).  The rows already recorded
+// (pot_shift + 1 + it) move over; the first reservation takes them from the members.
+int batch_pot_reserve(sem2d_batch* b, int rows) {
+  if (rows <= b->pot_cap) return 0;
+  const int cap = std::max(rows, 2 * b->pot_cap);
+  const size_t nc = 2 * (size_t)(b->ndof + 1);
+  const int have = b->pot ? b->pot_shift + 1 + b->it : b->pot_shift + 1;
+  double* p = nullptr;
+  BHIPCHK(b, hipMalloc(&p, (size_t)b->n * cap * nc * sizeof(double)));
+  for (int m = 0; m < b->n; ++m) {
+    const double* src = b->pot ? b->pot + (size_t)m * b->pot_cap * nc : b->mem[m]->flt.pot;
+    if (have > 0)
+      BHIPCHK(b, hipMemcpyAsync(p + (size_t)m * cap * nc, src, (size_t)have * nc * sizeof(double),
+                                hipMemcpyDeviceToDevice, b->stream));
+  }
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  if (b->pot) hipFree(b->pot);
+  b->pot = p;
+  b->pot_cap = cap;
+  batch_drop_graph(b);
+  return 0;
+}
+
 // concatenate the device-resident per-member arrays that only exist on the device
 int gather_fault(sem2d_batch* b) {
   const int nd = b->ndof;
@@ -1797,6 +1828,17 @@ int gather_fault(sem2d_batch* b) {
   F.nr_fail = b->flags;
   b->X.fault_on = true;
   b->X.F = F;
+  // potency: member pair ranges, and the rows recorded before the batch (one per member
+  // at fault set-up; equal for all members, none of which has been stepped)
+  std::vector<int> fb(b->fault_base);
+  fb.push_back(np);
+  if ((rc = bupload(b, &b->fbase_dev, fb))) return rc;
+  const int r0 = b->mem[0]->flt.pot_rows;
+  for (int m = 0; m < b->n; ++m)
+    if (b->mem[m]->flt.pot_rows != r0) return berr(b, 3, "member %d: potency rows differ from member 0's", m);
+  if (r0 < 1) return berr(b, 3, "batch: members have no potency row of step 0");
+  b->pot_shift = r0 - 1;
+  if ((rc = batch_pot_reserve(b, r0))) return rc;
   // fault records: output nodes oix1:oixn:oixd of every member, member after member
   std::vector<int> fidx;
   b->fout_base.assign(b->n, 0);
@@ -1833,6 +1875,9 @@ void launch_batch_steps(sem2d_batch* b, int nsteps) {
   const R c3 = (R)(b->gamma * b->dt), c4 = (R)(b->beta * b->dt * b->dt);
   for (int s = 0; s < nsteps; ++s) {
     launch_structured_step<R>(b->X, 0, dt, c1, c2, c3, c4);
+    if (b->X.fault_on)
+      k_batch_potency<R><<<b->n, 256, 0, b->stream>>>(b->X.F, b->fbase_dev, (const R*)b->X.d, (const R*)b->X.v,
+                                                      b->fstride, b->it_dev, b->pot_shift, b->pot_cap, b->pot);
     launch_batch_record<R>(b);
     k_tick<<<1, 64, 0, b->stream>>>(b->it_dev);
   }
@@ -1868,6 +1913,10 @@ void batch_set_config(sem2d_batch* b, int threads, int per_cu) {
 
 int batch_run(sem2d_batch* b, int nsteps, bool use_graph) {
   const int chunk = std::min(nsteps, 64);
+  if (b->X.fault_on) {
+    int rc = batch_pot_reserve(b, b->pot_shift + 1 + b->it + nsteps);
+    if (rc) return rc;
+  }
   BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
   int done = 0;
   if (use_graph && chunk > 0) {
@@ -2292,6 +2341,19 @@ int sem2d_batch_fetch_fault_state(sem2d_batch* b, int m, double* D, double* V, d
   return 0;
 }
 
+int sem2d_batch_fetch_potency(sem2d_batch* b, int m, double* out, int* nrows) {
+  if (!b) return 1;
+  if (m < 0 || m >= b->n) return berr(b, 3, "batch_fetch_potency: bad member %d", m);
+  if (!b->X.fault_on) return berr(b, 4, "batch_fetch_potency: no fault");
+  const int rows = b->pot_shift + 1 + b->it;
+  const size_t nc = 2 * (size_t)(b->ndof + 1);
+  if (out && rows)
+    BHIPCHK(b, hipMemcpy(out, b->pot + (size_t)m * b->pot_cap * nc, (size_t)rows * nc * sizeof(double),
+                         hipMemcpyDeviceToHost));
+  if (nrows) *nrows = rows;
+  return 0;
+}
+
 int sem2d_batch_get_step(sem2d_batch* b, int* it) {
   if (!b || !it) return 1;
   *it = b->it;
@@ -2311,6 +2373,7 @@ void sem2d_batch_destroy(sem2d_batch* b) {
   if (b->stream) hipStreamSynchronize(b->stream);
   batch_drop_graph(b);
   for (void* p : b->allocs) hipFree(p);
+  if (b->pot) hipFree(b->pot);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
   if (b->stream) hipStreamDestroy(b->stream);

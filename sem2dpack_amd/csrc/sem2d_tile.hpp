@@ -354,7 +354,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
   __shared__ R Gm[TX * TZ][3];
   // moduli of each thread's point tasks (compact mode), staged from the prefetch
   // registers by the predictor
-  __shared__ LV ML[MODE == COEF_COMPACT ? PB_MAXB : 1][ETHREADS];
+  __shared__ LV ML[MODE == COEF_COMPACT ? PB_MAXB : 1][MODE == COEF_COMPACT ? ETHREADS : 1];
 
   const int nwg = gridDim.x;
   const int bid = blockIdx.x;

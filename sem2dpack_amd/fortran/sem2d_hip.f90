@@ -47,7 +47,8 @@ module sem2d_hip
             sem2d_adjoint_timing, sem2d_adjoint_plan, sem2d_adjoint_release, &
             sem2d_batch_create, sem2d_batch_step, sem2d_batch_tune, sem2d_batch_set_config, &
             sem2d_batch_config, sem2d_batch_fetch_fields, sem2d_batch_fetch_seismo, &
-            sem2d_batch_fetch_fault, sem2d_batch_fetch_fault_state, sem2d_batch_get_step, &
+            sem2d_batch_fetch_fault, sem2d_batch_fetch_fault_state, sem2d_batch_fetch_potency, &
+            sem2d_batch_get_step, &
             sem2d_batch_last_step_ms, sem2d_batch_last_error, sem2d_batch_destroy
 
   interface
@@ -314,6 +315,14 @@ module sem2d_hip
       type(c_ptr), value :: b
       integer(c_int), value :: m
       real(c_double), intent(out) :: D(*), V(*), T(*), MU(*), Tstick(*)
+    end function
+
+    integer(c_int) function sem2d_batch_fetch_potency(b, m, out, nrows) bind(C, name='sem2d_batch_fetch_potency')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: m
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), intent(out) :: nrows
     end function
 
     integer(c_int) function sem2d_batch_get_step(b, it) bind(C, name='sem2d_batch_get_step')

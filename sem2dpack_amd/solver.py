@@ -334,6 +334,14 @@ class Batch:
                     "fetch_fault_state")
         return dict(D=D, V=V, T=T, MU=MU, Tstick=Ts)
 
+    def potency(self, m):
+        """Member m's potency rows, as Solver.potency: its row of step 0, then one per step."""
+        n = C.c_int()
+        self._check(self.api.batch_fetch_potency(self.h, m, None, C.byref(n)), "fetch_potency")
+        out = np.empty((n.value, 2 * (self.members[m].ndof + 1)))
+        self._check(self.api.batch_fetch_potency(self.h, m, _p(out), C.byref(n)), "fetch_potency")
+        return out
+
     def last_step_ms(self):
         ms = C.c_double()
         self._check(self.api.batch_last_step_ms(self.h, C.byref(ms)), "last_step_ms")

@@ -51,6 +51,7 @@ def assert_same(batch, solos, m, fault):
         for k in fs:
             assert np.array_equal(fb[k], fs[k]), k
         assert np.array_equal(batch.fault_records(m), solos[m].fault_records())
+        assert np.array_equal(batch.potency(m), solos[m].potency())
     if getattr(solos[m], "nrec", 0):
         assert np.array_equal(batch.seismograms(m), solos[m].seismograms())
 
