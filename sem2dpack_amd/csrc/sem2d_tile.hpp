@@ -316,7 +316,10 @@ __device__ __forceinline__ uint32_t node_index_full(const int4 gr, int X, int Z,
 // offsets from per-component bases (the caller guarantees that every component of
 // these arrays is below 4 GiB).
 template <typename R, int NDOF, int NELAST, int MODE, int NT, int EPB, bool OOP = false, bool FULL = false>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 4 : 2))) void k_tile_pipe(TileDev T, const int* __restrict__ tlist, int nlist,
+// Register budget: FP32 fields at 512 threads run two workgroups per CU (4 waves per
+// SIMD, 128 VGPRs); every other shape runs at most 2 waves per SIMD of 512-thread
+// workgroups or 3 of 256-thread ones, so the kernel may use the registers of 2.
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 && sizeof(R) == 4 ? 4 : 2))) void k_tile_pipe(TileDev T, const int* __restrict__ tlist, int nlist,
                                                             const int4* __restrict__ ttiles,
                                                             const int4* __restrict__ tgrids,
                                                             const int* __restrict__ tspc, R* __restrict__ d,
@@ -507,6 +510,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
     if (tid < tspc[t]) psp = T.spec_idx[t * MAXSPECIAL + tid];
   };
 
+  // FP64 fields at 512 threads: a thread keeps the predicted velocity and inverse mass
+  // of its patch nodes in registers from its predictor to its stores (the registers are
+  // there at two waves per SIMD); other shapes pass them through LDS.
+  constexpr bool REGV = sizeof(R) == 8 && NT == 512;
+  R vkeep[REGV ? NITER : 1][NDOF] = {}, rkeep[REGV ? NITER : 1] = {};
+
   // Stores of a finished tile (tile tp, descriptors tgp, grp) from LDS: a thread
   // stores the patch nodes it owns in the node loops (p = tnode + NT r), so the next
   // tile's predictor, run by the same thread on the same slots after this, needs no
@@ -527,12 +536,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
           for (int c = 0; c < NDOF; ++c) fld(bF[c], fbase + slot) = lv_get<R, NDOF>(Fn[p], c);
         } else {
           const auto g = nidx(grp, G, X, Z);
-          const R rm = Rm[p];
+          const R rm = REGV ? rkeep[r] : Rm[p];
 #pragma unroll
           for (int c = 0; c < NDOF; ++c) {
             const R ak = lv_get<R, NDOF>(Fn[p], c) * rm;
             fld(oA[c], g) = ak;
-            fld(oV[c], g) = lv_get<R, NDOF>(Vp[p], c) + c3 * ak;
+            fld(oV[c], g) = (REGV ? vkeep[r][c] : lv_get<R, NDOF>(Vp[p], c)) + c3 * ak;
             fld(oD[c], g) = lv_get<R, NDOF>(Ud[p], c) + c4 * ak;
           }
         }
@@ -580,15 +589,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 
     // ... then the previous tile's stores (this tile's element phase covers their
     // completion, and the next wait on a load is for loads issued after them) ...
     if (tp >= 0) store_tile(tp, tgp, grp);
-    // ... then this tile's node values into LDS
+    // ... then this tile's node values into LDS (FP64: predicted velocity and inverse
+    // mass stay in this thread's registers until its stores)
 #pragma unroll
     for (int r = 0; r < NITER; ++r) {
       const int p = tnode + NT * r;
       if (p < NPT) {
         Ud[p] = lv_make<R, NDOF>(u[r]);
-        Vp[p] = lv_make<R, NDOF>(w[r]);
+        if constexpr (REGV) {
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) vkeep[r][c] = w[r][c];
+          rkeep[r] = pm[r];
+        } else {
+          Vp[p] = lv_make<R, NDOF>(w[r]);
+          Rm[p] = pm[r];
+        }
         Fn[p] = LV{};
-        Rm[p] = pm[r];
         sflag[p] = -1;
       }
     }
