@@ -515,6 +515,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
   // there at two waves per SIMD); other shapes pass them through LDS.
   constexpr bool REGV = sizeof(R) == 8 && NT == 512;
   R vkeep[REGV ? NITER : 1][NDOF] = {}, rkeep[REGV ? NITER : 1] = {};
+  // likewise the moduli of the current tile's point tasks (compact mode): copied from
+  // the prefetch registers by the predictor instead of staged through LDS
+  constexpr bool REGM = REGV && MODE == COEF_COMPACT;
+  R lmc[REGM ? PB_MAXB : 1][NDOF] = {};
 
   // Stores of a finished tile (tile tp, descriptors tgp, grp) from LDS: a thread
   // stores the patch nodes it owns in the node loops (p = tnode + NT r), so the next
@@ -610,7 +614,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
     }
     if constexpr (MODE == COEF_COMPACT) {
       if (tid < G.nxe * G.nze * 3) Gm[tid / 3][tid - 3 * (tid / 3)] = pg;
-      if (tid < ETHREADS) {
+      if constexpr (REGM) {
+#pragma unroll
+        for (int q = 0; q < PB_MAXB; ++q)
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) lmc[q][c] = lm[q][c];
+      } else if (tid < ETHREADS) {
 #pragma unroll
         for (int q = 0; q < PB_MAXB; ++q) ML[q][tid] = lv_make<R, NDOF>(lm[q]);
       }
@@ -650,9 +659,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
           if constexpr (MODE == COEF_COMPACT) {
             const int el = lz * G.nxe + lx;
             const R g0 = Gm[el][0], g1 = Gm[el][1], g2 = Gm[el][2];
-            const LV mm = ML[q][tid];
+            R m0, m1;
+            if constexpr (REGM) {
+              m0 = lmc[q][0];
+              m1 = lmc[q][NDOF - 1];
+            } else {
+              const LV mm = ML[q][tid];
+              m0 = lv_get<R, NDOF>(mm, 0);
+              m1 = lv_get<R, NDOF>(mm, NDOF - 1);
+            }
             if constexpr (NDOF == 2) {
-              const R la = lv_get<R, NDOF>(mm, 0), mu = lv_get<R, NDOF>(mm, 1), kx = la + 2 * mu;
+              const R la = m0, mu = m1, kx = la + 2 * mu;
               A[0] = w2 * kx * g0;
               A[1] = w2 * la * g2;
               A[2] = w2 * kx * g1;
@@ -660,8 +677,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
               A[4] = w2 * mu * g2;
               A[5] = w2 * mu * g0;
             } else {
-              A[0] = w2 * lv_get<R, NDOF>(mm, 0) * g0;
-              A[1] = w2 * lv_get<R, NDOF>(mm, 0) * g1;
+              A[0] = w2 * m0 * g0;
+              A[1] = w2 * m0 * g1;
             }
           } else if constexpr (MODE == COEF_SHARED) {
 #pragma unroll
