@@ -276,8 +276,9 @@ int sem2d_batch_fetch_seismo(sem2d_batch *b, int member, double *out, int *nsamp
 int sem2d_batch_fetch_fault(sem2d_batch *b, int member, double *out, int *nsamp_taken);
 int sem2d_batch_fetch_fault_state(sem2d_batch *b, int member, double *D, double *V, double *T,
                                   double *MU, double *Tstick);
-/* Potency rows of a member, as sem2d_fetch_potency: the member's row of step 0, then one
-   row per batched step. */
+/* Potency rows of a member, as sem2d_fetch_potency (the FltXX_potency_sem2d.tab lines of
+   BC_DYNFLT_write, SRC/bc_dynflt.f90:757,832-855): the member's row of step 0, then one row
+   per batched step. */
 int sem2d_batch_fetch_potency(sem2d_batch *b, int member, double *out, int *nrows);
 int sem2d_batch_get_step(sem2d_batch *b, int *it);
 /* ms per step of the last sem2d_batch_step (HIP events on the batch stream) */
