@@ -79,7 +79,7 @@ struct StructState {
   int* spec_idx = nullptr;
   void* fpart = nullptr;
   size_t FPS = 0;
-  DefNode* defs = nullptr;
+  DefTable defs;
   int ndef = 0;
   int* part_off = nullptr;
   SrcEntry* dsrc = nullptr;
@@ -510,6 +510,28 @@ static bool fits32(size_t ngp, size_t fps, size_t ncoef, size_t rs) {
   return std::max(std::max(ngp, fps), ncoef) * rs < (size_t(1) << 32);
 }
 
+// device form of a deferred-node list (DefDev records and their side table); false when
+// a count does not fit its 8 bits
+bool pack_defs(const std::vector<DefNode>& in, std::vector<DefDev>& node, std::vector<int2>& ext) {
+  node.resize(in.size());
+  ext.clear();
+  for (size_t q = 0; q < in.size(); ++q) {
+    const DefNode& n = in[q];
+    if (n.pcnt < 0 || n.pcnt > 255 || n.scnt < 0 || n.scnt > 255 || n.acnt < 0 || n.acnt > 255) return false;
+    DefDev e;
+    e.g = n.g;
+    e.pstart = n.pstart;
+    e.cnt = n.pcnt | (n.scnt << 8) | (n.acnt << 16) | ((n.fault ? 1 : 0) << 24);
+    e.x = -1;
+    if (n.scnt > 0 || n.acnt > 0) {
+      e.x = (int)ext.size();
+      ext.push_back(make_int2(n.sstart, n.astart));
+    }
+    node[q] = e;
+  }
+  return true;
+}
+
 template <typename T>
 int prep_upload(sem2d_solver* h, T** p, const T* src, size_t n) {
   void* q = nullptr;
@@ -697,7 +719,18 @@ int prepare_structured(sem2d_solver* h) {
   if ((rc = prep_upload(h, &S.grids_dev, X.grids.data(), X.grids.size()))) return rc;
   if ((rc = prep_upload(h, &S.spec_cnt, X.spec_cnt.data(), X.spec_cnt.size()))) return rc;
   if ((rc = prep_upload(h, &S.spec_idx, X.spec_idx.data(), X.spec_idx.size()))) return rc;
-  if ((rc = prep_upload(h, &S.defs, X.defs.data(), X.defs.size()))) return rc;
+  {
+    std::vector<DefDev> dn;
+    std::vector<int2> dx;
+    if (!pack_defs(X.defs, dn, dx)) return set_err(h, 5, "prepare: more than 255 entries at a deferred node");
+    DefDev* pn = nullptr;
+    int2* px = nullptr;
+    if ((rc = prep_upload(h, &pn, dn.data(), dn.size()))) return rc;
+    if ((rc = prep_upload(h, &px, dx.data(), dx.size()))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));  // the packed copies are local
+    S.defs.node = pn;
+    S.defs.ext = px;
+  }
   if ((rc = prep_upload(h, &S.part_off, X.part_off.data(), X.part_off.size()))) return rc;
   if ((rc = prep_upload(h, &S.dsrc, X.dsrc.data(), X.dsrc.size()))) return rc;
   if ((rc = prep_upload(h, &S.dabso, X.dabso.data(), X.dabso.size()))) return rc;
@@ -752,7 +785,7 @@ struct StepDev {
   const void *rmass0 = nullptr, *coefs = nullptr, *geom = nullptr;
   void* fpart = nullptr;
   int ndef = 0;
-  const DefNode* defs = nullptr;
+  DefTable defs;
   const int* part_off = nullptr;
   const SrcEntry* dsrc = nullptr;
   const double* dabso = nullptr;
@@ -2055,7 +2088,8 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   const int ntiles = (int)A.tiles.size();
   int4 *tiles, *grids;
   int *sc, *si, *po;
-  DefNode* defs;
+  DefDev* defs;
+  int2* defx;
   SrcEntry* ds;
   double *dab, *amp;
   int* fg;
@@ -2063,7 +2097,13 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   if ((rc = bupload(b, &grids, A.grids))) return rc;
   if ((rc = bupload(b, &sc, A.spec_cnt))) return rc;
   if ((rc = bupload(b, &si, A.spec_idx))) return rc;
-  if ((rc = bupload(b, &defs, A.defs))) return rc;
+  {
+    std::vector<DefDev> dn;
+    std::vector<int2> dx;
+    if (!pack_defs(A.defs, dn, dx)) return berr(b, 5, "batch: more than 255 entries at a deferred node");
+    if ((rc = bupload(b, &defs, dn))) return rc;
+    if ((rc = bupload(b, &defx, dx))) return rc;
+  }
   if ((rc = bupload(b, &po, A.part_off))) return rc;
   if ((rc = bupload(b, &ds, A.dsrc))) return rc;
   if ((rc = bupload(b, &dab, A.dabso))) return rc;
@@ -2095,7 +2135,8 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   X.T.NGp = nb;
   X.T.FPS = (size_t)ntiles * SLOTS;
   X.ndef = (int)A.defs.size();
-  X.defs = defs;
+  X.defs.node = defs;
+  X.defs.ext = defx;
   X.part_off = po;
   X.dsrc = ds;
   X.dabso = dab;

@@ -774,6 +774,36 @@ struct DefNode {
   int fault;    // 1: split-fault node (finished by the fault kernels)
 };
 
+// Device form of a deferred-node list: 16 bytes per node (one vector load) plus a side
+// table for the few nodes with sources or absorbing terms.  cnt packs pcnt (bits 0-7),
+// scnt (bits 8-15), acnt (bits 16-23) and fault (bit 24); x is the node's
+// (sstart, astart) entry in ext, or -1 when it has neither.
+struct alignas(16) DefDev {
+  int g, pstart, cnt, x;
+};
+struct DefTable {
+  const DefDev* node = nullptr;
+  const int2* ext = nullptr;
+};
+__device__ __forceinline__ DefNode def_unpack(const DefTable& D, int q) {
+  const DefDev e = D.node[q];
+  DefNode n;
+  n.g = e.g;
+  n.pstart = e.pstart;
+  n.pcnt = e.cnt & 0xff;
+  n.scnt = (e.cnt >> 8) & 0xff;
+  n.acnt = (e.cnt >> 16) & 0xff;
+  n.fault = (e.cnt >> 24) & 1;
+  n.sstart = 0;
+  n.astart = 0;
+  if (e.x >= 0) {
+    const int2 x = D.ext[e.x];
+    n.sstart = x.x;
+    n.astart = x.y;
+  }
+  return n;
+}
+
 // Stacey terms of the absorbing entries of the deferred nodes: entry q (the q-th
 // absorbing coefficient set of the step, in boundary order) subtracts kxd[idx[t]],
 // t = off[q] .. off[q+1]-1 (its boundary elements in order), after its -C v term.
@@ -809,7 +839,7 @@ __global__ void k_stacey_pre(int nbe, const int* __restrict__ enode, const doubl
 }
 
 template <typename R, int NDOF, bool OOP = false>
-__global__ __launch_bounds__(256) void k_deferred(int ndef, const DefNode* __restrict__ defs,
+__global__ __launch_bounds__(256) void k_deferred(int ndef, DefTable defs,
                                                   const int* __restrict__ part_off, const R* __restrict__ fpart,
                                                   size_t FPS, const SrcEntry* __restrict__ src,
                                                   const double* __restrict__ ampl, int nt, int it_arg,
@@ -822,7 +852,7 @@ __global__ __launch_bounds__(256) void k_deferred(int ndef, const DefNode* __res
                                                   const R* __restrict__ ain = nullptr, StaceyRefs SR = {}) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= ndef) return;
-  const DefNode n = defs[q];
+  const DefNode n = def_unpack(defs, q);
   const size_t g = n.g;
   const int it = itp ? *itp : it_arg;  // step index: device counter under graph replay
   // out of place: a null vin / ain reads v / a (their step is then in place)
@@ -1259,7 +1289,7 @@ __device__ __forceinline__ void inject_node(const InjectPlan& P, int u, R* __res
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_adj_deferred(int ndef, const DefNode* __restrict__ defs,
+__global__ __launch_bounds__(256) void k_adj_deferred(int ndef, DefTable defs,
                                                       const int* __restrict__ part_off, const R* __restrict__ fpart,
                                                       size_t FPS, const double* __restrict__ abso,
                                                       const R* __restrict__ rmdef, R* __restrict__ ad,
@@ -1277,7 +1307,7 @@ __global__ __launch_bounds__(256) void k_adj_deferred(int ndef, const DefNode* _
     if (IP.g && q < ndef + IP.next) inject_node<R>(IP, IP.uext[q - ndef], (R*)IP.fld, GS);
     return;
   }
-  const DefNode n = defs[q];
+  const DefNode n = def_unpack(defs, q);
   const size_t g = n.g;
   double mb = 0;
 #pragma unroll
