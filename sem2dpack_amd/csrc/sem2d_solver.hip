@@ -512,7 +512,8 @@ static bool fits32(size_t ngp, size_t fps, size_t ncoef, size_t rs) {
 
 // device form of a deferred-node list (DefDev records and their side table); false when
 // a count does not fit its 8 bits
-bool pack_defs(const std::vector<DefNode>& in, std::vector<DefDev>& node, std::vector<int2>& ext) {
+bool pack_defs(const std::vector<DefNode>& in, const std::vector<int>& part_off, std::vector<DefDev>& node,
+               std::vector<int2>& ext) {
   node.resize(in.size());
   ext.clear();
   for (size_t q = 0; q < in.size(); ++q) {
@@ -526,6 +527,10 @@ bool pack_defs(const std::vector<DefNode>& in, std::vector<DefDev>& node, std::v
     if (n.scnt > 0 || n.acnt > 0) {
       e.x = (int)ext.size();
       ext.push_back(make_int2(n.sstart, n.astart));
+    } else if (n.pcnt == 2) {  // offsets inline
+      e.cnt |= 1 << 25;
+      e.pstart = part_off[n.pstart];
+      e.x = part_off[n.pstart + 1];
     }
     node[q] = e;
   }
@@ -722,7 +727,8 @@ int prepare_structured(sem2d_solver* h) {
   {
     std::vector<DefDev> dn;
     std::vector<int2> dx;
-    if (!pack_defs(X.defs, dn, dx)) return set_err(h, 5, "prepare: more than 255 entries at a deferred node");
+    if (!pack_defs(X.defs, X.part_off, dn, dx))
+      return set_err(h, 5, "prepare: more than 255 entries at a deferred node");
     DefDev* pn = nullptr;
     int2* px = nullptr;
     if ((rc = prep_upload(h, &pn, dn.data(), dn.size()))) return rc;
@@ -2100,7 +2106,7 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   {
     std::vector<DefDev> dn;
     std::vector<int2> dx;
-    if (!pack_defs(A.defs, dn, dx)) return berr(b, 5, "batch: more than 255 entries at a deferred node");
+    if (!pack_defs(A.defs, A.part_off, dn, dx)) return berr(b, 5, "batch: more than 255 entries at a deferred node");
     if ((rc = bupload(b, &defs, dn))) return rc;
     if ((rc = bupload(b, &defx, dx))) return rc;
   }

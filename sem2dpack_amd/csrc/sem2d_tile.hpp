@@ -772,12 +772,16 @@ struct DefNode {
   int astart;   // absorbing coefficients abso[astart .. astart+acnt), [ndof] each
   int acnt;
   int fault;    // 1: split-fault node (finished by the fault kernels)
+  int pinl;     // device form only: 1 when the two partial offsets are poff (pcnt == 2)
+  int poff[2];
 };
 
 // Device form of a deferred-node list: 16 bytes per node (one vector load) plus a side
 // table for the few nodes with sources or absorbing terms.  cnt packs pcnt (bits 0-7),
-// scnt (bits 8-15), acnt (bits 16-23) and fault (bit 24); x is the node's
-// (sstart, astart) entry in ext, or -1 when it has neither.
+// scnt (bits 8-15), acnt (bits 16-23), fault (bit 24) and "offsets inline" (bit 25); x
+// is the node's (sstart, astart) entry in ext, or -1 when it has neither.  A node with
+// two partial forces and no ext entry (a tile-edge node) holds its two partial offsets
+// in pstart and x instead (bit 25 set), so part_off is not read for it.
 struct alignas(16) DefDev {
   int g, pstart, cnt, x;
 };
@@ -794,9 +798,14 @@ __device__ __forceinline__ DefNode def_unpack(const DefTable& D, int q) {
   n.scnt = (e.cnt >> 8) & 0xff;
   n.acnt = (e.cnt >> 16) & 0xff;
   n.fault = (e.cnt >> 24) & 1;
+  n.pinl = (e.cnt >> 25) & 1;
   n.sstart = 0;
   n.astart = 0;
-  if (e.x >= 0) {
+  n.poff[0] = n.poff[1] = 0;
+  if (n.pinl) {
+    n.poff[0] = e.pstart;
+    n.poff[1] = e.x;
+  } else if (e.x >= 0) {
     const int2 x = D.ext[e.x];
     n.sstart = x.x;
     n.astart = x.y;
@@ -866,7 +875,12 @@ __global__ __launch_bounds__(256) void k_deferred(int ndef, DefTable defs,
     dp[c] = dk + dt * vk + c1 * ak;
     vpp[c] = vk + c2 * ak;
     R s = 0;
-    for (int k = 0; k < n.pcnt; ++k) s = s + fpart[c * FPS + part_off[n.pstart + k]];
+    if (n.pinl) {  // the loop below with pcnt = 2
+      s = s + fpart[c * FPS + n.poff[0]];
+      s = s + fpart[c * FPS + n.poff[1]];
+    } else {
+      for (int k = 0; k < n.pcnt; ++k) s = s + fpart[c * FPS + part_off[n.pstart + k]];
+    }
     f[c] = s;
   }
   for (int k = 0; k < n.scnt; ++k) {
@@ -1315,7 +1329,12 @@ __global__ __launch_bounds__(256) void k_adj_deferred(int ndef, DefTable defs,
     const R dk = ad[c * GS + g], vk = av[c * GS + g], ak = aa[c * GS + g];
     const R fb = (ak + c3 * vk) * rmdef[(size_t)c * ndef + q];
     R F = 0;
-    for (int k = 0; k < n.pcnt; ++k) F = F + fpart[c * FPS + part_off[n.pstart + k]];
+    if (n.pinl) {  // the loop below with pcnt = 2
+      F = F + fpart[c * FPS + n.poff[0]];
+      F = F + fpart[c * FPS + n.poff[1]];
+    } else {
+      for (int k = 0; k < n.pcnt; ++k) F = F + fpart[c * FPS + part_off[n.pstart + k]];
+    }
     double Cc = 0;
     for (int k = 0; k < n.acnt; ++k) Cc += abso[(size_t)(n.astart + k) * NDOF + c];
     const R vbp = (R)((double)vk - Cc * (double)fb);
