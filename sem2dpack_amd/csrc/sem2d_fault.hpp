@@ -31,7 +31,21 @@ struct FaultDev {
   double *T, *Tstick;                    // [2][np]
   double *MU, *sigma;                    // [np]
   int *nr_fail;                          // Newton iteration overflow flag
+  // structured grids: inverse mass of node 1 / node 2 of each pair [ndof][np], for the
+  // corrector that k_fault applies after the solve
+  const double *rmn1 = nullptr, *rmn2 = nullptr;
 };
+
+// Corrector of one split-fault node after the fault solve (a = f * rmass, then v and d;
+// SRC/solver.f90:78-82), in the bulk precision and outside the no-contraction region
+// below, like the corrector of every other node.
+template <typename R>
+__device__ __forceinline__ void fault_node_correct(R* d, R* v, R* a, size_t i, R rm, R c3, R c4) {
+  const R ak = a[i] * rm;
+  a[i] = ak;
+  v[i] = v[i] + c3 * ak;
+  d[i] = d[i] + c4 * ak;
+}
 
 #pragma clang fp contract(off)
 
@@ -149,9 +163,11 @@ __device__ inline double f_rsf_v(const FaultDev& F, int k, double tstick, double
   return f_nr_solver(F, k, fmin(0.0, tstick), fmax(0.0, tstick), tol, theta, tstick, sigma, Z, ok);
 }
 
-template <typename R>
-__global__ void k_fault(FaultDev F, const R* __restrict__ d, const R* __restrict__ v, R* __restrict__ f,
-                        int npoin) {
+// CORR (structured grids): the pair's two nodes are finished here too (fault_node_correct
+// with the inverse masses rmn1, rmn2); the general path corrects every node afterwards.
+template <typename R, bool CORR = false>
+__global__ void k_fault(FaultDev F, R* __restrict__ d, R* __restrict__ v, R* __restrict__ f, int npoin,
+                        R c3 = 0, R c4 = 0) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= F.np) return;
   const int np = F.np, nd = F.ndof;
@@ -234,6 +250,13 @@ __global__ void k_fault(FaultDev F, const R* __restrict__ d, const R* __restrict
     const double da = dA[c] - tf / (F.Z[c * np + k] * F.a2v);
     F.D[c * np + k] = dD[c] + F.a2d * da;
     F.V[c * np + k] = dV[c] + F.a2v * da;
+  }
+  if constexpr (CORR) {
+    for (int c = 0; c < nd; ++c) {
+      const size_t o = (size_t)c * npoin;
+      fault_node_correct<R>(d, v, f, o + i1, (R)F.rmn1[c * np + k], c3, c4);
+      fault_node_correct<R>(d, v, f, o + i2, (R)F.rmn2[c * np + k], c3, c4);
+    }
   }
 }
 

@@ -87,9 +87,6 @@ struct StructState {
   void* rmdef = nullptr;
   void* rmass0 = nullptr;
   void *coefs = nullptr, *geom = nullptr;  // cell-ordered per-element arrays
-  int* fault_g = nullptr;
-  void* fault_rm = nullptr;
-  int nfnodes = 0;
   std::vector<void*> prep_allocs;  // freed when the deferred tables are rebuilt
   std::vector<int> cell_elem;      // cell (iz*nx + ix) -> element (0-based)
   std::vector<int> def_ref;        // deferred node -> reference node (0-based)
@@ -701,6 +698,19 @@ int struct_tables(sem2d_solver* h, StructTables& X) {
 }
 
 // Upload the structured tables of the solver's own grid.
+// inverse masses of the split-fault nodes in pair order: r1 (node 1) and r2 (node 2),
+// [ndof][npairs] each, from the table's fault_rm ([ndof][node 1 of every pair, then node 2])
+void split_fault_rm(const StructTables& X, int nd, std::vector<double>& r1, std::vector<double>& r2) {
+  const size_t nf = X.fault_g.size(), np = nf / 2;
+  r1.assign((size_t)nd * np, 0.0);
+  r2.assign((size_t)nd * np, 0.0);
+  for (int c = 0; c < nd; ++c)
+    for (size_t k = 0; k < np; ++k) {
+      r1[(size_t)c * np + k] = X.fault_rm[(size_t)c * nf + k];
+      r2[(size_t)c * np + k] = X.fault_rm[(size_t)c * nf + np + k];
+    }
+}
+
 int prepare_structured(sem2d_solver* h) {
   StructState& S = h->st;
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -766,9 +776,16 @@ int prepare_structured(sem2d_solver* h) {
     S.prep_allocs.push_back(q);
     S.kxd = (double*)q;
   }
-  S.nfnodes = (int)X.fault_g.size();
-  if ((rc = prep_upload(h, &S.fault_g, X.fault_g.data(), X.fault_g.size()))) return rc;
-  if ((rc = prep_upload_bulk(h, &S.fault_rm, X.fault_rm))) return rc;
+  if (h->flt.on) {  // per-pair inverse masses for the corrector inside k_fault
+    std::vector<double> r1, r2;
+    split_fault_rm(X, h->ndof, r1, r2);
+    double *p1 = nullptr, *p2 = nullptr;
+    if ((rc = prep_upload(h, &p1, r1.data(), r1.size()))) return rc;
+    if ((rc = prep_upload(h, &p2, r2.data(), r2.size()))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));  // the vectors are local
+    h->flt.dev.rmn1 = p1;
+    h->flt.dev.rmn2 = p2;
+  }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   S.ready = true;
   return 0;
@@ -800,9 +817,6 @@ struct StepDev {
   int ampl_nt = 0;
   bool fault_on = false;
   FaultDev F{};
-  int nfnodes = 0;
-  const int* fault_g = nullptr;
-  const void* fault_rm = nullptr;
   const int* itp = nullptr;  // device step counter (graph replay) or null: step index by value
   hipStream_t stream = nullptr;
   // out-of-place step (adjoint replay): read the state here, write it to d, v, a
@@ -850,9 +864,6 @@ StepDev step_dev(sem2d_solver* h) {
   X.ampl_nt = h->ampl_nt;
   X.fault_on = h->flt.on;
   if (h->flt.on) X.F = h->flt.dev;
-  X.nfnodes = S.nfnodes;
-  X.fault_g = S.fault_g;
-  X.fault_rm = S.fault_rm;
   X.stream = h->stream;
   return X;
 }
@@ -945,13 +956,7 @@ void launch_structured_step(const StepDev& X, int it, R dt, R c1, R c2, R c3, R 
                                                 X.ampl, X.ampl_nt, it, X.itp, X.dabso, (const R*)X.rmdef, (R*)X.d,
                                                 (R*)X.v, (R*)X.a, X.fstride, dt, c1, c2, c3, c4);
   if (X.fault_on) {
-    k_fault<R><<<grid_for(X.F.np, 64), 64, 0, X.stream>>>(X.F, (const R*)X.d, (const R*)X.v, (R*)X.a, X.fstride);
-    if (X.ndof == 2)
-      k_fault_correct<R, 2><<<grid_for(X.nfnodes, 64), 64, 0, X.stream>>>(
-          X.nfnodes, X.fault_g, (const R*)X.fault_rm, (R*)X.d, (R*)X.v, (R*)X.a, X.fstride, c3, c4);
-    else
-      k_fault_correct<R, 1><<<grid_for(X.nfnodes, 64), 64, 0, X.stream>>>(
-          X.nfnodes, X.fault_g, (const R*)X.fault_rm, (R*)X.d, (R*)X.v, (R*)X.a, X.fstride, c3, c4);
+    k_fault<R, true><<<grid_for(X.F.np, 64), 64, 0, X.stream>>>(X.F, (R*)X.d, (R*)X.v, (R*)X.a, X.fstride, c3, c4);
   }
 }
 
@@ -2021,7 +2026,7 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   }
   // member tables, concatenated
   StructTables A;
-  std::vector<double> rmdef_c[2], frm_c[2];
+  std::vector<double> rmdef_c[2], frm_c[2], frm2_c[2];  // fault: inverse mass of node 1 / 2
   std::vector<SrcEntry> dsrc;
   std::vector<double> ampl;
   int ampl_nt = 0, nsrc = 0;
@@ -2065,9 +2070,17 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
     const size_t ndef = X.defs.size(), nf = X.fault_g.size();
     for (int c = 0; c < nd; ++c) {
       rmdef_c[c].insert(rmdef_c[c].end(), X.rmdef.begin() + c * ndef, X.rmdef.begin() + (c + 1) * ndef);
-      frm_c[c].insert(frm_c[c].end(), X.fault_rm.begin() + c * nf, X.fault_rm.begin() + (c + 1) * nf);
     }
-    for (int g : X.fault_g) A.fault_g.push_back(g + (int)nb);
+    (void)nf;
+    if (h->flt.on) {  // per-pair inverse masses, in the batch's pair order
+      std::vector<double> r1, r2;
+      split_fault_rm(X, nd, r1, r2);
+      const size_t npm = r1.size() / nd;
+      for (int c = 0; c < nd; ++c) {
+        frm_c[c].insert(frm_c[c].end(), r1.begin() + c * npm, r1.begin() + (c + 1) * npm);
+        frm2_c[c].insert(frm2_c[c].end(), r2.begin() + c * npm, r2.begin() + (c + 1) * npm);
+      }
+    }
     // source amplitude rows, padded to the common length
     b->src_base[m] = nsrc;
     for (int s = 0; s < h->nsrc; ++s) {
@@ -2083,7 +2096,6 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   b->fstride = nb;
   for (int c = 0; c < nd; ++c) {
     A.rmdef.insert(A.rmdef.end(), rmdef_c[c].begin(), rmdef_c[c].end());
-    A.fault_rm.insert(A.fault_rm.end(), frm_c[c].begin(), frm_c[c].end());
   }
   StepDev& X = b->X;
   X.ndof = nd;
@@ -2098,7 +2110,6 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   int2* defx;
   SrcEntry* ds;
   double *dab, *amp;
-  int* fg;
   if ((rc = bupload(b, &tiles, A.tiles))) return rc;
   if ((rc = bupload(b, &grids, A.grids))) return rc;
   if ((rc = bupload(b, &sc, A.spec_cnt))) return rc;
@@ -2115,10 +2126,8 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   if ((rc = bupload(b, &dab, A.dabso))) return rc;
   if (ampl.empty()) ampl.assign(1, 0.0);
   if ((rc = bupload(b, &amp, ampl))) return rc;
-  if ((rc = bupload(b, &fg, A.fault_g))) return rc;
-  void *rmdef, *frm;
+  void* rmdef;
   if ((rc = bupload_bulk(b, &rmdef, A.rmdef))) return rc;
-  if ((rc = bupload_bulk(b, &frm, A.fault_rm))) return rc;
   {
     int nfull = 0;
     const std::vector<int> list = tile_list(A.tiles, fits32(nb, (size_t)ntiles * SLOTS,
@@ -2149,9 +2158,6 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   X.rmdef = rmdef;
   X.ampl = amp;
   X.ampl_nt = ampl_nt;
-  X.nfnodes = (int)A.fault_g.size();
-  X.fault_g = fg;
-  X.fault_rm = frm;
   // bulk arrays: fields, inverse mass, coefficients, partial forces
   char *d, *v, *a, *rm0, *co, *ge = nullptr, *fp;
   if ((rc = balloc(b, &d, nd * nb * rs))) return rc;
@@ -2196,7 +2202,19 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   BHIPCHK(b, hipMemcpy(b->flags, zero4, sizeof zero4, hipMemcpyHostToDevice));
   BHIPCHK(b, hipMemcpy(b->it_dev, zero4, sizeof(int), hipMemcpyHostToDevice));
   X.itp = b->it_dev;
-  if (h0->flt.on && (rc = gather_fault(b))) return rc;
+  if (h0->flt.on) {
+    if ((rc = gather_fault(b))) return rc;
+    std::vector<double> r1, r2;
+    for (int c = 0; c < nd; ++c) {
+      r1.insert(r1.end(), frm_c[c].begin(), frm_c[c].end());
+      r2.insert(r2.end(), frm2_c[c].begin(), frm2_c[c].end());
+    }
+    double *p1, *p2;
+    if ((rc = bupload(b, &p1, r1))) return rc;
+    if ((rc = bupload(b, &p2, r2))) return rc;
+    b->X.F.rmn1 = p1;
+    b->X.F.rmn2 = p2;
+  }
   // receivers: members with stations, member after member
   std::vector<int> rn;
   b->rec_base.assign(n, 0);

@@ -923,23 +923,6 @@ __global__ __launch_bounds__(256) void k_deferred(int ndef, DefTable defs,
   }
 }
 
-// corrector on the split-fault nodes after the fault solve: a = f*rmass, v, d update
-template <typename R, int NDOF>
-__global__ void k_fault_correct(int n, const int* __restrict__ g, const R* __restrict__ rm, R* __restrict__ d,
-                                R* __restrict__ v, R* __restrict__ a, size_t GS, R c3, R c4) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const size_t k = g[q];
-#pragma unroll
-  for (int c = 0; c < NDOF; ++c) {
-    const R ak = a[c * GS + k] * rm[(size_t)c * n + q];
-    a[c * GS + k] = ak;
-    v[c * GS + k] = v[c * GS + k] + c3 * ak;
-    d[c * GS + k] = d[c * GS + k] + c4 * ak;
-  }
-}
-
-
 // ===========================================================================
 // Adjoint (reverse-mode) step on structured grids, P-SV with compact coefficients.
 //
