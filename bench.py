@@ -205,7 +205,7 @@ def run_adjoint(args):
     for fp32 in (False, True):
         m = adjoint.SeismogramModel(path, bulk_fp32=fp32)
         g_s = solver.Solver(pb, bulk_fp32=fp32)
-        m._state = (pb, g_s, tuple(np.ascontiguousarray(a) for a in back))
+        m._state = adjoint.ForwardState(pb, g_s, tuple(np.ascontiguousarray(a) for a in back))
         t2 = time.perf_counter()
         g_s.adjoint_init(nt, 0)
         plan = g_s.adjoint_plan()

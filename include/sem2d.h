@@ -150,7 +150,9 @@ int sem2d_set_receivers_interp(sem2d_solver *h, int nrec, const int *elem, const
 
 /* Advance nsteps Newmark steps: replaces the loop body of SRC/main.f90:51-99
    (solve -> solve_Newmark, SRC/solver.f90:42-84, plus REC_store and BC_write records).
-   May return before the device finishes; fetch calls synchronise. */
+   Returns once the device has finished the nsteps steps (their mean time is then
+   available from sem2d_last_step_ms); a nonzero code reports a launch failure or the
+   Newton solver's iteration overflow (the reference's IO_abort in nr_solver). */
 int sem2d_step(sem2d_solver *h, int nsteps);
 
 /* Fields [ndof][npoin] (any pointer may be NULL). */

@@ -12,15 +12,20 @@ LIB = os.path.join(HERE, "lib", "libsem2d_oracle.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
 
-LIB_ULP = os.path.join(HERE, "lib", "libsem2d_oracle_ulp.so")
-
-
-def load(ulp_perturbed=False):
-    """The oracle; ``ulp_perturbed=True``: the build whose libm results are moved by one
-    ulp (to measure the reference arithmetic's conditioning)."""
+def load(variant="", ulp_perturbed=False):
+    """The oracle.  ``variant`` names a build of the perturbation ensemble
+    (``build_oracle.ENSEMBLE``); ``ulp_perturbed=True`` is the ``ulp_p`` build."""
     from sem2dpack_amd import capi
-    lib = LIB_ULP if ulp_perturbed else LIB
+    from . import build_oracle
+    if ulp_perturbed:
+        variant = "ulp_p"
+    lib = build_oracle.lib_path(variant)
     if not os.path.exists(lib):
-        from . import build_oracle
         build_oracle.build()
     return capi.bind(ctypes.CDLL(lib), prefix="oracle_")
+
+
+def ensemble():
+    """[(name, api)] for every perturbed build of the oracle."""
+    from . import build_oracle
+    return [(v, load(v)) for v in build_oracle.ENSEMBLE]

@@ -37,21 +37,31 @@
 
 #include "../include/sem2d.h"
 
+/* Perturbed builds (test infrastructure for measuring the conditioning of the
+   reference arithmetic; the plain build defines none of these):
+   ORACLE_ULP_PERTURB=+1 / -1  every exp/log/sinh/cosh/asinh result moved by one ulp,
+                               in opposite directions for the two signs;
+   ORACLE_ORDER=1              element forces assembled in reverse element order;
+   ORACLE_ORDER=2              element forces assembled colour by colour (greedy
+                               first-fit colouring of the elements in element order,
+                               each colour in element order), the summation order of a
+                               colour-scheduled GPU assembly;
+   a build with -ffp-contract=fast -mfma contracts products and sums into FMAs. */
 #ifdef ORACLE_ULP_PERTURB
-/* Conditioning probe (a second, test-only build): every exp, log, sinh, cosh and asinh
-   result is moved by one ulp (relative 2^-52).  How far the outputs then move measures
-   the reference's own sensitivity to libm round-off, which bounds what any other
-   implementation of the same arithmetic can be held to. */
-static double ulp_exp(double x) { return exp(x) * (1.0 + 0x1p-52); }
-static double ulp_log(double x) { return log(x) * (1.0 - 0x1p-52); }
-static double ulp_sinh(double x) { return sinh(x) * (1.0 + 0x1p-52); }
-static double ulp_cosh(double x) { return cosh(x) * (1.0 - 0x1p-52); }
-static double ulp_asinh(double x) { return asinh(x) * (1.0 + 0x1p-52); }
+#define ULP_S ((double)(ORACLE_ULP_PERTURB) * 0x1p-52)
+static double ulp_exp(double x) { return exp(x) * (1.0 + ULP_S); }
+static double ulp_log(double x) { return log(x) * (1.0 - ULP_S); }
+static double ulp_sinh(double x) { return sinh(x) * (1.0 + ULP_S); }
+static double ulp_cosh(double x) { return cosh(x) * (1.0 - ULP_S); }
+static double ulp_asinh(double x) { return asinh(x) * (1.0 + ULP_S); }
 #define exp ulp_exp
 #define log ulp_log
 #define sinh ulp_sinh
 #define cosh ulp_cosh
 #define asinh ulp_asinh
+#endif
+#ifndef ORACLE_ORDER
+#define ORACLE_ORDER 0
 #endif
 
 #define NG 5
@@ -111,6 +121,8 @@ struct oracle_solver {
   int *rnode;        /* [nrec], or [nrec][25] element nodes when interpolated */
   double *rw;        /* [nrec][25] interpolation weights, NULL for receivers at nodes */
   double *kv_eta;    /* [nelem][25] Kelvin-Voigt eta (dt-scaled), NULL if none */
+  int *eorder;       /* assembly order of the elements */
+  double *floc;      /* [nelem][ndof][NG][NG] element forces of the current step */
   double *seis;
   fault_t *flt;
   int it;
@@ -369,7 +381,11 @@ static double rsf_v1(const fault_t *f, int k, double tstick, double sigma, doubl
   double dc = f->p[0][k], mus = f->p[1][k], a = f->p[2][k], b = f->p[3][k], vs = f->p[4][k];
   *ok = 1;
   if (f->kind == 1) {
-    double mnd = mus - b * theta / (theta + dc); /* rsf_mu_no_direct kind 1 */
+    /* rsf_mu_no_direct reads the state stored in the friction law (f%theta, the state
+       before this step: SRC/bc_dynflt_rsf.f90:189-196,338), not the updated estimate
+       passed in; the kind-1 branch does not use its theta argument */
+    double th0 = f->theta[k];
+    double mnd = mus - b * th0 / (th0 + dc);
     double v = (tstick + sigma * mnd) / Z;
     double tmp = v - vs + sigma * a / Z;
     v = 0.5 * (tmp + sqrt(tmp * tmp + 4.0 * v * vs));
@@ -560,29 +576,78 @@ static void kd2(const oracle_solver *h, const double *a, double U[2][NG][NG], do
 #undef A
 }
 
+/* element visiting order of the assembly: the reference's (element order), or one of
+   the perturbed orders described at the top of this file */
+static void make_order(oracle_solver *h) {
+  const int ne = h->nelem;
+  h->eorder = (int *)malloc(sizeof(int) * (size_t)ne);
+  if (ORACLE_ORDER == 1) {
+    for (int e = 0; e < ne; ++e) h->eorder[e] = ne - 1 - e;
+  } else if (ORACLE_ORDER == 2) {
+    unsigned long long *used = (unsigned long long *)calloc((size_t)h->npoin, sizeof *used);
+    unsigned char *col = (unsigned char *)malloc((size_t)ne);
+    int ncol = 0;
+    for (int e = 0; e < ne; ++e) {
+      unsigned long long m = 0;
+      const int *ib = h->ibool + (size_t)e * NG * NG;
+      for (int k = 0; k < NG * NG; ++k) m |= used[ib[k]];
+      int c = 0;
+      while (m >> c & 1ull) ++c;
+      col[e] = (unsigned char)c;
+      if (c + 1 > ncol) ncol = c + 1;
+      for (int k = 0; k < NG * NG; ++k) used[ib[k]] |= 1ull << c;
+    }
+    int q = 0;
+    for (int c = 0; c < ncol; ++c)
+      for (int e = 0; e < ne; ++e)
+        if (col[e] == c) h->eorder[q++] = e;
+    free(used);
+    free(col);
+  } else {
+    for (int e = 0; e < ne; ++e) h->eorder[e] = e;
+  }
+}
+
 /* compute_Fint (SRC/solver.f90:273-320); v != NULL adds the Kelvin-Voigt term
-   d + eta v in the viscous elements (MAT_KV_add_etav, SRC/mat_kelvin_voigt.f90:139-150) */
-static void fint(const oracle_solver *h, const double *d, double *f, const double *v) {
+   d + eta v in the viscous elements (MAT_KV_add_etav, SRC/mat_kelvin_voigt.f90:139-150).
+   The element forces are independent of each other and are computed in parallel into
+   a per-element buffer; the assembly f(ibool) = f(ibool) + floc then runs sequentially
+   in the element order, so the result is the reference's whatever the thread count. */
+static void fint(oracle_solver *h, const double *d, double *f, const double *v) {
   size_t np = h->npoin;
-  memset(f, 0, sizeof(double) * np * h->ndof);
-  double U[2][NG][NG], F[2][NG][NG];
-  for (int e = 0; e < h->nelem; ++e) {
+  const int nd = h->ndof, ne = h->nelem;
+  if (!h->eorder) make_order(h);
+  if (!h->floc) h->floc = (double *)malloc(sizeof(double) * (size_t)ne * nd * NG * NG);
+  double *floc = h->floc;
+#pragma omp parallel for schedule(static) if (ne >= 2048)
+  for (int e = 0; e < ne; ++e) {
+    double U[2][NG][NG], F[2][NG][NG];
     const int *ib = h->ibool + (size_t)e * NG * NG;
-    for (int c = 0; c < h->ndof; ++c)
+    for (int c = 0; c < nd; ++c)
       for (int j = 0; j < NG; ++j)
         for (int i = 0; i < NG; ++i) U[c][j][i] = d[c * np + ib[j * NG + i]];
     if (v && h->kv_eta) {
       const double *eta = h->kv_eta + (size_t)e * NG * NG;
-      for (int c = 0; c < h->ndof; ++c)
+      for (int c = 0; c < nd; ++c)
         for (int j = 0; j < NG; ++j)
           for (int i = 0; i < NG; ++i) U[c][j][i] = U[c][j][i] + eta[j * NG + i] * v[c * np + ib[j * NG + i]];
     }
     const double *a = h->coefs + (h->shared ? 0 : (size_t)e * h->nelast * NG * NG);
     kd2(h, a, U, F);
+    double *fl = floc + (size_t)e * nd * NG * NG;
+    for (int c = 0; c < nd; ++c)
+      for (int j = 0; j < NG; ++j)
+        for (int i = 0; i < NG; ++i) fl[(c * NG + j) * NG + i] = F[c][j][i];
+  }
+  memset(f, 0, sizeof(double) * np * nd);
+  for (int q = 0; q < ne; ++q) {
+    const int e = h->eorder[q];
+    const int *ib = h->ibool + (size_t)e * NG * NG;
+    const double *fl = floc + (size_t)e * nd * NG * NG;
     for (int j = 0; j < NG; ++j)
       for (int i = 0; i < NG; ++i) {
         int k = ib[j * NG + i];
-        for (int c = 0; c < h->ndof; ++c) f[c * np + k] = f[c * np + k] + F[c][j][i];
+        for (int c = 0; c < nd; ++c) f[c * np + k] = f[c * np + k] + fl[(c * NG + j) * NG + i];
       }
   }
 }
@@ -682,6 +747,7 @@ static void one_step(oracle_solver *h) {
   double dt = h->dt, beta = h->beta, gam = h->gamma;
   double *d = h->d, *v = h->v, *a = h->a;
   double c1 = (0.5 - beta) * dt * dt, c2 = (1.0 - gam) * dt;
+#pragma omp parallel for schedule(static) if (n >= 65536)
   for (size_t k = 0; k < n; ++k) {
     d[k] = d[k] + dt * v[k] + c1 * a[k];
     v[k] = v[k] + c2 * a[k];
@@ -755,9 +821,10 @@ static void one_step(oracle_solver *h) {
     }
   }
   if (h->flt) fault_apply(h, f);
-  for (size_t k = 0; k < n; ++k) a[k] = f[k] * h->rmass[k];
   double c3 = gam * dt, c4 = beta * dt * dt;
+#pragma omp parallel for schedule(static) if (n >= 65536)
   for (size_t k = 0; k < n; ++k) {
+    a[k] = f[k] * h->rmass[k];
     v[k] = v[k] + c3 * a[k];
     d[k] = d[k] + c4 * a[k];
   }
@@ -834,7 +901,7 @@ void oracle_destroy(oracle_solver *h) {
     free(h->src[s].ixi); free(h->src[s].ieta); free(h->src[s].cxi); free(h->src[s].ceta);
     free(h->src[s].ampl);
   }
-  free(h->rnode); free(h->rw); free(h->seis); free(h->kv_eta);
+  free(h->rnode); free(h->rw); free(h->seis); free(h->kv_eta); free(h->eorder); free(h->floc);
   if (h->flt) {
     fault_t *f = h->flt;
     free(f->n1); free(f->n2); free(f->nrm); free(f->B); free(f->invM1); free(f->invM2);

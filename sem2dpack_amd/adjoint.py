@@ -52,26 +52,36 @@ class SeismogramModel:
         return builder.build(self.par, nsteps=self.nsteps, fname_dir=self.fname_dir, per_element_coefs=False,
                              elem_props=(rho, lam, mu))
 
-    def forward(self, rho, lam, mu):
-        """Seismograms [nsamp][ndof][nrec] (FP64) of the model."""
-        rho, lam, mu = (np.ascontiguousarray(a, dtype=np.float64) for a in (rho, lam, mu))
+    def run_forward(self, rho, lam, mu):
+        """(seismograms [nsamp][ndof][nrec] FP64, state): a forward run whose state
+        (problem, solver with its stored forward history or checkpoints, model arrays)
+        stays alive, independent of every other run, until ``state.close()`` or until the
+        state is garbage-collected."""
+        rho, lam, mu = (np.ascontiguousarray(a, dtype=np.float64).copy() for a in (rho, lam, mu))
         pb = self.problem(rho, lam, mu)
         if pb.ndof != 2 or pb.receivers is None:
             raise ValueError("SeismogramModel needs a P-SV input with receivers")
-        self.close()  # the previous model's stored states go before the new ones are planned
         s = solver.Solver(pb, bulk_fp32=self.bulk_fp32)
         s.adjoint_init(pb.nt, self.seg)
         s.adjoint_forward()
-        seis = s.seismograms()
-        self._state = (pb, s, (rho, lam, mu))
+        return s.seismograms(), ForwardState(pb, s, (rho, lam, mu))
+
+    def forward(self, rho, lam, mu):
+        """Seismograms [nsamp][ndof][nrec] (FP64) of the model; :meth:`backward` without
+        a state then differentiates at this model (the previous one's state is freed)."""
+        self.close()  # the previous model's stored states go before the new ones are planned
+        seis, self._state = self.run_forward(rho, lam, mu)
         return seis
 
-    def backward(self, gseis):
+    def backward(self, gseis, state=None):
         """(grho, glam, gmu) over the cells, for dJ/d(seismograms) = gseis, at the model
-        of the last :meth:`forward`."""
-        if self._state is None:
+        of ``state`` (a :meth:`run_forward` state; default: the last :meth:`forward`)."""
+        st = state if state is not None else self._state
+        if st is None:
             raise RuntimeError("backward before forward")
-        pb, s, (rho, lam, mu) = self._state
+        if st.solver is None:
+            raise RuntimeError("backward on a closed forward state")
+        pb, s, (rho, lam, mu) = st.pb, st.solver, st.model
         s.adjoint_backward(gseis)
         self.last_timing = s.adjoint_timing()
         glam_e, gmu_e, gmass, gabso = s.adjoint_fetch()
@@ -108,8 +118,27 @@ class SeismogramModel:
 
     def close(self):
         if self._state is not None:
-            self._state[1].close()
+            self._state.close()
             self._state = None
+
+
+class ForwardState:
+    """One forward run kept for its backward: the problem, the solver (holding the
+    forward history or checkpoints in device memory) and the model arrays."""
+
+    def __init__(self, pb, s, model):
+        self.pb, self.solver, self.model = pb, s, model
+
+    def close(self):
+        if self.solver is not None:
+            self.solver.close()
+            self.solver = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _make_function():
@@ -119,15 +148,16 @@ def _make_function():
         @staticmethod
         def forward(ctx, rho, lam, mu, model):
             arrs = [t.detach().to("cpu", torch.float64).numpy() for t in (rho, lam, mu)]
-            seis = model.forward(*arrs)
+            seis, state = model.run_forward(*arrs)
             ctx.model = model
+            ctx.state = state  # this call's own forward state: several graphs may be alive
             ctx.like = rho
             return torch.as_tensor(seis, dtype=torch.float64, device=rho.device)
 
         @staticmethod
         def backward(ctx, gseis):
             g = gseis.detach().to("cpu", torch.float64).numpy()
-            grads = ctx.model.backward(g)
+            grads = ctx.model.backward(g, ctx.state)
             like = ctx.like
             return tuple(torch.as_tensor(x, dtype=like.dtype, device=like.device) for x in grads) + (None,)
 

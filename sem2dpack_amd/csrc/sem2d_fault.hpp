@@ -1,4 +1,5 @@
 // Dynamic fault boundary: pointwise traction solve on split-node pairs.
+// asinh, sinh and cosh follow the C library's decomposition (sem2d_libm.hpp).
 //
 // One lane per fault node pair.  All fault arithmetic is FP64 whatever the bulk
 // type, and floating-point contraction is disabled so that, given the same
@@ -12,6 +13,8 @@
 //   Coulomb normal law   SRC/bc_dynflt_normal.f90:118-136 (kind 1)
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include "sem2d_libm.hpp"
 
 namespace sem2d {
 
@@ -63,8 +66,8 @@ __device__ inline double f_rsf_mu(const FaultDev& F, int k, double v) {
   switch (F.kind) {
     case 1: return mus + a * fabs(v) / (fabs(v) + vs) - b * th / (th + dc);
     case 2:
-    case 3: return a * asinh(fabs(v) / (2.0 * vs) * exp((mus + b * log(vs * th / dc)) / a));
-    default: return a * asinh(fabs(v) / (2.0 * vs) * exp((mus + b * log(F.p5[k] * th / dc + 1)) / a));
+    case 3: return a * libm::asinh(fabs(v) / (2.0 * vs) * exp((mus + b * log(vs * th / dc)) / a));
+    default: return a * libm::asinh(fabs(v) / (2.0 * vs) * exp((mus + b * log(F.p5[k] * th / dc + 1)) / a));
   }
 }
 
@@ -94,9 +97,9 @@ __device__ inline void f_fric(const FaultDev& F, int k, double theta, double tau
   const double dc = F.p0[k], mus = F.p1[k], a = F.p2[k], b = F.p3[k], vs = F.p4[k];
   double tmp = (F.kind == 4) ? mus + b * log(F.p5[k] * theta / dc + 1.0) : mus + b * log(vs * theta / dc);
   tmp = 2.0 * vs * exp(-tmp / a);
-  v = sinh(tau / (-sigma * a)) * tmp;
+  v = libm::sinh(tau / (-sigma * a)) * tmp;
   fv = tstick - Z * v - tau;
-  const double dv = cosh(tau / (-sigma * a)) * tmp / (-sigma * a);
+  const double dv = libm::cosh(tau / (-sigma * a)) * tmp / (-sigma * a);
   dfv = -Z * dv - 1.0;
 }
 
@@ -152,7 +155,10 @@ __device__ inline double f_rsf_v(const FaultDev& F, int k, double tstick, double
   ok = true;
   if (F.kind == 1) {
     const double dc = F.p0[k], mus = F.p1[k], a = F.p2[k], b = F.p3[k], vs = F.p4[k];
-    const double mnd = mus - b * theta / (theta + dc);
+    // rsf_mu_no_direct reads the stored state (the state before this step), not the
+    // updated estimate: SRC/bc_dynflt_rsf.f90:189-196,338
+    const double th0 = F.theta[k];
+    const double mnd = mus - b * th0 / (th0 + dc);
     double v = (tstick + sigma * mnd) / Z;
     const double tmp = v - vs + sigma * a / Z;
     v = 0.5 * (tmp + sqrt(tmp * tmp + 4.0 * v * vs));

@@ -681,6 +681,17 @@ int struct_tables(sem2d_solver* h, StructTables& X) {
     i = j;
   }
   const size_t ndef = X.defs.size();
+  // every split-fault node must be a deferred node: k_deferred leaves those nodes
+  // uncorrected and k_fault finishes both nodes of each pair; a fault node inside a
+  // tile would be corrected by the tile kernel and again by k_fault
+  {
+    size_t nfault_def = 0;
+    for (const DefNode& n : X.defs) nfault_def += n.fault ? 1 : 0;
+    if (nfault_def != h->fault_n1.size() + h->fault_n2.size())
+      return set_err(h, 5, "structured grid: %zu of the %zu split-fault nodes lie on tile edges; the fault "
+                     "must follow a row of element edges (ezflt) for the structured path",
+                     nfault_def, h->fault_n1.size() + h->fault_n2.size());
+  }
   X.rmdef.assign((size_t)nd * ndef, 0.0);
   for (size_t q = 0; q < ndef; ++q)
     for (int c = 0; c < nd; ++c) X.rmdef[(size_t)c * ndef + q] = h->rmass_host[(size_t)c * h->npoin + def_ref[q]];

@@ -145,3 +145,28 @@ def test_torch_autograd(prob):
     J.backward()
     for t, ref in zip(ts, g):
         assert np.array_equal(t.grad.numpy(), ref)
+
+
+def test_torch_two_live_graphs(prob):
+    """Two forwards through the torch Function before any backward (two misfit terms):
+    each output keeps its own forward state, so each backward gives the gradient at its
+    own model, equal to a lone forward/backward there."""
+    import torch
+    back, true, obs = prob["back"], prob["true"], prob["obs"]
+    mid = [0.5 * (a + b) for a, b in zip(back, true)]
+    model = adjoint.SeismogramModel(prob["path"])
+    ta = [torch.tensor(p, dtype=torch.float64, requires_grad=True) for p in back]
+    tb = [torch.tensor(p, dtype=torch.float64, requires_grad=True) for p in mid]
+    sa = adjoint.seismograms(*ta, model)
+    sb = adjoint.seismograms(*tb, model)  # second forward before the first backward
+    o = torch.as_tensor(obs)
+    (0.5 * ((sb - o) ** 2).sum()).backward()
+    (0.5 * ((sa - o) ** 2).sum()).backward()
+    lone = adjoint.SeismogramModel(prob["path"])
+    ref_b = lone.backward(lone.forward(*mid) - obs)
+    lone.close()
+    for t, ref in zip(ta, prob["grad"]):
+        assert np.array_equal(t.grad.numpy(), ref)
+    for t, ref in zip(tb, ref_b):
+        assert np.array_equal(t.grad.numpy(), ref)
+    assert not any(np.array_equal(x, y) for x, y in zip(prob["grad"], ref_b))

@@ -60,17 +60,46 @@ def test_absorbing_coefficient_is_impedance():
     assert abs(left["C"][:, 1].sum() - 2670.0 * 3464.0 * w_total) / (2670.0 * 3464.0 * w_total) < 1e-13
 
 
-def test_newmark_oscillator_amplification():
-    """One free P-SV element-free check of the predictor/corrector: a single node with
-    M a = -k d advances with the closed-form explicit Newmark (beta=0, gamma=1/2) update."""
-    k, m, dt = 3.0, 2.0, 0.1
-    d, v, a = 1.0, 0.0, -k / m
-    for _ in range(10):
-        d = d + dt * v + 0.5 * dt * dt * a
-        v = v + 0.5 * dt * a
-        a = -k * d / m
-        v = v + 0.5 * dt * a
-    w2 = k / m
-    A = np.array([[1 - 0.5 * dt * dt * w2, dt], [-dt * w2 * (1 - 0.25 * dt * dt * w2), 1 - 0.5 * dt * dt * w2]])
-    ref = np.linalg.matrix_power(A, 10) @ np.array([1.0, 0.0])
-    assert np.allclose([d, v], ref, atol=1e-14)
+FREE_BOX = """&GENERAL iexec=1, ngll=5, fmax=1d0, ndof=2, title='free box', verbose='0000', ItInfo=1000 /
+&MESH_DEF method='CARTESIAN' /
+&MESH_CART xlim=0d0,4d3, zlim=0d0,3d3, nelem=8,6 /
+&MATERIAL tag=1, kind='ELAST' /
+&MAT_ELASTIC rho=2670d0, cp=6000d0, cs=3464d0 /
+&TIME kind='newmark', NbSteps=200, Courant=0.5d0 /
+"""
+
+
+def test_newmark_is_central_differences_and_conserves_energy(oracle_api):
+    """The oracle's time step on a free box (no source, no absorbing boundary, so
+    M a = -K d): explicit Newmark with beta = 0, gamma = 1/2 is the central-difference
+    scheme, so its displacement sequence alone fixes the other outputs,
+    v_n = (d_{n+1} - d_{n-1}) / (2 dt) and a_n = (d_{n+1} - 2 d_n + d_{n-1}) / dt^2,
+    and conserves the discrete energy
+    E_n = 1/2 |(d_{n+1} - d_n)/dt|_M^2 + 1/2 d_n . K d_{n+1}  (K from the oracle's -fint).
+    A wrong predictor or corrector coefficient, or a mass/force mismatch, breaks these."""
+    from sem2dpack_amd import parfile
+    pb = builder.build(parfile.parse_par(FREE_BOX))
+    assert not pb.abso and not pb.sources and pb.beta == 0.0 and pb.gamma == 0.5
+    s = solver.Solver(pb, api=oracle_api, compact=False)
+    rng = np.random.default_rng(3)
+    z = np.zeros((pb.ndof, pb.npoin))
+    s.set_fields(rng.standard_normal(z.shape) * 1e-3, z, z)
+    states = []
+    for _ in range(80):
+        s.step(1)
+        states.append(s.fields())
+    dt, M = pb.dt, 1.0 / pb.rmass
+    D = [st[0] for st in states]
+    scale_v = max(np.abs(st[1]).max() for st in states)
+    scale_a = max(np.abs(st[2]).max() for st in states)
+    for n in range(1, len(states) - 1):
+        v_cd = (D[n + 1] - D[n - 1]) / (2 * dt)
+        a_cd = (D[n + 1] - 2 * D[n] + D[n - 1]) / dt ** 2
+        assert np.abs(states[n][1] - v_cd).max() < 1e-9 * scale_v
+        assert np.abs(states[n][2] - a_cd).max() < 1e-6 * scale_a
+    E = []
+    for n in range(len(D) - 1):
+        vh = (D[n + 1] - D[n]) / dt
+        E.append(0.5 * np.sum(M * vh * vh) + 0.5 * np.sum(D[n] * -s.fint(D[n + 1])))
+    E = np.array(E)
+    assert np.abs(E - E[0]).max() < 1e-12 * abs(E[0])

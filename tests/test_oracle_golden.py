@@ -10,12 +10,11 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, case_par
+from conftest import GOLDEN, GOLDEN_CASES, case_par
 from sem2dpack_amd import builder, solver
 import refdump  # noqa: F401  (fixture readers live beside the tests)
 
-CASES = ["small_psv", "c1_psv", "sh_swf", "psv_rsf", "hete_psv", "rsf_ens", "multi_src", "interp_rec", "sh_kv", "psv_kv",
-         "psv_stacey", "mesh2d_psv", "mesh2d_curved", "mesh2d_sh", "mesh2d_flt"]
+CASES = GOLDEN_CASES
 
 
 def sha(a):
