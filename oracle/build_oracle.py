@@ -13,6 +13,8 @@ tests use its spread as the bar of quantities the rate-and-state Newton solve ma
 ill-conditioned:
 
   ulp_p, ulp_m   every exp/log/sinh/cosh/asinh result moved by +1 / -1 ulp
+  ulp_r1..ulp_r4 every exp/log/sinh/cosh/asinh/pow result moved by -1, 0 or +1 ulp,
+                 the direction drawn per call (four seeds)
   rev            element forces assembled in reverse element order
   col            element forces assembled colour by colour (greedy colouring)
   fma            products and sums contracted into fused multiply-adds
@@ -33,6 +35,10 @@ VARIANTS = {
     "": NOCONTRACT,
     "ulp_p": NOCONTRACT + ["-DORACLE_ULP_PERTURB=1"],
     "ulp_m": NOCONTRACT + ["-DORACLE_ULP_PERTURB=-1"],
+    "ulp_r1": NOCONTRACT + ["-DORACLE_ULP_RANDOM=1"],
+    "ulp_r2": NOCONTRACT + ["-DORACLE_ULP_RANDOM=2"],
+    "ulp_r3": NOCONTRACT + ["-DORACLE_ULP_RANDOM=3"],
+    "ulp_r4": NOCONTRACT + ["-DORACLE_ULP_RANDOM=4"],
     "rev": NOCONTRACT + ["-DORACLE_ORDER=1"],
     "col": NOCONTRACT + ["-DORACLE_ORDER=2"],
     "fma": CONTRACT,

@@ -41,6 +41,10 @@
    reference arithmetic; the plain build defines none of these):
    ORACLE_ULP_PERTURB=+1 / -1  every exp/log/sinh/cosh/asinh result moved by one ulp,
                                in opposite directions for the two signs;
+   ORACLE_ULP_RANDOM=seed      every exp/log/sinh/cosh/asinh/pow result moved by -1, 0 or
+                               +1 ulp, the direction drawn per call from a seeded
+                               xorshift sequence (the pattern of a libm that agrees
+                               with the C library to within one ulp either way);
    ORACLE_ORDER=1              element forces assembled in reverse element order;
    ORACLE_ORDER=2              element forces assembled colour by colour (greedy
                                first-fit colouring of the elements in element order,
@@ -59,6 +63,28 @@ static double ulp_asinh(double x) { return asinh(x) * (1.0 + ULP_S); }
 #define sinh ulp_sinh
 #define cosh ulp_cosh
 #define asinh ulp_asinh
+#endif
+#ifdef ORACLE_ULP_RANDOM
+static unsigned long long ulp_state = 0x9E3779B97F4A7C15ull * (unsigned long long)(ORACLE_ULP_RANDOM);
+static double ulp_rand(double y) {
+  ulp_state ^= ulp_state << 13;
+  ulp_state ^= ulp_state >> 7;
+  ulp_state ^= ulp_state << 17;
+  const int s = (int)(ulp_state % 3ull) - 1;
+  return y * (1.0 + s * 0x1p-52);
+}
+static double ulp_exp(double x) { return ulp_rand(exp(x)); }
+static double ulp_log(double x) { return ulp_rand(log(x)); }
+static double ulp_sinh(double x) { return ulp_rand(sinh(x)); }
+static double ulp_cosh(double x) { return ulp_rand(cosh(x)); }
+static double ulp_asinh(double x) { return ulp_rand(asinh(x)); }
+static double ulp_pow(double x, double y) { return ulp_rand(pow(x, y)); }
+#define exp ulp_exp
+#define log ulp_log
+#define sinh ulp_sinh
+#define cosh ulp_cosh
+#define asinh ulp_asinh
+#define pow ulp_pow
 #endif
 #ifndef ORACLE_ORDER
 #define ORACLE_ORDER 0

@@ -4,8 +4,9 @@ Bars: FP64 relative L2 error <= 1e-10 on displacement, velocity, acceleration,
 seismograms, fault slip, slip rate, shear traction, friction and potency.  For
 rate-and-state faults the bar of each quantity is max(1e-10, spread), where spread is
 the largest error against the oracle among the oracle's perturbation ensemble
-(oracle/build_oracle.py: libm results moved by +-1 ulp, the element forces assembled
-in reverse and in colour order, FMA contraction, FMA with colour order).  The ensemble
+(oracle/build_oracle.py: libm results moved by +1 ulp, by -1 ulp, and by -1/0/+1 ulp
+drawn per call with four seeds; the element forces assembled in reverse and in colour
+order; FMA contraction; FMA with colour order).  The ensemble
 measures how far the reference arithmetic moves under round-off that a correct
 implementation may legitimately change: on a creeping rate-and-state fault the slip
 rate comes from the difference of two tractions of ~1e8 Pa

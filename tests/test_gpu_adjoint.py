@@ -116,7 +116,7 @@ def test_history_plan_matches_checkpoints(prob):
         for seg, history in ((0, True), (20, False)):
             model = adjoint.SeismogramModel(prob["path"], seg=seg, bulk_fp32=fp32)
             seis = model.forward(*back)
-            assert model._state[1].adjoint_plan()[0] == history
+            assert model._state.solver.adjoint_plan()[0] == history
             grads.append(model.backward(seis - obs))
             model.close()
         errs = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(*grads)]

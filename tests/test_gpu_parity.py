@@ -113,8 +113,9 @@ def test_fp32_bulk_rupture_vs_fp64_oracle(name, oracle_api):
     ref.step(nt)
     g32.step(nt)
     rr, rg = ref.fault_records(), g32.fault_records()
-    errs = {"slip": rel(rg[:, 0], rr[:, 0]), "slip_rate": rel(rg[:, 1], rr[:, 1]),
-            "seismo": rel(g32.seismograms(), ref.seismograms())}
+    errs = {"slip": rel(rg[:, 0], rr[:, 0]), "slip_rate": rel(rg[:, 1], rr[:, 1])}
+    if pb.receivers is not None:
+        errs["seismo"] = rel(g32.seismograms(), ref.seismograms())
     print(f"\n  FP32 bulk {name}: " + "  ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
     conftest.report(name + " fp32", errs, {k: 1e-3 for k in errs})
     assert all(v < 1e-3 for v in errs.values()), errs
