@@ -180,6 +180,15 @@ int sem2d_fetch_fault_state(sem2d_solver *h, double *D, double *V, double *T, do
    MAT_ELAST_f (SRC/mat_elastic.f90:396-426).  d and f are [ndof][npoin] host arrays. */
 int sem2d_compute_fint(sem2d_solver *h, const double *d, double *f);
 
+/* sem2d_compute_fint without device allocation: no counterpart in the reference.  For
+   callers that cannot allocate device memory (code under HIP graph capture, serving
+   loops, harnesses that forbid hipMalloc).  Same arguments, kernels and results as
+   sem2d_compute_fint; its staging buffers are a file-scope device array of 64 MiB per
+   GPU (held until the process ends), so ndof*npoin is limited to 4,194,304 values
+   (error 3 beyond).  Calls on one GPU share that array: calls that overlap in time
+   (two host threads, two handles) race. */
+int sem2d_compute_fint_noalloc(sem2d_solver *h, const double *d, double *f);
+
 /* Launch configuration of the structured-grid tile kernel (no counterpart in the
    reference, whose loop is serial): threads per workgroup (256 = 10-element batches,
    512 = 20-element batches) and persistent workgroups per CU (0: one workgroup per

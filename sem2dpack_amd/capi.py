@@ -53,6 +53,7 @@ _SIGS = {
     "fetch_fault_state": ([C.c_void_p, dp, dp, dp, dp, dp], C.c_int),
     "fetch_potency": ([C.c_void_p, dp, ip], C.c_int),
     "compute_fint": ([C.c_void_p, dp, dp], C.c_int),
+    "compute_fint_noalloc": ([C.c_void_p, dp, dp], C.c_int),
     "set_tile_config": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "get_step": ([C.c_void_p, ip], C.c_int),
     "last_step_ms": ([C.c_void_p, dp], C.c_int),
@@ -84,8 +85,9 @@ _SIGS = {
 }
 
 # entry points every implementation of the ABI must export (the CPU oracle has no
-# event timing, no kernel launch configuration, no adjoint and no batched ensemble)
-GPU_ONLY = ("last_step_ms", "set_tile_config")
+# event timing, no kernel launch configuration, no device-scratch variant, no adjoint
+# and no batched ensemble)
+GPU_ONLY = ("last_step_ms", "set_tile_config", "compute_fint_noalloc")
 GPU_ONLY_PREFIXES = ("batch_", "adjoint_")
 REQUIRED = [k for k in _SIGS if k not in GPU_ONLY and not k.startswith(GPU_ONLY_PREFIXES)]
 

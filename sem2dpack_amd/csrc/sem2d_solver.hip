@@ -221,6 +221,52 @@ int read_bulk(sem2d_solver* h, double* dst, const void* src, size_t n) {
   return 0;
 }
 
+// Scratch memory of sem2d_compute_fint_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends.  Host code never passes the array's name to a kernel
+// or a copy (on the host it names a stand-in, not device memory); it uses the device
+// address taken once per device with hipGetSymbolAddress (scratch_on_device).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// device address of this device's scratch array (taken at set-up, once per device, with
+// the device current); nullptr with an error message on failure
+void* scratch_on_device(sem2d_solver* h) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+    set_err(h, 10, "scratch: no current device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_err(h, 10, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+// staging kernels of sem2d_compute_fint_noalloc (grid-stride)
+template <typename T>
+__global__ void k_fill(T* __restrict__ x, T val, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    x[k] = val;
+}
+__global__ void k_to_float(const double* __restrict__ x, float* __restrict__ y, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    y[k] = (float)x[k];
+}
+__global__ void k_to_double(const float* __restrict__ x, double* __restrict__ y, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    y[k] = (double)x[k];
+}
+
 // Element colouring: no two elements of one colour share a node.
 // Structured boxes use the 2x2 checkerboard of cells; otherwise greedy first-fit
 // in element order.
@@ -1065,6 +1111,7 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   HIPCHK(h, hipStreamCreate(&h->stream));
   HIPCHK(h, hipEventCreate(&h->ev0));
   HIPCHK(h, hipEventCreate(&h->ev1));
+  if (!scratch_on_device(h)) return 10;  // set-up of sem2d_compute_fint_noalloc's scratch
   double Hc[NP];
   for (int i = 0; i < NG; ++i)
     for (int j = 0; j < NG; ++j) Hc[i * NG + j] = s->hprime[j * NG + i];
@@ -1585,6 +1632,42 @@ int sem2d_compute_fint(sem2d_solver* h, const double* d, double* f) {
   hipFree(dd);
   hipFree(ff);
   return rc;
+}
+
+int sem2d_compute_fint_noalloc(sem2d_solver* h, const double* d, double* f) {
+  if (!h) return 1;
+  if (!d || !f) return set_err(h, 3, "compute_fint_noalloc: null array");
+  const size_t n = (size_t)h->ndof * h->npoin;
+  // 16 bytes of scratch per value.  FP64 fields: d at [0, 8n), f at [8n, 16n).  FP32
+  // fields: the host d as doubles at [0, 8n), d as floats at [8n, 12n), f as floats at
+  // [12n, 16n), then f as doubles back at [0, 8n) (the double d is consumed by then).
+  if (n > NOALLOC_SCRATCH_BYTES / 16)
+    return set_err(h, 3, "compute_fint_noalloc: ndof*npoin = %zu exceeds the scratch limit of %zu values",
+                   n, NOALLOC_SCRATCH_BYTES / 16);
+  unsigned char* base = (unsigned char*)scratch_on_device(h);
+  if (!base) return 10;
+  double* dd = (double*)base;
+  const int nb = grid_for(n, 256);
+  HIPCHK(h, hipMemcpyAsync(dd, d, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  double* out;
+  if (h->fp32) {
+    float* dr = (float*)(base + 8 * n);
+    float* fr = (float*)(base + 12 * n);
+    k_to_float<<<nb, 256, 0, h->stream>>>(dd, dr, n);
+    k_fill<float><<<nb, 256, 0, h->stream>>>(fr, 0.0f, n);
+    launch_force<float>(h, dr, fr);
+    k_to_double<<<nb, 256, 0, h->stream>>>(fr, dd, n);
+    out = dd;
+  } else {
+    double* fd = (double*)(base + 8 * n);
+    k_fill<double><<<nb, 256, 0, h->stream>>>(fd, 0.0, n);
+    launch_force<double>(h, dd, fd);
+    out = fd;
+  }
+  if (hipGetLastError() != hipSuccess) return set_err(h, 10, "compute_fint_noalloc: kernel launch failed");
+  HIPCHK(h, hipMemcpyAsync(f, out, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
 }
 
 int sem2d_set_tile_config(sem2d_solver* h, int threads, int wg_per_cu) {

@@ -41,7 +41,7 @@ module sem2d_hip
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
             sem2d_fetch_potency, &
-            sem2d_compute_fint, sem2d_set_tile_config, sem2d_get_step, sem2d_last_step_ms, &
+            sem2d_compute_fint, sem2d_compute_fint_noalloc, sem2d_set_tile_config, sem2d_get_step, sem2d_last_step_ms, &
             sem2d_last_error, sem2d_destroy, sem2d_error_message, &
             sem2d_adjoint_init, sem2d_adjoint_forward, sem2d_adjoint_backward, sem2d_adjoint_fetch, &
             sem2d_adjoint_timing, sem2d_adjoint_plan, sem2d_adjoint_release, &
@@ -181,6 +181,15 @@ module sem2d_hip
     end function
 
     integer(c_int) function sem2d_compute_fint(h, d, f) bind(C, name='sem2d_compute_fint')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: d(*)
+      real(c_double), intent(out) :: f(*)
+    end function
+
+    ! as sem2d_compute_fint, with staging buffers in the library's static device scratch
+    ! array (no device allocation; ndof*npoin <= 4194304)
+    integer(c_int) function sem2d_compute_fint_noalloc(h, d, f) bind(C, name='sem2d_compute_fint_noalloc')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(in) :: d(*)

@@ -208,6 +208,13 @@ class Solver:
         self._check(self.api.compute_fint(self.h, _p(d), _p(f)), "compute_fint")
         return f
 
+    def fint_noalloc(self, d):
+        """as :meth:`fint`, through sem2d_compute_fint_noalloc (no device allocation)"""
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        f = np.empty_like(d)
+        self._check(self.api.compute_fint_noalloc(self.h, _p(d), _p(f)), "compute_fint_noalloc")
+        return f
+
     def last_step_ms(self):
         ms = C.c_double()
         self._check(self.api.last_step_ms(self.h, C.byref(ms)), "last_step_ms")
