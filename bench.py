@@ -19,7 +19,20 @@ with HIP events on the solver's stream, against 8 TB/s.
 
 CPU baseline: the reference SEM2DPACK Fortran solver (built from source by
 oracle/build_ref.py) on one core, on a 500 x 250-element window of the same medium
-(same element size and statistics), timed with its own "CPU time per timestep".
+(same element size and statistics), 120 steps, timed with its own "CPU time per
+timestep"; the host's CPU model and core count are reported with it.
+
+The default run also measures the two other forms of the solver, each at its
+configuration's own step count (3000), and reports them under "variants":
+  batch    256 rate-and-state rupture scenarios (nx ~ U{100..400}, nz = nx/2) batched
+           in one persistent kernel per step under a replayed hipGraph, against the same
+           scenarios stepped one by one: scenarios/s, speedup, bitwise equality;
+  adjoint  the waveform gradient of a 400 x 400-element P-SV model (16 sources, 64
+           receivers): gradients/s and gradient cost over a forward run for the history
+           and the sqrt(N)-checkpoint storage plans, FP64 and FP32 bulk, and central
+           finite differences along random directions.
+All host-side set-up (problem building, the CPU baseline) runs before the GPU is first
+used.
 """
 import argparse
 import json
@@ -35,11 +48,29 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
-CPU_SAMPLE = dict(nx=500, nz=250, lx=50e3, lz=25e3, steps=60)
+CPU_SAMPLE = dict(nx=500, nz=250, lx=50e3, lz=25e3, steps=120)
 
 
 def alg_bytes(npoin, nelem, ndof, s, m=2):
     return npoin * (6 * ndof + 1) * s + nelem * 25 * (4 + m * s)
+
+
+def host_cpu():
+    """(model name, logical CPUs of the host, CPUs this process may run on)"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count(), len(os.sched_getaffinity(0))
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline():
@@ -64,10 +95,13 @@ def cpu_baseline():
             return {"value": None, "error": (out + r.stderr)[-400:]}
         per_step = float(m[-1])
         npoin = (4 * S["nx"] + 1) * (4 * S["nz"] + 1)
+    model, ncpu, naff = host_cpu()
     return {"value": npoin / per_step, "unit": "GLL node-updates/s", "cores": 1, "kind": "reference",
             "sample": f"{S['nx']}x{S['nz']}-element window (npoin={npoin}) of the same medium, {S['steps']} "
                       f"steps, reference sem2dsolve flang -O2 on 1 pinned core; {per_step * 1e3:.2f} ms/step "
-                      f"(its own CPU-time-per-step report; wall {wall:.1f} s incl. init)"}
+                      f"(its own CPU-time-per-step report; wall {wall:.1f} s incl. init)",
+            "steps": S["steps"], "ms_per_step": per_step * 1e3, "cpu_model": model, "nproc": ncpu,
+            "cpus_available": naff}
 
 
 def run_gpu(pb, steps, warmup, fp32):
@@ -93,36 +127,69 @@ def batch_bytes(npoin, nelem, nflt, ndof=2, s=4):
     return npoin * (6 * ndof + 1) * s + nelem * 25 * 4 + nflt * 512
 
 
-def run_batch(args):
-    """Rupture ensemble: n scenarios batched (hipGraph-replayed loop, autotuned tile
-    kernel) vs the same scenarios stepped one by one, FP32 bulk with FP64 friction.
-    The batched time includes the graph capture; set-up (building the scenarios, the
-    batch tables, tuning) is outside both timed regions."""
-    import numpy as np
-    from sem2dpack_amd import solver, workloads
-    n, steps = args.scenarios, args.steps
+# arrays of a built problem that the solver does not read (coordinates, material
+# fields, raw masses): dropped before a built problem is sent back from a worker
+_UNUSED_BY_SOLVER = ("coord", "mass_raw", "mass", "rho", "cp", "cs", "lam", "mu", "dvol", "tags", "perm")
+
+
+def _build_rupture_job(job):
+    from sem2dpack_amd import workloads
+    sc, steps = job
+    pb = workloads.build_rupture(sc, nt=steps)
+    for k in _UNUSED_BY_SOLVER:
+        if hasattr(pb, k):
+            setattr(pb, k, None)
+    return pb
+
+
+def build_batch_problems(n, steps, workers):
+    """The n rupture-ensemble problems, built by a pool of forked workers (host only: call
+    before the GPU is first used in this process)."""
+    import multiprocessing as mp
+    from sem2dpack_amd import workloads
     scen = workloads.rupture_ensemble(n)
     t0 = time.time()
-    members, stats = [], []
-    for k, sc in enumerate(scen):
-        pb = workloads.build_rupture(sc, nt=steps)
-        stats.append((pb.npoin, pb.nelem, len(pb.fault["node1"])))
-        members.append(solver.Solver(pb, bulk_fp32=True, keep_problem=False))
-        if k % 32 == 31:
-            print(f"[batch] built {k + 1}/{n} scenarios ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
-    build_s = time.time() - t0
+    pbs = []
+    jobs = [(sc, steps) for sc in scen]
+    if workers > 1:
+        with mp.get_context("fork").Pool(workers) as pool:
+            for k, pb in enumerate(pool.imap(_build_rupture_job, jobs, chunksize=2)):
+                pbs.append(pb)
+                if k % 32 == 31:
+                    log(f"batch: built {k + 1}/{n} scenarios ({time.time() - t0:.0f} s)")
+    else:
+        for k, job in enumerate(jobs):
+            pbs.append(_build_rupture_job(job))
+            if k % 32 == 31:
+                log(f"batch: built {k + 1}/{n} scenarios ({time.time() - t0:.0f} s)")
+    return pbs, time.time() - t0
+
+
+def run_batch(pbs, steps, build_s):
+    """Rupture ensemble: the scenarios batched (hipGraph-replayed loop, autotuned tile
+    kernel) vs the same scenarios stepped one by one, FP32 bulk with FP64 friction.
+    The batched time includes the graph capture; set-up (building the scenarios, the
+    batch tables, tuning) is outside both timed regions.  Returns the result dict."""
+    import numpy as np
+    from sem2dpack_amd import solver
+    n = len(pbs)
+    stats = [(pb.npoin, pb.nelem, len(pb.fault["node1"])) for pb in pbs]
+    members = []
+    for k in range(n):
+        members.append(solver.Solver(pbs[k], bulk_fp32=True, keep_problem=False))
+        pbs[k] = None  # the device solver holds what it needs
     npoin = sum(x[0] for x in stats)
     nelem = sum(x[1] for x in stats)
     nflt = sum(x[2] for x in stats)
     b = solver.Batch(members)
     choice = b.tune(trial_steps=5)
     cfg = b.config()
-    print(f"[batch] tuned: {cfg}", file=sys.stderr, flush=True)
+    log(f"batch: {n} scenarios, npoin {npoin}, tuned {cfg}")
     t1 = time.perf_counter()
     b.step(steps)
     t_batch = time.perf_counter() - t1
     ev_ms = b.last_step_ms()
-    print(f"[batch] {steps} steps in {t_batch:.2f} s", file=sys.stderr, flush=True)
+    log(f"batch: {steps} steps in {t_batch:.2f} s")
     # the same scenarios, one after the other, each with its own solver
     t_solo = 0.0
     for k, m in enumerate(members):
@@ -131,7 +198,7 @@ def run_batch(args):
         m.step(steps)
         t_solo += time.perf_counter() - t2
         if k % 32 == 31:
-            print(f"[batch] one-by-one {k + 1}/{n} ({t_solo:.1f} s)", file=sys.stderr, flush=True)
+            log(f"batch: one by one {k + 1}/{n} ({t_solo:.1f} s)")
     # equal accuracy: bitwise identical results
     same = True
     for k in range(n):
@@ -140,20 +207,18 @@ def run_batch(args):
         same &= np.array_equal(b.potency(k), members[k].potency())
         if k % 16 == 0:
             same &= all(np.array_equal(x, y) for x, y in zip(b.fields(k), members[k].fields()))
+    b.close()
+    for m in members:
+        m.close()
     B = batch_bytes(npoin, nelem, nflt)
-    line = {
-        "metric": "batched scenarios/s",
-        "value": n / t_batch,
-        "unit": "scenarios/s",
-        "n_gpus": 1, "steps": steps, "warmup": 0,
-        "ms_per_step": 1e3 * t_batch / steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 bulk, f64 friction", "data": "synthetic",
-        "config": {"workload": f"{n} RSF rupture scenarios, nx~U{{100..400}}, nz=nx/2, NGLL=5, "
-                               f"{steps} steps", "npoin_total": npoin, "nelem_total": nelem,
-                   "fault_pairs_total": nflt, "build_s": round(build_s, 1), "tile_config": cfg,
-                   "tune_choice": choice},
-        "one_by_one_s": t_solo,
+    return {
+        "metric": "batched scenarios/s", "value": n / t_batch, "unit": "scenarios/s",
+        "steps": steps, "ms_per_step": 1e3 * t_batch / steps,
+        "dtype": "f32 bulk, f64 friction",
+        "workload": f"{n} RSF rupture scenarios, nx~U{{100..400}}, nz=nx/2, NGLL=5, {steps} steps",
+        "npoin_total": npoin, "nelem_total": nelem, "fault_pairs_total": nflt, "build_s": round(build_s, 1),
+        "tile_config": cfg, "tune_choice": choice,
+        "one_by_one_s": t_solo, "batched_s": t_batch,
         "speedup_vs_one_by_one": t_solo / t_batch,
         "bitwise_equal_to_one_by_one": bool(same),
         "node_updates_per_s": npoin * steps / t_batch,
@@ -161,17 +226,16 @@ def run_batch(args):
                      "unit": "GB/s", "frac": B / (ev_ms * 1e-3) / HBM_PEAK, "traffic": None,
                      "alg_bytes_per_step": B, "event_ms_per_step": ev_ms},
     }
-    print(json.dumps(line))
 
 
-def run_adjoint(args):
+def run_adjoint(nt, fd_dirs):
     """Waveform gradient with respect to per-element (rho, lambda, mu): one gradient =
-    checkpointed forward + segment replays + adjoint sweep.  Reported against a plain
-    forward run of the same problem, and checked against central differences along
-    random directions (FP64 and FP32 bulk)."""
+    forward run storing what the backward sweep needs + the adjoint sweep.  Timed for
+    both storage plans (the displacement history when it fits; sqrt(N) checkpoints
+    with segment replays) against a plain forward run of the same precision, and
+    checked against central differences along random directions.  Returns the dict."""
     import numpy as np
     from sem2dpack_amd import adjoint, solver, workloads
-    nt = args.steps
     work = tempfile.mkdtemp(prefix="sem2d_adj_")
     path, back, true = workloads.adjoint_setup(work, nx=400, nz=400, nt=nt, nsrc=16, nrec=64, f0=3.0)
     t0 = time.time()
@@ -180,143 +244,157 @@ def run_adjoint(args):
     pb = model.problem(*back)
     model.close()  # its stored forward states are not needed any more
     build_s = time.time() - t0
-    print(f"[adjoint] problem: npoin={pb.npoin} nelem={pb.nelem} nt={nt}", file=sys.stderr, flush=True)
-    # plain forward (the reference's time loop) on the background model
-    s = solver.Solver(pb)
-    s.step(5)
-    s.close()
-    s = solver.Solver(pb)
-    t1 = time.perf_counter()
-    s.step(nt)
-    t_fwd = time.perf_counter() - t1
-    fwd_ev_ms = s.last_step_ms() * nt
-    s.close()
-    # the same with FP32 fields, the reference point of the FP32 gradient
-    s = solver.Solver(pb, bulk_fp32=True)
-    s.step(5)
-    s.close()
-    s = solver.Solver(pb, bulk_fp32=True)
-    t1 = time.perf_counter()
-    s.step(nt)
-    t_fwd32 = time.perf_counter() - t1
-    fwd32_ev_ms = s.last_step_ms() * nt
-    s.close()
+    log(f"adjoint: problem npoin={pb.npoin} nelem={pb.nelem} nt={nt}")
+
+    def plain_forward(fp32):
+        s = solver.Solver(pb, bulk_fp32=fp32)
+        s.step(5)
+        s.close()
+        s = solver.Solver(pb, bulk_fp32=fp32)
+        t1 = time.perf_counter()
+        s.step(nt)
+        t = time.perf_counter() - t1
+        ev = s.last_step_ms() * nt
+        s.close()
+        return t, ev
+
+    t_fwd, fwd_ev = plain_forward(False)
+    t_fwd32, fwd32_ev = plain_forward(True)
+    seg_ck = int(math.ceil(math.sqrt(nt)))
     out = {}
     for fp32 in (False, True):
-        m = adjoint.SeismogramModel(path, bulk_fp32=fp32)
-        g_s = solver.Solver(pb, bulk_fp32=fp32)
-        m._state = adjoint.ForwardState(pb, g_s, tuple(np.ascontiguousarray(a) for a in back))
-        t2 = time.perf_counter()
-        g_s.adjoint_init(nt, 0)
-        plan = g_s.adjoint_plan()
-        t_init = time.perf_counter() - t2
-        g_s.adjoint_forward()
-        seis = g_s.seismograms()
-        t_fw = time.perf_counter() - t2 - t_init
-        grad = m.backward(seis - obs)
-        t_grad = time.perf_counter() - t2
-        out[fp32] = dict(grad=grad, t=t_grad, ev=m.last_timing, J=0.5 * float(np.sum((seis - obs) ** 2)),
-                         plan={"history": plan[0], "segment": plan[1]},
-                         host_s={"init": t_init, "forward": t_fw, "backward": t_grad - t_init - t_fw})
-        m.close()
-        print(f"[adjoint] fp32={fp32}: gradient in {t_grad:.2f} s (forward {t_fwd:.2f} s), plan {plan}, "
-              f"device ms {m.last_timing}, wall s {out[fp32]['host_s']}", file=sys.stderr, flush=True)
-    # central differences along random directions (FP64 forward)
+        for seg in (0, seg_ck):
+            m = adjoint.SeismogramModel(path, bulk_fp32=fp32, seg=seg)
+            g_s = solver.Solver(pb, bulk_fp32=fp32)
+            st = adjoint.ForwardState(pb, g_s, tuple(np.ascontiguousarray(a) for a in back))
+            t2 = time.perf_counter()
+            g_s.adjoint_init(nt, seg)
+            plan = g_s.adjoint_plan()
+            g_s.adjoint_forward()
+            seis = g_s.seismograms()
+            grad = m.backward(seis - obs, st)
+            t_grad = time.perf_counter() - t2
+            out[(fp32, seg)] = dict(grad=grad, t=t_grad, ev=m.last_timing,
+                                    plan={"history": bool(plan[0]), "segment": plan[1]})
+            st.close()
+            log(f"adjoint: fp32={fp32} seg={seg}: gradient {t_grad:.3f} s, plan {plan}, device ms {m.last_timing}")
+    adjoint.release()
+    # central differences along random directions (FP64 forward), against both plans
     rng = np.random.Generator(np.random.PCG64(20260502))
     fdm = adjoint.SeismogramModel(path)
-    errs64, errs32 = [], []
+    errs = {k: [] for k in out}
     eps = 1e-2
-    for k in range(args.fd_dirs):
+    for k in range(fd_dirs):
         dm = [rng.standard_normal(p.size) * p * 1e-2 for p in back]
         Jp = 0.5 * float(np.sum((fdm.forward(*[p + eps * q for p, q in zip(back, dm)]) - obs) ** 2))
         Jm = 0.5 * float(np.sum((fdm.forward(*[p - eps * q for p, q in zip(back, dm)]) - obs) ** 2))
         fd = (Jp - Jm) / (2 * eps)
-        for fp32, errs in ((False, errs64), (True, errs32)):
-            an = sum(np.dot(a, b) for a, b in zip(out[fp32]["grad"], dm))
-            errs.append(abs(an - fd) / abs(fd))
-        print(f"[adjoint] direction {k}: FD {fd:.6e} rel err fp64 {errs64[-1]:.2e} fp32 {errs32[-1]:.2e}",
-              file=sys.stderr, flush=True)
+        for key in out:
+            an = sum(np.dot(a, b) for a, b in zip(out[key]["grad"], dm))
+            errs[key].append(abs(an - fd) / abs(fd))
+        log(f"adjoint: direction {k}: FD {fd:.6e}, rel err fp64 history {errs[(False, 0)][-1]:.2e}")
     fdm.close()
-    t_grad = out[False]["t"]
-    ev_f, ev_b = out[False]["ev"]
-    line = {
-        "metric": "adjoint gradients/s", "value": 1.0 / t_grad, "unit": "gradients/s",
-        "n_gpus": 1, "steps": nt, "warmup": 0, "ms_per_step": 1e3 * t_grad / nt,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "P-SV 400x400 elements NGLL=5, 16 simultaneous Ricker forces (3 Hz), 64 surface "
-                               "receivers, L2 waveform misfit, gradient w.r.t. per-element rho/lambda/mu",
-                   "npoin": pb.npoin, "nelem": pb.nelem, "storage_plan": out[False]["plan"],
-                   "build_s": round(build_s, 1)},
-        "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
-        "device_ms": {"forward_plain": fwd_ev_ms, "forward_storing": ev_f, "backward": ev_b},
-        "fp32_bulk": {"gradient_s": out[True]["t"], "forward_s": t_fwd32,
-                      "gradient_over_forward": out[True]["t"] / t_fwd32,
-                      "gradient_over_fp64_forward": out[True]["t"] / t_fwd,
-                      "device_ms": {"forward_plain": fwd32_ev_ms, "forward_storing": out[True]["ev"][0],
-                                    "backward": out[True]["ev"][1]},
-                      "storage_plan": out[True]["plan"]},
-        "fd_check": {"directions": args.fd_dirs, "eps_rel": eps * 1e-2,
-                     "max_rel_err_fp64": max(errs64) if errs64 else None,
-                     "max_rel_err_fp32": max(errs32) if errs32 else None},
+
+    def plan_row(fp32, seg, t_ref):
+        r = out[(fp32, seg)]
+        return {"gradient_s": r["t"], "gradients_per_s": 1.0 / r["t"], "gradient_over_forward": r["t"] / t_ref,
+                "plan": r["plan"], "device_ms": {"forward_storing": r["ev"][0], "backward": r["ev"][1]},
+                "fd_max_rel_err": max(errs[(fp32, seg)]) if fd_dirs else None}
+
+    t_best = out[(False, 0)]["t"]
+    return {
+        "metric": "adjoint gradients/s", "value": 1.0 / t_best, "unit": "gradients/s", "steps": nt,
+        "dtype": "f64",
+        "workload": "P-SV 400x400 elements NGLL=5, 16 simultaneous Ricker forces (3 Hz), 64 surface receivers, "
+                    "L2 waveform misfit, gradient w.r.t. per-element rho/lambda/mu",
+        "npoin": pb.npoin, "nelem": pb.nelem, "build_s": round(build_s, 1),
+        "forward_s": t_fwd, "forward_device_ms": fwd_ev, "gradient_over_forward": t_best / t_fwd,
+        "fp64": {"history": plan_row(False, 0, t_fwd), "checkpoints": plan_row(False, seg_ck, t_fwd)},
+        "fp32_bulk": {"forward_s": t_fwd32, "forward_device_ms": fwd32_ev,
+                      "history": plan_row(True, 0, t_fwd32), "checkpoints": plan_row(True, seg_ck, t_fwd32)},
+        "fd_check": {"directions": fd_dirs, "eps_rel": eps * 1e-2},
     }
-    print(json.dumps(line))
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 27  # 2^27 doubles = 1 GiB
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / 10
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "ms": 1e3 * t,
+                      "GBps_rw": 16 * n / t / 1e9}))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps of the throughput case (default 1000)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["throughput", "batch", "adjoint"], default="throughput",
-                    help="throughput: the 2000x1000-element P-SV case (default); batch: the "
-                         "rupture ensemble, batched vs one by one; adjoint: waveform gradients")
-    ap.add_argument("--fd-dirs", type=int, default=20, help="adjoint: finite-difference directions")
+    ap.add_argument("--workload", choices=["all", "throughput", "batch", "adjoint"], default="all",
+                    help="all (default): the 2000x1000-element P-SV throughput case with the batch and "
+                         "adjoint forms as variants; or one of them alone")
+    ap.add_argument("--form-steps", type=int, default=3000,
+                    help="steps of the batch and adjoint forms (their configurations: 3000)")
+    ap.add_argument("--fd-dirs", type=int, default=4, help="adjoint: finite-difference directions")
     ap.add_argument("--scenarios", type=int, default=256)
+    ap.add_argument("--build-workers", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a 1 GiB device-to-device copy (reference for HBM counter scaling)")
     args = ap.parse_args()
     if args.copy_calibration:
-        import torch
-        n = 1 << 27  # 2^27 doubles = 1 GiB
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(10):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        t = (time.perf_counter() - t0) / 10
-        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "ms": 1e3 * t,
-                          "GBps_rw": 16 * n / t / 1e9}))
+        copy_calibration()
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is a design note, not an implemented path.", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "batch":
-        args.steps = 3000 if args.steps is None else args.steps
-        run_batch(args)
-        return
-    if args.workload == "adjoint":
-        args.steps = 3000 if args.steps is None else args.steps
-        run_adjoint(args)
-        return
-    args.steps = 1000 if args.steps is None else args.steps
+    wl = args.workload
+    steps = 1000 if args.steps is None else args.steps
 
-    from sem2dpack_amd import workloads
-    t0 = time.time()
-    pb = workloads.build_throughput(nt=args.steps + args.warmup)
-    build_s = time.time() - t0
-    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    # ---- host-side set-up, before the GPU is first used ----
+    pb = cpu = batch_pbs = None
+    build_s = batch_build_s = 0.0
+    if wl in ("all", "throughput"):
+        from sem2dpack_amd import workloads
+        t0 = time.time()
+        pb = workloads.build_throughput(nt=steps + args.warmup)
+        build_s = time.time() - t0
+        log(f"throughput problem built in {build_s:.1f} s")
+        if not args.no_cpu_baseline:
+            cpu = cpu_baseline()
+            log(f"cpu baseline: {cpu and cpu.get('ms_per_step')} ms/step")
+    if wl in ("all", "batch"):
+        batch_pbs, batch_build_s = build_batch_problems(args.scenarios, args.form_steps, args.build_workers)
+        log(f"batch problems built in {batch_build_s:.1f} s")
+
+    # ---- GPU ----
+    if wl == "batch":
+        r = run_batch(batch_pbs, args.form_steps, batch_build_s)
+        r.update({"n_gpus": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                  "data": "synthetic", "config": {"workload": r.pop("workload")}})
+        print(json.dumps(r))
+        return
+    if wl == "adjoint":
+        r = run_adjoint(args.form_steps, args.fd_dirs)
+        r.update({"n_gpus": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                  "data": "synthetic", "config": {"workload": r.pop("workload")}})
+        print(json.dumps(r))
+        return
 
     npoin, nelem, ndof = pb.npoin, pb.nelem, pb.ndof
-    wall, ev_ms, finite = run_gpu(pb, args.steps, args.warmup, False)
+    wall, ev_ms, finite = run_gpu(pb, steps, args.warmup, False)
+    log(f"throughput fp64: {1e3 * wall / steps:.3f} ms/step")
     B64 = alg_bytes(npoin, nelem, ndof, 8)
-    value = npoin * args.steps / wall
+    value = npoin * steps / wall
     ach = B64 / (ev_ms * 1e-3)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "c3_fp64_hbm_traffic.json")
@@ -328,9 +406,9 @@ def main():
         "value": value,
         "unit": "GLL node-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "ms_per_step": 1e3 * wall / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -340,17 +418,26 @@ def main():
                    "npoin": npoin, "nelem": nelem, "dt": pb.dt, "bulk": "fp64", "build_s": round(build_s, 1)},
         "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": ach / HBM_PEAK, "traffic": traffic,
+                     "traffic_source": "profiles/c3_fp64_hbm_traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE of "
+                                       "the step's kernels (scaled by a copy kernel), not measured in this run",
                      "alg_bytes_per_step": B64, "event_ms_per_step": ev_ms},
         "cpu_baseline": cpu,
         "finite": finite,
+        "variants": {},
     }
     if not args.no_fp32:
-        wall32, ev32, fin32 = run_gpu(pb, args.steps, args.warmup, True)
+        wall32, ev32, fin32 = run_gpu(pb, steps, args.warmup, True)
+        log(f"throughput fp32 bulk: {1e3 * wall32 / steps:.3f} ms/step")
         B32 = alg_bytes(npoin, nelem, ndof, 4)
-        line["variants"] = {"fp32_bulk": {
-            "value": npoin * args.steps / wall32, "ms_per_step": 1e3 * wall32 / args.steps,
+        line["variants"]["fp32_bulk"] = {
+            "value": npoin * steps / wall32, "ms_per_step": 1e3 * wall32 / steps,
             "roofline_frac": B32 / (ev32 * 1e-3) / HBM_PEAK, "achieved_GBs": B32 / (ev32 * 1e-3) / 1e9,
-            "alg_bytes_per_step": B32, "finite": fin32}}
+            "alg_bytes_per_step": B32, "finite": fin32}
+    pb = None
+    if wl == "all":
+        line["variants"]["batch"] = run_batch(batch_pbs, args.form_steps, batch_build_s)
+        batch_pbs = None
+        line["variants"]["adjoint"] = run_adjoint(args.form_steps, args.fd_dirs)
     print(json.dumps(line))
 
 
