@@ -17,8 +17,10 @@ constexpr int NP = NG * NG;
 __constant__ double c_H[NP];
 __constant__ double c_W2[NP];
 
-// Coefficient storage modes (sem2d_desc.coef_mode)
-enum { COEF_PER_ELEM = 0, COEF_SHARED = 1, COEF_COMPACT = 2 };
+// Coefficient storage modes (sem2d_desc.coef_mode).  COEF_NODE is internal to the
+// structured tile kernel: the compact moduli stored once per global node (in the field
+// layout) when every element sharing a node holds the same values there.
+enum { COEF_PER_ELEM = 0, COEF_SHARED = 1, COEF_COMPACT = 2, COEF_NODE = 3 };
 
 template <typename R>
 struct Coef {

@@ -87,6 +87,9 @@ struct StructState {
   void* rmdef = nullptr;
   void* rmass0 = nullptr;
   void *coefs = nullptr, *geom = nullptr;  // cell-ordered per-element arrays
+  // compact moduli once per node in the field layout ([ndof][NS], bulk type), when every
+  // element sharing a node holds the same values there (COEF_NODE tile kernel)
+  void* nodemod = nullptr;
   std::vector<void*> prep_allocs;  // freed when the deferred tables are rebuilt
   std::vector<int> cell_elem;      // cell (iz*nx + ix) -> element (0-based)
   std::vector<int> def_ref;        // deferred node -> reference node (0-based)
@@ -863,6 +866,7 @@ struct StepDev {
   StaceyRefs SR{};
   int pipe_wgs = 0, pipe_threads = 512;
   const void *rmass0 = nullptr, *coefs = nullptr, *geom = nullptr;
+  const void* nodemod = nullptr;  // per-node moduli (COEF_NODE), or null
   void* fpart = nullptr;
   int ndef = 0;
   DefTable defs;
@@ -909,6 +913,7 @@ StepDev step_dev(sem2d_solver* h) {
   X.pipe_threads = S.pipe_threads;
   X.rmass0 = S.rmass0;
   X.coefs = S.coefs;
+  X.nodemod = S.nodemod;
   X.geom = S.geom;
   X.fpart = S.fpart;
   X.ndef = S.ndef;
@@ -931,14 +936,31 @@ void launch_pipe(const StepDev& X, const int* list, int n, R dt, R c1, R c2, R c
                  const R* vi, const R* ai) {
   if (n <= 0) return;
   const int wgs = std::min(X.pipe_wgs, (n + 7) / 8 * 8);
+  const R* co = (const R*)(MODE == COEF_NODE ? X.nodemod : X.coefs);
   if (X.pipe_threads == 256)
     k_tile_pipe<R, NDOF, NELAST, MODE, 256, 10, OOP, FULL><<<wgs, 256, 0, X.stream>>>(
-        X.T, list, n, X.T.tiles, X.T.grids, X.T.spec_cnt, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
-        (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
+        X.T, list, n, X.T.tiles, X.T.grids, X.T.spec_cnt, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, co,
+        (const R*)X.geom, (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
   else
     k_tile_pipe<R, NDOF, NELAST, MODE, 512, 20, OOP, FULL><<<wgs, 512, 0, X.stream>>>(
-        X.T, list, n, X.T.tiles, X.T.grids, X.T.spec_cnt, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
-        (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
+        X.T, list, n, X.T.tiles, X.T.grids, X.T.spec_cnt, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, co,
+        (const R*)X.geom, (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
+}
+
+// both launches (full tiles, then edge tiles) of the pipelined kernel; compact moduli
+// are read once per node when the grid has them in that form
+template <typename R, int NDOF, int NELAST, int MODE, bool OOP>
+void launch_pipe2(const StepDev& X, R dt, R c1, R c2, R c3, R c4, const R* di, const R* vi, const R* ai) {
+  if constexpr (MODE == COEF_COMPACT && (NELAST == 6 || NELAST == 2)) {
+    if (X.nodemod) {
+      launch_pipe<R, NDOF, NELAST, COEF_NODE, OOP, true>(X, X.tlist, X.nfull, dt, c1, c2, c3, c4, di, vi, ai);
+      launch_pipe<R, NDOF, NELAST, COEF_NODE, OOP, false>(X, X.tlist + X.nfull, X.npart, dt, c1, c2, c3, c4, di,
+                                                          vi, ai);
+      return;
+    }
+  }
+  launch_pipe<R, NDOF, NELAST, MODE, OOP, true>(X, X.tlist, X.nfull, dt, c1, c2, c3, c4, di, vi, ai);
+  launch_pipe<R, NDOF, NELAST, MODE, OOP, false>(X, X.tlist + X.nfull, X.npart, dt, c1, c2, c3, c4, di, vi, ai);
 }
 
 template <typename R, int NDOF, int NELAST, int MODE>
@@ -947,9 +969,7 @@ void launch_tile_t(const StepDev& X, R dt, R c1, R c2, R c3, R c4) {
     if (X.din) {  // out of place (only the adjoint replay, P-SV compact)
       const R *di = (const R*)X.din, *vi = (const R*)X.vin, *ai = (const R*)X.ain;
       if (X.pipe_wgs > 0) {
-        launch_pipe<R, NDOF, NELAST, MODE, true, true>(X, X.tlist, X.nfull, dt, c1, c2, c3, c4, di, vi, ai);
-        launch_pipe<R, NDOF, NELAST, MODE, true, false>(X, X.tlist + X.nfull, X.npart, dt, c1, c2, c3, c4, di,
-                                                        vi, ai);
+        launch_pipe2<R, NDOF, NELAST, MODE, true>(X, dt, c1, c2, c3, c4, di, vi, ai);
       } else {
         k_tile<R, NDOF, NELAST, MODE, true><<<8 * X.T.per_xcd, TILE_THREADS, 0, X.stream>>>(
             X.T, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
@@ -961,10 +981,7 @@ void launch_tile_t(const StepDev& X, R dt, R c1, R c2, R c3, R c4) {
   if (X.pipe_wgs > 0) {
     // persistent pipelined kernel: up to pipe_wgs workgroups, each on a run of tiles;
     // the full tiles and the others (grid edges) in two launches
-    launch_pipe<R, NDOF, NELAST, MODE, false, true>(X, X.tlist, X.nfull, dt, c1, c2, c3, c4, nullptr, nullptr,
-                                                     nullptr);
-    launch_pipe<R, NDOF, NELAST, MODE, false, false>(X, X.tlist + X.nfull, X.npart, dt, c1, c2, c3, c4, nullptr,
-                                                      nullptr, nullptr);
+    launch_pipe2<R, NDOF, NELAST, MODE, false>(X, dt, c1, c2, c3, c4, nullptr, nullptr, nullptr);
   } else {
     k_tile<R, NDOF, NELAST, MODE><<<8 * X.T.per_xcd, TILE_THREADS, 0, X.stream>>>(
         X.T, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom, (R*)X.fpart,
@@ -1184,6 +1201,24 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
         for (int cell = 0; cell < s->nelem; ++cell)
           for (int q = 0; q < 3; ++q) ge[(size_t)cell * 3 + q] = s->geom[(size_t)(s->elem_of_cell[cell] - 1) * 3 + q];
         if ((rc = upload_bulk(h, &S.geom, ge.data(), ge.size()))) return rc;
+        // the moduli once per node, if every element at a node holds bitwise the same
+        // values there (a medium given as a function of position)
+        const int nd = h->ndof;
+        std::vector<double> nm((size_t)nd * S.NS, 0.0);
+        std::vector<unsigned char> seen(S.NS, 0);
+        bool same = true;
+        for (int e = 0; e < s->nelem && same; ++e)
+          for (int p = 0; p < NP && same; ++p) {
+            const size_t g = S.gmap[s->ibool[(size_t)e * NP + p] - 1];
+            for (int c = 0; c < nd; ++c) {
+              const double x = s->lam_mu[((size_t)e * nd + c) * NP + p];
+              double& y = nm[(size_t)c * S.NS + g];
+              if (!seen[g]) y = x;
+              else if (std::memcmp(&x, &y, sizeof x) != 0) same = false;
+            }
+            seen[g] = 1;
+          }
+        if (same && (rc = upload_bulk(h, &S.nodemod, nm.data(), nm.size()))) return rc;
       }
     }
   }

@@ -358,6 +358,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
   // moduli of each thread's point tasks (compact mode), staged from the prefetch
   // registers by the predictor
   __shared__ LV ML[MODE == COEF_COMPACT ? PB_MAXB : 1][MODE == COEF_COMPACT ? ETHREADS : 1];
+  // COEF_NODE: the moduli of the patch nodes (coefs is then a per-node array in the
+  // field layout, component stride T.NGp), loaded with the node values
+  constexpr bool NODEM = MODE == COEF_NODE;
+  __shared__ LV Mn[NODEM ? PMAX : 1];
 
   const int nwg = gridDim.x;
   const int bid = blockIdx.x;
@@ -448,6 +452,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
   }
   const R* bRm = ubase(rmass0);
   const R* bCo = ubase(coefs);
+  const R* bMo[NDOF];
+#pragma unroll
+  for (int c = 0; c < NDOF; ++c) bMo[c] = NODEM ? ubase(coefs + c * S) : bCo;
   const R* bGe = ubase(geom);
   // cell index in the element arrays (32 bits for full tiles)
   auto cidx = [&](const int4 gr, int nx, const Geo& G, int lx, int lz) {
@@ -462,6 +469,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
   // special-node index.
   R pd[NITER][NDOF], pv[NITER][NDOF], pa[NITER][NDOF];
   R pm[NITER];
+  R pmod[NODEM ? NITER : 1][NDOF] = {};
   R lm[PB_MAXB][NDOF] = {};
   R sa[NELAST] = {}, sac[NELAST] = {};
   R pg = 0;
@@ -483,14 +491,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
           pa[r][c] = fld(bA[c], g);
         }
         pm[r] = fld(bRm, g);
+        if constexpr (NODEM) {
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) pmod[r][c] = fld(bMo[c], g);
+        }
       }
     }
-    if constexpr (MODE == COEF_COMPACT) {
+    if constexpr (MODE == COEF_COMPACT || NODEM) {
       if (tid < G.nxe * G.nze * 3) {
         const int e = tid / 3, k = tid - 3 * (tid / 3);
         const int lx = e % G.nxe, lz = e / G.nxe;
         pg = fld(bGe, cidx(gr, tg.z >> 16, G, lx, lz) * 3 + k);
       }
+    }
+    if constexpr (MODE == COEF_COMPACT) {
 #pragma unroll
       for (int q = 0; q < PB_MAXB; ++q) {
         const int cc = q / NBATCH, b0 = (q % NBATCH) * EPB;
@@ -610,7 +624,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
         }
         Fn[p] = LV{};
         sflag[p] = -1;
+        if constexpr (NODEM) Mn[p] = lv_make<R, NDOF>(pmod[r]);
       }
+    }
+    if constexpr (NODEM) {
+      if (tid < G.nxe * G.nze * 3) Gm[tid / 3][tid - 3 * (tid / 3)] = pg;
     }
     if constexpr (MODE == COEF_COMPACT) {
       if (tid < G.nxe * G.nze * 3) Gm[tid / 3][tid - 3 * (tid / 3)] = pg;
@@ -656,11 +674,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
           const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
           base = 4 * lz * PX + 4 * lx;
           R A[NELAST];
-          if constexpr (MODE == COEF_COMPACT) {
+          if constexpr (MODE == COEF_COMPACT || NODEM) {
             const int el = lz * G.nxe + lx;
             const R g0 = Gm[el][0], g1 = Gm[el][1], g2 = Gm[el][2];
             R m0, m1;
-            if constexpr (REGM) {
+            if constexpr (NODEM) {
+              const LV mm = Mn[base + jj * PX + ii];
+              m0 = lv_get<R, NDOF>(mm, 0);
+              m1 = lv_get<R, NDOF>(mm, NDOF - 1);
+            } else if constexpr (REGM) {
               m0 = lmc[q][0];
               m1 = lmc[q][NDOF - 1];
             } else {
