@@ -63,7 +63,10 @@ typedef struct sem2d_desc {
    as built by BC_DYNFLT_init, SRC/bc_dynflt.f90:231-520).  Arrays are [ncomp][npoin]. */
 typedef struct sem2d_fault_desc {
   int npoin;                          /* fault node pairs */
-  const int *node1, *node2;           /* 1-based nodes of side 1 and side 2 */
+  const int *node1, *node2;           /* 1-based nodes of side 1 and side 2; node2 = NULL
+                                         (and invM2 = NULL) for a fault with one side, on a
+                                         domain boundary (symmetry, SRC/bc_dynflt.f90:256-259,
+                                         get_jump/get_weighted_jump :692-719) */
   const double *n1;                   /* [2][npoin] normal of side 1 */
   const double *B, *invM1, *invM2, *Z;/* [ndof][npoin] */
   const double *T0;                   /* [2][npoin] initial traction (tangent, normal) */
@@ -188,6 +191,19 @@ int sem2d_compute_fint(sem2d_solver *h, const double *d, double *f);
    (error 3 beyond).  Calls on one GPU share that array: calls that overlap in time
    (two host threads, two handles) race. */
 int sem2d_compute_fint_noalloc(sem2d_solver *h, const double *d, double *f);
+
+/* Progress information of the time loop (SRC/main.f90:45,73-76: every ItInfo steps the
+   reference prints it, it*dt, maxval(abs(veloc)), maxval(abs(displ))).  sem2d_set_itinfo
+   makes sem2d_step record such a row on the device every itinfo steps (0: off, the
+   default); each row also counts the values of d, v, a that are not finite, and
+   sem2d_step returns 21 ("non-finite field values at step N") once a row has found one
+   (the reference has no such check; its progress line would show NaN).  The error stays
+   until the handle is destroyed. */
+int sem2d_set_itinfo(sem2d_solver *h, int itinfo);
+/* rows [nrows][4]: it, time, vmax, dmax */
+int sem2d_fetch_info(sem2d_solver *h, double *rows, int *nrows);
+/* the same quantities for the current fields, now; nonfinite = count over d, v, a */
+int sem2d_field_info(sem2d_solver *h, double *vmax, double *dmax, int *nonfinite);
 
 /* Launch configuration of the structured-grid tile kernel (no counterpart in the
    reference, whose loop is serial): threads per workgroup (256 = 10-element batches,

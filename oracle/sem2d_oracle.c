@@ -152,6 +152,8 @@ struct oracle_solver {
   double *seis;
   fault_t *flt;
   int it;
+  int itinfo, info_rows, info_cap, first_bad; /* progress rows every itinfo steps */
+  double *info;                               /* [rows][4]: it, time, vmax, dmax */
   char err[256];
 };
 typedef struct oracle_solver oracle_solver;
@@ -423,7 +425,13 @@ static double rsf_v1(const fault_t *f, int k, double tstick, double sigma, doubl
   return nr_solver(f, k, lo, hi, tol, theta, tstick, sigma, Z, ok);
 }
 
-/* BC_DYNFLT_potency (SRC/bc_dynflt.f90:832-855) of the field x, jump = x(node2) - x(node1):
+/* get_jump (SRC/bc_dynflt.f90:692-704): x(node2) - x(node1), or -2 x(node1) on a
+   one-sided fault */
+static double jump(const fault_t *f, const double *x, int k) {
+  return f->n2[k] >= 0 ? x[f->n2[k]] - x[f->n1[k]] : -2.0 * x[f->n1[k]];
+}
+
+/* BC_DYNFLT_potency (SRC/bc_dynflt.f90:832-855) of the field x (jumps as get_jump):
    P-SV p11, p22, p12; SH p13, p23 (sequential sums, as Fortran's SUM) */
 static void potency(const oracle_solver *h, const double *x, double *p) {
   const fault_t *f = h->flt;
@@ -431,15 +439,14 @@ static void potency(const oracle_solver *h, const double *x, double *p) {
   const double *n1x = f->nrm, *n1z = f->nrm + np, *B = f->B;
   if (h->ndof == 2) {
     double s1 = 0, s2 = 0, s3 = 0;
-    for (int k = 0; k < np; ++k) s1 += n1x[k] * (x[f->n2[k]] - x[f->n1[k]]) * B[k];
-    for (int k = 0; k < np; ++k) s2 += n1z[k] * (x[N + f->n2[k]] - x[N + f->n1[k]]) * B[k];
-    for (int k = 0; k < np; ++k)
-      s3 += (n1x[k] * (x[N + f->n2[k]] - x[N + f->n1[k]]) + n1z[k] * (x[f->n2[k]] - x[f->n1[k]])) * B[k];
+    for (int k = 0; k < np; ++k) s1 += n1x[k] * jump(f, x, k) * B[k];
+    for (int k = 0; k < np; ++k) s2 += n1z[k] * jump(f, x + N, k) * B[k];
+    for (int k = 0; k < np; ++k) s3 += (n1x[k] * jump(f, x + N, k) + n1z[k] * jump(f, x, k)) * B[k];
     p[0] = s1; p[1] = s2; p[2] = 0.5 * s3;
   } else {
     double s1 = 0, s2 = 0;
-    for (int k = 0; k < np; ++k) s1 += n1x[k] * (x[f->n2[k]] - x[f->n1[k]]) * B[k];
-    for (int k = 0; k < np; ++k) s2 += n1z[k] * (x[f->n2[k]] - x[f->n1[k]]) * B[k];
+    for (int k = 0; k < np; ++k) s1 += n1x[k] * jump(f, x, k) * B[k];
+    for (int k = 0; k < np; ++k) s2 += n1z[k] * jump(f, x, k) * B[k];
     p[0] = 0.5 * s1; p[1] = 0.5 * s2;
   }
 }
@@ -484,11 +491,13 @@ int oracle_set_fault(oracle_solver *h, const sem2d_fault_desc *d, int nsamp) {
   f->np = np; f->ndof = nd;
   f->n1 = (int *)malloc(sizeof(int) * np);
   f->n2 = (int *)malloc(sizeof(int) * np);
-  for (int k = 0; k < np; ++k) { f->n1[k] = d->node1[k] - 1; f->n2[k] = d->node2[k] - 1; }
+  /* a fault on a domain boundary (one side, symmetry: SRC/bc_dynflt.f90:256-259) has no
+     node2 and no invM2: n2 = -1 */
+  for (int k = 0; k < np; ++k) { f->n1[k] = d->node1[k] - 1; f->n2[k] = d->node2 ? d->node2[k] - 1 : -1; }
   f->nrm = dup_d(d->n1, 2 * (size_t)np);
   f->B = dup_d(d->B, (size_t)nd * np);
   f->invM1 = dup_d(d->invM1, (size_t)nd * np);
-  f->invM2 = dup_d(d->invM2, (size_t)nd * np);
+  f->invM2 = d->node2 ? dup_d(d->invM2, (size_t)nd * np) : NULL;
   f->Z = dup_d(d->Z, (size_t)nd * np);
   f->T0 = dup_d(d->T0, 2 * (size_t)np);
   f->coh = dup_d(d->cohesion, np);
@@ -696,9 +705,15 @@ static void fault_apply(oracle_solver *h, double *MxA) {
     int i1 = f->n1[k], i2 = f->n2[k];
     double dD[2] = {0, 0}, dV[2] = {0, 0}, dA[2] = {0, 0}, T[2] = {0, 0}, Ts[2];
     for (int c = 0; c < nd; ++c) {
-      dD[c] = h->d[c * N + i2] - h->d[c * N + i1];
-      dV[c] = h->v[c * N + i2] - h->v[c * N + i1];
-      dA[c] = f->invM2[c * np + k] * MxA[c * N + i2] - f->invM1[c * np + k] * MxA[c * N + i1];
+      if (i2 >= 0) {
+        dD[c] = h->d[c * N + i2] - h->d[c * N + i1];
+        dV[c] = h->v[c * N + i2] - h->v[c * N + i1];
+        dA[c] = f->invM2[c * np + k] * MxA[c * N + i2] - f->invM1[c * np + k] * MxA[c * N + i1];
+      } else { /* one side: get_jump, get_weighted_jump (SRC/bc_dynflt.f90:692-719) */
+        dD[c] = -2.0 * h->d[c * N + i1];
+        dV[c] = -2.0 * h->v[c * N + i1];
+        dA[c] = -2.0 * f->invM1[c * np + k] * MxA[c * N + i1];
+      }
       T[c] = f->Z[c * np + k] * (dV[c] + f->a2v * dA[c]);
     }
     double nx = f->nrm[k], nz = f->nrm[np + k];
@@ -708,9 +723,8 @@ static void fault_apply(oracle_solver *h, double *MxA) {
       r0 = nz * dV[0] - nx * dV[1]; r1 = nx * dV[0] + nz * dV[1]; dV[0] = r0; dV[1] = r1;
       r0 = nz * dA[0] - nx * dA[1]; r1 = nx * dA[0] + nz * dA[1]; dA[0] = r0; dA[1] = r1;
       r0 = nz * T[0] - nx * T[1]; r1 = nx * T[0] + nz * T[1]; T[0] = r0; T[1] = r1;
-    } else {
-      T[1] = 0.0;
     }
+    if (nd == 1 || i2 < 0) T[1] = 0.0; /* SRC/bc_dynflt.f90:604 */
     T[0] = T[0] + f->T0[k];
     T[1] = T[1] + f->T0[np + k];
     if (f->opening) T[1] = fmin(T[1], 0.0);
@@ -757,7 +771,7 @@ static void fault_apply(oracle_solver *h, double *MxA) {
     }
     for (int c = 0; c < nd; ++c) {
       MxA[c * N + i1] = MxA[c * N + i1] + f->B[c * np + k] * Tx[c];
-      MxA[c * N + i2] = MxA[c * N + i2] - f->B[c * np + k] * Tx[c];
+      if (i2 >= 0) MxA[c * N + i2] = MxA[c * N + i2] - f->B[c * np + k] * Tx[c];
     }
     for (int c = 0; c < nd; ++c) {
       double tf = (c == 0) ? f->T[k] : f->T[np + k];
@@ -859,8 +873,61 @@ static void one_step(oracle_solver *h) {
   if (h->flt) fault_record(h);
 }
 
+/* maxval(abs(veloc)), maxval(abs(displ)) (SRC/main.f90:45,73-76) and the number of
+   values of d, v, a that are not finite */
+static void field_info(const oracle_solver *h, double *vmax, double *dmax, int *bad) {
+  size_t n = (size_t)h->ndof * h->npoin;
+  double vm = 0.0, dm = 0.0;
+  int nb = 0;
+  for (size_t k = 0; k < n; ++k) {
+    nb += !isfinite(h->d[k]) + !isfinite(h->v[k]) + !isfinite(h->a[k]);
+    if (fabs(h->v[k]) > vm) vm = fabs(h->v[k]);
+    if (fabs(h->d[k]) > dm) dm = fabs(h->d[k]);
+  }
+  *vmax = vm; *dmax = dm; *bad = nb;
+}
+
 int oracle_step(oracle_solver *h, int nsteps) {
-  for (int s = 0; s < nsteps; ++s) one_step(h);
+  for (int s = 0; s < nsteps; ++s) {
+    one_step(h);
+    if (h->itinfo > 0 && h->it % h->itinfo == 0) {
+      if (h->info_rows == h->info_cap) {
+        h->info_cap = h->info_cap ? 2 * h->info_cap : 64;
+        h->info = (double *)realloc(h->info, sizeof(double) * 4 * h->info_cap);
+      }
+      double *r = &h->info[4 * h->info_rows++];
+      int bad;
+      r[0] = h->it; r[1] = h->it * h->dt;
+      field_info(h, &r[2], &r[3], &bad);
+      if (bad && !h->first_bad) h->first_bad = h->it;
+    }
+    if (h->first_bad) {
+      snprintf(h->err, sizeof h->err, "non-finite field values at step %d", h->first_bad);
+      return 21;
+    }
+  }
+  return 0;
+}
+
+int oracle_set_itinfo(oracle_solver *h, int itinfo) {
+  if (itinfo < 0) return fail(h, 3, "set_itinfo: negative interval");
+  h->itinfo = itinfo;
+  return 0;
+}
+
+int oracle_field_info(oracle_solver *h, double *vmax, double *dmax, int *nonfinite) {
+  double vm, dm;
+  int nb;
+  field_info(h, &vm, &dm, &nb);
+  if (vmax) *vmax = vm;
+  if (dmax) *dmax = dm;
+  if (nonfinite) *nonfinite = nb;
+  return 0;
+}
+
+int oracle_fetch_info(oracle_solver *h, double *rows, int *nrows) {
+  if (rows && h->info_rows) memcpy(rows, h->info, sizeof(double) * 4 * h->info_rows);
+  if (nrows) *nrows = h->info_rows;
   return 0;
 }
 
@@ -928,6 +995,7 @@ void oracle_destroy(oracle_solver *h) {
     free(h->src[s].ampl);
   }
   free(h->rnode); free(h->rw); free(h->seis); free(h->kv_eta); free(h->eorder); free(h->floc);
+  free(h->info);
   if (h->flt) {
     fault_t *f = h->flt;
     free(f->n1); free(f->n2); free(f->nrm); free(f->B); free(f->invM1); free(f->invM2);

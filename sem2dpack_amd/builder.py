@@ -889,7 +889,7 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
 
 
 # ---------------------------------------------------------------------------
-# distributions used by the fault inputs (constant, PWCONR, LINEAR)
+# distributions used by the fault inputs (constant, PWCONR, LINEAR, ORDER0)
 # ---------------------------------------------------------------------------
 def _read_cd(par, group_iter, value, name_h):
     """DIST_CD_Read (SRC/distribution_cd.f90:26-63): a constant or the next &DIST_<name> block."""
@@ -922,7 +922,40 @@ def _read_cd(par, group_iter, value, name_h):
             vals = " ".join(g.data_lines).replace("d", "e").replace("D", "e").split()
             tab = np.array([float(v) for v in vals[:2 * n]]).reshape(n, 2)
         return ("linear", dict(dim=dim, x=tab[:, 0].copy(), val=tab[:, 1].copy()))
+    if want == "DIST_ORDER0":  # read_order0_dist (SRC/distribution_order0.f90:47-68)
+        xn, zn = int(g.get("xn")), int(g.get("zn"))
+        recs = list(g.data_lines)
+        pos = 0
+        xb, pos = _list_read(recs, pos, xn - 1) if xn > 1 else ([], pos)
+        zb, pos = _list_read(recs, pos, zn - 1) if zn > 1 else ([], pos)
+        val = []
+        for _ in range(zn):  # val(:, iz), one read per z-zone
+            row, pos = _list_read(recs, pos, xn)
+            val.append(row)
+        return ("order0", dict(xn=xn, zn=zn, xb=xb, zb=zb, val=val))
     raise InputError(f"distribution {want} is not supported by this build")
+
+
+def _list_read(recs, pos, n):
+    """A Fortran list-directed READ of n reals from the records recs[pos:]: it takes values
+    from as many records as it needs and leaves the rest of its last record unread.
+    Returns (values, index of the next record)."""
+    out = []
+    while len(out) < n:
+        if pos >= len(recs):
+            raise InputError("list-directed read: end of data")
+        toks = recs[pos].replace("d", "e").replace("D", "e").replace(",", " ").split()
+        pos += 1
+        out.extend(float(t) for t in toks[:n - len(out)])
+    return out, pos
+
+
+def _order0_zone(c, nzones, bound):
+    """zone (SRC/distribution_order0.f90:85-98): the first k with c < bound(k), else nzones"""
+    for k in range(nzones - 1):
+        if c < bound[k]:
+            return k
+    return nzones - 1
 
 
 def _gen_cd(cd, coord):
@@ -940,6 +973,9 @@ def _gen_cd(cd, coord):
                     break
             out[n] = p["val"][z]
         return out
+    if kind == "order0":  # generate_order0_dist (SRC/distribution_order0.f90:72-82)
+        return np.array([p["val"][_order0_zone(z, p["zn"], p["zb"])][_order0_zone(x, p["xn"], p["xb"])]
+                         for x, z in coord[:, :2]])
     if kind == "linear":  # interpol of SRC/distribution_linear.f90
         xi, yi = p["x"], p["val"]
         xo = coord[:, p["dim"] - 1]
@@ -956,10 +992,17 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
     g = par.after(bcdef, "BC_DYNFLT")
     if g is None:
         raise InputError("BC_DYNFLT_read: BC_DYNFLT input block not found")
-    tags = bcdef.get("tags")
-    if not isinstance(tags, list) or len(tags) < 2 or int(tags[1]) <= 0:
-        raise InputError("DYNFLT on a single side (symmetry) is not supported by this build")
-    tags = [int(tags[0]), int(tags[1])]
+    # tags(1), tags(2); one tag (or tags(2) = 0): a fault on a domain boundary with one
+    # side (symmetry, SRC/bc_dynflt.f90:256-259)
+    tag = int(bcdef.get("tag", 0))
+    tv = bcdef.get("tags", None)
+    if tag > 0:
+        tags = [tag, 0]
+    elif isinstance(tv, list) and len(tv) >= 2:
+        tags = [int(tv[0]), int(tv[1])]
+    else:
+        tags = [int(tv if not isinstance(tv, list) else tv[0]), 0]
+    two_sides = tags[1] > 0
     it = iter(par.next_groups(g))
     cd = {}
     # order of DIST_CD_Read calls: SRC/bc_dynflt.f90:166-174
@@ -1007,16 +1050,19 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
         raise InputError("BC_DYNFLT_NOR kind other than 1 (Coulomb) is not supported by this build")
 
     t1 = _bound_topo(tags[0], mesh, num, perm_inv)
-    t2 = _bound_topo(tags[1], mesh, num, perm_inv)
-    if len(t1.elem) != len(t2.elem) or len(t1.node) != len(t2.node):
-        raise InputError("bc_dynflt_init: boundaries do not match")
-    keep = t1.node != t2.node
-    node1 = t1.node[keep]
-    node2 = t2.node[keep]
     coord = num["coord"]
+    if two_sides:
+        t2 = _bound_topo(tags[1], mesh, num, perm_inv)
+        if len(t1.elem) != len(t2.elem) or len(t1.node) != len(t2.node):
+            raise InputError("bc_dynflt_init: boundaries do not match")
+        keep = t1.node != t2.node
+        node2 = t2.node[keep]
+    else:
+        keep = np.ones(len(t1.node), dtype=bool)
+        node2 = None
+    node1 = t1.node[keep]
     c1 = coord[node1 - 1]
-    c2 = coord[node2 - 1]
-    if (np.abs(c1 - c2) > TINY_XABS).any():
+    if two_sides and (np.abs(c1 - coord[node2 - 1]) > TINY_XABS).any():
         raise InputError("bc_dynflt_init: coordinates on boundaries do not match properly")
     NORM, W = _normal_and_weights(t1, jac, wgll)
     n1 = NORM[keep]
@@ -1024,10 +1070,15 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
     coefA2V = gamma * dt
     coefA2D = beta * dt * dt
     invM1 = 1.0 / M[:, node1 - 1].T
-    invM2 = 1.0 / M[:, node2 - 1].T
-    Z = 1.0 / (coefA2V * B * (invM1 + invM2))
+    if two_sides:
+        invM2 = 1.0 / M[:, node2 - 1].T
+        Z = 1.0 / (coefA2V * B * (invM1 + invM2))
+    else:  # SRC/bc_dynflt.f90:353
+        invM2 = None
+        Z = 0.5 / (coefA2V * B * invM1)
     npf = len(node1)
-    f = dict(tags=tags, node1=node1.astype(np.int32), node2=node2.astype(np.int32), coord=c1, n1=n1, B=B,
+    f = dict(tags=tags, node1=node1.astype(np.int32), node2=None if node2 is None else node2.astype(np.int32),
+             coord=c1, n1=n1, B=B,
              invM1=invM1, invM2=invM2, Z=Z, coefA2V=coefA2V, coefA2D=coefA2D, law=law,
              opening=bool(g.get("opening", True)))
     if law == "SWF":
@@ -1404,6 +1455,10 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
 
     pb = Problem()
     pb.ngll, pb.ndof = ngll, ndof
+    # progress line interval (SRC/input.f90:95,122,131: default 100, must be positive)
+    pb.itinfo = int(gen.get("itinfo", 100))
+    if pb.itinfo <= 0:
+        raise InputError("GENERAL input block: itInfo must be positive")
     pb.mesh = mesh
     xgll, wgll, H = gll5()
     pb.xgll, pb.wgll, pb.hprime = xgll, wgll, H
@@ -1608,6 +1663,20 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
             stacey = bool(ab.get("stacey", False)) if ab is not None else False
             t = _bound_topo(t1, mesh, num, perm_inv)
             pb.abso.append(_abso_init(t, jac, wgll, (rho_p, cp_p, cs_p), M, coefA2Vrhs, ndof, stacey))
+        elif kind == "DIRNEU":
+            # BC_DIRNEU_read (SRC/bc_dirneu.f90:50-112): Neumann with no source time
+            # function is a free boundary, BC_DIRNEU_apply adds nothing
+            dn = par.after(bd, "BC_DIRNEU")
+            if dn is None:
+                raise InputError("bc_DIRNEU_read: no BC_DIRNEU block found")
+            for comp in ("h", "v"):
+                val = str(dn.get(comp, "N")).upper()
+                if val not in ("N", "D"):
+                    raise InputError(f"bc_DIRNEU_read: {comp} must be N or D")
+                if val == "D":
+                    raise InputError("BC_DIRNEU Dirichlet components are not supported by this build")
+                if str(dn.get(comp + "stf", "none")).lower() != "none":
+                    raise InputError("BC_DIRNEU time-dependent tractions are not supported by this build")
         elif kind == "DYNFLT":
             if pb.fault is not None:
                 raise InputError("more than one DYNFLT boundary is not supported by this build")

@@ -20,7 +20,8 @@ namespace sem2d {
 
 struct FaultDev {
   int np, ndof;
-  const int *n1, *n2;                    // 0-based node pairs
+  const int *n1, *n2;                    // 0-based node pairs; n2[k] = -1 on a fault with one
+                                         // side (on a domain boundary, SRC/bc_dynflt.f90:256-259)
   const double *nrm;                     // [2][np]
   const double *B, *invM1, *invM2, *Z;   // [ndof][np]
   const double *T0;                      // [2][np]
@@ -181,9 +182,15 @@ __global__ void k_fault(FaultDev F, R* __restrict__ d, R* __restrict__ v, R* __r
   double dD[2] = {0, 0}, dV[2] = {0, 0}, dA[2] = {0, 0}, T[2] = {0, 0}, Ts[2];
   for (int c = 0; c < nd; ++c) {
     const size_t o = (size_t)c * npoin;
-    dD[c] = (double)d[o + i2] - (double)d[o + i1];
-    dV[c] = (double)v[o + i2] - (double)v[o + i1];
-    dA[c] = F.invM2[c * np + k] * (double)f[o + i2] - F.invM1[c * np + k] * (double)f[o + i1];
+    if (i2 >= 0) {
+      dD[c] = (double)d[o + i2] - (double)d[o + i1];
+      dV[c] = (double)v[o + i2] - (double)v[o + i1];
+      dA[c] = F.invM2[c * np + k] * (double)f[o + i2] - F.invM1[c * np + k] * (double)f[o + i1];
+    } else {  // one side: get_jump, get_weighted_jump (SRC/bc_dynflt.f90:692-719)
+      dD[c] = -2.0 * (double)d[o + i1];
+      dV[c] = -2.0 * (double)v[o + i1];
+      dA[c] = -2.0 * F.invM1[c * np + k] * (double)f[o + i1];
+    }
     T[c] = F.Z[c * np + k] * (dV[c] + F.a2v * dA[c]);
   }
   const double nx = F.nrm[k], nz = F.nrm[np + k];
@@ -193,9 +200,8 @@ __global__ void k_fault(FaultDev F, R* __restrict__ d, R* __restrict__ v, R* __r
     r0 = nz * dV[0] - nx * dV[1]; r1 = nx * dV[0] + nz * dV[1]; dV[0] = r0; dV[1] = r1;
     r0 = nz * dA[0] - nx * dA[1]; r1 = nx * dA[0] + nz * dA[1]; dA[0] = r0; dA[1] = r1;
     r0 = nz * T[0] - nx * T[1];   r1 = nx * T[0] + nz * T[1];   T[0] = r0;  T[1] = r1;
-  } else {
-    T[1] = 0.0;
   }
+  if (nd == 1 || i2 < 0) T[1] = 0.0;  // SRC/bc_dynflt.f90:604
   T[0] = T[0] + F.T0[k];
   T[1] = T[1] + F.T0[np + k];
   if (F.opening) T[1] = fmin(T[1], 0.0);
@@ -249,7 +255,7 @@ __global__ void k_fault(FaultDev F, R* __restrict__ d, R* __restrict__ v, R* __r
     const size_t o = (size_t)c * npoin;
     const double bt = F.B[c * np + k] * Tx[c];
     f[o + i1] = (R)((double)f[o + i1] + bt);
-    f[o + i2] = (R)((double)f[o + i2] - bt);
+    if (i2 >= 0) f[o + i2] = (R)((double)f[o + i2] - bt);
   }
   for (int c = 0; c < nd; ++c) {
     const double tf = T[c];
@@ -261,7 +267,7 @@ __global__ void k_fault(FaultDev F, R* __restrict__ d, R* __restrict__ v, R* __r
     for (int c = 0; c < nd; ++c) {
       const size_t o = (size_t)c * npoin;
       fault_node_correct<R>(d, v, f, o + i1, (R)F.rmn1[c * np + k], c3, c4);
-      fault_node_correct<R>(d, v, f, o + i2, (R)F.rmn2[c * np + k], c3, c4);
+      if (i2 >= 0) fault_node_correct<R>(d, v, f, o + i2, (R)F.rmn2[c * np + k], c3, c4);
     }
   }
 }
@@ -295,10 +301,14 @@ __device__ __forceinline__ void potency_row(const FaultDev& F, int k0, int n, co
     const int k = k0 + kl;
     const int i1 = F.n1[k], i2 = F.n2[k];
     const double nx = F.nrm[k], nz = F.nrm[np + k], b = F.B[k];
-    const double jd0 = (double)d[i2] - (double)d[i1], jv0 = (double)v[i2] - (double)v[i1];
+    // get_jump (SRC/bc_dynflt.f90:692-704): x2 - x1, or -2 x1 on a one-sided fault
+    auto jump = [&](const R* x) {
+      return i2 >= 0 ? (double)x[i2] - (double)x[i1] : -2.0 * (double)x[i1];
+    };
+    const double jd0 = jump(d), jv0 = jump(v);
     if (F.ndof == 2) {
-      const double jd1 = (double)d[stride + i2] - (double)d[stride + i1];
-      const double jv1 = (double)v[stride + i2] - (double)v[stride + i1];
+      const double jd1 = jump(d + stride);
+      const double jv1 = jump(v + stride);
       s[0] += nx * jd0 * b;
       s[1] += nz * jd1 * b;
       s[2] += (nx * jd1 + nz * jd0) * b;

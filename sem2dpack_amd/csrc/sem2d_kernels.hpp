@@ -287,4 +287,51 @@ __global__ void k_receivers_interp(int nrec, int ndof, const int* __restrict__ n
 }
 #pragma clang fp contract(on)
 
+// Progress line of the reference's time loop (SRC/main.f90:45,73-76): max |v| and
+// max |d| over the fields, and the number of non-finite values of d, v, a (the
+// watchdog).  The maxima of non-negative doubles are kept as their bit patterns, whose
+// integer order is their numeric order, so the result does not depend on the order in
+// which waves finish.  acc: [0] max |v| bits, [1] max |d| bits; cnt: [0] non-finite
+// values, [1] first step at which the check found one (0: none).
+template <typename R>
+__global__ void k_field_max(size_t n, const R* __restrict__ d, const R* __restrict__ v, const R* __restrict__ a,
+                            unsigned long long* __restrict__ acc, int* __restrict__ cnt) {
+  double vm = 0.0, dm = 0.0;
+  int bad = 0;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const double x = (double)v[k], y = (double)d[k], z = (double)a[k];
+    bad += !isfinite(x) + !isfinite(y) + !isfinite(z);
+    vm = fmax(vm, fabs(x));  // fmax returns the other operand for a NaN
+    dm = fmax(dm, fabs(y));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    vm = fmax(vm, __shfl_xor(vm, o));
+    dm = fmax(dm, __shfl_xor(dm, o));
+    bad += __shfl_xor(bad, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (vm > 0.0) atomicMax(&acc[0], (unsigned long long)__double_as_longlong(vm));
+    if (dm > 0.0) atomicMax(&acc[1], (unsigned long long)__double_as_longlong(dm));
+    if (bad) atomicAdd(&cnt[0], bad);
+  }
+}
+
+// one progress row (it, time, max |v|, max |d|) from k_field_max's accumulators, which
+// it then clears for the next row; with row set, notes the first step with a non-finite
+// value (row null: a query outside the time loop, which leaves the watchdog alone)
+__global__ void k_info_row(int it, double t, unsigned long long* __restrict__ acc, int* __restrict__ cnt,
+                           double* __restrict__ row) {
+  if (threadIdx.x != 0) return;
+  if (row) {
+    row[0] = (double)it;
+    row[1] = t;
+    row[2] = __longlong_as_double((long long)acc[0]);
+    row[3] = __longlong_as_double((long long)acc[1]);
+  }
+  if (row && cnt[0] && !cnt[1]) cnt[1] = it > 0 ? it : -1;
+  acc[0] = acc[1] = 0ull;
+  cnt[0] = 0;
+}
+
 }  // namespace sem2d

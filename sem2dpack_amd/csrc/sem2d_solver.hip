@@ -148,6 +148,12 @@ struct sem2d_solver {
   AdjState* adj = nullptr;
   std::vector<int> elem_ibool;  // [nelem][25] 0-based reference nodes (interpolated receivers)
   void* kv_eta = nullptr;       // Kelvin-Voigt viscosity [element in colour order][25] (bulk type)
+  // progress rows (it, time, max |v|, max |d|) every itinfo steps, and the non-finite
+  // watchdog (k_field_max, k_info_row)
+  int itinfo = 0, info_rows = 0, info_cap = 0;
+  double* info = nullptr;                 // device [info_cap][4]
+  unsigned long long* info_acc = nullptr;  // device [2]
+  int* info_cnt = nullptr;                 // device [2]
 };
 
 namespace {
@@ -401,6 +407,32 @@ int record_receivers(sem2d_solver* h) {
 }
 
 // room for `rows` potency rows (grown outside any step loop)
+int info_reserve(sem2d_solver* h, int rows) {
+  if (rows <= h->info_cap) return 0;
+  const int cap = std::max(rows, std::max(64, 2 * h->info_cap));
+  double* p = nullptr;
+  HIPCHK(h, hipMalloc(&p, (size_t)cap * 4 * sizeof(double)));
+  if (h->info && h->info_rows)
+    HIPCHK(h, hipMemcpyAsync(p, h->info, (size_t)h->info_rows * 4 * sizeof(double), hipMemcpyDeviceToDevice,
+                             h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (h->info) hipFree(h->info);
+  h->info = p;
+  h->info_cap = cap;
+  return 0;
+}
+
+// the progress row of step it (after that step's kernels)
+template <typename R>
+void record_info(sem2d_solver* h, int it) {
+  if (h->itinfo <= 0 || it % h->itinfo != 0) return;
+  const size_t n = (size_t)h->ndof * h->fstride;
+  k_field_max<R><<<std::min<size_t>((n + 255) / 256, 4096), 256, 0, h->stream>>>(
+      n, (const R*)h->d, (const R*)h->v, (const R*)h->a, h->info_acc, h->info_cnt);
+  k_info_row<<<1, 64, 0, h->stream>>>(it, it * h->dt, h->info_acc, h->info_cnt, h->info + 4 * (size_t)h->info_rows);
+  ++h->info_rows;
+}
+
 int potency_reserve(sem2d_solver* h, int rows) {
   FaultHost& F = h->flt;
   if (rows <= F.pot_cap) return 0;
@@ -760,14 +792,15 @@ int struct_tables(sem2d_solver* h, StructTables& X) {
 // Upload the structured tables of the solver's own grid.
 // inverse masses of the split-fault nodes in pair order: r1 (node 1) and r2 (node 2),
 // [ndof][npairs] each, from the table's fault_rm ([ndof][node 1 of every pair, then node 2])
-void split_fault_rm(const StructTables& X, int nd, std::vector<double>& r1, std::vector<double>& r2) {
-  const size_t nf = X.fault_g.size(), np = nf / 2;
+// (np pairs; a one-sided fault has no node 2: r2 stays zero)
+void split_fault_rm(const StructTables& X, int nd, size_t np, std::vector<double>& r1, std::vector<double>& r2) {
+  const size_t nf = X.fault_g.size();
   r1.assign((size_t)nd * np, 0.0);
   r2.assign((size_t)nd * np, 0.0);
   for (int c = 0; c < nd; ++c)
     for (size_t k = 0; k < np; ++k) {
       r1[(size_t)c * np + k] = X.fault_rm[(size_t)c * nf + k];
-      r2[(size_t)c * np + k] = X.fault_rm[(size_t)c * nf + np + k];
+      if (nf == 2 * np) r2[(size_t)c * np + k] = X.fault_rm[(size_t)c * nf + np + k];
     }
 }
 
@@ -838,7 +871,7 @@ int prepare_structured(sem2d_solver* h) {
   }
   if (h->flt.on) {  // per-pair inverse masses for the corrector inside k_fault
     std::vector<double> r1, r2;
-    split_fault_rm(X, h->ndof, r1, r2);
+    split_fault_rm(X, h->ndof, h->fault_n1.size(), r1, r2);
     double *p1 = nullptr, *p2 = nullptr;
     if ((rc = prep_upload(h, &p1, r1.data(), r1.size()))) return rc;
     if ((rc = prep_upload(h, &p2, r2.data(), r2.size()))) return rc;
@@ -1048,6 +1081,8 @@ int do_steps(sem2d_solver* h, int nsteps) {
   const int gb = grid_for(n, 256);
   if (h->st.on && !h->st.ready && (rc = prepare_structured(h))) return rc;
   if (h->flt.on && (rc = potency_reserve(h, h->flt.pot_rows + nsteps))) return rc;
+  if (h->itinfo > 0 && (rc = info_reserve(h, h->info_rows + (h->it + nsteps) / h->itinfo - h->it / h->itinfo)))
+    return rc;
   StepDev X;
   if (h->st.on) X = step_dev(h);
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
@@ -1058,6 +1093,7 @@ int do_steps(sem2d_solver* h, int nsteps) {
       h->it = it;
       record_receivers<R>(h);
       record_fault(h);
+      record_info<R>(h, it);
       continue;
     }
     k_predict<R><<<gb, 256, 0, h->stream>>>(n, d, v, a, dt, c1, c2);
@@ -1079,6 +1115,7 @@ int do_steps(sem2d_solver* h, int nsteps) {
     h->it = it;
     record_receivers<R>(h);
     record_fault(h);
+    record_info<R>(h, it);
   }
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
   HIPCHK(h, hipGetLastError());
@@ -1090,6 +1127,11 @@ int do_steps(sem2d_solver* h, int nsteps) {
     int flag = 0;
     HIPCHK(h, hipMemcpy(&flag, h->flags, sizeof(int), hipMemcpyDeviceToHost));
     if (flag) return set_err(h, 20, "NR_Solver has exceeded the maximum iterations (200)");
+  }
+  if (h->itinfo > 0) {
+    int cnt[2] = {0, 0};
+    HIPCHK(h, hipMemcpy(cnt, h->info_cnt, sizeof cnt, hipMemcpyDeviceToHost));
+    if (cnt[1]) return set_err(h, 21, "non-finite field values at step %d", cnt[1]);
   }
   return 0;
 }
@@ -1227,6 +1269,10 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   if ((rc = dev_alloc(h, (char**)&h->v, nfs * rsize(h)))) return rc;
   if ((rc = dev_alloc(h, (char**)&h->a, nfs * rsize(h)))) return rc;
   if ((rc = dev_alloc(h, &h->flags, 4))) return rc;
+  if ((rc = dev_alloc(h, &h->info_acc, 2))) return rc;
+  if ((rc = dev_alloc(h, &h->info_cnt, 2))) return rc;
+  HIPCHK(h, hipMemsetAsync(h->info_acc, 0, 2 * sizeof(unsigned long long), h->stream));
+  HIPCHK(h, hipMemsetAsync(h->info_cnt, 0, 2 * sizeof(int), h->stream));
   HIPCHK(h, hipMemsetAsync(h->d, 0, nfs * rsize(h), h->stream));
   HIPCHK(h, hipMemsetAsync(h->v, 0, nfs * rsize(h), h->stream));
   HIPCHK(h, hipMemsetAsync(h->a, 0, nfs * rsize(h), h->stream));
@@ -1443,20 +1489,25 @@ int sem2d_set_fault(sem2d_solver* h, const sem2d_fault_desc* f, int nsamp) {
   FaultDev& D = F.dev;
   D.np = np;
   D.ndof = nd;
-  std::vector<int> n1(np), n2(np);
+  // node2 == NULL: a fault with one side, on a domain boundary (symmetry; the
+  // reference's bc2 and invM2 are then undefined, SRC/bc_dynflt.f90:256-259)
+  const bool two = f->node2 != nullptr;
+  if (two && !f->invM2) return set_err(h, 3, "set_fault: invM2 needed with node2");
+  std::vector<int> n1(np), n2(np, -1);
   for (int k = 0; k < np; ++k) {
-    if (f->node1[k] < 1 || f->node1[k] > h->npoin || f->node2[k] < 1 || f->node2[k] > h->npoin)
+    if (f->node1[k] < 1 || f->node1[k] > h->npoin || (two && (f->node2[k] < 1 || f->node2[k] > h->npoin)))
       return set_err(h, 3, "set_fault: node out of range");
     n1[k] = f->node1[k] - 1;
-    n2[k] = f->node2[k] - 1;
+    if (two) n2[k] = f->node2[k] - 1;
   }
   h->fault_n1 = n1;
-  h->fault_n2 = n2;
+  if (two) h->fault_n2 = n2;
+  else h->fault_n2.clear();
   h->st.ready = false;
   if (h->st.on)
     for (int k = 0; k < np; ++k) {
       n1[k] = h->st.gmap[n1[k]];
-      n2[k] = h->st.gmap[n2[k]];
+      if (two) n2[k] = h->st.gmap[n2[k]];
     }
   int rc;
   int* ip;
@@ -1471,7 +1522,8 @@ int sem2d_set_fault(sem2d_solver* h, const sem2d_fault_desc* f, int nsamp) {
   UP(nrm, f->n1, 2 * np);
   UP(B, f->B, nd * np);
   UP(invM1, f->invM1, nd * np);
-  UP(invM2, f->invM2, nd * np);
+  std::vector<double> zero_m((size_t)nd * np, 0.0);
+  UP(invM2, two ? f->invM2 : zero_m.data(), nd * np);
   UP(Z, f->Z, nd * np);
   UP(T0, f->T0, 2 * np);
   UP(coh, f->cohesion, np);
@@ -1705,6 +1757,45 @@ int sem2d_compute_fint_noalloc(sem2d_solver* h, const double* d, double* f) {
   return 0;
 }
 
+int sem2d_set_itinfo(sem2d_solver* h, int itinfo) {
+  if (!h) return 1;
+  if (itinfo < 0) return set_err(h, 3, "set_itinfo: negative interval");
+  h->itinfo = itinfo;
+  return 0;
+}
+
+int sem2d_field_info(sem2d_solver* h, double* vmax, double* dmax, int* nonfinite) {
+  if (!h) return 1;
+  const size_t n = (size_t)h->ndof * h->fstride;
+  const int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+  if (h->fp32)
+    k_field_max<float><<<nb, 256, 0, h->stream>>>(n, (const float*)h->d, (const float*)h->v, (const float*)h->a,
+                                                 h->info_acc, h->info_cnt);
+  else
+    k_field_max<double><<<nb, 256, 0, h->stream>>>(n, (const double*)h->d, (const double*)h->v,
+                                                  (const double*)h->a, h->info_acc, h->info_cnt);
+  unsigned long long acc[2];
+  int cnt[2];
+  HIPCHK(h, hipMemcpyAsync(acc, h->info_acc, sizeof acc, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(cnt, h->info_cnt, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
+  k_info_row<<<1, 64, 0, h->stream>>>(h->it, h->it * h->dt, h->info_acc, h->info_cnt, nullptr);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double x[2];
+  std::memcpy(x, acc, sizeof x);
+  if (vmax) *vmax = x[0];
+  if (dmax) *dmax = x[1];
+  if (nonfinite) *nonfinite = cnt[0];
+  return 0;
+}
+
+int sem2d_fetch_info(sem2d_solver* h, double* rows, int* nrows) {
+  if (!h) return 1;
+  if (rows && h->info_rows)
+    HIPCHK(h, hipMemcpy(rows, h->info, (size_t)h->info_rows * 4 * sizeof(double), hipMemcpyDeviceToHost));
+  if (nrows) *nrows = h->info_rows;
+  return 0;
+}
+
 int sem2d_set_tile_config(sem2d_solver* h, int threads, int wg_per_cu) {
   if (!h) return 1;
   if (!h->st.on) return set_err(h, 4, "set_tile_config: not a structured (MESH_CART) grid");
@@ -1742,6 +1833,7 @@ void sem2d_destroy(sem2d_solver* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   if (h->adj) adj_free(h->adj);
   if (h->flt.pot) hipFree(h->flt.pot);
+  if (h->info) hipFree(h->info);
   for (void* p : h->allocs) hipFree(p);
   for (void* p : h->st.prep_allocs) hipFree(p);
   for (auto& b : h->abso) { (void)b; }
@@ -1901,6 +1993,8 @@ int check_members(sem2d_batch* b) {
     if (h->flt.on != h0->flt.on) return berr(b, 3, "member %d: either every member has a fault or none", m);
     for (const AbsoHost& B : h->abso_host)
       if (!B.senode.empty()) return berr(b, 3, "member %d: Stacey boundaries are not batched", m);
+    if (h->flt.on && h->fault_n2.empty())
+      return berr(b, 3, "member %d: faults with one side (on a domain boundary) are not batched", m);
     if (h->flt.on) {
       const FaultDev &F = h->flt.dev, &F0 = h0->flt.dev;
       if (F.law != F0.law || F.kind != F0.kind || F.opening != F0.opening || F.healing != F0.healing ||
@@ -2203,7 +2297,7 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
     (void)nf;
     if (h->flt.on) {  // per-pair inverse masses, in the batch's pair order
       std::vector<double> r1, r2;
-      split_fault_rm(X, nd, r1, r2);
+      split_fault_rm(X, nd, h->fault_n1.size(), r1, r2);
       const size_t npm = r1.size() / nd;
       for (int c = 0; c < nd; ++c) {
         frm_c[c].insert(frm_c[c].end(), r1.begin() + c * npm, r1.begin() + (c + 1) * npm);

@@ -41,7 +41,8 @@ module sem2d_hip
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
             sem2d_fetch_potency, &
-            sem2d_compute_fint, sem2d_compute_fint_noalloc, sem2d_set_tile_config, sem2d_get_step, sem2d_last_step_ms, &
+            sem2d_compute_fint, sem2d_compute_fint_noalloc, sem2d_set_itinfo, sem2d_fetch_info, &
+            sem2d_field_info, sem2d_set_tile_config, sem2d_get_step, sem2d_last_step_ms, &
             sem2d_last_error, sem2d_destroy, sem2d_error_message, &
             sem2d_adjoint_init, sem2d_adjoint_forward, sem2d_adjoint_backward, sem2d_adjoint_fetch, &
             sem2d_adjoint_timing, sem2d_adjoint_plan, sem2d_adjoint_release, &
@@ -194,6 +195,28 @@ module sem2d_hip
       type(c_ptr), value :: h
       real(c_double), intent(in) :: d(*)
       real(c_double), intent(out) :: f(*)
+    end function
+
+    ! progress rows (it, it*dt, max|v|, max|d|) every itinfo steps and the non-finite
+    ! watchdog (sem2d_step returns 21 once a row finds a value that is not finite)
+    integer(c_int) function sem2d_set_itinfo(h, itinfo) bind(C, name='sem2d_set_itinfo')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: itinfo
+    end function
+
+    integer(c_int) function sem2d_fetch_info(h, rows, nrows) bind(C, name='sem2d_fetch_info')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: rows(*)
+      integer(c_int), intent(out) :: nrows
+    end function
+
+    integer(c_int) function sem2d_field_info(h, vmax, dmax, nonfinite) bind(C, name='sem2d_field_info')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: vmax, dmax
+      integer(c_int), intent(out) :: nonfinite
     end function
 
     integer(c_int) function sem2d_set_tile_config(h, threads, wg_per_cu) bind(C, name='sem2d_set_tile_config')

@@ -25,7 +25,8 @@ def _p(a, t=np.float64):
 class Solver:
     FIELD_CODE = {"D": 0, "V": 1, "A": 2}
 
-    def __init__(self, pb, bulk_fp32=False, api=None, structured=True, compact=True, keep_problem=True):
+    def __init__(self, pb, bulk_fp32=False, api=None, structured=True, compact=True, keep_problem=True,
+                 itinfo=0):
         self.api = api if api is not None else load_hip_lib()
         self.pb = pb
         self.ndof, self.npoin, self.nelem = pb.ndof, pb.npoin, pb.nelem
@@ -69,6 +70,8 @@ class Solver:
         if rc != 0:
             raise SolverError(f"{self.api.prefix}create failed ({rc})")
         self.h = h
+        if itinfo:
+            self.set_itinfo(pb.itinfo if itinfo is True else int(itinfo))
         self._setup()
         if not keep_problem:
             self.pb = None  # the device solver holds what it needs; free the host problem
@@ -137,9 +140,12 @@ class Solver:
 
         npf = len(f["node1"])
         fd.npoin = npf
-        fd.node1, fd.node2 = arr(f["node1"], np.int32), arr(f["node2"], np.int32)
+        one_side = f.get("node2") is None  # fault on a domain boundary (symmetry)
+        fd.node1 = arr(f["node1"], np.int32)
+        fd.node2 = None if one_side else arr(f["node2"], np.int32)
         fd.n1 = arr(f["n1"].T)
-        fd.B, fd.invM1, fd.invM2, fd.Z = arr(f["B"].T), arr(f["invM1"].T), arr(f["invM2"].T), arr(f["Z"].T)
+        fd.B, fd.invM1, fd.Z = arr(f["B"].T), arr(f["invM1"].T), arr(f["Z"].T)
+        fd.invM2 = None if one_side else arr(f["invM2"].T)
         fd.T0 = arr(f["T0"].T)
         fd.cohesion = arr(f["cohesion"])
         fd.coefA2V, fd.coefA2D = f["coefA2V"], f["coefA2D"]
@@ -207,6 +213,26 @@ class Solver:
         f = np.empty_like(d)
         self._check(self.api.compute_fint(self.h, _p(d), _p(f)), "compute_fint")
         return f
+
+    def set_itinfo(self, itinfo):
+        """progress rows every itinfo steps (SRC/main.f90:73-76) and the non-finite watchdog;
+        0 turns both off"""
+        self._check(self.api.set_itinfo(self.h, int(itinfo)), "set_itinfo")
+
+    def info_rows(self):
+        """[nrows][4]: it, time, max |v|, max |d| of every progress row so far"""
+        n = C.c_int()
+        self._check(self.api.fetch_info(self.h, None, C.byref(n)), "fetch_info")
+        rows = np.zeros((n.value, 4))
+        if n.value:
+            self._check(self.api.fetch_info(self.h, _p(rows), C.byref(n)), "fetch_info")
+        return rows
+
+    def field_info(self):
+        """(max |v|, max |d|, number of non-finite values of d, v, a) of the current fields"""
+        vm, dm, nb = C.c_double(), C.c_double(), C.c_int()
+        self._check(self.api.field_info(self.h, C.byref(vm), C.byref(dm), C.byref(nb)), "field_info")
+        return vm.value, dm.value, nb.value
 
     def fint_noalloc(self, d):
         """as :meth:`fint`, through sem2d_compute_fint_noalloc (no device allocation)"""

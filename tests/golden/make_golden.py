@@ -13,6 +13,8 @@ directory under tests/golden/cases/ and stores compact FP64 vectors:
   fault_* (fault cases)                                 node pairing and FP64 jumps
                                                         d2-d1, v2-v1 of the fault-node
                                                         fields, every fault_every steps
+                                                        (a fault with one side: its
+                                                        nodes, fault_one_sided = 1)
   flt_f32 (fault cases)                                 the reference's own float32
                                                         FltXX_sem2d.dat records
   potency_txt (fault cases)                             the reference's own
@@ -120,6 +122,25 @@ def run_case(name):
                 nout = (min(oxi[1], npf) - oxi[0]) // oxi[2] + 1
                 out["flt_f32_all"] = read_flt(flt, nout)
                 out["flt_oitd"], out["flt_oix1"], out["flt_oixd"] = oitd, oxi[0], oxi[2]
+        par = pf.read_par(os.path.join(src, "Par.inp"))
+        for bd in par.all("BC_DEF"):
+            # a fault with one side (BC_DEF tag = t, on a domain boundary): the harness dumps
+            # no jumps for it; keep its node list, float32 records and potency text
+            t = int(bd.get("tag", 0))
+            if str(bd.get("kind", "")).upper() != "DYNFLT" or t <= 0 or t not in g["bounds"]:
+                continue
+            n1 = g["bounds"][t]["node"]
+            out["fault_node1"] = n1
+            out["fault_one_sided"] = 1
+            every = max(1, g["nt"] // 40)
+            out["fault_every"] = every
+            pot = os.path.join(tmp, "Flt%02d_potency_sem2d.tab" % t)
+            if os.path.exists(pot):
+                with open(pot) as fh:
+                    out["potency_txt"] = np.array(fh.read())
+            flt = os.path.join(tmp, "Flt%02d_sem2d.dat" % t)
+            if os.path.exists(flt):
+                out["flt_f32"] = read_flt(flt, len(n1))[::every]
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
         print(name, "ok", os.path.getsize(os.path.join(HERE, name + ".npz")), "bytes")
     finally:

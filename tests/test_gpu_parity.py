@@ -45,7 +45,8 @@ def test_fint(name, oracle_api):
 CASES = [("small_psv", 300), ("c1_psv", 2000), ("sh_swf", 600), ("multi_src", 300), ("interp_rec", 300),
          ("sh_kv", 600), ("psv_kv", 500), ("psv_rsf", 500), ("hete_psv", 2000), ("rsf_ens", 400),
          ("psv_stacey", 400), ("mesh2d_psv", 500), ("mesh2d_curved", 500), ("mesh2d_sh", 500),
-         ("mesh2d_flt", 500)] + [(n, 200 if n == "rsf_slip" else 500) for n in conftest.FRICTION_CASES]
+         ("mesh2d_flt", 500), ("ratestate", 803)] + [(n, 200 if n == "rsf_slip" else 500)
+                                                    for n in conftest.FRICTION_CASES]
 
 
 @pytest.mark.parametrize("name,nsteps", CASES)
@@ -208,3 +209,29 @@ def test_pipelined_tile_kernel(oracle_api):
         if not fp32:
             for nm, x, y in zip("dva", base, fr):
                 assert rel(x, y) < TOL, nm
+
+
+@pytest.mark.parametrize("name,structured", [("c1_psv", True), ("mesh2d_psv", False), ("sh_swf", True)])
+def test_progress_rows_and_watchdog(name, structured, oracle_api):
+    """The progress rows of the time loop (it, it*dt, max|v|, max|d| every ItInfo steps,
+    SRC/main.f90:73-76) against the oracle's, at the parity bar; then a non-finite value
+    planted in the state from the host: the next row finds it and sem2d_step returns 21."""
+    nt, itinfo = 300, 50
+    pb = builder.build(case_par(name), nsteps=nt)
+    ref = solver.Solver(pb, api=oracle_api, compact=False, itinfo=itinfo)
+    gpu = solver.Solver(pb, structured=structured, itinfo=itinfo)
+    ref.step(nt)
+    gpu.step(nt)
+    rr, rg = ref.info_rows(), gpu.info_rows()
+    print(f"\n  {name}: {rg.shape[0]} progress rows, last {rg[-1].tolist()}", flush=True)
+    assert rr.shape == rg.shape == (nt // itinfo, 4)
+    assert np.array_equal(rg[:, :2], rr[:, :2])
+    assert rel(rg[:, 2:], rr[:, 2:]) < TOL
+    vm, dm, nb = gpu.field_info()
+    assert nb == 0 and vm == rg[-1, 2] and dm == rg[-1, 3]
+    d, v, a = gpu.fields()
+    v[0, pb.npoin // 2] = np.nan
+    gpu.set_fields(d, v, a)
+    assert gpu.field_info()[2] == 1
+    with pytest.raises(solver.SolverError, match=r"\(21\).*non-finite field values at step 350"):
+        gpu.step(itinfo)
