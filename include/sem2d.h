@@ -137,6 +137,14 @@ int sem2d_set_receivers(sem2d_solver *h, int nrec, const int *iglob, int isamp, 
    kept; a record is taken at registration when the current step >= oit. */
 int sem2d_set_fault(sem2d_solver *h, const sem2d_fault_desc *f, int nsamp);
 
+/* Dirichlet components of a boundary: replaces BC_DIRNEU_apply with h or v = 'D'
+   (SRC/bc_dirneu.f90:148-169): the force (MxA) of the n nodes (1-based) is set to zero in
+   the components of comp (1: component 1, 2: component 2, 3: both; SH: 1).  Boundary
+   conditions other than absorbing ones act in the order they are registered
+   (SRC/bc_gen.f90:273-281): a Dirichlet boundary registered after sem2d_set_fault acts
+   after the fault solve, one registered before it acts before. */
+int sem2d_set_dirichlet(sem2d_solver *h, int n, const int *node, int comp);
+
 /* Kelvin-Voigt damping: eta [nelem][ngll][ngll] per element point, already scaled by dt
    when ETAxDT (matwrk%eta of MAT_KV_init_elem_work, SRC/mat_kelvin_voigt.f90:121-135),
    0 outside Kelvin-Voigt elements; the element force becomes -K (d + eta v)

@@ -141,6 +141,9 @@ struct oracle_solver {
   double *d, *v, *a;
   int nabso;
   abso_t abso[16];
+  int ndir;            /* Dirichlet boundaries (BC_DIRNEU 'D' components) */
+  int *dir_node[16];   /* 0-based nodes */
+  int dir_n[16], dir_comp[16], dir_after_fault[16];
   int nsrc;
   source_t src[64];
   int nrec, isamp, field, nsamp, taken;
@@ -198,6 +201,30 @@ int oracle_set_abso(oracle_solver *h, int n, const int *node, const double *C) {
   for (int k = 0; k < n; ++k) b->node[k] = node[k] - 1;
   b->C = dup_d(C, (size_t)h->ndof * n);
   return 0;
+}
+
+/* BC_DIRNEU with Dirichlet components (SRC/bc_dirneu.f90:148-169): MxA(node, c) = 0 for the
+   components in comp (bit 0: component 1, bit 1: component 2), in registration order
+   among the boundary conditions other than absorbing ones (the fault's place included) */
+int oracle_set_dirichlet(oracle_solver *h, int n, const int *node, int comp) {
+  if (h->ndir >= 16) return fail(h, 3, "too many Dirichlet boundaries");
+  if (n <= 0 || !node || comp < 1 || comp > 3) return fail(h, 3, "set_dirichlet: bad arguments");
+  int b = h->ndir++;
+  h->dir_node[b] = (int *)malloc(sizeof(int) * n);
+  for (int k = 0; k < n; ++k) {
+    if (node[k] < 1 || node[k] > h->npoin) return fail(h, 3, "set_dirichlet: node out of range");
+    h->dir_node[b][k] = node[k] - 1;
+  }
+  h->dir_n[b] = n;
+  h->dir_comp[b] = comp;
+  h->dir_after_fault[b] = h->flt != NULL;
+  return 0;
+}
+
+static void dirichlet_apply(oracle_solver *h, int b, double *MxA) {
+  for (int c = 0; c < h->ndof; ++c)
+    if (h->dir_comp[b] >> c & 1)
+      for (int k = 0; k < h->dir_n[b]; ++k) MxA[(size_t)c * h->npoin + h->dir_node[b][k]] = 0.0;
 }
 
 int oracle_set_abso_normals(oracle_solver *h, int n, const double *nrm) {
@@ -860,7 +887,12 @@ static void one_step(oracle_solver *h) {
       free(kxd);
     }
   }
+  /* the other boundary conditions in their order (SRC/bc_gen.f90:273-281) */
+  for (int b = 0; b < h->ndir; ++b)
+    if (!h->dir_after_fault[b]) dirichlet_apply(h, b, f);
   if (h->flt) fault_apply(h, f);
+  for (int b = 0; b < h->ndir; ++b)
+    if (h->dir_after_fault[b]) dirichlet_apply(h, b, f);
   double c3 = gam * dt, c4 = beta * dt * dt;
 #pragma omp parallel for schedule(static) if (n >= 65536)
   for (size_t k = 0; k < n; ++k) {
@@ -996,6 +1028,7 @@ void oracle_destroy(oracle_solver *h) {
   }
   free(h->rnode); free(h->rw); free(h->seis); free(h->kv_eta); free(h->eorder); free(h->floc);
   free(h->info);
+  for (int b = 0; b < h->ndir; ++b) free(h->dir_node[b]);
   if (h->flt) {
     fault_t *f = h->flt;
     free(f->n1); free(f->n2); free(f->nrm); free(f->B); free(f->invM1); free(f->invM2);

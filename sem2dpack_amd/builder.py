@@ -685,9 +685,10 @@ def _materials(par, fname_dir=None):
             kv = par.after(g, "MAT_KV")
             if kv is None:
                 raise InputError("MAT_KV_read: MAT_KV input block not found")
-            if str(kv.get("etah", "")).strip():
-                raise InputError("MAT_KV etaH (distribution) is not supported by this build")
-            mats[tag]["kv_eta"] = float(kv.get("eta", 0.0))
+            # eta, or etaH and the distribution block that follows MAT_KV
+            # (MAT_setProp, SRC/mat_kelvin_voigt.f90:85)
+            mats[tag]["kv_eta"] = _read_cd(par, iter(par.next_groups(kv)), kv.get("eta", 0.0),
+                                           str(kv.get("etah", "")).strip())
             # NormalizeByDT is module-level: the last MAT_KV block read decides for all
             mats["_kv_by_dt"] = bool(kv.get("etaxdt", True))
     if not mats:
@@ -922,6 +923,14 @@ def _read_cd(par, group_iter, value, name_h):
             vals = " ".join(g.data_lines).replace("d", "e").replace("D", "e").split()
             tab = np.array([float(v) for v in vals[:2 * n]]).reshape(n, 2)
         return ("linear", dict(dim=dim, x=tab[:, 0].copy(), val=tab[:, 1].copy()))
+    if want == "DIST_GAUSSIAN":  # read_gaussian_dist (SRC/distribution_gaussian.f90:34-58)
+        c = g.get("centered_at", [0.0, 0.0])
+        ln = g.get("length", [1.0, 1.0])
+        c = c if isinstance(c, list) else [c, 0.0]
+        ln = ln if isinstance(ln, list) else [ln, 1.0]
+        return ("gaussian", dict(x0=float(c[0]), z0=float(c[1]), lx=float(ln[0]), lz=float(ln[1]),
+                                 level=float(g.get("offset", 0.0)), ampli=float(g.get("ampli", 1.0)),
+                                 order=int(g.get("order", 1))))
     if want == "DIST_ORDER0":  # read_order0_dist (SRC/distribution_order0.f90:47-68)
         xn, zn = int(g.get("xn")), int(g.get("zn"))
         recs = list(g.data_lines)
@@ -972,6 +981,14 @@ def _gen_cd(cd, coord):
                     z = iz
                     break
             out[n] = p["val"][z]
+        return out
+    if kind == "gaussian":  # generate_gaussian_dist (SRC/distribution_gaussian.f90:64-75)
+        # the C library's exp, as the reference's Fortran runtime calls it
+        out = np.empty(coord.shape[0])
+        for n, (x, z) in enumerate(coord[:, :2]):
+            a = (x - p["x0"]) / p["lx"]
+            b = (z - p["z0"]) / p["lz"]
+            out[n] = p["level"] + p["ampli"] * math.exp(-((a * a + b * b) ** p["order"]))
         return out
     if kind == "order0":  # generate_order0_dist (SRC/distribution_order0.f90:72-82)
         return np.array([p["val"][_order0_zone(z, p["zn"], p["zb"])][_order0_zone(x, p["xn"], p["xb"])]
@@ -1534,9 +1551,12 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     pb.rho, pb.cp, pb.cs, pb.lam, pb.mu = rho_p, cp_p, cs_p, lam_p, mu_p
     kv_eta0 = None
     if material is None and elem_props is None and kv_tags:
-        kv_eta0 = np.zeros(nelem)
-        for t, eta in kv_tags.items():
-            kv_eta0[tags == t] = eta
+        kv_eta0 = np.zeros((nelem, 5, 5))
+        for t, spec in kv_tags.items():
+            sel = tags == t
+            # MAT_KV_init_elem_prop: eta at the element's GLL points
+            pts = np.stack([ex[sel].ravel(), ez[sel].ravel()], axis=1)
+            kv_eta0[sel] = _gen_cd(spec, pts).reshape(-1, 5, 5)
     pb.homogeneous = all(homog_tag.values())
 
     jac = _jacobians(num["cx"], num["cz"], xgll)
@@ -1583,7 +1603,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     # SRC/mat_kelvin_voigt.f90:121-135): eta, times dt when ETAxDT
     pb.kv_eta = None
     if kv_eta0 is not None:
-        eta = np.broadcast_to(kv_eta0[:, None, None], (nelem, 5, 5)).copy()
+        eta = kv_eta0
         pb.kv_eta = dt * eta if mats.get("_kv_by_dt", True) else eta
     pb.nt = int(nbsteps if nsteps is None else nsteps)
     pb.grid_cfl = grid_cfl
@@ -1649,6 +1669,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     coefA2Vrhs = 1.0 * gamma * dt
     bcdefs = par.all("BC_DEF")
     present = set(_bound_elements(mesh).keys())
+    pb.dirichlet = []
     for bd in bcdefs:
         kind = bd.get("kind", " ")
         tag = bd.get("tag", 0)
@@ -1669,14 +1690,24 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
             dn = par.after(bd, "BC_DIRNEU")
             if dn is None:
                 raise InputError("bc_DIRNEU_read: no BC_DIRNEU block found")
-            for comp in ("h", "v"):
+            mask = 0
+            for bit, comp in enumerate(("h", "v")):
                 val = str(dn.get(comp, "N")).upper()
                 if val not in ("N", "D"):
                     raise InputError(f"bc_DIRNEU_read: {comp} must be N or D")
-                if val == "D":
-                    raise InputError("BC_DIRNEU Dirichlet components are not supported by this build")
-                if str(dn.get(comp + "stf", "none")).lower() != "none":
+                if val == "D" and bit < ndof:  # SH: only h (SRC/bc_dirneu.f90:161)
+                    mask |= 1 << bit
+                elif str(dn.get(comp + "stf", "none")).lower() != "none":
                     raise InputError("BC_DIRNEU time-dependent tractions are not supported by this build")
+            if mask:
+                # BC_DIRNEU_init (SRC/bc_dirneu.f90:117-143): a flat, vertical or horizontal boundary
+                t = _bound_topo(t1, mesh, num, perm_inv)
+                nrm, _ = _normal_and_weights(t, jac, wgll)
+                if not ((np.abs(nrm[:, 0]) < TINY_XABS).all() or (np.abs(nrm[:, 1]) < TINY_XABS).all()):
+                    raise InputError("BC_DIRNEU_init: boundary is not vertical or horizontal")
+                # applied in list order among the non-absorbing conditions (SRC/bc_gen.f90:273-281)
+                pb.dirichlet.append(dict(tag=t1, node=t.node.astype(np.int32), comp=mask,
+                                         after_fault=pb.fault is not None))
         elif kind == "DYNFLT":
             if pb.fault is not None:
                 raise InputError("more than one DYNFLT boundary is not supported by this build")

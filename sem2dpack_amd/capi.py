@@ -39,6 +39,7 @@ _SIGS = {
     "set_abso": ([C.c_void_p, C.c_int, ip, dp], C.c_int),
     "set_abso_stacey": ([C.c_void_p, C.c_int, ip, dp], C.c_int),
     "set_abso_normals": ([C.c_void_p, C.c_int, dp], C.c_int),
+    "set_dirichlet": ([C.c_void_p, C.c_int, ip, C.c_int], C.c_int),
     "set_force": ([C.c_void_p, C.c_int, dp, ip], C.c_int),
     "set_moment": ([C.c_void_p, C.c_int, ip, ip, dp, dp, ip], C.c_int),
     "set_stf": ([C.c_void_p, C.c_int, C.c_int, dp], C.c_int),

@@ -38,6 +38,10 @@ struct FaultDev {
   // structured grids: inverse mass of node 1 / node 2 of each pair [ndof][np], for the
   // corrector that k_fault applies after the solve
   const double *rmn1 = nullptr, *rmn2 = nullptr;
+  // structured grids: Dirichlet components of the Dirichlet boundaries that act after
+  // the fault, zeroed in the force of node 1 (bits 0-1) and node 2 (bits 2-3) before the
+  // corrector (null: none)
+  const unsigned char* dmask = nullptr;
 };
 
 // Corrector of one split-fault node after the fault solve (a = f * rmass, then v and d;
@@ -264,8 +268,11 @@ __global__ void k_fault(FaultDev F, R* __restrict__ d, R* __restrict__ v, R* __r
     F.V[c * np + k] = dV[c] + F.a2v * da;
   }
   if constexpr (CORR) {
+    const int dm = F.dmask ? F.dmask[k] : 0;
     for (int c = 0; c < nd; ++c) {
       const size_t o = (size_t)c * npoin;
+      if (dm >> c & 1) f[o + i1] = 0;
+      if (i2 >= 0 && (dm >> (2 + c) & 1)) f[o + i2] = 0;
       fault_node_correct<R>(d, v, f, o + i1, (R)F.rmn1[c * np + k], c3, c4);
       if (i2 >= 0) fault_node_correct<R>(d, v, f, o + i2, (R)F.rmn2[c * np + k], c3, c4);
     }

@@ -287,6 +287,17 @@ __global__ void k_receivers_interp(int nrec, int ndof, const int* __restrict__ n
 }
 #pragma clang fp contract(on)
 
+// BC_DIRNEU with Dirichlet components (SRC/bc_dirneu.f90:148-169): the force (MxA) of the
+// boundary's nodes set to zero in the components of comp (bit c: component c)
+template <typename R>
+__global__ void k_dirichlet(int n, const int* __restrict__ node, int comp, int ndof, R* __restrict__ f,
+                            size_t npoin) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  for (int c = 0; c < ndof; ++c)
+    if (comp >> c & 1) f[(size_t)c * npoin + node[k]] = 0;
+}
+
 // Progress line of the reference's time loop (SRC/main.f90:45,73-76): max |v| and
 // max |d| over the fields, and the number of non-finite values of d, v, a (the
 // watchdog).  The maxima of non-negative doubles are kept as their bit patterns, whose

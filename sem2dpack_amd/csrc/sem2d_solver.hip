@@ -121,6 +121,14 @@ struct sem2d_solver {
   int* ibool = nullptr;  // colour-sorted, 0-based
   std::vector<int> colour_start;  // ncolour+1 offsets into the sorted order
   std::vector<int> elem_order;    // sorted position -> original element (0-based)
+  // Dirichlet boundaries (BC_DIRNEU 'D' components), in registration order
+  struct DirBC {
+    std::vector<int> node;  // 0-based reference nodes
+    int comp = 0;           // bit c: component c set to zero
+    bool after_fault = false;
+    int* node_dev = nullptr;
+  };
+  std::vector<DirBC> dirs;
   // boundaries and sources
   std::vector<AbsoDev> abso;
   std::vector<AbsoHost> abso_host;
@@ -600,7 +608,7 @@ bool pack_defs(const std::vector<DefNode>& in, const std::vector<int>& part_off,
     DefDev e;
     e.g = n.g;
     e.pstart = n.pstart;
-    e.cnt = n.pcnt | (n.scnt << 8) | (n.acnt << 16) | ((n.fault ? 1 : 0) << 24);
+    e.cnt = n.pcnt | (n.scnt << 8) | (n.acnt << 16) | ((n.fault ? 1 : 0) << 24) | ((n.dmask & 3) << 26);
     e.x = -1;
     if (n.scnt > 0 || n.acnt > 0) {
       e.x = (int)ext.size();
@@ -648,6 +656,7 @@ struct StructTables {
   std::vector<double> rmdef;     // [ndof][ndef]
   std::vector<int> fault_g;
   std::vector<double> fault_rm;  // [ndof][nf]
+  std::vector<unsigned char> fault_dmask;  // [npairs] Dirichlet after the fault: node 1 bits 0-1, node 2 bits 2-3
   std::vector<int> s_enode, ak_off, ak_idx;  // Stacey (see StructState)
   std::vector<double> s_K;
 };
@@ -702,6 +711,15 @@ int struct_tables(sem2d_solver* h, StructTables& X) {
   std::vector<char> is_fault((size_t)h->npoin, 0);
   for (int k : h->fault_n1) is_fault[k] = 1;
   for (int k : h->fault_n2) is_fault[k] = 1;
+  // Dirichlet components per reference node: those applied by k_deferred (every
+  // Dirichlet boundary at a node off the fault; at a fault node only those that act
+  // before the fault) and those k_fault applies after the fault solve
+  std::vector<unsigned char> dmask_def((size_t)h->npoin, 0), dmask_aft((size_t)h->npoin, 0);
+  for (const auto& D : h->dirs)
+    for (int k : D.node) {
+      if (is_fault[k] && D.after_fault) dmask_aft[k] |= (unsigned char)D.comp;
+      else dmask_def[k] |= (unsigned char)D.comp;
+    }
   // Stacey: global list of boundary elements (in boundary order) and, per boundary and
   // boundary-local node, its element points e*NG+k in element order
   X.s_enode.clear();
@@ -757,11 +775,24 @@ int struct_tables(sem2d_solver* h, StructTables& X) {
       }
     n.acnt = (int)(X.dabso.size() / nd) - n.astart;
     n.fault = is_fault[ref];
+    n.dmask = dmask_def[ref];
     X.defs.push_back(n);
     def_ref.push_back(ref);
     i = j;
   }
   const size_t ndef = X.defs.size();
+  {  // every Dirichlet node is a deferred node (tile perimeters hold the box's sides)
+    size_t ndir_def = 0, ndir = 0;
+    for (size_t q = 0; q < X.defs.size(); ++q) ndir_def += X.defs[q].dmask || dmask_aft[def_ref[q]] ? 1 : 0;
+    for (size_t k = 0; k < dmask_def.size(); ++k) ndir += dmask_def[k] || dmask_aft[k] ? 1 : 0;
+    if (ndir_def != ndir)
+      return set_err(h, 5, "structured grid: %zu of the %zu Dirichlet nodes are not on tile edges", ndir_def, ndir);
+  }
+  X.fault_dmask.assign(h->fault_n1.size(), 0);
+  for (size_t k = 0; k < h->fault_n1.size(); ++k) {
+    X.fault_dmask[k] = dmask_aft[h->fault_n1[k]];
+    if (!h->fault_n2.empty()) X.fault_dmask[k] |= (unsigned char)(dmask_aft[h->fault_n2[k]] << 2);
+  }
   // every split-fault node must be a deferred node: k_deferred leaves those nodes
   // uncorrected and k_fault finishes both nodes of each pair; a fault node inside a
   // tile would be corrected by the tile kernel and again by k_fault
@@ -878,6 +909,15 @@ int prepare_structured(sem2d_solver* h) {
     HIPCHK(h, hipStreamSynchronize(h->stream));  // the vectors are local
     h->flt.dev.rmn1 = p1;
     h->flt.dev.rmn2 = p2;
+    h->flt.dev.dmask = nullptr;
+    bool any = false;
+    for (unsigned char m : X.fault_dmask) any |= m != 0;
+    if (any) {
+      unsigned char* pm = nullptr;
+      if ((rc = prep_upload(h, &pm, X.fault_dmask.data(), X.fault_dmask.size()))) return rc;
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      h->flt.dev.dmask = pm;
+    }
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   S.ready = true;
@@ -1110,7 +1150,16 @@ int do_steps(sem2d_solver* h, int nsteps) {
         k_stacey<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, b.node, b.soff, b.sidx, b.enode, b.K, d, a,
                                                                 h->npoin);
     }
+    // the other boundary conditions in their order (SRC/bc_gen.f90:273-281)
+    for (auto& D : h->dirs)
+      if (!D.after_fault)
+        k_dirichlet<R><<<grid_for(D.node.size(), 128), 128, 0, h->stream>>>((int)D.node.size(), D.node_dev, D.comp,
+                                                                           h->ndof, a, h->npoin);
     if (h->flt.on) k_fault<R><<<grid_for(h->flt.dev.np, 64), 64, 0, h->stream>>>(h->flt.dev, d, v, a, h->npoin);
+    for (auto& D : h->dirs)
+      if (D.after_fault)
+        k_dirichlet<R><<<grid_for(D.node.size(), 128), 128, 0, h->stream>>>((int)D.node.size(), D.node_dev, D.comp,
+                                                                           h->ndof, a, h->npoin);
     k_correct<R><<<gb, 256, 0, h->stream>>>(n, d, v, a, (const R*)h->rmass, c3, c4);
     h->it = it;
     record_receivers<R>(h);
@@ -1757,6 +1806,29 @@ int sem2d_compute_fint_noalloc(sem2d_solver* h, const double* d, double* f) {
   return 0;
 }
 
+int sem2d_set_dirichlet(sem2d_solver* h, int n, const int* node, int comp) {
+  if (!h) return 1;
+  if (n <= 0 || !node) return set_err(h, 3, "set_dirichlet: empty boundary");
+  if (comp < 1 || comp > 3 || (h->ndof == 1 && comp != 1))
+    return set_err(h, 3, "set_dirichlet: comp must be 1 (component 1), 2 (component 2) or 3 (both; P-SV)");
+  sem2d_solver::DirBC D;
+  D.node.resize(n);
+  for (int k = 0; k < n; ++k) {
+    if (node[k] < 1 || node[k] > h->npoin) return set_err(h, 3, "set_dirichlet: node out of range");
+    D.node[k] = node[k] - 1;
+  }
+  D.comp = comp;
+  D.after_fault = h->flt.on;
+  std::vector<int> dn(D.node);
+  if (h->st.on)
+    for (int& k : dn) k = h->st.gmap[k];
+  int rc = upload(h, &D.node_dev, dn.data(), dn.size());
+  if (rc) return rc;
+  h->dirs.push_back(std::move(D));
+  h->st.ready = false;
+  return 0;
+}
+
 int sem2d_set_itinfo(sem2d_solver* h, int itinfo) {
   if (!h) return 1;
   if (itinfo < 0) return set_err(h, 3, "set_itinfo: negative interval");
@@ -1995,6 +2067,7 @@ int check_members(sem2d_batch* b) {
       if (!B.senode.empty()) return berr(b, 3, "member %d: Stacey boundaries are not batched", m);
     if (h->flt.on && h->fault_n2.empty())
       return berr(b, 3, "member %d: faults with one side (on a domain boundary) are not batched", m);
+    if (!h->dirs.empty()) return berr(b, 3, "member %d: Dirichlet boundaries are not batched", m);
     if (h->flt.on) {
       const FaultDev &F = h->flt.dev, &F0 = h0->flt.dev;
       if (F.law != F0.law || F.kind != F0.kind || F.opening != F0.opening || F.healing != F0.healing ||

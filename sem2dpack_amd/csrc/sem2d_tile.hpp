@@ -794,13 +794,17 @@ struct DefNode {
   int astart;   // absorbing coefficients abso[astart .. astart+acnt), [ndof] each
   int acnt;
   int fault;    // 1: split-fault node (finished by the fault kernels)
+  int dmask;    // Dirichlet components zeroed in the force (bit c: component c), after the
+                // absorbing terms; at a fault node, those of the Dirichlet boundaries that
+                // act before the fault
   int pinl;     // device form only: 1 when the two partial offsets are poff (pcnt == 2)
   int poff[2];
 };
 
 // Device form of a deferred-node list: 16 bytes per node (one vector load) plus a side
 // table for the few nodes with sources or absorbing terms.  cnt packs pcnt (bits 0-7),
-// scnt (bits 8-15), acnt (bits 16-23), fault (bit 24) and "offsets inline" (bit 25); x
+// scnt (bits 8-15), acnt (bits 16-23), fault (bit 24), "offsets inline" (bit 25) and
+// the Dirichlet mask (bits 26-27); x
 // is the node's (sstart, astart) entry in ext, or -1 when it has neither.  A node with
 // two partial forces and no ext entry (a tile-edge node) holds its two partial offsets
 // in pstart and x instead (bit 25 set), so part_off is not read for it.
@@ -821,6 +825,7 @@ __device__ __forceinline__ DefNode def_unpack(const DefTable& D, int q) {
   n.acnt = (e.cnt >> 16) & 0xff;
   n.fault = (e.cnt >> 24) & 1;
   n.pinl = (e.cnt >> 25) & 1;
+  n.dmask = (e.cnt >> 26) & 3;
   n.sstart = 0;
   n.astart = 0;
   n.poff[0] = n.poff[1] = 0;
@@ -927,6 +932,11 @@ __global__ __launch_bounds__(256) void k_deferred(int ndef, DefTable defs,
       }
     }
   }
+  // Dirichlet components (BC_DIRNEU_apply, SRC/bc_dirneu.f90:148-169): absorbing
+  // boundaries act first, then the others in their order
+#pragma unroll
+  for (int c = 0; c < NDOF; ++c)
+    if (n.dmask >> c & 1) f[c] = 0;
   if (n.fault) {
 #pragma unroll
     for (int c = 0; c < NDOF; ++c) {

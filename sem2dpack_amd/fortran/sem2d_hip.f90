@@ -37,7 +37,8 @@ module sem2d_hip
   public :: sem2d_create, sem2d_set_kelvin_voigt, sem2d_set_abso, sem2d_set_abso_stacey, &
             sem2d_set_abso_normals, &
             sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
-            sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_step, &
+            sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_set_dirichlet, &
+            sem2d_step, &
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
             sem2d_fetch_potency, &
@@ -179,6 +180,14 @@ module sem2d_hip
       type(c_ptr), value :: h
       real(c_double), intent(out) :: out(*)
       integer(c_int), intent(out) :: nrows
+    end function
+
+    ! BC_DIRNEU Dirichlet components (comp: 1, 2 or 3), in BC order relative to the fault
+    integer(c_int) function sem2d_set_dirichlet(h, n, node, comp) bind(C, name='sem2d_set_dirichlet')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: n, comp
+      integer(c_int), intent(in) :: node(*)
     end function
 
     integer(c_int) function sem2d_compute_fint(h, d, f) bind(C, name='sem2d_compute_fint')

@@ -112,8 +112,17 @@ class Solver:
                                                 _p(cxi), _p(ceta), C.byref(isrc)), "set_moment")
             tab = np.ascontiguousarray(s["stf_table"], dtype=np.float64)
             self._check(self.api.set_stf(self.h, isrc.value, len(tab), _p(tab)), "set_stf")
+        # the other boundary conditions in their order: Dirichlet boundaries before or
+        # after the fault
+        dirs = getattr(pb, "dirichlet", [])
+        for dc in dirs:
+            if not dc["after_fault"]:
+                self._set_dirichlet(dc)
         if pb.fault is not None:
             self._set_fault(pb.fault)
+        for dc in dirs:
+            if dc["after_fault"]:
+                self._set_dirichlet(dc)
         r = pb.receivers
         if r is not None and not r.get("atnode", True):
             el = np.ascontiguousarray(r["elem"], dtype=np.int32)
@@ -127,6 +136,10 @@ class Solver:
             self.nrec, self.nsamp = len(node), r["nsamp"]
             self._check(self.api.set_receivers(self.h, len(node), _p(node, np.int32), int(r["isamp"]),
                                                self.FIELD_CODE[r["field"]], int(r["nsamp"])), "set_receivers")
+
+    def _set_dirichlet(self, dc):
+        node = np.ascontiguousarray(dc["node"], dtype=np.int32)
+        self._check(self.api.set_dirichlet(self.h, len(node), _p(node, np.int32), int(dc["comp"])), "set_dirichlet")
 
     def _set_fault(self, f):
         nd = self.ndof

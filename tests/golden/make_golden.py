@@ -139,8 +139,15 @@ def run_case(name):
                 with open(pot) as fh:
                     out["potency_txt"] = np.array(fh.read())
             flt = os.path.join(tmp, "Flt%02d_sem2d.dat" % t)
-            if os.path.exists(flt):
+            fb = par.first("BC_DYNFLT")
+            oxi = [int(x) for x in fb.get("oxi", [1, 2 ** 31 - 1, 1])]
+            oitd = max(1, int(np.floor(float(fb.get("otd", 0.0)) / g["dt"] + 0.5)))
+            if os.path.exists(flt) and oitd == 1 and oxi[2] == 1 and oxi[0] == 1:
                 out["flt_f32"] = read_flt(flt, len(n1))[::every]
+            elif os.path.exists(flt):
+                nout = (min(oxi[1], len(n1)) - oxi[0]) // oxi[2] + 1
+                out["flt_f32_all"] = read_flt(flt, nout)
+                out["flt_oitd"], out["flt_oix1"], out["flt_oixd"] = oitd, oxi[0], oxi[2]
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
         print(name, "ok", os.path.getsize(os.path.join(HERE, name + ".npz")), "bytes")
     finally:

@@ -45,8 +45,8 @@ def test_fint(name, oracle_api):
 CASES = [("small_psv", 300), ("c1_psv", 2000), ("sh_swf", 600), ("multi_src", 300), ("interp_rec", 300),
          ("sh_kv", 600), ("psv_kv", 500), ("psv_rsf", 500), ("hete_psv", 2000), ("rsf_ens", 400),
          ("psv_stacey", 400), ("mesh2d_psv", 500), ("mesh2d_curved", 500), ("mesh2d_sh", 500),
-         ("mesh2d_flt", 500), ("ratestate", 803)] + [(n, 200 if n == "rsf_slip" else 500)
-                                                    for n in conftest.FRICTION_CASES]
+         ("mesh2d_flt", 500), ("ratestate", 803), ("tpv3_kv", 800), ("tpv3_dirfirst", 600)] + \
+    [(n, 200 if n == "rsf_slip" else 500) for n in conftest.FRICTION_CASES]
 
 
 @pytest.mark.parametrize("name,nsteps", CASES)
@@ -235,3 +235,24 @@ def test_progress_rows_and_watchdog(name, structured, oracle_api):
     assert gpu.field_info()[2] == 1
     with pytest.raises(solver.SolverError, match=r"\(21\).*non-finite field values at step 350"):
         gpu.step(itinfo)
+
+
+def test_dirichlet_after_fault_on_the_tile_path(oracle_api):
+    """The TPV3-type case without Kelvin-Voigt damping runs on the fused tile kernels: its
+    Dirichlet side, listed after the fault, zeroes the vertical force of the corner node
+    after the fault solve (inside k_fault) and that of the other side nodes in
+    k_deferred; FP64 vs the oracle at the parity bar."""
+    from sem2dpack_amd import parfile
+    txt = open(case_par("tpv3_kv")).read()
+    txt = txt.replace("kind='ELAST','KV'", "kind='ELAST'").replace("&MAT_KV etaH='GAUSSIAN' /\n", "")
+    txt = txt.replace("&DIST_GAUSSIAN length=1d6,282.094, ampli=0.1d0 /\n", "")
+    pb = builder.build(parfile.parse_par(txt), nsteps=500)
+    assert pb.kv_eta is None and pb.dirichlet and pb.dirichlet[0]["after_fault"]
+    ref = solver.Solver(pb, api=oracle_api, compact=False)
+    gpu = solver.Solver(pb)
+    ref.step(500)
+    gpu.step(500)
+    errs, bars, _ = compare(pb, gpu, ref)
+    conftest.report("tpv3 no KV, tile path", errs, bars)
+    print("\n  error  " + "  ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+    assert all(v < bars[k] for k, v in errs.items()), errs
