@@ -151,6 +151,13 @@ int sem2d_set_dirichlet(sem2d_solver *h, int n, const int *node, int comp);
    (SRC/mat_gen.f90:435).  Call right after sem2d_create. */
 int sem2d_set_kelvin_voigt(sem2d_solver *h, const double *eta);
 
+/* 2.5D crustal-plane term (Lapusta-Rice; &GENERAL W): beta [nelem][ngll][ngll] per element
+   point (matwrk%beta of MAT_ELAST_init_25D, SRC/mat_elastic.f90:363-384); the element force
+   becomes -K d - beta d, d including the Kelvin-Voigt term (MAT_ELAST_add_25D_f,
+   SRC/mat_elastic.f90:447-459, called from MAT_Fint, SRC/mat_gen.f90:440).  Call right
+   after sem2d_create. */
+int sem2d_set_25d(sem2d_solver *h, const double *beta);
+
 /* Receivers inside elements (REC_LINE AtNode=F; REC_posit/REC_store,
    SRC/receivers.f90:274-285,326-341): element elem[r] (1-based) and its 25 weights
    interp[r][k] = h_i(xi) h_j(eta), k = (j-1)*5 + i, as SE_init_interpol

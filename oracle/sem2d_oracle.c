@@ -150,6 +150,7 @@ struct oracle_solver {
   int *rnode;        /* [nrec], or [nrec][25] element nodes when interpolated */
   double *rw;        /* [nrec][25] interpolation weights, NULL for receivers at nodes */
   double *kv_eta;    /* [nelem][25] Kelvin-Voigt eta (dt-scaled), NULL if none */
+  double *beta25;    /* [nelem][25] 2.5D coefficients (matwrk%beta), NULL if none */
   int *eorder;       /* assembly order of the elements */
   double *floc;      /* [nelem][ndof][NG][NG] element forces of the current step */
   double *seis;
@@ -696,6 +697,12 @@ static void fint(oracle_solver *h, const double *d, double *f, const double *v) 
     }
     const double *a = h->coefs + (h->shared ? 0 : (size_t)e * h->nelast * NG * NG);
     kd2(h, a, U, F);
+    if (h->beta25) { /* MAT_ELAST_add_25D_f (SRC/mat_elastic.f90:447-459) */
+      const double *b = h->beta25 + (size_t)e * NG * NG;
+      for (int c = 0; c < nd; ++c)
+        for (int j = 0; j < NG; ++j)
+          for (int i = 0; i < NG; ++i) F[c][j][i] = F[c][j][i] - b[j * NG + i] * U[c][j][i];
+    }
     double *fl = floc + (size_t)e * nd * NG * NG;
     for (int c = 0; c < nd; ++c)
       for (int j = 0; j < NG; ++j)
@@ -721,6 +728,12 @@ int oracle_compute_fint(oracle_solver *h, const double *d, double *f) {
 
 int oracle_set_kelvin_voigt(oracle_solver *h, const double *eta) {
   h->kv_eta = dup_d(eta, (size_t)h->nelem * NG * NG);
+  return 0;
+}
+
+int oracle_set_25d(oracle_solver *h, const double *beta) {
+  if (!beta) return fail(h, 3, "set_25d: null beta");
+  h->beta25 = dup_d(beta, (size_t)h->nelem * NG * NG);
   return 0;
 }
 
@@ -1028,6 +1041,7 @@ void oracle_destroy(oracle_solver *h) {
   }
   free(h->rnode); free(h->rw); free(h->seis); free(h->kv_eta); free(h->eorder); free(h->floc);
   free(h->info);
+  free(h->beta25);
   for (int b = 0; b < h->ndir; ++b) free(h->dir_node[b]);
   if (h->flt) {
     fault_t *f = h->flt;

@@ -33,6 +33,7 @@ time step, mass and coefficients agree bit for bit with the reference.
 """
 import ctypes
 import math
+import sys
 import os
 
 import numpy as np
@@ -1454,8 +1455,13 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         raise InputError("this build supports ngll=5 only")
     if ndof not in (1, 2):
         raise InputError("ndof must be 1 or 2")
-    if "w" in gen.values:
-        raise InputError("2.5D (W) is not supported by this build")
+    # seismogenic width of 2.5D runs (SRC/input.f90:82-83,119,130): W < huge adds the
+    # crustal-plane term of MAT_ELAST_add_25D_f (SRC/mat_elastic.f90:447-459)
+    W25 = float(gen.get("w", 0.0)) if "w" in gen.values else None
+    if W25 is not None and W25 <= 0.0:
+        raise InputError("GENERAL input block: W must be positive")
+    if W25 is not None and W25 >= sys.float_info.max:
+        W25 = None
     md = par.first("MESH_DEF")
     method = str(md.get("method", "")).upper() if md is not None else ""
     if method == "CARTESIAN":
@@ -1658,6 +1664,27 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
 
     # MAT_MASS_init: sequential element-order assembly (np.add.at is sequential)
     pb.dvol = dvol  # w_i w_j detJ at every element-local point (dM/drho)
+    # 2.5D: beta = dvol * mu * (pi/W)^2 (SH) or dvol * mu * (pi (1-nu)/W)^2 (P-SV), with
+    # nu = lambda/(lambda+mu)/2 and pi = 4 atan(1) (MAT_ELAST_init_25D, SRC/mat_elastic.f90:363-384)
+    pb.beta25 = None
+    if W25 is not None:
+        if elem_props is not None or material is not None:
+            raise InputError("2.5D (W) with per-element model arrays is not supported by this build")
+        pi = 4.0 * math.atan(1.0)
+        if ndof == 1:
+            q = pi / W25
+            beta25 = dvol * pb.mu * (q * q)
+        else:
+            nu = pb.lam / (pb.lam + pb.mu) / 2.0
+            q = pi * (1.0 - nu) / W25
+            beta25 = dvol * pb.mu * (q * q)
+        if flat:
+            # homogeneous materials on a flat grid share the work arrays (beta included)
+            # of the first element of their tag (SRC/mat_gen.f90:357-365)
+            for t in utags:
+                if homog_tag[int(t)]:
+                    beta25[tags == t] = beta25[first_of[int(t)]]
+        pb.beta25 = beta25
     massloc = rho_p * dvol
     M = np.zeros((ndof, npoin))
     np.add.at(M[0], ibool0.ravel(), massloc.ravel())

@@ -36,6 +36,7 @@ class FaultDesc(C.Structure):
 _SIGS = {
     "create": ([C.POINTER(Desc), C.POINTER(C.c_void_p)], C.c_int),
     "set_kelvin_voigt": ([C.c_void_p, dp], C.c_int),
+    "set_25d": ([C.c_void_p, dp], C.c_int),
     "set_abso": ([C.c_void_p, C.c_int, ip, dp], C.c_int),
     "set_abso_stacey": ([C.c_void_p, C.c_int, ip, dp], C.c_int),
     "set_abso_normals": ([C.c_void_p, C.c_int, dp], C.c_int),

@@ -79,7 +79,8 @@ __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int*
                                                       const R* __restrict__ coefs, const R* __restrict__ geom,
                                                       const R* __restrict__ d, R* __restrict__ f,
                                                       int npoin, const R* __restrict__ v = nullptr,
-                                                      const R* __restrict__ kv = nullptr) {
+                                                      const R* __restrict__ kv = nullptr,
+                                                      const R* __restrict__ beta25 = nullptr) {
   __shared__ R U[FE_EPB][NDOF][NP];
   __shared__ R T1[FE_EPB][NDOF][NP];
   __shared__ R T2[FE_EPB][NDOF][NP];
@@ -153,7 +154,10 @@ __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int*
         s2 += T2[le][c][k * NG + i] * Coef<R>::h(j, k);
       }
       R* fp = f + (size_t)c * npoin + node;
-      *fp = *fp + (s1 + s2);
+      if (beta25)  // 2.5D: MAT_ELAST_add_25D_f (SRC/mat_elastic.f90:447-459)
+        *fp = *fp + ((s1 + s2) - beta25[(size_t)e * NP + p] * U[le][c][p]);
+      else
+        *fp = *fp + (s1 + s2);
     }
   }
 }

@@ -156,6 +156,7 @@ struct sem2d_solver {
   AdjState* adj = nullptr;
   std::vector<int> elem_ibool;  // [nelem][25] 0-based reference nodes (interpolated receivers)
   void* kv_eta = nullptr;       // Kelvin-Voigt viscosity [element in colour order][25] (bulk type)
+  void* beta25 = nullptr;       // 2.5D coefficients [element in colour order][25] (bulk type)
   // progress rows (it, time, max |v|, max |d|) every itinfo steps, and the non-finite
   // watchdog (k_field_max, k_info_row)
   int itinfo = 0, info_rows = 0, info_cap = 0;
@@ -343,7 +344,7 @@ void launch_force_t(sem2d_solver* h, const R* d, R* f, const R* v) {
     const int e0 = h->colour_start[c], ne = h->colour_start[c + 1] - e0;
     if (ne <= 0) continue;
     k_force_colour<R, NDOF, NELAST, MODE><<<(ne + FE_EPB - 1) / FE_EPB, 256, 0, h->stream>>>(
-        e0, ne, h->ibool, (const R*)h->coefs, (const R*)h->geom, d, f, h->npoin, v, kv);
+        e0, ne, h->ibool, (const R*)h->coefs, (const R*)h->geom, d, f, h->npoin, v, kv, (const R*)h->beta25);
   }
 }
 
@@ -1474,14 +1475,10 @@ int sem2d_set_receivers(sem2d_solver* h, int nrec, const int* iglob, int isamp, 
   return rc;
 }
 
-int sem2d_set_kelvin_voigt(sem2d_solver* h, const double* eta) {
-  if (!h) return 1;
-  if (!eta) return set_err(h, 3, "set_kelvin_voigt: null eta");
-  if (h->it != 0 || h->nrec || h->flt.on || h->nsrc || !h->abso.empty())
-    return set_err(h, 4, "set_kelvin_voigt: call it right after sem2d_create");
+// the fused tile kernels have neither the viscous nor the 2.5D term: a structured grid
+// with either uses the general element path, with the fields in the reference numbering
+static int use_general_path(sem2d_solver* h) {
   if (h->st.on) {
-    // the fused tile kernels have no viscous term: use the general element path, with
-    // the fields in the reference numbering
     h->st.on = false;
     h->fstride = (size_t)h->npoin;
     const size_t nfs = (size_t)h->ndof * h->fstride * rsize(h);
@@ -1489,10 +1486,35 @@ int sem2d_set_kelvin_voigt(sem2d_solver* h, const double* eta) {
     HIPCHK(h, hipMemsetAsync(h->v, 0, nfs, h->stream));
     HIPCHK(h, hipMemsetAsync(h->a, 0, nfs, h->stream));
   }
+  return 0;
+}
+
+// per-element-point array in the colour-sorted element order of the general path
+static int upload_elem_points(sem2d_solver* h, void** dst, const double* src) {
   std::vector<double> e((size_t)h->nelem * NP);
   for (int k = 0; k < h->nelem; ++k)
-    std::memcpy(&e[(size_t)k * NP], eta + (size_t)h->elem_order[k] * NP, NP * sizeof(double));
-  return upload_bulk(h, &h->kv_eta, e.data(), e.size());
+    std::memcpy(&e[(size_t)k * NP], src + (size_t)h->elem_order[k] * NP, NP * sizeof(double));
+  return upload_bulk(h, dst, e.data(), e.size());
+}
+
+int sem2d_set_kelvin_voigt(sem2d_solver* h, const double* eta) {
+  if (!h) return 1;
+  if (!eta) return set_err(h, 3, "set_kelvin_voigt: null eta");
+  if (h->it != 0 || h->nrec || h->flt.on || h->nsrc || !h->abso.empty())
+    return set_err(h, 4, "set_kelvin_voigt: call it right after sem2d_create");
+  int rc = use_general_path(h);
+  if (rc) return rc;
+  return upload_elem_points(h, &h->kv_eta, eta);
+}
+
+int sem2d_set_25d(sem2d_solver* h, const double* beta) {
+  if (!h) return 1;
+  if (!beta) return set_err(h, 3, "set_25d: null beta");
+  if (h->it != 0 || h->nrec || h->flt.on || h->nsrc || !h->abso.empty())
+    return set_err(h, 4, "set_25d: call it right after sem2d_create");
+  int rc = use_general_path(h);
+  if (rc) return rc;
+  return upload_elem_points(h, &h->beta25, beta);
 }
 
 int sem2d_set_receivers_interp(sem2d_solver* h, int nrec, const int* elem, const double* interp, int isamp,

@@ -38,6 +38,7 @@ module sem2d_hip
             sem2d_set_abso_normals, &
             sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
             sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_set_dirichlet, &
+            sem2d_set_25d, &
             sem2d_step, &
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
@@ -180,6 +181,13 @@ module sem2d_hip
       type(c_ptr), value :: h
       real(c_double), intent(out) :: out(*)
       integer(c_int), intent(out) :: nrows
+    end function
+
+    ! 2.5D crustal-plane coefficients beta(ngll,ngll,nelem) (MAT_ELAST_init_25D)
+    integer(c_int) function sem2d_set_25d(h, beta) bind(C, name='sem2d_set_25d')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: beta(*)
     end function
 
     ! BC_DIRNEU Dirichlet components (comp: 1, 2 or 3), in BC order relative to the fault

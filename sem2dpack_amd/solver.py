@@ -87,6 +87,9 @@ class Solver:
         if getattr(pb, "kv_eta", None) is not None:
             eta = np.ascontiguousarray(pb.kv_eta, dtype=np.float64)
             self._check(self.api.set_kelvin_voigt(self.h, _p(eta)), "set_kelvin_voigt")
+        if getattr(pb, "beta25", None) is not None:
+            beta = np.ascontiguousarray(pb.beta25, dtype=np.float64)
+            self._check(self.api.set_25d(self.h, _p(beta)), "set_25d")
         for ab in pb.abso:
             node = np.ascontiguousarray(ab["node"], dtype=np.int32)
             Cm = np.ascontiguousarray(ab["C"].T, dtype=np.float64)  # [ndof][n]

@@ -83,7 +83,8 @@ def test_oracle_matches_reference(name, oracle_api):
             assert np.array_equal(ours[:, cols], ref32[:, cols])
 
 
-@pytest.mark.parametrize("name", ["sh_swf", "psv_rsf", "rsf_ens", "sh_kv", "ratestate", "tpv3_kv", "tpv3_dirfirst"])
+@pytest.mark.parametrize("name", ["sh_swf", "psv_rsf", "rsf_ens", "sh_kv", "ratestate", "tpv3_kv", "tpv3_dirfirst",
+                                  "vshape_25d", "psv_25d"])
 def test_potency_file_matches_reference(name, oracle_api, tmp_path):
     """FltXX_potency_sem2d.tab written from the oracle's potency rows is byte-identical to
     the reference's file (BC_DYNFLT_potency sums, (6D24.16) formatting)."""
