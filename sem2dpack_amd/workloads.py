@@ -175,6 +175,34 @@ def build_rupture(sc, nt=3000):
 
 
 # ---------------------------------------------------------------------------
+# Antiplane (SH) rupture: 60 km x 30 km, 600 x 300 elements, planar split fault at mid
+# depth, linear slip weakening (MuS 0.677, MuD 0.525, Dc 0.4 m), normal stress 120 MPa,
+# initial shear stress 70 MPa raised to 81.6 MPa within 1.5 km of the centre, absorbing
+# sides, Courant 0.3, fault records every step at every node.
+# ---------------------------------------------------------------------------
+def par_sh_rupture(nx=600, nz=300, lx=60e3, lz=30e3, nt=4000, courant="0.3d0"):
+    bcs = "".join(f"&BC_DEF tag={t}, kind='ABSORB' /\n" for t in (1, 2, 3, 4))
+    return (f"&GENERAL iexec=1, ngll=5, fmax=3d0, ndof=1, title='SH rupture {nx}x{nz}', verbose='0000', "
+            f"ItInfo=100000 /\n"
+            f"&MESH_DEF method='CARTESIAN' /\n"
+            f"&MESH_CART xlim={-0.5 * lx!r},{0.5 * lx!r}, zlim={-0.5 * lz!r},{0.5 * lz!r}, "
+            f"nelem={nx},{nz}, ezflt=-1 /\n"
+            f"&MATERIAL tag=1, kind='ELAST' /\n"
+            f"&MAT_ELASTIC rho=2670d0, cp=6000d0, cs=3464d0 /\n"
+            + bcs +
+            f"&BC_DEF tags=5,6, kind='DYNFLT' /\n"
+            f"&BC_DYNFLT friction='SWF', Tn=-120d6, TtH='PWCONR' /\n"
+            f"&DIST_PWCONR num=2, ref=0d0,0d0 /\n1.5d3\n81.6d6 70d6\n"
+            f"&BC_DYNFLT_SWF Dc=0.4d0, MuS=0.677d0, MuD=0.525d0 /\n"
+            f"&TIME kind='newmark', NbSteps={nt}, Courant={courant} /\n")
+
+
+def build_sh_rupture(nx=600, nz=300, nt=4000):
+    from . import builder, parfile
+    return builder.build(parfile.parse_par(par_sh_rupture(nx, nz, nt=nt)))
+
+
+# ---------------------------------------------------------------------------
 # Adjoint (waveform gradient) problem
 # ---------------------------------------------------------------------------
 # nx x nz elements of 100 m, free surface on top, absorbing bottom, left and right
