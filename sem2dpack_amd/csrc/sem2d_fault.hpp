@@ -176,8 +176,11 @@ __device__ inline double f_rsf_v(const FaultDev& F, int k, double tstick, double
 
 // CORR (structured grids): the pair's two nodes are finished here too (fault_node_correct
 // with the inverse masses rmn1, rmn2); the general path corrects every node afterwards.
+// launched with 64 lanes per workgroup: the Newton iteration's registers are then not
+// capped by the 1024-lane default (which spilled them to scratch)
 template <typename R, bool CORR = false>
-__global__ void k_fault(FaultDev F, R* __restrict__ d, R* __restrict__ v, R* __restrict__ f, int npoin,
+__global__ __launch_bounds__(64) void k_fault(FaultDev F, R* __restrict__ d, R* __restrict__ v, R* __restrict__ f,
+                                              int npoin,
                         R c3 = 0, R c4 = 0) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= F.np) return;
