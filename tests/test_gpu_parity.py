@@ -257,3 +257,17 @@ def test_dirichlet_after_fault_on_the_tile_path(oracle_api):
     conftest.report("tpv3 no KV, tile path", errs, bars)
     print("\n  error  " + "  ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
     assert all(v < bars[k] for k, v in errs.items()), errs
+
+
+def test_ratestate_example_shipped_series():
+    """The reference's full RateState example (803 steps) on the fused tile path
+    against the fault series it ships with (tests/example_series.py): relative L2
+    within the oracle's own distance from them, same peak steps."""
+    from example_series import BARS, check, compare as compare_series
+    pb = builder.build(case_par("ratestate"))
+    gpu = solver.Solver(pb)
+    gpu.step(pb.nt)
+    rows = compare_series(pb, gpu.fault_records())
+    for q, x, e, _, _ in rows:
+        conftest.REPORT.append(("ratestate vs shipped series", f"{q}@{x / 1e3:.0f}km", e, BARS[q]))
+    check(rows)
