@@ -207,6 +207,14 @@ int sem2d_compute_fint(sem2d_solver *h, const double *d, double *f);
    (two host threads, two handles) race. */
 int sem2d_compute_fint_noalloc(sem2d_solver *h, const double *d, double *f);
 
+/* Time scheme (the dispatch of solve, SRC/solver.f90:20-37): kind 0 is the Newmark
+   scheme of the descriptor's beta and gamma (solve_Newmark, SRC/solver.f90:42-84, the
+   default); kind 1 the central-difference scheme (solve_leapfrog, SRC/solver.f90:140-160:
+   d = d + dt v, forces, a = f/M, v = v + dt a, with v the mid-step velocity), which needs
+   beta = 0, gamma = 1/2 in the descriptor and a fault descriptor with CoefA2V = dt
+   (TIME_getCoefA2V, SRC/time.f90:445-458).  Call before the first step. */
+int sem2d_set_time_scheme(sem2d_solver *h, int kind);
+
 /* Progress information of the time loop (SRC/main.f90:45,73-76: every ItInfo steps the
    reference prints it, it*dt, maxval(abs(veloc)), maxval(abs(displ))).  sem2d_set_itinfo
    makes sem2d_step record such a row on the device every itinfo steps (0: off, the

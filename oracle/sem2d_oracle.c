@@ -138,6 +138,7 @@ struct oracle_solver {
   double H[NG][NG]; /* H[i][j] = h'_i(x_j) */
   double *rmass;
   double dt, beta, gamma;
+  int leapfrog;     /* 1: central-difference scheme (SRC/solver.f90:140-160) */
   double *d, *v, *a;
   int nabso;
   abso_t abso[16];
@@ -827,10 +828,17 @@ static void one_step(oracle_solver *h) {
   double dt = h->dt, beta = h->beta, gam = h->gamma;
   double *d = h->d, *v = h->v, *a = h->a;
   double c1 = (0.5 - beta) * dt * dt, c2 = (1.0 - gam) * dt;
+  if (h->leapfrog) {
+    /* solve_leapfrog (SRC/solver.f90:151): d = d + dt*v_mid */
 #pragma omp parallel for schedule(static) if (n >= 65536)
-  for (size_t k = 0; k < n; ++k) {
-    d[k] = d[k] + dt * v[k] + c1 * a[k];
-    v[k] = v[k] + c2 * a[k];
+    for (size_t k = 0; k < n; ++k) d[k] = d[k] + dt * v[k];
+  } else {
+    /* solve_Newmark predictors (SRC/solver.f90:58-60) */
+#pragma omp parallel for schedule(static) if (n >= 65536)
+    for (size_t k = 0; k < n; ++k) {
+      d[k] = d[k] + dt * v[k] + c1 * a[k];
+      v[k] = v[k] + c2 * a[k];
+    }
   }
   double *f = a; /* f => a (SRC/solver.f90:52) */
   fint(h, d, f, v);
@@ -907,11 +915,21 @@ static void one_step(oracle_solver *h) {
   for (int b = 0; b < h->ndir; ++b)
     if (h->dir_after_fault[b]) dirichlet_apply(h, b, f);
   double c3 = gam * dt, c4 = beta * dt * dt;
+  if (h->leapfrog) {
+    /* a = rmass*f; v_mid = v_mid + dt*a (SRC/solver.f90:157-158) */
 #pragma omp parallel for schedule(static) if (n >= 65536)
-  for (size_t k = 0; k < n; ++k) {
-    a[k] = f[k] * h->rmass[k];
-    v[k] = v[k] + c3 * a[k];
-    d[k] = d[k] + c4 * a[k];
+    for (size_t k = 0; k < n; ++k) {
+      a[k] = h->rmass[k] * f[k];
+      v[k] = v[k] + dt * a[k];
+    }
+  } else {
+    /* a = f*rmass; correctors (SRC/solver.f90:77-82) */
+#pragma omp parallel for schedule(static) if (n >= 65536)
+    for (size_t k = 0; k < n; ++k) {
+      a[k] = f[k] * h->rmass[k];
+      v[k] = v[k] + c3 * a[k];
+      d[k] = d[k] + c4 * a[k];
+    }
   }
   h->it = it;
   rec_store(h);
@@ -951,6 +969,16 @@ int oracle_step(oracle_solver *h, int nsteps) {
       return 21;
     }
   }
+  return 0;
+}
+
+/* kind 0: Newmark (solve_Newmark, SRC/solver.f90:42-84), 1: leapfrog (solve_leapfrog,
+   SRC/solver.f90:140-160); the dispatch of solve (SRC/solver.f90:20-37) */
+int oracle_set_time_scheme(oracle_solver *h, int kind) {
+  if (kind != 0 && kind != 1) return fail(h, 3, "set_time_scheme: kind must be 0 (newmark) or 1 (leapfrog)");
+  if (kind == 1 && (h->beta != 0.0 || h->gamma != 0.5))
+    return fail(h, 3, "set_time_scheme: leapfrog needs beta = 0, gamma = 1/2 in the descriptor");
+  h->leapfrog = kind;
   return 0;
 }
 

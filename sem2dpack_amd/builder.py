@@ -1005,7 +1005,7 @@ def _gen_cd(cd, coord):
     raise ValueError(kind)
 
 
-def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma, nt, ndof):
+def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma, nt, ndof, scheme="newmark"):
     """BC_DYNFLT_read + BC_DYNFLT_init (SRC/bc_dynflt.f90:102-520)."""
     g = par.after(bcdef, "BC_DYNFLT")
     if g is None:
@@ -1085,8 +1085,10 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
     NORM, W = _normal_and_weights(t1, jac, wgll)
     n1 = NORM[keep]
     B = np.repeat(W[keep][:, None], ndof, axis=1)
-    coefA2V = gamma * dt
-    coefA2D = beta * dt * dt
+    # TIME_getCoefA2V / TIME_getCoefA2D (SRC/time.f90:425-458): gamma dt and beta dt^2 for
+    # Newmark, dt and 0 for leapfrog
+    coefA2V = dt if scheme == "leapfrog" else gamma * dt
+    coefA2D = 0.0 if scheme == "leapfrog" else beta * dt * dt
     invM1 = 1.0 / M[:, node1 - 1].T
     if two_sides:
         invM2 = 1.0 / M[:, node2 - 1].T
@@ -1577,9 +1579,9 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     cvel = cp_p if ndof == 2 else cs_p
     grid_cfl = float((cvel[:, :-1, :-1] / np.minimum(d1, d2)).max())
     tg = par.first("TIME")
-    kind = tg.get("kind", "leapfrog") if tg is not None else "leapfrog"
-    if kind != "newmark":
-        raise InputError("this build supports TIME kind='newmark' only")
+    kind = str(tg.get("kind", "leapfrog")).strip().lower() if tg is not None else "leapfrog"
+    if kind not in ("newmark", "leapfrog"):
+        raise InputError("this build supports TIME kind='newmark' or 'leapfrog' only")
     nbsteps = int(tg.get("nbsteps", 0))
     dt_in = float(tg.get("dt", 0.0))
     courant = float(tg.get("courant", 0.5))
@@ -1588,7 +1590,9 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         raise InputError("TIME: Courant out of range [0,0.6]")
     if nbsteps * total != 0:
         raise InputError("TIME: bad combination of settings, NbSteps or TotalTime")
-    tn = par.after(tg, "TIME_NEWMARK")
+    # TIME_read (SRC/time.f90:195-215): alpha = 1, beta = 0, gamma = 1/2 unless a Newmark
+    # run gives TIME_NEWMARK
+    tn = par.after(tg, "TIME_NEWMARK") if kind == "newmark" else None
     beta = float(tn.get("beta", 0.0)) if tn is not None else 0.0
     gamma = float(tn.get("gamma", 0.5)) if tn is not None else 0.5
     if dt_in > 0:
@@ -1600,11 +1604,13 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         dt = courant / grid_cfl
         if total > 0:
             nbsteps = math.ceil(total / dt)
-    omega_max = math.sqrt(2.0 / gamma)
+    # Omega_max: 2 for leapfrog, sqrt(2/gamma) for Newmark (SRC/time.f90:205-222)
+    omega_max = 2.0 if kind == "leapfrog" else math.sqrt(2.0 / gamma)
     critical = omega_max * 3.0 / 7.0 / math.sqrt(2.0)
     if courant > critical:
         raise InputError("TIME_init: Courant number too high (IO_warning aborts)")
     pb.dt, pb.beta, pb.gamma, pb.alpha = dt, beta, gamma, 1.0
+    pb.scheme = kind
     # Kelvin-Voigt viscosity per element point (MAT_KV_init_elem_work,
     # SRC/mat_kelvin_voigt.f90:121-135): eta, times dt when ETAxDT
     pb.kv_eta = None
@@ -1738,7 +1744,8 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         elif kind == "DYNFLT":
             if pb.fault is not None:
                 raise InputError("more than one DYNFLT boundary is not supported by this build")
-            pb.fault = _dynflt_init(par, bd, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma, pb.nt, ndof)
+            pb.fault = _dynflt_init(par, bd, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma, pb.nt, ndof,
+                                    scheme=pb.scheme)
         else:
             raise InputError(f"BC kind {kind!r} is not supported by this build")
 

@@ -114,6 +114,7 @@ struct sem2d_solver {
   int ndof, nelem, npoin, nelast, coef_mode, fp32;
   size_t fstride = 0;  // component stride of the device field arrays
   double dt, beta, gamma;
+  int leapfrog = 0;  // 1: central-difference scheme (SRC/solver.f90:140-160)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // bulk arrays (type R = double or float, stored as raw bytes)
@@ -1108,6 +1109,21 @@ void launch_structured_step(const StepDev& X, int it, R dt, R c1, R c2, R c3, R 
   }
 }
 
+// Predictor / corrector coefficients of one step: dp = d + dt v + c1 a, vp = v + c2 a,
+// a' = f / M, v' = vp + c3 a', d' = dp + c4 a'.  Newmark (SRC/solver.f90:58-82):
+// c1 = (1/2 - beta) dt^2, c2 = (1 - gamma) dt, c3 = gamma dt, c4 = beta dt^2.  Leapfrog
+// (SRC/solver.f90:151-158: d = d + dt v_mid, v_mid = v_mid + dt a) is c1 = c2 = c4 = 0,
+// c3 = dt.
+template <typename R>
+struct StepCoefs {
+  R dt, c1, c2, c3, c4;
+};
+template <typename R>
+StepCoefs<R> step_coefs(double dt, double beta, double gamma, int leapfrog) {
+  if (leapfrog) return {(R)dt, (R)0, (R)0, (R)dt, (R)0};
+  return {(R)dt, (R)((0.5 - beta) * dt * dt), (R)((1.0 - gamma) * dt), (R)(gamma * dt), (R)(beta * dt * dt)};
+}
+
 template <typename R>
 int do_steps(sem2d_solver* h, int nsteps) {
   int rc = sync_sources(h);
@@ -1116,9 +1132,8 @@ int do_steps(sem2d_solver* h, int nsteps) {
   R* d = (R*)h->d;
   R* v = (R*)h->v;
   R* a = (R*)h->a;
-  const R dt = (R)h->dt;
-  const R c1 = (R)((0.5 - h->beta) * h->dt * h->dt), c2 = (R)((1.0 - h->gamma) * h->dt);
-  const R c3 = (R)(h->gamma * h->dt), c4 = (R)(h->beta * h->dt * h->dt);
+  const StepCoefs<R> K = step_coefs<R>(h->dt, h->beta, h->gamma, h->leapfrog);
+  const R dt = K.dt, c1 = K.c1, c2 = K.c2, c3 = K.c3, c4 = K.c4;
   const int gb = grid_for(n, 256);
   if (h->st.on && !h->st.ready && (rc = prepare_structured(h))) return rc;
   if (h->flt.on && (rc = potency_reserve(h, h->flt.pot_rows + nsteps))) return rc;
@@ -1851,6 +1866,16 @@ int sem2d_set_dirichlet(sem2d_solver* h, int n, const int* node, int comp) {
   return 0;
 }
 
+int sem2d_set_time_scheme(sem2d_solver* h, int kind) {
+  if (!h) return 1;
+  if (kind != 0 && kind != 1) return set_err(h, 3, "set_time_scheme: kind must be 0 (newmark) or 1 (leapfrog)");
+  if (kind == 1 && (h->beta != 0.0 || h->gamma != 0.5))
+    return set_err(h, 3, "set_time_scheme: leapfrog needs beta = 0, gamma = 1/2 in the descriptor");
+  if (h->it != 0) return set_err(h, 4, "set_time_scheme: the solver has already been stepped");
+  h->leapfrog = kind;
+  return 0;
+}
+
 int sem2d_set_itinfo(sem2d_solver* h, int itinfo) {
   if (!h) return 1;
   if (itinfo < 0) return set_err(h, 3, "set_itinfo: negative interval");
@@ -1990,6 +2015,7 @@ struct sem2d_batch {
   std::vector<sem2d_solver*> mem;
   int ndof = 0, nelast = 0, coef_mode = 0, fp32 = 0;
   double dt = 0, beta = 0, gamma = 0;
+  int leapfrog = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   size_t fstride = 0;
@@ -2082,8 +2108,8 @@ int check_members(sem2d_batch* b) {
     if (h->it != 0) return berr(b, 3, "member %d has already been stepped", m);
     if (h->ndof != h0->ndof || h->nelast != h0->nelast || h->coef_mode != h0->coef_mode || h->fp32 != h0->fp32)
       return berr(b, 3, "member %d: ndof, element kernel, coefficient mode and bulk precision must match", m);
-    if (h->dt != h0->dt || h->beta != h0->beta || h->gamma != h0->gamma)
-      return berr(b, 3, "member %d: time step and Newmark parameters must match", m);
+    if (h->dt != h0->dt || h->beta != h0->beta || h->gamma != h0->gamma || h->leapfrog != h0->leapfrog)
+      return berr(b, 3, "member %d: time step, time scheme and Newmark parameters must match", m);
     if (h->flt.on != h0->flt.on) return berr(b, 3, "member %d: either every member has a fault or none", m);
     for (const AbsoHost& B : h->abso_host)
       if (!B.senode.empty()) return berr(b, 3, "member %d: Stacey boundaries are not batched", m);
@@ -2232,9 +2258,8 @@ void launch_batch_record(sem2d_batch* b) {
 
 template <typename R>
 void launch_batch_steps(sem2d_batch* b, int nsteps) {
-  const R dt = (R)b->dt;
-  const R c1 = (R)((0.5 - b->beta) * b->dt * b->dt), c2 = (R)((1.0 - b->gamma) * b->dt);
-  const R c3 = (R)(b->gamma * b->dt), c4 = (R)(b->beta * b->dt * b->dt);
+  const StepCoefs<R> K = step_coefs<R>(b->dt, b->beta, b->gamma, b->leapfrog);
+  const R dt = K.dt, c1 = K.c1, c2 = K.c2, c3 = K.c3, c4 = K.c4;
   for (int s = 0; s < nsteps; ++s) {
     launch_structured_step<R>(b->X, 0, dt, c1, c2, c3, c4);
     if (b->X.fault_on)
@@ -2333,6 +2358,7 @@ int sem2d_batch_create(int n, sem2d_solver* const* members, sem2d_batch** out) {
   b->dt = h0->dt;
   b->beta = h0->beta;
   b->gamma = h0->gamma;
+  b->leapfrog = h0->leapfrog;
   const int nd = b->ndof;
   const size_t rs = b->fp32 ? sizeof(float) : sizeof(double);
   BHIPCHK(b, hipStreamCreate(&b->stream));
@@ -2929,9 +2955,8 @@ int adj_backward(sem2d_solver* h, const double* gseis_host) {
   AdjState& A = *h->adj;
   StructState& S = h->st;
   const size_t GS = h->fstride, nf = (size_t)h->ndof * GS;
-  const R dt = (R)h->dt;
-  const R c1 = (R)((0.5 - h->beta) * h->dt * h->dt), c2 = (R)((1.0 - h->gamma) * h->dt);
-  const R c3 = (R)(h->gamma * h->dt), c4 = (R)(h->beta * h->dt * h->dt);
+  const StepCoefs<R> K = step_coefs<R>(h->dt, h->beta, h->gamma, h->leapfrog);
+  const R dt = K.dt, c1 = K.c1, c2 = K.c2, c3 = K.c3, c4 = K.c4;
   const size_t ns = (size_t)h->nsamp * h->ndof * h->nrec;
   if (h->nrec && gseis_host)
     HIPCHK(h, hipMemcpyAsync(A.gseis, gseis_host, ns * sizeof(double), hipMemcpyHostToDevice, h->stream));
@@ -3036,9 +3061,8 @@ template <typename R>
 int adj_forward_hist(sem2d_solver* h) {
   AdjState& A = *h->adj;
   const size_t nfb = (size_t)h->ndof * h->fstride * sizeof(R);
-  const R dt = (R)h->dt;
-  const R c1 = (R)((0.5 - h->beta) * h->dt * h->dt), c2 = (R)((1.0 - h->gamma) * h->dt);
-  const R c3 = (R)(h->gamma * h->dt), c4 = (R)(h->beta * h->dt * h->dt);
+  const StepCoefs<R> K = step_coefs<R>(h->dt, h->beta, h->gamma, h->leapfrog);
+  const R dt = K.dt, c1 = K.c1, c2 = K.c2, c3 = K.c3, c4 = K.c4;
   int rc = sync_sources(h);
   if (rc) return rc;
   StepDev X = step_dev(h);
@@ -3083,7 +3107,8 @@ int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
   if (!h) return 1;
   if (!h->st.on || h->ndof != 2 || h->coef_mode != COEF_COMPACT || h->nelast != 6)
     return set_err(h, 4, "adjoint: needs a structured P-SV grid with compact (lambda, mu) coefficients");
-  if (h->beta != 0.0) return set_err(h, 4, "adjoint: needs the explicit scheme (beta = 0)");
+  if (h->beta != 0.0 || h->gamma != 0.5 || h->leapfrog)
+    return set_err(h, 4, "adjoint: needs the explicit Newmark scheme (beta = 0, gamma = 1/2)");
   if (h->flt.on) return set_err(h, 4, "adjoint: faults are not differentiated");
   for (const AbsoHost& B : h->abso_host)
     if (!B.senode.empty()) return set_err(h, 4, "adjoint: Stacey boundaries are not differentiated");

@@ -38,7 +38,7 @@ module sem2d_hip
             sem2d_set_abso_normals, &
             sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
             sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_set_dirichlet, &
-            sem2d_set_25d, &
+            sem2d_set_25d, sem2d_set_time_scheme, &
             sem2d_step, &
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
@@ -212,6 +212,13 @@ module sem2d_hip
       type(c_ptr), value :: h
       real(c_double), intent(in) :: d(*)
       real(c_double), intent(out) :: f(*)
+    end function
+
+    ! time scheme: 0 Newmark (default), 1 leapfrog (solve_leapfrog)
+    integer(c_int) function sem2d_set_time_scheme(h, kind) bind(C, name='sem2d_set_time_scheme')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: kind
     end function
 
     ! progress rows (it, it*dt, max|v|, max|d|) every itinfo steps and the non-finite

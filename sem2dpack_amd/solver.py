@@ -84,6 +84,8 @@ class Solver:
 
     def _setup(self):
         pb = self.pb
+        if getattr(pb, "scheme", "newmark") == "leapfrog":
+            self._check(self.api.set_time_scheme(self.h, 1), "set_time_scheme")
         if getattr(pb, "kv_eta", None) is not None:
             eta = np.ascontiguousarray(pb.kv_eta, dtype=np.float64)
             self._check(self.api.set_kelvin_voigt(self.h, _p(eta)), "set_kelvin_voigt")

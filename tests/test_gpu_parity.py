@@ -46,7 +46,7 @@ CASES = [("small_psv", 300), ("c1_psv", 2000), ("sh_swf", 600), ("multi_src", 30
          ("sh_kv", 600), ("psv_kv", 500), ("psv_rsf", 500), ("hete_psv", 2000), ("rsf_ens", 400),
          ("psv_stacey", 400), ("mesh2d_psv", 500), ("mesh2d_curved", 500), ("mesh2d_sh", 500),
          ("mesh2d_flt", 500), ("ratestate", 803), ("tpv3_kv", 800), ("tpv3_dirfirst", 600),
-         ("vshape_25d", 600), ("psv_25d", 600)] + \
+         ("vshape_25d", 600), ("psv_25d", 600), ("lf_psv", 500), ("lf_swf_kv", 600), ("lf_25d", 600)] + \
     [(n, 200 if n == "rsf_slip" else 500) for n in conftest.FRICTION_CASES]
 
 
