@@ -22,7 +22,11 @@
  * available from sem2d_last_error (the reference prints and stops instead,
  * SRC/stdio.f90:205-214).  Host arrays are only read during the call.  One
  * handle owns one HIP stream and its device memory; a handle is not re-entrant.
- * ngll must be 5 (the reference's optimised OPT_NGLL, SRC/constants.f90:6).
+ * ngll is 3 to 10.  At 5 (OPT_NGLL, SRC/constants.f90:6) the elements follow
+ * ELAST_KD2_* and structured boxes use the fused tile kernels; at other values the
+ * elements follow ELAST_KD1_* (SRC/mat_elastic.f90:464-518,645-681) on the general
+ * element kernel (coef_mode 0 or 1; the compact mode 2 needs ngll = 5).  The batch and
+ * the adjoint need ngll = 5.
  */
 #ifndef SEM2D_H
 #define SEM2D_H
@@ -36,7 +40,7 @@ typedef struct sem2d_solver sem2d_solver;
 /* Problem description for sem2d_create. */
 typedef struct sem2d_desc {
   int ndof;              /* 1 = SH (antiplane), 2 = P-SV (in-plane) */
-  int ngll;              /* must be 5 */
+  int ngll;              /* 3 .. 10 */
   int nelem, npoin;
   int nelast;            /* 2, 3 (SH flat/general), 6, 10 (P-SV flat/general) */
   int coef_mode;         /* 0: coefs per element; 1: one coefs set shared by all elements;

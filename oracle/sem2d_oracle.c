@@ -9,10 +9,11 @@
  * sequential C that follows the reference's Fortran statement by statement and in
  * the same floating-point operation order (compile with -ffp-contract=off):
  *
- *   oracle_step          solve_Newmark        SRC/solver.f90:42-84
+ *   oracle_step          solve_Newmark, solve_leapfrog  SRC/solver.f90:42-84,140-160
  *   fint()               compute_Fint         SRC/solver.f90:273-320 (element order)
  *     gather/scatter     FIELD_get_elem_sub / FIELD_add_elem_2  SRC/fields.f90:113-189
- *     kd2_psv, kd2_sh    ELAST_KD2_PSV/SH, My_MATMUL  SRC/mat_elastic.f90:589-641,739-799
+ *     kd()               ELAST_KD2_PSV/SH (NGLL 5), ELAST_KD1_PSV/SH (other NGLL),
+ *                        My_MATMUL, mxm   SRC/mat_elastic.f90:464-518,589-799, SRC/mxmlib.f90
  *   sources              SO_add, FORCE_add, SRC_MOMENT_add  SRC/src_gen.f90:290-317,
  *                        SRC/src_force.f90:77-90, SRC/src_moment.f90:217-235
  *   abso_apply           BC_ABSO_apply (flat and not flat; Stacey P-SV)  SRC/bc_abso.f90:286-336
@@ -90,7 +91,7 @@ static double ulp_pow(double x, double y) { return ulp_rand(pow(x, y)); }
 #define ORACLE_ORDER 0
 #endif
 
-#define NG 5
+#define NGMAX 10 /* largest NGLL (the reference's mxm library covers 2..10) */
 
 typedef struct {
   int n;
@@ -98,8 +99,8 @@ typedef struct {
   double *C;    /* [ndof][n] */
   double *nrm;  /* [2][n] node normals of a boundary that is not flat, NULL if flat */
   int nbe;      /* Stacey: boundary elements (0: none) */
-  int *enode;   /* [nbe][NG] boundary-local node of each edge point */
-  double *K;    /* [nbe][NG][2] */
+  int *enode;   /* [nbe][ngll] boundary-local node of each edge point */
+  double *K;    /* [nbe][ngll][2] */
 } abso_t;
 
 typedef struct {
@@ -107,8 +108,8 @@ typedef struct {
   int iglob;
   double dir[2];
   int nel;
-  int *ixi, *ieta;       /* [nel][NG] 0-based */
-  double *cxi, *ceta;    /* [nel][ndof][NG] */
+  int *ixi, *ieta;       /* [nel][ngll] 0-based */
+  double *cxi, *ceta;    /* [nel][ndof][ngll] */
   int nt;
   double *ampl;
 } source_t;
@@ -133,9 +134,10 @@ typedef struct {
 
 struct oracle_solver {
   int ndof, nelem, npoin, nelast, shared;
-  int *ibool; /* 0-based [nelem][NG][NG] */
+  int *ibool; /* 0-based [nelem][ngll][ngll] */
   double *coefs;
-  double H[NG][NG]; /* H[i][j] = h'_i(x_j) */
+  int ng, npt;      /* NGLL, NGLL^2 */
+  double H[NGMAX][NGMAX]; /* H[i][j] = h'_i(x_j) */
   double *rmass;
   double dt, beta, gamma;
   int leapfrog;     /* 1: central-difference scheme (SRC/solver.f90:140-160) */
@@ -153,7 +155,7 @@ struct oracle_solver {
   double *kv_eta;    /* [nelem][25] Kelvin-Voigt eta (dt-scaled), NULL if none */
   double *beta25;    /* [nelem][25] 2.5D coefficients (matwrk%beta), NULL if none */
   int *eorder;       /* assembly order of the elements */
-  double *floc;      /* [nelem][ndof][NG][NG] element forces of the current step */
+  double *floc;      /* [nelem][ndof][ngll][ngll] element forces of the current step */
   double *seis;
   fault_t *flt;
   int it;
@@ -176,17 +178,19 @@ static double *dup_d(const double *src, size_t n) {
 
 int oracle_create(const sem2d_desc *s, oracle_solver **out) {
   if (!s || !out) return 1;
-  if (s->ngll != NG) return 2;
+  if (s->ngll < 3 || s->ngll > NGMAX) return 2;
   oracle_solver *h = (oracle_solver *)calloc(1, sizeof *h);
+  h->ng = s->ngll;
+  h->npt = s->ngll * s->ngll;
   h->ndof = s->ndof; h->nelem = s->nelem; h->npoin = s->npoin; h->nelast = s->nelast;
   if (s->coef_mode == 2) { free(h); return 2; } /* the oracle uses the reference's a() arrays */
   h->shared = s->coef_mode == 1;
-  h->ibool = (int *)malloc(sizeof(int) * (size_t)s->nelem * NG * NG);
-  for (size_t k = 0; k < (size_t)s->nelem * NG * NG; ++k) h->ibool[k] = s->ibool[k] - 1;
-  size_t nc = (size_t)(h->shared ? 1 : s->nelem) * s->nelast * NG * NG;
+  h->ibool = (int *)malloc(sizeof(int) * (size_t)s->nelem * h->npt);
+  for (size_t k = 0; k < (size_t)s->nelem * h->npt; ++k) h->ibool[k] = s->ibool[k] - 1;
+  size_t nc = (size_t)(h->shared ? 1 : s->nelem) * s->nelast * h->npt;
   h->coefs = dup_d(s->coefs, nc);
-  for (int j = 0; j < NG; ++j)
-    for (int i = 0; i < NG; ++i) h->H[i][j] = s->hprime[j * NG + i];
+  for (int j = 0; j < h->ng; ++j)
+    for (int i = 0; i < h->ng; ++i) h->H[i][j] = s->hprime[j * h->ng + i];
   size_t nf = (size_t)s->ndof * s->npoin;
   h->rmass = dup_d(s->rmass, nf);
   h->dt = s->dt; h->beta = s->beta; h->gamma = s->gamma;
@@ -243,12 +247,12 @@ int oracle_set_abso_stacey(oracle_solver *h, int nbe, const int *enode, const do
   if (h->ndof != 2) return fail(h, 3, "set_abso_stacey: P-SV only");
   abso_t *b = &h->abso[h->nabso - 1];
   if (nbe < 1 || !enode || !K) return fail(h, 3, "set_abso_stacey: empty");
-  for (int k = 0; k < nbe * NG; ++k)
+  for (int k = 0; k < nbe * h->ng; ++k)
     if (enode[k] < 0 || enode[k] >= b->n) return fail(h, 3, "set_abso_stacey: node out of range");
   b->nbe = nbe;
-  b->enode = (int *)malloc(sizeof(int) * (size_t)nbe * NG);
-  memcpy(b->enode, enode, sizeof(int) * (size_t)nbe * NG);
-  b->K = dup_d(K, (size_t)nbe * NG * 2);
+  b->enode = (int *)malloc(sizeof(int) * (size_t)nbe * h->ng);
+  memcpy(b->enode, enode, sizeof(int) * (size_t)nbe * h->ng);
+  b->K = dup_d(K, (size_t)nbe * h->ng * 2);
   return 0;
 }
 
@@ -267,11 +271,11 @@ int oracle_set_moment(oracle_solver *h, int nel, const int *ixi, const int *ieta
   source_t *s = &h->src[h->nsrc];
   memset(s, 0, sizeof *s);
   s->kind = 2; s->nel = nel;
-  s->ixi = (int *)malloc(sizeof(int) * nel * NG);
-  s->ieta = (int *)malloc(sizeof(int) * nel * NG);
-  for (int k = 0; k < nel * NG; ++k) { s->ixi[k] = ixi[k] - 1; s->ieta[k] = ieta[k] - 1; }
-  s->cxi = dup_d(cxi, (size_t)nel * h->ndof * NG);
-  s->ceta = dup_d(ceta, (size_t)nel * h->ndof * NG);
+  s->ixi = (int *)malloc(sizeof(int) * nel * h->ng);
+  s->ieta = (int *)malloc(sizeof(int) * nel * h->ng);
+  for (int k = 0; k < nel * h->ng; ++k) { s->ixi[k] = ixi[k] - 1; s->ieta[k] = ieta[k] - 1; }
+  s->cxi = dup_d(cxi, (size_t)nel * h->ndof * h->ng);
+  s->ceta = dup_d(ceta, (size_t)nel * h->ndof * h->ng);
   if (isrc) *isrc = h->nsrc;
   h->nsrc++;
   return 0;
@@ -298,8 +302,8 @@ static void rec_store(oracle_solver *h) {
       double v;
       if (h->rw) { /* matmul(interp, vloc): SRC/receivers.f90:329-339 */
         v = 0.0;
-        for (int k = 0; k < NG * NG; ++k)
-          v += h->rw[(size_t)r * NG * NG + k] * f[(size_t)c * h->npoin + h->rnode[(size_t)r * NG * NG + k]];
+        for (int k = 0; k < h->npt; ++k)
+          v += h->rw[(size_t)r * h->npt + k] * f[(size_t)c * h->npoin + h->rnode[(size_t)r * h->npt + k]];
       } else {
         v = f[(size_t)c * h->npoin + h->rnode[r]];
       }
@@ -311,10 +315,10 @@ static void rec_store(oracle_solver *h) {
 int oracle_set_receivers_interp(oracle_solver *h, int nrec, const int *elem, const double *interp, int isamp,
                                 int field, int nsamp) {
   h->nrec = nrec; h->isamp = isamp; h->field = field; h->nsamp = nsamp;
-  h->rnode = (int *)malloc(sizeof(int) * nrec * NG * NG);
+  h->rnode = (int *)malloc(sizeof(int) * nrec * h->npt);
   for (int r = 0; r < nrec; ++r)
-    for (int k = 0; k < NG * NG; ++k) h->rnode[r * NG * NG + k] = h->ibool[(size_t)(elem[r] - 1) * NG * NG + k];
-  h->rw = dup_d(interp, (size_t)nrec * NG * NG);
+    for (int k = 0; k < h->npt; ++k) h->rnode[r * h->npt + k] = h->ibool[(size_t)(elem[r] - 1) * h->npt + k];
+  h->rw = dup_d(interp, (size_t)nrec * h->npt);
   h->seis = dup_d(NULL, (size_t)nsamp * h->ndof * nrec);
   h->taken = 0;
   rec_store(h);
@@ -561,82 +565,85 @@ int oracle_set_fault(oracle_solver *h, const sem2d_fault_desc *d, int nsamp) {
 }
 
 /* ---------------- element forces ---------------- */
-static void mm(const double A[NG][NG], const double B[NG][NG], double C[NG][NG]) {
-  /* My_MATMUL, C(i,j) = sum_k A(i,k) B(k,j); arrays indexed [col][row] = Fortran (row,col) */
-  for (int j = 0; j < NG; ++j)
-    for (int i = 0; i < NG; ++i) {
+static void mm(int ng, const double A[NGMAX][NGMAX], const double B[NGMAX][NGMAX], double C[NGMAX][NGMAX]) {
+  /* My_MATMUL / mxm: C(i,j) = sum_k A(i,k) B(k,j) in k order; arrays indexed [col][row] =
+     Fortran (row,col) */
+  for (int j = 0; j < ng; ++j)
+    for (int i = 0; i < ng; ++i) {
       double c = 0.0;
-      for (int k = 0; k < NG; ++k) c = c + A[k][i] * B[j][k];
+      for (int k = 0; k < ng; ++k) c = c + A[k][i] * B[j][k];
       C[j][i] = c;
     }
 }
 
-/* element arrays are stored [j][i] (Fortran (i,j)); H as Fortran matrices below */
-static void kd2(const oracle_solver *h, const double *a, double U[2][NG][NG], double F[2][NG][NG]) {
-  double Hm[NG][NG], Ht[NG][NG]; /* [col][row] */
-  for (int i = 0; i < NG; ++i)
-    for (int j = 0; j < NG; ++j) { Hm[j][i] = h->H[i][j]; Ht[j][i] = h->H[j][i]; }
-#define A(k, i, j) a[((k) * NG + (j)) * NG + (i)]
-  double tmp[NG][NG], t2[NG][NG];
+/* Element forces of one element.  NGLL = 5 (OPT_NGLL): ELAST_KD2_PSV / ELAST_KD2_SH
+   (SRC/mat_elastic.f90:589-641,739-775); other NGLL: ELAST_KD1_PSV / ELAST_KD1_SH
+   (SRC/mat_elastic.f90:464-518,645-681), which differ from the KD2 forms only in the
+   second term of the x force of flat P-SV elements: a4 dUx/deta + a5 dUz/dxi instead of
+   a4 (dUx/deta + dUz/dxi).  Element arrays are stored [j][i] (Fortran (i,j)). */
+static void kd(const oracle_solver *h, const double *a, double U[2][NGMAX][NGMAX], double F[2][NGMAX][NGMAX]) {
+  const int ng = h->ng;
+  const int kd1 = ng != 5;
+  double Hm[NGMAX][NGMAX], Ht[NGMAX][NGMAX]; /* [col][row] */
+  for (int i = 0; i < ng; ++i)
+    for (int j = 0; j < ng; ++j) { Hm[j][i] = h->H[i][j]; Ht[j][i] = h->H[j][i]; }
+#define A(k, i, j) a[((k) * ng + (j)) * ng + (i)]
+#define EACH for (int j = 0; j < ng; ++j) for (int i = 0; i < ng; ++i)
+  double tmp[NGMAX][NGMAX], t2[NGMAX][NGMAX];
   if (h->ndof == 2) {
-    double dxx[NG][NG], dzx[NG][NG], dxe[NG][NG], dze[NG][NG];
-    mm(Ht, U[0], dxx);
-    mm(Ht, U[1], dzx);
-    mm(U[0], Hm, dxe);
-    mm(U[1], Hm, dze);
+    double dxx[NGMAX][NGMAX], dzx[NGMAX][NGMAX], dxe[NGMAX][NGMAX], dze[NGMAX][NGMAX];
+    mm(ng, Ht, U[0], dxx);
+    mm(ng, Ht, U[1], dzx);
+    mm(ng, U[0], Hm, dxe);
+    mm(ng, U[1], Hm, dze);
     if (h->nelast == 6) {
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
-        tmp[j][i] = A(0, i, j) * dxx[j][i] + A(1, i, j) * dze[j][i];
-      mm(Hm, tmp, F[0]);
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
-        tmp[j][i] = A(3, i, j) * (dxe[j][i] + dzx[j][i]);
-      mm(tmp, Ht, t2);
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) F[0][j][i] = F[0][j][i] + t2[j][i];
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
-        tmp[j][i] = A(4, i, j) * dxe[j][i] + A(5, i, j) * dzx[j][i];
-      mm(Hm, tmp, F[1]);
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
-        tmp[j][i] = A(1, i, j) * dxx[j][i] + A(2, i, j) * dze[j][i];
-      mm(tmp, Ht, t2);
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) F[1][j][i] = F[1][j][i] + t2[j][i];
+      EACH tmp[j][i] = A(0, i, j) * dxx[j][i] + A(1, i, j) * dze[j][i];
+      mm(ng, Hm, tmp, F[0]);
+      if (kd1) {
+        EACH tmp[j][i] = A(3, i, j) * dxe[j][i] + A(4, i, j) * dzx[j][i];
+      } else {
+        EACH tmp[j][i] = A(3, i, j) * (dxe[j][i] + dzx[j][i]);
+      }
+      mm(ng, tmp, Ht, t2);
+      EACH F[0][j][i] = F[0][j][i] + t2[j][i];
+      EACH tmp[j][i] = A(4, i, j) * dxe[j][i] + A(5, i, j) * dzx[j][i];
+      mm(ng, Hm, tmp, F[1]);
+      EACH tmp[j][i] = A(1, i, j) * dxx[j][i] + A(2, i, j) * dze[j][i];
+      mm(ng, tmp, Ht, t2);
+      EACH F[1][j][i] = F[1][j][i] + t2[j][i];
     } else {
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
-        tmp[j][i] = A(0, i, j) * dxx[j][i] + A(6, i, j) * dxe[j][i] + A(7, i, j) * dzx[j][i] +
-                    A(1, i, j) * dze[j][i];
-      mm(Hm, tmp, F[0]);
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
-        tmp[j][i] = A(6, i, j) * dxx[j][i] + A(3, i, j) * dxe[j][i] + A(4, i, j) * dzx[j][i] +
-                    A(8, i, j) * dze[j][i];
-      mm(tmp, Ht, t2);
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) F[0][j][i] = F[0][j][i] + t2[j][i];
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
-        tmp[j][i] = A(7, i, j) * dxx[j][i] + A(4, i, j) * dxe[j][i] + A(5, i, j) * dzx[j][i] +
-                    A(9, i, j) * dze[j][i];
-      mm(Hm, tmp, F[1]);
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
-        tmp[j][i] = A(1, i, j) * dxx[j][i] + A(8, i, j) * dxe[j][i] + A(9, i, j) * dzx[j][i] +
-                    A(2, i, j) * dze[j][i];
-      mm(tmp, Ht, t2);
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) F[1][j][i] = F[1][j][i] + t2[j][i];
+      EACH tmp[j][i] = A(0, i, j) * dxx[j][i] + A(6, i, j) * dxe[j][i] + A(7, i, j) * dzx[j][i] +
+                       A(1, i, j) * dze[j][i];
+      mm(ng, Hm, tmp, F[0]);
+      EACH tmp[j][i] = A(6, i, j) * dxx[j][i] + A(3, i, j) * dxe[j][i] + A(4, i, j) * dzx[j][i] +
+                       A(8, i, j) * dze[j][i];
+      mm(ng, tmp, Ht, t2);
+      EACH F[0][j][i] = F[0][j][i] + t2[j][i];
+      EACH tmp[j][i] = A(7, i, j) * dxx[j][i] + A(4, i, j) * dxe[j][i] + A(5, i, j) * dzx[j][i] +
+                       A(9, i, j) * dze[j][i];
+      mm(ng, Hm, tmp, F[1]);
+      EACH tmp[j][i] = A(1, i, j) * dxx[j][i] + A(8, i, j) * dxe[j][i] + A(9, i, j) * dzx[j][i] +
+                       A(2, i, j) * dze[j][i];
+      mm(ng, tmp, Ht, t2);
+      EACH F[1][j][i] = F[1][j][i] + t2[j][i];
     }
   } else {
-    double dx[NG][NG], de[NG][NG];
-    mm(Ht, U[0], dx);
-    mm(U[0], Hm, de);
+    double dx[NGMAX][NGMAX], de[NGMAX][NGMAX];
+    mm(ng, Ht, U[0], dx);
+    mm(ng, U[0], Hm, de);
     if (h->nelast == 2) {
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) tmp[j][i] = A(0, i, j) * dx[j][i];
-      mm(Hm, tmp, F[0]);
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) tmp[j][i] = A(1, i, j) * de[j][i];
+      EACH tmp[j][i] = A(0, i, j) * dx[j][i];
+      mm(ng, Hm, tmp, F[0]);
+      EACH tmp[j][i] = A(1, i, j) * de[j][i];
     } else {
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
-        tmp[j][i] = A(0, i, j) * dx[j][i] + A(2, i, j) * de[j][i];
-      mm(Hm, tmp, F[0]);
-      for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i)
-        tmp[j][i] = A(2, i, j) * dx[j][i] + A(1, i, j) * de[j][i];
+      EACH tmp[j][i] = A(0, i, j) * dx[j][i] + A(2, i, j) * de[j][i];
+      mm(ng, Hm, tmp, F[0]);
+      EACH tmp[j][i] = A(2, i, j) * dx[j][i] + A(1, i, j) * de[j][i];
     }
-    mm(tmp, Ht, t2);
-    for (int j = 0; j < NG; ++j) for (int i = 0; i < NG; ++i) F[0][j][i] = F[0][j][i] + t2[j][i];
+    mm(ng, tmp, Ht, t2);
+    EACH F[0][j][i] = F[0][j][i] + t2[j][i];
   }
+#undef EACH
 #undef A
 }
 
@@ -653,13 +660,13 @@ static void make_order(oracle_solver *h) {
     int ncol = 0;
     for (int e = 0; e < ne; ++e) {
       unsigned long long m = 0;
-      const int *ib = h->ibool + (size_t)e * NG * NG;
-      for (int k = 0; k < NG * NG; ++k) m |= used[ib[k]];
+      const int *ib = h->ibool + (size_t)e * h->npt;
+      for (int k = 0; k < h->npt; ++k) m |= used[ib[k]];
       int c = 0;
       while (m >> c & 1ull) ++c;
       col[e] = (unsigned char)c;
       if (c + 1 > ncol) ncol = c + 1;
-      for (int k = 0; k < NG * NG; ++k) used[ib[k]] |= 1ull << c;
+      for (int k = 0; k < h->npt; ++k) used[ib[k]] |= 1ull << c;
     }
     int q = 0;
     for (int c = 0; c < ncol; ++c)
@@ -681,43 +688,43 @@ static void fint(oracle_solver *h, const double *d, double *f, const double *v) 
   size_t np = h->npoin;
   const int nd = h->ndof, ne = h->nelem;
   if (!h->eorder) make_order(h);
-  if (!h->floc) h->floc = (double *)malloc(sizeof(double) * (size_t)ne * nd * NG * NG);
+  if (!h->floc) h->floc = (double *)malloc(sizeof(double) * (size_t)ne * nd * h->npt);
   double *floc = h->floc;
 #pragma omp parallel for schedule(static) if (ne >= 2048)
   for (int e = 0; e < ne; ++e) {
-    double U[2][NG][NG], F[2][NG][NG];
-    const int *ib = h->ibool + (size_t)e * NG * NG;
+    double U[2][NGMAX][NGMAX], F[2][NGMAX][NGMAX];
+    const int *ib = h->ibool + (size_t)e * h->npt;
     for (int c = 0; c < nd; ++c)
-      for (int j = 0; j < NG; ++j)
-        for (int i = 0; i < NG; ++i) U[c][j][i] = d[c * np + ib[j * NG + i]];
+      for (int j = 0; j < h->ng; ++j)
+        for (int i = 0; i < h->ng; ++i) U[c][j][i] = d[c * np + ib[j * h->ng + i]];
     if (v && h->kv_eta) {
-      const double *eta = h->kv_eta + (size_t)e * NG * NG;
+      const double *eta = h->kv_eta + (size_t)e * h->npt;
       for (int c = 0; c < nd; ++c)
-        for (int j = 0; j < NG; ++j)
-          for (int i = 0; i < NG; ++i) U[c][j][i] = U[c][j][i] + eta[j * NG + i] * v[c * np + ib[j * NG + i]];
+        for (int j = 0; j < h->ng; ++j)
+          for (int i = 0; i < h->ng; ++i) U[c][j][i] = U[c][j][i] + eta[j * h->ng + i] * v[c * np + ib[j * h->ng + i]];
     }
-    const double *a = h->coefs + (h->shared ? 0 : (size_t)e * h->nelast * NG * NG);
-    kd2(h, a, U, F);
+    const double *a = h->coefs + (h->shared ? 0 : (size_t)e * h->nelast * h->npt);
+    kd(h, a, U, F);
     if (h->beta25) { /* MAT_ELAST_add_25D_f (SRC/mat_elastic.f90:447-459) */
-      const double *b = h->beta25 + (size_t)e * NG * NG;
+      const double *b = h->beta25 + (size_t)e * h->npt;
       for (int c = 0; c < nd; ++c)
-        for (int j = 0; j < NG; ++j)
-          for (int i = 0; i < NG; ++i) F[c][j][i] = F[c][j][i] - b[j * NG + i] * U[c][j][i];
+        for (int j = 0; j < h->ng; ++j)
+          for (int i = 0; i < h->ng; ++i) F[c][j][i] = F[c][j][i] - b[j * h->ng + i] * U[c][j][i];
     }
-    double *fl = floc + (size_t)e * nd * NG * NG;
+    double *fl = floc + (size_t)e * nd * h->npt;
     for (int c = 0; c < nd; ++c)
-      for (int j = 0; j < NG; ++j)
-        for (int i = 0; i < NG; ++i) fl[(c * NG + j) * NG + i] = F[c][j][i];
+      for (int j = 0; j < h->ng; ++j)
+        for (int i = 0; i < h->ng; ++i) fl[(c * h->ng + j) * h->ng + i] = F[c][j][i];
   }
   memset(f, 0, sizeof(double) * np * nd);
   for (int q = 0; q < ne; ++q) {
     const int e = h->eorder[q];
-    const int *ib = h->ibool + (size_t)e * NG * NG;
-    const double *fl = floc + (size_t)e * nd * NG * NG;
-    for (int j = 0; j < NG; ++j)
-      for (int i = 0; i < NG; ++i) {
-        int k = ib[j * NG + i];
-        for (int c = 0; c < nd; ++c) f[c * np + k] = f[c * np + k] + fl[(c * NG + j) * NG + i];
+    const int *ib = h->ibool + (size_t)e * h->npt;
+    const double *fl = floc + (size_t)e * nd * h->npt;
+    for (int j = 0; j < h->ng; ++j)
+      for (int i = 0; i < h->ng; ++i) {
+        int k = ib[j * h->ng + i];
+        for (int c = 0; c < nd; ++c) f[c * np + k] = f[c * np + k] + fl[(c * h->ng + j) * h->ng + i];
       }
   }
 }
@@ -728,13 +735,13 @@ int oracle_compute_fint(oracle_solver *h, const double *d, double *f) {
 }
 
 int oracle_set_kelvin_voigt(oracle_solver *h, const double *eta) {
-  h->kv_eta = dup_d(eta, (size_t)h->nelem * NG * NG);
+  h->kv_eta = dup_d(eta, (size_t)h->nelem * h->npt);
   return 0;
 }
 
 int oracle_set_25d(oracle_solver *h, const double *beta) {
   if (!beta) return fail(h, 3, "set_25d: null beta");
-  h->beta25 = dup_d(beta, (size_t)h->nelem * NG * NG);
+  h->beta25 = dup_d(beta, (size_t)h->nelem * h->npt);
   return 0;
 }
 
@@ -855,14 +862,14 @@ static void one_step(oracle_solver *h) {
     } else {
       for (int k = 0; k < S->nel; ++k) {
         for (int c = 0; c < h->ndof; ++c)
-          for (int q = 0; q < NG; ++q) {
-            int g = S->ixi[k * NG + q];
-            f[c * N + g] = f[c * N + g] + amp * S->cxi[((size_t)k * h->ndof + c) * NG + q];
+          for (int q = 0; q < h->ng; ++q) {
+            int g = S->ixi[k * h->ng + q];
+            f[c * N + g] = f[c * N + g] + amp * S->cxi[((size_t)k * h->ndof + c) * h->ng + q];
           }
         for (int c = 0; c < h->ndof; ++c)
-          for (int q = 0; q < NG; ++q) {
-            int g = S->ieta[k * NG + q];
-            f[c * N + g] = f[c * N + g] + amp * S->ceta[((size_t)k * h->ndof + c) * NG + q];
+          for (int q = 0; q < h->ng; ++q) {
+            int g = S->ieta[k * h->ng + q];
+            f[c * N + g] = f[c * N + g] + amp * S->ceta[((size_t)k * h->ndof + c) * h->ng + q];
           }
       }
     }
@@ -892,12 +899,12 @@ static void one_step(oracle_solver *h) {
          displacement, Ht(k,j) = h'_j(x_k) */
       double *kxd = (double *)calloc((size_t)B->n * 2, sizeof(double));
       for (int e = 0; e < B->nbe; ++e) {
-        const int *en = &B->enode[e * NG];
+        const int *en = &B->enode[e * h->ng];
         for (int c = 0; c < 2; ++c)
-          for (int kk = 0; kk < NG; ++kk) {
+          for (int kk = 0; kk < h->ng; ++kk) {
             double s = 0.0;
-            for (int j = 0; j < NG; ++j) s = s + h->H[j][kk] * d[c * N + B->node[en[j]]];
-            kxd[en[kk] * 2 + c] = kxd[en[kk] * 2 + c] + B->K[((size_t)e * NG + kk) * 2 + c] * s;
+            for (int j = 0; j < h->ng; ++j) s = s + h->H[j][kk] * d[c * N + B->node[en[j]]];
+            kxd[en[kk] * 2 + c] = kxd[en[kk] * 2 + c] + B->K[((size_t)e * h->ng + kk) * 2 + c] * s;
           }
       }
       for (int c = 0; c < 2; ++c)

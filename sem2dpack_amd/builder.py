@@ -208,7 +208,10 @@ def gll_info(n=5):
     z = [0.0] * n
     w = [0.0] * n
     nm1 = n - 2
-    if nm1 > 0:  # zwgjd on the interior with alpha = beta = 1
+    if nm1 == 1:  # zwgjd with one point (SRC/gll.f90:594-598), alpha = beta = 1
+        z[1] = (1.0 - 1.0) / (2.0 + 2.0)
+        w[1] = _gammaf(2.0) * _gammaf(2.0) / _gammaf(4.0) * 2.0 ** (2.0 + 1.0)
+    elif nm1 > 0:  # zwgjd on the interior with alpha = beta = 1
         zi = _jacg(nm1, 1.0, 1.0)
         np1, np2 = nm1, nm1 + 1
         dnp1, dnp2 = float(np1), float(np2)
@@ -356,20 +359,21 @@ def _local_code(ngll=5):
 
 
 def _numbering(mesh, perm, xgll):
-    ngll = 5
+    ngll = len(xgll)
+    deg = ngll - 1
     nx, nz, ezflt = mesh["nx"], mesh["nz"], mesh["ezflt"]
     nelem = nx * nz
-    NX1 = 4 * nx + 1
-    NZ1 = 4 * nz + 1 + (1 if ezflt > 0 else 0)
+    NX1 = deg * nx + 1
+    NZ1 = deg * nz + 1 + (1 if ezflt > 0 else 0)
     old = perm.astype(np.int64) - 1
     ix = old % nx
     iz = old // nx
     shift = (iz >= ezflt).astype(np.int64) if ezflt > 0 else np.zeros_like(iz)
     loc = np.arange(ngll)
     # node-grid index of each (e, j, i)
-    X = 4 * ix[:, None, None] + loc[None, None, :]
-    Z = 4 * iz[:, None, None] + loc[None, :, None] + shift[:, None, None]
-    G = Z * NX1 + X  # (nelem, 5, 5)
+    X = deg * ix[:, None, None] + loc[None, None, :]
+    Z = deg * iz[:, None, None] + loc[None, :, None] + shift[:, None, None]
+    G = Z * NX1 + X  # (nelem, ngll, ngll)
     npoin = NX1 * NZ1
     k = np.broadcast_to(np.arange(nelem)[:, None, None], G.shape)
     first = np.full(npoin, nelem, dtype=np.int64)
@@ -400,7 +404,7 @@ def _numbering(mesh, perm, xgll):
     cz = np.stack([zc[czr], zc[czr], zc[czr + 1], zc[czr + 1]], axis=1)
 
     # coordinates: written by the last element touching the node (SRC/spec_grid.f90:342-349)
-    shp = _shape_q4(xgll)  # (5(j), 5(i), 4)
+    shp = _shape_q4(xgll)  # (ngll(j), ngll(i), 4)
     coord = np.empty((npoin, 2))
     selL = last[G] == k
     e_of = np.broadcast_to(np.arange(nelem)[:, None, None], G.shape)[selL]
@@ -463,9 +467,9 @@ def _numbering_fem(mesh, xgll):
     written by every element touching a point, the last one in element order standing."""
     knods = mesh["knods"]
     nelem = len(knods)
-    ngll = 5
+    ngll = len(xgll)
     ivtx, jvtx = (0, ngll - 1, ngll - 1, 0), (0, 0, ngll - 1, ngll - 1)
-    iedg = [_edge_ij(n) for n in (EDGE_D, EDGE_R, EDGE_U, EDGE_L)]  # (i[k], j[k]) counterclockwise
+    iedg = [_edge_ij(n, ngll) for n in (EDGE_D, EDGE_R, EDGE_U, EDGE_L)]  # (i[k], j[k]) counterclockwise
     # FE_SetConnectivity (SRC/fem_grid.f90:103-204): edge n joins vertices n and n+1
     edge_of = {}
     for e in range(nelem):
@@ -519,7 +523,7 @@ def _numbering_fem(mesh, xgll):
 
 def _elastic_coefs_general(DxiDx, DxiDz, DetaDx, DetaDz, w2J, lam, mu, ndof):
     """General-geometry coefficients a(:,:,k) of SRC/mat_elastic.f90:327-358 (nelast 3 and
-    10, isotropic), arrays (n, 5, 5) -> (n, nelast, 5, 5)."""
+    10, isotropic), arrays (n, ngll, ngll) -> (n, nelast, ngll, ngll)."""
     if ndof == 1:
         a = np.stack([mu * DxiDx * DxiDx + mu * DxiDz * DxiDz,
                       mu * DetaDx * DetaDx + mu * DetaDz * DetaDz,
@@ -543,10 +547,11 @@ def _elastic_coefs_general(DxiDx, DxiDz, DetaDx, DetaDz, w2J, lam, mu, ndof):
 def _shape_q4(xgll):
     """Q4 shape functions at the GLL points (SRC/elem_q4.f90:37-53), [j, i, k]."""
     q = 0.25
-    out = np.empty((5, 5, 4))
-    for j in range(5):
+    n = len(xgll)
+    out = np.empty((n, n, 4))
+    for j in range(n):
         t = xgll[j]
-        for i in range(5):
+        for i in range(n):
             s = xgll[i]
             sp, sm, tp, tm = s + 1.0, s - 1.0, t + 1.0, t - 1.0
             out[j, i] = [q * sm * tm, -(q * sp * tm), q * sp * tp, -(q * sm * tp)]
@@ -556,10 +561,11 @@ def _shape_q4(xgll):
 def _dshape_q4(xgll):
     """Q4 shape derivatives (SRC/elem_q4.f90:57-78), [j, i, k, (dxi, deta)]."""
     q = 0.25
-    out = np.empty((5, 5, 4, 2))
-    for j in range(5):
+    n = len(xgll)
+    out = np.empty((n, n, 4, 2))
+    for j in range(n):
         t = xgll[j]
-        for i in range(5):
+        for i in range(n):
             s = xgll[i]
             sp, sm, tp, tm = s + 1.0, s - 1.0, t + 1.0, t - 1.0
             out[j, i, :, 0] = [q * tm, -(q * tm), q * tp, -(q * tp)]
@@ -715,7 +721,7 @@ def _elastic_coefs(DxiDx, DetaDz, w2J, lam, mu, ndof):
 
 
 def _elastic_coefs_batch(DxiDx, DetaDz, w2J, lam, mu, ndof):
-    """_elastic_coefs for a batch of elements: arrays (n, 5, 5) -> (n, nelast, 5, 5)."""
+    """_elastic_coefs for a batch of elements: arrays (n, ngll, ngll) -> (n, nelast, ngll, ngll)."""
     if ndof == 1:
         a = np.stack([mu * DxiDx * DxiDx, mu * DetaDz * DetaDz], axis=1)
     else:
@@ -745,7 +751,7 @@ def _bound_elements(mesh):
     return b
 
 
-def _edge_ij(edge, ngll=5):
+def _edge_ij(edge, ngll):
     k = np.arange(ngll)
     if edge == EDGE_D:
         return k, np.zeros(ngll, int)
@@ -767,12 +773,13 @@ def _bound_topo(tag, mesh, num, perm_inv):
     edges = np.broadcast_to(np.asarray(edge, dtype=np.int64), (len(elem),)).copy()  # one per element
     ibool = num["ibool"]
     coord = num["coord"]
+    ngll = ibool.shape[1]
     nodes = []
-    bibool = np.empty((len(elem), 5), dtype=np.int64)
+    bibool = np.empty((len(elem), ngll), dtype=np.int64)
     index = {}
     for n, e in enumerate(elem):
-        ii, jj = _edge_ij(edges[n])
-        for kloc in range(5):
+        ii, jj = _edge_ij(edges[n], ngll)
+        for kloc in range(ngll):
             g = int(ibool[e, jj[kloc], ii[kloc]])
             if g not in index:
                 index[g] = len(nodes)
@@ -791,22 +798,23 @@ def _bound_topo(tag, mesh, num, perm_inv):
     t.elem = elem
     t.edges = edges
     t.node = nodes[isort]              # 1-based global nodes
-    t.ibool = back[bibool]             # (nelem_b, 5) -> index into t.node
+    t.ibool = back[bibool]             # (nelem_b, ngll) -> index into t.node
     return t
 
 
 def _normal_and_weights(t, jac, wgll):
     """BC_get_normal_and_weights (SRC/spec_grid.f90:961-1011)."""
     J11, J12, J21, J22 = jac
+    ngll = len(wgll)
     npb = len(t.node)
     NORM = np.zeros((npb, 2))
     W = np.zeros(npb)
     for n, e in enumerate(t.elem):
         edge = t.edges[n]
-        ii, jj = _edge_ij(edge)
+        ii, jj = _edge_ij(edge, ngll)
         dim_t = 0 if edge in (EDGE_D, EDGE_U) else 1
         sign = -1.0 if edge in (EDGE_U, EDGE_L) else 1.0
-        for k in range(5):
+        for k in range(ngll):
             i, j = ii[k], jj[k]
             if dim_t == 0:
                 dx, dz = J11[e, j, i], J21[e, j, i]
@@ -819,7 +827,7 @@ def _normal_and_weights(t, jac, wgll):
             NORM[b, 1] += -tx
             W[b] += wgll[k] * jac1d
     for n in range(len(t.elem)):
-        for b in (t.ibool[n, 0], t.ibool[n, 4]):
+        for b in (t.ibool[n, 0], t.ibool[n, ngll - 1]):
             nrm = math.sqrt(NORM[b, 0] ** 2 + NORM[b, 1] ** 2)
             NORM[b] = NORM[b] / nrm
     return NORM, W
@@ -847,12 +855,13 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
     C = np.zeros((len(t.node), ndof))
     rho_p, cp_p, cs_p = props
     stacey = stacey and ndof == 2 and is_flat  # SRC/bc_abso.f90:176
-    K = np.zeros((len(t.elem), 5, 2)) if stacey else None
+    ngll = len(wgll)
+    K = np.zeros((len(t.elem), ngll, 2)) if stacey else None
     cb, ce, cf = [], [], []  # contributions: boundary node, element, weight w_k |J1D|
     for n, e in enumerate(t.elem):
         dim_t = 0 if t.edges[n] in (EDGE_D, EDGE_U) else 1
-        ii, jj = _edge_ij(t.edges[n])
-        for k in range(5):
+        ii, jj = _edge_ij(t.edges[n], ngll)
+        for k in range(ngll):
             i, j = ii[k], jj[k]
             rho = rho_p[e, j, i]
             c = [0.0, 0.0]
@@ -886,7 +895,7 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
     # normal (component geo_nor), cs along the boundary
     return dict(tag=t.tag, node=t.node.astype(np.int32), C=C, geo_nor=geo_nor,
                 contrib=(np.array(cb, dtype=np.int64), np.array(ce, dtype=np.int64), np.array(cf)),
-                stacey=(np.ascontiguousarray(t.ibool[:, :5], dtype=np.int32), K) if stacey else None,
+                stacey=(np.ascontiguousarray(t.ibool[:, :ngll], dtype=np.int32), K) if stacey else None,
                 normals=None if (is_flat or ndof == 1) else NORM)
 
 
@@ -1300,8 +1309,8 @@ def _moment_stencil(iglob, Mt, ibool, jac, hprime, ndof):
         a11, a12, a21, a22 = J11[e, j, i], J12[e, j, i], J21[e, j, i], J22[e, j, i]
         det = a11 * a22 - a12 * a21
         ji = np.array([[a22 / det, (-a12) / det], [(-a21) / det, a11 / det]])
-        cx = np.zeros((ndof, 5))
-        ce = np.zeros((ndof, 5))
+        cx = np.zeros((ndof, ibool.shape[1]))
+        ce = np.zeros((ndof, ibool.shape[1]))
         if ndof == 2:
             G = np.zeros((2, 2))
             jt = ji.T
@@ -1323,12 +1332,12 @@ def _moment_stencil(iglob, Mt, ibool, jac, hprime, ndof):
                 coef_xi=np.array(coef_xi), coef_eta=np.array(coef_eta))
 
 
-def _hgll(i, z, xgll, nz=5):
+def _hgll(i, z, xgll):
     """HGLL (SRC/gll.f90:270-296): Lagrange interpolant through the GLL points."""
     dz = z - xgll[i]
     if abs(dz) < 1.0e-5:
         return 1.0
-    n = nz - 1
+    n = len(xgll) - 1
     alfan = float(n) * (float(n) + 1.0)
     return -(((1.0 - z * z) * _pndleg(z, n)) / ((alfan * _pnleg(xgll[i], n)) * (z - xgll[i])))
 
@@ -1409,11 +1418,12 @@ def _receivers(par, coord, nt, num=None, xgll=None):
             ip, _ = _nearest_node(coord, p)
             e, j, i = (int(v) for v in np.argwhere(ibool == ip)[0])
             xi, eta, new = _find_point(p, num["cx"][e], num["cz"][e], xgll[i], xgll[j])
-            w = np.empty(25)
-            for jj in range(5):
+            ng = len(xgll)
+            w = np.empty(ng * ng)
+            for jj in range(ng):
                 fj = _hgll(jj, eta, xgll)
-                for ii in range(5):
-                    w[jj * 5 + ii] = _hgll(ii, xi, xgll) * fj
+                for ii in range(ng):
+                    w[jj * ng + ii] = _hgll(ii, xi, xgll) * fj
             elems.append(e + 1)
             interp.append(w)
             rc.append(new)
@@ -1453,8 +1463,8 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     ngll = int(gen.get("ngll", 9))
     ndof = int(gen.get("ndof", 2))
     fmax = float(gen.get("fmax", 0.0))
-    if ngll != 5:
-        raise InputError("this build supports ngll=5 only")
+    if not 3 <= ngll <= 10:
+        raise InputError("this build supports ngll from 3 to 10")
     if ndof not in (1, 2):
         raise InputError("ndof must be 1 or 2")
     # seismogenic width of 2.5D runs (SRC/input.f90:82-83,119,130): W < huge adds the
@@ -1485,7 +1495,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     if pb.itinfo <= 0:
         raise InputError("GENERAL input block: itInfo must be positive")
     pb.mesh = mesh
-    xgll, wgll, H = gll5()
+    xgll, wgll, H = gll_info(ngll)
     pb.xgll, pb.wgll, pb.hprime = xgll, wgll, H
     nx, nz = mesh["nx"], mesh["nz"]
     if flat:
@@ -1520,7 +1530,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     ez = pb.coord[ibool0, 1]
     # material properties at the GLL nodes of each element (MAT_init_prop,
     # SRC/mat_gen.f90:204-303; MAT_ELAST_init_elem_prop, SRC/mat_elastic.f90:194-234)
-    shp = (nelem, 5, 5)
+    shp = (nelem, ngll, ngll)
     rho_p, cp_p, cs_p, lam_p, mu_p = (np.empty(shp) for _ in range(5))
     homog_tag = {}
     if elem_props is not None:
@@ -1559,12 +1569,12 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     pb.rho, pb.cp, pb.cs, pb.lam, pb.mu = rho_p, cp_p, cs_p, lam_p, mu_p
     kv_eta0 = None
     if material is None and elem_props is None and kv_tags:
-        kv_eta0 = np.zeros((nelem, 5, 5))
+        kv_eta0 = np.zeros((nelem, ngll, ngll))
         for t, spec in kv_tags.items():
             sel = tags == t
             # MAT_KV_init_elem_prop: eta at the element's GLL points
             pts = np.stack([ex[sel].ravel(), ez[sel].ravel()], axis=1)
-            kv_eta0[sel] = _gen_cd(spec, pts).reshape(-1, 5, 5)
+            kv_eta0[sel] = _gen_cd(spec, pts).reshape(-1, ngll, ngll)
     pb.homogeneous = all(homog_tag.values())
 
     jac = _jacobians(num["cx"], num["cz"], xgll)
@@ -1630,7 +1640,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         # every element its own general-geometry array (SRC/mat_gen.f90:360-364 shares
         # arrays only on flat grids)
         pb.coef_mode = "per_element"
-        coefs = np.empty((nelem, pb.nelast, 5, 5))
+        coefs = np.empty((nelem, pb.nelast, ngll, ngll))
         for c0 in range(0, nelem, 65536):
             b = slice(c0, min(c0 + 65536, nelem))
             coefs[b] = _elastic_coefs_general(DxiDx[b], DxiDz[b], DetaDx[b], DetaDz[b], dvol[b], lam_p[b],
@@ -1642,8 +1652,8 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         pb.coef_mode = "shared"
     else:
         pb.coef_mode = "per_element"
-        if per_element_coefs or pb.homogeneous:
-            coefs = np.empty((nelem, pb.nelast, 5, 5))
+        if per_element_coefs or pb.homogeneous or ngll != 5:
+            coefs = np.empty((nelem, pb.nelast, ngll, ngll))
             for t in utags:
                 sel = tags == t
                 if homog_tag[int(t)]:
@@ -1661,7 +1671,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     # geometric factors of each element, -detJ*DxiDx^2, -detJ*DetaDz^2, -detJ*DxiDx*DetaDz
     # (taken at the element's first GLL node); a(:,:,k) = wgll2 * material * factor.
     pb.geom = pb.lam_mu = None
-    if flat and not (len(utags) == 1 and pb.homogeneous):
+    if flat and ngll == 5 and not (len(utags) == 1 and pb.homogeneous):
         detJ = J11[:, 0, 0] * J22[:, 0, 0] - J12[:, 0, 0] * J21[:, 0, 0]
         gx, gz = DxiDx[:, 0, 0], DetaDz[:, 0, 0]
         pb.geom = np.stack([-detJ * gx * gx, -detJ * gz * gz, -detJ * gx * gz], axis=1)

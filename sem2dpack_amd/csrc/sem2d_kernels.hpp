@@ -10,8 +10,9 @@
 
 namespace sem2d {
 
-constexpr int NG = 5;
+constexpr int NG = 5;  // NGLL of the structured tile kernels (OPT_NGLL, SRC/constants.f90:6)
 constexpr int NP = NG * NG;
+constexpr int NGMAX = 10;  // largest NGLL of the general element kernel
 
 // h'_i(x_j) stored at H[i*5+j], and wgll(i)*wgll(j) at W2[j*5+i] (they only depend on NGLL)
 __constant__ double c_H[NP];
@@ -33,20 +34,25 @@ struct Coef {
 // f = -K d for the elements e0 .. e0+ne-1 (consecutive in the colour-sorted
 // element order), assembled into f with plain read-modify-write: elements of
 // one colour share no node, so the update is race-free and deterministic.
-// Follows ELAST_KD2_PSV / ELAST_KD2_SH (SRC/mat_elastic.f90:589-641, 739-775):
+// Follows ELAST_KD2_PSV / ELAST_KD2_SH (SRC/mat_elastic.f90:589-641, 739-775) at
+// NGLL = 5 and ELAST_KD1_PSV / ELAST_KD1_SH (SRC/mat_elastic.f90:464-518, 645-681)
+// otherwise:
 //   dU/dxi(i,j)  = sum_k H(k,i) U(k,j)      dU/deta(i,j) = sum_k U(i,k) H(k,j)
 //   f(i,j)       = sum_k H(i,k) t1(k,j) + sum_k t2(i,k) H(j,k)
-// One thread per GLL point, 10 elements (250 threads) per workgroup.
+// (sums in k order, as My_MATMUL and mxm form them).  One thread per GLL point,
+// fe_epb(NGLL) elements per workgroup of at most 256 threads.
 // ---------------------------------------------------------------------------
-constexpr int FE_EPB = 10;
+constexpr int fe_epb(int ngl) { return 256 / (ngl * ngl) > 0 ? 256 / (ngl * ngl) : 1; }
+constexpr int FE_EPB = fe_epb(NG);
 constexpr int FE_THREADS = FE_EPB * NP;
 
 // Element coefficients a_k at point p of element e, for the three storage modes.
 // Compact flat form: a(i,j,k) = W2(i,j) * modulus(i,j) * geom_e (SRC/mat_elastic.f90:327-358
 // with -weights = -W2*detJ folded into geom_e).
-template <typename R, int NDOF, int NELAST, int MODE>
+template <typename R, int NDOF, int NELAST, int MODE, int NPL = NP>
 __device__ __forceinline__ void load_coefs(const R* __restrict__ coefs, const R* __restrict__ geom, size_t e, int p,
                                            R* A) {
+  static_assert(MODE != COEF_COMPACT || NPL == NP, "compact coefficients are built for NGLL = 5 only");
   if constexpr (MODE == COEF_COMPACT) {
     const R w = (R)c_W2[p];
     const R g0 = geom[(size_t)e * 3 + 0], g1 = geom[(size_t)e * 3 + 1], g2 = geom[(size_t)e * 3 + 2];
@@ -65,37 +71,51 @@ __device__ __forceinline__ void load_coefs(const R* __restrict__ coefs, const R*
       A[1] = w * mu * g1;
     }
   } else {
-    const R* a = coefs + (MODE == COEF_SHARED ? 0 : (size_t)e * NELAST * NP) + p;
+    const R* a = coefs + (MODE == COEF_SHARED ? 0 : (size_t)e * NELAST * NPL) + p;
 #pragma unroll
-    for (int k = 0; k < NELAST; ++k) A[k] = a[k * NP];
+    for (int k = 0; k < NELAST; ++k) A[k] = a[k * NPL];
   }
 }
 
 // Element forces of one colour.  With Kelvin-Voigt elements (kv != nullptr, [element
-// in colour order][25]) the element displacement is d + eta v (MAT_KV_add_etav,
-// SRC/mat_kelvin_voigt.f90:139-150, applied by MAT_Fint, SRC/mat_gen.f90:435).
-template <typename R, int NDOF, int NELAST, int MODE>
+// in colour order][NGLL^2]) the element displacement is d + eta v (MAT_KV_add_etav,
+// SRC/mat_kelvin_voigt.f90:139-150, applied by MAT_Fint, SRC/mat_gen.f90:435).  NGL is
+// NGLL; for NGL != 5, hg holds h'_i(x_j) at hg[i*NGL+j] (for NGL = 5 it is c_H).
+template <typename R, int NDOF, int NELAST, int MODE, int NGL = NG>
 __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int* __restrict__ ibool,
                                                       const R* __restrict__ coefs, const R* __restrict__ geom,
                                                       const R* __restrict__ d, R* __restrict__ f,
                                                       int npoin, const R* __restrict__ v = nullptr,
                                                       const R* __restrict__ kv = nullptr,
-                                                      const R* __restrict__ beta25 = nullptr) {
-  __shared__ R U[FE_EPB][NDOF][NP];
-  __shared__ R T1[FE_EPB][NDOF][NP];
-  __shared__ R T2[FE_EPB][NDOF][NP];
+                                                      const R* __restrict__ beta25 = nullptr,
+                                                      const double* __restrict__ hg = nullptr) {
+  constexpr int NPL = NGL * NGL;
+  constexpr int EPB = fe_epb(NGL);
+  // ELAST_KD1_PSV: the x force's second term is a4 dUx/deta + a5 dUz/dxi
+  constexpr bool KD1 = NGL != NG;
+  __shared__ R U[EPB][NDOF][NPL];
+  __shared__ R T1[EPB][NDOF][NPL];
+  __shared__ R T2[EPB][NDOF][NPL];
+  __shared__ double sH[KD1 ? NPL : 1];
   const int t = threadIdx.x;
-  const int le = t / NP;
-  const int p = t - le * NP;
-  const int i = p % NG, j = p / NG;
-  const int el = blockIdx.x * FE_EPB + le;
-  const bool active = (t < FE_THREADS) && (el < ne);
+  if constexpr (KD1) {
+    if (t < NPL) sH[t] = hg[t];
+  }
+  auto H = [&](int a, int b) -> R {
+    if constexpr (KD1) return (R)sH[a * NGL + b];
+    else return Coef<R>::h(a, b);
+  };
+  const int le = t / NPL;
+  const int p = t - le * NPL;
+  const int i = p % NGL, j = p / NGL;
+  const int el = blockIdx.x * EPB + le;
+  const bool active = (t < EPB * NPL) && (el < ne);
   const int e = e0 + el;
   int node = 0;
   if (active) {
-    node = ibool[(size_t)e * NP + p];
+    node = ibool[(size_t)e * NPL + p];
     if (kv) {
-      const R eta = kv[(size_t)e * NP + p];
+      const R eta = kv[(size_t)e * NPL + p];
 #pragma unroll
       for (int c = 0; c < NDOF; ++c) U[le][c][p] = d[(size_t)c * npoin + node] + eta * v[(size_t)c * npoin + node];
     } else {
@@ -106,15 +126,15 @@ __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int*
   __syncthreads();
   if (active) {
     R A[NELAST];
-    load_coefs<R, NDOF, NELAST, MODE>(coefs, geom, e, p, A);
+    load_coefs<R, NDOF, NELAST, MODE, NPL>(coefs, geom, e, p, A);
     R dxi[NDOF], deta[NDOF];
 #pragma unroll
     for (int c = 0; c < NDOF; ++c) {
       R s1 = 0, s2 = 0;
 #pragma unroll
-      for (int k = 0; k < NG; ++k) {
-        s1 += Coef<R>::h(k, i) * U[le][c][j * NG + k];
-        s2 += U[le][c][k * NG + i] * Coef<R>::h(k, j);
+      for (int k = 0; k < NGL; ++k) {
+        s1 += H(k, i) * U[le][c][j * NGL + k];
+        s2 += U[le][c][k * NGL + i] * H(k, j);
       }
       dxi[c] = s1;
       deta[c] = s2;
@@ -123,7 +143,8 @@ __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int*
       if constexpr (NELAST == 6) {
         const R a1 = A[0], a2 = A[1], a3 = A[2], a4 = A[3], a5 = A[4], a6 = A[5];
         T1[le][0][p] = a1 * dxi[0] + a2 * deta[1];
-        T2[le][0][p] = a4 * (deta[0] + dxi[1]);
+        if constexpr (KD1) T2[le][0][p] = a4 * deta[0] + a5 * dxi[1];
+        else T2[le][0][p] = a4 * (deta[0] + dxi[1]);
         T1[le][1][p] = a5 * deta[0] + a6 * dxi[1];
         T2[le][1][p] = a2 * dxi[0] + a3 * deta[1];
       } else {
@@ -149,13 +170,13 @@ __global__ __launch_bounds__(256) void k_force_colour(int e0, int ne, const int*
     for (int c = 0; c < NDOF; ++c) {
       R s1 = 0, s2 = 0;
 #pragma unroll
-      for (int k = 0; k < NG; ++k) {
-        s1 += Coef<R>::h(i, k) * T1[le][c][j * NG + k];
-        s2 += T2[le][c][k * NG + i] * Coef<R>::h(j, k);
+      for (int k = 0; k < NGL; ++k) {
+        s1 += H(i, k) * T1[le][c][j * NGL + k];
+        s2 += T2[le][c][k * NGL + i] * H(j, k);
       }
       R* fp = f + (size_t)c * npoin + node;
       if (beta25)  // 2.5D: MAT_ELAST_add_25D_f (SRC/mat_elastic.f90:447-459)
-        *fp = *fp + ((s1 + s2) - beta25[(size_t)e * NP + p] * U[le][c][p]);
+        *fp = *fp + ((s1 + s2) - beta25[(size_t)e * NPL + p] * U[le][c][p]);
       else
         *fp = *fp + (s1 + s2);
     }
@@ -243,21 +264,23 @@ __global__ void k_abso_normal(int n, const int* __restrict__ node, const double*
 
 // Stacey term of one flat P-SV absorbing boundary (SRC/bc_abso.f90:321-333), general path:
 // node q of the boundary (global node node[q]) subtracts KxD = sum over its boundary
-// elements (t = off[q] .. off[q+1]-1, element order; idx[t] = e * NG + k) of
-// K[e][k][c] * sum_j h'_j(x_k) d_c(enode[e][j]), d the predicted displacement.
+// elements (t = off[q] .. off[q+1]-1, element order; idx[t] = e * ng + k) of
+// K[e][k][c] * sum_j h'_j(x_k) d_c(enode[e][j]), d the predicted displacement; hg holds
+// h'_i(x_j) at hg[i*ng+j].
 template <typename R>
 __global__ void k_stacey(int n, const int* __restrict__ node, const int* __restrict__ off,
                          const int* __restrict__ idx, const int* __restrict__ enode, const double* __restrict__ K,
-                         const R* __restrict__ d, R* __restrict__ f, int npoin) {
+                         const R* __restrict__ d, R* __restrict__ f, int npoin, int ng,
+                         const double* __restrict__ hg) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const int g = node[q];
   for (int c = 0; c < 2; ++c) {
     double s = 0.0;
     for (int t = off[q]; t < off[q + 1]; ++t) {
-      const int ek = idx[t], e = ek / NG, k = ek - NG * e;
+      const int ek = idx[t], e = ek / ng, k = ek - ng * e;
       double hd = 0.0;
-      for (int j = 0; j < NG; ++j) hd += c_H[j * NG + k] * (double)d[(size_t)c * npoin + enode[e * NG + j]];
+      for (int j = 0; j < ng; ++j) hd += hg[j * ng + k] * (double)d[(size_t)c * npoin + enode[e * ng + j]];
       s += K[(size_t)ek * 2 + c] * hd;
     }
     R* fp = f + (size_t)c * npoin + g;
@@ -275,17 +298,17 @@ __global__ void k_receivers(int nrec, int ndof, const int* __restrict__ node, co
 }
 
 // Receivers inside elements (REC_store with AtNode = F, SRC/receivers.f90:326-341): the
-// 25 weights h_i(xi) h_j(eta) times the field at the element's nodes, summed in the
-// element's (j, i) order without contraction.
+// np = NGLL^2 weights h_i(xi) h_j(eta) times the field at the element's nodes, summed in
+// the element's (j, i) order without contraction.
 #pragma clang fp contract(off)
 template <typename R>
 __global__ void k_receivers_interp(int nrec, int ndof, const int* __restrict__ node, const double* __restrict__ w,
-                                   const R* __restrict__ field, size_t npoin, double* __restrict__ out) {
+                                   const R* __restrict__ field, size_t npoin, double* __restrict__ out, int np) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrec) return;
   for (int c = 0; c < ndof; ++c) {
     double s = 0.0;
-    for (int k = 0; k < NP; ++k) s += w[(size_t)r * NP + k] * (double)field[(size_t)c * npoin + node[(size_t)r * NP + k]];
+    for (int k = 0; k < np; ++k) s += w[(size_t)r * np + k] * (double)field[(size_t)c * npoin + node[(size_t)r * np + k]];
     out[(size_t)c * nrec + r] = s;
   }
 }

@@ -115,6 +115,8 @@ struct sem2d_solver {
   size_t fstride = 0;  // component stride of the device field arrays
   double dt, beta, gamma;
   int leapfrog = 0;  // 1: central-difference scheme (SRC/solver.f90:140-160)
+  int ng = NG, np = NP;     // NGLL and GLL points per element
+  double* hg = nullptr;     // h'_i(x_j) at hg[i*ng+j] (general element kernel, Stacey)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // bulk arrays (type R = double or float, stored as raw bytes)
@@ -291,6 +293,7 @@ __global__ void k_to_double(const float* __restrict__ x, double* __restrict__ y,
 // in element order.
 int build_colours(sem2d_solver* h, const sem2d_desc* s, std::vector<int>& colour, int& ncol) {
   const int ne = s->nelem;
+  const int NP = s->ngll * s->ngll;  // points per element
   colour.assign(ne, -1);
   if (s->nx > 0 && s->elem_of_cell) {
     if ((long)s->nx * s->nz != ne) return set_err(h, 3, "nx*nz (%d*%d) != nelem (%d)", s->nx, s->nz, ne);
@@ -338,37 +341,54 @@ int build_colours(sem2d_solver* h, const sem2d_desc* s, std::vector<int>& colour
   return 0;
 }
 
-template <typename R, int NDOF, int NELAST, int MODE>
+template <typename R, int NDOF, int NELAST, int MODE, int NGL>
 void launch_force_t(sem2d_solver* h, const R* d, R* f, const R* v) {
   const R* kv = v ? (const R*)h->kv_eta : nullptr;
+  constexpr int EPB = fe_epb(NGL);
   for (size_t c = 0; c + 1 < h->colour_start.size(); ++c) {
     const int e0 = h->colour_start[c], ne = h->colour_start[c + 1] - e0;
     if (ne <= 0) continue;
-    k_force_colour<R, NDOF, NELAST, MODE><<<(ne + FE_EPB - 1) / FE_EPB, 256, 0, h->stream>>>(
-        e0, ne, h->ibool, (const R*)h->coefs, (const R*)h->geom, d, f, h->npoin, v, kv, (const R*)h->beta25);
+    k_force_colour<R, NDOF, NELAST, MODE, NGL><<<(ne + EPB - 1) / EPB, 256, 0, h->stream>>>(
+        e0, ne, h->ibool, (const R*)h->coefs, (const R*)h->geom, d, f, h->npoin, v, kv, (const R*)h->beta25,
+        h->hg);
   }
 }
 
-template <typename R, int NDOF, int NELAST>
+template <typename R, int NDOF, int NELAST, int NGL>
 void launch_force_m(sem2d_solver* h, const R* d, R* f, const R* v) {
   switch (h->coef_mode) {
-    case COEF_SHARED: launch_force_t<R, NDOF, NELAST, COEF_SHARED>(h, d, f, v); break;
-    case COEF_COMPACT:
-      if constexpr (NELAST == 2 || NELAST == 6) launch_force_t<R, NDOF, NELAST, COEF_COMPACT>(h, d, f, v);
+    case COEF_SHARED: launch_force_t<R, NDOF, NELAST, COEF_SHARED, NGL>(h, d, f, v); break;
+    case COEF_COMPACT:  // built for NGLL = 5 only (checked at creation)
+      if constexpr ((NELAST == 2 || NELAST == 6) && NGL == NG)
+        launch_force_t<R, NDOF, NELAST, COEF_COMPACT, NGL>(h, d, f, v);
       break;
-    default: launch_force_t<R, NDOF, NELAST, COEF_PER_ELEM>(h, d, f, v);
+    default: launch_force_t<R, NDOF, NELAST, COEF_PER_ELEM, NGL>(h, d, f, v);
+  }
+}
+
+template <typename R, int NGL>
+void launch_force_n(sem2d_solver* h, const R* d, R* f, const R* v) {
+  if (h->ndof == 2) {
+    if (h->nelast == 6) launch_force_m<R, 2, 6, NGL>(h, d, f, v);
+    else launch_force_m<R, 2, 10, NGL>(h, d, f, v);
+  } else {
+    if (h->nelast == 2) launch_force_m<R, 1, 2, NGL>(h, d, f, v);
+    else launch_force_m<R, 1, 3, NGL>(h, d, f, v);
   }
 }
 
 // f = -K d, or -K (d + eta v) in Kelvin-Voigt elements when v is given
 template <typename R>
 void launch_force(sem2d_solver* h, const R* d, R* f, const R* v = nullptr) {
-  if (h->ndof == 2) {
-    if (h->nelast == 6) launch_force_m<R, 2, 6>(h, d, f, v);
-    else launch_force_m<R, 2, 10>(h, d, f, v);
-  } else {
-    if (h->nelast == 2) launch_force_m<R, 1, 2>(h, d, f, v);
-    else launch_force_m<R, 1, 3>(h, d, f, v);
+  switch (h->ng) {
+    case 3: launch_force_n<R, 3>(h, d, f, v); break;
+    case 4: launch_force_n<R, 4>(h, d, f, v); break;
+    case 6: launch_force_n<R, 6>(h, d, f, v); break;
+    case 7: launch_force_n<R, 7>(h, d, f, v); break;
+    case 8: launch_force_n<R, 8>(h, d, f, v); break;
+    case 9: launch_force_n<R, 9>(h, d, f, v); break;
+    case 10: launch_force_n<R, 10>(h, d, f, v); break;
+    default: launch_force_n<R, NG>(h, d, f, v);
   }
 }
 
@@ -408,7 +428,7 @@ int record_receivers(sem2d_solver* h) {
   const R* fld = (const R*)(h->field == 0 ? h->d : (h->field == 1 ? h->v : h->a));
   if (h->rec_w)
     k_receivers_interp<R><<<grid_for(h->nrec, 64), 64, 0, h->stream>>>(
-        h->nrec, h->ndof, h->rec_node, h->rec_w, fld, h->fstride, h->seis + (size_t)s * h->ndof * h->nrec);
+        h->nrec, h->ndof, h->rec_node, h->rec_w, fld, h->fstride, h->seis + (size_t)s * h->ndof * h->nrec, h->np);
   else
     k_receivers<R><<<grid_for(h->nrec, 64), 64, 0, h->stream>>>(h->nrec, h->ndof, h->rec_node, fld, h->fstride,
                                                                 h->seis + (size_t)s * h->ndof * h->nrec);
@@ -1164,7 +1184,7 @@ int do_steps(sem2d_solver* h, int nsteps) {
         k_abso<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, h->ndof, b.node, b.C, v, a, h->npoin);
       if (b.nbe)
         k_stacey<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, b.node, b.soff, b.sidx, b.enode, b.K, d, a,
-                                                                h->npoin);
+                                                                h->npoin, h->ng, h->hg);
     }
     // the other boundary conditions in their order (SRC/bc_gen.f90:273-281)
     for (auto& D : h->dirs)
@@ -1208,13 +1228,14 @@ extern "C" {
 int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   if (!s || !out) return 1;
   *out = nullptr;
-  if (s->ngll != NG) return 2;
+  if (s->ngll < 3 || s->ngll > NGMAX) return 2;
   if (!(s->ndof == 1 || s->ndof == 2)) return 2;
   if (s->ndof == 2 && s->nelast != 6 && s->nelast != 10) return 2;
   if (s->ndof == 1 && s->nelast != 2 && s->nelast != 3) return 2;
   if (s->nelem <= 0 || s->npoin <= 0 || !s->ibool || !s->hprime || !s->rmass) return 2;
   if (s->coef_mode < 0 || s->coef_mode > 2) return 2;
-  if (s->coef_mode == COEF_COMPACT && (!s->lam_mu || !s->geom || !s->wgll || (s->nelast != 2 && s->nelast != 6)))
+  if (s->coef_mode == COEF_COMPACT &&
+      (!s->lam_mu || !s->geom || !s->wgll || (s->nelast != 2 && s->nelast != 6) || s->ngll != NG))
     return 2;
   if (s->coef_mode != COEF_COMPACT && !s->coefs) return 2;
   sem2d_solver* h = new sem2d_solver();
@@ -1228,6 +1249,9 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   h->dt = s->dt;
   h->beta = s->beta;
   h->gamma = s->gamma;
+  h->ng = s->ngll;
+  h->np = s->ngll * s->ngll;
+  const int NG = h->ng, NP = h->np;  // this problem's NGLL from here on
   for (size_t k = 0; k < (size_t)s->nelem * NP; ++k)
     if (s->ibool[k] < 1 || s->ibool[k] > s->npoin) return set_err(h, 3, "ibool entry %zu out of range", k);
   h->elem_ibool.resize((size_t)s->nelem * NP);
@@ -1236,12 +1260,16 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   HIPCHK(h, hipEventCreate(&h->ev0));
   HIPCHK(h, hipEventCreate(&h->ev1));
   if (!scratch_on_device(h)) return 10;  // set-up of sem2d_compute_fint_noalloc's scratch
-  double Hc[NP];
+  std::vector<double> Hc((size_t)NP);
   for (int i = 0; i < NG; ++i)
     for (int j = 0; j < NG; ++j) Hc[i * NG + j] = s->hprime[j * NG + i];
-  HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_H), Hc, sizeof Hc, 0, hipMemcpyHostToDevice));
+  int rc = upload(h, &h->hg, Hc.data(), Hc.size());
+  if (rc) return rc;
+  // the NGLL = 5 kernels read H from constant memory (the same values for every problem)
+  if (NG == sem2d::NG)
+    HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_H), Hc.data(), NP * sizeof(double), 0, hipMemcpyHostToDevice));
   if (s->coef_mode == COEF_COMPACT) {
-    double W2[NP];
+    double W2[sem2d::NP];
     for (int j = 0; j < NG; ++j)
       for (int i = 0; i < NG; ++i) W2[j * NG + i] = s->wgll[i] * s->wgll[j];
     HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_W2), W2, sizeof W2, 0, hipMemcpyHostToDevice));
@@ -1249,7 +1277,7 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
 
   std::vector<int> colour;
   int ncol = 0;
-  int rc = build_colours(h, s, colour, ncol);
+  rc = build_colours(h, s, colour, ncol);
   if (rc) return rc;
   h->colour_start.assign(ncol + 1, 0);
   for (int e = 0; e < s->nelem; ++e) h->colour_start[colour[e] + 1]++;
@@ -1284,7 +1312,7 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   if ((rc = upload_bulk(h, &h->rmass, s->rmass, nf))) return rc;
   h->rmass_host.assign(s->rmass, s->rmass + nf);
   h->fstride = (size_t)h->npoin;
-  if (structured_map(h, s)) {
+  if (NG == sem2d::NG && structured_map(h, s)) {  // tile kernels: NGLL = 5
     StructState& S = h->st;
     S.on = true;
     S.cell_elem.resize(s->nelem);
@@ -1388,6 +1416,7 @@ int sem2d_set_abso_stacey(sem2d_solver* h, int nbe, const int* enode, const doub
   AbsoDev& ad = h->abso.back();
   if (ad.nbe) return set_err(h, 3, "set_abso_stacey: this boundary already has its Stacey term");
   const int n = (int)ah.node.size();
+  const int NG = h->ng;  // edge points per boundary element
   for (int k = 0; k < nbe * NG; ++k)
     if (enode[k] < 0 || enode[k] >= n) return set_err(h, 3, "set_abso_stacey: node out of range");
   ah.senode.assign(enode, enode + (size_t)nbe * NG);
@@ -1434,6 +1463,7 @@ int sem2d_set_moment(sem2d_solver* h, int nel, const int* ixi, const int* ieta, 
   if (!h) return 1;
   if (nel <= 0 || !ixi || !ieta || !cxi || !ceta) return set_err(h, 3, "set_moment: empty stencil");
   const int s = h->nsrc++;
+  const int NG = h->ng;  // stencil points per element line
   for (int k = 0; k < nel; ++k) {
     // order of SRC/src_moment.f90:229-232: xi line (all components), then eta line
     for (int c = 0; c < h->ndof; ++c)
@@ -1506,6 +1536,7 @@ static int use_general_path(sem2d_solver* h) {
 
 // per-element-point array in the colour-sorted element order of the general path
 static int upload_elem_points(sem2d_solver* h, void** dst, const double* src) {
+  const int NP = h->np;
   std::vector<double> e((size_t)h->nelem * NP);
   for (int k = 0; k < h->nelem; ++k)
     std::memcpy(&e[(size_t)k * NP], src + (size_t)h->elem_order[k] * NP, NP * sizeof(double));
@@ -1538,6 +1569,7 @@ int sem2d_set_receivers_interp(sem2d_solver* h, int nrec, const int* elem, const
   if (nrec <= 0 || !elem || !interp || isamp < 1 || field < 0 || field > 2 || nsamp < 1)
     return set_err(h, 3, "set_receivers_interp: bad arguments");
   if (h->elem_ibool.empty()) return set_err(h, 4, "set_receivers_interp: element table not kept");
+  const int NP = h->np;
   std::vector<int> nd((size_t)nrec * NP);
   for (int r = 0; r < nrec; ++r) {
     if (elem[r] < 1 || elem[r] > h->nelem) return set_err(h, 3, "set_receivers_interp: element out of range");
@@ -3184,7 +3216,7 @@ int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
   // injection plan: the receivers' nodes (or element nodes with weights), grouped by
   // node, contributions in receiver order
   if (h->nrec) {
-    const int per = h->rec_w ? NP : 1;
+    const int per = h->rec_w ? h->np : 1;
     std::vector<int> rn((size_t)h->nrec * per);
     std::vector<double> rw((size_t)h->nrec * per, 1.0);
     HIPCHK(h, hipMemcpy(rn.data(), h->rec_node, rn.size() * sizeof(int), hipMemcpyDeviceToHost));

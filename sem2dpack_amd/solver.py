@@ -35,7 +35,7 @@ class Solver:
         hprime = np.ascontiguousarray(pb.hprime.T, dtype=np.float64)  # [j][i]
         rmass = np.ascontiguousarray(pb.rmass, dtype=np.float64)
         d = capi.Desc()
-        d.ndof, d.ngll, d.nelem, d.npoin = pb.ndof, 5, pb.nelem, pb.npoin
+        d.ndof, d.ngll, d.nelem, d.npoin = pb.ndof, pb.ngll, pb.nelem, pb.npoin
         d.nelast = pb.nelast
         keep = []
         if pb.coef_mode == "shared":

@@ -98,7 +98,11 @@ def run_case(name):
             keep = n1 != n2
             npf = int(keep.sum())
             out["fault_node1"], out["fault_node2"] = n1[keep], n2[keep]
-            every = max(1, g["nt"] // 40)
+            bd = pf.read_par(os.path.join(src, "Par.inp")).first("BC_DYNFLT")
+            otd = float(bd.get("otd", 0.0))
+            oitd = max(1, int(np.floor(otd / g["dt"] + 0.5)))
+            # jumps every `every` steps, a multiple of the fault output interval
+            every = oitd * max(1, g["nt"] // 40 // oitd)
             out["fault_every"] = every
             sel = slice(0, None, every)
             k1 = nrec + np.nonzero(keep)[0]
@@ -111,10 +115,7 @@ def run_case(name):
                 with open(pot) as fh:
                     out["potency_txt"] = np.array(fh.read())
             flt = os.path.join(tmp, "Flt05_sem2d.dat")
-            bd = pf.read_par(os.path.join(src, "Par.inp")).first("BC_DYNFLT")
             oxi = [int(x) for x in bd.get("oxi", [1, 2 ** 31 - 1, 1])]
-            otd = float(bd.get("otd", 0.0))
-            oitd = max(1, int(np.floor(otd / g["dt"] + 0.5)))
             if os.path.exists(flt) and oitd == 1 and oxi[2] == 1:
                 out["flt_f32"] = read_flt(flt, npf)[::every]
             elif os.path.exists(flt):
