@@ -1164,14 +1164,20 @@ def _nint(x):
     return int(math.copysign(math.floor(abs(x) + 0.5), x))
 
 
+# exp, log, asinh and pow of the C library, as the reference's runtime calls them (the
+# vectorised numpy versions may differ from them in the last bit)
+_exp, _log, _asinh = np.vectorize(math.exp), np.vectorize(math.log), np.vectorize(math.asinh)
+_pow = np.vectorize(math.pow)
+
+
 def swf_mu(kind, theta, dc, mus, mud, p, alpha):
     """swf_mu (SRC/bc_dynflt_swf.f90:142-162)."""
     if kind == 1:
         mu = mus - (mus - mud) * np.minimum(theta / dc, 1.0)
     elif kind == 2:
-        mu = mud - (mud - mus) * np.exp(-theta / dc)
+        mu = mud - (mud - mus) * _exp(-theta / dc)
     else:
-        mu = mud + (mus - mud) / (1.0 + theta / dc) ** p
+        mu = mud + (mus - mud) / _pow(1.0 + theta / dc, np.broadcast_to(p, np.shape(theta)))
     return mu + alpha * theta
 
 
@@ -1181,8 +1187,8 @@ def rsf_mu(kind, v, f):
     if kind == 1:
         return mus + a * np.abs(v) / (np.abs(v) + vstar) - b * th / (th + dc)
     if kind in (2, 3):
-        return a * np.arcsinh(np.abs(v) / (2.0 * vstar) * np.exp((mus + b * np.log(vstar * th / dc)) / a))
-    return a * np.arcsinh(np.abs(v) / (2.0 * vstar) * np.exp((mus + b * np.log(f["vc"] * th / dc + 1)) / a))
+        return a * _asinh(np.abs(v) / (2.0 * vstar) * _exp((mus + b * _log(vstar * th / dc)) / a))
+    return a * _asinh(np.abs(v) / (2.0 * vstar) * _exp((mus + b * _log(f["vc"] * th / dc + 1)) / a))
 
 
 # ---------------------------------------------------------------------------

@@ -14,12 +14,77 @@
 * Potency (``BC_DYNFLT_write`` / ``BC_DYNFLT_potency``, ``SRC/bc_dynflt.f90:757,
   832-855``): ``FltXX_potency_sem2d.tab``, one formatted ``(6D24.16)`` line per step.
 
-Numbers in the text headers use Python formatting; the binary files are byte
-compatible with the reference's.
+The binary files and the text headers (list-directed ``WRITE(*,*)`` records, see
+:func:`list_directed`) are byte compatible with the reference's.
 """
+import decimal
 import os
 
 import numpy as np
+
+
+def _shortest_digits(v, kind):
+    """(digits, X) with |v| read back from 0.digits x 10**X, as the reference's Fortran
+    runtime chooses them: the fewest significant digits n for which n-digit decimals lie
+    strictly between the midpoints to |v|'s two neighbours, and among those the middle
+    one ((lowest + highest) // 2 in units of the n-th digit)."""
+    ftype = np.float32 if kind == 4 else np.float64
+    a = ftype(abs(v))
+    x = decimal.Decimal(float(a))
+    below = decimal.Decimal(float(np.nextafter(a, ftype(0))))
+    above = decimal.Decimal(float(np.nextafter(a, ftype(np.inf))))
+    less, more = (x + below) / 2, (x + above) / 2
+    e10 = x.adjusted()  # x = d.ddd x 10**e10
+    with decimal.localcontext() as ctx:
+        ctx.prec = 80
+        for n in range(1, 40):
+            unit = decimal.Decimal(1).scaleb(e10 - n + 1)
+            lo = (less / unit).to_integral_value(rounding=decimal.ROUND_FLOOR) + 1
+            hi = (more / unit).to_integral_value(rounding=decimal.ROUND_CEILING) - 1
+            if lo <= hi:
+                d = (lo + hi) // 2
+                digits = str(int(d))
+                X = e10 + 1 + (len(digits) - n)
+                return digits.rstrip("0") or "0", X
+    raise ValueError(v)
+
+
+def _real_item(x, kind):
+    """One REAL of kind 4 or 8 as the reference's Fortran runtime writes it in
+    list-directed output: the digits of :func:`_shortest_digits`, in F form (no leading
+    zero: ``.677``, ``25.``, ``81600000.``) while the decimal exponent X of 0.ddd x 10**X
+    lies in 0 .. max(6, precision + 1) (7 for kind 4, 16 for kind 8), else in 1P E form
+    (``7.914186712941471E-03``, ``5.E-04``)."""
+    v = np.float32(x) if kind == 4 else np.float64(x)
+    if v == 0:
+        return "-0." if np.signbit(v) else "0."
+    digits, X = _shortest_digits(v, kind)
+    sign = "-" if v < 0 else ""
+    if X < 0 or X > (7 if kind == 4 else 16):
+        e = X - 1
+        return f"{sign}{digits[0]}.{digits[1:]}E{'-' if e < 0 else '+'}{abs(e):02d}"
+    if X >= len(digits):
+        return f"{sign}{digits}{'0' * (X - len(digits))}."
+    return f"{sign}{digits[:X]}.{digits[X:]}"
+
+
+def list_directed(*items):
+    """One record of list-directed output (``WRITE(unit,*) items``): every item after
+    one blank; ints as integers, Python floats and numpy float64 as REAL(8), numpy
+    float32 as REAL(4), strings as they are."""
+    out = []
+    for it in items:
+        if isinstance(it, str):
+            out.append(it)
+        elif isinstance(it, (bool, np.bool_)):
+            out.append("T" if it else "F")
+        elif isinstance(it, (int, np.integer)):
+            out.append(str(int(it)))
+        elif isinstance(it, np.float32):
+            out.append(_real_item(it, 4))
+        else:
+            out.append(_real_item(float(it), 8))
+    return "".join(" " + o for o in out) + "\n"
 
 
 def write_seismograms(outdir, seis, dt, isamp, coords):
@@ -31,12 +96,19 @@ def write_seismograms(outdir, seis, dt, isamp, coords):
         with open(os.path.join(outdir, nme), "wb") as fh:
             for r in range(nrec):
                 fh.write(np.ascontiguousarray(s32[:, c, r]).tobytes())
+    write_seismogram_header(outdir, dt, isamp, nsamp, coords)
+
+
+def write_seismogram_header(outdir, dt, isamp, nsamp, coords):
+    """SeisHeader_sem2d.hdr of REC_init (SRC/receivers.f90:214-221): real(tsamp), the
+    number of samples and stations, then the station coordinates."""
+    nrec = len(coords)
     with open(os.path.join(outdir, "SeisHeader_sem2d.hdr"), "w") as fh:
-        fh.write(" DT NSAMP NSTA\n")
-        fh.write(f" {float(np.float32(dt * isamp))!r} {nsamp} {nrec}\n")
-        fh.write(" XSTA ZSTA\n")
+        fh.write(list_directed("DT NSAMP NSTA"))
+        fh.write(list_directed(np.float32(dt * isamp), nsamp, nrec))
+        fh.write(list_directed("XSTA ZSTA"))
         for x, z in coords:
-            fh.write(f" {x!r} {z!r}\n")
+            fh.write(list_directed(float(x), float(z)))
 
 
 def read_seismograms(outdir, nsamp, nrec, ndof):
@@ -59,18 +131,26 @@ def write_fault(outdir, fault, records, dt, nt):
                 fh.write(marker)
                 fh.write(np.ascontiguousarray(rec[q]).tobytes())
                 fh.write(marker)
+    write_fault_headers(outdir, fault, nt)
+
+
+def write_fault_headers(outdir, fault, nt):
+    """FltXX_sem2d.hdr and FltXX_init_sem2d.tab of BC_DYNFLT_init
+    (SRC/bc_dynflt.f90:446-452,479-500) for a run of nt steps."""
+    tag = int(fault["tags"][0])
     sel = slice(fault["oix1"] - 1, fault["oixn"], fault["oixd"])
+    nout = (fault["oixn"] - fault["oix1"]) // fault["oixd"] + 1
     nsamp = (nt - fault["oit"]) // fault["oitd"] + 1
     with open(os.path.join(outdir, f"Flt{tag:02d}_sem2d.hdr"), "w") as fh:
-        fh.write(" NPTS NDAT NSAMP DELT\n")
-        fh.write(f" {nout} 6 {nsamp} {fault['odt']!r}\n")
+        fh.write(list_directed("NPTS NDAT NSAMP DELT"))
+        fh.write(list_directed(nout, 6, nsamp, float(fault["odt"])))
         fh.write(" Slip:Slip_Rate:Shear_Stress:Normal_Stress:Friction:T_stick\n")
-        fh.write(" XPTS ZPTS\n")
+        fh.write(list_directed("XPTS ZPTS"))
         for x, z in fault["coord"][sel]:
-            fh.write(f" {x!r} {z!r}\n")
+            fh.write(list_directed(float(x), float(z)))
     with open(os.path.join(outdir, f"Flt{tag:02d}_init_sem2d.tab"), "w") as fh:
         for t0, tn, mu, b in zip(fault["T0"][sel, 0], fault["T0"][sel, 1], fault["MU0"][sel], fault["B"][sel, 0]):
-            fh.write(f" {t0!r} {tn!r} {mu!r} {b!r}\n")
+            fh.write(list_directed(float(t0), float(tn), float(mu), float(b)))
 
 
 def fortran_d(x, w=24, d=16):
