@@ -189,6 +189,15 @@ int sem2d_fetch_seismo(sem2d_solver *h, double *out, int *nsamp_taken);
    with nout = (oixn-oix1)/oixd + 1. */
 int sem2d_fetch_fault(sem2d_solver *h, double *out, int *nsamp_taken);
 
+/* Per-side fault records (BC_DYNFLT osides = T; export_side, SRC/bc_dynflt.f90:769-812):
+   after sem2d_set_fault and before the first step, on = 1 makes every fault record also
+   keep [4*ndof][nout] values: the displacement of side 1, of side 2, the velocity of side
+   1, of side 2, each in the fault frame (t, n) for P-SV (rotate, SRC/bc_dynflt.f90:722-739)
+   and side 2 = -(side 1) on a fault with one side.  sem2d_fetch_fault_sides returns them
+   as [nsamp][4*ndof][nout], the rows the reference appends to each FltXX_sem2d.dat record. */
+int sem2d_set_fault_sides(sem2d_solver *h, int on);
+int sem2d_fetch_fault_sides(sem2d_solver *h, double *out, int *nsamp_taken);
+
 /* Fault potency rows [nrows][2*(ndof+1)], one per step from it = 0: P-SV p11, p22, p12
    of the displacement jump then of the velocity jump, SH p13, p23 of each -- the
    FltXX_potency_sem2d.tab lines of BC_DYNFLT_write (SRC/bc_dynflt.f90:757,832-855). */

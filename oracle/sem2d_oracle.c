@@ -128,6 +128,8 @@ typedef struct {
   int oit, oitd, oix1, oixn, oixd, nout;
   int nsamp, taken;
   double *rec; /* [nsamp][6][nout] */
+  int osides;   /* per-side records (osides = T) */
+  double *srec; /* [nsamp][4*ndof][nout]: d of side 1, side 2, v of side 1, side 2 */
   int pot_rows, pot_cap;
   double *pot; /* [rows][2*(ndof+1)] potency of d then v, one row per step from it = 0 */
 } fault_t;
@@ -498,6 +500,35 @@ static void potency_row(oracle_solver *h) {
   f->pot_rows++;
 }
 
+/* export_side of get_side (SRC/bc_dynflt.f90:769-812) for the output nodes: rows
+   d of side 1, d of side 2, v of side 1, v of side 2, each in the fault frame (t, n)
+   (rotate, SRC/bc_dynflt.f90:722-739) for P-SV; side 2 of a fault with one side is
+   -(side 1) */
+static void record_sides(const oracle_solver *h, double *q) {
+  const fault_t *f = h->flt;
+  const double *fld[2] = {h->d, h->v};
+  int row = 0;
+  for (int w = 0; w < 2; ++w)
+    for (int side = 1; side <= 2; ++side) {
+      int o = 0;
+      for (int k = f->oix1 - 1; k <= f->oixn - 1; k += f->oixd, ++o) {
+        double x[2] = {0.0, 0.0};
+        for (int c = 0; c < f->ndof; ++c) {
+          const double *u = fld[w] + (size_t)c * h->npoin;
+          x[c] = side == 1 ? u[f->n1[k]] : (f->n2[k] >= 0 ? u[f->n2[k]] : -u[f->n1[k]]);
+        }
+        if (f->ndof == 1) {
+          q[(size_t)row * f->nout + o] = x[0];
+        } else {
+          const double n1x = f->nrm[k], n1z = f->nrm[f->np + k];
+          q[(size_t)row * f->nout + o] = n1z * x[0] - n1x * x[1];
+          q[(size_t)(row + 1) * f->nout + o] = n1x * x[0] + n1z * x[1];
+        }
+      }
+      row += f->ndof;
+    }
+}
+
 static void fault_record(oracle_solver *h) {
   fault_t *f = h->flt;
   potency_row(h);
@@ -513,9 +544,31 @@ static void fault_record(oracle_solver *h) {
       r[4 * f->nout + o] = f->MU[k];
       r[5 * f->nout + o] = f->Tstick[k];
     }
+    if (f->osides) record_sides(h, f->srec + (size_t)f->taken * 4 * f->ndof * f->nout);
     f->taken++;
   }
   f->oit += f->oitd;
+}
+
+/* BC_DYNFLT osides = T: every fault record also keeps the displacement and velocity of
+   each side in the fault frame (SRC/bc_dynflt.f90:769-812) */
+int oracle_set_fault_sides(oracle_solver *h, int on) {
+  fault_t *f = h->flt;
+  if (!f) return fail(h, 4, "set_fault_sides: no fault");
+  if (f->taken > 1 || h->it) return fail(h, 4, "set_fault_sides: call it before the first step");
+  f->osides = on != 0;
+  free(f->srec);
+  f->srec = on ? dup_d(NULL, (size_t)f->nsamp * 4 * f->ndof * f->nout) : NULL;
+  if (on && f->taken) record_sides(h, f->srec); /* the record of step 0, taken by set_fault */
+  return 0;
+}
+
+int oracle_fetch_fault_sides(oracle_solver *h, double *out, int *taken) {
+  fault_t *f = h->flt;
+  if (!f || !f->osides) return fail(h, 4, "fetch_fault_sides: no per-side records");
+  memcpy(out, f->srec, sizeof(double) * f->taken * 4 * f->ndof * f->nout);
+  if (taken) *taken = f->taken;
+  return 0;
 }
 
 int oracle_set_fault(oracle_solver *h, const sem2d_fault_desc *d, int nsamp) {
@@ -1084,7 +1137,7 @@ void oracle_destroy(oracle_solver *h) {
     free(f->Z); free(f->T0); free(f->coh);
     for (int q = 0; q < 6; ++q) free(f->p[q]);
     free(f->theta); free(f->Tc); free(f->coeft); free(f->D); free(f->V); free(f->T);
-    free(f->Tstick); free(f->MU); free(f->sigma); free(f->rec); free(f->pot); free(f);
+    free(f->Tstick); free(f->MU); free(f->sigma); free(f->rec); free(f->srec); free(f->pot); free(f);
   }
   free(h);
 }

@@ -1152,8 +1152,6 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
     oixn = min(int(oxi[1]), npf)
     oixd = int(oxi[2])
     oitd = max(1, _nint(float(g.get("otd", 0.0)) / dt))
-    if bool(g.get("osides", False)):
-        raise InputError("BC_DYNFLT osides=T (per-side fault outputs) is not supported by this build")
     f.update(oix1=oix1, oixn=oixn, oixd=oixd, oitd=oitd, odt=dt * oitd,
              oit=_nint(float(g.get("ot1", 0.0)) / dt), osides=bool(g.get("osides", False)))
     return f

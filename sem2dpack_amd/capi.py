@@ -57,6 +57,8 @@ _SIGS = {
     "compute_fint": ([C.c_void_p, dp, dp], C.c_int),
     "compute_fint_noalloc": ([C.c_void_p, dp, dp], C.c_int),
     "set_time_scheme": ([C.c_void_p, C.c_int], C.c_int),
+    "set_fault_sides": ([C.c_void_p, C.c_int], C.c_int),
+    "fetch_fault_sides": ([C.c_void_p, dp, ip], C.c_int),
     "set_itinfo": ([C.c_void_p, C.c_int], C.c_int),
     "fetch_info": ([C.c_void_p, dp, ip], C.c_int),
     "field_info": ([C.c_void_p, dp, dp, ip], C.c_int),

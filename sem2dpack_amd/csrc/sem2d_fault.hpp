@@ -295,6 +295,40 @@ __global__ void k_fault_record(FaultDev F, int oix1, int oixd, int nout, double*
   out[5 * nout + o] = F.Tstick[k];
 }
 
+// Per-side fault record (osides = T; export_side of get_side, SRC/bc_dynflt.f90:769-812):
+// out[4*ndof][nout], rows d of side 1, d of side 2, v of side 1, v of side 2, each in
+// the fault frame (t, n) for P-SV (rotate, SRC/bc_dynflt.f90:722-739); side 2 of a
+// fault with one side is -(side 1).  FP64, no contraction, like the fault solve.
+#pragma clang fp contract(off)
+template <typename R>
+__global__ void k_fault_sides(FaultDev F, const R* __restrict__ d, const R* __restrict__ v, size_t stride,
+                              int oix1, int oixd, int nout, double* __restrict__ out) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nout) return;
+  const int k = oix1 - 1 + o * oixd;
+  const int i1 = F.n1[k], i2 = F.n2[k];
+  const int nd = F.ndof;
+  int row = 0;
+  for (int w = 0; w < 2; ++w)
+    for (int side = 1; side <= 2; ++side) {
+      const R* u = w == 0 ? d : v;
+      double x[2] = {0.0, 0.0};
+      for (int c = 0; c < nd; ++c) {
+        const R* uc = u + (size_t)c * stride;
+        x[c] = side == 1 ? (double)uc[i1] : (i2 >= 0 ? (double)uc[i2] : -(double)uc[i1]);
+      }
+      if (nd == 1) {
+        out[(size_t)row * nout + o] = x[0];
+      } else {
+        const double n1x = F.nrm[k], n1z = F.nrm[F.np + k];
+        out[(size_t)row * nout + o] = n1z * x[0] - n1x * x[1];
+        out[(size_t)(row + 1) * nout + o] = n1x * x[0] + n1z * x[1];
+      }
+      row += nd;
+    }
+}
+#pragma clang fp contract(on)
+
 // Potency of the displacement and velocity jumps (BC_DYNFLT_potency,
 // SRC/bc_dynflt.f90:832-855): P-SV p11, p22, p12 of d then of v, SH p13, p23 of d
 // then of v, one row of 2*(ndof+1) values, over the fault pairs k0 .. k0+n-1.  One

@@ -101,6 +101,7 @@ struct FaultHost {
   std::vector<void*> allocs;
   int oit, oitd, oix1, oixn, oixd, nout, nsamp, taken;
   double* rec;  // device [nsamp][6][nout]
+  double* srec = nullptr;  // osides: device [nsamp][4*ndof][nout] (d, v of each side)
   double* pot = nullptr;  // device [pot_cap][2*(ndof+1)] potency rows, one per step from it = 0
   int pot_rows = 0, pot_cap = 0;
 };
@@ -488,6 +489,18 @@ void record_potency(sem2d_solver* h) {
   F.pot_rows++;
 }
 
+// per-side record (osides) of sample slot s from the current fields
+void record_sides(sem2d_solver* h, int s) {
+  FaultHost& F = h->flt;
+  double* out = F.srec + (size_t)s * 4 * h->ndof * F.nout;
+  if (h->fp32)
+    k_fault_sides<float><<<grid_for(F.nout, 64), 64, 0, h->stream>>>(F.dev, (const float*)h->d, (const float*)h->v,
+                                                                      h->fstride, F.oix1, F.oixd, F.nout, out);
+  else
+    k_fault_sides<double><<<grid_for(F.nout, 64), 64, 0, h->stream>>>(
+        F.dev, (const double*)h->d, (const double*)h->v, h->fstride, F.oix1, F.oixd, F.nout, out);
+}
+
 int record_fault(sem2d_solver* h) {
   FaultHost& F = h->flt;
   if (!F.on) return 0;
@@ -497,6 +510,7 @@ int record_fault(sem2d_solver* h) {
   if (F.taken < F.nsamp) {
     k_fault_record<<<grid_for(F.nout, 64), 64, 0, h->stream>>>(F.dev, F.oix1, F.oixd, F.nout,
                                                                F.rec + (size_t)F.taken * 6 * F.nout);
+    if (F.srec) record_sides(h, F.taken);
     F.taken++;
   }
   F.oit += F.oitd;
@@ -1793,6 +1807,33 @@ int sem2d_fetch_fault(sem2d_solver* h, double* out, int* taken) {
   return 0;
 }
 
+int sem2d_set_fault_sides(sem2d_solver* h, int on) {
+  if (!h) return 1;
+  FaultHost& F = h->flt;
+  if (!F.on) return set_err(h, 4, "set_fault_sides: call sem2d_set_fault first");
+  if (h->it != 0) return set_err(h, 4, "set_fault_sides: the solver has already been stepped");
+  if (!on) {
+    F.srec = nullptr;  // the block stays with the handle's allocations
+    return 0;
+  }
+  int rc = dev_alloc(h, &F.srec, (size_t)F.nsamp * 4 * h->ndof * F.nout);
+  if (rc) return rc;
+  if (F.taken) record_sides(h, 0);  // the record of step 0, taken by sem2d_set_fault
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int sem2d_fetch_fault_sides(sem2d_solver* h, double* out, int* nsamp_taken) {
+  if (!h) return 1;
+  FaultHost& F = h->flt;
+  if (!F.on || !F.srec) return set_err(h, 4, "fetch_fault_sides: no per-side records (sem2d_set_fault_sides)");
+  if (out)
+    HIPCHK(h, hipMemcpy(out, F.srec, (size_t)F.taken * 4 * h->ndof * F.nout * sizeof(double),
+                        hipMemcpyDeviceToHost));
+  if (nsamp_taken) *nsamp_taken = F.taken;
+  return 0;
+}
+
 int sem2d_fetch_potency(sem2d_solver* h, double* out, int* nrows) {
   if (!h) return 1;
   if (!h->flt.on) return set_err(h, 4, "fetch_potency: no fault");
@@ -2143,6 +2184,7 @@ int check_members(sem2d_batch* b) {
     if (h->dt != h0->dt || h->beta != h0->beta || h->gamma != h0->gamma || h->leapfrog != h0->leapfrog)
       return berr(b, 3, "member %d: time step, time scheme and Newmark parameters must match", m);
     if (h->flt.on != h0->flt.on) return berr(b, 3, "member %d: either every member has a fault or none", m);
+    if (h->flt.srec) return berr(b, 3, "member %d: per-side fault records (osides) are not batched", m);
     for (const AbsoHost& B : h->abso_host)
       if (!B.senode.empty()) return berr(b, 3, "member %d: Stacey boundaries are not batched", m);
     if (h->flt.on && h->fault_n2.empty())

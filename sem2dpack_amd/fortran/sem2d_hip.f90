@@ -38,7 +38,7 @@ module sem2d_hip
             sem2d_set_abso_normals, &
             sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
             sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_set_dirichlet, &
-            sem2d_set_25d, sem2d_set_time_scheme, &
+            sem2d_set_25d, sem2d_set_time_scheme, sem2d_set_fault_sides, sem2d_fetch_fault_sides, &
             sem2d_step, &
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
@@ -163,6 +163,20 @@ module sem2d_hip
     end function
 
     integer(c_int) function sem2d_fetch_fault(h, out, nsamp_taken) bind(C, name='sem2d_fetch_fault')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), intent(out) :: nsamp_taken
+    end function
+
+    ! per-side fault records (osides): set after sem2d_set_fault; out(nout, 4*ndof, nsamp)
+    integer(c_int) function sem2d_set_fault_sides(h, on) bind(C, name='sem2d_set_fault_sides')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: on
+    end function
+
+    integer(c_int) function sem2d_fetch_fault_sides(h, out, nsamp_taken) bind(C, name='sem2d_fetch_fault_sides')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(out) :: out(*)

@@ -14,7 +14,7 @@
 * Potency (``BC_DYNFLT_write`` / ``BC_DYNFLT_potency``, ``SRC/bc_dynflt.f90:757,
   832-855``): ``FltXX_potency_sem2d.tab``, one formatted ``(6D24.16)`` line per step.
 
-The binary files and the text headers (list-directed ``WRITE(*,*)`` records, see
+The binary files (with the per-side rows of osides = T) and the text headers (list-directed ``WRITE(*,*)`` records, see
 :func:`list_directed`) are byte compatible with the reference's.
 """
 import decimal
@@ -120,14 +120,17 @@ def read_seismograms(outdir, nsamp, nrec, ndof):
     return out
 
 
-def write_fault(outdir, fault, records, dt, nt):
-    """records: [nsamp][6][nout] FP64 fault records (sem2d_fetch_fault)."""
+def write_fault(outdir, fault, records, dt, nt, sides=None):
+    """records: [nsamp][6][nout] FP64 fault records (sem2d_fetch_fault); sides, with
+    osides = T: [nsamp][4*ndof][nout] per-side records (sem2d_fetch_fault_sides), written
+    after the six rows of each output time (BC_DYNFLT_write, SRC/bc_dynflt.f90:769-774)."""
     tag = int(fault["tags"][0])
     nout = records.shape[2]
+    rows = records if sides is None else np.concatenate([records, sides], axis=1)
     with open(os.path.join(outdir, f"Flt{tag:02d}_sem2d.dat"), "wb") as fh:
         marker = np.array([4 * nout], dtype=np.int32).tobytes()
-        for rec in records.astype(np.float32):
-            for q in range(6):
+        for rec in rows.astype(np.float32):
+            for q in range(rec.shape[0]):
                 fh.write(marker)
                 fh.write(np.ascontiguousarray(rec[q]).tobytes())
                 fh.write(marker)
@@ -141,10 +144,16 @@ def write_fault_headers(outdir, fault, nt):
     sel = slice(fault["oix1"] - 1, fault["oixn"], fault["oixd"])
     nout = (fault["oixn"] - fault["oix1"]) // fault["oixd"] + 1
     nsamp = (nt - fault["oit"]) // fault["oitd"] + 1
+    ndof = fault["B"].shape[1]
+    names = "Slip:Slip_Rate:Shear_Stress:Normal_Stress:Friction:T_stick"
+    ndat = 6
+    if fault.get("osides"):
+        ndat += 4 * ndof
+        names += ":D1t:D2t:V1t:V2t" if ndof == 1 else ":D1t:D1n:D2t:D2n:V1t:V1n:V2t:V2n"
     with open(os.path.join(outdir, f"Flt{tag:02d}_sem2d.hdr"), "w") as fh:
         fh.write(list_directed("NPTS NDAT NSAMP DELT"))
-        fh.write(list_directed(nout, 6, nsamp, float(fault["odt"])))
-        fh.write(" Slip:Slip_Rate:Shear_Stress:Normal_Stress:Friction:T_stick\n")
+        fh.write(list_directed(nout, ndat, nsamp, float(fault["odt"])))
+        fh.write(" " + names + "\n")
         fh.write(list_directed("XPTS ZPTS"))
         for x, z in fault["coord"][sel]:
             fh.write(list_directed(float(x), float(z)))

@@ -183,6 +183,8 @@ class Solver:
         self.fault_nsamp = (self.pb.nt - f["oit"]) // f["oitd"] + 1
         self.npf = npf
         self._check(self.api.set_fault(self.h, C.byref(fd), int(self.fault_nsamp)), "set_fault")
+        if f.get("osides"):
+            self._check(self.api.set_fault_sides(self.h, 1), "set_fault_sides")
 
     # ------------------------------------------------------------------
     def step(self, n):
@@ -209,6 +211,14 @@ class Solver:
         out = np.empty((self.fault_nsamp, 6, self.nout))
         n = C.c_int()
         self._check(self.api.fetch_fault(self.h, _p(out), C.byref(n)), "fetch_fault")
+        return out[:n.value]
+
+    def fault_sides(self):
+        """[nsamp][4*ndof][nout] per-side records (osides = T): d of side 1, side 2, v of
+        side 1, side 2, in the fault frame for P-SV."""
+        out = np.empty((self.fault_nsamp, 4 * self.ndof, self.nout))
+        n = C.c_int()
+        self._check(self.api.fetch_fault_sides(self.h, _p(out), C.byref(n)), "fetch_fault_sides")
         return out[:n.value]
 
     def potency(self):

@@ -40,7 +40,7 @@ def pytest_terminal_summary(terminalreporter, exitstatus, config):
         by_case.setdefault(case, []).append((q, e, b))
     for case, rows in by_case.items():
         worst = max(rows, key=lambda r: r[1] / r[2])
-        keys = [r for r in rows if r[0] in ("d", "v", "a", "seismo", "slip", "slip_rate", "shear", "potency")]
+        keys = [r for r in rows if r[0] in ("d", "v", "a", "seismo", "slip", "slip_rate", "shear", "potency", "sides")]
         body = " ".join(f"{q}={e:.1e}" + ("" if b == 1e-10 else f"/{b:.0e}") for q, e, b in keys)
         tr.write_line(f"{case[:30]:30s} worst {worst[0]} {worst[1]:.2e}/{worst[2]:.1e} | {body}")
 

@@ -45,8 +45,13 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def read_flt(path, nout, ndat=6):
-    """Sequential unformatted float32 records with 4-byte markers (POST/python/sem2d_read_fault.py)."""
+def read_flt(path, nout, ndat=None):
+    """Sequential unformatted float32 records with 4-byte markers (POST/python/sem2d_read_fault.py);
+    ndat (rows per output time) from the FltXX_sem2d.hdr beside the file unless given."""
+    if ndat is None:
+        with open(path[:-4] + ".hdr") as fh:
+            fh.readline()
+            ndat = int(fh.readline().split()[1])
     raw = np.fromfile(path, dtype=np.float32)
     rec = nout + 2
     n = raw.size // rec

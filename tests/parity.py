@@ -45,6 +45,8 @@ def quantities(pb, s):
             else:
                 err = rel
             rows[nm] = (r[:, q], err)
+        if pb.fault.get("osides"):  # per-side records: d, v of each side
+            rows["sides"] = (s.fault_sides(), rel)
         p = s.potency()
         # against the largest potency of the same field (displacement, velocity columns):
         # single columns can be pure round-off (no opening on a planar fault)

@@ -55,14 +55,17 @@ def test_oracle_matches_reference(name, oracle_api):
     if "flt_f32_all" in g:
         # decimated fault output: every oitd steps at nodes oix1:oixn:oixd
         rec = s.fault_records()
-        assert rec.shape == g["flt_f32_all"].shape
+        assert rec.shape[0] == g["flt_f32_all"].shape[0] and rec.shape[2] == g["flt_f32_all"].shape[2]
         assert np.array_equal(rec[:, [0, 1, 2, 4]].astype(np.float32), g["flt_f32_all"][:, [0, 1, 2, 4]])
+        if pb.fault.get("osides"):  # per-side rows after the six of each record
+            assert np.array_equal(s.fault_sides().astype(np.float32), g["flt_f32_all"][:, 6:])
         oitd, oix1, oixd = int(g["flt_oitd"]), int(g["flt_oix1"]), int(g["flt_oixd"])
         st_every = int(g["fault_every"])
         assert st_every % oitd == 0
         if "fault_dd" in g:  # split nodes: slip from the reference's FP64 jumps
-            n1 = pb.fault["n1"][oix1 - 1::oixd]
-            dd = g["fault_dd"][:, :, oix1 - 1::oixd]
+            sel = slice(oix1 - 1, pb.fault["oixn"], oixd)
+            n1 = pb.fault["n1"][sel]
+            dd = g["fault_dd"][:, :, sel]
             slip = n1[:, 1] * dd[:, 0] - n1[:, 0] * dd[:, 1] if pb.ndof == 2 else dd[:, 0]
             assert np.array_equal(rec[::st_every // oitd, 0], slip)
     elif "fault_node1" in g:
@@ -81,6 +84,8 @@ def test_oracle_matches_reference(name, oracle_api):
             ours = rec[::st_every].astype(np.float32)
             cols = [0, 1, 2, 4] + ([5] if pb.fault["law"] == "SWF" else [])
             assert np.array_equal(ours[:, cols], ref32[:, cols])
+            if pb.fault.get("osides"):  # per-side rows after the six of each record
+                assert np.array_equal(s.fault_sides()[::st_every].astype(np.float32), ref32[:, 6:])
 
 
 @pytest.mark.parametrize("name", ["sh_swf", "psv_rsf", "rsf_ens", "sh_kv", "ratestate", "tpv3_kv", "tpv3_dirfirst",
@@ -102,7 +107,7 @@ def test_potency_file_matches_reference(name, oracle_api, tmp_path):
     assert ours == str(g["potency_txt"])
 
 
-@pytest.mark.parametrize("name", ["small_psv", "sh_swf"])
+@pytest.mark.parametrize("name", ["small_psv", "sh_swf", "sides_sh1"])
 def test_output_files_match_reference(name, oracle_api, tmp_path):
     """Seismogram and fault files written from the oracle's FP64 series are byte-compatible
     with the reference's float32 files."""
@@ -117,9 +122,11 @@ def test_output_files_match_reference(name, oracle_api, tmp_path):
     back = outputs.read_seismograms(str(tmp_path), r["nsamp"], len(r["iglob"]), pb.ndof)
     assert np.array_equal(back, g["seis_f32"])
     if pb.fault is not None:
-        outputs.write_fault(str(tmp_path), pb.fault, s.fault_records(), pb.dt, nt)
+        sides = s.fault_sides() if pb.fault["osides"] else None
+        outputs.write_fault(str(tmp_path), pb.fault, s.fault_records(), pb.dt, nt, sides=sides)
         nout = (pb.fault["oixn"] - pb.fault["oix1"]) // pb.fault["oixd"] + 1
-        f32 = outputs.read_fault(str(tmp_path / "Flt05_sem2d.dat"), nout)
+        tag = int(pb.fault["tags"][0])
+        f32 = outputs.read_fault(str(tmp_path / ("Flt%02d_sem2d.dat" % tag)), nout, g["flt_f32"].shape[1])
         assert np.array_equal(f32[::int(g["fault_every"])], g["flt_f32"])
 
 
