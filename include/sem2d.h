@@ -77,7 +77,8 @@ typedef struct sem2d_fault_desc {
   const double *cohesion;             /* [npoin] */
   double coefA2V, coefA2D;
   int allow_opening;
-  int law;                            /* 1 = slip weakening, 2 = rate and state */
+  int law;                            /* 1 = slip weakening, 2 = rate and state, 3 = time
+                                         weakening alone (sem2d_set_fault_twf) */
   int kind;                           /* SWF kind 1..3 (SRC/bc_dynflt_swf.f90:148-158),
                                          RSF kind 1..4 (SRC/bc_dynflt_rsf.f90:165-184) */
   int healing;                        /* SWF instantaneous healing */
@@ -196,6 +197,15 @@ int sem2d_fetch_fault(sem2d_solver *h, double *out, int *nsamp_taken);
    and side 2 = -(side 1) on a fault with one side.  sem2d_fetch_fault_sides returns them
    as [nsamp][4*ndof][nout], the rows the reference appends to each FltXX_sem2d.dat record. */
 int sem2d_set_fault_sides(sem2d_solver *h, int on);
+
+/* Time-weakening friction (BC_DYNFLT_TWF, SRC/bc_dynflt_twf.f90:129-183; the forced
+   nucleation of BC_DYNFLT friction = 'SWF','TWF'): after sem2d_set_fault and before the
+   first step.  kind 1..3, par = MuS, MuD, Mu0, X, Z, V, L, T, Dc, coord [2][npoin] the fault
+   node coordinates (bc%coord).  The friction coefficient becomes the minimum of the
+   slip-weakening or rate-and-state one and twf_mu(time = it*dt) (SRC/bc_dynflt.f90:404-413,
+   632,653), or twf_mu alone with law 3.  With kind 3 the reference leaves the coefficient
+   of nodes beyond V*T unset; here it is huge(1d0), i.e. no weakening there. */
+int sem2d_set_fault_twf(sem2d_solver *h, int kind, const double *par, const double *coord);
 int sem2d_fetch_fault_sides(sem2d_solver *h, double *out, int *nsamp_taken);
 
 /* Fault potency rows [nrows][2*(ndof+1)], one per step from it = 0: P-SV p11, p22, p12

@@ -31,6 +31,7 @@
  * Tstick array unset (SRC/bc_dynflt.f90:620-636 never assigns it); here Tstick is
  * the stick traction minus T0 for both friction laws.
  */
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -128,6 +129,9 @@ typedef struct {
   int oit, oitd, oix1, oixn, oixd, nout;
   int nsamp, taken;
   double *rec; /* [nsamp][6][nout] */
+  int twf;       /* time-weakening friction kind (0: none), law 3: time weakening alone */
+  double tw[9];  /* MuS, MuD, Mu0, X, Z, V, L, T, Dc */
+  double *xz;    /* [2][np] fault node coordinates (bc%coord) */
   int osides;   /* per-side records (osides = T) */
   double *srec; /* [nsamp][4*ndof][nout]: d of side 1, side 2, v of side 1, side 2 */
   int pot_rows, pot_cap;
@@ -339,6 +343,38 @@ int oracle_set_receivers(oracle_solver *h, int nrec, const int *iglob, int isamp
 }
 
 /* ---------------- friction ---------------- */
+/* twf_mu (SRC/bc_dynflt_twf.f90:129-183) at node k, time t, slip d; kind 3 leaves
+   nodes beyond V*T unset in the reference: they get huge(1d0) here (no weakening) */
+static double twf_mu1(const fault_t *f, int k, double time, double d) {
+  const double mus = f->tw[0], mud = f->tw[1], mu0 = f->tw[2], X = f->tw[3], Z = f->tw[4];
+  const double V = f->tw[5], L = f->tw[6], T = f->tw[7], Dc = f->tw[8];
+  const double big = DBL_MAX;
+  const double dx = f->xz[k] - X, dz = f->xz[f->np + k] - Z;
+  const double dist = sqrt(dx * dx + dz * dz);
+  if (f->twf == 1 || f->twf == 2) {
+    double t, r;
+    if (f->twf == 1) {
+      t = time + (mus - mu0) * L / ((mus - mud) * V);
+      if (t > T) t = 0.0;
+      r = V * t;
+    } else {
+      t = time + 0.5 * T * (1.0 - sqrt(1.0 - 4.0 * (mus - mu0) * L / ((mus - mud) * T * V)));
+      t = fmin(t, T);
+      r = V * t * (1.0 - t / T);
+    }
+    r = dist - r;
+    if (r < -L) return mud;
+    if (r <= L) return mus + (mus - mud) / L * r;
+    return big;
+  }
+  if (dist <= V * T && d <= Dc) {
+    if (dist < V * time - L) return mud;
+    if (dist >= V * time - L && dist <= V * time) return mus + (mus - mud) / L * (dist - V * time);
+    return big;
+  }
+  return big;
+}
+
 static double swf_mu1(const fault_t *f, int k) {
   double th = f->theta[k], dc = f->p[0][k], mus = f->p[1][k], mud = f->p[2][k];
   double mu;
@@ -550,6 +586,28 @@ static void fault_record(oracle_solver *h) {
   f->oit += f->oitd;
 }
 
+/* superimposed (law 1, 2) or sole (law 3) time-weakening friction (BC_DYNFLT_TWF,
+   SRC/bc_dynflt_twf.f90); par = MuS, MuD, Mu0, X, Z, V, L, T, Dc; coord [2][np] */
+int oracle_set_fault_twf(oracle_solver *h, int kind, const double *par, const double *coord) {
+  fault_t *f = h->flt;
+  if (!f) return fail(h, 4, "set_fault_twf: no fault");
+  if (kind < 1 || kind > 3 || !par || !coord) return fail(h, 3, "set_fault_twf: bad arguments");
+  f->twf = kind;
+  memcpy(f->tw, par, sizeof f->tw);
+  free(f->xz);
+  f->xz = dup_d(coord, 2 * (size_t)f->np);
+  /* initial friction (SRC/bc_dynflt.f90:404-413) */
+  for (int k = 0; k < f->np; ++k) {
+    const double mt = twf_mu1(f, k, 0.0, f->D[k]);
+    f->MU[k] = f->law == 3 ? mt : fmin(f->MU[k], mt);
+  }
+  if (f->taken) { /* the friction row of the record of step 0, taken by set_fault */
+    int o = 0;
+    for (int k = f->oix1 - 1; k <= f->oixn - 1; k += f->oixd, ++o) f->rec[4 * f->nout + o] = f->MU[k];
+  }
+  return 0;
+}
+
 /* BC_DYNFLT osides = T: every fault record also keeps the displacement and velocity of
    each side in the fault frame (SRC/bc_dynflt.f90:769-812) */
 int oracle_set_fault_sides(oracle_solver *h, int on) {
@@ -605,7 +663,7 @@ int oracle_set_fault(oracle_solver *h, const sem2d_fault_desc *d, int nsamp) {
   f->MU = dup_d(NULL, np);
   f->sigma = dup_d(NULL, np);
   for (int k = 0; k < np; ++k) {
-    f->MU[k] = f->law == 1 ? swf_mu1(f, k) : rsf_mu1(f, k, f->V[k]);
+    f->MU[k] = f->law == 1 ? swf_mu1(f, k) : (f->law == 2 ? rsf_mu1(f, k, f->V[k]) : DBL_MAX);
     f->sigma[k] = f->T0[np + k];
   }
   f->oit = d->oit; f->oitd = d->oitd; f->oix1 = d->oix1; f->oixn = d->oixn; f->oixd = d->oixd;
@@ -798,7 +856,7 @@ int oracle_set_25d(oracle_solver *h, const double *beta) {
   return 0;
 }
 
-static void fault_apply(oracle_solver *h, double *MxA) {
+static void fault_apply(oracle_solver *h, double *MxA, double time) {
   fault_t *f = h->flt;
   int np = f->np, nd = h->ndof;
   size_t N = h->npoin;
@@ -842,8 +900,14 @@ static void fault_apply(oracle_solver *h, double *MxA) {
       f->theta[k] = th;
       f->V[k] = vn;
       f->MU[k] = rsf_mu1(f, k, f->V[k]);
+      /* superimposed time weakening (SRC/bc_dynflt.f90:632) */
+      if (f->twf) f->MU[k] = fmin(f->MU[k], twf_mu1(f, k, time, f->D[k]));
       double strength = -f->MU[k] * sigma;
       T[0] = copysign(fabs(strength), T[0]);
+    } else if (f->law == 3) { /* time weakening alone (SRC/bc_dynflt.f90:656-657) */
+      f->MU[k] = twf_mu1(f, k, time, f->D[k]);
+      double strength = f->coh[k] - f->MU[k] * sigma;
+      T[0] = copysign(fmin(fabs(T[0]), strength), T[0]);
     } else {
       if (f->a2d == 0.0) {
         if (f->healing) {
@@ -856,6 +920,7 @@ static void fault_apply(oracle_solver *h, double *MxA) {
         f->theta[k] = fabs(f->D[k]);
       }
       f->MU[k] = swf_mu1(f, k);
+      if (f->twf) f->MU[k] = fmin(f->MU[k], twf_mu1(f, k, time, f->D[k])); /* :653 */
       double strength = f->coh[k] - f->MU[k] * sigma;
       T[0] = copysign(fmin(fabs(T[0]), strength), T[0]);
     }
@@ -971,7 +1036,7 @@ static void one_step(oracle_solver *h) {
   /* the other boundary conditions in their order (SRC/bc_gen.f90:273-281) */
   for (int b = 0; b < h->ndir; ++b)
     if (!h->dir_after_fault[b]) dirichlet_apply(h, b, f);
-  if (h->flt) fault_apply(h, f);
+  if (h->flt) fault_apply(h, f, (double)it * dt); /* time = it*dt (SRC/main.f90:53) */
   for (int b = 0; b < h->ndir; ++b)
     if (h->dir_after_fault[b]) dirichlet_apply(h, b, f);
   double c3 = gam * dt, c4 = beta * dt * dt;
@@ -1137,7 +1202,7 @@ void oracle_destroy(oracle_solver *h) {
     free(f->Z); free(f->T0); free(f->coh);
     for (int q = 0; q < 6; ++q) free(f->p[q]);
     free(f->theta); free(f->Tc); free(f->coeft); free(f->D); free(f->V); free(f->T);
-    free(f->Tstick); free(f->MU); free(f->sigma); free(f->rec); free(f->srec); free(f->pot); free(f);
+    free(f->Tstick); free(f->MU); free(f->sigma); free(f->rec); free(f->srec); free(f->xz); free(f->pot); free(f);
   }
   free(h);
 }

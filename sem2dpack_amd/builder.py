@@ -281,7 +281,9 @@ class Problem:
 # ---------------------------------------------------------------------------
 # mesh + numbering
 # ---------------------------------------------------------------------------
-def _cart_mesh(g):
+def _cart_mesh(g, domains=()):
+    """CART_read (SRC/mesh_cartesian.f90:86-191): the box, its fault row and fault zone,
+    and the MESH_CART_DOMAIN subdomains (tag, element ranges ex, ez, 1-based)."""
     xlim = g.get("xlim")
     zlim = g.get("zlim")
     nel = g.get("nelem")
@@ -301,8 +303,21 @@ def _cart_mesh(g):
         raise InputError("CART_read: ezflt must be >= -1")
     if g.get("split", False):
         raise InputError("MESH_CART split is not supported by this build")
+    doms = []
+    for d in domains:
+        tag = int(d.get("tag", 0))
+        ex = [int(v) for v in d.get("ex", [0, 0])]
+        ez = [int(v) for v in d.get("ez", [0, 0])]
+        if tag < 1:
+            raise InputError("CART_read: tag null, negative or missing")
+        if not all(1 <= v <= nx for v in ex) or not all(1 <= v <= nz for v in ez):
+            raise InputError("CART_read: ex or ez out of bounds or missing")
+        doms.append((tag, ex, ez))
+    if not doms:  # a single domain spans the whole box
+        doms = [(1, [1, nx], [1, nz])]
     return dict(xmin=float(xlim[0]), xmax=float(xlim[1]), zmin=float(zlim[0]), zmax=float(zlim[1]),
-                nx=nx, nz=nz, ezflt=ezflt, fztag=int(g.get("fztag", 0)), fznz=int(g.get("fznz", 1)))
+                nx=nx, nz=nz, ezflt=ezflt, fztag=int(g.get("fztag", 0)), fznz=int(g.get("fznz", 1)),
+                domains=doms)
 
 
 def _rcm_perm(nx, nz):
@@ -1038,16 +1053,27 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
         cd[key] = _read_cd(par, it, g.get(key, dflt), g.get(key + "h", ""))
     friction = g.get("friction", "SWF")
     fr = friction if isinstance(friction, list) else [friction]
-    fr = [f for f in fr if f]
-    if len(fr) != 1 or fr[0] not in ("SWF", "RSF"):
-        raise InputError("BC_DYNFLT friction must be 'SWF' or 'RSF' in this build")
-    law = fr[0]
-    # friction block: the next &BC_DYNFLT_SWF / _RSF after the distributions
+    fr = [str(f).strip().upper() for f in fr if f]
+    # friction(1:2) (SRC/bc_dynflt.f90:188-202): SWF or RSF, optionally with TWF (the
+    # minimum of both coefficients), or TWF alone
+    base = [f for f in fr if f != "TWF"]
+    if len(fr) > 2 or len(base) > 1 or any(f not in ("SWF", "RSF", "TWF") for f in fr) or not fr:
+        raise InputError("BC_DYNFLT friction must be 'SWF' or 'RSF', optionally with 'TWF', or 'TWF'")
+    law = base[0] if base else "TWF"
+    # friction blocks, read in the order of friction(:) from the input position after
+    # the distributions: &BC_DYNFLT_SWF / _RSF (and their distributions), &BC_DYNFLT_TWF
     rest = list(it)
-    fb = next((x for x in rest if x.name == "BC_DYNFLT_" + law), None)
+    fb = next((x for x in rest if x.name == "BC_DYNFLT_" + law), None) if law != "TWF" else None
     fr_it = iter(rest[rest.index(fb) + 1:]) if fb is not None else iter([])
+    twf = None
+    if "TWF" in fr:
+        after = rest[rest.index(fb) + 1:] if (fb is not None and fr.index("TWF") > fr.index(law)) else rest
+        tb = next((x for x in after if x.name == "BC_DYNFLT_TWF"), None)
+        twf = _twf_read(tb)
     fp = {}
-    if law == "SWF":
+    if law == "TWF":
+        pass
+    elif law == "SWF":
         vals = {"kind": 1, "dc": 0.5, "mus": 0.6, "mud": 0.5, "alpha": 0.0, "p": 3.0, "healing": False}
         if fb is not None:
             for k in vals:
@@ -1110,7 +1136,10 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
              coord=c1, n1=n1, B=B,
              invM1=invM1, invM2=invM2, Z=Z, coefA2V=coefA2V, coefA2D=coefA2D, law=law,
              opening=bool(g.get("opening", True)))
-    if law == "SWF":
+    f["twf"] = twf
+    if law == "TWF":
+        f["theta"] = np.zeros(npf)
+    elif law == "SWF":
         f["swf_kind"] = fp["kind"]
         f["healing"] = fp["healing"]
         for k in ("dc", "mus", "mud", "alpha", "p"):
@@ -1144,8 +1173,11 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
     # initial friction coefficient (SRC/bc_dynflt.f90:404-413)
     if law == "SWF":
         f["MU0"] = swf_mu(f["swf_kind"], f["theta"], f["dc"], f["mus"], f["mud"], f["p"], f["alpha"])
-    else:
+    elif law == "RSF":
         f["MU0"] = rsf_mu(f["rsf_kind"], V0[:, 0], f)
+    if twf is not None:
+        mu_t = twf_mu(twf, c1, 0.0, np.zeros(npf))
+        f["MU0"] = mu_t if law == "TWF" else np.minimum(f["MU0"], mu_t)
     # outputs (SRC/bc_dynflt.f90:419-520)
     oxi = g.get("oxi", [1, 2 ** 31 - 1, 1])
     oix1 = max(int(oxi[0]), 1)
@@ -1166,6 +1198,49 @@ def _nint(x):
 # vectorised numpy versions may differ from them in the last bit)
 _exp, _log, _asinh = np.vectorize(math.exp), np.vectorize(math.log), np.vectorize(math.asinh)
 _pow = np.vectorize(math.pow)
+
+
+def _twf_read(tb):
+    """twf_read (SRC/bc_dynflt_twf.f90:53-107): kind, MuS, MuD, Mu0, X, Z, V, L, T, Dc."""
+    v = dict(kind=1, mus=0.6, mud=0.5, mu0=0.6, x=0.0, z=0.0, v=1e3, l=1.0, t=sys.float_info.max,
+             dc=sys.float_info.max)
+    if tb is not None:
+        for k in v:
+            if k in tb:
+                v[k] = tb.get(k)
+    v = {k: (int(x) if k == "kind" else float(x)) for k, x in v.items()}
+    if not 1 <= v["kind"] <= 3:
+        raise InputError("BC_DYNFLT_TWF: kind must be 1, 2 or 3")
+    if min(v["mus"], v["mud"], v["mu0"]) < 0 or min(v["l"], v["v"], v["t"], v["dc"]) <= 0:
+        raise InputError("BC_DYNFLT_TWF: MuS, MuD, Mu0 must be positive, L, V, T, Dc strictly positive")
+    return v
+
+
+def twf_mu(tw, coord, time, d):
+    """twf_mu (SRC/bc_dynflt_twf.f90:129-183) at the fault nodes (coord [n][2]) at time
+    ``time`` with slip d; kind 3 leaves nodes beyond V*T unset in the reference, they
+    get the huge value here (no weakening)."""
+    big = sys.float_info.max
+    mus, mud, L, V, T = tw["mus"], tw["mud"], tw["l"], tw["v"], tw["t"]
+    dx, dz = coord[:, 0] - tw["x"], coord[:, 1] - tw["z"]
+    dist = np.sqrt(dx * dx + dz * dz)
+    if tw["kind"] in (1, 2):
+        if tw["kind"] == 1:
+            t = time + (mus - tw["mu0"]) * L / ((mus - mud) * V)
+            if t > T:
+                t = 0.0
+            r = V * t
+        else:
+            t = time + 0.5 * T * (1.0 - math.sqrt(1.0 - 4.0 * (mus - tw["mu0"]) * L / ((mus - mud) * T * V)))
+            t = min(t, T)
+            r = V * t * (1.0 - t / T)
+        r = dist - r
+        return np.where(r < -L, mud, np.where(r <= L, mus + (mus - mud) / L * r, big))
+    vt = V * time
+    inside = dist <= V * T
+    mu = np.where(dist < vt - L, mud, np.where((dist >= vt - L) & (dist <= vt), mus + (mus - mud) / L * (dist - vt),
+                                                 big))
+    return np.where(inside & (d <= tw["dc"]), mu, big)
 
 
 def swf_mu(kind, theta, dc, mus, mud, p, alpha):
@@ -1481,9 +1556,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     md = par.first("MESH_DEF")
     method = str(md.get("method", "")).upper() if md is not None else ""
     if method == "CARTESIAN":
-        mesh = _cart_mesh(par.first("MESH_CART"))
-        for d in par.all("MESH_CART_DOMAIN"):
-            raise InputError("MESH_CART_DOMAIN is not supported by this build")
+        mesh = _cart_mesh(par.first("MESH_CART"), par.all("MESH_CART_DOMAIN"))
     elif method == "MESH2D":
         mesh = _read_mesh2d(par.first("MESH_MESH2D"), fname_dir)
     else:
@@ -1521,13 +1594,23 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     pb.npoin = npoin = num["npoin"]
     pb.elem_ix, pb.elem_iz = num["ix"], num["iz"]
 
-    # element tags (MESH_CART: single domain tag 1, optional fault zone tag)
-    tags = np.ones(nelem, dtype=np.int64) if flat else mesh["tag"].copy()
+    # element tags (CART_build, SRC/mesh_cartesian.f90:277-297): the domains in input
+    # order (a later one overrides an earlier one), then the fault zone
+    if flat:
+        tags = np.zeros(nelem, dtype=np.int64)
+        for tag, ex, ez in mesh["domains"]:
+            inside = ((num["ix"] + 1 >= ex[0]) & (num["ix"] + 1 <= ex[1]) &
+                      (num["iz"] + 1 >= ez[0]) & (num["iz"] + 1 <= ez[1]))
+            tags[inside] = tag
+    else:
+        tags = mesh["tag"].copy()
     if flat and mesh["fztag"] > 0:
         j1 = max(mesh["ezflt"] + 1 - mesh["fznz"], 1)
         j2 = min(mesh["ezflt"] + mesh["fznz"], nz)
         rows = num["iz"] + 1
         tags[(rows >= j1) & (rows <= j2)] = mesh["fztag"]
+    if (tags == 0).any():
+        raise InputError("CART_build: Domain tags not entirely set")
     pb.tags = tags
     ibool0 = pb.ibool - 1
     ex = pb.coord[ibool0, 0]

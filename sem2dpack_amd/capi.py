@@ -58,6 +58,7 @@ _SIGS = {
     "compute_fint_noalloc": ([C.c_void_p, dp, dp], C.c_int),
     "set_time_scheme": ([C.c_void_p, C.c_int], C.c_int),
     "set_fault_sides": ([C.c_void_p, C.c_int], C.c_int),
+    "set_fault_twf": ([C.c_void_p, C.c_int, dp, dp], C.c_int),
     "fetch_fault_sides": ([C.c_void_p, dp, ip], C.c_int),
     "set_itinfo": ([C.c_void_p, C.c_int], C.c_int),
     "fetch_info": ([C.c_void_p, dp, ip], C.c_int),

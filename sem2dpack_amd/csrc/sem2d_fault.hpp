@@ -42,6 +42,11 @@ struct FaultDev {
   // the fault, zeroed in the force of node 1 (bits 0-1) and node 2 (bits 2-3) before the
   // corrector (null: none)
   const unsigned char* dmask = nullptr;
+  // time-weakening friction (BC_DYNFLT_TWF): kind (0: none; law 3: time weakening alone),
+  // parameters MuS, MuD, Mu0, X, Z, V, L, T, Dc and the fault node coordinates [2][np]
+  int twf = 0;
+  double tw[9] = {};
+  const double* xz = nullptr;
 };
 
 // Corrector of one split-fault node after the fault solve (a = f * rmass, then v and d;
@@ -64,6 +69,42 @@ __device__ inline double f_swf_mu(const FaultDev& F, int k) {
   else if (F.kind == 2) mu = mud - (mud - mus) * exp(-th / dc);
   else mu = mud + (mus - mud) / pow(1.0 + th / dc, F.p4[k]);
   return mu + F.p3[k] * th;
+}
+
+// twf_mu (SRC/bc_dynflt_twf.f90:129-183) of kind 1..3 with tw = MuS, MuD, Mu0, X, Z, V, L,
+// T, Dc at point (x, z), time t, slip d; kind 3 leaves the points beyond V*T unset in the
+// reference: they get huge(1d0) here (no weakening).  Host and device.
+__host__ __device__ inline double twf_mu(int kind, const double* tw, double x, double z, double time, double d) {
+  const double mus = tw[0], mud = tw[1], mu0 = tw[2], X = tw[3], Z = tw[4];
+  const double V = tw[5], L = tw[6], T = tw[7], Dc = tw[8];
+  const double big = 1.7976931348623157e308;
+  const double dx = x - X, dz = z - Z;
+  const double dist = sqrt(dx * dx + dz * dz);
+  if (kind == 1 || kind == 2) {
+    double t, r;
+    if (kind == 1) {
+      t = time + (mus - mu0) * L / ((mus - mud) * V);
+      if (t > T) t = 0.0;
+      r = V * t;
+    } else {
+      t = time + 0.5 * T * (1.0 - sqrt(1.0 - 4.0 * (mus - mu0) * L / ((mus - mud) * T * V)));
+      t = fmin(t, T);
+      r = V * t * (1.0 - t / T);
+    }
+    r = dist - r;
+    if (r < -L) return mud;
+    if (r <= L) return mus + (mus - mud) / L * r;
+    return big;
+  }
+  if (dist <= V * T && d <= Dc) {
+    if (dist < V * time - L) return mud;
+    if (dist >= V * time - L && dist <= V * time) return mus + (mus - mud) / L * (dist - V * time);
+  }
+  return big;
+}
+
+__device__ inline double f_twf_mu(const FaultDev& F, int k, double time, double d) {
+  return twf_mu(F.twf, F.tw, F.xz[k], F.xz[F.np + k], time, d);
 }
 
 __device__ inline double f_rsf_mu(const FaultDev& F, int k, double v) {
@@ -177,11 +218,11 @@ __device__ inline double f_rsf_v(const FaultDev& F, int k, double tstick, double
 // CORR (structured grids): the pair's two nodes are finished here too (fault_node_correct
 // with the inverse masses rmn1, rmn2); the general path corrects every node afterwards.
 // launched with 64 lanes per workgroup: the Newton iteration's registers are then not
-// capped by the 1024-lane default (which spilled them to scratch)
+// capped by the 1024-lane default (which spilled them to scratch).  time = it*dt of the
+// step (SRC/main.f90:53), read by time-weakening friction only.
 template <typename R, bool CORR = false>
 __global__ __launch_bounds__(64) void k_fault(FaultDev F, R* __restrict__ d, R* __restrict__ v, R* __restrict__ f,
-                                              int npoin,
-                        R c3 = 0, R c4 = 0) {
+                                              int npoin, R c3 = 0, R c4 = 0, double time = 0.0) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= F.np) return;
   const int np = F.np, nd = F.ndof;
@@ -227,8 +268,14 @@ __global__ __launch_bounds__(64) void k_fault(FaultDev F, R* __restrict__ d, R* 
     F.theta[k] = th;
     F.V[k] = vn;
     F.MU[k] = f_rsf_mu(F, k, vn);
+    // superimposed time weakening (SRC/bc_dynflt.f90:632)
+    if (F.twf) F.MU[k] = fmin(F.MU[k], f_twf_mu(F, k, time, F.D[k]));
     const double strength = -F.MU[k] * sigma;
     T[0] = copysign(fabs(strength), T[0]);
+  } else if (F.law == 3) {  // time weakening alone (SRC/bc_dynflt.f90:656-657)
+    F.MU[k] = f_twf_mu(F, k, time, F.D[k]);
+    const double strength = F.coh[k] - F.MU[k] * sigma;
+    T[0] = copysign(fmin(fabs(T[0]), strength), T[0]);
   } else {
     if (F.a2d == 0.0) {
       if (F.healing) {
@@ -242,6 +289,7 @@ __global__ __launch_bounds__(64) void k_fault(FaultDev F, R* __restrict__ d, R* 
       F.theta[k] = fabs(F.D[k]);
     }
     F.MU[k] = f_swf_mu(F, k);
+    if (F.twf) F.MU[k] = fmin(F.MU[k], f_twf_mu(F, k, time, F.D[k]));  // :653
     const double strength = F.coh[k] - F.MU[k] * sigma;
     T[0] = copysign(fmin(fabs(T[0]), strength), T[0]);
   }

@@ -1139,7 +1139,8 @@ void launch_structured_step(const StepDev& X, int it, R dt, R c1, R c2, R c3, R 
                                                 X.ampl, X.ampl_nt, it, X.itp, X.dabso, (const R*)X.rmdef, (R*)X.d,
                                                 (R*)X.v, (R*)X.a, X.fstride, dt, c1, c2, c3, c4);
   if (X.fault_on) {
-    k_fault<R, true><<<grid_for(X.F.np, 64), 64, 0, X.stream>>>(X.F, (R*)X.d, (R*)X.v, (R*)X.a, X.fstride, c3, c4);
+    k_fault<R, true><<<grid_for(X.F.np, 64), 64, 0, X.stream>>>(X.F, (R*)X.d, (R*)X.v, (R*)X.a, X.fstride, c3, c4,
+                                                                (double)it * X.F.dt);
   }
 }
 
@@ -1205,7 +1206,9 @@ int do_steps(sem2d_solver* h, int nsteps) {
       if (!D.after_fault)
         k_dirichlet<R><<<grid_for(D.node.size(), 128), 128, 0, h->stream>>>((int)D.node.size(), D.node_dev, D.comp,
                                                                            h->ndof, a, h->npoin);
-    if (h->flt.on) k_fault<R><<<grid_for(h->flt.dev.np, 64), 64, 0, h->stream>>>(h->flt.dev, d, v, a, h->npoin);
+    if (h->flt.on)
+      k_fault<R><<<grid_for(h->flt.dev.np, 64), 64, 0, h->stream>>>(h->flt.dev, d, v, a, h->npoin, R(0), R(0),
+                                                                    (double)it * h->dt);
     for (auto& D : h->dirs)
       if (D.after_fault)
         k_dirichlet<R><<<grid_for(D.node.size(), 128), 128, 0, h->stream>>>((int)D.node.size(), D.node_dev, D.comp,
@@ -1611,7 +1614,8 @@ int sem2d_set_fault(sem2d_solver* h, const sem2d_fault_desc* f, int nsamp) {
   if (!h) return 1;
   if (!f || f->npoin <= 0 || nsamp < 1) return set_err(h, 3, "set_fault: bad arguments");
   if (h->flt.on) return set_err(h, 3, "set_fault: only one fault per solver");
-  if (f->law != 1 && f->law != 2) return set_err(h, 3, "set_fault: law must be 1 (SWF) or 2 (RSF)");
+  if (f->law < 1 || f->law > 3)
+    return set_err(h, 3, "set_fault: law must be 1 (SWF), 2 (RSF) or 3 (time weakening, sem2d_set_fault_twf)");
   if ((f->law == 1 && (f->kind < 1 || f->kind > 3)) || (f->law == 2 && (f->kind < 1 || f->kind > 4)))
     return set_err(h, 3, "set_fault: invalid friction kind");
   if (f->oitd < 1 || f->oixd < 1 || f->oix1 < 1 || f->oixn > f->npoin || f->oixn < f->oix1)
@@ -1692,7 +1696,9 @@ int sem2d_set_fault(sem2d_solver* h, const sem2d_fault_desc* f, int nsamp) {
   for (int k = 0; k < np; ++k) {
     sig[k] = f->T0[np + k];
     const double dc = pp[0][k], mus = pp[1][k];
-    if (f->law == 1) {
+    if (f->law == 3) {  // time weakening alone: set by sem2d_set_fault_twf
+      mu[k] = 1.7976931348623157e308;
+    } else if (f->law == 1) {
       const double mud = pp[2][k], t = th[k];
       double m;
       if (f->kind == 1) m = mus - (mus - mud) * std::fmin(t / dc, 1.0);
@@ -1804,6 +1810,35 @@ int sem2d_fetch_fault(sem2d_solver* h, double* out, int* taken) {
     HIPCHK(h, hipMemcpy(out, h->flt.rec, (size_t)h->flt.taken * 6 * h->flt.nout * sizeof(double),
                         hipMemcpyDeviceToHost));
   if (taken) *taken = h->flt.taken;
+  return 0;
+}
+
+int sem2d_set_fault_twf(sem2d_solver* h, int kind, const double* par, const double* coord) {
+  if (!h) return 1;
+  FaultHost& F = h->flt;
+  if (!F.on) return set_err(h, 4, "set_fault_twf: call sem2d_set_fault first");
+  if (h->it != 0) return set_err(h, 4, "set_fault_twf: the solver has already been stepped");
+  if (kind < 1 || kind > 3 || !par || !coord) return set_err(h, 3, "set_fault_twf: bad arguments");
+  FaultDev& D = F.dev;
+  const int np = D.np;
+  D.twf = kind;
+  for (int q = 0; q < 9; ++q) D.tw[q] = par[q];
+  double* xz = nullptr;
+  int rc = upload(h, &xz, coord, (size_t)2 * np);
+  if (rc) return rc;
+  D.xz = xz;
+  // initial friction (SRC/bc_dynflt.f90:404-413): the minimum with the time-weakening
+  // coefficient at t = 0, or that coefficient alone (law 3); slip D = 0
+  std::vector<double> mu(np);
+  HIPCHK(h, hipMemcpy(mu.data(), D.MU, np * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < np; ++k) {
+    const double mt = twf_mu(kind, par, coord[k], coord[np + k], 0.0, 0.0);
+    mu[k] = D.law == 3 ? mt : std::fmin(mu[k], mt);
+  }
+  HIPCHK(h, hipMemcpy(D.MU, mu.data(), np * sizeof(double), hipMemcpyHostToDevice));
+  if (F.taken)  // the friction row of the record of step 0, taken by sem2d_set_fault
+    k_fault_record<<<grid_for(F.nout, 64), 64, 0, h->stream>>>(D, F.oix1, F.oixd, F.nout, F.rec);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
 
@@ -2185,6 +2220,7 @@ int check_members(sem2d_batch* b) {
       return berr(b, 3, "member %d: time step, time scheme and Newmark parameters must match", m);
     if (h->flt.on != h0->flt.on) return berr(b, 3, "member %d: either every member has a fault or none", m);
     if (h->flt.srec) return berr(b, 3, "member %d: per-side fault records (osides) are not batched", m);
+    if (h->flt.on && h->flt.dev.twf) return berr(b, 3, "member %d: time-weakening friction is not batched", m);
     for (const AbsoHost& B : h->abso_host)
       if (!B.senode.empty()) return berr(b, 3, "member %d: Stacey boundaries are not batched", m);
     if (h->flt.on && h->fault_n2.empty())

@@ -39,6 +39,7 @@ module sem2d_hip
             sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
             sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_set_dirichlet, &
             sem2d_set_25d, sem2d_set_time_scheme, sem2d_set_fault_sides, sem2d_fetch_fault_sides, &
+            sem2d_set_fault_twf, &
             sem2d_step, &
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
@@ -167,6 +168,15 @@ module sem2d_hip
       type(c_ptr), value :: h
       real(c_double), intent(out) :: out(*)
       integer(c_int), intent(out) :: nsamp_taken
+    end function
+
+    ! time-weakening friction (BC_DYNFLT_TWF): par = MuS, MuD, Mu0, X, Z, V, L, T, Dc,
+    ! coord(npoin, 2) the fault node coordinates; after sem2d_set_fault
+    integer(c_int) function sem2d_set_fault_twf(h, kind, par, coord) bind(C, name='sem2d_set_fault_twf')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: kind
+      real(c_double), intent(in) :: par(9), coord(*)
     end function
 
     ! per-side fault records (osides): set after sem2d_set_fault; out(nout, 4*ndof, nsamp)

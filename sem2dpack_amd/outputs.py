@@ -32,7 +32,10 @@ def _shortest_digits(v, kind):
     a = ftype(abs(v))
     x = decimal.Decimal(float(a))
     below = decimal.Decimal(float(np.nextafter(a, ftype(0))))
-    above = decimal.Decimal(float(np.nextafter(a, ftype(np.inf))))
+    if a < np.finfo(ftype).max:
+        above = decimal.Decimal(float(np.nextafter(a, ftype(np.inf))))
+    else:  # the largest finite value: its rounding interval is as wide above as below
+        above = 2 * x - below
     less, more = (x + below) / 2, (x + above) / 2
     e10 = x.adjusted()  # x = d.ddd x 10**e10
     with decimal.localcontext() as ctx:
@@ -127,9 +130,11 @@ def write_fault(outdir, fault, records, dt, nt, sides=None):
     tag = int(fault["tags"][0])
     nout = records.shape[2]
     rows = records if sides is None else np.concatenate([records, sides], axis=1)
+    with np.errstate(over="ignore"):  # real() of huge(1d0) (time-weakening friction) is +Inf
+        rows32 = rows.astype(np.float32)
     with open(os.path.join(outdir, f"Flt{tag:02d}_sem2d.dat"), "wb") as fh:
         marker = np.array([4 * nout], dtype=np.int32).tobytes()
-        for rec in rows.astype(np.float32):
+        for rec in rows32:
             for q in range(rec.shape[0]):
                 fh.write(marker)
                 fh.write(np.ascontiguousarray(rec[q]).tobytes())

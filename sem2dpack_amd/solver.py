@@ -168,7 +168,10 @@ class Solver:
         fd.cohesion = arr(f["cohesion"])
         fd.coefA2V, fd.coefA2D = f["coefA2V"], f["coefA2D"]
         fd.allow_opening = 1 if f["opening"] else 0
-        if f["law"] == "SWF":
+        if f["law"] == "TWF":
+            fd.law, fd.kind, fd.healing = 3, 1, 0
+            names = ()
+        elif f["law"] == "SWF":
             fd.law, fd.kind, fd.healing = 1, f["swf_kind"], 1 if f["healing"] else 0
             names = ("dc", "mus", "mud", "alpha", "p")
         else:
@@ -183,6 +186,10 @@ class Solver:
         self.fault_nsamp = (self.pb.nt - f["oit"]) // f["oitd"] + 1
         self.npf = npf
         self._check(self.api.set_fault(self.h, C.byref(fd), int(self.fault_nsamp)), "set_fault")
+        tw = f.get("twf")
+        if tw is not None:
+            par = arr([tw[k] for k in ("mus", "mud", "mu0", "x", "z", "v", "l", "t", "dc")])
+            self._check(self.api.set_fault_twf(self.h, int(tw["kind"]), par, arr(f["coord"].T)), "set_fault_twf")
         if f.get("osides"):
             self._check(self.api.set_fault_sides(self.h, 1), "set_fault_sides")
 

@@ -81,7 +81,8 @@ def test_oracle_matches_reference(name, oracle_api):
             assert np.array_equal(rec[::st_every, 0], slip)
         if "flt_f32" in g:
             ref32 = g["flt_f32"]
-            ours = rec[::st_every].astype(np.float32)
+            with np.errstate(over="ignore"):  # huge(1d0) friction (time weakening) -> +Inf, as real()
+                ours = rec[::st_every].astype(np.float32)
             cols = [0, 1, 2, 4] + ([5] if pb.fault["law"] == "SWF" else [])
             assert np.array_equal(ours[:, cols], ref32[:, cols])
             if pb.fault.get("osides"):  # per-side rows after the six of each record
