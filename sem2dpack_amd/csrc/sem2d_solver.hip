@@ -65,6 +65,9 @@ struct StructState {
   std::vector<int4> tiles;
   int ntiles = 0, per_xcd = 0;
   int pipe_wgs = 0;  // > 0: persistent pipelined tile kernel with this many workgroups
+  // Kelvin-Voigt eta and 2.5D coefficient per element point, cell order [cell][25] (bulk
+  // type), or null; either makes the tile kernel form d + eta v at the element points
+  void *eta = nullptr, *b25 = nullptr;
   int pipe_threads = 512;
   int4* tiles_dev = nullptr;
   int4* grids_dev = nullptr;
@@ -913,11 +916,11 @@ int prepare_structured(sem2d_solver* h) {
   if ((rc = prep_upload_bulk(h, &S.rmdef, X.rmdef))) return rc;
   S.FPS = (size_t)S.ntiles * SLOTS;
   {
-    const std::vector<int> list = tile_list(X.tiles, fits32(h->fstride, S.FPS,
-                                                          h->coef_mode == COEF_SHARED
-                                                              ? (size_t)h->nelast * NP
-                                                              : (size_t)h->nelem * std::max(h->ndof, h->nelast) * NP,
-                                                          rsize(h)), S.nfull);
+    // per-cell arrays: the element coefficients and the element-point terms (if any)
+    size_t ncell = h->coef_mode == COEF_SHARED ? (size_t)h->nelast * NP
+                                               : (size_t)h->nelem * std::max(h->ndof, h->nelast) * NP;
+    if (S.eta || S.b25) ncell = std::max(ncell, (size_t)h->nelem * NP);
+    const std::vector<int> list = tile_list(X.tiles, fits32(h->fstride, S.FPS, ncell, rsize(h)), S.nfull);
     S.npart = S.ntiles - S.nfull;
     if ((rc = prep_upload(h, &S.tlist_dev, list.data(), list.size()))) return rc;
   }
@@ -976,6 +979,7 @@ struct StepDev {
   int pipe_wgs = 0, pipe_threads = 512;
   const void *rmass0 = nullptr, *coefs = nullptr, *geom = nullptr;
   const void* nodemod = nullptr;  // per-node moduli (COEF_NODE), or null
+  const void *eta = nullptr, *b25 = nullptr;  // element-point terms (Kelvin-Voigt, 2.5D), or null
   void* fpart = nullptr;
   int ndef = 0;
   DefTable defs;
@@ -1023,6 +1027,8 @@ StepDev step_dev(sem2d_solver* h) {
   X.rmass0 = S.rmass0;
   X.coefs = S.coefs;
   X.nodemod = S.nodemod;
+  X.eta = S.eta;
+  X.b25 = S.b25;
   X.geom = S.geom;
   X.fpart = S.fpart;
   X.ndef = S.ndef;
@@ -1039,13 +1045,20 @@ StepDev step_dev(sem2d_solver* h) {
   return X;
 }
 
-// Pipelined tile kernel over n listed tiles (FULL: all of them full tiles)
-template <typename R, int NDOF, int NELAST, int MODE, bool OOP, bool FULL>
+// Pipelined tile kernel over n listed tiles (FULL: all of them full tiles; EP: with the
+// element-point terms, 512 threads)
+template <typename R, int NDOF, int NELAST, int MODE, bool OOP, bool FULL, bool EP = false>
 void launch_pipe(const StepDev& X, const int* list, int n, R dt, R c1, R c2, R c3, R c4, const R* di,
                  const R* vi, const R* ai) {
   if (n <= 0) return;
   const int wgs = std::min(X.pipe_wgs, (n + 7) / 8 * 8);
   const R* co = (const R*)(MODE == COEF_NODE ? X.nodemod : X.coefs);
+  if constexpr (EP) {
+    k_tile_pipe<R, NDOF, NELAST, MODE, 512, 20, OOP, FULL, true><<<wgs, 512, 0, X.stream>>>(
+        X.T, list, n, X.T.tiles, X.T.grids, X.T.spec_cnt, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, co,
+        (const R*)X.geom, (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai, (const R*)X.eta, (const R*)X.b25);
+    return;
+  }
   if (X.pipe_threads == 256)
     k_tile_pipe<R, NDOF, NELAST, MODE, 256, 10, OOP, FULL><<<wgs, 256, 0, X.stream>>>(
         X.T, list, n, X.T.tiles, X.T.grids, X.T.spec_cnt, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, co,
@@ -1058,18 +1071,19 @@ void launch_pipe(const StepDev& X, const int* list, int n, R dt, R c1, R c2, R c
 
 // both launches (full tiles, then edge tiles) of the pipelined kernel; compact moduli
 // are read once per node when the grid has them in that form
-template <typename R, int NDOF, int NELAST, int MODE, bool OOP>
+template <typename R, int NDOF, int NELAST, int MODE, bool OOP, bool EP = false>
 void launch_pipe2(const StepDev& X, R dt, R c1, R c2, R c3, R c4, const R* di, const R* vi, const R* ai) {
   if constexpr (MODE == COEF_COMPACT && (NELAST == 6 || NELAST == 2)) {
     if (X.nodemod) {
-      launch_pipe<R, NDOF, NELAST, COEF_NODE, OOP, true>(X, X.tlist, X.nfull, dt, c1, c2, c3, c4, di, vi, ai);
-      launch_pipe<R, NDOF, NELAST, COEF_NODE, OOP, false>(X, X.tlist + X.nfull, X.npart, dt, c1, c2, c3, c4, di,
-                                                          vi, ai);
+      launch_pipe<R, NDOF, NELAST, COEF_NODE, OOP, true, EP>(X, X.tlist, X.nfull, dt, c1, c2, c3, c4, di, vi, ai);
+      launch_pipe<R, NDOF, NELAST, COEF_NODE, OOP, false, EP>(X, X.tlist + X.nfull, X.npart, dt, c1, c2, c3, c4,
+                                                              di, vi, ai);
       return;
     }
   }
-  launch_pipe<R, NDOF, NELAST, MODE, OOP, true>(X, X.tlist, X.nfull, dt, c1, c2, c3, c4, di, vi, ai);
-  launch_pipe<R, NDOF, NELAST, MODE, OOP, false>(X, X.tlist + X.nfull, X.npart, dt, c1, c2, c3, c4, di, vi, ai);
+  launch_pipe<R, NDOF, NELAST, MODE, OOP, true, EP>(X, X.tlist, X.nfull, dt, c1, c2, c3, c4, di, vi, ai);
+  launch_pipe<R, NDOF, NELAST, MODE, OOP, false, EP>(X, X.tlist + X.nfull, X.npart, dt, c1, c2, c3, c4, di, vi,
+                                                     ai);
 }
 
 template <typename R, int NDOF, int NELAST, int MODE>
@@ -1084,6 +1098,12 @@ void launch_tile_t(const StepDev& X, R dt, R c1, R c2, R c3, R c4) {
             X.T, (R*)X.d, (R*)X.v, (R*)X.a, (const R*)X.rmass0, (const R*)X.coefs, (const R*)X.geom,
             (R*)X.fpart, dt, c1, c2, c3, c4, di, vi, ai);
       }
+      return;
+    }
+  }
+  if constexpr ((NDOF == 2 && NELAST == 6) || (NDOF == 1 && NELAST == 2)) {
+    if (X.eta || X.b25) {  // Kelvin-Voigt or 2.5D: pipelined kernel with element-point terms
+      launch_pipe2<R, NDOF, NELAST, MODE, false, true>(X, dt, c1, c2, c3, c4, nullptr, nullptr, nullptr);
       return;
     }
   }
@@ -1560,11 +1580,31 @@ static int upload_elem_points(sem2d_solver* h, void** dst, const double* src) {
   return upload_bulk(h, dst, e.data(), e.size());
 }
 
+// per-element-point array in cell order for the tile kernels of a structured grid; the
+// tile kernels then run pipelined (they hold these terms in that form only)
+static int upload_cell_points(sem2d_solver* h, void** dst, const double* src) {
+  StructState& S = h->st;
+  const int NP = h->np;
+  std::vector<double> c((size_t)h->nelem * NP);
+  for (int cell = 0; cell < h->nelem; ++cell)
+    std::memcpy(&c[(size_t)cell * NP], src + (size_t)S.cell_elem[cell] * NP, NP * sizeof(double));
+  int rc = upload_bulk(h, dst, c.data(), c.size());
+  if (rc) return rc;
+  // one 512-thread workgroup per CU: the variant's LDS and registers allow no more
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  S.pipe_wgs = ((ncu + 7) / 8) * 8;
+  S.pipe_threads = 512;
+  S.ready = false;
+  return 0;
+}
+
 int sem2d_set_kelvin_voigt(sem2d_solver* h, const double* eta) {
   if (!h) return 1;
   if (!eta) return set_err(h, 3, "set_kelvin_voigt: null eta");
   if (h->it != 0 || h->nrec || h->flt.on || h->nsrc || !h->abso.empty())
     return set_err(h, 4, "set_kelvin_voigt: call it right after sem2d_create");
+  if (h->st.on && (h->nelast == 6 || h->nelast == 2)) return upload_cell_points(h, &h->st.eta, eta);
   int rc = use_general_path(h);
   if (rc) return rc;
   return upload_elem_points(h, &h->kv_eta, eta);
@@ -1575,9 +1615,11 @@ int sem2d_set_25d(sem2d_solver* h, const double* beta) {
   if (!beta) return set_err(h, 3, "set_25d: null beta");
   if (h->it != 0 || h->nrec || h->flt.on || h->nsrc || !h->abso.empty())
     return set_err(h, 4, "set_25d: call it right after sem2d_create");
-  int rc = use_general_path(h);
+  // the general element kernel's copy serves sem2d_compute_fint
+  int rc = upload_elem_points(h, &h->beta25, beta);
   if (rc) return rc;
-  return upload_elem_points(h, &h->beta25, beta);
+  if (h->st.on && (h->nelast == 6 || h->nelast == 2)) return upload_cell_points(h, &h->st.b25, beta);
+  return use_general_path(h);
 }
 
 int sem2d_set_receivers_interp(sem2d_solver* h, int nrec, const int* elem, const double* interp, int isamp,
@@ -2223,6 +2265,7 @@ int check_members(sem2d_batch* b) {
     if (h->flt.on && h->flt.dev.twf) return berr(b, 3, "member %d: time-weakening friction is not batched", m);
     for (const AbsoHost& B : h->abso_host)
       if (!B.senode.empty()) return berr(b, 3, "member %d: Stacey boundaries are not batched", m);
+    if (h->st.eta || h->st.b25) return berr(b, 3, "member %d: Kelvin-Voigt and 2.5D terms are not batched", m);
     if (h->flt.on && h->fault_n2.empty())
       return berr(b, 3, "member %d: faults with one side (on a domain boundary) are not batched", m);
     if (!h->dirs.empty()) return berr(b, 3, "member %d: Dirichlet boundaries are not batched", m);
@@ -3220,6 +3263,7 @@ int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
   if (h->beta != 0.0 || h->gamma != 0.5 || h->leapfrog)
     return set_err(h, 4, "adjoint: needs the explicit Newmark scheme (beta = 0, gamma = 1/2)");
   if (h->flt.on) return set_err(h, 4, "adjoint: faults are not differentiated");
+  if (h->st.eta || h->st.b25) return set_err(h, 4, "adjoint: Kelvin-Voigt and 2.5D terms are not differentiated");
   for (const AbsoHost& B : h->abso_host)
     if (!B.senode.empty()) return set_err(h, 4, "adjoint: Stacey boundaries are not differentiated");
   if (nsteps < 1) return set_err(h, 3, "adjoint_init: nsteps must be >= 1");

@@ -315,11 +315,19 @@ __device__ __forceinline__ uint32_t node_index_full(const int4 gr, int X, int Z,
 // field, inverse-mass, partial-force and per-cell arrays are addressed with 32-bit
 // offsets from per-component bases (the caller guarantees that every component of
 // these arrays is below 4 GiB).
-template <typename R, int NDOF, int NELAST, int MODE, int NT, int EPB, bool OOP = false, bool FULL = false>
+//
+// EP: element-point terms, per element point in cell order [cell][25] (either may be
+// null): Kelvin-Voigt viscosity eta, the element displacement being d + eta v at each
+// point (MAT_KV_add_etav, SRC/mat_kelvin_voigt.f90:139-150), and the 2.5D coefficient
+// b25, the point force then losing b25 (d + eta v) (MAT_ELAST_add_25D_f,
+// SRC/mat_elastic.f90:447-459).  The predicted velocity then passes through LDS.
+template <typename R, int NDOF, int NELAST, int MODE, int NT, int EPB, bool OOP = false, bool FULL = false,
+          bool EP = false>
 // Register budget: FP32 fields at 512 threads run two workgroups per CU (4 waves per
-// SIMD, 128 VGPRs); every other shape runs at most 2 waves per SIMD of 512-thread
-// workgroups or 3 of 256-thread ones, so the kernel may use the registers of 2.
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 && sizeof(R) == 4 ? 4 : 2))) void k_tile_pipe(TileDev T, const int* __restrict__ tlist, int nlist,
+// SIMD, 128 VGPRs); every other shape (and the element-point variant, whose LDS allows
+// one workgroup per CU) runs at most 2 waves per SIMD of 512-thread workgroups or 3 of
+// 256-thread ones, so the kernel may use the registers of 2.
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 && sizeof(R) == 4 && !EP ? 4 : 2))) void k_tile_pipe(TileDev T, const int* __restrict__ tlist, int nlist,
                                                             const int4* __restrict__ ttiles,
                                                             const int4* __restrict__ tgrids,
                                                             const int* __restrict__ tspc, R* __restrict__ d,
@@ -329,7 +337,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
                                                             R* __restrict__ fpart, R dt, R c1, R c2, R c3, R c4,
                                                             const R* __restrict__ din = nullptr,
                                                             const R* __restrict__ vin = nullptr,
-                                                            const R* __restrict__ ain = nullptr) {
+                                                            const R* __restrict__ ain = nullptr,
+                                                            const R* __restrict__ eta = nullptr,
+                                                            const R* __restrict__ b25 = nullptr) {
   // OOP: the step reads the state (din, vin, ain) and writes (d, v, a); otherwise in place
   // out of place: a null vin / ain reads v / a (their step is then in place)
   const R* __restrict__ dsrc = OOP ? din : d;
@@ -362,6 +372,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
   // field layout, component stride T.NGp), loaded with the node values
   constexpr bool NODEM = MODE == COEF_NODE;
   __shared__ LV Mn[NODEM ? PMAX : 1];
+  // EP: eta and b25 of the tile's element points, [local element lz*nxe+lx][point]
+  constexpr int NEPT = EP ? TX * TZ * NP : 1;
+  __shared__ R Eta[NEPT];
+  __shared__ R Bet[NEPT];
 
   const int nwg = gridDim.x;
   const int bid = blockIdx.x;
@@ -456,6 +470,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
 #pragma unroll
   for (int c = 0; c < NDOF; ++c) bMo[c] = NODEM ? ubase(coefs + c * S) : bCo;
   const R* bGe = ubase(geom);
+  const R* bEt = EP && eta ? ubase(eta) : nullptr;
+  const R* bBe = EP && b25 ? ubase(b25) : nullptr;
   // cell index in the element arrays (32 bits for full tiles)
   auto cidx = [&](const int4 gr, int nx, const Geo& G, int lx, int lz) {
     if constexpr (FULL) return (uint32_t)gr.w + (uint32_t)(G.tz0 + lz) * (uint32_t)nx + (uint32_t)(G.tx0 + lx);
@@ -474,6 +490,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
   R sa[NELAST] = {}, sac[NELAST] = {};
   R pg = 0;
   int psp = -1;
+  constexpr int NEP = EP ? (TX * TZ * NP + NT - 1) / NT : 1;  // element points per thread
+  R peta[NEP] = {}, pbet[NEP] = {};
   auto issue = [&](int t) {
     const int4 tg = ttiles[t];
     const int4 gr = tgrids[t];
@@ -521,13 +539,25 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
     } else if constexpr (MODE == COEF_SHARED) {
       if (tid < ETHREADS) load_coefs<R, NDOF, NELAST, MODE>(grid_coefs<R, MODE>(coefs, gr), geom, 0, pe, sa);
     }
+    if constexpr (EP) {
+#pragma unroll
+      for (int j = 0; j < NEP; ++j) {
+        const int idx = tid + NT * j;
+        if (idx < G.nxe * G.nze * NP) {
+          const int e = idx / NP, p = idx - NP * (idx / NP);
+          const auto cell = cidx(gr, tg.z >> 16, G, e % G.nxe, e / G.nxe);
+          if (bEt) peta[j] = fld(bEt, cell * NP + p);
+          if (bBe) pbet[j] = fld(bBe, cell * NP + p);
+        }
+      }
+    }
     if (tid < tspc[t]) psp = T.spec_idx[t * MAXSPECIAL + tid];
   };
 
   // FP64 fields at 512 threads: a thread keeps the predicted velocity and inverse mass
   // of its patch nodes in registers from its predictor to its stores (the registers are
   // there at two waves per SIMD); other shapes pass them through LDS.
-  constexpr bool REGV = sizeof(R) == 8 && NT == 512;
+  constexpr bool REGV = sizeof(R) == 8 && NT == 512 && !EP;
   R vkeep[REGV ? NITER : 1][NDOF] = {}, rkeep[REGV ? NITER : 1] = {};
   // likewise the moduli of the current tile's point tasks (compact mode): copied from
   // the prefetch registers by the predictor instead of staged through LDS
@@ -630,6 +660,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
     if constexpr (NODEM) {
       if (tid < G.nxe * G.nze * 3) Gm[tid / 3][tid - 3 * (tid / 3)] = pg;
     }
+    if constexpr (EP) {
+#pragma unroll
+      for (int j = 0; j < NEP; ++j) {
+        const int idx = tid + NT * j;
+        if (idx < G.nxe * G.nze * NP) {
+          Eta[idx] = peta[j];
+          Bet[idx] = pbet[j];
+        }
+      }
+    }
     if constexpr (MODE == COEF_COMPACT) {
       if (tid < G.nxe * G.nze * 3) Gm[tid / 3][tid - 3 * (tid / 3)] = pg;
       if constexpr (REGM) {
@@ -657,11 +697,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
     // the elements of one colour share no node, so the per-node summation order is the
     // colour order, as in the two-barrier form.
     bool pv_valid = false, pv_active = false;
-    int pv_base = 0, nvb = 0;
+    int pv_base = 0, pv_el = 0, nvb = 0;
 #pragma unroll
     for (int q = 0; q <= PB_MAXB; ++q) {
       bool has = false, active = false;
-      int base = 0;
+      int base = 0, elc = 0;
       if (q < PB_MAXB) {
         const int cc = q / NBATCH, b0 = (q % NBATCH) * EPB;
         const int cx = cc & 1, cz = cc >> 1;
@@ -673,6 +713,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
         if (active) {
           const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
           base = 4 * lz * PX + 4 * lx;
+          elc = lz * G.nxe + lx;
           R A[NELAST];
           if constexpr (MODE == COEF_COMPACT || NODEM) {
             const int el = lz * G.nxe + lx;
@@ -714,11 +755,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
           for (int c = 0; c < NDOF; ++c) dxi[c] = deta[c] = 0;
 #pragma unroll
           for (int k = 0; k < NG; ++k) {
-            const LV u1 = Ud[base + jj * PX + k], u2 = Ud[base + k * PX + ii];
+            const int n1 = base + jj * PX + k, n2 = base + k * PX + ii;
+            const LV u1 = Ud[n1], u2 = Ud[n2];
+            if constexpr (EP) {  // d + eta v at the element's points (k, jj) and (ii, k)
+              const LV w1 = Vp[n1], w2 = Vp[n2];
+              const R e1 = Eta[elc * NP + jj * NG + k], e2 = Eta[elc * NP + k * NG + ii];
 #pragma unroll
-            for (int c = 0; c < NDOF; ++c) {
-              dxi[c] += hA[k] * lv_get<R, NDOF>(u1, c);
-              deta[c] += lv_get<R, NDOF>(u2, c) * hB[k];
+              for (int c = 0; c < NDOF; ++c) {
+                dxi[c] += hA[k] * (lv_get<R, NDOF>(u1, c) + e1 * lv_get<R, NDOF>(w1, c));
+                deta[c] += (lv_get<R, NDOF>(u2, c) + e2 * lv_get<R, NDOF>(w2, c)) * hB[k];
+              }
+            } else {
+#pragma unroll
+              for (int c = 0; c < NDOF; ++c) {
+                dxi[c] += hA[k] * lv_get<R, NDOF>(u1, c);
+                deta[c] += lv_get<R, NDOF>(u2, c) * hB[k];
+              }
             }
           }
           LV(*T1) = T1b[nvb & 1][le];
@@ -766,14 +818,25 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 &&
         }
         LV* fp = &Fn[pv_base + jj * PX + ii];
         const LV f0 = *fp;
+        if (EP && bBe) {  // 2.5D: the point force loses b25 (d + eta v) of its own point
+          const int own = pv_base + jj * PX + ii;
+          const LV uo = Ud[own], wo = Vp[own];
+          const R eo = Eta[pv_el * NP + pe], bo = Bet[pv_el * NP + pe];
 #pragma unroll
-        for (int c = 0; c < NDOF; ++c) f[c] = lv_get<R, NDOF>(f0, c) + (s1[c] + s2[c]);
+          for (int c = 0; c < NDOF; ++c)
+            f[c] = lv_get<R, NDOF>(f0, c) +
+                   ((s1[c] + s2[c]) - bo * (lv_get<R, NDOF>(uo, c) + eo * lv_get<R, NDOF>(wo, c)));
+        } else {
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) f[c] = lv_get<R, NDOF>(f0, c) + (s1[c] + s2[c]);
+        }
         *fp = lv_make<R, NDOF>(f);
       }
       if (has || pv_valid) __syncthreads();
       pv_valid = has;
       pv_active = active;
       pv_base = base;
+      pv_el = elc;
       if (has) ++nvb;
     }
     // the last element batch ended with a barrier: Fn is complete

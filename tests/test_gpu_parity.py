@@ -115,12 +115,12 @@ def test_rsf_ens_history_and_float32_records(oracle_api, oracle_ensemble):
         assert e <= max(TOL, sp), (nm, e, sp)
 
 
-@pytest.mark.parametrize("name", ["sh_swf", "psv_rsf", "rsf_ens"])
+@pytest.mark.parametrize("name", ["sh_swf", "psv_rsf", "rsf_ens", "psv_kv", "twf_inplane"])
 def test_fp32_bulk_rupture_vs_fp64_oracle(name, oracle_api):
     """FP32 bulk update with FP64 friction, state and fault records (the reduced-
     precision form of the rupture configurations) against the FP64 oracle: slip, slip
     rate and seismogram error reported, bar 1e-3 relative L2 (as the FP32 wave cases)."""
-    nt = {"sh_swf": 600, "psv_rsf": 500, "rsf_ens": 400}[name]
+    nt = {"sh_swf": 600, "psv_rsf": 500, "rsf_ens": 400, "psv_kv": 500, "twf_inplane": 600}[name]
     pb, ref, g32 = _pair(name, nt, oracle_api, bulk_fp32=True)
     ref.step(nt)
     g32.step(nt)
