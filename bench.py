@@ -30,7 +30,10 @@ configuration's own step count (3000), and reports them under "variants":
   adjoint  the waveform gradient of a 400 x 400-element P-SV model (16 sources, 64
            receivers): gradients/s and gradient cost over a forward run for the history
            and the sqrt(N)-checkpoint storage plans, FP64 and FP32 bulk, and central
-           finite differences along random directions.
+           finite differences along random directions;
+  c1       the reference's own CPU-runnable case (10 km box, 100 x 100 elements, point
+           force, 2000 steps, 9 receivers) run whole from a fresh solver, against the
+           whole reference run on one core, with the seismograms compared.
 All host-side set-up (problem building, the CPU baseline) runs before the GPU is first
 used.
 """
@@ -102,6 +105,65 @@ def cpu_baseline():
                       f"(its own CPU-time-per-step report; wall {wall:.1f} s incl. init)",
             "steps": S["steps"], "ms_per_step": per_step * 1e3, "cpu_model": model, "nproc": ncpu,
             "cpus_available": naff}
+
+
+def cpu_c1():
+    """Reference sem2dsolve on the whole 2000-step 100x100-element P-SV box (about 20 s on
+    one core); returns its timing and its float32 seismograms [nsamp][2][9]."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "sem2dsolve")
+    if not os.path.exists(exe):
+        return None, None
+    from sem2dpack_amd import workloads
+    import numpy as np
+    with tempfile.TemporaryDirectory(prefix="sem2d_c1_") as tmp:
+        with open(os.path.join(tmp, "Par.inp"), "w") as fh:
+            fh.write(workloads.par_c1())
+        t0 = time.time()
+        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec taskset -c 0 {exe}"], cwd=tmp,
+                           capture_output=True, text=True, timeout=600)
+        wall = time.time() - t0
+        out = r.stdout
+        per = re.findall(r"CPU time per timestep[ .]*([0-9.+\-Ee]+)", out)
+        tot = re.findall(r"Total solver CPU time[ .]*([0-9.+\-Ee]+)", out)
+        if r.returncode != 0 or "FATAL" in out or not per:
+            return {"value": None, "error": (out + r.stderr)[-400:]}, None
+        seis = np.stack([np.fromfile(os.path.join(tmp, n), np.float32).reshape(9, -1).T
+                         for n in ("Ux_sem2d.dat", "Uz_sem2d.dat")], axis=1)
+    return {"ms_per_step": float(per[-1]) * 1e3, "solver_s": float(tot[-1]) if tot else None, "wall_s": wall,
+            "cores": 1, "kind": "reference",
+            "sample": "the whole 2000-step run, reference sem2dsolve flang -O2 on 1 pinned core"}, seis
+
+
+def run_c1(pb, cpu, ref_seis):
+    """The 100x100-element P-SV point-force box: all 2000 steps on the GPU, from a fresh
+    solver, with its 9 velocity receivers recorded every step."""
+    import numpy as np
+    from sem2dpack_amd import solver
+    t0 = time.perf_counter()
+    s = solver.Solver(pb)
+    s.fields()
+    t1 = time.perf_counter()
+    s.step(pb.nt)
+    t2 = time.perf_counter()
+    seis = s.seismograms()
+    ev_ms = s.last_step_ms()
+    npoin = pb.npoin
+    s.close()
+    r = {"workload": "P-SV 10x10 km box 100x100 elements NGLL=5, point force, 4 ABCs, 2000 steps, 9 receivers",
+         "npoin": npoin, "steps": pb.nt, "setup_s": t1 - t0, "run_s": t2 - t1,
+         "ms_per_step": 1e3 * (t2 - t1) / pb.nt, "event_ms_per_step": ev_ms,
+         "value": npoin * pb.nt / (t2 - t1), "unit": "GLL node-updates/s", "dtype": "f64",
+         "finite": bool(np.isfinite(seis).all())}
+    if cpu and cpu.get("ms_per_step"):
+        cpu = dict(cpu)
+        cpu["value"] = npoin / (cpu["ms_per_step"] * 1e-3)
+        cpu["unit"] = "GLL node-updates/s"
+        r["cpu_baseline"] = cpu
+        r["speedup_vs_cpu_run"] = cpu["solver_s"] / (t2 - t1) if cpu.get("solver_s") else None
+    if ref_seis is not None and ref_seis.shape == seis.shape:
+        # the reference's seismogram files hold float32: agreement to ~1e-7 is all they show
+        r["seismo_rel_l2_vs_reference_f32"] = float(np.linalg.norm(seis - ref_seis) / np.linalg.norm(ref_seis))
+    return r
 
 
 def run_gpu(pb, steps, warmup, fp32):
@@ -337,9 +399,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps of the throughput case (default 1000)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["all", "throughput", "batch", "adjoint"], default="all",
+    ap.add_argument("--workload", choices=["all", "throughput", "c1", "batch", "adjoint"], default="all",
                     help="all (default): the 2000x1000-element P-SV throughput case with the batch and "
-                         "adjoint forms as variants; or one of them alone")
+                         "adjoint forms and the 100x100-element box (c1) as variants; or one of them alone")
     ap.add_argument("--form-steps", type=int, default=3000,
                     help="steps of the batch and adjoint forms (their configurations: 3000)")
     ap.add_argument("--fd-dirs", type=int, default=4, help="adjoint: finite-difference directions")
@@ -372,6 +434,13 @@ def main():
         if not args.no_cpu_baseline:
             cpu = cpu_baseline()
             log(f"cpu baseline: {cpu and cpu.get('ms_per_step')} ms/step")
+    c1_pb = c1_cpu = c1_ref = None
+    if wl in ("all", "c1"):
+        from sem2dpack_amd import workloads
+        c1_pb = workloads.build_c1()
+        if not args.no_cpu_baseline:
+            c1_cpu, c1_ref = cpu_c1()
+            log(f"c1 cpu baseline: {c1_cpu and c1_cpu.get('ms_per_step')} ms/step")
     if wl in ("all", "batch"):
         batch_pbs, batch_build_s = build_batch_problems(args.scenarios, args.form_steps, args.build_workers)
         log(f"batch problems built in {batch_build_s:.1f} s")
@@ -381,6 +450,13 @@ def main():
         r = run_batch(batch_pbs, args.form_steps, batch_build_s)
         r.update({"n_gpus": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                   "data": "synthetic", "config": {"workload": r.pop("workload")}})
+        print(json.dumps(r))
+        return
+    if wl == "c1":
+        r = run_c1(c1_pb, c1_cpu, c1_ref)
+        r.update({"metric": "GLL node-updates/s", "n_gpus": 1, "warmup": 0, "higher_is_better": True,
+                  "scaling": "weak", "vs_baseline": None, "data": "synthetic",
+                  "config": {"workload": r.pop("workload")}})
         print(json.dumps(r))
         return
     if wl == "adjoint":
@@ -435,6 +511,9 @@ def main():
             "alg_bytes_per_step": B32, "finite": fin32}
     pb = None
     if wl == "all":
+        line["variants"]["c1"] = run_c1(c1_pb, c1_cpu, c1_ref)
+        log(f"c1: {line['variants']['c1']['ms_per_step']:.4f} ms/step")
+        c1_pb = None
         line["variants"]["batch"] = run_batch(batch_pbs, args.form_steps, batch_build_s)
         batch_pbs = None
         line["variants"]["adjoint"] = run_adjoint(args.form_steps, args.fd_dirs)

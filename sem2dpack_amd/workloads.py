@@ -108,6 +108,38 @@ def build_throughput(nx=2000, nz=1000, lx=200e3, lz=100e3, nt=5000, seed=2026020
     return builder.build(par, nsteps=nt, material=hete1_material(*grid), per_element_coefs=per_element_coefs)
 
 
+# The reference's own CPU-runnable case: a 10 km x 10 km homogeneous P-SV box of
+# 100 x 100 elements, vertical point force (Ricker f0 = 2 Hz, onset 0.5 s) at the
+# centre, four absorbing sides, Courant 0.3, 2000 Newmark steps, 9 velocity
+# receivers on the line z = 7 km, 2 km above the source.
+C1_PAR = """&GENERAL iexec=1, ngll=5, fmax=5d0, ndof=2, title='P-SV box, point force', verbose='1101', ItInfo=500 /
+&MESH_DEF method='CARTESIAN' /
+&MESH_CART xlim=0d0,10d3, zlim=0d0,10d3, nelem=100,100 /
+&MATERIAL tag=1, kind='ELAST' /
+&MAT_ELASTIC rho=2670d0, cp=6000d0, cs=3464d0 /
+&BC_DEF tag=1, kind='ABSORB' /
+&BC_DEF tag=2, kind='ABSORB' /
+&BC_DEF tag=3, kind='ABSORB' /
+&BC_DEF tag=4, kind='ABSORB' /
+&TIME kind='newmark', NbSteps={nt}, Courant=0.3d0 /
+&SRC_DEF stf='RICKER', coord=5d3,5d3, mechanism='FORCE' /
+&STF_RICKER f0=2d0, onset=0.5d0, ampli=1d0 /
+&SRC_FORCE angle=0d0 /
+&REC_LINE number=9, first=3d3,7d3, last=7d3,7d3, field='V' /
+&SNAP_DEF itd=100000, ps=F, bin=F /
+"""
+
+
+def par_c1(nt=2000):
+    return C1_PAR.format(nt=nt)
+
+
+def build_c1(nt=2000):
+    """Build the 100x100-element P-SV point-force box in memory."""
+    from . import builder, parfile
+    return builder.build(parfile.parse_par(par_c1(nt)), nsteps=nt)
+
+
 # ---------------------------------------------------------------------------
 # Batched rupture ensemble
 # ---------------------------------------------------------------------------
