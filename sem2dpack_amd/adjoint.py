@@ -93,7 +93,10 @@ class SeismogramModel:
         nx = pb.mesh["nx"]
         cell = pb.elem_iz * nx + pb.elem_ix  # builder element -> cell
         r, la, m = rho[cell], lam[cell], mu[cell]
-        grho_e = (pb.dvol * gmass[pb.ibool - 1]).sum(axis=(1, 2))
+        ib0 = getattr(pb, "_ibool0", None)
+        if ib0 is None:
+            ib0 = pb._ibool0 = (pb.ibool - 1).astype(np.intp)
+        grho_e = np.einsum("eij,eij->e", pb.dvol, np.take(gmass, ib0))
         glam_e = glam_e.copy()
         gmu_e = gmu_e.copy()
         for ab in pb.abso:

@@ -2979,6 +2979,7 @@ void sem2d_batch_destroy(sem2d_batch* b) {
 // ===========================================================================
 struct AdjState {
   int nsteps = 0, seg = 0, nck = 0;
+  int wgs = 256;  // workgroups of the adjoint tile kernel: one per CU, a multiple of 8
   // history mode: the displacement after every step (FP32 bulk: and the acceleration)
   // kept in device memory, so that the backward sweep needs no replay
   bool hist = false;
@@ -3084,6 +3085,25 @@ int adj_alloc(sem2d_solver* h, void** p, size_t bytes) {
   return 0;
 }
 
+// Inverse mass per component in storage order for the adjoint: the single-component
+// array of the tile kernels (k_adj_rmass), then the deferred nodes' own values
+// (k_adj_rmass_def).
+template <typename R>
+__global__ void k_adj_rmass(size_t GS, int ndof, const R* __restrict__ rm0, R* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= GS) return;
+  const R m = rm0[i];
+  for (int c = 0; c < ndof; ++c) out[(size_t)c * GS + i] = m;
+}
+template <typename R>
+__global__ void k_adj_rmass_def(int ndef, int ndof, DefTable defs, const R* __restrict__ rmdef, size_t GS,
+                                R* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ndef) return;
+  const size_t g = defs.node[q].g;
+  for (int c = 0; c < ndof; ++c) out[(size_t)c * GS + g] = rmdef[(size_t)c * ndef + q];
+}
+
 // copy a whole state (d, v, a) between two state buffers or the solver's fields
 int copy_state(sem2d_solver* h, char* dst_d, char* dst_v, char* dst_a, const char* d, const char* v, const char* a) {
   const size_t n = (size_t)h->ndof * h->fstride * rsize(h);
@@ -3099,7 +3119,12 @@ inline char* st_a(sem2d_solver* h, char* s) { return s + 2 * (size_t)h->ndof * h
 
 // One adjoint tile step: the full tiles of the launch list with the constant-shape
 // variant (every array it addresses with 32-bit offsets below 4 GiB, the double
-// mass-sensitivity array included), the others with the general one.
+// mass-sensitivity array included), the others with the general one.  FP64 fields with
+// a displacement history (the acceleration formed from d_{n-1}, d_n, d_{n+1}):
+// persistent pipelined kernels of one 512-thread workgroup per CU (their LDS allows no
+// more; 162 against 182 us per 400x400-element step).  Otherwise one 256-thread
+// workgroup per tile (stored acceleration, FP64: 139 against 156 us; FP32 fields, four
+// workgroups per CU: 91 against 114 us).
 template <typename R>
 void launch_adj_tile(sem2d_solver* h, const StepDev& X, const R* dfw, const R* afw, const R* dp, const R* dn,
                      R dt, R c1, R c2, R c3, double idt2) {
@@ -3107,14 +3132,28 @@ void launch_adj_tile(sem2d_solver* h, const StepDev& X, const R* dfw, const R* a
   StructState& S = h->st;
   const int nf = (h->fstride * sizeof(double) < (size_t(1) << 32)) ? S.nfull : 0;
   const int np = S.ntiles - nf;
-  if (nf > 0)
-    k_adj_tile<R, true><<<8 * ((nf + 7) / 8), TILE_THREADS, 0, h->stream>>>(
-        X.T, S.tlist_dev, nf, (R*)A.ad, (R*)A.av, (R*)A.aa, (const R*)A.rmass, dfw, afw, (const R*)S.coefs,
-        (const R*)S.geom, (R*)S.fpart, A.glam, A.gmu, A.mbar, dt, c1, c2, c3, dp, dn, idt2);
-  if (np > 0)
-    k_adj_tile<R, false><<<8 * ((np + 7) / 8), TILE_THREADS, 0, h->stream>>>(
-        X.T, S.tlist_dev + nf, np, (R*)A.ad, (R*)A.av, (R*)A.aa, (const R*)A.rmass, dfw, afw,
-        (const R*)S.coefs, (const R*)S.geom, (R*)S.fpart, A.glam, A.gmu, A.mbar, dt, c1, c2, c3, dp, dn, idt2);
+  if (sizeof(R) == 4 || afw) {
+    if (nf > 0)
+      k_adj_tile<R, true><<<8 * ((nf + 7) / 8), TILE_THREADS, 0, h->stream>>>(
+          X.T, S.tlist_dev, nf, (R*)A.ad, (R*)A.av, (R*)A.aa, (const R*)A.rmass, dfw, afw, (const R*)S.coefs,
+          (const R*)S.geom, (R*)S.fpart, A.glam, A.gmu, A.mbar, dt, c1, c2, c3, dp, dn, idt2);
+    if (np > 0)
+      k_adj_tile<R, false><<<8 * ((np + 7) / 8), TILE_THREADS, 0, h->stream>>>(
+          X.T, S.tlist_dev + nf, np, (R*)A.ad, (R*)A.av, (R*)A.aa, (const R*)A.rmass, dfw, afw,
+          (const R*)S.coefs, (const R*)S.geom, (R*)S.fpart, A.glam, A.gmu, A.mbar, dt, c1, c2, c3, dp, dn, idt2);
+  } else if constexpr (sizeof(R) == 8) {
+    auto wgs = [&](int n) { return std::min(A.wgs, (n + 7) / 8 * 8); };
+    if (nf > 0)
+      k_adj_pipe<R, true><<<wgs(nf), AP_NT, 0, h->stream>>>(
+          X.T, S.tlist_dev, nf, X.T.tiles, X.T.grids, X.T.spec_cnt, (R*)A.ad, (R*)A.av, (R*)A.aa,
+          (const R*)A.rmass, dfw, afw, (const R*)S.coefs, (const R*)S.geom, (R*)S.fpart, A.glam, A.gmu, A.mbar,
+          dt, c1, c2, c3, dp, dn, idt2);
+    if (np > 0)
+      k_adj_pipe<R, false><<<wgs(np), AP_NT, 0, h->stream>>>(
+          X.T, S.tlist_dev + nf, np, X.T.tiles, X.T.grids, X.T.spec_cnt, (R*)A.ad, (R*)A.av, (R*)A.aa,
+          (const R*)A.rmass, dfw, afw, (const R*)S.coefs, (const R*)S.geom, (R*)S.fpart, A.glam, A.gmu, A.mbar,
+          dt, c1, c2, c3, dp, dn, idt2);
+  }
 }
 
 template <typename R>
@@ -3290,6 +3329,11 @@ int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
   if (h->flt.pot) hipFree(h->flt.pot);
   h->adj = new AdjState();
   AdjState& A = *h->adj;
+  {
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    A.wgs = std::max(8, (ncu + 7) / 8 * 8);
+  }
   A.nsteps = nsteps;
   A.seg = std::min(seg, nsteps);
   A.nck = (nsteps + A.seg - 1) / A.seg;
@@ -3409,14 +3453,50 @@ int sem2d_adjoint_init(sem2d_solver* h, int nsteps, int seg) {
     if (!uext.empty())
       HIPCHK(h, hipMemcpy(A.uext, uext.data(), uext.size() * sizeof(int), hipMemcpyHostToDevice));
   }
-  // inverse mass of both components in storage order
-  std::vector<double> rm((size_t)h->ndof * h->fstride, 0.0);
-  for (int c = 0; c < h->ndof; ++c)
-    for (int k = 0; k < h->npoin; ++k)
-      rm[(size_t)c * h->fstride + h->st.gmap[k]] = h->rmass_host[(size_t)c * h->npoin + k];
+  // inverse mass of both components in storage order.  The tile kernels' single-
+  // component array holds it at every node whose components are equal; the deferred
+  // nodes' per-component array covers the others (absorbing-boundary nodes), so the
+  // device assembles it from those two when every node with unequal components is a
+  // deferred node (else from the host copy).
   if ((rc = adj_alloc(h, &p, nf))) return rc;
   A.rmass = (char*)p;
-  if ((rc = write_bulk(h, A.rmass, rm.data(), rm.size()))) return rc;
+  const StructState& S = h->st;
+  bool dev_ok = true;
+  {
+    std::vector<char> isdef(h->npoin, 0);
+    for (int q = 0; q < S.ndef; ++q) isdef[S.def_ref[q]] = 1;
+    const double* rh = h->rmass_host.data();
+    for (int c = 1; c < h->ndof && dev_ok; ++c)
+      for (int k = 0; k < h->npoin; ++k)
+        if (rh[(size_t)c * h->npoin + k] != rh[k] && !isdef[k]) {
+          dev_ok = false;
+          break;
+        }
+  }
+  if (dev_ok) {
+    const size_t GS = h->fstride;
+    if (h->fp32) {
+      k_adj_rmass<float><<<grid_for(GS, 256), 256, 0, h->stream>>>(GS, h->ndof, (const float*)S.rmass0,
+                                                                  (float*)A.rmass);
+      if (S.ndef > 0)
+        k_adj_rmass_def<float><<<grid_for(S.ndef, 256), 256, 0, h->stream>>>(
+            S.ndef, h->ndof, S.defs, (const float*)S.rmdef, GS, (float*)A.rmass);
+    } else {
+      k_adj_rmass<double><<<grid_for(GS, 256), 256, 0, h->stream>>>(GS, h->ndof, (const double*)S.rmass0,
+                                                                    (double*)A.rmass);
+      if (S.ndef > 0)
+        k_adj_rmass_def<double><<<grid_for(S.ndef, 256), 256, 0, h->stream>>>(
+            S.ndef, h->ndof, S.defs, (const double*)S.rmdef, GS, (double*)A.rmass);
+    }
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  } else {
+    std::vector<double> rm((size_t)h->ndof * h->fstride, 0.0);
+    for (int c = 0; c < h->ndof; ++c)
+      for (int k = 0; k < h->npoin; ++k)
+        rm[(size_t)c * h->fstride + h->st.gmap[k]] = h->rmass_host[(size_t)c * h->npoin + k];
+    if ((rc = write_bulk(h, A.rmass, rm.data(), rm.size()))) return rc;
+  }
   return 0;
 }
 

@@ -1380,6 +1380,394 @@ __device__ __forceinline__ void inject_node(const InjectPlan& P, int u, R* __res
   }
 }
 
+// Persistent, pipelined form of k_adj_tile, with the same arithmetic and the same
+// summation orders (results equal bit for bit): AP_NT threads, one colour of a tile's
+// elements per batch (two barriers per batch), each workgroup on a run of tiles (the
+// XCD slices of k_tile_pipe).  While a tile's element batches run, the adjoint state,
+// inverse mass, forward displacements (or acceleration), mass sensitivity, moduli,
+// geometric factors and element sensitivities of the workgroup's next tile load into
+// registers; a finished tile is stored by the threads that own its patch slots, right
+// before they overwrite those slots with the next tile's values.  The adjoint state of
+// the patch and the running mass sensitivity wait in LDS for the store (the registers
+// hold the prefetch).  The launch covers tlist[0 .. nlist); the grid is a multiple of 8.
+constexpr int AP_NT = 512;
+constexpr int AP_EPB = 20;  // elements per batch: a whole colour of a full tile (16)
+template <typename R, bool FULL = false>
+__global__ __launch_bounds__(AP_NT) __attribute__((amdgpu_waves_per_eu(2))) void k_adj_pipe(
+    TileDev T, const int* __restrict__ tlist, int nlist, const int4* __restrict__ ttiles,
+    const int4* __restrict__ tgrids, const int* __restrict__ tspc, R* __restrict__ ad, R* __restrict__ av,
+    R* __restrict__ aa, const R* __restrict__ rmass, const R* __restrict__ dfw, const R* __restrict__ afw,
+    const R* __restrict__ coefs, const R* __restrict__ geom, R* __restrict__ fpart, double* __restrict__ glam,
+    double* __restrict__ gmu, double* __restrict__ mbar, R dt, R c1, R c2, R c3,
+    const R* __restrict__ dprev = nullptr, const R* __restrict__ dnext = nullptr, double idt2 = 0.0) {
+  constexpr int NDOF = 2;
+  using LV = typename LVec<R, NDOF>::T;
+  constexpr int NITER = (PMAX + AP_NT - 1) / AP_NT;        // patch nodes per thread
+  constexpr int NBC = (TX * TZ / 4 + AP_EPB - 1) / AP_EPB;  // batches per colour
+  constexpr int NB = 4 * NBC;                               // batches per tile
+  constexpr int ETH = AP_EPB * NP;
+  static_assert(ETH <= AP_NT && 3 * TX * TZ <= AP_NT && MAXSPECIAL <= AP_NT, "tables larger than the workgroup");
+  __shared__ LV Uf[PMAX];  // fb
+  __shared__ LV Ud[PMAX];  // forward d_n
+  __shared__ LV Fn[PMAX];  // -K^T fb, assembled
+  __shared__ LV Ad[PMAX];  // adjoint displacement and velocity, kept for the store
+  __shared__ LV Av[PMAX];
+  __shared__ double Mb[PMAX];  // running mass sensitivity of the interior nodes
+  __shared__ LV T1[AP_EPB][NP];
+  __shared__ LV T2[AP_EPB][NP];
+  __shared__ double Gr[AP_EPB][NP][2];
+  __shared__ double Gs[AP_EPB][NG][2];
+  __shared__ double Ge[TX * TZ][2];
+  __shared__ R Gm[TX * TZ][3];
+  __shared__ short sflag[PMAX];
+  __shared__ LV ML[NB][ETH];  // this tile's moduli (lambda, mu) of each lane's point, per batch
+  __shared__ double sH[NP];
+
+  const int nwg = gridDim.x;
+  const int bid = blockIdx.x;
+  const int W = nwg >> 3;
+  const int slice = (nlist + 7) >> 3;
+  const int s0 = (bid & 7) * slice;
+  const int s1 = min(s0 + slice, nlist);
+  const int q0 = s0 + (bid >> 3);
+  if (q0 >= s1) return;
+  const int tid = threadIdx.x;
+  const size_t S = T.NGp;
+  // lane indices of the node loops and of the element batches, re-materialised per
+  // tile (below) so that the index arithmetic derived from them is recomputed in each
+  // tile rather than held in registers through the loop
+  int tnode = tid, lel = tid / NP;
+  // H entries from LDS (the registers hold the prefetch)
+  if (tid < NP) sH[tid] = c_H[tid];
+  struct Geo {
+    int tx0, tz0, nxe, nze, Z0, PX, PZ, NPT, X0, kL, kR;
+  };
+  auto geo = [&](const int4 tg) {
+    Geo g;
+    g.tx0 = tg.x;
+    g.tz0 = tg.y;
+    g.nxe = FULL ? TX : (tg.z & 0xff);
+    g.nze = FULL ? TZ : ((tg.z >> 8) & 0xff);
+    g.Z0 = tg.w;
+    g.PX = 4 * g.nxe + 1;
+    g.PZ = 4 * g.nze + 1;
+    g.NPT = g.PX * g.PZ;
+    g.X0 = 4 * g.tx0;
+    g.kL = g.tx0 / TX;
+    g.kR = FULL ? g.kL + 1 : (g.tx0 + g.nxe + TX - 1) / TX;
+    return g;
+  };
+  auto nidx = [&](const int4 gr, const Geo& G, int X, int Z) {
+    if constexpr (FULL) return node_index_full(gr, X, Z, G.X0, G.Z0, G.kL);
+    else return node_index(gr, X, Z, G.PX, G.X0, G.Z0, G.kL, G.kR);
+  };
+  auto cidx = [&](const int4 gr, const int4 tg, const Geo& G, int lx, int lz) {
+    if constexpr (FULL)
+      return (uint32_t)gr.w + (uint32_t)(G.tz0 + lz) * (uint32_t)(tg.z >> 16) + (uint32_t)(G.tx0 + lx);
+    else return cell_index(gr, tg.z >> 16, G.tx0, G.tz0, lx, lz);
+  };
+  auto ub = [&](auto* p) {
+    if constexpr (FULL) return sgpr_base(p);
+    else return p;
+  };
+  auto at = [&](auto* base, auto g) -> auto& {
+    if constexpr (FULL) return at32(base, (uint32_t)g);
+    else return base[g];
+  };
+  R *bad[NDOF], *bav[NDOF], *baa[NDOF], *bfp[NDOF];
+  const R *brm[NDOF], *bdf[NDOF], *baf[NDOF], *bdp[NDOF], *bdn[NDOF];
+#pragma unroll
+  for (int c = 0; c < NDOF; ++c) {
+    bad[c] = ub(ad + c * S);
+    bav[c] = ub(av + c * S);
+    baa[c] = ub(aa + c * S);
+    brm[c] = ub(rmass + c * S);
+    bdf[c] = ub(dfw + c * S);
+    baf[c] = afw ? ub(afw + c * S) : nullptr;
+    bdp[c] = dprev ? ub(dprev + c * S) : nullptr;
+    bdn[c] = dnext ? ub(dnext + c * S) : nullptr;
+    bfp[c] = ub(fpart + c * T.FPS);
+  }
+  // x1: the forward acceleration (stored) or d_{n-1}; x2: d_{n+1} (history of d only)
+  const R *bx1[NDOF], *bx2[NDOF];
+#pragma unroll
+  for (int c = 0; c < NDOF; ++c) {
+    bx1[c] = afw ? baf[c] : bdp[c];
+    bx2[c] = bdn[c];
+  }
+  double* bmb = ub(mbar);
+  double* bgl = ub(glam);
+  double* bgm = ub(gmu);
+  const R* bco = ub(coefs);
+  const R* bge = ub(geom);
+
+  // next tile's registers: node values (x1, x2: the forward acceleration, or d_{n-1}
+  // and d_{n+1}), the mass sensitivity of interior nodes, this lane's moduli in each
+  // batch, one geometric factor, one element's two sensitivities, one special node
+  R pad[NITER][NDOF] = {}, pav[NITER][NDOF] = {}, paa[NITER][NDOF] = {}, prm[NITER][NDOF] = {};
+  R pdf[NITER][NDOF] = {}, px1[NITER][NDOF] = {}, px2[NITER][NDOF] = {};
+  double pmb[NITER] = {};
+  R plm[NB][2] = {};
+  R pg = 0;
+  double pge0 = 0, pge1 = 0;
+  int psp = -1;
+  auto issue = [&](int t) {
+    const int4 tg = ttiles[t];
+    const int4 gr = tgrids[t];
+    const Geo G = geo(tg);
+#pragma unroll
+    for (int r = 0; r < NITER; ++r) {
+      const int p = tnode + AP_NT * r;
+      if (p < G.NPT) {
+        const int Z = p / G.PX, X = p - Z * G.PX;
+        const auto g = nidx(gr, G, X, Z);
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          pad[r][c] = at(bad[c], g);
+          pav[r][c] = at(bav[c], g);
+          paa[r][c] = at(baa[c], g);
+          prm[r][c] = at(brm[c], g);
+          pdf[r][c] = at(bdf[c], g);
+          px1[r][c] = at(bx1[c], g);
+          if (!afw) px2[r][c] = at(bx2[c], g);
+        }
+        pmb[r] = perim_slot(X, Z, G.PX, G.PZ) < 0 ? at(bmb, g) : 0.0;
+      }
+    }
+    const int pl = tnode - lel * NP;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int cc = q / NBC, b0 = (q % NBC) * AP_EPB;
+      const int cx = cc & 1, cz = cc >> 1;
+      const int ncx = (G.nxe - cx + 1) >> 1, ncz = (G.nze - cz + 1) >> 1;
+      const int ce = b0 + lel;
+      if (tnode < ETH && ce < ncx * ncz) {
+        const auto cell = cidx(gr, tg, G, cx + 2 * (ce % ncx), cz + 2 * (ce / ncx));
+        plm[q][0] = at(bco, cell * 2 * NP + pl);
+        plm[q][1] = at(bco, cell * 2 * NP + NP + pl);
+      }
+    }
+    if (tid < G.nxe * G.nze * 3) {
+      const int e = tid / 3, k = tid - 3 * (tid / 3);
+      pg = at(bge, cidx(gr, tg, G, e % G.nxe, e / G.nxe) * 3 + k);
+    }
+    if (tid < G.nxe * G.nze) {
+      const auto cell = cidx(gr, tg, G, tid % G.nxe, tid / G.nxe);
+      pge0 = at(bgl, cell);
+      pge1 = at(bgm, cell);
+    }
+    if (tid < tspc[t]) psp = T.spec_idx[t * MAXSPECIAL + tid];
+  };
+
+  // Stores of a finished tile from LDS (the element sensitivities, the partial forces
+  // of perimeter and special nodes, the new adjoint state and mass sensitivity of the
+  // others), by the threads that own the slots
+  auto store_tile = [&](const int tp, const int4 tgp, const int4 grp) {
+    const Geo G = geo(tgp);
+    if (tid < G.nxe * G.nze) {
+      const auto cell = cidx(grp, tgp, G, tid % G.nxe, tid / G.nxe);
+      at(bgl, cell) = Ge[tid][0];
+      at(bgm, cell) = Ge[tid][1];
+    }
+    const uint32_t fbase = (uint32_t)tp * SLOTS;
+#pragma unroll
+    for (int r = 0; r < NITER; ++r) {
+      const int p = tnode + AP_NT * r;
+      if (p < G.NPT) {
+        const int Z = p / G.PX, X = p - Z * G.PX;
+        int slot = perim_slot(X, Z, G.PX, G.PZ);
+        if (slot < 0 && sflag[p] >= 0) slot = PERIM_MAX + sflag[p];
+        const LV fn = Fn[p];
+        if (slot >= 0) {
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) at(bfp[c], fbase + slot) = lv_get<R, NDOF>(fn, c);
+        } else {
+          const auto g = nidx(grp, G, X, Z);
+          const LV a0 = Ad[p], v0 = Av[p];
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) {
+            const R dn = lv_get<R, NDOF>(a0, c) + lv_get<R, NDOF>(fn, c);
+            const R vbp = lv_get<R, NDOF>(v0, c);
+            at(bad[c], g) = dn;
+            at(bav[c], g) = vbp + dt * dn;
+            at(baa[c], g) = c2 * vbp + c1 * dn;
+          }
+          at(bmb, g) = Mb[p];
+        }
+      }
+    }
+  };
+
+  issue(tlist[q0]);
+  int tp = -1;
+  int4 tgp = {}, grp = {};
+#pragma unroll 1
+  for (int qi = q0; qi < s1; qi += W) {
+    tnode = tid;
+    lel = tid / NP;
+    asm volatile("" : "+v"(tnode));
+    asm volatile("" : "+v"(lel));
+    __builtin_assume(tnode >= 0 && tnode < AP_NT);
+    __builtin_assume(lel >= 0 && lel <= (AP_NT - 1) / NP);
+    const int pe = tnode - lel * NP;
+    const int ii = pe % NG, jj = pe / NG;
+    const R w = (R)c_W2[pe];
+    const int t = tlist[qi];
+    const int tn = qi + W < s1 ? tlist[qi + W] : -1;
+    const int4 tg = ttiles[t];
+    const int4 gr = tgrids[t];
+    const Geo G = geo(tg);
+    const int PX = G.PX, NPT = G.NPT;
+    // this tile's fb, forward displacement and mass-sensitivity increment from the
+    // prefetched values ...
+    R f[NITER][NDOF], u[NITER][NDOF];
+    double mb[NITER];
+#pragma unroll
+    for (int r = 0; r < NITER; ++r) {
+      const int p = tnode + AP_NT * r;
+      if (p < NPT) {
+        double m = 0;
+#pragma unroll
+        for (int c = 0; c < NDOF; ++c) {
+          f[r][c] = (paa[r][c] + c3 * pav[r][c]) * prm[r][c];
+          u[r][c] = pdf[r][c];
+          double acc;
+          if (afw) {
+            acc = (double)px1[r][c];
+          } else {
+            const double dc = (double)u[r][c];
+            acc = (((double)px2[r][c] - dc) - (dc - (double)px1[r][c])) * idt2;
+          }
+          m -= (double)f[r][c] * acc;
+        }
+        mb[r] = m;
+      }
+    }
+    // ... then the previous tile's stores ...
+    if (tp >= 0) store_tile(tp, tgp, grp);
+    // ... then this tile's values into the slots just stored
+#pragma unroll
+    for (int r = 0; r < NITER; ++r) {
+      const int p = tnode + AP_NT * r;
+      if (p < NPT) {
+        const int Z = p / PX, X = p - Z * PX;
+        Uf[p] = lv_make<R, NDOF>(f[r]);
+        Ud[p] = lv_make<R, NDOF>(u[r]);
+        Fn[p] = LV{};
+        Ad[p] = lv_make<R, NDOF>(pad[r]);
+        Av[p] = lv_make<R, NDOF>(pav[r]);
+        Mb[p] = perim_slot(X, Z, PX, G.PZ) < 0 ? pmb[r] + mb[r] : 0.0;
+        sflag[p] = -1;
+      }
+    }
+    if (tid < G.nxe * G.nze * 3) Gm[tid / 3][tid - 3 * (tid / 3)] = pg;
+    if (tid < G.nxe * G.nze) {
+      Ge[tid][0] = pge0;
+      Ge[tid][1] = pge1;
+    }
+    if (tnode < ETH) {
+#pragma unroll
+      for (int q = 0; q < NB; ++q) ML[q][tnode] = lv_make<R, NDOF>(plm[q]);
+    }
+    __syncthreads();
+    if (psp >= 0) sflag[psp] = (short)tid;
+    psp = -1;
+    if (tn >= 0) issue(tn);
+
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int cc = q / NBC, b0 = (q % NBC) * AP_EPB;
+      const int cx = cc & 1, cz = cc >> 1;
+      const int ncx = (G.nxe - cx + 1) >> 1, ncz = (G.nze - cz + 1) >> 1;
+      const int nce = ncx * ncz;
+      if (b0 < nce) {  // uniform
+        const int ce = b0 + lel;
+        const bool active = (tnode < ETH) && (ce < nce);
+        int base = 0, el = 0;
+        if (active) {
+          const int lx = cx + 2 * (ce % ncx), lz = cz + 2 * (ce / ncx);
+          base = 4 * lz * PX + 4 * lx;
+          el = lz * G.nxe + lx;
+          const R g0 = Gm[el][0], g1 = Gm[el][1], g2 = Gm[el][2];
+          const LV mm = ML[q][tnode];
+          const R la = lv_get<R, NDOF>(mm, 0), mu = lv_get<R, NDOF>(mm, 1);
+          const R kx = la + 2 * mu;
+          const R A0 = w * kx * g0, A1 = w * la * g2, A2 = w * kx * g1;
+          const R A3 = w * mu * g1, A4 = w * mu * g2, A5 = w * mu * g0;
+          R pxi[NDOF] = {0, 0}, peta[NDOF] = {0, 0}, gxi[NDOF] = {0, 0}, geta[NDOF] = {0, 0};
+#pragma unroll
+          for (int k = 0; k < NG; ++k) {
+            const LV f1 = Uf[base + jj * PX + k], f2 = Uf[base + k * PX + ii];
+            const LV u1 = Ud[base + jj * PX + k], u2 = Ud[base + k * PX + ii];
+            const R ha = (R)sH[k * NG + ii], hb = (R)sH[k * NG + jj];
+#pragma unroll
+            for (int c = 0; c < NDOF; ++c) {
+              pxi[c] += ha * lv_get<R, NDOF>(f1, c);
+              peta[c] += lv_get<R, NDOF>(f2, c) * hb;
+              gxi[c] += ha * lv_get<R, NDOF>(u1, c);
+              geta[c] += lv_get<R, NDOF>(u2, c) * hb;
+            }
+          }
+          const R pxx = pxi[0], pex = peta[0], pxz = pxi[1], pez = peta[1];
+          const R gxx = gxi[0], gex = geta[0], gxz = gxi[1], gez = geta[1];
+          T1[lel][pe] = LV{A0 * pxx + A1 * pez, A3 * pex + A5 * pxz};
+          T2[lel][pe] = LV{A3 * pex + A4 * pxz, A1 * pxx + A2 * pez};
+          const double wd = (double)w;
+          Gr[lel][pe][0] =
+              wd * ((double)g0 * pxx * gxx + (double)g2 * (pxx * gez + pez * gxx) + (double)g1 * pez * gez);
+          Gr[lel][pe][1] = wd * (2.0 * g0 * pxx * gxx + 2.0 * g1 * pez * gez + (double)g1 * pex * (gex + gxz) +
+                                (double)g2 * pxz * gex + (double)g0 * pxz * gxz);
+        }
+        __syncthreads();
+        if (active) {
+          R s1[NDOF] = {0, 0}, s2[NDOF] = {0, 0}, fo[NDOF];
+#pragma unroll
+          for (int k = 0; k < NG; ++k) {
+            const LV a1 = T1[lel][jj * NG + k], a2 = T2[lel][k * NG + ii];
+            const R hc = (R)sH[ii * NG + k], hd = (R)sH[jj * NG + k];
+#pragma unroll
+            for (int c = 0; c < NDOF; ++c) {
+              s1[c] += hc * lv_get<R, NDOF>(a1, c);
+              s2[c] += lv_get<R, NDOF>(a2, c) * hd;
+            }
+          }
+          LV* fp = &Fn[base + jj * PX + ii];
+          const LV f0 = *fp;
+#pragma unroll
+          for (int c = 0; c < NDOF; ++c) fo[c] = lv_get<R, NDOF>(f0, c) + (s1[c] + s2[c]);
+          *fp = lv_make<R, NDOF>(fo);
+        }
+        if (active && pe < NG) {
+          double rl = 0, rm = 0;
+#pragma unroll
+          for (int q2 = 0; q2 < NG; ++q2) {
+            rl += Gr[lel][pe * NG + q2][0];
+            rm += Gr[lel][pe * NG + q2][1];
+          }
+          Gs[lel][pe][0] = rl;
+          Gs[lel][pe][1] = rm;
+        }
+        __syncthreads();
+        if (active && pe == 0) {
+          double sl = 0, sm = 0;
+#pragma unroll
+          for (int q2 = 0; q2 < NG; ++q2) {
+            sl += Gs[lel][q2][0];
+            sm += Gs[lel][q2][1];
+          }
+          Ge[el][0] += sl;
+          Ge[el][1] += sm;
+        }
+      }
+    }
+    __syncthreads();  // the last batch's element sums are in Ge
+    tp = t;
+    tgp = tg;
+    grp = gr;
+  }
+  if (tp >= 0) store_tile(tp, tgp, grp);
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_adj_deferred(int ndef, DefTable defs,
                                                       const int* __restrict__ part_off, const R* __restrict__ fpart,
