@@ -208,6 +208,24 @@ int sem2d_set_fault_sides(sem2d_solver *h, int on);
 int sem2d_set_fault_twf(sem2d_solver *h, int kind, const double *par, const double *coord);
 int sem2d_fetch_fault_sides(sem2d_solver *h, double *out, int *nsamp_taken);
 
+/* Normal stress response of the fault (BC_DYNFLT_NOR; normal_init / normal_update,
+   SRC/bc_dynflt_normal.f90:104-140): after sem2d_set_fault and before the first step.
+   kind 0: the friction sees the initial normal stress throughout; 1: Coulomb (the trial
+   normal traction, the default); 2: Prakash-Clifton with a time scale, sigma = Tn +
+   coef (sigma - Tn), coef = exp(-dt/T); 3: with a length scale, sigma = Tn +
+   exp(-(|V| + v) coef) (sigma - Tn), coef = dt/L, V the trial slip rate. */
+int sem2d_set_fault_normal(sem2d_solver *h, int kind, double coef, double v);
+
+/* Periodic boundaries (BC_DEF kind = 'PERIOD'; BC_PERIO_init / BC_PERIO_apply,
+   SRC/bc_periodic.f90:45-104): n node pairs (1-based), the master boundary's nodes and the
+   slave boundary's matching ones.  Each step, after the sources and before every other
+   boundary condition (SRC/bc_gen.f90:273-275), the force of each master node becomes the
+   sum of both and the slave node takes it.  The mass given to sem2d_create is the
+   assembled one (master and slave equal).  Call it right after sem2d_create, before
+   sem2d_set_kelvin_voigt and sem2d_set_25d: the problem then steps on the general element
+   path. */
+int sem2d_set_periodic(sem2d_solver *h, int n, const int *master, const int *slave);
+
 /* Fault potency rows [nrows][2*(ndof+1)], one per step from it = 0: P-SV p11, p22, p12
    of the displacement jump then of the velocity jump, SH p13, p23 of each -- the
    FltXX_potency_sem2d.tab lines of BC_DYNFLT_write (SRC/bc_dynflt.f90:757,832-855). */

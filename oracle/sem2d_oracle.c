@@ -21,7 +21,8 @@
  *     swf                swf_mu, swf_update_state  SRC/bc_dynflt_swf.f90:142-194
  *     rsf                rsf_solver, rsf_update_theta, rsf_update_V, nr_solver,
  *                        nr_fric_func_tau, rsf_mu  SRC/bc_dynflt_rsf.f90:165-568
- *     normal             normal_update kind 1  SRC/bc_dynflt_normal.f90:118-136
+ *     normal             normal_update kinds 0-3  SRC/bc_dynflt_normal.f90:122-140
+ *   periodic             BC_PERIO_apply       SRC/bc_periodic.f90:78-90, SRC/bc_gen.f90:273-275
  *   receivers            REC_store (AtNode)   SRC/receivers.f90:309-344
  *   fault records        BC_DYNFLT_write      SRC/bc_dynflt.f90:751-778
  *   potency              BC_DYNFLT_potency    SRC/bc_dynflt.f90:832-855
@@ -132,6 +133,9 @@ typedef struct {
   int twf;       /* time-weakening friction kind (0: none), law 3: time weakening alone */
   double tw[9];  /* MuS, MuD, Mu0, X, Z, V, L, T, Dc */
   double *xz;    /* [2][np] fault node coordinates (bc%coord) */
+  int nor_kind;      /* normal stress response: 0 cohesion, 1 Coulomb, 2-3 Prakash-Clifton */
+  double nor_coef;   /* kind 2: exp(-dt/T); kind 3: dt/L */
+  double nor_v;      /* kind 3: V */
   int osides;   /* per-side records (osides = T) */
   double *srec; /* [nsamp][4*ndof][nout]: d of side 1, side 2, v of side 1, side 2 */
   int pot_rows, pot_cap;
@@ -158,6 +162,8 @@ struct oracle_solver {
   int nrec, isamp, field, nsamp, taken;
   int *rnode;        /* [nrec], or [nrec][25] element nodes when interpolated */
   double *rw;        /* [nrec][25] interpolation weights, NULL for receivers at nodes */
+  int nper;          /* periodic node pairs (BC_PERIO): master, slave (0-based) */
+  int *per_m, *per_s;
   double *kv_eta;    /* [nelem][25] Kelvin-Voigt eta (dt-scaled), NULL if none */
   double *beta25;    /* [nelem][25] 2.5D coefficients (matwrk%beta), NULL if none */
   int *eorder;       /* assembly order of the elements */
@@ -610,6 +616,39 @@ int oracle_set_fault_twf(oracle_solver *h, int kind, const double *par, const do
 
 /* BC_DYNFLT osides = T: every fault record also keeps the displacement and velocity of
    each side in the fault frame (SRC/bc_dynflt.f90:769-812) */
+int oracle_set_fault_normal(oracle_solver *h, int kind, double coef, double v) {
+  if (!h->flt) return fail(h, 4, "set_fault_normal: call set_fault first");
+  if (h->it != 0) return fail(h, 4, "set_fault_normal: the solver has already been stepped");
+  if (kind < 0 || kind > 3) return fail(h, 3, "set_fault_normal: kind must be 0 to 3");
+  h->flt->nor_kind = kind;
+  h->flt->nor_coef = coef;
+  h->flt->nor_v = v;
+  return 0;
+}
+
+int oracle_set_periodic(oracle_solver *h, int n, const int *master, const int *slave) {
+  if (n <= 0 || !master || !slave) return fail(h, 3, "set_periodic: bad arguments");
+  if (h->nper) return fail(h, 3, "set_periodic: one periodic boundary per solver");
+  char *mark = (char *)calloc((size_t)h->npoin, 1);
+  for (int k = 0; k < n; ++k) {
+    int m = master[k] - 1, s = slave[k] - 1;
+    if (m < 0 || m >= h->npoin || s < 0 || s >= h->npoin || mark[m] || mark[s] || m == s) {
+      free(mark);
+      return fail(h, 3, "set_periodic: nodes out of range or repeated");
+    }
+    mark[m] = mark[s] = 1;
+  }
+  free(mark);
+  h->per_m = (int *)malloc((size_t)n * sizeof(int));
+  h->per_s = (int *)malloc((size_t)n * sizeof(int));
+  for (int k = 0; k < n; ++k) {
+    h->per_m[k] = master[k] - 1;
+    h->per_s[k] = slave[k] - 1;
+  }
+  h->nper = n;
+  return 0;
+}
+
 int oracle_set_fault_sides(oracle_solver *h, int on) {
   fault_t *f = h->flt;
   if (!f) return fail(h, 4, "set_fault_sides: no fault");
@@ -631,6 +670,7 @@ int oracle_fetch_fault_sides(oracle_solver *h, double *out, int *taken) {
 
 int oracle_set_fault(oracle_solver *h, const sem2d_fault_desc *d, int nsamp) {
   fault_t *f = (fault_t *)calloc(1, sizeof *f);
+  f->nor_kind = 1;
   int np = d->npoin, nd = h->ndof;
   f->np = np; f->ndof = nd;
   f->n1 = (int *)malloc(sizeof(int) * np);
@@ -887,7 +927,11 @@ static void fault_apply(oracle_solver *h, double *MxA, double time) {
     T[0] = T[0] + f->T0[k];
     T[1] = T[1] + f->T0[np + k];
     if (f->opening) T[1] = fmin(T[1], 0.0);
-    f->sigma[k] = T[1]; /* normal_update kind 1 */
+    /* normal_update (SRC/bc_dynflt_normal.f90:122-140), dV(:,1) the trial slip rate */
+    if (f->nor_kind == 1) f->sigma[k] = T[1];
+    else if (f->nor_kind == 2) f->sigma[k] = T[1] + f->nor_coef * (f->sigma[k] - T[1]);
+    else if (f->nor_kind == 3)
+      f->sigma[k] = T[1] + exp(-(fabs(dV[0]) + f->nor_v) * f->nor_coef) * (f->sigma[k] - T[1]);
     double sigma = f->sigma[k];
     Ts[0] = T[0]; Ts[1] = T[1];
     if (f->law == 2) {
@@ -991,6 +1035,12 @@ static void one_step(oracle_solver *h) {
           }
       }
     }
+  }
+  /* periodic boundaries first (SRC/bc_gen.f90:273-275): assemble in the master nodes,
+     then copy to the slave nodes (BC_PERIO_set_field, SRC/bc_periodic.f90:78-90) */
+  for (int c = 0; c < h->ndof; ++c) {
+    for (int k = 0; k < h->nper; ++k) f[c * N + h->per_m[k]] = f[c * N + h->per_m[k]] + f[c * N + h->per_s[k]];
+    for (int k = 0; k < h->nper; ++k) f[c * N + h->per_s[k]] = f[c * N + h->per_m[k]];
   }
   for (int b = 0; b < h->nabso; ++b) {
     abso_t *B = &h->abso[b];
@@ -1193,6 +1243,7 @@ void oracle_destroy(oracle_solver *h) {
     free(h->src[s].ampl);
   }
   free(h->rnode); free(h->rw); free(h->seis); free(h->kv_eta); free(h->eorder); free(h->floc);
+  free(h->per_m); free(h->per_s);
   free(h->info);
   free(h->beta25);
   for (int b = 0; b < h->ndir; ++b) free(h->dir_node[b]);

@@ -817,8 +817,10 @@ def _bound_topo(tag, mesh, num, perm_inv):
     return t
 
 
-def _normal_and_weights(t, jac, wgll):
-    """BC_get_normal_and_weights (SRC/spec_grid.f90:961-1011)."""
+def _normal_and_weights(t, jac, wgll, periodic=False):
+    """BC_get_normal_and_weights (SRC/spec_grid.f90:961-1011).  ``periodic``: the boundary
+    ends on a periodic boundary at both ends, whose two end nodes are then one node
+    (normal and weight summed over both, SRC/spec_grid.f90:998-1003)."""
     J11, J12, J21, J22 = jac
     ngll = len(wgll)
     npb = len(t.node)
@@ -841,6 +843,11 @@ def _normal_and_weights(t, jac, wgll):
             NORM[b, 0] += tz
             NORM[b, 1] += -tx
             W[b] += wgll[k] * jac1d
+    if periodic:
+        NORM[0] = NORM[0] + NORM[npb - 1]
+        NORM[npb - 1] = NORM[0]
+        W[0] = W[0] + W[npb - 1]
+        W[npb - 1] = W[0]
     for n in range(len(t.elem)):
         for b in (t.ibool[n, 0], t.ibool[n, ngll - 1]):
             nrm = math.sqrt(NORM[b, 0] ** 2 + NORM[b, 1] ** 2)
@@ -848,7 +855,42 @@ def _normal_and_weights(t, jac, wgll):
     return NORM, W
 
 
-def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
+def _perio_init(bd, mesh, num, perm_inv, M):
+    """BC_PERIO_init (SRC/bc_periodic.f90:45-73): master and slave boundaries (tags(1),
+    tags(2)) with matching node lists (equal counts, one coordinate shift); the mass of
+    each master node becomes the sum of both and the slave node takes it
+    (BC_PERIO_set_field, SRC/bc_periodic.f90:78-90)."""
+    tv = bd.get("tags", None)
+    if not isinstance(tv, list) or len(tv) < 2:
+        raise InputError("BC_DEF PERIOD needs tags = master, slave")
+    tm = _bound_topo(int(tv[0]), mesh, num, perm_inv)
+    ts = _bound_topo(int(tv[1]), mesh, num, perm_inv)
+    if len(tm.elem) != len(ts.elem):
+        raise InputError("bc_perio_init: number of boundary elements do not match")
+    if len(tm.node) != len(ts.node):
+        raise InputError("bc_perio_init: number of nodes on boundaries do not match")
+    coord = num["coord"]
+    cm, cs = coord[tm.node - 1], coord[ts.node - 1]
+    shift = cm[0] - cs[0]
+    if (np.abs(cm[:, 0] - cs[:, 0] - shift[0]) > TINY_XABS).any() or \
+            (np.abs(cm[:, 1] - cs[:, 1] - shift[1]) > TINY_XABS).any():
+        raise InputError("bc_perio_init: coordinates on boundaries do not match properly")
+    im, is_ = tm.node - 1, ts.node - 1
+    M[:, im] = M[:, im] + M[:, is_]
+    M[:, is_] = M[:, im]
+    return dict(master=tm.node.astype(np.int32), slave=ts.node.astype(np.int32))
+
+
+def _perio_intersects(t, perio):
+    """BC_PERIO_intersects (SRC/bc_periodic.f90:106-120): both end nodes of boundary t are
+    on the periodic boundaries."""
+    if perio is None:
+        return False
+    both = set(perio["master"].tolist()) | set(perio["slave"].tolist())
+    return int(t.node[0]) in both and int(t.node[-1]) in both
+
+
+def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey, periodic=False):
     """BC_ABSO_init (SRC/bc_abso.f90:115-266), Clayton-Engquist; on a boundary that is not
     flat (normals not all along x or z) the P-SV coefficients are rho cp w|J| along the
     normal and rho cs w|J| along the boundary, applied with the node normals, and the mass
@@ -856,7 +898,7 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
     (SRC/bc_abso.f90:176-180,209-224):
     K[e, k, geo_tan] = -K[e, k, geo_nor] = w_k |J1D| * DLocDGlob(tan, tan) * rho cs (2 cs - cp)."""
     J11, J12, J21, J22 = jac
-    NORM, _ = _normal_and_weights(t, jac, wgll)
+    NORM, _ = _normal_and_weights(t, jac, wgll, periodic)
     is_flat = True
     if np.all(np.abs(NORM[:, 0]) < TINY_XABS):
         geo_tan, geo_nor = 0, 1
@@ -903,6 +945,9 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey):
     if stacey:
         K[:, :, 1] = K[:, :, 0]
         K[:, :, geo_tan] = -K[:, :, geo_tan]
+    if periodic and is_flat:  # both end nodes are one periodic node (SRC/bc_abso.f90:231-235)
+        C[0] = C[0] + C[-1]
+        C[-1] = C[0]
     nodes0 = t.node - 1
     if is_flat:  # SRC/bc_abso.f90:243
         M[:, nodes0] = M[:, nodes0] + coefA2Vrhs * C.T
@@ -1029,7 +1074,8 @@ def _gen_cd(cd, coord):
     raise ValueError(kind)
 
 
-def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma, nt, ndof, scheme="newmark"):
+def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma, nt, ndof, scheme="newmark",
+                 perio=None):
     """BC_DYNFLT_read + BC_DYNFLT_init (SRC/bc_dynflt.f90:102-520)."""
     g = par.after(bcdef, "BC_DYNFLT")
     if g is None:
@@ -1097,10 +1143,20 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
         fp["kind"] = int(vals["kind"])
         for k in ("dc", "mus", "a", "b", "vstar", "theta", "vc"):
             fp[k + "_cd"] = _read_cd(par, fr_it, vals[k], fb.get(k + "h", "") if fb else "")
+    # normal stress response (normal_read / normal_init, SRC/bc_dynflt_normal.f90:40-118):
+    # 0 cohesion (sigma stays the initial normal stress), 1 Coulomb, 2 Prakash-Clifton
+    # with time scale T (coef = exp(-dt/T)), 3 with length scale L and velocity V
+    # (coef = dt/L)
     nor = par.after(g, "BC_DYNFLT_NOR")
     nkind = int(nor.get("kind", 1)) if nor is not None else 1
-    if nkind != 1:
-        raise InputError("BC_DYNFLT_NOR kind other than 1 (Coulomb) is not supported by this build")
+    nT = float(nor.get("t", 1.0)) if nor is not None else 1.0
+    nL = float(nor.get("l", 1.0)) if nor is not None else 1.0
+    nV = float(nor.get("v", 1.0)) if nor is not None else 1.0
+    if nkind > 3 or nkind < 0:
+        raise InputError("BC_SWFF_init: invalid kind in BC_DYNFLT_NOR input block")
+    if nL <= 0.0 or nV <= 0.0 or nT <= 0.0:
+        raise InputError("BC_SWFF_init: L, V and T must be positive in BC_DYNFLT_NOR input block")
+    nor_coef = math.exp(-dt / nT) if nkind == 2 else (dt / nL if nkind == 3 else 0.0)
 
     t1 = _bound_topo(tags[0], mesh, num, perm_inv)
     coord = num["coord"]
@@ -1117,7 +1173,7 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
     c1 = coord[node1 - 1]
     if two_sides and (np.abs(c1 - coord[node2 - 1]) > TINY_XABS).any():
         raise InputError("bc_dynflt_init: coordinates on boundaries do not match properly")
-    NORM, W = _normal_and_weights(t1, jac, wgll)
+    NORM, W = _normal_and_weights(t1, jac, wgll, _perio_intersects(t1, perio))
     n1 = NORM[keep]
     B = np.repeat(W[keep][:, None], ndof, axis=1)
     # TIME_getCoefA2V / TIME_getCoefA2D (SRC/time.f90:425-458): gamma dt and beta dt^2 for
@@ -1135,7 +1191,7 @@ def _dynflt_init(par, bcdef, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma,
     f = dict(tags=tags, node1=node1.astype(np.int32), node2=None if node2 is None else node2.astype(np.int32),
              coord=c1, n1=n1, B=B,
              invM1=invM1, invM2=invM2, Z=Z, coefA2V=coefA2V, coefA2D=coefA2D, law=law,
-             opening=bool(g.get("opening", True)))
+             opening=bool(g.get("opening", True)), nor_kind=nkind, nor_coef=nor_coef, nor_v=nV)
     f["twf"] = twf
     if law == "TWF":
         f["theta"] = np.zeros(npf)
@@ -1795,25 +1851,46 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         M[1] = M[0]
     pb.mass_raw = M.copy()
 
-    # BC_init in list order (periodic unsupported)
+    # BC_init (SRC/bc_gen.f90:190-248): periodic boundaries first, then the others in
+    # list order
     coefA2Vrhs = 1.0 * gamma * dt
     bcdefs = par.all("BC_DEF")
     present = set(_bound_elements(mesh).keys())
     pb.dirichlet = []
-    for bd in bcdefs:
-        kind = bd.get("kind", " ")
+    pb.periodic = None
+
+    def _tags_of(bd):
         tag = bd.get("tag", 0)
         tagsv = bd.get("tags", None)
-        t1 = int(tag) if int(tag) > 0 else (int(tagsv[0]) if isinstance(tagsv, list) else 0)
+        t1 = int(tag) if int(tag) > 0 else (int(tagsv[0]) if isinstance(tagsv, list) else
+                                              (int(tagsv) if tagsv is not None else 0))
+        t2 = int(tagsv[1]) if isinstance(tagsv, list) and len(tagsv) > 1 else 0
         if t1 <= 0:
             raise InputError("bc_read: tag(s) are null or not set")
-        if t1 not in present:
+        return t1, t2
+
+    for bd in bcdefs:
+        if bd.get("kind", " ") != "PERIOD":
+            continue
+        t1, t2 = _tags_of(bd)
+        if t1 not in present or (t2 and t2 not in present):
             continue  # IS_EMPTY
+        if pb.periodic is not None:
+            raise InputError("more than one PERIOD boundary is not supported by this build")
+        pb.periodic = _perio_init(bd, mesh, num, perm_inv, M)
+    for bd in bcdefs:
+        kind = bd.get("kind", " ")
+        t1, t2 = _tags_of(bd)
+        if t1 not in present or (t2 and t2 not in present):
+            continue  # IS_EMPTY
+        if kind == "PERIOD":
+            continue
         if kind == "ABSORB":
             ab = par.after(bd, "BC_ABSORB")
             stacey = bool(ab.get("stacey", False)) if ab is not None else False
             t = _bound_topo(t1, mesh, num, perm_inv)
-            pb.abso.append(_abso_init(t, jac, wgll, (rho_p, cp_p, cs_p), M, coefA2Vrhs, ndof, stacey))
+            pb.abso.append(_abso_init(t, jac, wgll, (rho_p, cp_p, cs_p), M, coefA2Vrhs, ndof, stacey,
+                                      _perio_intersects(t, pb.periodic)))
         elif kind == "DIRNEU":
             # BC_DIRNEU_read (SRC/bc_dirneu.f90:50-112): Neumann with no source time
             # function is a free boundary, BC_DIRNEU_apply adds nothing
@@ -1832,7 +1909,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
             if mask:
                 # BC_DIRNEU_init (SRC/bc_dirneu.f90:117-143): a flat, vertical or horizontal boundary
                 t = _bound_topo(t1, mesh, num, perm_inv)
-                nrm, _ = _normal_and_weights(t, jac, wgll)
+                nrm, _ = _normal_and_weights(t, jac, wgll, _perio_intersects(t, pb.periodic))
                 if not ((np.abs(nrm[:, 0]) < TINY_XABS).all() or (np.abs(nrm[:, 1]) < TINY_XABS).all()):
                     raise InputError("BC_DIRNEU_init: boundary is not vertical or horizontal")
                 # applied in list order among the non-absorbing conditions (SRC/bc_gen.f90:273-281)
@@ -1842,7 +1919,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
             if pb.fault is not None:
                 raise InputError("more than one DYNFLT boundary is not supported by this build")
             pb.fault = _dynflt_init(par, bd, mesh, num, perm_inv, jac, wgll, M, dt, beta, gamma, pb.nt, ndof,
-                                    scheme=pb.scheme)
+                                    scheme=pb.scheme, perio=pb.periodic)
         else:
             raise InputError(f"BC kind {kind!r} is not supported by this build")
 

@@ -47,7 +47,24 @@ struct FaultDev {
   int twf = 0;
   double tw[9] = {};
   const double* xz = nullptr;
+  // normal stress response (BC_DYNFLT_NOR): 0 cohesion (sigma stays the initial normal
+  // stress), 1 Coulomb, 2 Prakash-Clifton with time scale (nor_coef = exp(-dt/T)), 3 with
+  // length scale (nor_coef = dt/L, nor_v = V)
+  int nor_kind = 1;
+  double nor_coef = 0.0, nor_v = 0.0;
 };
+
+// normal_update (SRC/bc_dynflt_normal.f90:122-140): the normal stress the friction law
+// sees, from the previous one, the trial normal traction tn and the trial slip rate dv
+__host__ __device__ inline double normal_sigma(int kind, double coef, double vc, double sig_old, double tn,
+                                               double dv) {
+  switch (kind) {
+    case 0: return sig_old;
+    case 2: return tn + coef * (sig_old - tn);
+    case 3: return tn + exp(-(fabs(dv) + vc) * coef) * (sig_old - tn);
+    default: return tn;
+  }
+}
 
 // Corrector of one split-fault node after the fault solve (a = f * rmass, then v and d;
 // SRC/solver.f90:78-82), in the bulk precision and outside the no-contraction region
@@ -253,8 +270,8 @@ __global__ __launch_bounds__(64) void k_fault(FaultDev F, R* __restrict__ d, R* 
   T[0] = T[0] + F.T0[k];
   T[1] = T[1] + F.T0[np + k];
   if (F.opening) T[1] = fmin(T[1], 0.0);
-  F.sigma[k] = T[1];
-  const double sigma = T[1];
+  const double sigma = normal_sigma(F.nor_kind, F.nor_coef, F.nor_v, F.sigma[k], T[1], dV[0]);
+  F.sigma[k] = sigma;
   Ts[0] = T[0];
   Ts[1] = T[1];
   if (F.law == 2) {

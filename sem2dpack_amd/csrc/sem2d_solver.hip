@@ -162,6 +162,8 @@ struct sem2d_solver {
   std::vector<void*> allocs;
   AdjState* adj = nullptr;
   std::vector<int> elem_ibool;  // [nelem][25] 0-based reference nodes (interpolated receivers)
+  int nper = 0;                 // periodic node pairs (master, slave; 0-based)
+  int *per_m = nullptr, *per_s = nullptr;
   void* kv_eta = nullptr;       // Kelvin-Voigt viscosity [element in colour order][25] (bulk type)
   void* beta25 = nullptr;       // 2.5D coefficients [element in colour order][25] (bulk type)
   // progress rows (it, time, max |v|, max |d|) every itinfo steps, and the non-finite
@@ -1193,6 +1195,21 @@ StepCoefs<R> step_coefs(double dt, double beta, double gamma, int leapfrog) {
   return {(R)dt, (R)((0.5 - beta) * dt * dt), (R)((1.0 - gamma) * dt), (R)(gamma * dt), (R)(beta * dt * dt)};
 }
 
+// Periodic boundaries (BC_PERIO_apply, SRC/bc_periodic.f90:78-90): the force of master
+// node m[k] becomes f(m) + f(s), slave node s[k] takes it.  The pairs share no node.
+template <typename R>
+__global__ void k_periodic(int n, const int* __restrict__ m, const int* __restrict__ s, R* __restrict__ f,
+                           int npoin, int ndof) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  for (int c = 0; c < ndof; ++c) {
+    const size_t o = (size_t)c * npoin;
+    const R x = f[o + m[k]] + f[o + s[k]];
+    f[o + m[k]] = x;
+    f[o + s[k]] = x;
+  }
+}
+
 template <typename R>
 int do_steps(sem2d_solver* h, int nsteps) {
   int rc = sync_sources(h);
@@ -1226,6 +1243,10 @@ int do_steps(sem2d_solver* h, int nsteps) {
     if (!h->src_host.empty())
       k_sources<R><<<1, 64, 0, h->stream>>>((int)h->src_host.size(), h->src_dev, h->ampl_dev, h->ampl_nt, it, a,
                                            h->npoin);
+    // periodic boundaries first among the boundary conditions (SRC/bc_gen.f90:273-275)
+    if (h->nper)
+      k_periodic<R><<<grid_for(h->nper, 128), 128, 0, h->stream>>>(h->nper, h->per_m, h->per_s, a, h->npoin,
+                                                                  h->ndof);
     for (auto& b : h->abso) {
       if (b.nrm)
         k_abso_normal<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, b.node, b.C, b.nrm, v, a, h->npoin);
@@ -1636,6 +1657,33 @@ int sem2d_set_25d(sem2d_solver* h, const double* beta) {
   return use_general_path(h);
 }
 
+int sem2d_set_periodic(sem2d_solver* h, int n, const int* master, const int* slave) {
+  if (!h) return 1;
+  if (n <= 0 || !master || !slave) return set_err(h, 3, "set_periodic: bad arguments");
+  if (h->it != 0 || h->nrec || h->flt.on || h->nsrc || !h->abso.empty())
+    return set_err(h, 4, "set_periodic: call it right after sem2d_create");
+  if (h->st.eta || h->st.b25 || h->kv_eta || h->beta25)
+    return set_err(h, 4, "set_periodic: call it before sem2d_set_kelvin_voigt and sem2d_set_25d");
+  if (h->nper) return set_err(h, 3, "set_periodic: one periodic boundary per solver");
+  std::vector<char> mark(h->npoin, 0);
+  std::vector<int> m(n), sl(n);
+  for (int k = 0; k < n; ++k) {
+    m[k] = master[k] - 1;
+    sl[k] = slave[k] - 1;
+    if (m[k] < 0 || m[k] >= h->npoin || sl[k] < 0 || sl[k] >= h->npoin || m[k] == sl[k] || mark[m[k]] ||
+        mark[sl[k]])
+      return set_err(h, 3, "set_periodic: nodes out of range or repeated");
+    mark[m[k]] = mark[sl[k]] = 1;
+  }
+  // the pairs act on the reference numbering: the general element path
+  int rc = use_general_path(h);
+  if (rc) return rc;
+  if ((rc = upload(h, &h->per_m, m.data(), n))) return rc;
+  if ((rc = upload(h, &h->per_s, sl.data(), n))) return rc;
+  h->nper = n;
+  return 0;
+}
+
 int sem2d_set_receivers_interp(sem2d_solver* h, int nrec, const int* elem, const double* interp, int isamp,
                                int field, int nsamp) {
   if (!h) return 1;
@@ -1895,6 +1943,18 @@ int sem2d_set_fault_twf(sem2d_solver* h, int kind, const double* par, const doub
   if (F.taken)  // the friction row of the record of step 0, taken by sem2d_set_fault
     k_fault_record<<<grid_for(F.nout, 64), 64, 0, h->stream>>>(D, F.oix1, F.oixd, F.nout, F.rec);
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int sem2d_set_fault_normal(sem2d_solver* h, int kind, double coef, double v) {
+  if (!h) return 1;
+  FaultHost& F = h->flt;
+  if (!F.on) return set_err(h, 4, "set_fault_normal: call sem2d_set_fault first");
+  if (h->it != 0) return set_err(h, 4, "set_fault_normal: the solver has already been stepped");
+  if (kind < 0 || kind > 3) return set_err(h, 3, "set_fault_normal: kind must be 0 to 3");
+  F.dev.nor_kind = kind;
+  F.dev.nor_coef = coef;
+  F.dev.nor_v = v;
   return 0;
 }
 
@@ -2277,6 +2337,8 @@ int check_members(sem2d_batch* b) {
     if (h->flt.on != h0->flt.on) return berr(b, 3, "member %d: either every member has a fault or none", m);
     if (h->flt.srec) return berr(b, 3, "member %d: per-side fault records (osides) are not batched", m);
     if (h->flt.on && h->flt.dev.twf) return berr(b, 3, "member %d: time-weakening friction is not batched", m);
+    if (h->flt.on && h->flt.dev.nor_kind != 1)
+      return berr(b, 3, "member %d: normal stress laws other than Coulomb are not batched", m);
     for (const AbsoHost& B : h->abso_host)
       if (!B.senode.empty()) return berr(b, 3, "member %d: Stacey boundaries are not batched", m);
     if (h->st.eta || h->st.b25) return berr(b, 3, "member %d: Kelvin-Voigt and 2.5D terms are not batched", m);

@@ -39,7 +39,7 @@ module sem2d_hip
             sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
             sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_set_dirichlet, &
             sem2d_set_25d, sem2d_set_time_scheme, sem2d_set_fault_sides, sem2d_fetch_fault_sides, &
-            sem2d_set_fault_twf, &
+            sem2d_set_fault_twf, sem2d_set_fault_normal, sem2d_set_periodic, &
             sem2d_step, &
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
@@ -177,6 +177,24 @@ module sem2d_hip
       type(c_ptr), value :: h
       integer(c_int), value :: kind
       real(c_double), intent(in) :: par(9), coord(*)
+    end function
+
+    ! normal stress response (BC_DYNFLT_NOR): kind 0..3, coef = exp(-dt/T) (kind 2) or
+    ! dt/L (kind 3), v = V (kind 3); after sem2d_set_fault
+    integer(c_int) function sem2d_set_fault_normal(h, kind, coef, v) bind(C, name='sem2d_set_fault_normal')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: kind
+      real(c_double), value :: coef, v
+    end function
+
+    ! periodic boundaries (BC_DEF kind = 'PERIOD'): n node pairs (1-based), master and
+    ! slave; right after sem2d_create
+    integer(c_int) function sem2d_set_periodic(h, n, master, slave) bind(C, name='sem2d_set_periodic')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: n
+      integer(c_int), intent(in) :: master(*), slave(*)
     end function
 
     ! per-side fault records (osides): set after sem2d_set_fault; out(nout, 4*ndof, nsamp)

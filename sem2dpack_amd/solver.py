@@ -86,6 +86,11 @@ class Solver:
         pb = self.pb
         if getattr(pb, "scheme", "newmark") == "leapfrog":
             self._check(self.api.set_time_scheme(self.h, 1), "set_time_scheme")
+        per = getattr(pb, "periodic", None)
+        if per is not None:
+            pm = np.ascontiguousarray(per["master"], dtype=np.int32)
+            ps = np.ascontiguousarray(per["slave"], dtype=np.int32)
+            self._check(self.api.set_periodic(self.h, len(pm), _p(pm, np.int32), _p(ps, np.int32)), "set_periodic")
         if getattr(pb, "kv_eta", None) is not None:
             eta = np.ascontiguousarray(pb.kv_eta, dtype=np.float64)
             self._check(self.api.set_kelvin_voigt(self.h, _p(eta)), "set_kelvin_voigt")
@@ -192,6 +197,9 @@ class Solver:
             self._check(self.api.set_fault_twf(self.h, int(tw["kind"]), par, arr(f["coord"].T)), "set_fault_twf")
         if f.get("osides"):
             self._check(self.api.set_fault_sides(self.h, 1), "set_fault_sides")
+        if f.get("nor_kind", 1) != 1:
+            self._check(self.api.set_fault_normal(self.h, int(f["nor_kind"]), float(f["nor_coef"]),
+                                                  float(f["nor_v"])), "set_fault_normal")
 
     # ------------------------------------------------------------------
     def step(self, n):
