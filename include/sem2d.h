@@ -216,6 +216,16 @@ int sem2d_fetch_fault_sides(sem2d_solver *h, double *out, int *nsamp_taken);
    exp(-(|V| + v) coef) (sigma - Tn), coef = dt/L, V the trial slip rate. */
 int sem2d_set_fault_normal(sem2d_solver *h, int kind, double coef, double v);
 
+/* HHT-alpha time scheme (TIME kind = 'HHT-alpha'; solve_HHT_alpha, SRC/solver.f90:89-131):
+   the descriptor carries the scheme's beta and gamma (gamma = 3/2 - alpha, beta from rho,
+   SRC/time.f90:219-235).  Each step predicts d and v as Newmark does, forms the fields at
+   n + alpha, da = alpha d + (1 - alpha) d_old and va likewise, and computes the element
+   forces (Kelvin-Voigt with va) and the absorbing terms from them; sources and boundary
+   conditions see t_alpha = it dt + (alpha - 1) dt (the caller's amplitude tables are taken
+   at t_alpha).  Call it right after sem2d_create, before sem2d_set_kelvin_voigt and
+   sem2d_set_25d: the problem then steps on the general element path. */
+int sem2d_set_hht(sem2d_solver *h, double alpha);
+
 /* Periodic boundaries (BC_DEF kind = 'PERIOD'; BC_PERIO_init / BC_PERIO_apply,
    SRC/bc_periodic.f90:45-104): n node pairs (1-based), the master boundary's nodes and the
    slave boundary's matching ones.  Each step, after the sources and before every other

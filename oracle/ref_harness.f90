@@ -112,13 +112,12 @@ program ref_harness
   allocate(recnodes(n))
   do k = 1, nrec
     ! receivers inside elements (AtNode=F) sit on no node: dump node 1 for them (their
-    ! series are taken from the reference's own seismogram files instead)
+    ! series are taken from the reference's own seismogram files instead).  Of the nodes
+    ! at the receiver's coordinates (the two sides of a split fault) the last one, the
+    ! node SE_find_nearest_node keeps (d2 <= d2min)
     recnodes(k) = 1
     do i = 1, npoin
-      if (pb%grid%coord(1,i) == rcoord(1,k) .and. pb%grid%coord(2,i) == rcoord(2,k)) then
-        recnodes(k) = i
-        exit
-      endif
+      if (pb%grid%coord(1,i) == rcoord(1,k) .and. pb%grid%coord(2,i) == rcoord(2,k)) recnodes(k) = i
     enddo
   enddo
   n = nrec

@@ -9,7 +9,8 @@
  * sequential C that follows the reference's Fortran statement by statement and in
  * the same floating-point operation order (compile with -ffp-contract=off):
  *
- *   oracle_step          solve_Newmark, solve_leapfrog  SRC/solver.f90:42-84,140-160
+ *   oracle_step          solve_Newmark, solve_HHT_alpha, solve_leapfrog
+ *                        SRC/solver.f90:42-84,89-131,140-160
  *   fint()               compute_Fint         SRC/solver.f90:273-320 (element order)
  *     gather/scatter     FIELD_get_elem_sub / FIELD_add_elem_2  SRC/fields.f90:113-189
  *     kd()               ELAST_KD2_PSV/SH (NGLL 5), ELAST_KD1_PSV/SH (other NGLL),
@@ -151,6 +152,9 @@ struct oracle_solver {
   double *rmass;
   double dt, beta, gamma;
   int leapfrog;     /* 1: central-difference scheme (SRC/solver.f90:140-160) */
+  int hht;          /* 1: HHT-alpha (SRC/solver.f90:89-131) */
+  double alpha;
+  double *da, *va;  /* displ_alpha, veloc_alpha */
   double *d, *v, *a;
   int nabso;
   abso_t abso[16];
@@ -997,10 +1001,24 @@ static void one_step(oracle_solver *h) {
   double dt = h->dt, beta = h->beta, gam = h->gamma;
   double *d = h->d, *v = h->v, *a = h->a;
   double c1 = (0.5 - beta) * dt * dt, c2 = (1.0 - gam) * dt;
+  double *df = d, *vf = v; /* the fields the forces see (displ_alpha, veloc_alpha) */
   if (h->leapfrog) {
     /* solve_leapfrog (SRC/solver.f90:151): d = d + dt*v_mid */
 #pragma omp parallel for schedule(static) if (n >= 65536)
     for (size_t k = 0; k < n; ++k) d[k] = d[k] + dt * v[k];
+  } else if (h->hht) {
+    /* solve_HHT_alpha (SRC/solver.f90:108-113) */
+    double al = h->alpha;
+    df = h->da;
+    vf = h->va;
+    for (size_t k = 0; k < n; ++k) {
+      df[k] = d[k];
+      vf[k] = v[k];
+      d[k] = d[k] + dt * v[k] + c1 * a[k];
+      v[k] = v[k] + c2 * a[k];
+      df[k] = al * d[k] + (1.0 - al) * df[k];
+      vf[k] = al * v[k] + (1.0 - al) * vf[k];
+    }
   } else {
     /* solve_Newmark predictors (SRC/solver.f90:58-60) */
 #pragma omp parallel for schedule(static) if (n >= 65536)
@@ -1010,7 +1028,7 @@ static void one_step(oracle_solver *h) {
     }
   }
   double *f = a; /* f => a (SRC/solver.f90:52) */
-  fint(h, d, f, v);
+  fint(h, df, f, vf);
   int it = h->it + 1;
   for (int s = 0; s < h->nsrc; ++s) {
     source_t *S = &h->src[s];
@@ -1049,16 +1067,16 @@ static void one_step(oracle_solver *h) {
       for (int k = 0; k < B->n; ++k) {
         int g = B->node[k];
         double n1 = B->nrm[k], n2 = B->nrm[B->n + k];
-        double dot = v[g] * n1 + v[N + g] * n2;
+        double dot = vf[g] * n1 + vf[N + g] * n2;
         double vn1 = dot * n1, vn2 = dot * n2;
-        f[g] = f[g] - B->C[k] * vn1 - B->C[B->n + k] * (v[g] - vn1);
-        f[N + g] = f[N + g] - B->C[k] * vn2 - B->C[B->n + k] * (v[N + g] - vn2);
+        f[g] = f[g] - B->C[k] * vn1 - B->C[B->n + k] * (vf[g] - vn1);
+        f[N + g] = f[N + g] - B->C[k] * vn2 - B->C[B->n + k] * (vf[N + g] - vn2);
       }
     } else {
       for (int c = 0; c < h->ndof; ++c)
         for (int k = 0; k < B->n; ++k) {
           int g = B->node[k];
-          f[c * N + g] = f[c * N + g] - B->C[c * B->n + k] * v[c * N + g];
+          f[c * N + g] = f[c * N + g] - B->C[c * B->n + k] * vf[c * N + g];
         }
     }
     if (B->nbe) {
@@ -1071,7 +1089,7 @@ static void one_step(oracle_solver *h) {
         for (int c = 0; c < 2; ++c)
           for (int kk = 0; kk < h->ng; ++kk) {
             double s = 0.0;
-            for (int j = 0; j < h->ng; ++j) s = s + h->H[j][kk] * d[c * N + B->node[en[j]]];
+            for (int j = 0; j < h->ng; ++j) s = s + h->H[j][kk] * df[c * N + B->node[en[j]]];
             kxd[en[kk] * 2 + c] = kxd[en[kk] * 2 + c] + B->K[((size_t)e * h->ng + kk) * 2 + c] * s;
           }
       }
@@ -1086,7 +1104,11 @@ static void one_step(oracle_solver *h) {
   /* the other boundary conditions in their order (SRC/bc_gen.f90:273-281) */
   for (int b = 0; b < h->ndir; ++b)
     if (!h->dir_after_fault[b]) dirichlet_apply(h, b, f);
-  if (h->flt) fault_apply(h, f, (double)it * dt); /* time = it*dt (SRC/main.f90:53) */
+  /* time = it*dt (SRC/main.f90:53); HHT-alpha: t_alpha = time + (alpha-1)*dt
+     (SRC/solver.f90:114-119) */
+  double tbc = (double)it * dt;
+  if (h->hht) tbc = tbc + (h->alpha - 1.0) * dt;
+  if (h->flt) fault_apply(h, f, tbc);
   for (int b = 0; b < h->ndir; ++b)
     if (h->dir_after_fault[b]) dirichlet_apply(h, b, f);
   double c3 = gam * dt, c4 = beta * dt * dt;
@@ -1149,6 +1171,20 @@ int oracle_step(oracle_solver *h, int nsteps) {
 
 /* kind 0: Newmark (solve_Newmark, SRC/solver.f90:42-84), 1: leapfrog (solve_leapfrog,
    SRC/solver.f90:140-160); the dispatch of solve (SRC/solver.f90:20-37) */
+int oracle_set_hht(oracle_solver *h, double alpha) {
+  if (!(alpha >= 0.0 && alpha <= 1.0)) return fail(h, 3, "set_hht: alpha must lie in [0, 1]");
+  if (h->it != 0) return fail(h, 4, "set_hht: the solver has already been stepped");
+  if (h->leapfrog) return fail(h, 4, "set_hht: the solver is set to leapfrog");
+  size_t n = (size_t)h->ndof * h->npoin;
+  if (!h->da) {
+    h->da = (double *)calloc(n, sizeof(double));
+    h->va = (double *)calloc(n, sizeof(double));
+  }
+  h->hht = 1;
+  h->alpha = alpha;
+  return 0;
+}
+
 int oracle_set_time_scheme(oracle_solver *h, int kind) {
   if (kind != 0 && kind != 1) return fail(h, 3, "set_time_scheme: kind must be 0 (newmark) or 1 (leapfrog)");
   if (kind == 1 && (h->beta != 0.0 || h->gamma != 0.5))
@@ -1243,7 +1279,7 @@ void oracle_destroy(oracle_solver *h) {
     free(h->src[s].ampl);
   }
   free(h->rnode); free(h->rw); free(h->seis); free(h->kv_eta); free(h->eorder); free(h->floc);
-  free(h->per_m); free(h->per_s);
+  free(h->per_m); free(h->per_s); free(h->da); free(h->va);
   free(h->info);
   free(h->beta25);
   for (int b = 0; b < h->ndir; ++b) free(h->dir_node[b]);

@@ -1342,9 +1342,11 @@ def _f32(x):
     return float(np.float32(x))
 
 
-def _sources(par, num, jac, hprime, ndof, dt, nt, fname_dir=None):
+def _sources(par, num, jac, hprime, ndof, dt, nt, fname_dir=None, t_shift=None):
     """SO_read / SO_init (SRC/src_gen.f90:96-215): one source at ``coord``, or one per
-    line of ``file`` (x z [tdelay [ampli]]), all with the same STF and mechanism."""
+    line of ``file`` (x z [tdelay [ampli]]), all with the same STF and mechanism.
+    ``t_shift``: the amplitudes are taken at it*dt + t_shift (HHT-alpha: t_alpha =
+    time + (alpha-1) dt, SRC/solver.f90:114-115)."""
     g = par.first("SRC_DEF")
     if g is None:
         return []
@@ -1365,10 +1367,11 @@ def _sources(par, num, jac, hprime, ndof, dt, nt, fname_dir=None):
         if coord_in is None:
             raise InputError("SO_read: coord not set")
         rows = [(coord_in, 0.0, 1.0)]
-    return [_one_source(par, g, stf, mech, c, td, am, num, jac, hprime, ndof, dt, nt) for c, td, am in rows]
+    return [_one_source(par, g, stf, mech, c, td, am, num, jac, hprime, ndof, dt, nt, t_shift)
+            for c, td, am in rows]
 
 
-def _one_source(par, g, stf, mech, coord_in, tdelay, ampli, num, jac, hprime, ndof, dt, nt):
+def _one_source(par, g, stf, mech, coord_in, tdelay, ampli, num, jac, hprime, ndof, dt, nt, t_shift=None):
     if stf != "RICKER":
         raise InputError(f"stf {stf!r} is not supported by this build")
     # STF_RICKER reals are single precision (SRC/stf_ricker.f90:53-73)
@@ -1427,9 +1430,14 @@ def _one_source(par, g, stf, mech, coord_in, tdelay, ampli, num, jac, hprime, nd
         src["kind"] = "moment"
     else:
         raise InputError(f"mechanism {mech!r} is not supported by this build")
-    # amplitude table at t = it*dt, it = 1..nt (SRC/src_gen.f90:290-317)
-    src["stf_table"] = np.array([ricker(it * dt - src["tdelay"], f0, onset, amp) * src["ampli"]
-                                 for it in range(1, nt + 1)])
+    # amplitude table at t = it*dt, it = 1..nt (SRC/src_gen.f90:290-317), or at
+    # t_alpha = it*dt + (alpha-1) dt for HHT-alpha
+    if t_shift is None:
+        src["stf_table"] = np.array([ricker(it * dt - src["tdelay"], f0, onset, amp) * src["ampli"]
+                                     for it in range(1, nt + 1)])
+    else:
+        src["stf_table"] = np.array([ricker((it * dt + t_shift) - src["tdelay"], f0, onset, amp) * src["ampli"]
+                                     for it in range(1, nt + 1)])
     return src
 
 
@@ -1733,8 +1741,8 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     grid_cfl = float((cvel[:, :-1, :-1] / np.minimum(d1, d2)).max())
     tg = par.first("TIME")
     kind = str(tg.get("kind", "leapfrog")).strip().lower() if tg is not None else "leapfrog"
-    if kind not in ("newmark", "leapfrog"):
-        raise InputError("this build supports TIME kind='newmark' or 'leapfrog' only")
+    if kind not in ("newmark", "leapfrog", "hht-alpha"):
+        raise InputError("this build supports TIME kind='newmark', 'leapfrog' or 'HHT-alpha' only")
     nbsteps = int(tg.get("nbsteps", 0))
     dt_in = float(tg.get("dt", 0.0))
     courant = float(tg.get("courant", 0.5))
@@ -1748,6 +1756,22 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     tn = par.after(tg, "TIME_NEWMARK") if kind == "newmark" else None
     beta = float(tn.get("beta", 0.0)) if tn is not None else 0.0
     gamma = float(tn.get("gamma", 0.5)) if tn is not None else 0.5
+    alpha = 1.0
+    if kind == "hht-alpha":
+        # TIME_HHTA (SRC/time.f90:219-235): alpha and rho (defaults 0.5), gamma = 3/2 -
+        # alpha, beta from rho (0 when alpha = 1); second-order explicit HHT
+        th = par.after(tg, "TIME_HHTA")
+        alpha = float(th.get("alpha", 0.5)) if th is not None else 0.5
+        rho = float(th.get("rho", 0.5)) if th is not None else 0.5
+        if alpha < 0.0 or alpha > 1.0:
+            raise InputError("TIME_HHTA: alpha is out of range [0,1]")
+        if rho < 0.5 or rho > 1.0:
+            raise InputError("TIME_HHTA: rho is out of range [0.5,1]")
+        gamma = 1.5 - alpha
+        if alpha != 1.0:
+            beta = 1.0 - alpha - rho ** 2 * (rho - 1.0) / ((1.0 - alpha) * (1.0 + rho) ** 3)
+        else:
+            beta = 0.0
     if dt_in > 0:
         if total > 0:
             nbsteps = math.ceil(total / dt_in)
@@ -1757,13 +1781,18 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         dt = courant / grid_cfl
         if total > 0:
             nbsteps = math.ceil(total / dt)
-    # Omega_max: 2 for leapfrog, sqrt(2/gamma) for Newmark (SRC/time.f90:205-222)
-    omega_max = 2.0 if kind == "leapfrog" else math.sqrt(2.0 / gamma)
+    # Omega_max: 2 for leapfrog, sqrt(2/gamma) for Newmark, the HHT-alpha bound of rho
+    # (SRC/time.f90:205-235)
+    if kind == "hht-alpha":
+        omega_max = math.sqrt(-4.0 * (1.0 + rho) ** 5 / (rho ** 5 + 9.0 * rho ** 4 - 34.0 * rho ** 3 + 6.0 * rho ** 2
+                                                          - 15.0 * rho + 1.0))
+    else:
+        omega_max = 2.0 if kind == "leapfrog" else math.sqrt(2.0 / gamma)
     critical = omega_max * 3.0 / 7.0 / math.sqrt(2.0)
     if courant > critical:
         raise InputError("TIME_init: Courant number too high (IO_warning aborts)")
-    pb.dt, pb.beta, pb.gamma, pb.alpha = dt, beta, gamma, 1.0
-    pb.scheme = kind
+    pb.dt, pb.beta, pb.gamma, pb.alpha = dt, beta, gamma, alpha
+    pb.scheme = "hht" if kind == "hht-alpha" else kind
     # Kelvin-Voigt viscosity per element point (MAT_KV_init_elem_work,
     # SRC/mat_kelvin_voigt.f90:121-135): eta, times dt when ETAxDT
     pb.kv_eta = None
@@ -1853,7 +1882,9 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
 
     # BC_init (SRC/bc_gen.f90:190-248): periodic boundaries first, then the others in
     # list order
-    coefA2Vrhs = 1.0 * gamma * dt
+    # TIME_getCoefA2Vrhs (SRC/time.f90:467-486): alpha gamma dt (Newmark, HHT-alpha), dt/2
+    # (leapfrog, gamma = 1/2)
+    coefA2Vrhs = 0.5 * dt if pb.scheme == "leapfrog" else pb.alpha * (gamma * dt)
     bcdefs = par.all("BC_DEF")
     present = set(_bound_elements(mesh).keys())
     pb.dirichlet = []
@@ -1924,7 +1955,8 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
             raise InputError(f"BC kind {kind!r} is not supported by this build")
 
     pb.receivers = _receivers(par, pb.coord, pb.nt, num, xgll)
-    pb.sources = _sources(par, num, jac, H, ndof, dt, pb.nt, fname_dir)
+    pb.sources = _sources(par, num, jac, H, ndof, dt, pb.nt, fname_dir,
+                          t_shift=(pb.alpha - 1.0) * dt if pb.scheme == "hht" else None)
     pb.rmass = 1.0 / M
     pb.mass = M
     return pb

@@ -199,6 +199,24 @@ __global__ void k_predict(size_t n, R* __restrict__ d, R* __restrict__ v, R* __r
   }
 }
 
+// HHT-alpha predictor (solve_HHT_alpha, SRC/solver.f90:108-113): the Newmark predictor,
+// then the fields at n + alpha the forces see, da = alpha d' + (1 - alpha) d and
+// va = alpha v' + (1 - alpha) v (om = 1 - alpha)
+template <typename R>
+__global__ void k_predict_hht(size_t n, R* __restrict__ d, R* __restrict__ v, R* __restrict__ a,
+                              R* __restrict__ da, R* __restrict__ va, R dt, R c1, R c2, R al, R om) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const R ak = a[k], vk = v[k], dk = d[k];
+    const R dn = dk + dt * vk + c1 * ak;
+    const R vn = vk + c2 * ak;
+    d[k] = dn;
+    v[k] = vn;
+    a[k] = 0;
+    da[k] = al * dn + om * dk;
+    va[k] = al * vn + om * vk;
+  }
+}
+
 // a = f*rmass, corrector (SRC/solver.f90:78-82)
 template <typename R>
 __global__ void k_correct(size_t n, R* __restrict__ d, R* __restrict__ v, R* __restrict__ a,

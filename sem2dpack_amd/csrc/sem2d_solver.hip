@@ -119,6 +119,9 @@ struct sem2d_solver {
   size_t fstride = 0;  // component stride of the device field arrays
   double dt, beta, gamma;
   int leapfrog = 0;  // 1: central-difference scheme (SRC/solver.f90:140-160)
+  int hht = 0;       // 1: HHT-alpha (SRC/solver.f90:89-131): forces at d, v of n + alpha
+  double alpha = 1.0;
+  void *da = nullptr, *va = nullptr;  // the fields at n + alpha (bulk type)
   int ng = NG, np = NP;     // NGLL and GLL points per element
   double* hg = nullptr;     // h'_i(x_j) at hg[i*ng+j] (general element kernel, Stacey)
   hipStream_t stream = nullptr;
@@ -1238,8 +1241,18 @@ int do_steps(sem2d_solver* h, int nsteps) {
       record_info<R>(h, it);
       continue;
     }
-    k_predict<R><<<gb, 256, 0, h->stream>>>(n, d, v, a, dt, c1, c2);
-    launch_force<R>(h, d, a, h->kv_eta ? v : nullptr);
+    // HHT-alpha: the forces, absorbing terms and sources see the fields at n + alpha
+    R* df = d;
+    R* vf = v;
+    if (h->hht) {
+      df = (R*)h->da;
+      vf = (R*)h->va;
+      k_predict_hht<R><<<gb, 256, 0, h->stream>>>(n, d, v, a, df, vf, dt, c1, c2, (R)h->alpha,
+                                                  (R)(1.0 - h->alpha));
+    } else {
+      k_predict<R><<<gb, 256, 0, h->stream>>>(n, d, v, a, dt, c1, c2);
+    }
+    launch_force<R>(h, df, a, h->kv_eta ? vf : nullptr);
     if (!h->src_host.empty())
       k_sources<R><<<1, 64, 0, h->stream>>>((int)h->src_host.size(), h->src_dev, h->ampl_dev, h->ampl_nt, it, a,
                                            h->npoin);
@@ -1249,11 +1262,11 @@ int do_steps(sem2d_solver* h, int nsteps) {
                                                                   h->ndof);
     for (auto& b : h->abso) {
       if (b.nrm)
-        k_abso_normal<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, b.node, b.C, b.nrm, v, a, h->npoin);
+        k_abso_normal<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, b.node, b.C, b.nrm, vf, a, h->npoin);
       else
-        k_abso<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, h->ndof, b.node, b.C, v, a, h->npoin);
+        k_abso<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, h->ndof, b.node, b.C, vf, a, h->npoin);
       if (b.nbe)
-        k_stacey<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, b.node, b.soff, b.sidx, b.enode, b.K, d, a,
+        k_stacey<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, b.node, b.soff, b.sidx, b.enode, b.K, df, a,
                                                                 h->npoin, h->ng, h->hg);
     }
     // the other boundary conditions in their order (SRC/bc_gen.f90:273-281)
@@ -1261,9 +1274,12 @@ int do_steps(sem2d_solver* h, int nsteps) {
       if (!D.after_fault)
         k_dirichlet<R><<<grid_for(D.node.size(), 128), 128, 0, h->stream>>>((int)D.node.size(), D.node_dev, D.comp,
                                                                            h->ndof, a, h->npoin);
+    // the boundary conditions' time: it*dt, or t_alpha = it*dt + (alpha-1) dt for HHT-alpha
+    // (SRC/main.f90:53, SRC/solver.f90:114-119)
+    const double tbc = h->hht ? (double)it * h->dt + (h->alpha - 1.0) * h->dt : (double)it * h->dt;
     if (h->flt.on)
       k_fault<R><<<grid_for(h->flt.dev.np, 64), 64, 0, h->stream>>>(h->flt.dev, d, v, a, h->npoin, R(0), R(0),
-                                                                    (double)it * h->dt);
+                                                                    tbc);
     for (auto& D : h->dirs)
       if (D.after_fault)
         k_dirichlet<R><<<grid_for(D.node.size(), 128), 128, 0, h->stream>>>((int)D.node.size(), D.node_dev, D.comp,
@@ -1946,6 +1962,36 @@ int sem2d_set_fault_twf(sem2d_solver* h, int kind, const double* par, const doub
   return 0;
 }
 
+int sem2d_set_hht(sem2d_solver* h, double alpha) {
+  if (!h) return 1;
+  if (!(alpha >= 0.0 && alpha <= 1.0)) return set_err(h, 3, "set_hht: alpha must lie in [0, 1]");
+  if (h->it != 0 || h->nrec || h->flt.on || h->nsrc || !h->abso.empty())
+    return set_err(h, 4, "set_hht: call it right after sem2d_create");
+  if (h->st.eta || h->st.b25 || h->kv_eta || h->beta25)
+    return set_err(h, 4, "set_hht: call it before sem2d_set_kelvin_voigt and sem2d_set_25d");
+  if (h->leapfrog) return set_err(h, 4, "set_hht: the solver is set to leapfrog");
+  // the fields at n + alpha live in the reference numbering: the general element path
+  int rc = use_general_path(h);
+  if (rc) return rc;
+  if (!h->da) {
+    const size_t n = (size_t)h->ndof * h->npoin;
+    if (h->fp32) {
+      float *p1, *p2;
+      if ((rc = dev_alloc(h, &p1, n)) || (rc = dev_alloc(h, &p2, n))) return rc;
+      h->da = p1;
+      h->va = p2;
+    } else {
+      double *p1, *p2;
+      if ((rc = dev_alloc(h, &p1, n)) || (rc = dev_alloc(h, &p2, n))) return rc;
+      h->da = p1;
+      h->va = p2;
+    }
+  }
+  h->hht = 1;
+  h->alpha = alpha;
+  return 0;
+}
+
 int sem2d_set_fault_normal(sem2d_solver* h, int kind, double coef, double v) {
   if (!h) return 1;
   FaultHost& F = h->flt;
@@ -2096,6 +2142,7 @@ int sem2d_set_time_scheme(sem2d_solver* h, int kind) {
   if (kind == 1 && (h->beta != 0.0 || h->gamma != 0.5))
     return set_err(h, 3, "set_time_scheme: leapfrog needs beta = 0, gamma = 1/2 in the descriptor");
   if (h->it != 0) return set_err(h, 4, "set_time_scheme: the solver has already been stepped");
+  if (kind == 1 && h->hht) return set_err(h, 4, "set_time_scheme: the solver is set to HHT-alpha");
   h->leapfrog = kind;
   return 0;
 }

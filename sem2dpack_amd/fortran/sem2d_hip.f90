@@ -39,7 +39,7 @@ module sem2d_hip
             sem2d_set_force, sem2d_set_moment, sem2d_set_stf, &
             sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_set_dirichlet, &
             sem2d_set_25d, sem2d_set_time_scheme, sem2d_set_fault_sides, sem2d_fetch_fault_sides, &
-            sem2d_set_fault_twf, sem2d_set_fault_normal, sem2d_set_periodic, &
+            sem2d_set_fault_twf, sem2d_set_fault_normal, sem2d_set_periodic, sem2d_set_hht, &
             sem2d_step, &
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
@@ -195,6 +195,13 @@ module sem2d_hip
       type(c_ptr), value :: h
       integer(c_int), value :: n
       integer(c_int), intent(in) :: master(*), slave(*)
+    end function
+
+    ! HHT-alpha time scheme (TIME kind = 'HHT-alpha'): right after sem2d_create
+    integer(c_int) function sem2d_set_hht(h, alpha) bind(C, name='sem2d_set_hht')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: alpha
     end function
 
     ! per-side fault records (osides): set after sem2d_set_fault; out(nout, 4*ndof, nsamp)

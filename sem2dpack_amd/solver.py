@@ -86,6 +86,8 @@ class Solver:
         pb = self.pb
         if getattr(pb, "scheme", "newmark") == "leapfrog":
             self._check(self.api.set_time_scheme(self.h, 1), "set_time_scheme")
+        if getattr(pb, "scheme", "newmark") == "hht":
+            self._check(self.api.set_hht(self.h, float(pb.alpha)), "set_hht")
         per = getattr(pb, "periodic", None)
         if per is not None:
             pm = np.ascontiguousarray(per["master"], dtype=np.int32)

@@ -52,7 +52,8 @@ CASES = [("small_psv", 300), ("c1_psv", 2000), ("sh_swf", 600), ("multi_src", 30
          ("testsh", 1987), ("lamb", 3000), ("tpv3_anti", 1500), ("ng3_psv", 400), ("ng4_kv", 500),
          ("ng7_swf", 600), ("ng10_psv", 400), ("sides_psv", 500), ("sides_sh1", 500), ("domains", 500),
          ("twf_inplane", 600), ("twf_rsf", 500), ("twf_only", 600), ("perio_psv", 700), ("perio_sh", 800),
-         ("bimat", 156), ("nor3_rsf", 400), ("nor0_swf", 500)] + \
+         ("bimat", 156), ("nor3_rsf", 400), ("nor0_swf", 500), ("hht_flt1d", 455), ("hht_psv", 600),
+         ("hht_twf", 400)] + \
     [(n, 200 if n == "rsf_slip" else 500) for n in conftest.FRICTION_CASES]
 
 

@@ -78,7 +78,13 @@ def test_oracle_matches_reference(name, oracle_api):
                 slip = n1[:, 1] * dd[:, 0] - n1[:, 0] * dd[:, 1]
             else:
                 slip = dd[:, 0]
-            assert np.array_equal(rec[::st_every, 0], slip)
+            if pb.beta == 0.0:
+                assert np.array_equal(rec[::st_every, 0], slip)
+            else:
+                # beta /= 0 (HHT-alpha): the fault's slip is dD + beta dt^2 dA, equal to the
+                # jump of the corrected fields up to rounding (the float32 records below
+                # are compared exactly)
+                assert np.abs(rec[::st_every, 0] - slip).max() <= 1e-13 * np.abs(slip).max()
         if "flt_f32" in g:
             ref32 = g["flt_f32"]
             with np.errstate(over="ignore"):  # huge(1d0) friction (time weakening) -> +Inf, as real()
