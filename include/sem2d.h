@@ -216,6 +216,33 @@ int sem2d_fetch_fault_sides(sem2d_solver *h, double *out, int *nsamp_taken);
    exp(-(|V| + v) coef) (sigma - Tn), coef = dt/L, V the trial slip rate. */
 int sem2d_set_fault_normal(sem2d_solver *h, int kind, double coef, double v);
 
+/* Incident plane wave (SRC_DEF mechanism = 'WAVE', &SRC_WAVE) through the absorbing
+   boundary registered last by sem2d_set_abso (BC_ABSO_apply with an incident wave,
+   SRC/bc_abso.f90:300-306; SO_WAVE_get_VT, SRC/src_gen.f90:322-339).  xz [2][n] the
+   boundary nodes' coordinates, vdir and tdir [ndof][n] the incident velocity and traction
+   for unit amplitude (SRC_WAVE_get_VT, SRC/src_wave.f90:134-171, with the node normals),
+   B [n] the boundary weights, par = s1, s2, x0, z0 (slowness and phase origin), tdelay,
+   and the Ricker f0, onset and ampli.  Each step the force at node k becomes
+   f - C (v - a Vdir) + B a Tdir, a = ricker(t - (x - x0) s1 - (z - z0) s2 - tdelay), t the
+   boundary conditions' time.  The problem steps on the general element path. */
+int sem2d_set_abso_wave(sem2d_solver *h, int n, const double *xz, const double *vdir, const double *tdir,
+                        const double *B, const double *par);
+
+/* The absorbing boundary registered last ends on a periodic boundary at both ends
+   (bc%periodic, SRC/bc_abso.f90:142): after sem2d_set_abso_stacey, its two end nodes
+   each take the Stacey terms of both (SRC/bc_abso.f90:325-328).  General element path. */
+int sem2d_set_abso_periodic(sem2d_solver *h);
+
+/* Linear slip fault (BC_DEF kind = 'LISFLT'; BC_LSF_init / BC_LSF_apply,
+   SRC/bc_lsf.f90:101-186): n node pairs (1-based) of the two sides, the side-1 normals
+   [2][n], B, W1 = 1/(B (1 + M1/M2)), W2 = 1/(B (M2/M1 + 1)) [ndof][n], the tangential and
+   normal stiffness kt, kn (DBL_MAX: welded).  After the absorbing boundaries, each step
+   T = W2 f2 - W1 f1 and dD = d2 - d1 in the fault frame, T_t = kt dD_t, T_n = kn dD_n,
+   f1 += B T, f2 -= B T.  Not combined with sem2d_set_fault or Dirichlet boundaries; the
+   problem steps on the general element path. */
+int sem2d_set_lsf(sem2d_solver *h, int n, const int *node1, const int *node2, const double *nrm,
+                  const double *B, const double *W1, const double *W2, double kt, double kn);
+
 /* HHT-alpha time scheme (TIME kind = 'HHT-alpha'; solve_HHT_alpha, SRC/solver.f90:89-131):
    the descriptor carries the scheme's beta and gamma (gamma = 3/2 - alpha, beta from rho,
    SRC/time.f90:219-235).  Each step predicts d and v as Newmark does, forms the fields at

@@ -102,6 +102,9 @@ typedef struct {
   double *C;    /* [ndof][n] */
   double *nrm;  /* [2][n] node normals of a boundary that is not flat, NULL if flat */
   int nbe;      /* Stacey: boundary elements (0: none) */
+  int perends;  /* the boundary ends on periodic boundaries: Stacey end terms summed */
+  double *wxz, *wV, *wT, *wB; /* incident plane wave: [2][n], [ndof][n], [ndof][n], [n] */
+  double wpar[8];             /* s1, s2, x0, z0, tdelay, f0, t0, ampli */
   int *enode;   /* [nbe][ngll] boundary-local node of each edge point */
   double *K;    /* [nbe][ngll][2] */
 } abso_t;
@@ -166,6 +169,9 @@ struct oracle_solver {
   int nrec, isamp, field, nsamp, taken;
   int *rnode;        /* [nrec], or [nrec][25] element nodes when interpolated */
   double *rw;        /* [nrec][25] interpolation weights, NULL for receivers at nodes */
+  int nlsf;          /* linear slip fault (BC_LSF): node pairs (0-based) and arrays */
+  int *lsf1, *lsf2;
+  double *lsf_nrm, *lsf_B, *lsf_W1, *lsf_W2, lsf_kt, lsf_kn;
   int nper;          /* periodic node pairs (BC_PERIO): master, slave (0-based) */
   int *per_m, *per_s;
   double *kv_eta;    /* [nelem][25] Kelvin-Voigt eta (dt-scaled), NULL if none */
@@ -1060,9 +1066,26 @@ static void one_step(oracle_solver *h) {
     for (int k = 0; k < h->nper; ++k) f[c * N + h->per_m[k]] = f[c * N + h->per_m[k]] + f[c * N + h->per_s[k]];
     for (int k = 0; k < h->nper; ++k) f[c * N + h->per_s[k]] = f[c * N + h->per_m[k]];
   }
+  double tbc = (double)it * dt; /* time = it*dt (SRC/main.f90:53) */
+  if (h->hht) tbc = tbc + (h->alpha - 1.0) * dt; /* t_alpha (SRC/solver.f90:114-119) */
   for (int b = 0; b < h->nabso; ++b) {
     abso_t *B = &h->abso[b];
-    if (B->nrm) {
+    if (B->wxz) {
+      /* incident wave (SRC/bc_abso.f90:300-306, SRC/src_gen.f90:322-339): MxA - C (V - Vin) + B Tin */
+      const double *w = B->wpar;
+      for (int k = 0; k < B->n; ++k) {
+        double phase = tbc - (B->wxz[k] - w[2]) * w[0] - (B->wxz[B->n + k] - w[3]) * w[1];
+        phase = phase - w[4];
+        double arg = 3.141592653589793 * w[5] * (phase - w[6]);
+        arg = arg * arg;
+        double amp = -w[7] * (1.0 - 2.0 * arg) * exp(-arg);
+        for (int c = 0; c < h->ndof; ++c) {
+          int g = B->node[k];
+          double vin = amp * B->wV[c * B->n + k], tin = amp * B->wT[c * B->n + k];
+          f[c * N + g] = f[c * N + g] - B->C[c * B->n + k] * (vf[c * N + g] - vin) + B->wB[k] * tin;
+        }
+      }
+    } else if (B->nrm) {
       /* not flat (SRC/bc_abso.f90:313-318): Vn = (V.n) n, MxA -= C1 Vn + C2 (V - Vn) */
       for (int k = 0; k < B->n; ++k) {
         int g = B->node[k];
@@ -1093,6 +1116,12 @@ static void one_step(oracle_solver *h) {
             kxd[en[kk] * 2 + c] = kxd[en[kk] * 2 + c] + B->K[((size_t)e * h->ng + kk) * 2 + c] * s;
           }
       }
+      if (B->perends) { /* SRC/bc_abso.f90:325-328 */
+        for (int c = 0; c < 2; ++c) {
+          kxd[c] = kxd[c] + kxd[(B->n - 1) * 2 + c];
+          kxd[(B->n - 1) * 2 + c] = kxd[c];
+        }
+      }
       for (int c = 0; c < 2; ++c)
         for (int k = 0; k < B->n; ++k) {
           int g = B->node[k];
@@ -1104,10 +1133,37 @@ static void one_step(oracle_solver *h) {
   /* the other boundary conditions in their order (SRC/bc_gen.f90:273-281) */
   for (int b = 0; b < h->ndir; ++b)
     if (!h->dir_after_fault[b]) dirichlet_apply(h, b, f);
-  /* time = it*dt (SRC/main.f90:53); HHT-alpha: t_alpha = time + (alpha-1)*dt
-     (SRC/solver.f90:114-119) */
-  double tbc = (double)it * dt;
-  if (h->hht) tbc = tbc + (h->alpha - 1.0) * dt;
+  if (h->nlsf) {
+    /* BC_LSF_apply (SRC/bc_lsf.f90:161-186), D the displacement the forces see */
+    int n = h->nlsf, nd = h->ndof;
+    double *T = (double *)calloc((size_t)2 * n, sizeof(double));
+    for (int k = 0; k < n; ++k) {
+      int i1 = h->lsf1[k], i2 = h->lsf2[k];
+      double t[2] = {0, 0}, dd[2] = {0, 0};
+      for (int c = 0; c < nd; ++c) {
+        t[c] = h->lsf_W2[c * n + k] * f[c * N + i2] - h->lsf_W1[c * n + k] * f[c * N + i1];
+        dd[c] = df[c * N + i2] - df[c * N + i1];
+      }
+      if (nd == 2) {
+        double nx = h->lsf_nrm[k], nz = h->lsf_nrm[n + k];
+        double t0 = nz * t[0] - nx * t[1], t1 = nx * t[0] + nz * t[1];
+        double d0 = nz * dd[0] - nx * dd[1], d1 = nx * dd[0] + nz * dd[1];
+        if (h->lsf_kt < DBL_MAX) t0 = h->lsf_kt * d0;
+        if (h->lsf_kn < DBL_MAX) t1 = h->lsf_kn * d1;
+        t[0] = nz * t0 + nx * t1;
+        t[1] = -nx * t0 + nz * t1;
+      } else if (h->lsf_kt < DBL_MAX) {
+        t[0] = h->lsf_kt * dd[0];
+      }
+      T[2 * k] = t[0];
+      T[2 * k + 1] = t[1];
+    }
+    for (int c = 0; c < nd; ++c) {
+      for (int k = 0; k < n; ++k) f[c * N + h->lsf1[k]] = f[c * N + h->lsf1[k]] + h->lsf_B[c * n + k] * T[2 * k + c];
+      for (int k = 0; k < n; ++k) f[c * N + h->lsf2[k]] = f[c * N + h->lsf2[k]] - h->lsf_B[c * n + k] * T[2 * k + c];
+    }
+    free(T);
+  }
   if (h->flt) fault_apply(h, f, tbc);
   for (int b = 0; b < h->ndir; ++b)
     if (h->dir_after_fault[b]) dirichlet_apply(h, b, f);
@@ -1171,6 +1227,48 @@ int oracle_step(oracle_solver *h, int nsteps) {
 
 /* kind 0: Newmark (solve_Newmark, SRC/solver.f90:42-84), 1: leapfrog (solve_leapfrog,
    SRC/solver.f90:140-160); the dispatch of solve (SRC/solver.f90:20-37) */
+int oracle_set_abso_wave(oracle_solver *h, int n, const double *xz, const double *vdir, const double *tdir,
+                         const double *Bw, const double *par) {
+  if (h->nabso < 1) return fail(h, 3, "set_abso_wave: no absorbing boundary");
+  abso_t *b = &h->abso[h->nabso - 1];
+  if (n != b->n || !xz || !vdir || !tdir || !Bw || !par) return fail(h, 3, "set_abso_wave: bad arguments");
+  b->wxz = dup_d(xz, (size_t)2 * n);
+  b->wV = dup_d(vdir, (size_t)h->ndof * n);
+  b->wT = dup_d(tdir, (size_t)h->ndof * n);
+  b->wB = dup_d(Bw, (size_t)n);
+  for (int q = 0; q < 8; ++q) b->wpar[q] = par[q];
+  return 0;
+}
+
+int oracle_set_abso_periodic(oracle_solver *h) {
+  if (h->nabso < 1 || !h->abso[h->nabso - 1].nbe) return fail(h, 3, "set_abso_periodic: no Stacey boundary");
+  h->abso[h->nabso - 1].perends = 1;
+  return 0;
+}
+
+int oracle_set_lsf(oracle_solver *h, int n, const int *node1, const int *node2, const double *nrm,
+                   const double *Bv, const double *W1, const double *W2, double kt, double kn) {
+  if (n <= 0 || !node1 || !node2 || !nrm || !Bv || !W1 || !W2) return fail(h, 3, "set_lsf: bad arguments");
+  if (h->nlsf) return fail(h, 3, "set_lsf: one linear slip fault per solver");
+  h->lsf1 = (int *)malloc((size_t)n * sizeof(int));
+  h->lsf2 = (int *)malloc((size_t)n * sizeof(int));
+  for (int k = 0; k < n; ++k) {
+    if (node1[k] < 1 || node1[k] > h->npoin || node2[k] < 1 || node2[k] > h->npoin)
+      return fail(h, 3, "set_lsf: node out of range");
+    h->lsf1[k] = node1[k] - 1;
+    h->lsf2[k] = node2[k] - 1;
+  }
+  size_t nd = (size_t)h->ndof * n;
+  h->lsf_nrm = dup_d(nrm, (size_t)2 * n);
+  h->lsf_B = dup_d(Bv, nd);
+  h->lsf_W1 = dup_d(W1, nd);
+  h->lsf_W2 = dup_d(W2, nd);
+  h->lsf_kt = kt;
+  h->lsf_kn = kn;
+  h->nlsf = n;
+  return 0;
+}
+
 int oracle_set_hht(oracle_solver *h, double alpha) {
   if (!(alpha >= 0.0 && alpha <= 1.0)) return fail(h, 3, "set_hht: alpha must lie in [0, 1]");
   if (h->it != 0) return fail(h, 4, "set_hht: the solver has already been stepped");
@@ -1280,6 +1378,10 @@ void oracle_destroy(oracle_solver *h) {
   }
   free(h->rnode); free(h->rw); free(h->seis); free(h->kv_eta); free(h->eorder); free(h->floc);
   free(h->per_m); free(h->per_s); free(h->da); free(h->va);
+  free(h->lsf1); free(h->lsf2); free(h->lsf_nrm); free(h->lsf_B); free(h->lsf_W1); free(h->lsf_W2);
+  for (int b = 0; b < h->nabso; ++b) {
+    free(h->abso[b].wxz); free(h->abso[b].wV); free(h->abso[b].wT); free(h->abso[b].wB);
+  }
   free(h->info);
   free(h->beta25);
   for (int b = 0; b < h->ndir; ++b) free(h->dir_node[b]);

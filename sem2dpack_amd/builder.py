@@ -890,7 +890,7 @@ def _perio_intersects(t, perio):
     return int(t.node[0]) in both and int(t.node[-1]) in both
 
 
-def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey, periodic=False):
+def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey, periodic=False, let_wave=True):
     """BC_ABSO_init (SRC/bc_abso.f90:115-266), Clayton-Engquist; on a boundary that is not
     flat (normals not all along x or z) the P-SV coefficients are rho cp w|J| along the
     normal and rho cs w|J| along the boundary, applied with the node normals, and the mass
@@ -898,7 +898,7 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey, periodic=False)
     (SRC/bc_abso.f90:176-180,209-224):
     K[e, k, geo_tan] = -K[e, k, geo_nor] = w_k |J1D| * DLocDGlob(tan, tan) * rho cs (2 cs - cp)."""
     J11, J12, J21, J22 = jac
-    NORM, _ = _normal_and_weights(t, jac, wgll, periodic)
+    NORM, WB = _normal_and_weights(t, jac, wgll, periodic)
     is_flat = True
     if np.all(np.abs(NORM[:, 0]) < TINY_XABS):
         geo_tan, geo_nor = 0, 1
@@ -956,7 +956,10 @@ def _abso_init(t, jac, wgll, props, M, coefA2Vrhs, ndof, stacey, periodic=False)
     return dict(tag=t.tag, node=t.node.astype(np.int32), C=C, geo_nor=geo_nor,
                 contrib=(np.array(cb, dtype=np.int64), np.array(ce, dtype=np.int64), np.array(cf)),
                 stacey=(np.ascontiguousarray(t.ibool[:, :ngll], dtype=np.int32), K) if stacey else None,
-                normals=None if (is_flat or ndof == 1) else NORM)
+                normals=None if (is_flat or ndof == 1) else NORM,
+                # incident waves (SRC/bc_abso.f90:247-264): flat boundaries with let_wave
+                let_wave=bool(let_wave and is_flat), wave_n=NORM, wave_B=WB,
+                periodic_ends=bool(periodic and stacey))
 
 
 # ---------------------------------------------------------------------------
@@ -1387,6 +1390,9 @@ def _one_source(par, g, stf, mech, coord_in, tdelay, ampli, num, jac, hprime, nd
         raise InputError("RICKER_read: Onset time too small (IO_warning aborts)")
     coord = num["coord"]
     ibool = num["ibool"]
+    if mech == "WAVE":
+        return dict(kind="wave", f0=f0, t0=onset, stf_ampli=amp, tdelay=float(tdelay), mechanism=mech,
+                    wave=_wave_read(par, g))
     iglob, _ = _nearest_node(coord, coord_in)
     src = dict(iglob=iglob, f0=f0, t0=onset, stf_ampli=amp, tdelay=float(tdelay), ampli=float(ampli),
                mechanism=mech)
@@ -1439,6 +1445,96 @@ def _one_source(par, g, stf, mech, coord_in, tdelay, ampli, num, jac, hprime, nd
         src["stf_table"] = np.array([ricker((it * dt + t_shift) - src["tdelay"], f0, onset, amp) * src["ampli"]
                                      for it in range(1, nt + 1)])
     return src
+
+
+def _wave_read(par, g):
+    """WAVE_read (SRC/src_wave.f90:33-70): incidence angle (folded into [-180, 180]) and phase."""
+    wg = par.after(g, "SRC_WAVE")
+    if wg is None:
+        raise InputError("WAVE_read: SRC_WAVE input block not found")
+    angle = float(wg.get("angle", 0.0))
+    angle = ((angle + 180.0) % 360.0) - 180.0  # modulo(angle+180, 360) - 180
+    phase = str(wg.get("phase", "S")).strip().upper()[:1]
+    if phase not in ("P", "S"):
+        raise InputError("WAVE_read: phase must be S or P")
+    return dict(angle=angle, phase=phase)
+
+
+def _wave_init(w, num, props):
+    """WAVE_init (SRC/src_wave.f90:75-120).  The reference point wave%coord is never set
+    there: the reference's phase origin and the node whose material gives the wave speed
+    are those of its allocated (zeroed) value, (0, 0); the material of the first element
+    holding that node (SE_node_belongs_to).  k = (-sin a, cos a), polarisation p = k (P)
+    or (cos a, sin a) (S), slowness s = k / c."""
+    rho_p, cp_p, cs_p, lam_p, mu_p = props
+    ip, _ = _nearest_node(num["coord"], (0.0, 0.0))
+    e, j, i = (int(v) for v in np.argwhere(num["ibool"] == ip)[0])
+    cp, cs, mu, lam = cp_p[e, j, i], cs_p[e, j, i], mu_p[e, j, i], lam_p[e, j, i]
+    a = PI * w["angle"] / 180.0
+    k = np.array([-math.sin(a), math.cos(a)])
+    if w["phase"] == "P":
+        p, c = np.array([-math.sin(a), math.cos(a)]), cp
+    else:
+        p, c = np.array([math.cos(a), math.sin(a)]), cs
+    w.update(k=k, p=p, c=c, s=k / c, cp=cp, cs=cs, mu=mu, lam=lam, x0=0.0, z0=0.0)
+    return w
+
+
+def _wave_dirs(w, N, ndof):
+    """SRC_WAVE_get_VT (SRC/src_wave.f90:134-171) per boundary node, before the source time
+    function: Vin and Tin for unit amplitude, N the node normals [n][2]."""
+    K, P = w["k"], w["p"]
+    KN = K[0] * N[:, 0] + K[1] * N[:, 1]
+    n = N.shape[0]
+    V = np.empty((n, ndof))
+    T = np.empty((n, ndof))
+    if ndof == 1:
+        V[:, 0] = 1.0
+        T[:, 0] = -w["mu"] / w["cs"] * KN
+    else:
+        V[:, 0], V[:, 1] = P[0], P[1]
+        if w["phase"] == "P":
+            T[:, 0] = -1.0 / w["cp"] * (w["lam"] * N[:, 0] + 2.0 * w["mu"] * KN * K[0])
+            T[:, 1] = -1.0 / w["cp"] * (w["lam"] * N[:, 1] + 2.0 * w["mu"] * KN * K[1])
+        else:
+            PN = P[0] * N[:, 0] + P[1] * N[:, 1]
+            T[:, 0] = -w["mu"] / w["cs"] * (PN * K[0] + KN * P[0])
+            T[:, 1] = -w["mu"] / w["cs"] * (PN * K[1] + KN * P[1])
+    return V, T
+
+
+def _lsf_init(par, bd, mesh, num, perm_inv, jac, wgll, M, ndof, perio):
+    """BC_LSF_read / BC_LSF_init (SRC/bc_lsf.f90:48-156): linear slip fault between the
+    node lists of tags(1) and tags(2); stiffness from the compliance when given, huge(1d0)
+    (welded) when neither is set (the reference leaves kt/kn unset then).  B the assembled
+    weights of side 1, W1 = 1/(B (1 + M1/M2)), W2 = 1/(B (M2/M1 + 1))."""
+    g = par.after(bd, "BC_LSF")
+    if g is None:
+        raise InputError("BC_LSF_read: BC_LSF input block not found")
+    inf = sys.float_info.max
+    ctang, cnorm = float(g.get("ctang", 0.0)), float(g.get("cnorm", 0.0))
+    ktang, knorm = float(g.get("ktang", inf)), float(g.get("knorm", inf))
+    if ctang != 0.0:
+        ktang = 1.0 / ctang
+    if cnorm != 0.0:
+        knorm = 1.0 / cnorm
+    tv = bd.get("tags", None)
+    t1 = _bound_topo(int(tv[0]), mesh, num, perm_inv)
+    t2 = _bound_topo(int(tv[1]), mesh, num, perm_inv)
+    if len(t1.elem) != len(t2.elem):
+        raise InputError("bc_lsf_init: number of boundary elements do not match")
+    if len(t1.node) != len(t2.node):
+        raise InputError("bc_lsf_init: number of nodes on boundaries do not match")
+    coord = num["coord"]
+    if (np.abs(coord[t1.node - 1] - coord[t2.node - 1]) > TINY_XABS).any():
+        raise InputError("bc_lsf_init: coordinates on boundaries do not match properly")
+    NORM, W = _normal_and_weights(t1, jac, wgll, _perio_intersects(t1, perio))
+    B = np.repeat(W[:, None], ndof, axis=1)
+    M1, M2 = M[:, t1.node - 1].T, M[:, t2.node - 1].T
+    W1 = 1.0 / (B * (1.0 + M1 / M2))
+    W2 = 1.0 / (B * (M2 / M1 + 1.0))
+    return dict(node1=t1.node.astype(np.int32), node2=t2.node.astype(np.int32), n=NORM, B=B, W1=W1, W2=W2,
+                kt=ktang, kn=knorm)
 
 
 def _moment_stencil(iglob, Mt, ibool, jac, hprime, ndof):
@@ -1889,6 +1985,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     present = set(_bound_elements(mesh).keys())
     pb.dirichlet = []
     pb.periodic = None
+    pb.lsf = None
 
     def _tags_of(bd):
         tag = bd.get("tag", 0)
@@ -1919,9 +2016,16 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         if kind == "ABSORB":
             ab = par.after(bd, "BC_ABSORB")
             stacey = bool(ab.get("stacey", False)) if ab is not None else False
+            let_wave = bool(ab.get("let_wave", True)) if ab is not None else True
             t = _bound_topo(t1, mesh, num, perm_inv)
-            pb.abso.append(_abso_init(t, jac, wgll, (rho_p, cp_p, cs_p), M, coefA2Vrhs, ndof, stacey,
-                                      _perio_intersects(t, pb.periodic)))
+            ad = _abso_init(t, jac, wgll, (rho_p, cp_p, cs_p), M, coefA2Vrhs, ndof, stacey,
+                            _perio_intersects(t, pb.periodic), let_wave)
+            ad["coord"] = num["coord"][t.node - 1]
+            pb.abso.append(ad)
+        elif kind == "LISFLT":
+            if pb.lsf is not None:
+                raise InputError("more than one LISFLT boundary is not supported by this build")
+            pb.lsf = _lsf_init(par, bd, mesh, num, perm_inv, jac, wgll, M, ndof, pb.periodic)
         elif kind == "DIRNEU":
             # BC_DIRNEU_read (SRC/bc_dirneu.f90:50-112): Neumann with no source time
             # function is a free boundary, BC_DIRNEU_apply adds nothing
@@ -1955,8 +2059,23 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
             raise InputError(f"BC kind {kind!r} is not supported by this build")
 
     pb.receivers = _receivers(par, pb.coord, pb.nt, num, xgll)
-    pb.sources = _sources(par, num, jac, H, ndof, dt, pb.nt, fname_dir,
-                          t_shift=(pb.alpha - 1.0) * dt if pb.scheme == "hht" else None)
+    srcs = _sources(par, num, jac, H, ndof, dt, pb.nt, fname_dir,
+                    t_shift=(pb.alpha - 1.0) * dt if pb.scheme == "hht" else None)
+    # incident plane waves (mechanism 'WAVE') enter through the absorbing boundaries
+    # (SO_add skips them; BC_ABSO_init takes the first wave source, SRC/bc_abso.f90:249-258)
+    waves = [x for x in srcs if x["kind"] == "wave"]
+    pb.sources = [x for x in srcs if x["kind"] != "wave"]
+    pb.wave = None
+    if waves:
+        w = _wave_init(dict(waves[0]["wave"]), num, (rho_p, cp_p, cs_p, lam_p, mu_p))
+        w.update(f0=waves[0]["f0"], t0=waves[0]["t0"], stf_ampli=waves[0]["stf_ampli"],
+                 tdelay=waves[0]["tdelay"])
+        pb.wave = w
+        for ad in pb.abso:
+            if ad["let_wave"]:
+                ad["wave_V"], ad["wave_T"] = _wave_dirs(w, ad["wave_n"], ndof)
+    if pb.lsf is not None and (pb.fault is not None or pb.dirichlet):
+        raise InputError("LISFLT with DYNFLT or Dirichlet boundaries is not supported by this build")
     pb.rmass = 1.0 / M
     pb.mass = M
     return pb

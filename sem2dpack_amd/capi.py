@@ -62,6 +62,9 @@ _SIGS = {
     "set_fault_normal": ([C.c_void_p, C.c_int, C.c_double, C.c_double], C.c_int),
     "set_periodic": ([C.c_void_p, C.c_int, ip, ip], C.c_int),
     "set_hht": ([C.c_void_p, C.c_double], C.c_int),
+    "set_abso_wave": ([C.c_void_p, C.c_int, dp, dp, dp, dp, dp], C.c_int),
+    "set_abso_periodic": ([C.c_void_p], C.c_int),
+    "set_lsf": ([C.c_void_p, C.c_int, ip, ip, dp, dp, dp, dp, C.c_double, C.c_double], C.c_int),
     "fetch_fault_sides": ([C.c_void_p, dp, ip], C.c_int),
     "set_itinfo": ([C.c_void_p, C.c_int], C.c_int),
     "fetch_info": ([C.c_void_p, dp, ip], C.c_int),

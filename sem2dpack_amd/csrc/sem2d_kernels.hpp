@@ -263,6 +263,72 @@ __global__ void k_abso(int n, int ndof, const int* __restrict__ node, const doub
   }
 }
 
+// Incident plane wave through a flat absorbing boundary (BC_ABSO_apply with bc%wav,
+// SRC/bc_abso.f90:300-306; SO_WAVE_get_VT, SRC/src_gen.f90:322-339): at node k the phase
+// t - (x - x0) s1 - (z - z0) s2 - tdelay, the Ricker amplitude there (ricker,
+// SRC/stf_ricker.f90:89-101), Vin = amp V, Tin = amp T, and the force
+// f - C (v - Vin) + B Tin.  par = s1, s2, x0, z0, tdelay, f0, t0, ampli.
+struct WavePar {
+  double s1, s2, x0, z0, tdelay, f0, t0, ampli;
+};
+template <typename R>
+__global__ void k_abso_wave(int n, int ndof, const int* __restrict__ node, const double* __restrict__ C,
+                            const double* __restrict__ xz, const double* __restrict__ V,
+                            const double* __restrict__ T, const double* __restrict__ B, WavePar W,
+                            double time, const R* __restrict__ v, R* __restrict__ f, int npoin) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double phase = time - (xz[k] - W.x0) * W.s1 - (xz[n + k] - W.z0) * W.s2 - W.tdelay;
+  double arg = 3.141592653589793 * W.f0 * (phase - W.t0);
+  arg = arg * arg;
+  const double amp = -W.ampli * (1.0 - 2.0 * arg) * exp(-arg);
+  const int g = node[k];
+  for (int c = 0; c < ndof; ++c) {
+    R* fp = f + (size_t)c * npoin + g;
+    const double vin = amp * V[(size_t)c * n + k], tin = amp * T[(size_t)c * n + k];
+    *fp = (R)(((double)*fp - C[(size_t)c * n + k] * ((double)v[(size_t)c * npoin + g] - vin)) + B[k] * tin);
+  }
+}
+
+// Linear slip fault (BC_LSF_apply, SRC/bc_lsf.f90:161-186): per node pair, the traction
+// T = W2 f2 - W1 f1 and the jump dD = d2 - d1 (d the displacement the forces see), in the
+// fault frame for P-SV (t, n from the side-1 normal), the components with finite
+// stiffness replaced by k dD, back to (x, z); f1 += B T, f2 -= B T.  kt, kn = DBL_MAX:
+// welded (the component is left as computed).
+template <typename R>
+__global__ void k_lsf(int n, int ndof, const int* __restrict__ n1, const int* __restrict__ n2,
+                      const double* __restrict__ nrm, const double* __restrict__ B, const double* __restrict__ W1,
+                      const double* __restrict__ W2, double kt, double kn, const R* __restrict__ d,
+                      R* __restrict__ f, int npoin) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i1 = n1[k], i2 = n2[k];
+  double T[2] = {0, 0}, dD[2] = {0, 0};
+  for (int c = 0; c < ndof; ++c) {
+    const size_t o = (size_t)c * npoin;
+    T[c] = W2[(size_t)c * n + k] * (double)f[o + i2] - W1[(size_t)c * n + k] * (double)f[o + i1];
+    dD[c] = (double)d[o + i2] - (double)d[o + i1];
+  }
+  const double big = 1.7976931348623157e308;
+  if (ndof == 2) {
+    const double nx = nrm[k], nz = nrm[n + k];
+    double t0 = nz * T[0] - nx * T[1], t1 = nx * T[0] + nz * T[1];
+    const double d0 = nz * dD[0] - nx * dD[1], d1 = nx * dD[0] + nz * dD[1];
+    if (kt < big) t0 = kt * d0;
+    if (kn < big) t1 = kn * d1;
+    T[0] = nz * t0 + nx * t1;
+    T[1] = -nx * t0 + nz * t1;
+  } else if (kt < big) {
+    T[0] = kt * dD[0];
+  }
+  for (int c = 0; c < ndof; ++c) {
+    const size_t o = (size_t)c * npoin;
+    const double bt = B[(size_t)c * n + k] * T[c];
+    f[o + i1] = (R)((double)f[o + i1] + bt);
+    f[o + i2] = (R)((double)f[o + i2] - bt);
+  }
+}
+
 // Absorbing boundary that is not flat, P-SV (SRC/bc_abso.f90:313-318): with the node normal
 // n, Vn = (v.n) n and the force loses C0 Vn + C1 (v - Vn), C = [rho cp, rho cs] w|J|.
 template <typename R>

@@ -44,6 +44,10 @@ struct AbsoDev {
   int *enode = nullptr, *soff = nullptr, *sidx = nullptr;
   double* K = nullptr;
   double* nrm = nullptr;  // [2][n] node normals of a boundary that is not flat (general path)
+  // incident plane wave (general path): node coordinates [2][n], unit-amplitude Vin and
+  // Tin [ndof][n], weights B [n]
+  double *wxz = nullptr, *wV = nullptr, *wT = nullptr, *wB = nullptr;
+  WavePar wp = {};
 };
 
 struct AbsoHost {
@@ -165,6 +169,11 @@ struct sem2d_solver {
   std::vector<void*> allocs;
   AdjState* adj = nullptr;
   std::vector<int> elem_ibool;  // [nelem][25] 0-based reference nodes (interpolated receivers)
+  // linear slip fault (BC_LSF): node pairs, side-1 normals [2][n], B, W1, W2 [ndof][n]
+  int nlsf = 0;
+  int *lsf_n1 = nullptr, *lsf_n2 = nullptr;
+  double *lsf_nrm = nullptr, *lsf_B = nullptr, *lsf_W1 = nullptr, *lsf_W2 = nullptr;
+  double lsf_kt = 0, lsf_kn = 0;
   int nper = 0;                 // periodic node pairs (master, slave; 0-based)
   int *per_m = nullptr, *per_s = nullptr;
   void* kv_eta = nullptr;       // Kelvin-Voigt viscosity [element in colour order][25] (bulk type)
@@ -1260,8 +1269,14 @@ int do_steps(sem2d_solver* h, int nsteps) {
     if (h->nper)
       k_periodic<R><<<grid_for(h->nper, 128), 128, 0, h->stream>>>(h->nper, h->per_m, h->per_s, a, h->npoin,
                                                                   h->ndof);
+    // the boundary conditions' time: it*dt, or t_alpha = it*dt + (alpha-1) dt for HHT-alpha
+    // (SRC/main.f90:53, SRC/solver.f90:114-119)
+    const double tbc = h->hht ? (double)it * h->dt + (h->alpha - 1.0) * h->dt : (double)it * h->dt;
     for (auto& b : h->abso) {
-      if (b.nrm)
+      if (b.wxz)
+        k_abso_wave<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, h->ndof, b.node, b.C, b.wxz, b.wV, b.wT,
+                                                                  b.wB, b.wp, tbc, vf, a, h->npoin);
+      else if (b.nrm)
         k_abso_normal<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, b.node, b.C, b.nrm, vf, a, h->npoin);
       else
         k_abso<R><<<grid_for(b.n, 128), 128, 0, h->stream>>>(b.n, h->ndof, b.node, b.C, vf, a, h->npoin);
@@ -1274,9 +1289,10 @@ int do_steps(sem2d_solver* h, int nsteps) {
       if (!D.after_fault)
         k_dirichlet<R><<<grid_for(D.node.size(), 128), 128, 0, h->stream>>>((int)D.node.size(), D.node_dev, D.comp,
                                                                            h->ndof, a, h->npoin);
-    // the boundary conditions' time: it*dt, or t_alpha = it*dt + (alpha-1) dt for HHT-alpha
-    // (SRC/main.f90:53, SRC/solver.f90:114-119)
-    const double tbc = h->hht ? (double)it * h->dt + (h->alpha - 1.0) * h->dt : (double)it * h->dt;
+    if (h->nlsf)
+      k_lsf<R><<<grid_for(h->nlsf, 128), 128, 0, h->stream>>>(h->nlsf, h->ndof, h->lsf_n1, h->lsf_n2, h->lsf_nrm,
+                                                             h->lsf_B, h->lsf_W1, h->lsf_W2, h->lsf_kt, h->lsf_kn,
+                                                             df, a, h->npoin);
     if (h->flt.on)
       k_fault<R><<<grid_for(h->flt.dev.np, 64), 64, 0, h->stream>>>(h->flt.dev, d, v, a, h->npoin, R(0), R(0),
                                                                     tbc);
@@ -1528,6 +1544,37 @@ int sem2d_set_abso_stacey(sem2d_solver* h, int nbe, const int* enode, const doub
   return 0;
 }
 
+static int use_general_path(sem2d_solver* h);
+
+int sem2d_set_abso_periodic(sem2d_solver* h) {
+  if (!h) return 1;
+  if (h->abso.empty()) return set_err(h, 3, "set_abso_periodic: call sem2d_set_abso first");
+  AbsoHost& ah = h->abso_host.back();
+  AbsoDev& ad = h->abso.back();
+  if (!ad.nbe) return set_err(h, 3, "set_abso_periodic: the boundary has no Stacey term");
+  if (h->st.eta || h->st.b25)
+    return set_err(h, 4, "set_abso_periodic: not with Kelvin-Voigt or 2.5D terms on a structured grid");
+  int rc = use_general_path(h);
+  if (rc) return rc;
+  // both end nodes take the Stacey contributions of the first end node, then the last's
+  // (SRC/bc_abso.f90:325-328)
+  const int n = (int)ah.node.size(), NG = h->ng, nbe = ad.nbe;
+  std::vector<std::vector<int>> lists(n);
+  for (int k = 0; k < nbe * NG; ++k) lists[ah.senode[k]].push_back(k);  // element order
+  std::vector<int> ends = lists[0];
+  ends.insert(ends.end(), lists[n - 1].begin(), lists[n - 1].end());
+  lists[0] = ends;
+  lists[n - 1] = ends;
+  std::vector<int> off(n + 1, 0), idx;
+  for (int q = 0; q < n; ++q) {
+    idx.insert(idx.end(), lists[q].begin(), lists[q].end());
+    off[q + 1] = (int)idx.size();
+  }
+  if ((rc = upload(h, &ad.soff, off.data(), off.size()))) return rc;
+  if ((rc = upload(h, &ad.sidx, idx.data(), idx.size()))) return rc;
+  return 0;
+}
+
 int sem2d_set_force(sem2d_solver* h, int iglob, const double* dir, int* isrc) {
   if (!h) return 1;
   if (iglob < 1 || iglob > h->npoin) return set_err(h, 3, "set_force: node out of range");
@@ -1734,6 +1781,7 @@ int sem2d_set_fault(sem2d_solver* h, const sem2d_fault_desc* f, int nsamp) {
   if (!h) return 1;
   if (!f || f->npoin <= 0 || nsamp < 1) return set_err(h, 3, "set_fault: bad arguments");
   if (h->flt.on) return set_err(h, 3, "set_fault: only one fault per solver");
+  if (h->nlsf) return set_err(h, 4, "set_fault: not combined with a linear slip fault");
   if (f->law < 1 || f->law > 3)
     return set_err(h, 3, "set_fault: law must be 1 (SWF), 2 (RSF) or 3 (time weakening, sem2d_set_fault_twf)");
   if ((f->law == 1 && (f->kind < 1 || f->kind > 3)) || (f->law == 2 && (f->kind < 1 || f->kind > 4)))
@@ -1962,6 +2010,55 @@ int sem2d_set_fault_twf(sem2d_solver* h, int kind, const double* par, const doub
   return 0;
 }
 
+int sem2d_set_abso_wave(sem2d_solver* h, int n, const double* xz, const double* vdir, const double* tdir,
+                        const double* B, const double* par) {
+  if (!h) return 1;
+  if (h->abso.empty()) return set_err(h, 3, "set_abso_wave: call sem2d_set_abso first");
+  AbsoDev& ad = h->abso.back();
+  if (n != ad.n || !xz || !vdir || !tdir || !B || !par) return set_err(h, 3, "set_abso_wave: bad arguments");
+  if (h->it != 0) return set_err(h, 4, "set_abso_wave: the solver has already been stepped");
+  if (h->st.eta || h->st.b25)
+    return set_err(h, 4, "set_abso_wave: not with Kelvin-Voigt or 2.5D terms on a structured grid");
+  int rc = use_general_path(h);  // the incident field acts on the reference numbering
+  if (rc) return rc;
+  if ((rc = upload(h, &ad.wxz, xz, (size_t)2 * n))) return rc;
+  if ((rc = upload(h, &ad.wV, vdir, (size_t)h->ndof * n))) return rc;
+  if ((rc = upload(h, &ad.wT, tdir, (size_t)h->ndof * n))) return rc;
+  if ((rc = upload(h, &ad.wB, B, (size_t)n))) return rc;
+  ad.wp = WavePar{par[0], par[1], par[2], par[3], par[4], par[5], par[6], par[7]};
+  return 0;
+}
+
+int sem2d_set_lsf(sem2d_solver* h, int n, const int* node1, const int* node2, const double* nrm, const double* B,
+                  const double* W1, const double* W2, double kt, double kn) {
+  if (!h) return 1;
+  if (n <= 0 || !node1 || !node2 || !nrm || !B || !W1 || !W2) return set_err(h, 3, "set_lsf: bad arguments");
+  if (h->nlsf) return set_err(h, 3, "set_lsf: one linear slip fault per solver");
+  if (h->it != 0) return set_err(h, 4, "set_lsf: the solver has already been stepped");
+  if (h->flt.on || !h->dirs.empty())
+    return set_err(h, 4, "set_lsf: not combined with a dynamic fault or Dirichlet boundaries");
+  if (h->st.eta || h->st.b25)
+    return set_err(h, 4, "set_lsf: not with Kelvin-Voigt or 2.5D terms on a structured grid");
+  std::vector<int> a(n), b(n);
+  for (int k = 0; k < n; ++k) {
+    a[k] = node1[k] - 1;
+    b[k] = node2[k] - 1;
+    if (a[k] < 0 || a[k] >= h->npoin || b[k] < 0 || b[k] >= h->npoin)
+      return set_err(h, 3, "set_lsf: node out of range");
+  }
+  int rc = use_general_path(h);
+  if (rc) return rc;
+  const size_t nd = (size_t)h->ndof * n;
+  if ((rc = upload(h, &h->lsf_n1, a.data(), n)) || (rc = upload(h, &h->lsf_n2, b.data(), n)) ||
+      (rc = upload(h, &h->lsf_nrm, nrm, (size_t)2 * n)) || (rc = upload(h, &h->lsf_B, B, nd)) ||
+      (rc = upload(h, &h->lsf_W1, W1, nd)) || (rc = upload(h, &h->lsf_W2, W2, nd)))
+    return rc;
+  h->lsf_kt = kt;
+  h->lsf_kn = kn;
+  h->nlsf = n;
+  return 0;
+}
+
 int sem2d_set_hht(sem2d_solver* h, double alpha) {
   if (!h) return 1;
   if (!(alpha >= 0.0 && alpha <= 1.0)) return set_err(h, 3, "set_hht: alpha must lie in [0, 1]");
@@ -2118,6 +2215,7 @@ int sem2d_set_dirichlet(sem2d_solver* h, int n, const int* node, int comp) {
   if (n <= 0 || !node) return set_err(h, 3, "set_dirichlet: empty boundary");
   if (comp < 1 || comp > 3 || (h->ndof == 1 && comp != 1))
     return set_err(h, 3, "set_dirichlet: comp must be 1 (component 1), 2 (component 2) or 3 (both; P-SV)");
+  if (h->nlsf) return set_err(h, 4, "set_dirichlet: not combined with a linear slip fault");
   sem2d_solver::DirBC D;
   D.node.resize(n);
   for (int k = 0; k < n; ++k) {

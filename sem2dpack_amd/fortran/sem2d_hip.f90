@@ -40,6 +40,7 @@ module sem2d_hip
             sem2d_set_receivers, sem2d_set_receivers_interp, sem2d_set_fault, sem2d_set_dirichlet, &
             sem2d_set_25d, sem2d_set_time_scheme, sem2d_set_fault_sides, sem2d_fetch_fault_sides, &
             sem2d_set_fault_twf, sem2d_set_fault_normal, sem2d_set_periodic, sem2d_set_hht, &
+            sem2d_set_abso_wave, sem2d_set_lsf, sem2d_set_abso_periodic, &
             sem2d_step, &
             sem2d_fetch_fields, &
             sem2d_set_fields, sem2d_fetch_seismo, sem2d_fetch_fault, sem2d_fetch_fault_state, &
@@ -195,6 +196,31 @@ module sem2d_hip
       type(c_ptr), value :: h
       integer(c_int), value :: n
       integer(c_int), intent(in) :: master(*), slave(*)
+    end function
+
+    ! incident plane wave (mechanism = 'WAVE') through the absorbing boundary set last:
+    ! xz(n, 2), vdir(n, ndof), tdir(n, ndof), B(n), par = s1, s2, x0, z0, tdelay, f0, t0, ampli
+    integer(c_int) function sem2d_set_abso_wave(h, n, xz, vdir, tdir, B, par) bind(C, name='sem2d_set_abso_wave')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: xz(*), vdir(*), tdir(*), B(*), par(8)
+    end function
+
+    ! the absorbing boundary set last ends on periodic boundaries: Stacey end terms summed
+    integer(c_int) function sem2d_set_abso_periodic(h) bind(C, name='sem2d_set_abso_periodic')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    ! linear slip fault (BC_DEF kind = 'LISFLT'): node pairs, normals(n, 2), B, W1, W2 (n, ndof)
+    integer(c_int) function sem2d_set_lsf(h, n, node1, node2, nrm, B, W1, W2, kt, kn) bind(C, name='sem2d_set_lsf')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: n
+      integer(c_int), intent(in) :: node1(*), node2(*)
+      real(c_double), intent(in) :: nrm(*), B(*), W1(*), W2(*)
+      real(c_double), value :: kt, kn
     end function
 
     ! HHT-alpha time scheme (TIME kind = 'HHT-alpha'): right after sem2d_create

@@ -110,6 +110,25 @@ class Solver:
                 enode, K = ab["stacey"]
                 self._check(self.api.set_abso_stacey(self.h, len(enode), _p(enode, np.int32), _p(K)),
                             "set_abso_stacey")
+                if ab.get("periodic_ends"):
+                    self._check(self.api.set_abso_periodic(self.h), "set_abso_periodic")
+            if ab.get("wave_V") is not None:
+                w = pb.wave
+                xz = np.ascontiguousarray(ab["coord"].T, dtype=np.float64)  # [2][n]
+                wv = np.ascontiguousarray(ab["wave_V"].T, dtype=np.float64)  # [ndof][n]
+                wt = np.ascontiguousarray(ab["wave_T"].T, dtype=np.float64)
+                wb = np.ascontiguousarray(ab["wave_B"], dtype=np.float64)
+                par = np.array([w["s"][0], w["s"][1], w["x0"], w["z0"], w["tdelay"], w["f0"], w["t0"],
+                                w["stf_ampli"]], dtype=np.float64)
+                self._check(self.api.set_abso_wave(self.h, len(node), _p(xz), _p(wv), _p(wt), _p(wb), _p(par)),
+                            "set_abso_wave")
+        lsf = getattr(pb, "lsf", None)
+        if lsf is not None:
+            n1 = np.ascontiguousarray(lsf["node1"], dtype=np.int32)
+            n2 = np.ascontiguousarray(lsf["node2"], dtype=np.int32)
+            arrs = [np.ascontiguousarray(lsf[k].T, dtype=np.float64) for k in ("n", "B", "W1", "W2")]
+            self._check(self.api.set_lsf(self.h, len(n1), _p(n1, np.int32), _p(n2, np.int32),
+                                         *[_p(x) for x in arrs], float(lsf["kt"]), float(lsf["kn"])), "set_lsf")
         for s in pb.sources:
             isrc = C.c_int()
             if s["kind"] == "force":

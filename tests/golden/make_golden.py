@@ -98,7 +98,8 @@ def run_case(name):
             nsamp = out["seismo"].shape[0]
             out["seis_f32"] = np.stack([np.fromfile(os.path.join(tmp, n), np.float32).reshape(nrec, nsamp).T
                                         for n in names], axis=1)
-        if 5 in g["bounds"] and 6 in g["bounds"]:
+        has_dynflt = pf.read_par(os.path.join(src, "Par.inp")).first("BC_DYNFLT") is not None
+        if 5 in g["bounds"] and 6 in g["bounds"] and has_dynflt:
             n1, n2 = g["bounds"][5]["node"], g["bounds"][6]["node"]
             keep = n1 != n2
             npf = int(keep.sum())
