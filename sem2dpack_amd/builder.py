@@ -536,16 +536,15 @@ def _numbering_fem(mesh, xgll):
                 ix=np.zeros(nelem, dtype=np.int64), iz=np.zeros(nelem, dtype=np.int64))
 
 
-def _elastic_coefs_general(DxiDx, DxiDz, DetaDx, DetaDz, w2J, lam, mu, ndof):
+def _elastic_coefs_general(DxiDx, DxiDz, DetaDx, DetaDz, w2J, Kx, Kz, lam, mu, mux, ndof):
     """General-geometry coefficients a(:,:,k) of SRC/mat_elastic.f90:327-358 (nelast 3 and
-    10, isotropic), arrays (n, ngll, ngll) -> (n, nelast, ngll, ngll)."""
+    10), arrays (n, ngll, ngll) -> (n, nelast, ngll, ngll).  Moduli Kx, Kz, lam, mu (= muz),
+    mux as in SRC/mat_elastic.f90:307-321."""
     if ndof == 1:
-        a = np.stack([mu * DxiDx * DxiDx + mu * DxiDz * DxiDz,
-                      mu * DetaDx * DetaDx + mu * DetaDz * DetaDz,
-                      mu * DxiDx * DetaDx + mu * DxiDz * DetaDz], axis=1)
+        a = np.stack([mux * DxiDx * DxiDx + mu * DxiDz * DxiDz,
+                      mux * DetaDx * DetaDx + mu * DetaDz * DetaDz,
+                      mux * DxiDx * DetaDx + mu * DxiDz * DetaDz], axis=1)
     else:
-        Kx = lam + 2.0 * mu
-        Kz = Kx
         a = np.stack([Kx * DxiDx * DxiDx + mu * DxiDz * DxiDz,
                       lam * DxiDx * DetaDz + mu * DxiDz * DetaDx,
                       Kz * DetaDz * DetaDz + mu * DetaDx * DetaDx,
@@ -668,16 +667,29 @@ def _materials(par, fname_dir=None):
         me = par.after(g, "MAT_ELASTIC")
         if me is None:
             raise InputError("MAT_ELAST_read: MAT_ELASTIC block not found")
-        for key in ("c11", "c11h", "c13", "c13h"):
-            if me.get(key):
-                raise InputError(f"MAT_ELASTIC {key} (anisotropy) is not supported by this build")
         rho, cp, cs = float(me.get("rho", 0)), float(me.get("cp", 0)), float(me.get("cs", 0))
         rhoH, cpH, csH = me.get("rhoh", ""), me.get("cph", ""), me.get("csh", "")
         if rho <= 0 and not rhoH:
             raise InputError("MAT_ELAST_read: undefined density (rho)")
+        cij = {k: float(me.get(k, 0)) for k in ("c11", "c13", "c33", "c55", "c66")}
+        cijH = {k: str(me.get(k + "h", "")).strip() for k in cij}
+        has = {k: cij[k] > 0 or bool(cijH[k]) for k in cij}
         if not ((cp > 0 or cpH) and (cs > 0 or csH)):
-            raise InputError("MAT_ELAST_read: cp and cs must be set")
-        if not (rhoH or cpH or csH):
+            # transverse anisotropy, vertical symmetry axis (SRC/mat_elastic.f90:131-152):
+            # c11, c13, c33, c55 all given, or c55 and c66 (SH)
+            if not ((has["c11"] and has["c13"] and has["c33"] and has["c55"])
+                    or (has["c55"] and has["c66"])):
+                raise InputError("MAT_ELAST_read: incomplete input")
+            if rhoH or any(cijH.values()):
+                raise InputError("MAT_ELASTIC distributions of anisotropic moduli are not supported "
+                                 "by this build")
+            # homogeneous (one shared coefficient array) only when all five moduli are set
+            homog = all(v > 0 for v in cij.values())
+            # wave speeds for the time step, absorbing boundaries and faults:
+            # cp = sqrt(c11/rho), cs = sqrt(c55/rho) (SRC/mat_elastic.f90:224-230)
+            mats[tag] = dict(homog=homog, aniso=True, rho=rho, cp=math.sqrt(cij["c11"] / rho),
+                             cs=math.sqrt(cij["c55"] / rho), **cij)
+        elif not (rhoH or cpH or csH):
             mu = rho * cs * cs
             kmod = rho * cp * cp
             lam = rho * (cp * cp - 2.0 * cs * cs)
@@ -724,24 +736,19 @@ def _eval_dist(d, x, z):
     return _gen_hete1(d[1], x, z)
 
 
-def _elastic_coefs(DxiDx, DetaDz, w2J, lam, mu, ndof):
-    """Flat-grid coefficients a(:,:,k) of SRC/mat_elastic.f90:327-358, one element."""
-    if ndof == 1:
-        a = np.stack([mu * DxiDx * DxiDx, mu * DetaDz * DetaDz])
-    else:
-        Kx = lam + 2.0 * mu
-        a = np.stack([Kx * DxiDx * DxiDx, lam * DxiDx * DetaDz, Kx * DetaDz * DetaDz,
-                      mu * DetaDz * DetaDz, mu * DxiDx * DetaDz, mu * DxiDx * DxiDx])
-    return (-w2J)[None] * a
+def _elastic_coefs(DxiDx, DetaDz, w2J, Kx, Kz, lam, mu, mux, ndof):
+    """Flat-grid coefficients a(:,:,k) of SRC/mat_elastic.f90:324-339, one element; moduli
+    as in _elastic_coefs_general."""
+    return _elastic_coefs_batch(DxiDx[None], DetaDz[None], w2J[None], Kx[None], Kz[None],
+                                lam[None], mu[None], mux[None], ndof)[0]
 
 
-def _elastic_coefs_batch(DxiDx, DetaDz, w2J, lam, mu, ndof):
+def _elastic_coefs_batch(DxiDx, DetaDz, w2J, Kx, Kz, lam, mu, mux, ndof):
     """_elastic_coefs for a batch of elements: arrays (n, ngll, ngll) -> (n, nelast, ngll, ngll)."""
     if ndof == 1:
-        a = np.stack([mu * DxiDx * DxiDx, mu * DetaDz * DetaDz], axis=1)
+        a = np.stack([mux * DxiDx * DxiDx, mu * DetaDz * DetaDz], axis=1)
     else:
-        Kx = lam + 2.0 * mu
-        a = np.stack([Kx * DxiDx * DxiDx, lam * DxiDx * DetaDz, Kx * DetaDz * DetaDz,
+        a = np.stack([Kx * DxiDx * DxiDx, lam * DxiDx * DetaDz, Kz * DetaDz * DetaDz,
                       mu * DetaDz * DetaDz, mu * DxiDx * DetaDz, mu * DxiDx * DxiDx], axis=1)
     return (-w2J)[:, None] * a
 
@@ -1780,6 +1787,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     shp = (nelem, ngll, ngll)
     rho_p, cp_p, cs_p, lam_p, mu_p = (np.empty(shp) for _ in range(5))
     homog_tag = {}
+    aniso_tags = {}
     if elem_props is not None:
         cell = num["iz"] * nx + num["ix"]
         r, la, m = (np.asarray(a, dtype=np.float64)[cell][:, None, None] for a in elem_props)
@@ -1802,7 +1810,11 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
             m = mats[int(t)]
             sel = tags == t
             homog_tag[int(t)] = m["homog"]
-            if m["homog"]:
+            if m.get("aniso"):
+                rho_p[sel], cp_p[sel], cs_p[sel] = m["rho"], m["cp"], m["cs"]
+                lam_p[sel], mu_p[sel] = m["c13"], m["c55"]
+                aniso_tags[int(t)] = m
+            elif m["homog"]:
                 rho_p[sel], cp_p[sel], cs_p[sel] = m["rho"], m["cp"], m["cs"]
                 lam_p[sel], mu_p[sel] = m["lam"], m["mu"]
             else:
@@ -1814,6 +1826,18 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
                 lam_p[sel] = r * (c1 * c1 - 2.0 * c2 * c2)
                 mu_p[sel] = r * c2 * c2
     pb.rho, pb.cp, pb.cs, pb.lam, pb.mu = rho_p, cp_p, cs_p, lam_p, mu_p
+    # moduli of the coefficient arrays (SRC/mat_elastic.f90:307-321): isotropic
+    # Kx = Kz = lambda + 2 mu, mux = muz = mu; transversely anisotropic Kx = c11, Kz = c33,
+    # la = c13, muz = c55, mux = c66 (lam_p and mu_p then hold c13 and c55)
+    kx_p = lam_p + 2.0 * mu_p
+    kz_p, mux_p = kx_p, mu_p
+    if aniso_tags:
+        kz_p, mux_p = kx_p.copy(), mu_p.copy()
+        for t, m in aniso_tags.items():
+            sel = tags == t
+            kx_p[sel], kz_p[sel], mux_p[sel] = m["c11"], m["c33"], m["c66"]
+    pb.anisotropic = bool(aniso_tags)
+    mod = (kx_p, kz_p, lam_p, mu_p, mux_p)
     kv_eta0 = None
     if material is None and elem_props is None and kv_tags:
         kv_eta0 = np.zeros((nelem, ngll, ngll))
@@ -1911,27 +1935,27 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         coefs = np.empty((nelem, pb.nelast, ngll, ngll))
         for c0 in range(0, nelem, 65536):
             b = slice(c0, min(c0 + 65536, nelem))
-            coefs[b] = _elastic_coefs_general(DxiDx[b], DxiDz[b], DetaDx[b], DetaDz[b], dvol[b], lam_p[b],
-                                              mu_p[b], ndof)
+            coefs[b] = _elastic_coefs_general(DxiDx[b], DxiDz[b], DetaDx[b], DetaDz[b], dvol[b],
+                                              *(q[b] for q in mod), ndof)
         pb.coefs = coefs
     elif len(utags) == 1 and pb.homogeneous:
         e1 = 0
-        pb.coefs = _elastic_coefs(DxiDx[e1], DetaDz[e1], dvol[e1], lam_p[e1], mu_p[e1], ndof)
+        pb.coefs = _elastic_coefs(DxiDx[e1], DetaDz[e1], dvol[e1], *(q[e1] for q in mod), ndof)
         pb.coef_mode = "shared"
     else:
         pb.coef_mode = "per_element"
-        if per_element_coefs or pb.homogeneous or ngll != 5:
+        if per_element_coefs or pb.homogeneous or ngll != 5 or pb.anisotropic:
             coefs = np.empty((nelem, pb.nelast, ngll, ngll))
             for t in utags:
                 sel = tags == t
                 if homog_tag[int(t)]:
                     e1 = first_of[int(t)]
-                    coefs[sel] = _elastic_coefs(DxiDx[e1], DetaDz[e1], dvol[e1], lam_p[e1], mu_p[e1], ndof)
+                    coefs[sel] = _elastic_coefs(DxiDx[e1], DetaDz[e1], dvol[e1], *(q[e1] for q in mod), ndof)
                 else:
                     idx = np.nonzero(sel)[0]
                     for c0 in range(0, len(idx), 65536):
                         b = idx[c0:c0 + 65536]
-                        coefs[b] = _elastic_coefs_batch(DxiDx[b], DetaDz[b], dvol[b], lam_p[b], mu_p[b], ndof)
+                        coefs[b] = _elastic_coefs_batch(DxiDx[b], DetaDz[b], dvol[b], *(q[b] for q in mod), ndof)
             pb.coefs = coefs
         else:
             pb.coefs = None
@@ -1939,7 +1963,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     # geometric factors of each element, -detJ*DxiDx^2, -detJ*DetaDz^2, -detJ*DxiDx*DetaDz
     # (taken at the element's first GLL node); a(:,:,k) = wgll2 * material * factor.
     pb.geom = pb.lam_mu = None
-    if flat and ngll == 5 and not (len(utags) == 1 and pb.homogeneous):
+    if flat and ngll == 5 and not (len(utags) == 1 and pb.homogeneous) and not pb.anisotropic:
         detJ = J11[:, 0, 0] * J22[:, 0, 0] - J12[:, 0, 0] * J21[:, 0, 0]
         gx, gz = DxiDx[:, 0, 0], DetaDz[:, 0, 0]
         pb.geom = np.stack([-detJ * gx * gx, -detJ * gz * gz, -detJ * gx * gz], axis=1)
@@ -1954,6 +1978,9 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     if W25 is not None:
         if elem_props is not None or material is not None:
             raise InputError("2.5D (W) with per-element model arrays is not supported by this build")
+        if pb.anisotropic:
+            # MAT_ELAST_init_25D reads the isotropic lambda and mu
+            raise InputError("2.5D (W) with anisotropic materials is not supported")
         pi = 4.0 * math.atan(1.0)
         if ndof == 1:
             q = pi / W25
@@ -2067,6 +2094,9 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     pb.sources = [x for x in srcs if x["kind"] != "wave"]
     pb.wave = None
     if waves:
+        if pb.anisotropic:
+            # WAVE_init reads the isotropic lambda and mu of its reference node
+            raise InputError("WAVE sources with anisotropic materials are not supported")
         w = _wave_init(dict(waves[0]["wave"]), num, (rho_p, cp_p, cs_p, lam_p, mu_p))
         w.update(f0=waves[0]["f0"], t0=waves[0]["t0"], stf_ampli=waves[0]["stf_ampli"],
                  tdelay=waves[0]["tdelay"])
