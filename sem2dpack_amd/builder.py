@@ -320,6 +320,92 @@ def _cart_mesh(g, domains=()):
                 domains=doms)
 
 
+def _layered_mesh(par, g, fname_dir=None):
+    """MESH_LAYERS_read / MESH_LAYERS_build (SRC/mesh_layers.f90:39-113,161-230,266-330,
+    340-470) for four-node elements: layer tops that are constant (ztop) or DIST_LINEAR
+    profiles (ztopH='LINEAR'), from MESH_LAYER blocks (the first block is the top layer,
+    default tag = its position) or from a file of 'ztop nz tag' lines (top layer first).
+    Control rows: row 0 at zmin, then in each layer (bottom up) the rows
+    bot + j/nz (top - bot), j = 1..nz; the row on top of element row ezflt is doubled."""
+    xlim, zmin = g.get("xlim"), g.get("zmin")
+    if xlim is None:
+        raise InputError("MESH_LAYERS_read: you must set xlim")
+    if zmin is None:
+        raise InputError("MESH_LAYERS_read: you must set zmin")
+    nx, nlayer = int(g.get("nx", 1)), int(g.get("nlayer", 0))
+    fname = str(g.get("file", "")).strip()
+    ezflt, fztag, fznz = int(g.get("ezflt", 0)), int(g.get("fztag", 0)), int(g.get("fznz", 1))
+    if nx <= 0:
+        raise InputError("MESH_LAYERS_read: nx must be positive")
+    if nlayer <= 0 and not fname:
+        raise InputError("MESH_LAYERS_read: nlayer or file must be set")
+    if ezflt < -1:
+        raise InputError("MESH_LAYERS_read: ezflt must be >= -1")
+    if ezflt == 0:
+        fztag = 0
+    if fztag < 0:
+        raise InputError("MESH_LAYERS_read: fztag must be positive")
+    if fznz < 1:
+        raise InputError("MESH_LAYERS_read: fznz must be strictly positive")
+    layers = []  # top layer first, as read
+    if fname:
+        path = os.path.join(fname_dir, fname) if fname_dir else fname
+        with open(path) as fh:
+            rows = [ln.replace("d", "e").replace("D", "e").split() for ln in fh if ln.strip()]
+        for r in rows:
+            layers.append(dict(top=("const", float(r[0])), nez=int(float(r[1])), tag=int(float(r[2]))))
+    else:
+        blocks = par.all("MESH_LAYER")
+        if len(blocks) < nlayer:
+            raise InputError("MESH_LAYER input block not found")
+        for pos, b in enumerate(blocks[:nlayer]):
+            zh = str(b.get("ztoph", "")).strip()
+            if zh and zh.upper() != "LINEAR":
+                # QSPLINE, CSPLINE and the other distributions need 8- or 9-node elements
+                raise InputError(f"MESH_LAYER ztopH={zh!r} (curved layer interfaces) is not supported "
+                                 "by this build")
+            if not zh and b.get("ztop") is None:
+                raise InputError("MESH_LAYERS_read: ztop or ztopH must be set")
+            top = _read_cd(par, iter(par.next_groups(b)), b.get("ztop", 0.0), zh)
+            layers.append(dict(top=top, nez=int(b.get("nz", 0)), tag=int(b.get("tag", pos + 1))))
+    for L in layers:
+        if L["nez"] <= 0:
+            raise InputError("MESH_LAYERS_read: nz null, negative or missing")
+        if L["tag"] < 1:
+            raise InputError("MESH_LAYERS_read: tag null, negative or missing")
+    layers = layers[::-1]  # bottom layer first
+    nz = sum(L["nez"] for L in layers)
+    if ezflt == -1:
+        ezflt = nz // 2
+    if ezflt >= nz:
+        raise InputError("MESH_LAYERS_read: ezflt must be < nz")
+    xmin, xmax, zmin = float(xlim[0]), float(xlim[1]), float(zmin)
+    nxp = nx + 1
+    xc = xmin + (xmax - xmin) / float(nxp - 1) * np.arange(nxp, dtype=np.float64)
+    zrows = [np.full(nxp, zmin)]
+    doms = []
+    jlast = 0
+    # a layer's rows interpolate between the previous layer's top profile (not its last
+    # computed row, which may differ from it in the last bit) and its own top
+    bot = zrows[0]
+    for L in layers:
+        top = _gen_cd(L["top"], np.stack([xc, np.zeros(nxp)], axis=1))
+        if (top < zmin).any():
+            raise InputError("MESH_LAYERS_build: zmin in MESH_LAYERED should be deeper than all "
+                             "layer interfaces")
+        for j in range(1, L["nez"] + 1):
+            zrows.append(bot + float(j) / float(L["nez"]) * (top - bot))
+            if len(zrows) - 1 == ezflt:
+                zrows.append(zrows[-1])
+        doms.append((L["tag"], [1, nx], [jlast + 1, jlast + L["nez"]]))
+        jlast += L["nez"]
+        bot = top
+    zc = np.array(zrows)
+    return dict(kind="layered", xmin=xmin, xmax=xmax, zmin=zmin, nx=nx, nz=nz, ezflt=max(ezflt, 0),
+                fztag=fztag, fznz=fznz, domains=doms, ctrl_x=np.broadcast_to(xc, zc.shape).copy(),
+                ctrl_z=zc)
+
+
 def _rcm_perm(nx, nz):
     """Element permutation of MESH_STRUCTURED_renumber (SRC/mesh_structured.f90:204-269)."""
     jj, ii = np.meshgrid(np.arange(1, nz + 1), np.arange(1, nx + 1), indexing="ij")
@@ -406,6 +492,13 @@ def _numbering(mesh, perm, xgll):
     iglob[order] = np.arange(1, npoin + 1)
     ibool = iglob[G].astype(np.int32)  # [e, j, i], 1-based
 
+    czr = iz + shift  # control row of the element's lower edge
+    if "ctrl_z" in mesh:
+        # layered mesh: control node coordinates per (row, column)
+        X_, Z_ = mesh["ctrl_x"], mesh["ctrl_z"]
+        cx = np.stack([X_[czr, ix], X_[czr, ix + 1], X_[czr + 1, ix + 1], X_[czr + 1, ix]], axis=1)
+        cz = np.stack([Z_[czr, ix], Z_[czr, ix + 1], Z_[czr + 1, ix + 1], Z_[czr + 1, ix]], axis=1)
+        return _numbering_coords(ibool, npoin, iglob, G, last, cx, cz, xgll, ix, iz, shift, NX1, NZ1)
     # control nodes (SRC/mesh_cartesian.f90:253-267)
     xc = mesh["xmin"] + (mesh["xmax"] - mesh["xmin"]) / float(nx) * np.arange(nx + 1, dtype=np.float64)
     if ezflt > 0:
@@ -414,10 +507,16 @@ def _numbering(mesh, perm, xgll):
         zi = np.arange(0, nz + 1, dtype=np.float64)
     zc = mesh["zmin"] + (mesh["zmax"] - mesh["zmin"]) / float(nz) * zi
     # element corners (knods order 1..4 = (i,j),(i+1,j),(i+1,j+1),(i,j+1))
-    czr = iz + shift  # control row of the element's lower edge
     cx = np.stack([xc[ix], xc[ix + 1], xc[ix + 1], xc[ix]], axis=1)
     cz = np.stack([zc[czr], zc[czr], zc[czr + 1], zc[czr + 1]], axis=1)
+    return _numbering_coords(ibool, npoin, iglob, G, last, cx, cz, xgll, ix, iz, shift, NX1, NZ1)
 
+
+def _numbering_coords(ibool, npoin, iglob, G, last, cx, cz, xgll, ix, iz, shift, NX1, NZ1):
+    """GLL node coordinates of a structured grid from its element corners cx, cz."""
+    nelem, ngll = G.shape[0], G.shape[1]
+    loc = np.arange(ngll)
+    k = np.broadcast_to(np.arange(nelem)[:, None, None], G.shape)
     # coordinates: written by the last element touching the node (SRC/spec_grid.f90:342-349)
     shp = _shape_q4(xgll)  # (ngll(j), ngll(i), 4)
     coord = np.empty((npoin, 2))
@@ -1726,11 +1825,18 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
         mesh = _cart_mesh(par.first("MESH_CART"), par.all("MESH_CART_DOMAIN"))
     elif method == "MESH2D":
         mesh = _read_mesh2d(par.first("MESH_MESH2D"), fname_dir)
+    elif method == "LAYERED":
+        g = par.first("MESH_LAYERED")
+        if g is None:
+            raise InputError("MESH_LAYERS_read: input block not found")
+        mesh = _layered_mesh(par, g, fname_dir)
     else:
-        raise InputError("this build supports MESH_DEF method='CARTESIAN' or 'MESH2D'")
-    # a MESH_CART box is a flat grid (SRC/mesh_cartesian.f90:242); MESH2D meshes are not
-    # (fem_grid_type default), which selects the general element coefficients
-    flat = mesh.get("kind") != "mesh2d"
+        raise InputError("this build supports MESH_DEF method='CARTESIAN', 'LAYERED' or 'MESH2D'")
+    # a MESH_CART box is a flat grid (SRC/mesh_cartesian.f90:242); MESH2D and LAYERED
+    # meshes are not (SRC/mesh_layers.f90:369), which selects the general element
+    # coefficients.  CARTESIAN and LAYERED share the structured numbering.
+    flat = mesh.get("kind") not in ("mesh2d", "layered")
+    structured = mesh.get("kind") != "mesh2d"
 
     pb = Problem()
     pb.ngll, pb.ndof = ngll, ndof
@@ -1742,7 +1848,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
     xgll, wgll, H = gll_info(ngll)
     pb.xgll, pb.wgll, pb.hprime = xgll, wgll, H
     nx, nz = mesh["nx"], mesh["nz"]
-    if flat:
+    if structured:
         pb.nelem = nelem = nx * nz
         perm = _rcm_perm(nx, nz)
         perm_inv = np.empty(nelem, dtype=np.int64)
@@ -1763,7 +1869,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
 
     # element tags (CART_build, SRC/mesh_cartesian.f90:277-297): the domains in input
     # order (a later one overrides an earlier one), then the fault zone
-    if flat:
+    if structured:
         tags = np.zeros(nelem, dtype=np.int64)
         for tag, ex, ez in mesh["domains"]:
             inside = ((num["ix"] + 1 >= ex[0]) & (num["ix"] + 1 <= ex[1]) &
@@ -1771,7 +1877,7 @@ def build(par, nsteps=None, material=None, fname_dir=None, per_element_coefs=Tru
             tags[inside] = tag
     else:
         tags = mesh["tag"].copy()
-    if flat and mesh["fztag"] > 0:
+    if structured and mesh["fztag"] > 0:
         j1 = max(mesh["ezflt"] + 1 - mesh["fznz"], 1)
         j2 = min(mesh["ezflt"] + mesh["fznz"], nz)
         rows = num["iz"] + 1

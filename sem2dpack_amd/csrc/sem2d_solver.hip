@@ -1416,7 +1416,8 @@ int sem2d_create(const sem2d_desc* s, sem2d_solver** out) {
   if ((rc = upload_bulk(h, &h->rmass, s->rmass, nf))) return rc;
   h->rmass_host.assign(s->rmass, s->rmass + nf);
   h->fstride = (size_t)h->npoin;
-  if (NG == sem2d::NG && structured_map(h, s)) {  // tile kernels: NGLL = 5
+  // tile kernels: NGLL 5 and flat-element coefficients (nelast 6 or 2)
+  if (NG == sem2d::NG && (h->nelast == 6 || h->nelast == 2) && structured_map(h, s)) {
     StructState& S = h->st;
     S.on = true;
     S.cell_elem.resize(s->nelem);

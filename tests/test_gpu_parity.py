@@ -53,7 +53,8 @@ CASES = [("small_psv", 300), ("c1_psv", 2000), ("sh_swf", 600), ("multi_src", 30
          ("ng7_swf", 600), ("ng10_psv", 400), ("sides_psv", 500), ("sides_sh1", 500), ("domains", 500),
          ("twf_inplane", 600), ("twf_rsf", 500), ("twf_only", 600), ("perio_psv", 700), ("perio_sh", 800),
          ("bimat", 156), ("nor3_rsf", 400), ("nor0_swf", 500), ("hht_flt1d", 455), ("hht_psv", 600),
-         ("hht_twf", 400), ("wave_psv", 825), ("wave_sh", 251), ("lsf_psv", 886), ("aniso_psv", 559), ("aniso_sh", 403)] + \
+         ("hht_twf", 400), ("wave_psv", 825), ("wave_sh", 251), ("lsf_psv", 886), ("aniso_psv", 559), ("aniso_sh", 403),
+         ("layer_psv", 869), ("layer_sh", 532)] + \
     [(n, 200 if n == "rsf_slip" else 500) for n in conftest.FRICTION_CASES]
 
 
